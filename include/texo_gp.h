@@ -1,0 +1,332 @@
+/*
+ * texo_gp.h — C ABI of the MI355X electrostatic global-placement library.
+ *
+ * This boundary replaces the body of texo-pnr's crate-private entry point
+ *
+ *   fn place(graph, units, hypergraph, initial_x, initial_y,
+ *            fixed_coordinates, cell_offsets, routing_capacity,
+ *            register_controls, best_effort) -> Result<Vec<Point>, EplaceError>
+ *
+ * (crates/texo-pnr/src/eplace.rs:242-253, single caller
+ * crates/texo-pnr/src/lib.rs:4129-4143).  The caller flattens its borrowed
+ * inputs into `texo_gp_problem` (plain pointers and sizes, all borrowed for the
+ * duration of the call), receives integer tile targets in a caller-allocated
+ * `uint32_t[2 * n_units]`, and maps a nonzero status to its `EplaceError`
+ * variant with the payload carried in `texo_gp_report`.  Nothing unwinds or
+ * aborts across this boundary.
+ *
+ * All coordinates are in device tiles; one Poisson bin is one tile
+ * (crates/texo-pnr/src/eplace.rs:131-133).
+ */
+#ifndef TEXO_GP_H
+#define TEXO_GP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TEXO_GP_ABI_VERSION 1u
+
+/* ResourceKind codes.  Numeric order equals the `derive(Ord)` order of
+ * texo_model::ResourceKind (crates/texo-model/src/lib.rs:68-87), so sorting
+ * codes reproduces the reference's BTreeMap field order:
+ * Logic < Lut(n) < Register < Memory < Clock < Io < Constant < Dsp. */
+#define TEXO_KIND_LOGIC 0x000u
+#define TEXO_KIND_LUT(n) (0x100u + (uint32_t)(n)) /* n <= 255 */
+#define TEXO_KIND_REGISTER 0x200u
+#define TEXO_KIND_MEMORY 0x300u
+#define TEXO_KIND_CLOCK 0x400u
+#define TEXO_KIND_IO 0x500u
+#define TEXO_KIND_CONSTANT 0x600u
+#define TEXO_KIND_DSP 0x700u
+
+/* Maximum number of electrostatic density fields of one problem. Only
+ * Lut(n), Register and Memory are exchangeable density kinds
+ * (crates/texo-pnr/src/eplace.rs:1417-1422). */
+#define TEXO_GP_MAX_FIELDS 4
+
+/* Status codes.  Each names one variant of EplaceError
+ * (crates/texo-pnr/src/eplace.rs:27-38), ElectrostaticError
+ * (crates/texo-pnr/src/electrostatic_placement.rs:15-69), NesterovError
+ * (crates/texo-pnr/src/global_placement.rs:81-109) or InstanceAreaError
+ * (crates/texo-pnr/src/instance_area.rs:55-61); codes >= 900 are
+ * boundary-level failures with no reference counterpart. */
+enum texo_gp_status {
+  TEXO_GP_OK = 0,
+  /* EplaceError */
+  TEXO_GP_ERR_DENSITY_FIELD_CHANGED = 10,
+  TEXO_GP_ERR_INVALID_NORMALIZATION = 11,
+  TEXO_GP_ERR_DID_NOT_CONVERGE = 12,
+  /* ElectrostaticError (EplaceError::Density) */
+  TEXO_GP_ERR_EMPTY_GRID = 100,
+  TEXO_GP_ERR_GRID_TOO_LARGE = 101,
+  TEXO_GP_ERR_DENSITY_LENGTH = 102,
+  TEXO_GP_ERR_NON_FINITE_DENSITY = 103,
+  TEXO_GP_ERR_EMPTY_UNIT = 104,
+  TEXO_GP_ERR_NON_FINITE_UNIT_COORDINATE = 105,
+  TEXO_GP_ERR_INVALID_MEMBER_CHARGE = 106,
+  TEXO_GP_ERR_POINT_OUTSIDE_GRID = 107,
+  TEXO_GP_ERR_FIXED_CAPACITY_EXHAUSTED = 108,
+  TEXO_GP_ERR_INSUFFICIENT_CAPACITY = 109,
+  TEXO_GP_ERR_INSUFFICIENT_AREA = 110,
+  TEXO_GP_ERR_NON_FINITE_FILLER = 111,
+  TEXO_GP_ERR_NEGATIVE_FILLER_CHARGE = 112,
+  TEXO_GP_ERR_UNEXPECTED_FILLER_KIND = 113,
+  TEXO_GP_ERR_FILLER_CHARGE_MISMATCH = 114,
+  /* NesterovError (EplaceError::Optimizer) */
+  TEXO_GP_ERR_INVALID_BOUNDS = 200,
+  TEXO_GP_ERR_INVALID_METRIC_LENGTH = 201,
+  TEXO_GP_ERR_NON_POSITIVE_METRIC = 202,
+  TEXO_GP_ERR_INVALID_INITIAL_COORDINATE = 203,
+  TEXO_GP_ERR_INVALID_MOMENTUM = 204,
+  TEXO_GP_ERR_INVALID_LIPSCHITZ = 205,
+  TEXO_GP_ERR_INVALID_OBJECTIVE = 206,
+  TEXO_GP_ERR_INVALID_GRADIENT_LENGTH = 207,
+  TEXO_GP_ERR_NON_FINITE_GRADIENT = 208,
+  TEXO_GP_ERR_LINE_SEARCH_OVERFLOW = 209,
+  TEXO_GP_ERR_ITERATION_OVERFLOW = 210,
+  TEXO_GP_ERR_EVALUATION = 211,
+  /* InstanceAreaError (EplaceError::InstanceArea) */
+  TEXO_GP_ERR_INVALID_MEMBER_AREA = 300,
+  TEXO_GP_ERR_INVALID_FILLER_AREA = 301,
+  TEXO_GP_ERR_POSTCONDITION_VIOLATION = 302,
+  TEXO_GP_ERR_ARITHMETIC_OVERFLOW = 303,
+  /* Boundary-level failures (no reference counterpart). */
+  TEXO_GP_ERR_INVALID_ARGUMENT = 900,
+  TEXO_GP_ERR_DEVICE = 901,
+  TEXO_GP_ERR_OUT_OF_MEMORY = 902,
+  TEXO_GP_ERR_UNSUPPORTED = 903,
+  TEXO_GP_ERR_CANCELLED = 904,
+  TEXO_GP_ERR_NOT_READY = 905,
+  TEXO_GP_ERR_INJECTED = 906
+};
+
+/* Precision of the Poisson transform (the only contraction on MFMA).
+ * Every other quantity keeps the same arithmetic in all modes: int64
+ * fixed-point density, FP64 coordinates and global reductions. */
+enum texo_gp_dct_precision {
+  TEXO_GP_DCT_FP32 = 0,       /* plain FP32 contraction */
+  TEXO_GP_DCT_BF16X3 = 1,     /* split hi/lo BF16 on MFMA, FP32 accumulate */
+  TEXO_GP_DCT_FP16_SCALED = 2 /* per-field scaled FP16 on MFMA, FP32 acc */
+};
+
+/* Flattened placement problem.  Every pointer is borrowed for the call.
+ *
+ * Units are PlacementUnits (crates/texo-pnr/src/lib.rs:3729-3733).  A unit
+ * with one choice is fixed (`unit_fixed`); its exchangeable BELs are listed in
+ * the fixed-occupancy arrays (crates/texo-pnr/src/eplace.rs:263-271).  A
+ * movable unit's density members are the exchangeable cells of assignment
+ * row 0, with offsets relative to that row's origin BEL
+ * (crates/texo-pnr/src/eplace.rs:272-293); a movable unit with no such member
+ * is not a density unit. */
+typedef struct texo_gp_problem {
+  uint32_t abi_version; /* TEXO_GP_ABI_VERSION */
+  uint32_t width;       /* device grid (Device::width) */
+  uint32_t height;      /* device grid (Device::height) */
+
+  /* Cells (CellId order).  Fixed cells carry exact coordinates
+   * (`fixed_coordinates`, crates/texo-pnr/src/lib.rs:4003-4033); every cell
+   * carries its offset from its unit origin (`cell_offsets`). */
+  uint32_t n_cells;
+  const uint32_t *cell_unit;     /* [n_cells] owning unit */
+  const uint8_t *cell_fixed;     /* [n_cells] 1 = fixed coordinate present */
+  const double *cell_fixed_xy;   /* [2*n_cells] read where cell_fixed */
+  const double *cell_offset_xy;  /* [2*n_cells] */
+
+  /* Units. */
+  uint32_t n_units;
+  const uint8_t *unit_fixed;        /* [n_units] choices.len() == 1 */
+  const double *unit_initial_xy;    /* [2*n_units] initial_x/initial_y */
+  const double *unit_bounds;        /* [4*n_units] lo_x, lo_y, hi_x, hi_y
+                                       (choice_bounds, eplace.rs:1470-1481) */
+
+  /* Density members of movable units, CSR by unit (unit order, then the
+   * unit's cell order). */
+  const uint32_t *unit_member_start; /* [n_units+1] */
+  const uint16_t *member_kind;       /* [n_members] TEXO_KIND_* */
+  const double *member_offset_xy;    /* [2*n_members] */
+  const uint32_t *member_cell;       /* [n_members] */
+
+  /* BEL capacity per exchangeable kind, before fixed occupancy is removed
+   * (DensityModel::new, electrostatic_placement.rs:139-154). */
+  uint32_t n_capacity_kinds;
+  const uint16_t *capacity_kind; /* [n_capacity_kinds] */
+  const uint32_t *capacity;      /* [n_capacity_kinds * width * height]
+                                    row-major y*width+x */
+
+  /* Fixed occupancy (kind, point) removed from capacity. */
+  uint32_t n_fixed_occupancy;
+  const uint16_t *fixed_occupancy_kind; /* [n] */
+  const uint32_t *fixed_occupancy_xy;   /* [2n] */
+
+  /* AnalyticalHypergraph (analytical_placement.rs:116-168): retained
+   * non-clock nets with at least one sink, pins listed driver first. */
+  uint32_t n_nets;
+  const uint32_t *net_pin_start; /* [n_nets+1] */
+  const uint32_t *pin_cell;      /* [n_pins] */
+  /* Sink-local timed arcs with extra weight w-1 > 0. */
+  uint32_t n_arcs;
+  const uint32_t *arc_cells;        /* [2*n_arcs] driver, sink */
+  const uint64_t *arc_extra_weight; /* [n_arcs] */
+
+  /* Optional RoutingCapacityMap (routing_demand.rs:22-27) on the same grid;
+   * NULL selects the pure density objective. */
+  const uint32_t *routing_horizontal; /* [width*height] or NULL */
+  const uint32_t *routing_vertical;   /* [width*height] or NULL */
+
+  /* RegisterControlSet records (crates/texo-pnr/src/lib.rs:2156-2164). */
+  uint32_t n_register_controls;
+  const uint32_t *register_cell;     /* [n] */
+  const uint64_t *register_clock_lsr; /* [2n] */
+  const uint64_t *register_ce;       /* [n] */
+
+  uint8_t best_effort; /* ElectrostaticBestEffort */
+} texo_gp_problem;
+
+/* Options.  Zero-initialised options reproduce the reference exactly:
+ * stop rules (eplace.rs:1397-1415), gamma base 8 tiles (eplace.rs:133) and the
+ * 3,000-iteration cap (eplace.rs:134). Extensions are recorded in reports. */
+typedef struct texo_gp_options {
+  uint32_t dct_precision;    /* texo_gp_dct_precision */
+  uint32_t fixed_iterations; /* >0: run exactly this many Nesterov iterations
+                                of the hot loop, no stop rule, no area rounds */
+  uint32_t max_iterations;   /* 0 -> 3000 */
+  uint32_t trace;            /* 1: print reference-shaped metrics lines */
+  double gamma_base;         /* 0 -> 8.0 tiles */
+  int32_t device;            /* HIP device ordinal */
+  uint32_t inject_failure_at_evaluation; /* >0: evaluation N fails
+                                            (TEXO_GP_ERR_INJECTED) */
+} texo_gp_options;
+
+/* Result record.  Payload fields follow the failing variant:
+ * err_index/err_index2 carry unit/member/filler/coordinate indices,
+ * err_kind a TEXO_KIND_* code, err_x/err_y a grid point, err_value[0..1]
+ * demand/available or expected/actual values. */
+typedef struct texo_gp_report {
+  int32_t status;
+  int64_t err_index;
+  int64_t err_index2;
+  uint32_t err_kind;
+  uint32_t err_x, err_y;
+  double err_value[2];
+
+  uint64_t iterations;         /* dynamic_nesterov_step calls */
+  uint64_t evaluations;        /* objective evaluations */
+  uint64_t line_search_trials;
+  uint64_t area_rounds;
+  uint32_t dct_precision;
+  uint32_t fixed_iterations;
+  double gamma_base;
+
+  double final_wirelength;     /* WA value at the final coordinates */
+  double final_hpwl;           /* continuous bbox HPWL, FP64 */
+  uint32_t n_fields;
+  uint32_t field_kind[TEXO_GP_MAX_FIELDS];
+  double field_overflow[TEXO_GP_MAX_FIELDS]; /* normalized, physical */
+  double field_energy[TEXO_GP_MAX_FIELDS];
+  double elapsed_seconds;
+} texo_gp_report;
+
+/* ---- one-call entry point (replaces eplace::place) ---------------------- */
+
+/* Runs electrostatic global placement on the GPU and writes
+ * `out_targets_xy[2*n_units]` (Point{x,y} per unit, rounded like
+ * rounded_coordinate, crates/texo-pnr/src/lib.rs:4566-4568).  Returns the
+ * status also stored in `report->status`. */
+int texo_gp_place(const texo_gp_problem *problem, const texo_gp_options *options,
+                  uint32_t *out_targets_xy, texo_gp_report *report);
+
+/* ---- session API: evaluate-level seams used by parity tests -------------- */
+
+typedef struct texo_gp_session texo_gp_session;
+
+/* Builds the density model, fillers and device-resident problem
+ * (eplace.rs:255-336).  On failure returns NULL and fills `report`. */
+texo_gp_session *texo_gp_session_create(const texo_gp_problem *problem,
+                                        const texo_gp_options *options,
+                                        texo_gp_report *report);
+void texo_gp_session_destroy(texo_gp_session *session);
+
+/* Sizes: coordinate count C = 2*(n_units + n_fillers), field count, filler
+ * count, density-unit count. */
+int texo_gp_session_sizes(const texo_gp_session *session, uint64_t *coordinates,
+                          uint32_t *fields, uint64_t *fillers,
+                          uint64_t *density_units);
+
+/* Initial interleaved coordinates [x0,y0,...] units then fillers, clamped to
+ * bounds (initial_coordinates_and_bounds, eplace.rs:1441-1468), plus the
+ * per-field kind codes and per-filler (kind, charge). */
+int texo_gp_session_initial(const texo_gp_session *session, double *coordinates,
+                            double *lower, double *upper, uint32_t *field_kind,
+                            uint32_t *filler_field, double *filler_charge);
+
+/* Field of one objective evaluation (ContinuousProblem::evaluate,
+ * eplace.rs:137-205, with DensityModel::evaluate_with_positions,
+ * electrostatic_placement.rs:265-421).  Any output pointer may be NULL. */
+typedef struct texo_gp_field_eval {
+  double energy;
+  double normalized_overflow; /* physical, before fillers */
+  double net_charge;
+  double force_l1;
+  double real_area;
+  double filler_charge;
+} texo_gp_field_eval;
+
+/* Evaluates WA wirelength and every density field at `coordinates[C]` with
+ * global smoothing `gamma`.  Outputs: wa_value; wa_gradient[2*n_units]
+ * interleaved; per field (f-major): unit_gradient[F*2*n_density_units],
+ * filler_gradient[2*n_fillers] (each filler in its own field), density[F*W*H],
+ * potential[F*W*H]. */
+int texo_gp_session_evaluate(texo_gp_session *session, const double *coordinates,
+                             double gamma, double *wa_value, double *wa_gradient,
+                             texo_gp_field_eval *fields, double *unit_gradient,
+                             double *filler_gradient, double *density,
+                             double *potential);
+
+/* Poisson solve of one FP64 density grid per field on the device transform
+ * path (mean removal, C·rho·C^T, 1/lambda, inverse, energy); `density` is
+ * [F*W*H] and outputs are potential[F*W*H], energy[F].  Used by parity tests
+ * of the transform alone (Poisson2d::solve, electrostatic_placement.rs:725-776). */
+int texo_gp_session_poisson(texo_gp_session *session, const double *density,
+                            double *potential, double *energy);
+
+/* Runs the placement loop from the session's current state.  Targets and
+ * report as texo_gp_place. */
+int texo_gp_session_run(texo_gp_session *session, uint32_t *out_targets_xy,
+                        texo_gp_report *report);
+
+/* Final interleaved coordinates after a run. */
+int texo_gp_session_coordinates(const texo_gp_session *session, double *coordinates);
+
+/* Continuous RUDY demand of the current coordinates
+ * (continuous_routing_demand, eplace.rs:856-937, over
+ * external_baseline_net_bounding_boxes, analytical_placement.rs:238-276):
+ * horizontal[W*H], vertical[W*H]. */
+int texo_gp_session_rudy(texo_gp_session *session, const double *coordinates,
+                         double *horizontal, double *vertical);
+
+/* Kernel timing: when enabled, each launch of the named kernel on the
+ * session stream is bracketed by HIP events; `avg_ms` is the mean duration
+ * and `count` the number of launches since the last reset. */
+int texo_gp_session_set_kernel_timing(texo_gp_session *session, const char *kernel,
+                                      int enable);
+int texo_gp_session_kernel_timing(texo_gp_session *session, double *avg_ms,
+                                  uint64_t *count);
+
+/* Runs exactly `iterations` Nesterov iterations of the hot loop
+ * (optimize_density_once, eplace.rs:503-574) with the stop rule disabled:
+ * the measured unit of work of the benchmark. */
+int texo_gp_session_iterate(texo_gp_session *session, uint32_t iterations,
+                            texo_gp_report *report);
+
+/* Last error message of the calling thread (static storage). */
+const char *texo_gp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEXO_GP_H */

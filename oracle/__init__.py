@@ -1,0 +1,92 @@
+"""FP64 CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by the tests, `__graft_entry__.smoke()` and the benchmark's CPU-baseline
+leg as the checker.  The product (`texo_amd`) never imports this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_gp.so")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    """Loads the oracle library, building it first if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    from texo_amd import abi
+    lib = C.CDLL(LIB_PATH)
+    abi.declare_session_api(lib, "oracle")
+    d, P, u32, u64 = C.c_double, C.POINTER, C.c_uint32, C.c_uint64
+    lib.oracle_set_threads.argtypes = [C.c_int]
+    lib.oracle_poisson_solve.argtypes = [u32, u32, P(d), P(d), P(d)]
+    lib.oracle_poisson_solve_matrix.argtypes = [u32, u32, P(d), P(d), P(d)]
+    lib.oracle_bspline_axis.argtypes = [u32, d, P(u32), P(d), P(d)]
+    lib.oracle_bspline_axis.restype = None
+    lib.oracle_deposit_bspline.argtypes = [P(d), u32, u32, d, d, d]
+    lib.oracle_deposit_bspline.restype = None
+    lib.oracle_sample_bspline.argtypes = [P(d), u32, u32, d, d, P(d)]
+    lib.oracle_sample_bspline.restype = None
+    lib.oracle_weighted_average.argtypes = [u32, u32, P(d), P(C.c_uint8), P(d), P(d), u32, P(u32), P(u32),
+                                            P(u32), u32, P(u32), P(u32), P(u64), d, P(d), P(d), P(d)]
+    lib.oracle_wirelength_preconditioner.argtypes = [u32, u32, P(u32), P(u32), u32, P(u32), P(u64), P(d), P(d)]
+    lib.oracle_box_metric_kkt_residual.argtypes = [u64, P(d), P(d), P(d), P(d), P(d)]
+    lib.oracle_box_metric_kkt_residual.restype = d
+    for name, args in [("field_wirelength_gamma", [d, d, d]), ("multiplier_growth", [d]),
+                       ("density_overflow_target", [u32]), ("routability_optimized_area", [d, d, d]),
+                       ("soft_division_ceiling", [d, d])]:
+        fn = getattr(lib, "oracle_" + name)
+        fn.argtypes = args
+        fn.restype = d
+    lib.oracle_weighted_gamma_mean.argtypes = [u32, P(d), P(d), d]
+    lib.oracle_weighted_gamma_mean.restype = d
+    lib.oracle_augmented_density.argtypes = [d, d, d, P(d), P(d)]
+    lib.oracle_augmented_density.restype = None
+    lib.oracle_rounded_coordinate.argtypes = [d, u32]
+    lib.oracle_rounded_coordinate.restype = u32
+    lib.oracle_continuous_routing_demand.argtypes = [u32, u32, u64, P(d), P(d), P(d)]
+    lib.oracle_session_hpwl.argtypes = [C.c_void_p, P(d)]
+    lib.oracle_session_hpwl.restype = d
+    lib.oracle_session_combined.argtypes = [C.c_void_p, P(d), d, P(d), P(d), P(d), P(d)]
+    lib.oracle_session_preconditioner.argtypes = [C.c_void_p, P(d), P(d)]
+    lib.oracle_clustering_areas.argtypes = [u64, P(u32), P(d), u64, P(u32), P(u64), P(u64), u64, P(d)]
+    lib.oracle_adjust_instance_areas.argtypes = [u64, C.c_void_p, u64, P(u32), P(d), P(d), P(d),
+                                                 P(C.c_int)]
+    _lib = lib
+    return lib
+
+
+class OracleAreaMember(C.Structure):
+    _fields_ = [("unit", C.c_uint32), ("kind", C.c_uint32), ("current_area", C.c_double),
+                ("routability_area", C.c_double), ("pin_area", C.c_double),
+                ("clustering_area", C.c_double)]
+
+
+class NesterovState(C.Structure):
+    _fields_ = [("coordinates", C.POINTER(C.c_double)), ("previous_coordinates", C.POINTER(C.c_double)),
+                ("momentum", C.c_double), ("lipschitz", C.c_double), ("has_lipschitz", C.c_int)]
+
+
+class NesterovResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("accepted", C.c_int32), ("objective", C.c_double),
+                ("stationarity", C.c_double), ("coordinate_change", C.c_double),
+                ("line_search_trials", C.c_uint64), ("evaluations", C.c_uint64),
+                ("payload_evaluation", C.c_int64), ("error_index", C.c_int64)]
+
+
+EVALUATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_double),
+                          C.POINTER(C.c_double))

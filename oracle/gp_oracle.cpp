@@ -1,0 +1,2660 @@
+// gp_oracle.cpp — FP64 CPU restatement of texo-pnr electrostatic global
+// placement.  TEST INFRASTRUCTURE ONLY: the parity oracle for the HIP path and
+// the timed CPU baseline.  The product library never links or calls this.
+//
+// Reference files followed (paths relative to the texo repository):
+//   crates/texo-pnr/src/eplace.rs                  driver, lambda/gamma, stop
+//   crates/texo-pnr/src/electrostatic_placement.rs density model and Poisson
+//   crates/texo-pnr/src/analytical_placement.rs    hypergraph and WA
+//   crates/texo-pnr/src/global_placement.rs        dynamic Nesterov step
+//   crates/texo-pnr/src/instance_area.rs           area adjustment
+//   crates/texo-pnr/src/register_clustering.rs     clustering areas
+//
+// The reference DCT is rustdct 0.7.1 (not vendored).  This file restates the
+// orthonormal DCT-II/III it is scaled to (electrostatic_placement.rs:616-672)
+// with Makhoul's N-point FFT algorithm for power-of-two lengths and a direct
+// cosine sum otherwise; both are pinned to the closed-form matrix oracle.
+
+#include "gp_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using std::size_t;
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
+
+// eplace.rs:125-134
+constexpr double AUGMENTED_DENSITY_BETA = 2000.0;
+constexpr double INITIAL_DENSITY_WEIGHT = 1.0e-4;
+constexpr double AREA_RESET_DENSITY_WEIGHT = 0.1;
+constexpr double MULTIPLIER_ALPHA_LOW = 1.05;
+constexpr double MULTIPLIER_ALPHA_HIGH = 1.06;
+constexpr double AREA_ADJUSTMENT_OVERFLOW_TARGET = 0.15;
+constexpr double WIRELENGTH_GAMMA_BASE_TILES = 8.0;
+constexpr uint64_t MAX_EPLACE_ITERATIONS = 3000;
+
+int g_threads = 1;
+
+// Typed failure carried to the ABI boundary.
+struct Failure {
+  int status;
+  int64_t index = -1;
+  int64_t index2 = -1;
+  uint32_t kind = 0;
+  uint32_t x = 0, y = 0;
+  double v0 = 0.0, v1 = 0.0;
+};
+
+[[noreturn]] void fail(int status, int64_t index = -1, int64_t index2 = -1) {
+  Failure f;
+  f.status = status;
+  f.index = index;
+  f.index2 = index2;
+  throw f;
+}
+
+bool is_lut(uint32_t kind) { return kind >= 0x100u && kind < 0x200u; }
+bool exchangeable(uint32_t kind) {
+  // eplace.rs:1417-1422
+  return is_lut(kind) || kind == TEXO_KIND_REGISTER || kind == TEXO_KIND_MEMORY;
+}
+
+// eplace.rs:1300-1308
+double density_overflow_target(uint32_t kind) {
+  if (kind == TEXO_KIND_MEMORY || kind == TEXO_KIND_DSP) return 0.20;
+  if (is_lut(kind) || kind == TEXO_KIND_REGISTER) return 0.10;
+  return 0.0;
+}
+
+// crates/texo-pnr/src/lib.rs:4566-4568 (f64::round is half away from zero).
+uint32_t rounded_coordinate(double value, uint32_t extent) {
+  double r = std::round(value);
+  double hi = static_cast<double>(extent - 1);
+  if (r < 0.0) r = 0.0;
+  if (r > hi) r = hi;
+  if (std::isnan(r)) r = 0.0;  // Rust `as u32` saturates NaN to 0
+  return static_cast<uint32_t>(r);
+}
+
+double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ---------------------------------------------------------------------------
+// Orthonormal DCT (electrostatic_placement.rs:616-672).
+// ---------------------------------------------------------------------------
+struct Dct {
+  size_t n = 0;
+  bool fast = false;
+  std::vector<std::complex<double>> twiddle;  // FFT twiddles e^{-2 pi i k/n}
+  std::vector<std::complex<double>> shift;    // e^{-i pi k/(2n)}
+  std::vector<size_t> bitrev;
+  std::vector<double> cosine;                 // direct table [k*n+x]
+  double dc_scale = 0, ac_scale = 0;
+
+  explicit Dct(size_t length) : n(length) {
+    dc_scale = std::sqrt(1.0 / static_cast<double>(n));
+    ac_scale = std::sqrt(2.0 / static_cast<double>(n));
+    fast = n >= 4 && (n & (n - 1)) == 0;
+    if (fast) {
+      twiddle.resize(n / 2);
+      for (size_t k = 0; k < n / 2; ++k)
+        twiddle[k] = std::polar(1.0, -2.0 * kPi * static_cast<double>(k) / static_cast<double>(n));
+      shift.resize(n);
+      for (size_t k = 0; k < n; ++k)
+        shift[k] = std::polar(1.0, -kPi * static_cast<double>(k) / (2.0 * static_cast<double>(n)));
+      bitrev.resize(n);
+      size_t bits = 0;
+      while ((size_t(1) << bits) < n) ++bits;
+      for (size_t i = 0; i < n; ++i) {
+        size_t r = 0;
+        for (size_t b = 0; b < bits; ++b) r |= ((i >> b) & 1u) << (bits - 1 - b);
+        bitrev[i] = r;
+      }
+    } else {
+      cosine.resize(n * n);
+      for (size_t k = 0; k < n; ++k)
+        for (size_t x = 0; x < n; ++x)
+          cosine[k * n + x] = std::cos(kPi * static_cast<double>(k) *
+                                       (static_cast<double>(x) + 0.5) / static_cast<double>(n));
+    }
+  }
+
+  void fft(std::complex<double> *a, bool inverse) const {
+    for (size_t i = 0; i < n; ++i)
+      if (bitrev[i] > i) std::swap(a[i], a[bitrev[i]]);
+    for (size_t len = 2; len <= n; len <<= 1) {
+      size_t step = n / len;
+      for (size_t i = 0; i < n; i += len) {
+        for (size_t j = 0; j < len / 2; ++j) {
+          std::complex<double> w = twiddle[j * step];
+          if (inverse) w = std::conj(w);
+          std::complex<double> u = a[i + j];
+          std::complex<double> v = a[i + j + len / 2] * w;
+          a[i + j] = u + v;
+          a[i + j + len / 2] = u - v;
+        }
+      }
+    }
+  }
+
+  // Orthonormal DCT-II in place: X[k] = s_k sum_x v[x] cos(pi k (x+1/2)/n).
+  void forward(double *v, std::vector<std::complex<double>> &scratch) const {
+    if (!fast) {
+      std::vector<double> out(n);
+      for (size_t k = 0; k < n; ++k) {
+        double acc = 0.0;
+        for (size_t x = 0; x < n; ++x) acc += cosine[k * n + x] * v[x];
+        out[k] = acc;
+      }
+      for (size_t k = 0; k < n; ++k) v[k] = out[k] * (k == 0 ? dc_scale : ac_scale);
+      return;
+    }
+    scratch.resize(n);
+    // Makhoul reordering: even samples ascending, odd samples descending.
+    for (size_t m = 0; m < n / 2; ++m) {
+      scratch[m] = v[2 * m];
+      scratch[n - 1 - m] = v[2 * m + 1];
+    }
+    fft(scratch.data(), false);
+    for (size_t k = 0; k < n; ++k) {
+      double raw = (scratch[k] * shift[k]).real();
+      v[k] = raw * (k == 0 ? dc_scale : ac_scale);
+    }
+  }
+
+  // Exact inverse of forward (orthonormal DCT-III).
+  void inverse(double *v, std::vector<std::complex<double>> &scratch) const {
+    if (!fast) {
+      std::vector<double> out(n);
+      for (size_t x = 0; x < n; ++x) {
+        double acc = 0.0;
+        for (size_t k = 0; k < n; ++k)
+          acc += cosine[k * n + x] * v[k] * (k == 0 ? dc_scale : ac_scale);
+        out[x] = acc;
+      }
+      std::copy(out.begin(), out.end(), v);
+      return;
+    }
+    scratch.resize(n);
+    // Unnormalised coefficients X_k = orthonormal_k / s_k, then
+    // V_k = e^{+i pi k/2n} (X_k - i X_{n-k}), v = IDFT(V).
+    auto raw = [&](size_t k) -> double {
+      if (k >= n) return 0.0;
+      return v[k] / (k == 0 ? dc_scale : ac_scale);
+    };
+    for (size_t k = 0; k < n; ++k) {
+      std::complex<double> z(raw(k), -(k == 0 ? 0.0 : raw(n - k)));
+      scratch[k] = z * std::conj(shift[k]);
+    }
+    fft(scratch.data(), true);
+    double inv = 1.0 / static_cast<double>(n);
+    for (size_t m = 0; m < n / 2; ++m) {
+      v[2 * m] = scratch[m].real() * inv;
+      v[2 * m + 1] = scratch[n - 1 - m].real() * inv;
+    }
+  }
+};
+
+double dot(const double *a, const double *b, size_t n) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+// Poisson2d (electrostatic_placement.rs:595-776).
+struct Poisson2d {
+  size_t width = 0, height = 0;
+  std::unique_ptr<Dct> tx, ty;
+  std::vector<double> inverse_eigenvalues;
+
+  Poisson2d() = default;
+  Poisson2d(uint32_t w, uint32_t h) {
+    if (w == 0 || h == 0) fail(TEXO_GP_ERR_EMPTY_GRID);
+    width = w;
+    height = h;
+    tx.reset(new Dct(width));
+    ty.reset(new Dct(height));
+    inverse_eigenvalues.resize(width * height);
+    for (size_t ky = 0; ky < height; ++ky) {
+      double sy = std::sin(kPi * static_cast<double>(ky) / (2.0 * static_cast<double>(height)));
+      double ly = 4.0 * sy * sy;
+      for (size_t kx = 0; kx < width; ++kx) {
+        double sx = std::sin(kPi * static_cast<double>(kx) / (2.0 * static_cast<double>(width)));
+        double lx = 4.0 * sx * sx;
+        inverse_eigenvalues[ky * width + kx] = (kx == 0 && ky == 0) ? 0.0 : 1.0 / (lx + ly);
+      }
+    }
+  }
+
+  // Poisson2d::solve (electrostatic_placement.rs:725-776).
+  double solve(const std::vector<double> &density, std::vector<double> &potential) const {
+    size_t expected = width * height;
+    if (density.size() != expected) fail(TEXO_GP_ERR_DENSITY_LENGTH, (int64_t)expected, (int64_t)density.size());
+    for (size_t i = 0; i < expected; ++i)
+      if (!std::isfinite(density[i])) fail(TEXO_GP_ERR_NON_FINITE_DENSITY, (int64_t)i);
+    double sum = 0.0;
+    for (double v : density) sum += v;
+    double mean = sum / static_cast<double>(expected);
+    std::vector<double> centered(expected);
+    for (size_t i = 0; i < expected; ++i) centered[i] = density[i] - mean;
+    std::vector<double> transformed = centered;
+    std::vector<std::complex<double>> scratch;
+    for (size_t y = 0; y < height; ++y) tx->forward(&transformed[y * width], scratch);
+    std::vector<double> column(height);
+    potential.assign(expected, 0.0);
+    for (size_t kx = 0; kx < width; ++kx) {
+      for (size_t y = 0; y < height; ++y) column[y] = transformed[y * width + kx];
+      ty->forward(column.data(), scratch);
+      for (size_t ky = 0; ky < height; ++ky) column[ky] *= inverse_eigenvalues[ky * width + kx];
+      ty->inverse(column.data(), scratch);
+      for (size_t y = 0; y < height; ++y) potential[y * width + kx] = column[y];
+    }
+    for (size_t y = 0; y < height; ++y) tx->inverse(&potential[y * width], scratch);
+    return 0.5 * dot(centered.data(), potential.data(), expected);
+  }
+
+  // Matrix-form oracle (electrostatic_placement.rs:779-883, test-only there).
+  double solve_matrix(const std::vector<double> &density, std::vector<double> &potential) const {
+    size_t W = width, H = height, G = W * H;
+    auto basis = [](size_t n) {
+      std::vector<double> b(n * n);
+      for (size_t k = 0; k < n; ++k) {
+        double s = k == 0 ? std::sqrt(1.0 / (double)n) : std::sqrt(2.0 / (double)n);
+        for (size_t x = 0; x < n; ++x)
+          b[k * n + x] = s * std::cos(kPi * (double)k * ((double)x + 0.5) / (double)n);
+      }
+      return b;
+    };
+    std::vector<double> bx = basis(W), by = basis(H);
+    double sum = 0.0;
+    for (double v : density) sum += v;
+    double mean = sum / (double)G;
+    std::vector<double> centered(G);
+    for (size_t i = 0; i < G; ++i) centered[i] = density[i] - mean;
+    std::vector<double> rs(G), sp(G), ci(G);
+    for (size_t y = 0; y < H; ++y)
+      for (size_t kx = 0; kx < W; ++kx)
+        rs[y * W + kx] = dot(&bx[kx * W], &centered[y * W], W);
+    for (size_t ky = 0; ky < H; ++ky)
+      for (size_t kx = 0; kx < W; ++kx) {
+        double c = 0.0;
+        for (size_t y = 0; y < H; ++y) c += by[ky * H + y] * rs[y * W + kx];
+        sp[ky * W + kx] = c * inverse_eigenvalues[ky * W + kx];
+      }
+    for (size_t y = 0; y < H; ++y)
+      for (size_t kx = 0; kx < W; ++kx) {
+        double v = 0.0;
+        for (size_t ky = 0; ky < H; ++ky) v += by[ky * H + y] * sp[ky * W + kx];
+        ci[y * W + kx] = v;
+      }
+    potential.assign(G, 0.0);
+    for (size_t y = 0; y < H; ++y)
+      for (size_t x = 0; x < W; ++x) {
+        double v = 0.0;
+        for (size_t kx = 0; kx < W; ++kx) v += bx[kx * W + x] * ci[y * W + kx];
+        potential[y * W + x] = v;
+      }
+    return 0.5 * dot(centered.data(), potential.data(), G);
+  }
+};
+
+// quadratic_bspline_axis (electrostatic_placement.rs:950-976).
+struct AxisEntry {
+  size_t index;
+  double weight;
+  double derivative;
+};
+
+void bspline_axis(size_t length, double coordinate, AxisEntry out[3]) {
+  double maximum = static_cast<double>(length - 1);
+  double c = clampd(coordinate, 0.0, maximum);
+  double center_coordinate = std::min(std::floor(c + 0.5), maximum);
+  size_t center = static_cast<size_t>(center_coordinate);
+  double offset = c - center_coordinate;
+  out[0] = {center == 0 ? 0 : center - 1, 0.5 * (0.5 - offset) * (0.5 - offset), offset - 0.5};
+  out[1] = {center, 0.75 - offset * offset, -2.0 * offset};
+  out[2] = {std::min(center + 1, length - 1), 0.5 * (0.5 + offset) * (0.5 + offset), offset + 0.5};
+}
+
+// deposit_quadratic_bspline (electrostatic_placement.rs:900-916).
+void deposit_bspline(double *density, size_t width, size_t height, double x, double y, double charge) {
+  AxisEntry ax[3], ay[3];
+  bspline_axis(width, x, ax);
+  bspline_axis(height, y, ay);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      density[ay[r].index * width + ax[c].index] += charge * ax[c].weight * ay[r].weight;
+}
+
+// sample_quadratic_bspline_with_gradient (electrostatic_placement.rs:919-941).
+void sample_bspline(const double *phi, size_t width, size_t height, double x, double y,
+                    double &value, double &gx, double &gy) {
+  AxisEntry ax[3], ay[3];
+  bspline_axis(width, x, ax);
+  bspline_axis(height, y, ay);
+  value = gx = gy = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double p = phi[ay[r].index * width + ax[c].index];
+      value += ax[c].weight * ay[r].weight * p;
+      gx += ax[c].derivative * ay[r].weight * p;
+      gy += ax[c].weight * ay[r].derivative * p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Density model (electrostatic_placement.rs:71-593).
+// ---------------------------------------------------------------------------
+struct Member {
+  uint32_t kind;
+  double ox, oy, charge;
+  uint32_t cell;
+};
+struct DensityUnit {
+  std::vector<Member> members;
+};
+struct Filler {
+  uint32_t kind;
+  double x, y, charge;
+};
+struct FieldResult {
+  uint32_t kind = 0;
+  size_t available = 0;
+  size_t real_charge = 0;
+  double real_area = 0, filler_charge = 0;
+  std::vector<double> density, potential;
+  double energy = 0, overflow = 0, net_charge = 0, force_l1 = 0;
+  std::vector<std::pair<double, double>> unit_gradients;
+  struct FillerGrad {
+    size_t index;
+    double gx, gy;
+  };
+  std::vector<FillerGrad> filler_gradients;
+};
+
+struct DensityModel {
+  Poisson2d solver;
+  std::map<uint32_t, std::vector<uint32_t>> capacity;
+  std::map<uint32_t, std::vector<double>> smoothed;
+
+  size_t W() const { return solver.width; }
+  size_t H() const { return solver.height; }
+
+  // capacity_and_total (electrostatic_placement.rs:423-433).
+  std::pair<const std::vector<uint32_t> *, size_t> capacity_and_total(uint32_t kind) const {
+    static const std::vector<uint32_t> empty;
+    auto it = capacity.find(kind);
+    const std::vector<uint32_t> *cap = it == capacity.end() ? &empty : &it->second;
+    size_t total = 0;
+    for (uint32_t c : *cap) total += c;
+    return {cap, total};
+  }
+};
+
+// Checks used by initial_fillers / evaluate (electrostatic_placement.rs:455-483).
+void ensure_capacity(uint32_t kind, size_t demand, size_t available) {
+  if (demand > available) {
+    Failure f;
+    f.status = TEXO_GP_ERR_INSUFFICIENT_CAPACITY;
+    f.kind = kind;
+    f.v0 = (double)demand;
+    f.v1 = (double)available;
+    throw f;
+  }
+}
+void ensure_area_capacity(uint32_t kind, double demand, double available) {
+  if (!std::isfinite(demand) || demand < 0.0 || demand > available + 1.0e-9) {
+    Failure f;
+    f.status = TEXO_GP_ERR_INSUFFICIENT_AREA;
+    f.kind = kind;
+    f.v0 = demand;
+    f.v1 = available;
+    throw f;
+  }
+}
+
+// DensityModel::new (electrostatic_placement.rs:139-196).
+DensityModel make_density_model(uint32_t width, uint32_t height,
+                                const std::map<uint32_t, std::vector<uint32_t>> &raw,
+                                const std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> &fixed) {
+  DensityModel m;
+  m.solver = Poisson2d(width, height);
+  m.capacity = raw;
+  for (auto &f : fixed) {
+    uint32_t kind = f.first, x = f.second.first, y = f.second.second;
+    if (x >= width || y >= height) {
+      Failure e;
+      e.status = TEXO_GP_ERR_POINT_OUTSIDE_GRID;
+      e.x = x;
+      e.y = y;
+      throw e;
+    }
+    auto it = m.capacity.find(kind);
+    size_t index = (size_t)y * width + x;
+    if (it == m.capacity.end() || it->second[index] == 0) {
+      Failure e;
+      e.status = TEXO_GP_ERR_FIXED_CAPACITY_EXHAUSTED;
+      e.kind = kind;
+      e.x = x;
+      e.y = y;
+      throw e;
+    }
+    it->second[index] -= 1;
+  }
+  for (auto &kv : m.capacity) {
+    std::vector<double> smooth(m.W() * m.H(), 0.0);
+    for (size_t index = 0; index < kv.second.size(); ++index) {
+      uint32_t charge = kv.second[index];
+      if (charge == 0) continue;
+      deposit_bspline(smooth.data(), m.W(), m.H(), (double)(index % m.W()), (double)(index / m.W()),
+                      (double)charge);
+    }
+    m.smoothed[kv.first] = std::move(smooth);
+  }
+  return m;
+}
+
+std::map<uint32_t, size_t> real_counts(const std::vector<DensityUnit> &units) {
+  std::map<uint32_t, size_t> counts;
+  for (auto &u : units)
+    for (auto &m : u.members) counts[m.kind] += 1;
+  return counts;
+}
+std::map<uint32_t, double> real_areas(const std::vector<DensityUnit> &units) {
+  std::map<uint32_t, double> areas;
+  for (auto &u : units)
+    for (auto &m : u.members) areas[m.kind] += m.charge;
+  return areas;
+}
+
+template <class Pos>
+void validate_units(const std::vector<DensityUnit> &units, const Pos &pos) {
+  for (size_t ui = 0; ui < units.size(); ++ui) {
+    if (units[ui].members.empty()) fail(TEXO_GP_ERR_EMPTY_UNIT, (int64_t)ui);
+    auto p = pos(ui);
+    for (size_t mi = 0; mi < units[ui].members.size(); ++mi) {
+      const Member &m = units[ui].members[mi];
+      if (!std::isfinite(p.first) || !std::isfinite(p.second) || !std::isfinite(m.ox) || !std::isfinite(m.oy))
+        fail(TEXO_GP_ERR_NON_FINITE_UNIT_COORDINATE, (int64_t)ui, (int64_t)mi);
+      if (!std::isfinite(m.charge) || m.charge <= 0.0)
+        fail(TEXO_GP_ERR_INVALID_MEMBER_CHARGE, (int64_t)ui, (int64_t)mi);
+    }
+  }
+}
+
+// initial_fillers (electrostatic_placement.rs:198-240).
+template <class UPos>
+std::vector<Filler> initial_fillers(const DensityModel &model, const std::vector<DensityUnit> &units,
+                                   const UPos &origin) {
+  validate_units(units, origin);
+  auto counts = real_counts(units);
+  auto areas = real_areas(units);
+  std::vector<Filler> fillers;
+  for (auto &kv : counts) {
+    uint32_t kind = kv.first;
+    auto ct = model.capacity_and_total(kind);
+    ensure_capacity(kind, kv.second, ct.second);
+    double available_area = (double)ct.second;
+    double real_area = areas[kind];
+    ensure_area_capacity(kind, real_area, available_area);
+    double filler_area = available_area - real_area;
+    if (filler_area == 0.0) continue;
+    std::vector<size_t> occupied;
+    for (size_t i = 0; i < ct.first->size(); ++i)
+      if ((*ct.first)[i] != 0) occupied.push_back(i);
+    double assigned = 0.0;
+    for (size_t ord = 0; ord < occupied.size(); ++ord) {
+      size_t index = occupied[ord];
+      double charge = ord + 1 == occupied.size()
+                          ? filler_area - assigned
+                          : filler_area * (double)(*ct.first)[index] / available_area;
+      assigned += charge;
+      fillers.push_back({kind, (double)(index % model.W()), (double)(index / model.W()), charge});
+    }
+  }
+  return fillers;
+}
+
+// Runs `n` independent jobs on up to g_threads threads, in index order of
+// results (each job writes its own slot).
+void parallel_for(size_t n, const std::function<void(size_t)> &job) {
+  if (g_threads <= 1 || n <= 1) {
+    for (size_t i = 0; i < n; ++i) job(i);
+    return;
+  }
+  std::vector<std::thread> threads;
+  std::vector<std::exception_ptr> errors(n);
+  for (size_t i = 0; i < n; ++i)
+    threads.emplace_back([&, i] {
+      try {
+        job(i);
+      } catch (...) {
+        errors[i] = std::current_exception();
+      }
+    });
+  for (auto &t : threads) t.join();
+  for (auto &e : errors)
+    if (e) std::rethrow_exception(e);
+}
+
+// evaluate_with_positions (electrostatic_placement.rs:265-421).
+template <class UPos, class FPos>
+std::vector<FieldResult> evaluate_density(const DensityModel &model, const std::vector<DensityUnit> &units,
+                                          const std::vector<Filler> &fillers, const UPos &upos,
+                                          const FPos &fpos) {
+  validate_units(units, upos);
+  for (size_t i = 0; i < fillers.size(); ++i) {
+    auto p = fpos(i);
+    if (!std::isfinite(p.first) || !std::isfinite(p.second) || !std::isfinite(fillers[i].charge))
+      fail(TEXO_GP_ERR_NON_FINITE_FILLER, (int64_t)i);
+    if (fillers[i].charge < 0.0) fail(TEXO_GP_ERR_NEGATIVE_FILLER_CHARGE, (int64_t)i);
+  }
+  auto counts = real_counts(units);
+  auto areas = real_areas(units);
+  // validate_filler_totals (electrostatic_placement.rs:506-539)
+  {
+    std::map<uint32_t, double> actual;
+    for (size_t i = 0; i < fillers.size(); ++i) {
+      if (!areas.count(fillers[i].kind)) {
+        Failure f;
+        f.status = TEXO_GP_ERR_UNEXPECTED_FILLER_KIND;
+        f.index = (int64_t)i;
+        f.kind = fillers[i].kind;
+        throw f;
+      }
+      actual[fillers[i].kind] += fillers[i].charge;
+    }
+    for (auto &kv : areas) {
+      double available = (double)model.capacity_and_total(kv.first).second;
+      ensure_area_capacity(kv.first, kv.second, available);
+      double expected = available - kv.second;
+      double supplied = actual.count(kv.first) ? actual[kv.first] : 0.0;
+      if (std::fabs(supplied - expected) > 1.0e-9 * std::max(expected, 1.0)) {
+        Failure f;
+        f.status = TEXO_GP_ERR_FILLER_CHARGE_MISMATCH;
+        f.kind = kv.first;
+        f.v0 = expected;
+        f.v1 = supplied;
+        throw f;
+      }
+    }
+  }
+  std::vector<uint32_t> kinds;
+  for (auto &kv : counts) kinds.push_back(kv.first);
+  std::vector<FieldResult> fields(kinds.size());
+  size_t W = model.W(), H = model.H();
+  parallel_for(kinds.size(), [&](size_t fi) {
+    uint32_t kind = kinds[fi];
+    FieldResult &r = fields[fi];
+    r.kind = kind;
+    size_t available = model.capacity_and_total(kind).second;
+    ensure_capacity(kind, counts[kind], available);
+    ensure_area_capacity(kind, areas.at(kind), (double)available);
+    const std::vector<double> &smooth = model.smoothed.at(kind);
+    r.density.resize(W * H);
+    for (size_t i = 0; i < W * H; ++i) r.density[i] = -smooth[i];
+    for (size_t ui = 0; ui < units.size(); ++ui) {
+      auto p = upos(ui);
+      for (auto &m : units[ui].members) {
+        if (m.kind != kind) continue;
+        deposit_bspline(r.density.data(), W, H, p.first + m.ox, p.second + m.oy, m.charge);
+      }
+    }
+    double physical = 0.0;
+    for (double v : r.density) physical += std::max(v, 0.0);
+    for (size_t i = 0; i < fillers.size(); ++i) {
+      if (fillers[i].kind != kind) continue;
+      auto p = fpos(i);
+      deposit_bspline(r.density.data(), W, H, p.first, p.second, fillers[i].charge);
+    }
+    r.energy = model.solver.solve(r.density, r.potential);
+    r.unit_gradients.assign(units.size(), {0.0, 0.0});
+    double force = 0.0;
+    for (size_t ui = 0; ui < units.size(); ++ui) {
+      auto p = upos(ui);
+      for (auto &m : units[ui].members) {
+        if (m.kind != kind) continue;
+        double v, gx, gy;
+        sample_bspline(r.potential.data(), W, H, p.first + m.ox, p.second + m.oy, v, gx, gy);
+        force += m.charge * (std::fabs(gx) + std::fabs(gy));
+        r.unit_gradients[ui].first += m.charge * gx;
+        r.unit_gradients[ui].second += m.charge * gy;
+      }
+    }
+    for (size_t i = 0; i < fillers.size(); ++i) {
+      if (fillers[i].kind != kind) continue;
+      auto p = fpos(i);
+      double v, gx, gy;
+      sample_bspline(r.potential.data(), W, H, p.first, p.second, v, gx, gy);
+      force += fillers[i].charge * (std::fabs(gx) + std::fabs(gy));
+      r.filler_gradients.push_back({i, fillers[i].charge * gx, fillers[i].charge * gy});
+    }
+    double net = 0.0;
+    for (double v : r.density) net += v;
+    r.available = available;
+    r.real_charge = counts[kind];
+    r.real_area = areas.at(kind);
+    r.filler_charge = (double)available - r.real_area;
+    r.overflow = physical / r.real_area;
+    r.net_charge = net;
+    r.force_l1 = force;
+  });
+  return fields;
+}
+
+// ---------------------------------------------------------------------------
+// Hypergraph and weighted-average wirelength (analytical_placement.rs).
+// ---------------------------------------------------------------------------
+struct Pin {
+  uint32_t unit, cell;
+};
+struct Hypergraph {
+  std::vector<uint32_t> net_start;  // CSR
+  std::vector<Pin> pins;
+  struct Arc {
+    Pin driver, sink;
+    uint64_t extra;
+  };
+  std::vector<Arc> arcs;
+  size_t n_nets() const { return net_start.empty() ? 0 : net_start.size() - 1; }
+};
+
+struct CellGeometry {
+  const std::vector<uint8_t> *fixed;
+  const std::vector<double> *fixed_xy;
+  const std::vector<double> *offsets;
+};
+
+// u64_to_f64 (analytical_placement.rs:705-709)
+double u64_to_f64(uint64_t v) {
+  return (double)(uint32_t)(v >> 32) * 4294967296.0 + (double)(uint32_t)(v & 0xffffffffu);
+}
+
+// baseline_net_preconditioner_weight (analytical_placement.rs:514-518)
+double baseline_net_weight(size_t pin_count) { return 1.0 / (double)(uint32_t)(pin_count - 1); }
+
+// weighted_average_axis (analytical_placement.rs:636-703)
+template <class Coord, class Add>
+double weighted_average_axis(size_t pin_count, double gamma, const Coord &coordinate, const Add &add,
+                             std::vector<double> &sc, std::vector<double> &pw, std::vector<double> &nw) {
+  if (sc.size() < pin_count) {
+    sc.resize(pin_count);
+    pw.resize(pin_count);
+    nw.resize(pin_count);
+  }
+  double first = coordinate(0);
+  sc[0] = first;
+  double mn = first, mx = first;
+  for (size_t i = 1; i < pin_count; ++i) {
+    double v = coordinate(i);
+    sc[i] = v;
+    mn = std::min(mn, v);
+    mx = std::max(mx, v);
+  }
+  double psum = 0, pdelta = 0, nsum = 0, ndelta = 0;
+  for (size_t i = 0; i < pin_count; ++i) {
+    double v = sc[i];
+    double dmax = v - mx, dmin = v - mn;
+    double p = dmax == 0.0 ? 1.0 : std::exp(dmax / gamma);
+    double q = dmin == 0.0 ? 1.0 : std::exp(-dmin / gamma);
+    pw[i] = p;
+    nw[i] = q;
+    psum += p;
+    pdelta += p * dmax;
+    nsum += q;
+    ndelta += q * dmin;
+  }
+  double pmean = pdelta / psum, nmean = ndelta / nsum;
+  for (size_t i = 0; i < pin_count; ++i) {
+    double v = sc[i];
+    double dmax = v - mx, dmin = v - mn;
+    double pp = pw[i] / psum, np = nw[i] / nsum;
+    double d = pp * (1.0 + (dmax - pmean) / gamma) - np * (1.0 - (dmin - nmean) / gamma);
+    add(i, d);
+  }
+  return (mx - mn) + pmean - nmean;
+}
+
+struct WaResult {
+  double value = 0;
+  std::vector<double> gx, gy;
+};
+
+// add_weighted_average_axis (analytical_placement.rs:576-611) over all nets
+// (weighted_average_hypernets_axis :520-545) or arcs (:547-573).
+double wa_axis(const Hypergraph &hg, bool arcs, bool y_axis, double gamma, const std::vector<double> &positions,
+               const CellGeometry &geo, std::vector<double> &grad) {
+  std::vector<double> sc, pw, nw;
+  double value = 0.0;
+  auto coord_of = [&](Pin pin) {
+    if ((*geo.fixed)[pin.cell]) return (*geo.fixed_xy)[2 * pin.cell + (y_axis ? 1 : 0)];
+    return positions[2 * pin.unit + (y_axis ? 1 : 0)] + (*geo.offsets)[2 * pin.cell + (y_axis ? 1 : 0)];
+  };
+  if (!arcs) {
+    for (size_t n = 0; n < hg.n_nets(); ++n) {
+      const Pin *pins = &hg.pins[hg.net_start[n]];
+      size_t count = hg.net_start[n + 1] - hg.net_start[n];
+      double v = weighted_average_axis(
+          count, gamma, [&](size_t i) { return coord_of(pins[i]); },
+          [&](size_t i, double d) {
+            if (!(*geo.fixed)[pins[i].cell]) grad[pins[i].unit] += 1.0 * d;
+          },
+          sc, pw, nw);
+      value += 1.0 * v;
+    }
+  } else {
+    for (auto &arc : hg.arcs) {
+      Pin pins[2] = {arc.driver, arc.sink};
+      double w = u64_to_f64(arc.extra);
+      double v = weighted_average_axis(
+          2, gamma, [&](size_t i) { return coord_of(pins[i]); },
+          [&](size_t i, double d) {
+            if (!(*geo.fixed)[pins[i].cell]) grad[pins[i].unit] += w * d;
+          },
+          sc, pw, nw);
+      value += w * v;
+    }
+  }
+  return value;
+}
+
+// weighted_average_objective_with_global_gamma (analytical_placement.rs:366-463)
+WaResult weighted_average(const Hypergraph &hg, double gamma, const std::vector<double> &positions,
+                          size_t n_units, const CellGeometry &geo) {
+  std::vector<double> g[4];
+  double v[4];
+  for (auto &x : g) x.assign(n_units, 0.0);
+  parallel_for(4, [&](size_t i) {
+    bool arcs = i >= 2, y_axis = (i & 1) != 0;
+    v[i] = wa_axis(hg, arcs, y_axis, gamma, positions, geo, g[i]);
+  });
+  WaResult r;
+  double baseline = v[0] + v[1];
+  double timing = v[2] + v[3];
+  r.value = baseline + timing;
+  r.gx = std::move(g[0]);
+  r.gy = std::move(g[1]);
+  for (size_t u = 0; u < n_units; ++u) {
+    r.gx[u] += g[2][u];
+    r.gy[u] += g[3][u];
+  }
+  return r;
+}
+
+// wirelength_preconditioner (analytical_placement.rs:177-211)
+std::vector<double> baseline_wirelength_preconditioner(const Hypergraph &hg, size_t n_units) {
+  std::vector<double> diag(n_units, 0.0);
+  std::vector<uint32_t> units;
+  for (size_t n = 0; n < hg.n_nets(); ++n) {
+    units.clear();
+    for (uint32_t p = hg.net_start[n]; p < hg.net_start[n + 1]; ++p) units.push_back(hg.pins[p].unit);
+    std::sort(units.begin(), units.end());
+    units.erase(std::unique(units.begin(), units.end()), units.end());
+    if (units.size() < 2) continue;
+    double w = baseline_net_weight(hg.net_start[n + 1] - hg.net_start[n]);
+    for (uint32_t u : units) diag[u] += w;
+  }
+  return diag;
+}
+std::vector<double> wirelength_preconditioner(const Hypergraph &hg, size_t n_units) {
+  std::vector<double> diag = baseline_wirelength_preconditioner(hg, n_units);
+  for (auto &arc : hg.arcs) {
+    if (arc.driver.unit == arc.sink.unit) continue;
+    double w = u64_to_f64(arc.extra);
+    diag[arc.driver.unit] += w;
+    diag[arc.sink.unit] += w;
+  }
+  return diag;
+}
+
+// baseline_cell_incidence_weights (analytical_placement.rs:219-231)
+std::vector<double> baseline_cell_incidence_weights(const Hypergraph &hg, size_t n_cells) {
+  std::vector<double> w(n_cells, 0.0);
+  std::vector<uint32_t> cells;
+  for (size_t n = 0; n < hg.n_nets(); ++n) {
+    cells.clear();
+    for (uint32_t p = hg.net_start[n]; p < hg.net_start[n + 1]; ++p) cells.push_back(hg.pins[p].cell);
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    double weight = baseline_net_weight(hg.net_start[n + 1] - hg.net_start[n]);
+    for (uint32_t c : cells) w[c] += weight;
+  }
+  return w;
+}
+
+// external_baseline_net_bounding_boxes (analytical_placement.rs:238-276)
+std::vector<std::array<double, 4>> net_bounding_boxes(const Hypergraph &hg, const std::vector<double> &positions,
+                                                      const CellGeometry &geo) {
+  std::vector<std::array<double, 4>> boxes;
+  auto coord = [&](Pin pin, int axis) {
+    if ((*geo.fixed)[pin.cell]) return (*geo.fixed_xy)[2 * pin.cell + axis];
+    return positions[2 * pin.unit + axis] + (*geo.offsets)[2 * pin.cell + axis];
+  };
+  for (size_t n = 0; n < hg.n_nets(); ++n) {
+    uint32_t b = hg.net_start[n], e = hg.net_start[n + 1];
+    if (b == e) continue;
+    bool same = true;
+    for (uint32_t p = b; p < e; ++p) same &= hg.pins[p].unit == hg.pins[b].unit;
+    if (same) continue;
+    double x0 = coord(hg.pins[b], 0), y0 = coord(hg.pins[b], 1);
+    std::array<double, 4> box = {x0, x0, y0, y0};
+    for (uint32_t p = b + 1; p < e; ++p) {
+      double x = coord(hg.pins[p], 0), y = coord(hg.pins[p], 1);
+      box[0] = std::min(box[0], x);
+      box[1] = std::max(box[1], x);
+      box[2] = std::min(box[2], y);
+      box[3] = std::max(box[3], y);
+    }
+    boxes.push_back(box);
+  }
+  return boxes;
+}
+
+// Continuous HPWL of every retained net (FP64), the measure reported beside
+// WA; the flow's integer analogue is placement_bbox_hpwl.
+double continuous_hpwl(const Hypergraph &hg, const std::vector<double> &positions, const CellGeometry &geo) {
+  double total = 0.0;
+  auto coord = [&](Pin pin, int axis) {
+    if ((*geo.fixed)[pin.cell]) return (*geo.fixed_xy)[2 * pin.cell + axis];
+    return positions[2 * pin.unit + axis] + (*geo.offsets)[2 * pin.cell + axis];
+  };
+  for (size_t n = 0; n < hg.n_nets(); ++n) {
+    uint32_t b = hg.net_start[n], e = hg.net_start[n + 1];
+    if (b == e) continue;
+    double x0 = coord(hg.pins[b], 0), y0 = coord(hg.pins[b], 1);
+    double xmin = x0, xmax = x0, ymin = y0, ymax = y0;
+    for (uint32_t p = b + 1; p < e; ++p) {
+      double x = coord(hg.pins[p], 0), y = coord(hg.pins[p], 1);
+      xmin = std::min(xmin, x);
+      xmax = std::max(xmax, x);
+      ymin = std::min(ymin, y);
+      ymax = std::max(ymax, y);
+    }
+    total += (xmax - xmin) + (ymax - ymin);
+  }
+  return total;
+}
+
+// continuous_routing_demand (eplace.rs:856-937)
+void continuous_routing_demand(uint32_t width, uint32_t height, const std::vector<std::array<double, 4>> &boxes,
+                               std::vector<double> &hd, std::vector<double> &vd) {
+  size_t tiles = (size_t)width * height;
+  hd.assign(tiles, 0.0);
+  vd.assign(tiles, 0.0);
+  std::vector<std::pair<uint32_t, double>> yo, xo;
+  for (auto &b : boxes) {
+    double minx = b[0], maxx = b[1], miny = b[2], maxy = b[3];
+    if (!std::isfinite(minx) || !std::isfinite(maxx) || !std::isfinite(miny) || !std::isfinite(maxy) ||
+        minx > maxx || miny > maxy)
+      fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    double dx = maxx - minx, dy = maxy - miny;
+    double rminx = minx - 0.5, rmaxx = maxx + 0.5, rminy = miny - 0.5, rmaxy = maxy + 0.5;
+    double area = (dx + 1.0) * (dy + 1.0);
+    if (!std::isfinite(area) || area <= 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    double hdens = dx / area, vdens = dy / area;
+    yo.clear();
+    xo.clear();
+    for (uint32_t y = 0; y < height; ++y) {
+      double tmin = (double)y - 0.5, tmax = (double)y + 0.5;
+      double o = std::max(std::min(tmax, rmaxy) - std::max(tmin, rminy), 0.0);
+      if (o > 0.0) yo.push_back({y, o});
+    }
+    for (uint32_t x = 0; x < width; ++x) {
+      double tmin = (double)x - 0.5, tmax = (double)x + 0.5;
+      double o = std::max(std::min(tmax, rmaxx) - std::max(tmin, rminx), 0.0);
+      if (o > 0.0) xo.push_back({x, o});
+    }
+    for (auto &yy : yo)
+      for (auto &xx : xo) {
+        double overlap = xx.second * yy.second;
+        size_t index = (size_t)yy.first * width + xx.first;
+        hd[index] += hdens * overlap;
+        vd[index] += vdens * overlap;
+      }
+  }
+  for (size_t i = 0; i < tiles; ++i)
+    if (!std::isfinite(hd[i]) || hd[i] < 0.0 || !std::isfinite(vd[i]) || vd[i] < 0.0)
+      fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+}
+
+// ---------------------------------------------------------------------------
+// Dynamic Nesterov step (global_placement.rs:121-335, helpers :518-712).
+// ---------------------------------------------------------------------------
+struct Bounds {
+  std::vector<double> lower, upper;
+};
+struct NState {
+  std::vector<double> x, xp;
+  double momentum = 1.0;
+  bool has_l = false;
+  double l = 0.0;
+};
+struct Eval {
+  double value;
+  std::vector<double> gradient;
+};
+
+double kkt_component(size_t i, double g, const double *x, const Bounds &b) {
+  bool equal_bounds = b.lower[i] == b.upper[i] && !(std::signbit(b.lower[i]) != std::signbit(b.upper[i]));
+  if (equal_bounds || (x[i] <= b.lower[i] && g > 0.0) || (x[i] >= b.upper[i] && g < 0.0)) return 0.0;
+  return std::fabs(g);
+}
+double box_metric_kkt(const std::vector<double> &g, const std::vector<double> &x, const Bounds &b,
+                      const std::vector<double> &m) {
+  double r = 0.0;
+  for (size_t i = 0; i < g.size(); ++i) r = std::max(r, kkt_component(i, g[i], x.data(), b) / std::sqrt(m[i]));
+  return r;
+}
+double half_nonzero(double v) {
+  double h = v * 0.5;
+  return h == 0.0 ? v : h;
+}
+double rounding_margin(double a, double b) { return DBL_EPSILON * (1.0 + std::fabs(a) + std::fabs(b)); }
+
+std::vector<double> project(const std::vector<double> &x, const Bounds &b) {
+  std::vector<double> r(x.size());
+  for (size_t i = 0; i < x.size(); ++i) r[i] = clampd(x[i], b.lower[i], b.upper[i]);
+  return r;
+}
+
+void validate_bounds(const std::vector<double> &x, const Bounds &b) {
+  if (b.lower.size() != x.size() || b.upper.size() != x.size()) fail(TEXO_GP_ERR_INVALID_BOUNDS);
+  for (size_t i = 0; i < x.size(); ++i) {
+    if (!std::isfinite(b.lower[i]) || !std::isfinite(b.upper[i]) || b.lower[i] > b.upper[i])
+      fail(TEXO_GP_ERR_INVALID_BOUNDS);
+    if (!std::isfinite(x[i])) fail(TEXO_GP_ERR_INVALID_INITIAL_COORDINATE, (int64_t)i);
+  }
+}
+
+void checked_evaluation(size_t n, const Eval &e) {
+  if (!std::isfinite(e.value)) fail(TEXO_GP_ERR_INVALID_OBJECTIVE);
+  if (e.gradient.size() != n) fail(TEXO_GP_ERR_INVALID_GRADIENT_LENGTH, (int64_t)n, (int64_t)e.gradient.size());
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(e.gradient[i])) fail(TEXO_GP_ERR_NON_FINITE_GRADIENT, (int64_t)i);
+}
+
+struct NStep {
+  bool accepted;
+  double objective, stationarity, coordinate_change;
+  uint64_t trials;
+  size_t payload;  // index of the evaluation whose payload is returned
+};
+
+// `evaluate(x, keep)` returns the evaluation and a payload slot id.
+NStep dynamic_nesterov_step(NState &state, const Bounds &bounds, const std::vector<double> &metric,
+                            const std::function<Eval(const std::vector<double> &, size_t &)> &evaluate) {
+  validate_bounds(state.x, bounds);
+  validate_bounds(state.xp, bounds);
+  if (metric.size() != state.x.size()) fail(TEXO_GP_ERR_INVALID_METRIC_LENGTH, (int64_t)state.x.size(), (int64_t)metric.size());
+  for (size_t i = 0; i < metric.size(); ++i)
+    if (!std::isfinite(metric[i]) || metric[i] <= 0.0) fail(TEXO_GP_ERR_NON_POSITIVE_METRIC, (int64_t)i);
+  if (!std::isfinite(state.momentum) || state.momentum < 1.0) fail(TEXO_GP_ERR_INVALID_MOMENTUM);
+  if (state.has_l && (!std::isfinite(state.l) || state.l <= 0.0)) fail(TEXO_GP_ERR_INVALID_LIPSCHITZ);
+
+  const size_t n = state.x.size();
+  std::vector<double> origin = project(state.x, bounds);
+  std::vector<double> previous = project(state.xp, bounds);
+  size_t origin_payload = 0;
+  Eval origin_eval = evaluate(origin, origin_payload);
+  checked_evaluation(n, origin_eval);
+  bool restart = false;
+  bool has_wl = state.has_l;
+  double working_l = state.l;
+  uint64_t trials = 0;
+  for (;;) {
+    double momentum = restart ? 1.0 : state.momentum;
+    double next_momentum = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * momentum * momentum));
+    if (!std::isfinite(next_momentum)) fail(TEXO_GP_ERR_ITERATION_OVERFLOW);
+    double extrapolation = (momentum - 1.0) / next_momentum;
+    std::vector<double> y(n);
+    if (restart) {
+      y = origin;
+    } else {
+      for (size_t i = 0; i < n; ++i)
+        y[i] = clampd(origin[i] + extrapolation * (origin[i] - previous[i]), bounds.lower[i], bounds.upper[i]);
+    }
+    Eval y_eval;
+    if (y == origin) {
+      y_eval = origin_eval;
+    } else {
+      size_t ignored = 0;
+      y_eval = evaluate(y, ignored);
+      checked_evaluation(n, y_eval);
+    }
+    double y_stat = box_metric_kkt(y_eval.gradient, y, bounds, metric);
+    if (!std::isfinite(y_stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+    auto stationary_return = [&](double l, bool set_l) {
+      state.x = origin;
+      state.xp = origin;
+      state.momentum = 1.0;
+      if (set_l) {
+        state.has_l = true;
+        state.l = l;
+      }
+      NStep s;
+      s.accepted = false;
+      s.objective = origin_eval.value;
+      s.stationarity = box_metric_kkt(origin_eval.gradient, origin, bounds, metric);
+      s.coordinate_change = 0.0;
+      s.trials = trials;
+      s.payload = origin_payload;
+      return s;
+    };
+    if (y_stat == 0.0) {
+      if (y != origin) {
+        restart = true;
+        continue;
+      }
+      return stationary_return(0.0, false);
+    }
+    double initial_l = has_wl ? working_l : y_stat;
+    double trial_l = half_nonzero(initial_l);
+    bool may_reduce = true;
+    bool restart_outer = false;
+    for (;;) {
+      trials += 1;
+      bool underflow = false;
+      for (double m : metric) {
+        double d = trial_l * m;
+        if (!std::isfinite(d)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+        underflow |= d == 0.0;
+      }
+      if (underflow) {
+        trial_l *= 2.0;
+        may_reduce = false;
+        if (!std::isfinite(trial_l)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+        continue;
+      }
+      std::vector<double> cand(n);
+      for (size_t i = 0; i < n; ++i)
+        cand[i] = clampd(y[i] - y_eval.gradient[i] / (trial_l * metric[i]), bounds.lower[i], bounds.upper[i]);
+      if (cand == y) {
+        if (may_reduce) {
+          double smaller = half_nonzero(trial_l);
+          if (smaller < trial_l) {
+            trial_l = smaller;
+            continue;
+          }
+        }
+        if (y != origin) {
+          restart = true;
+          has_wl = true;
+          working_l = trial_l;
+          restart_outer = true;
+          break;
+        }
+        return stationary_return(trial_l, true);
+      }
+      size_t cand_payload = 0;
+      Eval ce = evaluate(cand, cand_payload);
+      checked_evaluation(n, ce);
+      std::vector<double> delta(n);
+      for (size_t i = 0; i < n; ++i) delta[i] = cand[i] - y[i];
+      double msq = 0.0;
+      for (size_t i = 0; i < n; ++i) msq += metric[i] * delta[i] * delta[i];
+      double upper = y_eval.value + dot(y_eval.gradient.data(), delta.data(), n) + 0.5 * trial_l * msq;
+      if (!std::isfinite(upper)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+      bool smooth = ce.value <= upper + rounding_margin(ce.value, upper);
+      bool monotone = ce.value <= origin_eval.value + rounding_margin(ce.value, origin_eval.value);
+      if (smooth && monotone) {
+        double change = 0.0;
+        for (size_t i = 0; i < n; ++i) change = std::max(change, std::fabs(cand[i] - origin[i]));
+        double stat = box_metric_kkt(ce.gradient, cand, bounds, metric);
+        if (!std::isfinite(stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+        state.xp = origin;
+        state.x = cand;
+        state.momentum = next_momentum;
+        state.has_l = true;
+        state.l = trial_l;
+        NStep s;
+        s.accepted = true;
+        s.objective = ce.value;
+        s.stationarity = stat;
+        s.coordinate_change = change;
+        s.trials = trials;
+        s.payload = cand_payload;
+        return s;
+      }
+      if (smooth && !restart) {
+        restart = true;
+        has_wl = true;
+        working_l = trial_l;
+        restart_outer = true;
+        break;
+      }
+      trial_l *= 2.0;
+      may_reduce = false;
+      if (!std::isfinite(trial_l)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+    }
+    if (restart_outer) continue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Schedules (eplace.rs:1111-1439).
+// ---------------------------------------------------------------------------
+std::pair<double, double> augmented_density(double energy, double initial, double multiplier) {
+  double e = energy / initial;
+  return {multiplier * initial * (e + 0.5 * AUGMENTED_DENSITY_BETA * e * e),
+          multiplier * (1.0 + AUGMENTED_DENSITY_BETA * e)};
+}
+
+double field_wirelength_gamma(double overflow, double target, double base) {
+  if (!std::isfinite(overflow) || overflow < 0.0 || !std::isfinite(target) || !(target >= 0.0 && target < 1.0) ||
+      !std::isfinite(base) || base <= 0.0)
+    fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  double slope = 2.0 / (1.0 - target);
+  double intercept = 1.0 - slope;
+  double gamma = base * std::pow(10.0, slope * overflow + intercept);
+  if (!(std::isfinite(gamma) && gamma > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return gamma;
+}
+
+double weighted_gamma_mean(const std::vector<std::pair<double, double>> &gw, double fallback) {
+  if (!std::isfinite(fallback) || fallback <= 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  double ws = 0.0, w = 0.0;
+  for (auto &p : gw) {
+    if (!std::isfinite(p.first) || p.first <= 0.0 || !std::isfinite(p.second) || p.second < 0.0)
+      fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    ws += p.first * p.second;
+    w += p.second;
+  }
+  double mean = w == 0.0 ? fallback : ws / w;
+  if (!(std::isfinite(mean) && mean > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return mean;
+}
+
+double multiplier_growth(double logarithm) {
+  if (!std::isfinite(logarithm) || logarithm < 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  double interp = 1.0 - 1.0 / (1.0 + logarithm);
+  double growth = MULTIPLIER_ALPHA_LOW + interp * (MULTIPLIER_ALPHA_HIGH - MULTIPLIER_ALPHA_LOW);
+  if (!(std::isfinite(growth) && growth >= MULTIPLIER_ALPHA_LOW && growth <= MULTIPLIER_ALPHA_HIGH))
+    fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return growth;
+}
+
+// ---------------------------------------------------------------------------
+// Instance area (instance_area.rs:64-372) and register clustering
+// (register_clustering.rs:17-96).
+// ---------------------------------------------------------------------------
+double routability_optimized_area(double area, double h, double v) {
+  if (!std::isfinite(area) || area < 0.0 || !std::isfinite(h) || h < 0.0 || !std::isfinite(v) || v < 0.0)
+    fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+  double u = std::max(h, v);
+  double factor = std::min(u * u, 2.0);
+  double r = area * factor;
+  if (!std::isfinite(r)) fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+  return r;
+}
+
+struct AreaAdjustment {
+  std::vector<double> member_areas, filler_areas;
+  bool requires_reset = false;
+};
+
+void checked_add(std::map<uint32_t, double> &t, uint32_t kind, double v) {
+  double &x = t[kind];
+  x += v;
+  if (!std::isfinite(x)) fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+}
+
+AreaAdjustment adjust_instance_areas(const std::vector<oracle_area_member> &members,
+                                     const std::vector<std::pair<uint32_t, double>> &fillers) {
+  for (size_t i = 0; i < members.size(); ++i) {
+    auto &m = members[i];
+    if (!std::isfinite(m.current_area) || m.current_area < 0.0 || !std::isfinite(m.routability_area) ||
+        m.routability_area < 0.0 || !std::isfinite(m.pin_area) || m.pin_area < 0.0 ||
+        !std::isfinite(m.clustering_area) || m.clustering_area < 0.0)
+      fail(TEXO_GP_ERR_INVALID_MEMBER_AREA, (int64_t)i);
+  }
+  for (size_t i = 0; i < fillers.size(); ++i)
+    if (!std::isfinite(fillers[i].second) || fillers[i].second < 0.0) fail(TEXO_GP_ERR_INVALID_FILLER_AREA, (int64_t)i);
+  std::vector<double> requested(members.size()), deltas(members.size());
+  for (size_t i = 0; i < members.size(); ++i) {
+    auto &m = members[i];
+    requested[i] = std::max(std::max(std::max(m.routability_area, m.pin_area), m.clustering_area), m.current_area);
+    deltas[i] = requested[i] - m.current_area;
+  }
+  std::map<uint32_t, double> desired, available;
+  for (size_t i = 0; i < members.size(); ++i) checked_add(desired, members[i].kind, deltas[i]);
+  for (auto &f : fillers) checked_add(available, f.first, f.second);
+  std::map<uint32_t, double> scales, target;
+  for (auto &kv : desired) {
+    double a = available.count(kv.first) ? available[kv.first] : 0.0;
+    scales[kv.first] = kv.second == 0.0 ? 1.0 : std::min(a / kv.second, 1.0);
+    target[kv.first] = std::min(kv.second, a);
+  }
+  std::map<uint32_t, size_t> last;
+  for (size_t i = 0; i < members.size(); ++i) last[members[i].kind] = i;
+  std::map<uint32_t, double> actual;
+  AreaAdjustment adj;
+  for (size_t i = 0; i < members.size(); ++i) {
+    auto &m = members[i];
+    double scale = scales[m.kind];
+    double proportional = deltas[i] * scale;
+    double increase;
+    if (last[m.kind] == i) {
+      double assigned = actual.count(m.kind) ? actual[m.kind] : 0.0;
+      increase = clampd(target[m.kind] - assigned, 0.0, deltas[i]);
+    } else {
+      increase = proportional;
+    }
+    double adjusted = scale == 1.0 ? requested[i] : m.current_area + increase;
+    if (!std::isfinite(adjusted)) fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+    checked_add(actual, m.kind, adjusted - m.current_area);
+    adj.member_areas.push_back(adjusted);
+  }
+  // adjusted_filler_areas (instance_area.rs:293-332)
+  bool all_zero = true;
+  for (auto &kv : actual) all_zero &= kv.second == 0.0;
+  if (all_zero) {
+    for (auto &f : fillers) adj.filler_areas.push_back(f.second);
+  } else {
+    std::map<uint32_t, double> remaining, assigned;
+    for (auto &kv : available) {
+      double inc = actual.count(kv.first) ? actual[kv.first] : 0.0;
+      remaining[kv.first] = std::max(kv.second - inc, 0.0);
+    }
+    std::map<uint32_t, size_t> last_f;
+    for (size_t i = 0; i < fillers.size(); ++i) last_f[fillers[i].first] = i;
+    for (size_t i = 0; i < fillers.size(); ++i) {
+      uint32_t kind = fillers[i].first;
+      double cur = fillers[i].second;
+      double total = available[kind];
+      double tgt = remaining[kind];
+      double area;
+      if (last_f[kind] == i) {
+        double a = assigned.count(kind) ? assigned[kind] : 0.0;
+        area = std::max(tgt - a, 0.0);
+      } else if (total == 0.0) {
+        area = 0.0;
+      } else {
+        area = cur * tgt / total;
+      }
+      double tolerance = 64.0 * DBL_EPSILON * std::max(std::fabs(cur), 1.0);
+      if (!std::isfinite(area) || area > cur + tolerance) fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+      area = std::min(area, cur);
+      checked_add(assigned, kind, area);
+      adj.filler_areas.push_back(area);
+    }
+  }
+  // validate_postconditions (instance_area.rs:334-367)
+  {
+    std::map<uint32_t, double> before, after;
+    for (size_t i = 0; i < members.size(); ++i) {
+      double a = adj.member_areas[i];
+      if (a < members[i].current_area || !std::isfinite(a)) fail(TEXO_GP_ERR_POSTCONDITION_VIOLATION);
+      checked_add(before, members[i].kind, members[i].current_area);
+      checked_add(after, members[i].kind, a);
+    }
+    for (size_t i = 0; i < fillers.size(); ++i) {
+      double a = adj.filler_areas[i];
+      if (a < 0.0 || a > fillers[i].second || !std::isfinite(a)) fail(TEXO_GP_ERR_POSTCONDITION_VIOLATION);
+      checked_add(before, fillers[i].first, fillers[i].second);
+      checked_add(after, fillers[i].first, a);
+    }
+    double terms = std::max((double)(uint32_t)(members.size() + fillers.size()), 1.0);
+    for (auto &kv : before) {
+      double a = after.count(kv.first) ? after[kv.first] : 0.0;
+      double tol = 64.0 * DBL_EPSILON * terms * std::max(std::fabs(kv.second), 1.0);
+      if (std::fabs(kv.second - a) > tol) fail(TEXO_GP_ERR_POSTCONDITION_VIOLATION);
+    }
+  }
+  for (auto &kv : actual) adj.requires_reset |= kv.second > 0.0;
+  return adj;
+}
+
+double soft_division_ceiling(double value, double divisor) {
+  double q = value / divisor;
+  double fl = std::floor(q);
+  if (q - fl < 1.0 / divisor) return value + (1.0 - divisor) * fl;
+  return std::ceil(q);
+}
+
+struct Control {
+  uint64_t clock, lsr, ce;
+};
+
+double gaussian_interval_probability(double low, double high, double mean, double sigma) {
+  double scale = sigma * kSqrt2;
+  return clampd(0.5 * (std::erf((high - mean) / scale) - std::erf((low - mean) / scale)), 0.0, 1.0);
+}
+
+// clustering_areas (register_clustering.rs:17-81).  Returns false for None.
+bool clustering_areas(const std::vector<std::pair<uint32_t, std::pair<double, double>>> &regs,
+                      const std::map<uint32_t, Control> &controls, uint64_t movable, std::vector<double> &areas) {
+  areas.assign(regs.size(), 0.0);
+  if (regs.empty() || movable == 0) return true;
+  if (movable > 0xffffffffull) return false;
+  double sigma = std::sqrt(1.0e-5 * (double)(uint32_t)movable);
+  if (!std::isfinite(sigma) || sigma <= 0.0) return false;
+  double half = 2.5 * sigma;
+  std::map<std::pair<uint64_t, uint64_t>, std::vector<size_t>> groups;
+  for (size_t i = 0; i < regs.size(); ++i) {
+    auto it = controls.find(regs[i].first);
+    if (it == controls.end()) return false;
+    groups[{it->second.clock, it->second.lsr}].push_back(i);
+  }
+  bool ok = true;
+  std::vector<char> bad(regs.size(), 0);
+  parallel_for(std::min<size_t>((size_t)std::max(g_threads, 1), regs.size()), [&](size_t t) {
+    size_t stride = (size_t)std::max(g_threads, 1);
+    for (size_t r = t; r < regs.size(); r += stride) {
+      const Control &focal = controls.at(regs[r].first);
+      std::map<uint64_t, double> expected;
+      for (size_t o : groups[{focal.clock, focal.lsr}]) {
+        const Control &oc = controls.at(regs[o].first);
+        double px = gaussian_interval_probability(regs[r].second.first - half, regs[r].second.first + half,
+                                                  regs[o].second.first, sigma);
+        double py = gaussian_interval_probability(regs[r].second.second - half, regs[r].second.second + half,
+                                                  regs[o].second.second, sigma);
+        expected[oc.ce] += px * py;
+      }
+      double ef = expected.count(focal.ce) ? expected[focal.ce] : 0.0;
+      if (!(std::isfinite(ef) && ef > 0.0)) {
+        bad[r] = 1;
+        continue;
+      }
+      double total = 0.0;
+      for (auto &kv : expected) total += soft_division_ceiling(kv.second, 2.0);
+      double focal_demand = soft_division_ceiling(ef, 2.0);
+      double area = 8.0 / ef * focal_demand / total * soft_division_ceiling(total, 4.0);
+      if (!(std::isfinite(area) && area >= 0.0)) {
+        bad[r] = 1;
+        continue;
+      }
+      areas[r] = area;
+    }
+  });
+  for (char b : bad) ok &= !b;
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
+// Placement problem (eplace.rs:242-421) and its optimisation loop.
+// ---------------------------------------------------------------------------
+struct Problem {
+  uint32_t W = 0, H = 0;
+  size_t n_units = 0, n_cells = 0;
+  std::vector<uint8_t> unit_fixed;
+  std::vector<double> initial_xy, bounds4;
+  std::vector<uint8_t> cell_fixed;
+  std::vector<double> cell_fixed_xy, cell_offsets;
+  std::vector<uint32_t> cell_unit;
+  Hypergraph hg;
+  bool has_routing = false;
+  std::vector<uint32_t> routing_h, routing_v;
+  std::map<uint32_t, Control> controls;
+  size_t n_controls_raw = 0;
+  bool best_effort = false;
+
+  // ContinuousProblem (eplace.rs:52-65)
+  DensityModel model;
+  std::vector<size_t> movable_units;
+  std::vector<DensityUnit> density_units;
+  std::map<uint32_t, double> gamma_weights;
+  std::vector<Filler> fillers;
+  uint64_t movable_instance_count = 0;
+  std::vector<double> wl_precond;
+
+  CellGeometry geo() const { return CellGeometry{&cell_fixed, &cell_fixed_xy, &cell_offsets}; }
+};
+
+Problem load_problem(const texo_gp_problem *p) {
+  if (!p || p->abi_version != TEXO_GP_ABI_VERSION) fail(TEXO_GP_ERR_INVALID_ARGUMENT);
+  Problem P;
+  P.W = p->width;
+  P.H = p->height;
+  P.n_units = p->n_units;
+  P.n_cells = p->n_cells;
+  P.unit_fixed.assign(p->unit_fixed, p->unit_fixed + p->n_units);
+  P.initial_xy.assign(p->unit_initial_xy, p->unit_initial_xy + 2 * p->n_units);
+  P.bounds4.assign(p->unit_bounds, p->unit_bounds + 4 * p->n_units);
+  P.cell_fixed.assign(p->cell_fixed, p->cell_fixed + p->n_cells);
+  P.cell_fixed_xy.assign(p->cell_fixed_xy, p->cell_fixed_xy + 2 * p->n_cells);
+  P.cell_offsets.assign(p->cell_offset_xy, p->cell_offset_xy + 2 * p->n_cells);
+  P.cell_unit.assign(p->cell_unit, p->cell_unit + p->n_cells);
+  for (size_t c = 0; c < P.n_cells; ++c)
+    if (P.cell_unit[c] >= P.n_units) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)c);
+  P.hg.net_start.assign(p->net_pin_start, p->net_pin_start + p->n_nets + 1);
+  size_t n_pins = p->n_nets ? p->net_pin_start[p->n_nets] : 0;
+  for (size_t i = 0; i < n_pins; ++i) {
+    uint32_t cell = p->pin_cell[i];
+    if (cell >= P.n_cells) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)i);
+    P.hg.pins.push_back({P.cell_unit[cell], cell});
+  }
+  for (size_t n = 0; n < p->n_nets; ++n)
+    if (p->net_pin_start[n + 1] < p->net_pin_start[n] + 2) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)n);
+  for (size_t a = 0; a < p->n_arcs; ++a) {
+    uint32_t d = p->arc_cells[2 * a], s = p->arc_cells[2 * a + 1];
+    if (d >= P.n_cells || s >= P.n_cells || p->arc_extra_weight[a] == 0) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)a);
+    P.hg.arcs.push_back({{P.cell_unit[d], d}, {P.cell_unit[s], s}, p->arc_extra_weight[a]});
+  }
+  P.has_routing = p->routing_horizontal && p->routing_vertical;
+  if (P.has_routing) {
+    P.routing_h.assign(p->routing_horizontal, p->routing_horizontal + (size_t)P.W * P.H);
+    P.routing_v.assign(p->routing_vertical, p->routing_vertical + (size_t)P.W * P.H);
+  }
+  P.n_controls_raw = p->n_register_controls;
+  for (size_t i = 0; i < p->n_register_controls; ++i)
+    P.controls[p->register_cell[i]] = {p->register_clock_lsr[2 * i], p->register_clock_lsr[2 * i + 1],
+                                       p->register_ce[i]};
+  P.best_effort = p->best_effort != 0;
+
+  // density units (eplace.rs:261-301)
+  for (size_t u = 0; u < P.n_units; ++u) {
+    uint32_t b = p->unit_member_start[u], e = p->unit_member_start[u + 1];
+    if (P.unit_fixed[u] || b == e) continue;
+    DensityUnit du;
+    for (uint32_t m = b; m < e; ++m) {
+      if (!exchangeable(p->member_kind[m])) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)m);
+      du.members.push_back({p->member_kind[m], p->member_offset_xy[2 * m], p->member_offset_xy[2 * m + 1], 1.0,
+                            p->member_cell[m]});
+    }
+    P.movable_units.push_back(u);
+    P.density_units.push_back(std::move(du));
+  }
+  for (size_t c = 0; c < P.n_cells; ++c)
+    if (!P.unit_fixed[P.cell_unit[c]]) P.movable_instance_count += 1;
+  return P;
+}
+
+void finish_setup(Problem &P, const texo_gp_problem *p) {
+  std::map<uint32_t, std::vector<uint32_t>> raw;
+  size_t G = (size_t)P.W * P.H;
+  for (size_t k = 0; k < p->n_capacity_kinds; ++k)
+    raw[p->capacity_kind[k]] = std::vector<uint32_t>(p->capacity + k * G, p->capacity + (k + 1) * G);
+  std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> fixed;
+  for (size_t i = 0; i < p->n_fixed_occupancy; ++i)
+    fixed.push_back({p->fixed_occupancy_kind[i], {p->fixed_occupancy_xy[2 * i], p->fixed_occupancy_xy[2 * i + 1]}});
+  P.model = make_density_model(P.W, P.H, raw, fixed);
+  P.fillers = initial_fillers(P.model, P.density_units, [&](size_t i) {
+    size_t u = P.movable_units[i];
+    return std::make_pair(P.initial_xy[2 * u], P.initial_xy[2 * u + 1]);
+  });
+  // density_gamma_weights (eplace.rs:1424-1439)
+  std::vector<double> cw = baseline_cell_incidence_weights(P.hg, P.n_cells);
+  for (auto &du : P.density_units)
+    for (auto &m : du.members) P.gamma_weights[m.kind] += cw[m.cell];
+  P.wl_precond = wirelength_preconditioner(P.hg, P.n_units);
+  if (P.controls.size() != P.n_controls_raw) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+}
+
+struct Evaluation {
+  WaResult wa;
+  std::vector<FieldResult> fields;
+};
+
+// ContinuousProblem::evaluate (eplace.rs:137-205)
+Evaluation evaluate(const Problem &P, const std::vector<double> &coords, double gamma) {
+  std::vector<double> unit_pos(coords.begin(), coords.begin() + 2 * P.n_units);
+  Evaluation ev;
+  auto upos = [&](size_t i) {
+    size_t u = P.movable_units[i];
+    return std::make_pair(unit_pos[2 * u], unit_pos[2 * u + 1]);
+  };
+  auto fpos = [&](size_t i) {
+    size_t c = 2 * (P.n_units + i);
+    return std::make_pair(coords[c], coords[c + 1]);
+  };
+  if (g_threads > 1) {
+    std::exception_ptr err;
+    std::thread t([&] {
+      try {
+        ev.wa = weighted_average(P.hg, gamma, unit_pos, P.n_units, P.geo());
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+    std::exception_ptr derr;
+    try {
+      ev.fields = evaluate_density(P.model, P.density_units, P.fillers, upos, fpos);
+    } catch (...) {
+      derr = std::current_exception();
+    }
+    t.join();
+    if (derr) std::rethrow_exception(derr);
+    if (err) std::rethrow_exception(err);
+  } else {
+    ev.wa = weighted_average(P.hg, gamma, unit_pos, P.n_units, P.geo());
+    ev.fields = evaluate_density(P.model, P.density_units, P.fillers, upos, fpos);
+  }
+  return ev;
+}
+
+// add_scaled_field_gradient (eplace.rs:213-231)
+void add_scaled_field_gradient(const Problem &P, const FieldResult &f, double coef, std::vector<double> &g) {
+  for (size_t i = 0; i < P.movable_units.size(); ++i) {
+    size_t u = P.movable_units[i];
+    g[2 * u] += coef * f.unit_gradients[i].first;
+    g[2 * u + 1] += coef * f.unit_gradients[i].second;
+  }
+  for (auto &fg : f.filler_gradients) {
+    size_t c = 2 * (P.n_units + fg.index);
+    g[c] += coef * fg.gx;
+    g[c + 1] += coef * fg.gy;
+  }
+}
+
+std::vector<double> wirelength_gradient(const WaResult &wa) {
+  std::vector<double> g(2 * wa.gx.size());
+  for (size_t u = 0; u < wa.gx.size(); ++u) {
+    g[2 * u] = wa.gx[u];
+    g[2 * u + 1] = wa.gy[u];
+  }
+  return g;
+}
+
+// combined_objective (eplace.rs:1022-1044)
+Eval combined_objective(const Problem &P, const Evaluation &ev, const std::vector<double> &scales,
+                        const std::vector<double> &mult, size_t C) {
+  Eval e;
+  e.value = ev.wa.value;
+  e.gradient = wirelength_gradient(ev.wa);
+  e.gradient.resize(C, 0.0);
+  for (size_t f = 0; f < ev.fields.size(); ++f) {
+    auto vc = augmented_density(ev.fields[f].energy, scales[f], mult[f]);
+    e.value += vc.first;
+    add_scaled_field_gradient(P, ev.fields[f], vc.second, e.gradient);
+  }
+  return e;
+}
+
+// diagonal_preconditioner (eplace.rs:1046-1100)
+std::vector<double> diagonal_preconditioner(const Problem &P, const Evaluation &ev, const std::vector<double> &mult,
+                                            size_t C) {
+  std::vector<double> d;
+  d.reserve(C);
+  for (double w : P.wl_precond) {
+    d.push_back(w);
+    d.push_back(w);
+  }
+  d.resize(C, 0.0);
+  for (size_t f = 0; f < ev.fields.size(); ++f) {
+    double coef = mult[f];
+    if (!std::isfinite(coef) || coef < 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    uint32_t kind = ev.fields[f].kind;
+    for (size_t i = 0; i < P.movable_units.size(); ++i) {
+      double charge = 0.0;
+      for (auto &m : P.density_units[i].members)
+        if (m.kind == kind) charge += m.charge;
+      if (charge == 0.0) continue;
+      size_t u = P.movable_units[i];
+      d[2 * u] += coef * charge;
+      d[2 * u + 1] += coef * charge;
+    }
+    for (size_t i = 0; i < P.fillers.size(); ++i) {
+      if (P.fillers[i].kind != kind) continue;
+      size_t c = 2 * (P.n_units + i);
+      d[c] += coef * P.fillers[i].charge;
+      d[c + 1] += coef * P.fillers[i].charge;
+    }
+  }
+  for (double &x : d) {
+    if (!std::isfinite(x) || x < 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    x = std::max(x, 1.0);
+  }
+  return d;
+}
+
+double l1(const std::vector<double> &v) {
+  double s = 0.0;
+  for (double x : v) s += std::fabs(x);
+  return s;
+}
+
+double initial_density_multiplier(const Problem &P, const Evaluation &ev, size_t C) {
+  double wf = l1(wirelength_gradient(ev.wa));
+  double df = 0.0;
+  for (auto &f : ev.fields) {
+    std::vector<double> g(C, 0.0);
+    add_scaled_field_gradient(P, f, 1.0, g);
+    df += l1(g);
+  }
+  double m = (wf == 0.0 || df == 0.0) ? 1.0 : INITIAL_DENSITY_WEIGHT * wf / df;
+  if (!(std::isfinite(m) && m > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return m;
+}
+
+void validate_fields(const std::vector<FieldResult> &fields, const std::vector<double> &scales) {
+  if (fields.size() != scales.size()) fail(TEXO_GP_ERR_DENSITY_FIELD_CHANGED);
+  for (size_t i = 0; i < fields.size(); ++i)
+    if (!std::isfinite(fields[i].energy) || !std::isfinite(scales[i]) || scales[i] <= 0.0)
+      fail(TEXO_GP_ERR_DENSITY_FIELD_CHANGED);
+}
+
+void validate_initial_evaluation(const Evaluation &ev) {
+  bool ok = std::isfinite(ev.wa.value);
+  for (double g : ev.wa.gx) ok &= std::isfinite(g);
+  for (double g : ev.wa.gy) ok &= std::isfinite(g);
+  for (auto &f : ev.fields) {
+    ok &= std::isfinite(f.energy) && f.energy >= 0.0 && std::isfinite(f.overflow) && std::isfinite(f.net_charge) &&
+          std::isfinite(f.force_l1) && f.force_l1 >= 0.0;
+    for (auto &g : f.unit_gradients) ok &= std::isfinite(g.first) && std::isfinite(g.second);
+    for (auto &g : f.filler_gradients) ok &= std::isfinite(g.gx) && std::isfinite(g.gy);
+  }
+  if (!ok) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+}
+
+std::vector<double> normalized_density_subgradient(const std::vector<FieldResult> &fields,
+                                                   const std::vector<double> &e0) {
+  validate_fields(fields, e0);
+  std::vector<double> s;
+  for (size_t i = 0; i < fields.size(); ++i) {
+    double e = fields[i].energy / e0[i];
+    s.push_back(e + 0.5 * AUGMENTED_DENSITY_BETA * e * e);
+  }
+  for (double v : s)
+    if (!(std::isfinite(v) && v >= 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return s;
+}
+
+std::vector<double> area_adjusted_density_multipliers(const Evaluation &ev, const std::vector<double> &e0) {
+  std::vector<double> s = normalized_density_subgradient(ev.fields, e0);
+  double wf = l1(wirelength_gradient(ev.wa));
+  double df = 0.0;
+  for (size_t i = 0; i < ev.fields.size(); ++i) df += ev.fields[i].force_l1 * s[i];
+  if (!std::isfinite(wf) || wf <= 0.0 || !std::isfinite(df) || df <= 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  double scale = AREA_RESET_DENSITY_WEIGHT * wf / df;
+  bool any = false;
+  for (double &v : s) {
+    v *= scale;
+    if (!(std::isfinite(v) && v >= 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    any |= v > 0.0;
+  }
+  if (!any) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return s;
+}
+
+double area_adjusted_multiplier_step(const std::vector<double> &m) {
+  double n = 0.0;
+  for (double v : m) n += v * v;
+  double step = (MULTIPLIER_ALPHA_HIGH - 1.0) * std::sqrt(n);
+  if (!(std::isfinite(step) && step > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return step;
+}
+
+void update_density_multipliers(std::vector<double> &m, double step, const std::vector<FieldResult> &fields,
+                                const std::vector<double> &e0) {
+  std::vector<double> s = normalized_density_subgradient(fields, e0);
+  double n = 0.0;
+  for (double v : s) n += v * v;
+  n = std::sqrt(n);
+  if (n == 0.0) return;
+  if (!std::isfinite(n) || !std::isfinite(step) || step <= 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  for (size_t i = 0; i < m.size(); ++i) {
+    m[i] += step * s[i] / n;
+    if (!std::isfinite(m[i]) || m[i] < 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  }
+}
+
+double next_multiplier_step(double prev, const std::vector<FieldResult> &fields, const std::vector<double> &e0) {
+  validate_fields(fields, e0);
+  double n = 0.0;
+  for (size_t i = 0; i < fields.size(); ++i) {
+    double e = fields[i].energy / e0[i];
+    n += e * e;
+  }
+  n = std::sqrt(n);
+  double lg = std::log(AUGMENTED_DENSITY_BETA * n + 1.0);
+  double next = prev * multiplier_growth(lg);
+  if (!(std::isfinite(next) && next > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return next;
+}
+
+std::pair<double, double> overflow_excess_score(const std::vector<FieldResult> &fields) {
+  double mx = 0.0, total = 0.0;
+  for (auto &f : fields) {
+    double ex = std::max(f.overflow - density_overflow_target(f.kind), 0.0);
+    mx = std::max(mx, ex);
+    total += ex;
+  }
+  return {mx, total};
+}
+
+double adaptive_wirelength_gamma(const Problem &P, const std::vector<FieldResult> &fields, double base) {
+  std::vector<std::pair<double, double>> gw;
+  for (auto &f : fields) {
+    double g = field_wirelength_gamma(f.overflow, density_overflow_target(f.kind), base);
+    double w = P.gamma_weights.count(f.kind) ? P.gamma_weights.at(f.kind) : 0.0;
+    gw.push_back({g, w});
+  }
+  return weighted_gamma_mean(gw, base);
+}
+
+bool density_is_converged(const std::vector<FieldResult> &fields) {
+  for (auto &f : fields)
+    if (!(f.overflow <= density_overflow_target(f.kind))) return false;
+  return true;
+}
+bool area_adjustment_is_ready(const std::vector<FieldResult> &fields) {
+  for (auto &f : fields)
+    if ((is_lut(f.kind) || f.kind == TEXO_KIND_REGISTER) && !(f.overflow <= AREA_ADJUSTMENT_OVERFLOW_TARGET))
+      return false;
+  return true;
+}
+enum class Stop { AreaReady, Final };
+bool density_stop_reached(const std::vector<FieldResult> &fields, Stop s) {
+  return s == Stop::AreaReady ? area_adjustment_is_ready(fields) : density_is_converged(fields);
+}
+
+std::vector<uint32_t> rounded_targets(const std::vector<double> &coords, size_t n_units, uint32_t W, uint32_t H) {
+  std::vector<uint32_t> t(2 * n_units);
+  for (size_t u = 0; u < n_units; ++u) {
+    t[2 * u] = rounded_coordinate(coords[2 * u], W);
+    t[2 * u + 1] = rounded_coordinate(coords[2 * u + 1], H);
+  }
+  return t;
+}
+
+struct Checkpoint {
+  std::vector<uint32_t> targets;
+  double max_excess, total_excess;
+  void consider(std::vector<uint32_t> t, const std::vector<FieldResult> &fields) {
+    auto s = overflow_excess_score(fields);
+    bool better = s.first < max_excess || (s.first == max_excess && s.second < total_excess);
+    if (better) {
+      targets = std::move(t);
+      max_excess = s.first;
+      total_excess = s.second;
+    }
+  }
+};
+
+struct RunStats {
+  uint64_t iterations = 0, evaluations = 0, trials = 0, area_rounds = 0;
+};
+
+struct OptResult {
+  std::vector<double> coords;
+  std::vector<uint32_t> best_targets;
+  std::vector<double> scales;
+  bool exhausted = false;
+  std::vector<FieldResult> last_fields;
+  double last_wa = 0.0;
+};
+
+[[noreturn]] void did_not_converge(uint64_t iterations, const std::vector<FieldResult> &fields) {
+  Failure f;
+  f.status = TEXO_GP_ERR_DID_NOT_CONVERGE;
+  f.index = (int64_t)iterations;
+  if (!fields.empty()) f.v0 = fields[0].overflow;
+  if (fields.size() > 1) f.v1 = fields[1].overflow;
+  throw f;
+}
+
+struct Options {
+  uint32_t fixed_iterations = 0;
+  uint64_t max_iterations = MAX_EPLACE_ITERATIONS;
+  double gamma_base = WIRELENGTH_GAMMA_BASE_TILES;
+  bool trace = false;
+  uint32_t inject = 0;
+};
+
+// optimize_density_once (eplace.rs:439-582).  With fixed_iterations > 0 the
+// stop rules are disabled and exactly that many iterations run.
+OptResult optimize_density_once(const Problem &P, const std::vector<double> &coords, const Bounds &bounds,
+                                const std::vector<double> *fixed_scales, bool reset_after_area, Stop stop,
+                                RunStats &stats, const Options &opt) {
+  size_t C = coords.size();
+  auto eval_counted = [&](const std::vector<double> &x, double gamma) {
+    stats.evaluations += 1;
+    if (opt.inject && stats.evaluations == opt.inject) fail(TEXO_GP_ERR_INJECTED);
+    return evaluate(P, x, gamma);
+  };
+  Evaluation probe = eval_counted(coords, opt.gamma_base);
+  validate_initial_evaluation(probe);
+  double g0 = adaptive_wirelength_gamma(P, probe.fields, opt.gamma_base);
+  Evaluation initial = eval_counted(coords, g0);
+  validate_initial_evaluation(initial);
+  std::vector<double> scales;
+  if (fixed_scales) {
+    scales = *fixed_scales;
+  } else {
+    for (auto &f : initial.fields) scales.push_back(std::max(f.energy, 1.0));
+  }
+  validate_fields(initial.fields, scales);
+  NState st;
+  st.x = project(coords, bounds);
+  st.xp = st.x;
+  Checkpoint best;
+  best.targets = rounded_targets(st.x, P.n_units, P.W, P.H);
+  auto s0 = overflow_excess_score(initial.fields);
+  best.max_excess = s0.first;
+  best.total_excess = s0.second;
+  OptResult out;
+  bool fixed_mode = opt.fixed_iterations > 0;
+  if (!fixed_mode && density_stop_reached(initial.fields, stop)) {
+    out.coords = st.x;
+    out.best_targets = best.targets;
+    out.scales = scales;
+    out.last_fields = initial.fields;
+    out.last_wa = initial.wa.value;
+    return out;
+  }
+  std::vector<double> mult;
+  double mstep;
+  if (reset_after_area) {
+    mult = area_adjusted_density_multipliers(initial, scales);
+    mstep = area_adjusted_multiplier_step(mult);
+  } else {
+    double m0 = initial_density_multiplier(P, initial, C);
+    mult.assign(scales.size(), m0);
+    mstep = MULTIPLIER_ALPHA_HIGH - 1.0;
+  }
+  Evaluation current = std::move(initial);
+  std::vector<uint32_t> prev_stationary;
+  bool has_prev_stationary = false;
+  uint64_t local_iterations = 0;
+  for (;;) {
+    if (fixed_mode) {
+      if (local_iterations >= opt.fixed_iterations) break;
+    } else {
+      if (density_stop_reached(current.fields, stop)) break;
+      if (stats.iterations >= opt.max_iterations) {
+        if (!P.best_effort) did_not_converge(stats.iterations, current.fields);
+        out.exhausted = true;
+        break;
+      }
+    }
+    double gamma = adaptive_wirelength_gamma(P, current.fields, opt.gamma_base);
+    stats.iterations += 1;
+    local_iterations += 1;
+    std::vector<double> precond = diagonal_preconditioner(P, current, mult, C);
+    std::vector<Evaluation> payloads;
+    NStep step = dynamic_nesterov_step(st, bounds, precond, [&](const std::vector<double> &x, size_t &slot) {
+      Evaluation ev = eval_counted(x, gamma);
+      Eval e = combined_objective(P, ev, scales, mult, x.size());
+      payloads.push_back(std::move(ev));
+      slot = payloads.size() - 1;
+      return e;
+    });
+    stats.trials += step.trials;
+    current = std::move(payloads[step.payload]);
+    validate_fields(current.fields, scales);
+    if (opt.trace) {
+      std::fprintf(stderr,
+                   "TEXO_PNR_METRICS eplace iteration=%llu objective=%.9e stationarity=%.9e coordinate_change=%.9e "
+                   "line_search_trials=%llu gamma=%.9e wa=%.9e\n",
+                   (unsigned long long)stats.iterations, step.objective, step.stationarity, step.coordinate_change,
+                   (unsigned long long)step.trials, gamma, current.wa.value);
+    }
+    std::vector<uint32_t> rt = rounded_targets(st.x, P.n_units, P.W, P.H);
+    best.consider(rt, current.fields);
+    if (!fixed_mode) {
+      if (density_stop_reached(current.fields, stop)) break;
+      // stationary_target_is_fixed (eplace.rs:973-985)
+      bool fixed_target = false;
+      if (step.accepted) {
+        has_prev_stationary = false;
+      } else {
+        fixed_target = has_prev_stationary && prev_stationary == rt;
+        prev_stationary = rt;
+        has_prev_stationary = true;
+      }
+      if (fixed_target) {
+        if (!P.best_effort) did_not_converge(stats.iterations, current.fields);
+        out.exhausted = true;
+        break;
+      }
+    }
+    update_density_multipliers(mult, mstep, current.fields, scales);
+    mstep = next_multiplier_step(mstep, current.fields, scales);
+  }
+  out.coords = st.x;
+  out.best_targets = best.targets;
+  out.scales = scales;
+  out.last_fields = current.fields;
+  out.last_wa = current.wa.value;
+  return out;
+}
+
+// area_adjustment_round_limit (eplace.rs:584-625)
+size_t area_round_limit(const Problem &P) {
+  double initial = 0.0;
+  std::set<uint32_t> kinds;
+  for (auto &du : P.density_units)
+    for (auto &m : du.members) {
+      initial += m.charge;
+      kinds.insert(m.kind);
+    }
+  double available = 0.0;
+  for (uint32_t k : kinds) available += (double)(uint32_t)P.model.capacity_and_total(k).second;
+  if (!(std::isfinite(initial) && initial > 0.0 && available >= initial)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  double t = initial;
+  size_t rounds = 0;
+  while (t < available) {
+    t *= 1.01;
+    if (!std::isfinite(t)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    rounds += 1;
+  }
+  return rounds + 1;
+}
+
+// routability_adjusted_member_area (eplace.rs:741-758)
+double routability_adjusted_member_area(uint32_t kind, double area, double h, double v) {
+  double inflated = routability_optimized_area(area, h, v);
+  if (is_lut(kind) || kind == TEXO_KIND_REGISTER) return inflated;
+  return area;
+}
+
+// ContinuousRoutingDemand::utilization (eplace.rs:822-854)
+double utilization(const Problem &P, const std::vector<double> &demand, const std::vector<uint32_t> &cap,
+                   uint32_t x, uint32_t y) {
+  if (x >= P.W || y >= P.H) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  size_t index = (size_t)y * P.W + x;
+  double d = demand[index];
+  uint32_t c = cap[index];
+  if (c == 0) return d == 0.0 ? 0.0 : kSqrt2;
+  double u = d / (double)c;
+  if (!(std::isfinite(u) && u >= 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return u;
+}
+
+struct AreaMetrics {
+  double total_relative_increase = 0.0;
+  AreaAdjustment adjustment;
+};
+
+// propose_area_adjustment (eplace.rs:627-729)
+AreaMetrics propose_area_adjustment(const Problem &P, const std::vector<double> &coords) {
+  if (P.W == 0 || P.H == 0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  std::vector<double> pos(coords.begin(), coords.begin() + 2 * P.n_units);
+  auto boxes = net_bounding_boxes(P.hg, pos, P.geo());
+  std::vector<double> hd, vd;
+  continuous_routing_demand(P.W, P.H, boxes, hd, vd);
+  // propose_register_clustering_areas (eplace.rs:760-791)
+  std::map<uint32_t, double> clustering;
+  if (!P.controls.empty()) {
+    std::vector<std::pair<uint32_t, std::pair<double, double>>> regs;
+    for (size_t i = 0; i < P.movable_units.size(); ++i) {
+      size_t u = P.movable_units[i];
+      for (auto &m : P.density_units[i].members)
+        if (m.kind == TEXO_KIND_REGISTER) regs.push_back({m.cell, {pos[2 * u] + m.ox, pos[2 * u + 1] + m.oy}});
+    }
+    std::vector<double> areas;
+    if (!clustering_areas(regs, P.controls, P.movable_instance_count, areas)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    for (size_t i = 0; i < regs.size(); ++i) clustering[regs[i].first] = areas[i];
+  }
+  std::vector<oracle_area_member> members;
+  for (size_t i = 0; i < P.movable_units.size(); ++i) {
+    size_t u = P.movable_units[i];
+    for (auto &m : P.density_units[i].members) {
+      uint32_t px = rounded_coordinate(pos[2 * u] + m.ox, P.W);
+      uint32_t py = rounded_coordinate(pos[2 * u + 1] + m.oy, P.H);
+      double h = utilization(P, hd, P.routing_h, px, py);
+      double v = utilization(P, vd, P.routing_v, px, py);
+      double ra = routability_adjusted_member_area(m.kind, m.charge, std::min(h, kSqrt2), std::min(v, kSqrt2));
+      oracle_area_member am;
+      am.unit = (uint32_t)u;
+      am.kind = m.kind;
+      am.current_area = m.charge;
+      am.routability_area = ra;
+      am.pin_area = 0.0;
+      auto it = clustering.find(m.cell);
+      am.clustering_area = it == clustering.end() ? 0.0 : it->second;
+      members.push_back(am);
+    }
+  }
+  std::vector<std::pair<uint32_t, double>> fillers;
+  for (auto &f : P.fillers) fillers.push_back({f.kind, f.charge});
+  AreaMetrics metrics;
+  metrics.adjustment = adjust_instance_areas(members, fillers);
+  std::map<uint32_t, double> before, increase;
+  for (size_t i = 0; i < members.size(); ++i) {
+    before[members[i].kind] += members[i].current_area;
+    increase[members[i].kind] += metrics.adjustment.member_areas[i] - members[i].current_area;
+  }
+  double bt = 0.0, it = 0.0;
+  for (auto &kv : before) bt += kv.second;
+  for (auto &kv : increase) it += kv.second;
+  metrics.total_relative_increase = it / bt;
+  if (!std::isfinite(metrics.total_relative_increase)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return metrics;
+}
+
+void apply_area_adjustment(Problem &P, const AreaAdjustment &adj) {
+  size_t k = 0;
+  for (auto &du : P.density_units)
+    for (auto &m : du.members) m.charge = adj.member_areas[k++];
+  for (size_t i = 0; i < P.fillers.size(); ++i) P.fillers[i].charge = adj.filler_areas[i];
+}
+
+Bounds make_bounds(const Problem &P) {
+  Bounds b;
+  for (size_t u = 0; u < P.n_units; ++u) {
+    b.lower.push_back(P.bounds4[4 * u]);
+    b.lower.push_back(P.bounds4[4 * u + 1]);
+    b.upper.push_back(P.bounds4[4 * u + 2]);
+    b.upper.push_back(P.bounds4[4 * u + 3]);
+  }
+  for (size_t i = 0; i < P.fillers.size(); ++i) {
+    b.lower.push_back(0.0);
+    b.lower.push_back(0.0);
+    b.upper.push_back((double)(P.W - 1));
+    b.upper.push_back((double)(P.H - 1));
+  }
+  return b;
+}
+
+std::vector<double> initial_coordinates(const Problem &P, const Bounds &b) {
+  std::vector<double> c;
+  for (size_t u = 0; u < P.n_units; ++u) {
+    c.push_back(P.initial_xy[2 * u]);
+    c.push_back(P.initial_xy[2 * u + 1]);
+  }
+  for (auto &f : P.fillers) {
+    c.push_back(f.x);
+    c.push_back(f.y);
+  }
+  for (size_t i = 0; i < c.size(); ++i) c[i] = clampd(c[i], b.lower[i], b.upper[i]);
+  return c;
+}
+
+Options make_options(const texo_gp_options *o) {
+  Options opt;
+  if (!o) return opt;
+  opt.fixed_iterations = o->fixed_iterations;
+  if (o->max_iterations) opt.max_iterations = o->max_iterations;
+  if (o->gamma_base > 0.0) opt.gamma_base = o->gamma_base;
+  opt.trace = o->trace != 0;
+  opt.inject = o->inject_failure_at_evaluation;
+  return opt;
+}
+
+void fill_failure(texo_gp_report *r, const Failure &f) {
+  if (!r) return;
+  r->status = f.status;
+  r->err_index = f.index;
+  r->err_index2 = f.index2;
+  r->err_kind = f.kind;
+  r->err_x = f.x;
+  r->err_y = f.y;
+  r->err_value[0] = f.v0;
+  r->err_value[1] = f.v1;
+}
+
+}  // namespace
+
+struct oracle_session {
+  Problem P;
+  Options opt;
+  Bounds bounds;
+  std::vector<double> coords;
+  bool empty = false;  // no density units
+  RunStats stats;
+  // state kept across oracle_session_iterate calls
+  bool started = false;
+  std::vector<double> scales, mult;
+  double mstep = 0.0;
+  NState st;
+  Evaluation current;
+};
+
+namespace {
+
+void fill_report_tail(oracle_session *s, texo_gp_report *r, const std::vector<FieldResult> *fields,
+                      const std::vector<double> &coords) {
+  if (!r) return;
+  r->iterations = s->stats.iterations;
+  r->evaluations = s->stats.evaluations;
+  r->line_search_trials = s->stats.trials;
+  r->area_rounds = s->stats.area_rounds;
+  r->fixed_iterations = s->opt.fixed_iterations;
+  r->gamma_base = s->opt.gamma_base;
+  std::vector<double> pos(coords.begin(), coords.begin() + 2 * s->P.n_units);
+  r->final_hpwl = continuous_hpwl(s->P.hg, pos, s->P.geo());
+  if (fields) {
+    r->n_fields = (uint32_t)std::min<size_t>(fields->size(), TEXO_GP_MAX_FIELDS);
+    for (size_t f = 0; f < r->n_fields; ++f) {
+      r->field_kind[f] = (*fields)[f].kind;
+      r->field_overflow[f] = (*fields)[f].overflow;
+      r->field_energy[f] = (*fields)[f].energy;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
+
+oracle_session *oracle_session_create(const texo_gp_problem *problem, const texo_gp_options *options,
+                                      texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  try {
+    std::unique_ptr<oracle_session> s(new oracle_session());
+    s->P = load_problem(problem);
+    s->opt = make_options(options);
+    if (s->P.density_units.empty()) {
+      s->empty = true;
+      s->coords.assign(s->P.initial_xy.begin(), s->P.initial_xy.end());
+      s->bounds = make_bounds(s->P);
+      return s.release();
+    }
+    finish_setup(s->P, problem);
+    s->bounds = make_bounds(s->P);
+    s->coords = initial_coordinates(s->P, s->bounds);
+    return s.release();
+  } catch (const Failure &f) {
+    fill_failure(report, f);
+    return nullptr;
+  }
+}
+
+void oracle_session_destroy(oracle_session *s) { delete s; }
+
+int oracle_session_sizes(const oracle_session *s, uint64_t *coordinates, uint32_t *fields, uint64_t *fillers,
+                         uint64_t *density_units) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  if (coordinates) *coordinates = s->coords.size();
+  if (fields) {
+    std::set<uint32_t> kinds;
+    for (auto &du : s->P.density_units)
+      for (auto &m : du.members) kinds.insert(m.kind);
+    *fields = (uint32_t)kinds.size();
+  }
+  if (fillers) *fillers = s->P.fillers.size();
+  if (density_units) *density_units = s->P.density_units.size();
+  return 0;
+}
+
+int oracle_session_initial(const oracle_session *s, double *coordinates, double *lower, double *upper,
+                           uint32_t *field_kind, uint32_t *filler_field, double *filler_charge) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  if (coordinates) std::copy(s->coords.begin(), s->coords.end(), coordinates);
+  if (lower) std::copy(s->bounds.lower.begin(), s->bounds.lower.end(), lower);
+  if (upper) std::copy(s->bounds.upper.begin(), s->bounds.upper.end(), upper);
+  std::vector<uint32_t> kinds;
+  {
+    std::set<uint32_t> k;
+    for (auto &du : s->P.density_units)
+      for (auto &m : du.members) k.insert(m.kind);
+    kinds.assign(k.begin(), k.end());
+  }
+  if (field_kind) std::copy(kinds.begin(), kinds.end(), field_kind);
+  for (size_t i = 0; i < s->P.fillers.size(); ++i) {
+    if (filler_field)
+      filler_field[i] = (uint32_t)(std::find(kinds.begin(), kinds.end(), s->P.fillers[i].kind) - kinds.begin());
+    if (filler_charge) filler_charge[i] = s->P.fillers[i].charge;
+  }
+  return 0;
+}
+
+int oracle_session_evaluate(oracle_session *s, const double *coordinates, double gamma, double *wa_value,
+                            double *wa_gradient, texo_gp_field_eval *fields, double *unit_gradient,
+                            double *filler_gradient, double *density, double *potential) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    std::vector<double> x(coordinates, coordinates + s->coords.size());
+    Evaluation ev;
+    if (s->empty) {
+      std::vector<double> pos(x.begin(), x.begin() + 2 * s->P.n_units);
+      ev.wa = weighted_average(s->P.hg, gamma, pos, s->P.n_units, s->P.geo());
+    } else {
+      ev = evaluate(s->P, x, gamma);
+    }
+    if (wa_value) *wa_value = ev.wa.value;
+    if (wa_gradient)
+      for (size_t u = 0; u < s->P.n_units; ++u) {
+        wa_gradient[2 * u] = ev.wa.gx[u];
+        wa_gradient[2 * u + 1] = ev.wa.gy[u];
+      }
+    size_t G = (size_t)s->P.W * s->P.H, DU = s->P.density_units.size();
+    for (size_t f = 0; f < ev.fields.size(); ++f) {
+      const FieldResult &r = ev.fields[f];
+      if (fields) {
+        fields[f].energy = r.energy;
+        fields[f].normalized_overflow = r.overflow;
+        fields[f].net_charge = r.net_charge;
+        fields[f].force_l1 = r.force_l1;
+        fields[f].real_area = r.real_area;
+        fields[f].filler_charge = r.filler_charge;
+      }
+      if (unit_gradient)
+        for (size_t i = 0; i < DU; ++i) {
+          unit_gradient[(f * DU + i) * 2] = r.unit_gradients[i].first;
+          unit_gradient[(f * DU + i) * 2 + 1] = r.unit_gradients[i].second;
+        }
+      if (filler_gradient)
+        for (auto &fg : r.filler_gradients) {
+          filler_gradient[2 * fg.index] = fg.gx;
+          filler_gradient[2 * fg.index + 1] = fg.gy;
+        }
+      if (density) std::copy(r.density.begin(), r.density.end(), density + f * G);
+      if (potential) std::copy(r.potential.begin(), r.potential.end(), potential + f * G);
+    }
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int oracle_session_combined(oracle_session *s, const double *coordinates, double gamma, const double *multipliers,
+                            const double *scales, double *value, double *gradient) {
+  if (!s || s->empty) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    std::vector<double> x(coordinates, coordinates + s->coords.size());
+    Evaluation ev = evaluate(s->P, x, gamma);
+    std::vector<double> m(multipliers, multipliers + ev.fields.size());
+    std::vector<double> sc(scales, scales + ev.fields.size());
+    Eval e = combined_objective(s->P, ev, sc, m, x.size());
+    *value = e.value;
+    std::copy(e.gradient.begin(), e.gradient.end(), gradient);
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int oracle_session_preconditioner(oracle_session *s, const double *multipliers, double *diagonal) {
+  if (!s || s->empty) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    Evaluation ev;
+    std::set<uint32_t> kinds;
+    for (auto &du : s->P.density_units)
+      for (auto &m : du.members) kinds.insert(m.kind);
+    for (uint32_t k : kinds) {
+      FieldResult r;
+      r.kind = k;
+      ev.fields.push_back(r);
+    }
+    std::vector<double> m(multipliers, multipliers + ev.fields.size());
+    std::vector<double> d = diagonal_preconditioner(s->P, ev, m, s->coords.size());
+    std::copy(d.begin(), d.end(), diagonal);
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int oracle_session_run(oracle_session *s, uint32_t *out_targets_xy, texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    Problem &P = s->P;
+    if (s->empty) {
+      std::vector<uint32_t> t = rounded_targets(P.initial_xy, P.n_units, P.W, P.H);
+      std::copy(t.begin(), t.end(), out_targets_xy);
+      fill_report_tail(s, report, nullptr, s->coords);
+      return 0;
+    }
+    // place() area-round loop (eplace.rs:353-420)
+    bool adjust_area = P.has_routing && s->opt.fixed_iterations == 0;
+    size_t limit = area_round_limit(P);
+    bool has_prev_change = false;
+    double prev_change = 0.0;
+    std::vector<double> fixed_scales;
+    bool has_scales = false;
+    bool reset = false;
+    size_t area_round = 0;
+    for (;;) {
+      Stop stop = (adjust_area && !(has_prev_change && prev_change < 0.01)) ? Stop::AreaReady : Stop::Final;
+      OptResult r = optimize_density_once(P, s->coords, s->bounds, has_scales ? &fixed_scales : nullptr, reset, stop,
+                                          s->stats, s->opt);
+      if (!has_scales) {
+        fixed_scales = r.scales;
+        has_scales = true;
+      }
+      s->coords = r.coords;
+      if (stop == Stop::Final || r.exhausted) {
+        std::copy(r.best_targets.begin(), r.best_targets.end(), out_targets_xy);
+        fill_report_tail(s, report, &r.last_fields, s->coords);
+        if (report) report->final_wirelength = r.last_wa;
+        return 0;
+      }
+      AreaMetrics m = propose_area_adjustment(P, s->coords);
+      if (!m.adjustment.requires_reset) {
+        has_prev_change = true;
+        prev_change = 0.0;
+        reset = true;
+        continue;
+      }
+      if (area_round >= limit) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+      apply_area_adjustment(P, m.adjustment);
+      has_prev_change = true;
+      prev_change = m.total_relative_increase;
+      reset = true;
+      area_round += 1;
+      s->stats.area_rounds += 1;
+    }
+  } catch (const Failure &f) {
+    fill_failure(report, f);
+    if (report) fill_report_tail(s, report, nullptr, s->coords);
+    if (report) report->status = f.status;
+    return f.status;
+  }
+}
+
+int oracle_session_iterate(oracle_session *s, uint32_t iterations, texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  if (!s || s->empty) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    Problem &P = s->P;
+    size_t C = s->coords.size();
+    if (!s->started) {
+      // Same set-up as optimize_density_once (eplace.rs:455-499).
+      s->stats.evaluations += 2;
+      Evaluation probe = evaluate(P, s->coords, s->opt.gamma_base);
+      validate_initial_evaluation(probe);
+      double g0 = adaptive_wirelength_gamma(P, probe.fields, s->opt.gamma_base);
+      s->current = evaluate(P, s->coords, g0);
+      validate_initial_evaluation(s->current);
+      s->scales.clear();
+      for (auto &f : s->current.fields) s->scales.push_back(std::max(f.energy, 1.0));
+      double m0 = initial_density_multiplier(P, s->current, C);
+      s->mult.assign(s->scales.size(), m0);
+      s->mstep = MULTIPLIER_ALPHA_HIGH - 1.0;
+      s->st.x = project(s->coords, s->bounds);
+      s->st.xp = s->st.x;
+      s->started = true;
+    }
+    for (uint32_t it = 0; it < iterations; ++it) {
+      double gamma = adaptive_wirelength_gamma(P, s->current.fields, s->opt.gamma_base);
+      s->stats.iterations += 1;
+      std::vector<double> precond = diagonal_preconditioner(P, s->current, s->mult, C);
+      std::vector<Evaluation> payloads;
+      NStep step = dynamic_nesterov_step(s->st, s->bounds, precond, [&](const std::vector<double> &x, size_t &slot) {
+        s->stats.evaluations += 1;
+        Evaluation ev = evaluate(P, x, gamma);
+        Eval e = combined_objective(P, ev, s->scales, s->mult, x.size());
+        payloads.push_back(std::move(ev));
+        slot = payloads.size() - 1;
+        return e;
+      });
+      s->stats.trials += step.trials;
+      s->current = std::move(payloads[step.payload]);
+      validate_fields(s->current.fields, s->scales);
+      update_density_multipliers(s->mult, s->mstep, s->current.fields, s->scales);
+      s->mstep = next_multiplier_step(s->mstep, s->current.fields, s->scales);
+    }
+    s->coords = s->st.x;
+    fill_report_tail(s, report, &s->current.fields, s->coords);
+    if (report) report->final_wirelength = s->current.wa.value;
+    return 0;
+  } catch (const Failure &f) {
+    fill_failure(report, f);
+    return f.status;
+  }
+}
+
+int oracle_session_coordinates(const oracle_session *s, double *coordinates) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::copy(s->coords.begin(), s->coords.end(), coordinates);
+  return 0;
+}
+
+int oracle_session_rudy(oracle_session *s, const double *coordinates, double *horizontal, double *vertical) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    std::vector<double> pos(coordinates, coordinates + 2 * s->P.n_units);
+    auto boxes = net_bounding_boxes(s->P.hg, pos, s->P.geo());
+    std::vector<double> hd, vd;
+    continuous_routing_demand(s->P.W, s->P.H, boxes, hd, vd);
+    std::copy(hd.begin(), hd.end(), horizontal);
+    std::copy(vd.begin(), vd.end(), vertical);
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+double oracle_session_hpwl(oracle_session *s, const double *coordinates) {
+  std::vector<double> pos(coordinates, coordinates + 2 * s->P.n_units);
+  return continuous_hpwl(s->P.hg, pos, s->P.geo());
+}
+
+int oracle_place(const texo_gp_problem *problem, const texo_gp_options *options, uint32_t *out_targets_xy,
+                 texo_gp_report *report) {
+  oracle_session *s = oracle_session_create(problem, options, report);
+  if (!s) return report ? report->status : TEXO_GP_ERR_INVALID_ARGUMENT;
+  int rc = oracle_session_run(s, out_targets_xy, report);
+  oracle_session_destroy(s);
+  return rc;
+}
+
+int oracle_poisson_solve(uint32_t width, uint32_t height, const double *density, double *potential, double *energy) {
+  try {
+    Poisson2d solver(width, height);
+    std::vector<double> d(density, density + (size_t)width * height), phi;
+    *energy = solver.solve(d, phi);
+    std::copy(phi.begin(), phi.end(), potential);
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int oracle_poisson_solve_matrix(uint32_t width, uint32_t height, const double *density, double *potential,
+                                double *energy) {
+  try {
+    Poisson2d solver(width, height);
+    std::vector<double> d(density, density + (size_t)width * height), phi;
+    *energy = solver.solve_matrix(d, phi);
+    std::copy(phi.begin(), phi.end(), potential);
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+void oracle_bspline_axis(uint32_t length, double coordinate, uint32_t *index, double *weight, double *derivative) {
+  AxisEntry a[3];
+  bspline_axis(length, coordinate, a);
+  for (int i = 0; i < 3; ++i) {
+    index[i] = (uint32_t)a[i].index;
+    weight[i] = a[i].weight;
+    derivative[i] = a[i].derivative;
+  }
+}
+
+void oracle_deposit_bspline(double *density, uint32_t width, uint32_t height, double x, double y, double charge) {
+  deposit_bspline(density, width, height, x, y, charge);
+}
+
+void oracle_sample_bspline(const double *potential, uint32_t width, uint32_t height, double x, double y,
+                           double *out) {
+  sample_bspline(potential, width, height, x, y, out[0], out[1], out[2]);
+}
+
+int oracle_weighted_average(uint32_t n_units, uint32_t n_cells, const double *positions, const uint8_t *fixed_mask,
+                            const double *fixed_xy, const double *offsets, uint32_t n_nets,
+                            const uint32_t *net_pin_start, const uint32_t *pin_unit, const uint32_t *pin_cell,
+                            uint32_t n_arcs, const uint32_t *arc_units, const uint32_t *arc_cells,
+                            const uint64_t *arc_extra, double gamma, double *value, double *gradient_x,
+                            double *gradient_y) {
+  Hypergraph hg;
+  hg.net_start.assign(net_pin_start, net_pin_start + n_nets + 1);
+  for (uint32_t i = 0; i < (n_nets ? net_pin_start[n_nets] : 0); ++i) hg.pins.push_back({pin_unit[i], pin_cell[i]});
+  for (uint32_t a = 0; a < n_arcs; ++a)
+    hg.arcs.push_back({{arc_units[2 * a], arc_cells[2 * a]}, {arc_units[2 * a + 1], arc_cells[2 * a + 1]}, arc_extra[a]});
+  std::vector<uint8_t> fm(fixed_mask, fixed_mask + n_cells);
+  std::vector<double> fx(fixed_xy, fixed_xy + 2 * n_cells), of(offsets, offsets + 2 * n_cells);
+  CellGeometry geo{&fm, &fx, &of};
+  std::vector<double> pos(positions, positions + 2 * n_units);
+  WaResult r = weighted_average(hg, gamma, pos, n_units, geo);
+  *value = r.value;
+  std::copy(r.gx.begin(), r.gx.end(), gradient_x);
+  std::copy(r.gy.begin(), r.gy.end(), gradient_y);
+  return 0;
+}
+
+int oracle_wirelength_preconditioner(uint32_t n_units, uint32_t n_nets, const uint32_t *net_pin_start,
+                                     const uint32_t *pin_unit, uint32_t n_arcs, const uint32_t *arc_units,
+                                     const uint64_t *arc_extra, double *diagonal, double *baseline) {
+  Hypergraph hg;
+  hg.net_start.assign(net_pin_start, net_pin_start + n_nets + 1);
+  for (uint32_t i = 0; i < (n_nets ? net_pin_start[n_nets] : 0); ++i) hg.pins.push_back({pin_unit[i], 0});
+  for (uint32_t a = 0; a < n_arcs; ++a)
+    hg.arcs.push_back({{arc_units[2 * a], 0}, {arc_units[2 * a + 1], 0}, arc_extra[a]});
+  auto d = wirelength_preconditioner(hg, n_units);
+  auto b = baseline_wirelength_preconditioner(hg, n_units);
+  std::copy(d.begin(), d.end(), diagonal);
+  if (baseline) std::copy(b.begin(), b.end(), baseline);
+  return 0;
+}
+
+int oracle_nesterov_step(oracle_nesterov_state *state, uint64_t n, const double *lower, const double *upper,
+                         const double *metric, oracle_evaluate_fn evaluate_fn, void *context,
+                         oracle_nesterov_result *result) {
+  std::memset(result, 0, sizeof(*result));
+  NState st;
+  st.x.assign(state->coordinates, state->coordinates + n);
+  st.xp.assign(state->previous_coordinates, state->previous_coordinates + n);
+  st.momentum = state->momentum;
+  st.has_l = state->has_lipschitz != 0;
+  st.l = state->lipschitz;
+  Bounds b;
+  b.lower.assign(lower, lower + n);
+  b.upper.assign(upper, upper + n);
+  std::vector<double> m(metric, metric + n);
+  uint64_t evaluations = 0;
+  try {
+    NStep s = dynamic_nesterov_step(st, b, m, [&](const std::vector<double> &x, size_t &slot) {
+      Eval e;
+      e.gradient.assign(x.size(), 0.0);
+      evaluations += 1;
+      int rc = evaluate_fn(context, x.data(), x.size(), &e.value, e.gradient.data());
+      if (rc != 0) fail(TEXO_GP_ERR_EVALUATION, (int64_t)rc);
+      slot = evaluations;
+      return e;
+    });
+    std::copy(st.x.begin(), st.x.end(), state->coordinates);
+    std::copy(st.xp.begin(), st.xp.end(), state->previous_coordinates);
+    state->momentum = st.momentum;
+    state->has_lipschitz = st.has_l ? 1 : 0;
+    state->lipschitz = st.l;
+    result->status = 0;
+    result->accepted = s.accepted ? 1 : 0;
+    result->objective = s.objective;
+    result->stationarity = s.stationarity;
+    result->coordinate_change = s.coordinate_change;
+    result->line_search_trials = s.trials;
+    result->evaluations = evaluations;
+    result->payload_evaluation = (int64_t)s.payload;
+    return 0;
+  } catch (const Failure &f) {
+    result->status = f.status;
+    result->error_index = f.index;
+    result->evaluations = evaluations;
+    return f.status;
+  }
+}
+
+double oracle_box_metric_kkt_residual(uint64_t n, const double *gradient, const double *coordinates,
+                                      const double *lower, const double *upper, const double *metric) {
+  Bounds b;
+  b.lower.assign(lower, lower + n);
+  b.upper.assign(upper, upper + n);
+  return box_metric_kkt(std::vector<double>(gradient, gradient + n), std::vector<double>(coordinates, coordinates + n),
+                        b, std::vector<double>(metric, metric + n));
+}
+
+double oracle_field_wirelength_gamma(double overflow, double target, double base) {
+  try {
+    return field_wirelength_gamma(overflow, target, base);
+  } catch (const Failure &) {
+    return NAN;
+  }
+}
+double oracle_weighted_gamma_mean(uint32_t n, const double *gamma, const double *weight, double fallback) {
+  try {
+    std::vector<std::pair<double, double>> gw;
+    for (uint32_t i = 0; i < n; ++i) gw.push_back({gamma[i], weight[i]});
+    return weighted_gamma_mean(gw, fallback);
+  } catch (const Failure &) {
+    return NAN;
+  }
+}
+double oracle_multiplier_growth(double logarithm) {
+  try {
+    return multiplier_growth(logarithm);
+  } catch (const Failure &) {
+    return NAN;
+  }
+}
+void oracle_augmented_density(double energy, double initial_energy, double multiplier, double *value,
+                              double *coefficient) {
+  auto r = augmented_density(energy, initial_energy, multiplier);
+  *value = r.first;
+  *coefficient = r.second;
+}
+double oracle_density_overflow_target(uint32_t kind) { return density_overflow_target(kind); }
+uint32_t oracle_rounded_coordinate(double value, uint32_t extent) { return rounded_coordinate(value, extent); }
+
+int oracle_continuous_routing_demand(uint32_t width, uint32_t height, uint64_t n_boxes, const double *boxes,
+                                     double *horizontal, double *vertical) {
+  try {
+    std::vector<std::array<double, 4>> b(n_boxes);
+    for (uint64_t i = 0; i < n_boxes; ++i)
+      for (int k = 0; k < 4; ++k) b[i][k] = boxes[4 * i + k];
+    std::vector<double> hd, vd;
+    continuous_routing_demand(width, height, b, hd, vd);
+    std::copy(hd.begin(), hd.end(), horizontal);
+    std::copy(vd.begin(), vd.end(), vertical);
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+double oracle_routability_optimized_area(double area, double h, double v) {
+  try {
+    return routability_optimized_area(area, h, v);
+  } catch (const Failure &) {
+    return NAN;
+  }
+}
+
+int oracle_adjust_instance_areas(uint64_t n_members, const oracle_area_member *members, uint64_t n_fillers,
+                                 const uint32_t *filler_kind, const double *filler_area, double *member_areas,
+                                 double *filler_areas, int *requires_reset) {
+  try {
+    std::vector<oracle_area_member> m(members, members + n_members);
+    std::vector<std::pair<uint32_t, double>> f;
+    for (uint64_t i = 0; i < n_fillers; ++i) f.push_back({filler_kind[i], filler_area[i]});
+    AreaAdjustment a = adjust_instance_areas(m, f);
+    std::copy(a.member_areas.begin(), a.member_areas.end(), member_areas);
+    std::copy(a.filler_areas.begin(), a.filler_areas.end(), filler_areas);
+    *requires_reset = a.requires_reset ? 1 : 0;
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int oracle_clustering_areas(uint64_t n_registers, const uint32_t *register_cell, const double *register_xy,
+                            uint64_t n_controls, const uint32_t *control_cell, const uint64_t *control_clock_lsr,
+                            const uint64_t *control_ce, uint64_t movable_instance_count, double *areas) {
+  std::vector<std::pair<uint32_t, std::pair<double, double>>> regs;
+  for (uint64_t i = 0; i < n_registers; ++i)
+    regs.push_back({register_cell[i], {register_xy[2 * i], register_xy[2 * i + 1]}});
+  std::map<uint32_t, Control> controls;
+  for (uint64_t i = 0; i < n_controls; ++i)
+    controls[control_cell[i]] = {control_clock_lsr[2 * i], control_clock_lsr[2 * i + 1], control_ce[i]};
+  std::vector<double> out;
+  if (!clustering_areas(regs, controls, movable_instance_count, out)) return -1;
+  std::copy(out.begin(), out.end(), areas);
+  return 0;
+}
+
+double oracle_soft_division_ceiling(double value, double divisor) { return soft_division_ceiling(value, divisor); }
+
+}  // extern "C"

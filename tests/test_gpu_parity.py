@@ -1,0 +1,178 @@
+"""HIP path vs the FP64 oracle, through the C ABI (texo_gp_session_*).
+
+Tolerances: integer/index work is bit-exact (B-spline bin indices, RUDY tile
+ranges, targets); the int64 fixed-point density differs from the FP64
+sequential sum by at most 9 * charges * 2^-41 per bin; WA runs in FP64
+(relative 1e-12); the Poisson transform carries FP32 (relative field error
+<= 1e-5) or BF16x3 (<= 1e-3) error.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import texo_amd
+from texo_amd import abi, synth
+from tests import problems
+
+pytestmark = pytest.mark.gpu
+
+FX_QUANTUM = 2.0 ** -40
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def pair(problem, precision=abi.DCT_FP32):
+    g = texo_amd.Session(problem, abi.options(dct_precision=precision))
+    o = abi.Session(oracle.load(), "oracle", problem)
+    return g, o
+
+
+@pytest.mark.parametrize("name", ["plain_1k", "ecp5_25k"])
+@pytest.mark.parametrize("precision,field_tol", [(abi.DCT_FP32, 1e-5), (abi.DCT_BF16X3, 1e-3)])
+def test_evaluate_matches_oracle(name, precision, field_tol):
+    print(f"evaluate parity {name} precision={precision}", flush=True)
+    p = synth.CONFIGS[name]()
+    g, o = pair(p, precision)
+    assert g.n_coordinates == o.n_coordinates and g.n_fillers == o.n_fillers
+    assert np.array_equal(g.initial, o.initial)
+    assert np.array_equal(g.filler_charge, o.filler_charge)
+    rng = np.random.default_rng(7)
+    for x in (g.initial, np.clip(g.initial + rng.normal(0, 3.0, g.initial.size), g.lower, g.upper)):
+        eg = g.evaluate(x, 5.0, want_grids=True)
+        eo = o.evaluate(x, 5.0, want_grids=True)
+        assert abs(eg["wa_value"] - eo["wa_value"]) <= 1e-12 * eo["wa_value"]
+        assert rel(eg["wa_gradient"], eo["wa_gradient"]) <= 1e-12
+        charges = p.n_members + g.n_fillers
+        assert np.abs(eg["density"] - eo["density"]).max() <= 9 * charges * FX_QUANTUM
+        assert np.allclose(eg["overflow"], eo["overflow"], rtol=1e-10, atol=1e-12)
+        assert np.allclose(eg["net_charge"], eo["net_charge"], atol=9 * charges * FX_QUANTUM + 1e-9)
+        assert rel(eg["potential"], eo["potential"]) <= field_tol
+        assert np.all(np.abs(eg["energy"] - eo["energy"]) <= field_tol * eo["energy"])
+        assert rel(eg["unit_gradient"], eo["unit_gradient"]) <= 20 * field_tol
+        assert rel(eg["filler_gradient"], eo["filler_gradient"]) <= 20 * field_tol
+        assert np.allclose(eg["force_l1"], eo["force_l1"], rtol=20 * field_tol)
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("W,H", [(7, 5), (11, 1), (127, 96), (64, 64), (256, 256)])
+@pytest.mark.parametrize("precision,tol", [(abi.DCT_FP32, 2e-6), (abi.DCT_BF16X3, 2e-4)])
+def test_poisson_matches_oracle(W, H, precision, tol):
+    print(f"poisson parity {W}x{H} precision={precision}", flush=True)
+    # density pattern of fast_prime_grid_solver_matches_orthonormal_matrix_oracle
+    # (electrostatic_placement.rs:1239-1264), on any grid
+    i = np.arange(W * H)
+    density = ((i * 31 + 7) % 101) - 50.0 + ((i * 13 + 5) % 17) / 17.0
+    g = texo_amd.Session(problems.grid_problem(W, H), abi.options(dct_precision=precision))
+    phi, energy = g.poisson(density)
+    import ctypes as C
+    ref = np.zeros(W * H)
+    e = C.c_double()
+    oracle.load().oracle_poisson_solve(W, H, abi.ptr(density, C.c_double), abi.ptr(ref, C.c_double), C.byref(e))
+    scale = max(np.abs(ref).max(), 1.0)
+    assert np.abs(phi.reshape(-1) - ref).max() <= tol * scale * 10
+    assert abs(energy[0] - e.value) <= tol * abs(e.value)
+    g.close()
+
+
+def test_constant_density_gives_zero_potential():
+    # constant_density_is_only_the_removed_dc_mode (electrostatic_placement.rs:1114-1120)
+    g = texo_amd.Session(problems.grid_problem(7, 5))
+    phi, energy = g.poisson(np.full(35, 3.25))
+    assert np.abs(phi).max() <= 1e-5 and abs(energy[0]) <= 1e-9
+    g.close()
+
+
+def test_cosine_is_an_eigenmode():
+    # manufactured_cosine_is_one_laplacian_eigenmode (electrostatic_placement.rs:1137-1154)
+    x = np.arange(11)
+    rho = np.cos(np.pi * 3.0 * (x + 0.5) / 11.0)
+    lam = 4.0 * np.sin(np.pi * 3.0 / 22.0) ** 2
+    g = texo_amd.Session(problems.grid_problem(11, 1))
+    phi, _ = g.poisson(rho)
+    assert np.abs(phi.reshape(-1) - rho / lam).max() <= 1e-5
+    g.close()
+
+
+def test_legal_sites_cancel_capacity_exactly():
+    # legal_site_charges_exactly_cancel_the_smoothed_capacity (electrostatic_placement.rs:1664-1684)
+    W, H, cap = problems.mixed_device_capacity()
+    cap = {problems.LUT4: cap[problems.LUT4]}
+    p = problems.density_problem(W, H, cap, [(0.0, 0.0, [(problems.LUT4, float(o), 0.0) for o in range(4)])])
+    g = texo_amd.Session(p)
+    assert g.n_fillers == 0
+    e = g.evaluate(g.initial, 1.0, want_grids=True)
+    assert np.all(e["density"] == 0.0)
+    assert e["energy"][0] == 0.0 and e["overflow"][0] == 0.0
+    g.close()
+
+
+def test_fillers_match_reference_known_answers():
+    # initial_fillers_are_capacity_proportional_and_kind_separated (electrostatic_placement.rs:1686-1724)
+    W, H, cap = problems.mixed_device_capacity()
+    p = problems.density_problem(W, H, cap, [(1.0, 0.5, [(problems.LUT4, 0.0, 0.0), (problems.FF, 0.0, 0.0)])])
+    g = texo_amd.Session(p)
+    assert g.n_fillers == 8
+    assert np.all(np.abs(g.filler_charge[:8] - 0.75) <= np.finfo(float).eps)
+    e = g.evaluate(g.initial, 1.0)
+    assert np.all(np.abs(e["net_charge"]) <= 9 * 10 * FX_QUANTUM)
+    g.close()
+
+
+def test_rudy_matches_oracle_bit_exact_ranges():
+    p = synth.ecp5_85k()
+    g, o = pair(p)
+    rng = np.random.default_rng(3)
+    x = np.clip(g.initial + rng.normal(0, 20.0, g.initial.size), g.lower, g.upper)
+    hg, vg = g.rudy(x)
+    ho, vo = o.rudy(x)
+    assert np.array_equal(hg > 0, ho > 0) and np.array_equal(vg > 0, vo > 0)
+    nets = p.n_nets
+    assert np.abs(hg - ho).max() <= nets * FX_QUANTUM and np.abs(vg - vo).max() <= nets * FX_QUANTUM
+    # RUDY conservation: directional sums reproduce directional HPWL (routing_demand.rs:29-33)
+    g.close()
+    o.close()
+
+
+def test_runs_are_bit_identical():
+    p = synth.plain_1k()
+    a = texo_amd.Session(p)
+    b = texo_amd.Session(p)
+    ra = a.iterate(30)
+    rb = b.iterate(30)
+    assert np.array_equal(a.coordinates(), b.coordinates())
+    assert ra.final_hpwl == rb.final_hpwl and ra.evaluations == rb.evaluations
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [abi.DCT_FP32, abi.DCT_BF16X3])
+def test_fixed_iterations_track_the_oracle(precision):
+    p = synth.plain_1k()
+    g, o = pair(p, precision)
+    rg = g.iterate(200)
+    ro = o.iterate(200)
+    print("gpu", rg.as_dict(), "oracle", ro.as_dict(), flush=True)
+    assert abs(rg.final_hpwl - ro.final_hpwl) <= 0.005 * ro.final_hpwl
+    assert abs(rg.field_overflow[0] - ro.field_overflow[0]) <= 0.02
+    g.close()
+    o.close()
+
+
+def test_place_with_reference_stop_rules_matches_oracle():
+    p = synth.plain_1k()
+    tg, rg = texo_amd.place(p)
+    to = np.zeros(2 * p.n_units, np.uint32)
+    rep = abi.texo_gp_report()
+    import ctypes as C
+    rc = oracle.load().oracle_place(C.byref(p.struct()), C.byref(abi.options()), abi.ptr(to, C.c_uint32),
+                                    C.byref(rep))
+    assert rc == 0
+    print("gpu", rg, "oracle", rep.as_dict(), flush=True)
+    assert rg["field_overflow"][0] <= 0.10
+    assert abs(rg["final_hpwl"] - rep.final_hpwl) <= 0.005 * rep.final_hpwl
+    assert tg.shape == (p.n_units, 2)
+    assert np.all(tg[:, 0] < p.width) and np.all(tg[:, 1] < p.height)

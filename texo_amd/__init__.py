@@ -1,0 +1,94 @@
+"""MI355X-native electrostatic global placement (texo-pnr eplace hot path).
+
+`place(problem, ...)` mirrors texo-pnr's `eplace::place`
+(crates/texo-pnr/src/eplace.rs:242-253): it returns one integer tile target
+per placement unit or raises `PlacementError` carrying the EplaceError
+variant.  All computation runs in libtexo_gp.so (hand-written gfx950 HIP
+kernels behind the C ABI of include/texo_gp.h); there is no CPU fallback, and
+loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .abi import PlacementError, Problem, options  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtexo_gp.so")
+
+_lib = None
+
+
+def load():
+    """Loads libtexo_gp.so (built by texo_amd.build).  Raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m texo_amd.build` "
+                           "(the placement path has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    abi.declare_session_api(lib, "texo_gp")
+    P, d = C.POINTER, C.c_double
+    lib.texo_gp_last_error.restype = C.c_char_p
+    lib.texo_gp_last_error.argtypes = []
+    lib.texo_gp_session_poisson.argtypes = [C.c_void_p, P(d), P(d), P(d)]
+    lib.texo_gp_session_poisson.restype = C.c_int
+    lib.texo_gp_session_set_kernel_timing.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    lib.texo_gp_session_set_kernel_timing.restype = C.c_int
+    lib.texo_gp_session_kernel_timing.argtypes = [C.c_void_p, P(d), P(C.c_uint64)]
+    lib.texo_gp_session_kernel_timing.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().texo_gp_last_error().decode(errors="replace")
+
+
+class Session(abi.Session):
+    """Device-resident placement session (one HIP stream, one HBM arena)."""
+
+    def __init__(self, problem: Problem, opts: Optional[abi.texo_gp_options] = None):
+        super().__init__(load(), "texo_gp", problem, opts)
+
+    def poisson(self, density: np.ndarray):
+        F, G = self.n_fields, self.problem.width * self.problem.height
+        rho = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
+        assert rho.size == F * G
+        phi = np.zeros(F * G)
+        energy = np.zeros(F)
+        rc = self.lib.texo_gp_session_poisson(self.handle, abi.ptr(rho, C.c_double), abi.ptr(phi, C.c_double),
+                                              abi.ptr(energy, C.c_double))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return phi.reshape(F, self.problem.height, self.problem.width), energy
+
+    def set_kernel_timing(self, kernel: Optional[str]):
+        self.lib.texo_gp_session_set_kernel_timing(self.handle, (kernel or "").encode(), 1 if kernel else 0)
+
+    def kernel_timing(self):
+        avg, count = C.c_double(), C.c_uint64()
+        rc = self.lib.texo_gp_session_kernel_timing(self.handle, C.byref(avg), C.byref(count))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return avg.value, count.value
+
+
+def place(problem: Problem, dct_precision: int = abi.DCT_FP32, fixed_iterations: int = 0, trace: bool = False,
+          gamma_base: float = 0.0, device: int = 0):
+    """eplace::place: returns (targets[n_units, 2] uint32, report dict)."""
+    lib = load()
+    targets = np.zeros(2 * problem.n_units, np.uint32)
+    rep = abi.texo_gp_report()
+    opts = options(dct_precision=dct_precision, fixed_iterations=fixed_iterations, trace=trace,
+                   gamma_base=gamma_base, device=device)
+    rc = lib.texo_gp_place(C.byref(problem.struct()), C.byref(opts), abi.ptr(targets, C.c_uint32), C.byref(rep))
+    if rc != 0:
+        raise PlacementError(rep)
+    return targets.reshape(-1, 2), rep.as_dict()
