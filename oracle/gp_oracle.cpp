@@ -327,9 +327,10 @@ void bspline_axis(size_t length, double coordinate, AxisEntry out[3]) {
   double center_coordinate = std::min(std::floor(c + 0.5), maximum);
   size_t center = static_cast<size_t>(center_coordinate);
   double offset = c - center_coordinate;
-  out[0] = {center == 0 ? 0 : center - 1, 0.5 * (0.5 - offset) * (0.5 - offset), offset - 0.5};
+  // 0.5 * (0.5 - o).powi(2): the square is rounded before the halving
+  out[0] = {center == 0 ? 0 : center - 1, 0.5 * ((0.5 - offset) * (0.5 - offset)), offset - 0.5};
   out[1] = {center, 0.75 - offset * offset, -2.0 * offset};
-  out[2] = {std::min(center + 1, length - 1), 0.5 * (0.5 + offset) * (0.5 + offset), offset + 0.5};
+  out[2] = {std::min(center + 1, length - 1), 0.5 * ((0.5 + offset) * (0.5 + offset)), offset + 0.5};
 }
 
 // deposit_quadratic_bspline (electrostatic_placement.rs:900-916).

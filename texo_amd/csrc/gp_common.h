@@ -14,6 +14,8 @@ constexpr double FX_SCALE = 1099511627776.0;     // 2^40 fixed-point density qua
 constexpr double FX_INV = 1.0 / 1099511627776.0;
 constexpr uint32_t NO_UNIT = 0xffffffffu;        // fixed pin marker
 constexpr double AUGMENTED_DENSITY_BETA = 2000.0;  // eplace.rs:125
+constexpr int GF_STRIDE = 2 * MAXF + 2;  // filler gather partials: sums, kkt max, flag
+constexpr int UG_STRIDE = 3;             // unit gradient partials: wa l1, kkt max, flag
 
 // Reductions finished by k_finalize_eval for one objective evaluation.
 struct EvalScalars {
@@ -149,8 +151,8 @@ void launch_finalize_eval(const FinalizeArgs &a, EvalScalars *scalars, hipStream
 
 void launch_preconditioner(int n_units, int n_fillers, int F, int DU, const double *wl_diag,
                            const int32_t *unit_du, const double *du_fcharge, const uint8_t *fl_field,
-                           const double *fl_charge, const double *lambda, double *metric, MetricScalars *ms,
-                           hipStream_t s);
+                           const double *fl_charge, const double *lambda, double *metric, double *partials,
+                           MetricScalars *ms, hipStream_t s);
 void launch_extrapolate(size_t n, const double *x, const double *xp, const double *lower, const double *upper,
                         double beta, double *y, unsigned int *differs, hipStream_t s);
 void launch_candidate(size_t n, const double *y, const double *gy, const double *metric, double lipschitz,
@@ -174,6 +176,9 @@ struct DctPlan {
   const uint16_t *bhi[4];
   const uint16_t *blo[4];
   const uint16_t *bh16[4];
+  // transposed (N x K) BF16 hi/lo bases for the register-fragment kernels
+  const uint16_t *bthi[4];
+  const uint16_t *btlo[4];
   const float *inv_lambda;  // [H*W], layout ky*W+kx
   float *t0, *t1;           // [F*G] scratch
 };

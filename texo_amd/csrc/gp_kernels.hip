@@ -57,6 +57,32 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *p, double 
   atomicMax(p, static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// Block-wide maximum of a non-negative value and OR of a flag; thread 0
+// writes out[0] = max, out[1] = flag (as 0.0 / 1.0).  Used instead of
+// same-address atomics, which serialise across thousands of waves.
+__device__ void block_max_flag(double v, bool flag, double *out) {
+  __shared__ double s[BLOCK / 64];
+  __shared__ int f[BLOCK / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_max(v);
+  const bool any = __any(flag);
+  if (lane == 0) {
+    s[wave] = v;
+    f[wave] = any ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    int a = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+      m = fmax(m, s[w]);
+      a |= f[w];
+    }
+    out[0] = m;
+    out[1] = a ? 1.0 : 0.0;
+  }
+}
+
 // kkt_component (global_placement.rs:675-689) divided by sqrt(metric).
 __device__ __forceinline__ double kkt_scaled(double g, double x, double lo, double hi, double m) {
   const bool equal = __double_as_longlong(lo) == __double_as_longlong(hi);
@@ -298,10 +324,8 @@ __global__ void __launch_bounds__(BLOCK) k_gather_fillers(DensityArgs a, GatherA
                  kkt_scaled(vy, y, lower[c + 1], upper[c + 1], metric[c + 1]));
     }
   }
-  block_sum<2 * MAXF>(red, partials + (size_t)blockIdx.x * 2 * MAXF);
-  kkt = wave_max(kkt);
-  if ((threadIdx.x & 63) == 0 && kkt > 0.0) atomic_max_nonneg(&scalars->kkt_bits, kkt);
-  if (bad) scalars->nonfinite = 1u;
+  block_sum<2 * MAXF>(red, partials + (size_t)blockIdx.x * GF_STRIDE);
+  block_max_flag(kkt, bad, partials + (size_t)blockIdx.x * GF_STRIDE + 2 * MAXF);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,9 +338,9 @@ __device__ __forceinline__ double2 pin_coordinate(const WaArgs &a, const double 
   return make_double2(coords[2 * (size_t)u] + v.x, coords[2 * (size_t)u + 1] + v.y);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_wa_thread(WaArgs a, const double *__restrict__ coords, double gamma,
-                                                     double *__restrict__ partials) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
+__device__ void wa_thread_nets(const WaArgs &a, const double *__restrict__ coords, double gamma,
+                               double *__restrict__ partials, int block) {
+  const int t = block * BLOCK + threadIdx.x;
   double value[1] = {0.0};
   if (t < a.n_small) {
     const uint32_t net = a.small_nets[t];
@@ -366,10 +390,10 @@ __global__ void __launch_bounds__(BLOCK) k_wa_thread(WaArgs a, const double *__r
 }
 
 // One wavefront per net for high-fanout nets.
-__global__ void __launch_bounds__(BLOCK) k_wa_wave(WaArgs a, const double *__restrict__ coords, double gamma,
-                                                   double *__restrict__ partials) {
+__device__ void wa_wave_nets(const WaArgs &a, const double *__restrict__ coords, double gamma,
+                             double *__restrict__ partials, int block) {
   const int lane = threadIdx.x & 63;
-  const int t = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+  const int t = block * (BLOCK / 64) + (threadIdx.x >> 6);
   double value[1] = {0.0};
   if (t < a.n_large) {
     const uint32_t net = a.large_nets[t];
@@ -429,6 +453,17 @@ __global__ void __launch_bounds__(BLOCK) k_wa_wave(WaArgs a, const double *__res
   block_sum<1>(value, partials + blockIdx.x);
 }
 
+// One launch for both shapes: the first `wave_blocks` blocks take the
+// high-fanout nets (long serial chains, scheduled first), the rest take one
+// small net per thread.  Block b writes partial b.
+__global__ void __launch_bounds__(BLOCK) k_wa(WaArgs a, const double *__restrict__ coords, double gamma,
+                                              double *__restrict__ partials, int wave_blocks) {
+  if ((int)blockIdx.x < wave_blocks)
+    wa_wave_nets(a, coords, gamma, partials, blockIdx.x);
+  else
+    wa_thread_nets(a, coords, gamma, partials, blockIdx.x - wave_blocks);
+}
+
 // Combined unit gradient (combined_objective, eplace.rs:1022-1044): WA
 // baseline pins, then timed-arc pins, then each field in order.
 __global__ void __launch_bounds__(BLOCK) k_unit_gradient(WaArgs a, int F, int DU, const double2 *__restrict__ gfu,
@@ -478,31 +513,52 @@ __global__ void __launch_bounds__(BLOCK) k_unit_gradient(WaArgs a, int F, int DU
                  kkt_scaled(gy, coords[c + 1], lower[c + 1], upper[c + 1], metric[c + 1]));
     }
   }
-  block_sum<1>(l1, partials + blockIdx.x);
-  kkt = wave_max(kkt);
-  if ((threadIdx.x & 63) == 0 && kkt > 0.0) atomic_max_nonneg(&scalars->kkt_bits, kkt);
-  if (bad) scalars->nonfinite = 1u;
+  block_sum<1>(l1, partials + (size_t)blockIdx.x * UG_STRIDE);
+  block_max_flag(kkt, bad, partials + (size_t)blockIdx.x * UG_STRIDE + 1);
+}
+
+// Single-block reductions of per-block partials.  Each thread accumulates a
+// fixed strided subset, waves reduce by shuffles and four wave results are
+// combined in wave order: a fixed order, so results are reproducible.
+template <int K>
+__device__ void sum_partials(const double *__restrict__ p, int count, double (&out)[K], int stride, int offset) {
+  __shared__ double s[K][BLOCK / 64];
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < count; b += BLOCK)
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += p[(size_t)b * stride + offset + k];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    acc[k] = wave_sum(acc[k]);
+    if (lane == 0) s[k][wave] = acc[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = ((s[k][0] + s[k][1]) + s[k][2]) + s[k][3];
+  __syncthreads();
 }
 
 template <int K>
-__device__ void sum_partials(const double *__restrict__ p, int count, double (&out)[K], int stride, int offset) {
-  __shared__ double s[K][BLOCK];
+__device__ void max_partials(const double *__restrict__ p, int count, double (&out)[K], int stride, int offset) {
+  __shared__ double s[K][BLOCK / 64];
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < count; b += BLOCK)
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = fmax(acc[k], p[(size_t)b * stride + offset + k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < count; b += BLOCK) acc += p[(size_t)b * stride + offset + k];
-    s[k][threadIdx.x] = acc;
+    acc[k] = wave_max(acc[k]);
+    if (lane == 0) s[k][wave] = acc[k];
   }
   __syncthreads();
-  for (int width = BLOCK / 2; width > 0; width >>= 1) {
-    if (threadIdx.x < width) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + width];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) out[k] = s[k][0];
+  for (int k = 0; k < K; ++k) out[k] = fmax(fmax(s[k][0], s[k][1]), fmax(s[k][2], s[k][3]));
   __syncthreads();
 }
 
@@ -510,30 +566,27 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_eval(FinalizeArgs a, EvalSca
   double v1[1];
   sum_partials<1>(a.wa_partials, a.nb_wa, v1, 1, 0);
   if (threadIdx.x == 0) scalars->wa_value = v1[0];
-  sum_partials<1>(a.ug_partials, a.nb_ug, v1, 1, 0);
+  sum_partials<1>(a.ug_partials, a.nb_ug, v1, UG_STRIDE, 0);
   if (threadIdx.x == 0) scalars->wa_l1 = v1[0];
-  double e[MAXF];
-  sum_partials<MAXF>(a.energy_partials, a.nb_energy, e, MAXF, 0);
-  double gu[2 * MAXF], gf[2 * MAXF];
+  double gu[2 * MAXF], gf[2 * MAXF], mu[2], mf[2];
   sum_partials<2 * MAXF>(a.gu_partials, a.nb_gu, gu, 2 * MAXF, 0);
-  sum_partials<2 * MAXF>(a.gf_partials, a.nb_gf, gf, 2 * MAXF, 0);
-  if (threadIdx.x == 0) {
-    for (int f = 0; f < MAXF; ++f) {
-      scalars->energy[f] = e[f];
-      scalars->force_l1[f] = gu[f] + gf[f];
-      scalars->raw_l1[f] = gu[MAXF + f] + gf[MAXF + f];
-    }
-  }
-}
-
-// Energy partials are summed by blocks in the gather kernels with a serial
-// loop; the finalize kernel uses the same serial order for the reported
-// energy so both agree bit for bit.
-__global__ void k_energy_serial(const double *__restrict__ p, int nb, EvalScalars *scalars, int F) {
-  if (threadIdx.x < F) {
+  sum_partials<2 * MAXF>(a.gf_partials, a.nb_gf, gf, GF_STRIDE, 0);
+  max_partials<2>(a.ug_partials, a.nb_ug, mu, UG_STRIDE, 1);
+  max_partials<2>(a.gf_partials, a.nb_gf, mf, GF_STRIDE, 2 * MAXF);
+  if (threadIdx.x < MAXF) {
+    // the energy uses the same serial order as the coefficient computed in
+    // the gather kernels, so both agree bit for bit
+    const int f = threadIdx.x;
     double e = 0.0;
-    for (int b = 0; b < nb; ++b) e += p[b * MAXF + threadIdx.x];
-    scalars->energy[threadIdx.x] = e;
+    for (int b = 0; b < a.nb_energy; ++b) e += a.energy_partials[b * MAXF + f];
+    scalars->energy[f] = e;
+    scalars->force_l1[f] = gu[f] + gf[f];
+    scalars->raw_l1[f] = gu[MAXF + f] + gf[MAXF + f];
+  }
+  if (threadIdx.x == 0) {
+    const double kkt = fmax(mu[0], mf[0]);
+    scalars->kkt_bits = static_cast<unsigned long long>(__double_as_longlong(kkt));
+    scalars->nonfinite = (mu[1] > 0.0 || mf[1] > 0.0) ? 1u : 0u;
   }
 }
 
@@ -548,7 +601,7 @@ __global__ void __launch_bounds__(BLOCK) k_preconditioner(int n_units, int n_fil
                                                           const uint8_t *__restrict__ fl_field,
                                                           const double *__restrict__ fl_charge, double l0, double l1,
                                                           double l2, double l3, double *__restrict__ metric,
-                                                          MetricScalars *ms) {
+                                                          double *__restrict__ partials) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   const double lambda[MAXF] = {l0, l1, l2, l3};
   double d = 0.0;
@@ -569,26 +622,68 @@ __global__ void __launch_bounds__(BLOCK) k_preconditioner(int n_units, int n_fil
     valid = true;
   }
   double lo = INFINITY, hi = 0.0;
+  bool bad = false;
   if (valid) {
-    if (!isfinite(d) || d < 0.0) ms->nonfinite = 1u;
+    bad = !isfinite(d) || d < 0.0;
     d = fmax(d, 1.0);
     metric[2 * (size_t)i] = d;
     metric[2 * (size_t)i + 1] = d;
     lo = d;
     hi = d;
   }
+  // block max / -min / flag, reduced by k_metric_finalize
+  __shared__ double smax[BLOCK / 64], smin[BLOCK / 64];
+  __shared__ int sbad[BLOCK / 64];
   lo = wave_min(lo);
   hi = wave_max(hi);
-  if ((threadIdx.x & 63) == 0 && hi > 0.0) {
-    atomic_max_nonneg(&ms->max_bits, hi);
-    atomicMin(&ms->min_bits, static_cast<unsigned long long>(__double_as_longlong(lo)));
+  const bool anybad = __any(bad);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    smax[wave] = hi;
+    smin[wave] = lo;
+    sbad[wave] = anybad ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mx = 0.0, mn = INFINITY;
+    int b = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      mx = fmax(mx, smax[w]);
+      mn = fmin(mn, smin[w]);
+      b |= sbad[w];
+    }
+    partials[3 * (size_t)blockIdx.x] = mx;
+    partials[3 * (size_t)blockIdx.x + 1] = mn;
+    partials[3 * (size_t)blockIdx.x + 2] = b ? 1.0 : 0.0;
   }
 }
 
-__global__ void k_metric_reset(MetricScalars *ms) {
-  ms->max_bits = 0;
-  ms->min_bits = 0x7ff0000000000000ull;
-  ms->nonfinite = 0;
+__global__ void __launch_bounds__(BLOCK) k_metric_finalize(const double *__restrict__ partials, int nb,
+                                                           MetricScalars *ms) {
+  __shared__ double smax[BLOCK], smin[BLOCK], sbad[BLOCK];
+  double mx = 0.0, mn = INFINITY, b = 0.0;
+  for (int i = threadIdx.x; i < nb; i += BLOCK) {
+    mx = fmax(mx, partials[3 * (size_t)i]);
+    mn = fmin(mn, partials[3 * (size_t)i + 1]);
+    b = fmax(b, partials[3 * (size_t)i + 2]);
+  }
+  smax[threadIdx.x] = mx;
+  smin[threadIdx.x] = mn;
+  sbad[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+      sbad[threadIdx.x] = fmax(sbad[threadIdx.x], sbad[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ms->max_bits = static_cast<unsigned long long>(__double_as_longlong(smax[0]));
+    ms->min_bits = static_cast<unsigned long long>(__double_as_longlong(smin[0]));
+    ms->nonfinite = sbad[0] > 0.0 ? 1u : 0u;
+  }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_extrapolate(size_t n, const double *__restrict__ x,
@@ -597,13 +692,16 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(size_t n, const double *_
                                                        const double *__restrict__ upper, double beta,
                                                        double *__restrict__ y, unsigned int *differs) {
   const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  const double cur = x[i];
-  double v = cur + beta * (cur - xp[i]);
-  const double lo = lower[i], hi = upper[i];
-  v = v < lo ? lo : (v > hi ? hi : v);
-  y[i] = v;
-  if (v != cur) *differs = 1u;
+  bool changed = false;
+  if (i < n) {
+    const double cur = x[i];
+    double v = cur + beta * (cur - xp[i]);
+    const double lo = lower[i], hi = upper[i];
+    v = v < lo ? lo : (v > hi ? hi : v);
+    y[i] = v;
+    changed = v != cur;
+  }
+  if (__syncthreads_or(changed) && threadIdx.x == 0) *differs = 1u;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_candidate(size_t n, const double *__restrict__ y,
@@ -612,10 +710,11 @@ __global__ void __launch_bounds__(BLOCK) k_candidate(size_t n, const double *__r
                                                      const double *__restrict__ lower,
                                                      const double *__restrict__ upper,
                                                      const double *__restrict__ origin, double *__restrict__ cand,
-                                                     double *__restrict__ partials, StepScalars *step) {
+                                                     double *__restrict__ partials) {
   const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
   double red[2] = {0.0, 0.0};
   double change = 0.0;
+  bool changed = false;
   if (i < n) {
     const double m = metric[i];
     const double yi = y[i], g = gy[i];
@@ -623,24 +722,26 @@ __global__ void __launch_bounds__(BLOCK) k_candidate(size_t n, const double *__r
     const double lo = lower[i], hi = upper[i];
     c = c < lo ? lo : (c > hi ? hi : c);
     cand[i] = c;
-    if (c != yi) step->changed = 1u;
+    changed = c != yi;
     const double d = c - yi;
     red[0] = g * d;
     red[1] = m * d * d;
     change = fabs(c - origin[i]);
   }
-  block_sum<2>(red, partials + 2 * (size_t)blockIdx.x);
-  change = wave_max(change);
-  if ((threadIdx.x & 63) == 0 && change > 0.0) atomic_max_nonneg(&step->change_bits, change);
+  block_sum<2>(red, partials + 4 * (size_t)blockIdx.x);
+  block_max_flag(change, changed, partials + 4 * (size_t)blockIdx.x + 2);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_step_finalize(const double *__restrict__ partials, int nb,
                                                          StepScalars *step) {
-  double v[2];
-  sum_partials<2>(partials, nb, v, 2, 0);
+  double v[2], m[2];
+  sum_partials<2>(partials, nb, v, 4, 0);
+  max_partials<2>(partials, nb, m, 4, 2);
   if (threadIdx.x == 0) {
     step->gdelta = v[0];
     step->msq = v[1];
+    step->change_bits = static_cast<unsigned long long>(__double_as_longlong(m[0]));
+    step->changed = m[1] > 0.0 ? 1u : 0u;
   }
 }
 
@@ -799,13 +900,13 @@ void launch_gather(const DensityArgs &a, const GatherArgs &g, const double *coor
 }
 
 void launch_wa(const WaArgs &a, const double *coords, double gamma, double *partials, int *nb, hipStream_t s) {
-  const int nbs = std::max(1, blocks_for(a.n_small));
-  const int nbl = std::max(1, (a.n_large + BLOCK / 64 - 1) / (BLOCK / 64));
-  timer_begin(g_timer, "wa", s);
-  hipLaunchKernelGGL(k_wa_thread, dim3(nbs), dim3(BLOCK), 0, s, a, coords, gamma, partials);
-  timer_end(g_timer, "wa", s);
-  hipLaunchKernelGGL(k_wa_wave, dim3(nbl), dim3(BLOCK), 0, s, a, coords, gamma, partials + nbs);
+  const int nbs = a.n_small > 0 ? blocks_for(a.n_small) : 0;
+  const int nbl = a.n_large > 0 ? (a.n_large + BLOCK / 64 - 1) / (BLOCK / 64) : 0;
   *nb = nbs + nbl;
+  if (*nb == 0) return;
+  timer_begin(g_timer, "wa", s);
+  hipLaunchKernelGGL(k_wa, dim3(*nb), dim3(BLOCK), 0, s, a, coords, gamma, partials, nbl);
+  timer_end(g_timer, "wa", s);
 }
 
 void launch_unit_gradient(const WaArgs &a, int F, int DU, const double2 *gfu, const double *coords,
@@ -818,17 +919,16 @@ void launch_unit_gradient(const WaArgs &a, int F, int DU, const double2 *gfu, co
 
 void launch_finalize_eval(const FinalizeArgs &a, EvalScalars *scalars, hipStream_t s) {
   hipLaunchKernelGGL(k_finalize_eval, dim3(1), dim3(BLOCK), 0, s, a, scalars);
-  hipLaunchKernelGGL(k_energy_serial, dim3(1), dim3(64), 0, s, a.energy_partials, a.nb_energy, scalars, MAXF);
 }
 
 void launch_preconditioner(int n_units, int n_fillers, int F, int DU, const double *wl_diag, const int32_t *unit_du,
                            const double *du_fcharge, const uint8_t *fl_field, const double *fl_charge,
-                           const double *lambda, double *metric, MetricScalars *ms, hipStream_t s) {
-  hipLaunchKernelGGL(k_metric_reset, dim3(1), dim3(1), 0, s, ms);
+                           const double *lambda, double *metric, double *partials, MetricScalars *ms, hipStream_t s) {
   const int n = n_units + n_fillers;
-  hipLaunchKernelGGL(k_preconditioner, dim3(std::max(1, blocks_for(n))), dim3(BLOCK), 0, s, n_units, n_fillers, F, DU,
-                     wl_diag, unit_du, du_fcharge, fl_field, fl_charge, lambda[0], lambda[1], lambda[2], lambda[3],
-                     metric, ms);
+  const int nb = std::max(1, blocks_for(n));
+  hipLaunchKernelGGL(k_preconditioner, dim3(nb), dim3(BLOCK), 0, s, n_units, n_fillers, F, DU, wl_diag, unit_du,
+                     du_fcharge, fl_field, fl_charge, lambda[0], lambda[1], lambda[2], lambda[3], metric, partials);
+  hipLaunchKernelGGL(k_metric_finalize, dim3(1), dim3(BLOCK), 0, s, partials, nb, ms);
 }
 
 void launch_extrapolate(size_t n, const double *x, const double *xp, const double *lower, const double *upper,
@@ -842,7 +942,7 @@ void launch_candidate(size_t n, const double *y, const double *gy, const double 
                       StepScalars *step, hipStream_t s) {
   const int nb = std::max(1, blocks_for(n));
   hipLaunchKernelGGL(k_candidate, dim3(nb), dim3(BLOCK), 0, s, n, y, gy, metric, lipschitz, lower, upper, origin,
-                     cand, partials, step);
+                     cand, partials);
   hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(BLOCK), 0, s, partials, nb, step);
 }
 

@@ -202,6 +202,8 @@ struct Control {
 struct SessionImpl {
   Options opt;
   hipStream_t stream = nullptr;
+  hipStream_t wa_stream = nullptr;  // WA runs beside the density chain
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int device = 0;
   KernelTimer timer;
 
@@ -252,13 +254,13 @@ struct SessionImpl {
   int32_t *d_unit_du = nullptr;
   float *d_rho = nullptr, *d_phi = nullptr, *d_t0 = nullptr, *d_t1 = nullptr, *d_inv_lambda = nullptr;
   float *d_b32[4] = {};
-  uint16_t *d_bhi[4] = {}, *d_blo[4] = {};
+  uint16_t *d_bhi[4] = {}, *d_blo[4] = {}, *d_bthi[4] = {}, *d_btlo[4] = {};
   double *d_lower = nullptr, *d_upper = nullptr, *d_metric = nullptr;
   double *d_vec[4] = {};   // coordinate buffers (rotated)
   double *d_grad[3] = {};  // origin, y, candidate gradients
   double2 *d_gfu = nullptr;
   double *d_part_wa = nullptr, *d_part_energy = nullptr, *d_part_gu = nullptr, *d_part_gf = nullptr,
-         *d_part_ug = nullptr, *d_part_step = nullptr;
+         *d_part_ug = nullptr, *d_part_step = nullptr, *d_part_metric = nullptr;
   EvalScalars *d_eval = nullptr;  // 4 slots
   StepScalars *d_step = nullptr;
   MetricScalars *d_metric_s = nullptr;
@@ -301,6 +303,9 @@ struct SessionImpl {
     if (h_step) (void)hipHostFree(h_step);
     if (h_metric) (void)hipHostFree(h_metric);
     if (h_flag) (void)hipHostFree(h_flag);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (wa_stream) (void)hipStreamDestroy(wa_stream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -660,6 +665,14 @@ void SessionImpl::build_dct() {
     HIP(hipMemcpy(d_b32[j], f32.data(), n * n * sizeof(float), hipMemcpyHostToDevice));
     HIP(hipMemcpy(d_bhi[j], hi.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP(hipMemcpy(d_blo[j], lo.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    std::vector<uint16_t> thi(n * n), tlo(n * n);
+    for (size_t r = 0; r < n; ++r)
+      for (size_t k = 0; k < n; ++k) {
+        thi[k * n + r] = hi[r * n + k];
+        tlo[k * n + r] = lo[r * n + k];
+      }
+    HIP(hipMemcpy(d_bthi[j], thi.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_btlo[j], tlo.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
   }
   // Poisson2d::new (electrostatic_placement.rs:675-714)
   std::vector<float> il(G);
@@ -680,6 +693,8 @@ void SessionImpl::build_dct() {
     dct.b32[j] = d_b32[j];
     dct.bhi[j] = d_bhi[j];
     dct.blo[j] = d_blo[j];
+    dct.bthi[j] = d_bthi[j];
+    dct.btlo[j] = d_btlo[j];
     dct.bh16[j] = nullptr;
   }
   dct.inv_lambda = d_inv_lambda;
@@ -691,6 +706,9 @@ void SessionImpl::upload() {
   HIP(hipSetDevice(opt.device));
   device = opt.device;
   HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP(hipStreamCreateWithFlags(&wa_stream, hipStreamNonBlocking));
+  HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   HIP(hipHostMalloc(reinterpret_cast<void **>(&h_eval), 4 * sizeof(EvalScalars), hipHostMallocDefault));
   HIP(hipHostMalloc(reinterpret_cast<void **>(&h_step), sizeof(StepScalars), hipHostMallocDefault));
   HIP(hipHostMalloc(reinterpret_cast<void **>(&h_metric), sizeof(MetricScalars), hipHostMallocDefault));
@@ -730,6 +748,8 @@ void SessionImpl::upload() {
     arena.reserve(d_b32[j], dims[j] * dims[j]);
     arena.reserve(d_bhi[j], dims[j] * dims[j]);
     arena.reserve(d_blo[j], dims[j] * dims[j]);
+    arena.reserve(d_bthi[j], dims[j] * dims[j]);
+    arena.reserve(d_btlo[j], dims[j] * dims[j]);
   }
   arena.reserve(d_lower, Cc);
   arena.reserve(d_upper, Cc);
@@ -754,9 +774,10 @@ void SessionImpl::upload() {
   arena.reserve(d_part_wa, nb_wa);
   arena.reserve(d_part_energy, nb_energy * MAXF);
   arena.reserve(d_part_gu, blocks(DU, 256) * 2 * MAXF);
-  arena.reserve(d_part_gf, blocks(Fl, 256) * 2 * MAXF);
-  arena.reserve(d_part_ug, blocks(U, 256));
-  arena.reserve(d_part_step, blocks(Cc, 256) * 2);
+  arena.reserve(d_part_gf, blocks(Fl, 256) * GF_STRIDE);
+  arena.reserve(d_part_ug, blocks(U, 256) * UG_STRIDE);
+  arena.reserve(d_part_step, blocks(Cc, 256) * 4);
+  arena.reserve(d_part_metric, blocks(U + Fl, 256) * 3);
   arena.reserve(d_eval, 4);
   arena.reserve(d_step, 1);
   arena.reserve(d_metric_s, 1);
@@ -877,6 +898,12 @@ void SessionImpl::evaluate(const double *coords, double gamma, const double *met
   if (opt.inject && evaluations == opt.inject) fail(TEXO_GP_ERR_INJECTED);
   EvalScalars *sc = &d_eval[slot];
   int nb_energy = 0, nb_gu = 0, nb_gf = 0, nb_wa = 0, nb_ug = 0;
+  // WA depends only on the coordinates: fork it onto the second stream so it
+  // overlaps the deposit / transform / gather chain, join before combining.
+  HIP(hipEventRecord(ev_fork, stream));
+  HIP(hipStreamWaitEvent(wa_stream, ev_fork, 0));
+  launch_wa(wa, coords, gamma, d_part_wa, &nb_wa, wa_stream);
+  HIP(hipEventRecord(ev_join, wa_stream));
   if (F == 0) {
     launch_zero_eval(sc, stream);
   } else {
@@ -895,7 +922,7 @@ void SessionImpl::evaluate(const double *coords, double gamma, const double *met
   if (F > 0)
     launch_gather(dens, g, coords, d_gfu, parity ? d_raw_units : nullptr, parity ? d_raw_fillers : nullptr, grad,
                   metric, d_lower, d_upper, d_part_gu, d_part_gf, sc, &nb_gu, &nb_gf, stream);
-  launch_wa(wa, coords, gamma, d_part_wa, &nb_wa, stream);
+  HIP(hipStreamWaitEvent(stream, ev_join, 0));
   launch_unit_gradient(wa, F, (int)DU, d_gfu, coords, metric, d_lower, d_upper, grad, parity ? d_wa_grad : nullptr,
                        d_part_ug, sc, &nb_ug, stream);
   FinalizeArgs fa{};
@@ -1010,7 +1037,7 @@ void SessionImpl::set_preconditioner() {
     lam[f] = mult[f];
   }
   launch_preconditioner((int)U, (int)Fl, F, (int)DU, d_wl_diag, d_unit_du, d_du_fcharge, d_fl_field, d_fl_charge, lam,
-                        d_metric, d_metric_s, stream);
+                        d_metric, d_part_metric, d_metric_s, stream);
   HIP(hipMemcpyAsync(h_metric, d_metric_s, sizeof(MetricScalars), hipMemcpyDeviceToHost, stream));
 }
 
