@@ -322,6 +322,40 @@ int texo_gp_session_kernel_timing(texo_gp_session *session, double *avg_ms,
 int texo_gp_session_iterate(texo_gp_session *session, uint32_t iterations,
                             texo_gp_report *report);
 
+/* ---- serving: continuous batching of many placement jobs --------------- */
+/* No counterpart in the reference (it places one design per call).  A server
+ * owns one GPU and a worker thread that runs every admitted job in lockstep
+ * rounds: each round issues one launch per kernel family over all jobs.
+ * Submitted problems are deep-copied and validated on the caller's thread;
+ * jobs are admitted in submission order while their HBM footprint fits
+ * `hbm_budget_bytes` and fewer than `max_batch` jobs run; jobs that arrive
+ * while others run join at the next round.  A job's result is bit-identical
+ * to the same job run alone. */
+typedef struct texo_gp_server texo_gp_server;
+
+typedef struct texo_gp_server_stats {
+  uint64_t submitted, admitted, completed, failed;
+  uint64_t rounds;          /* lockstep rounds executed */
+  uint64_t active, pending; /* jobs on the GPU / waiting for admission */
+  uint64_t hbm_in_use;      /* bytes held by admitted jobs */
+  double busy_seconds;      /* wall time spent in rounds */
+} texo_gp_server_stats;
+
+texo_gp_server *texo_gp_server_create(int32_t device, uint64_t hbm_budget_bytes, uint32_t max_batch);
+/* Returns a job id > 0, or -(status) if the problem is rejected. */
+int64_t texo_gp_server_submit(texo_gp_server *server, const texo_gp_problem *problem,
+                              const texo_gp_options *options);
+/* TEXO_GP_ERR_NOT_READY while the job runs; otherwise the job's status, with
+ * targets and report filled (report->elapsed_seconds = submission to
+ * completion).  A finished job's result is released by the first successful
+ * poll/wait. */
+int texo_gp_server_poll(texo_gp_server *server, int64_t job, uint32_t *out_targets_xy,
+                        texo_gp_report *report);
+int texo_gp_server_wait(texo_gp_server *server, int64_t job, uint32_t *out_targets_xy,
+                        texo_gp_report *report);
+int texo_gp_server_stats_get(texo_gp_server *server, texo_gp_server_stats *stats);
+void texo_gp_server_destroy(texo_gp_server *server);
+
 /* Last error message of the calling thread (static storage). */
 const char *texo_gp_last_error(void);
 

@@ -1,0 +1,94 @@
+"""Continuous-batching server (texo_gp_server_*): a job's result is
+bit-identical to the same job placed alone, whatever else shares its rounds
+(other grid sizes, other transform precisions, jobs that join mid-run)."""
+import time
+
+import numpy as np
+import pytest
+
+import texo_amd
+from texo_amd import abi, synth
+from tests import problems
+
+pytestmark = pytest.mark.gpu
+
+ITERS = 40
+
+
+def mix():
+    return [
+        (synth.plain_1k(3), abi.DCT_FP32),
+        (synth.serving_job(11, 2000, 4), abi.DCT_FP32),
+        (synth.serving_job(12, 6000, 2), abi.DCT_BF16X3),
+        (synth.serving_job(13, 2000, 1), abi.DCT_BF16X3),
+    ]
+
+
+def alone(problem, precision, fixed=ITERS):
+    return texo_amd.place(problem, dct_precision=precision, fixed_iterations=fixed)
+
+
+def same(a, b):
+    (ta, ra), (tb, rb) = a, b
+    assert np.array_equal(ta, tb)
+    for k in ("iterations", "evaluations", "line_search_trials", "area_rounds", "final_hpwl",
+              "final_wirelength", "field_overflow", "field_energy"):
+        assert ra[k] == rb[k], k
+
+
+def test_batch_is_bit_identical_to_alone():
+    jobs = mix()
+    print("serving mix alone", [p.meta["name"] for p, _ in jobs], flush=True)
+    ref = [alone(p, prec) for p, prec in jobs]
+    with texo_amd.Server() as srv:
+        ids = [srv.submit(p, abi.options(dct_precision=prec, fixed_iterations=ITERS)) for p, prec in jobs]
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    print("stats", st, flush=True)
+    for a, b in zip(ref, got):
+        same(a, b)
+    assert st["completed"] == len(jobs) and st["failed"] == 0 and st["active"] == 0
+    # lockstep: the batch needs about as many rounds as its longest job, not the sum
+    assert st["rounds"] < sum(r["evaluations"] for _, r in ref)
+
+
+def test_job_joining_mid_run_is_bit_identical():
+    (p0, prec0), (p1, prec1) = mix()[1], mix()[0]
+    ref0, ref1 = alone(p0, prec0, 60), alone(p1, prec1)
+    with texo_amd.Server() as srv:
+        a = srv.submit(p0, abi.options(dct_precision=prec0, fixed_iterations=60))
+        deadline = time.time() + 60
+        while srv.stats()["rounds"] < 20 and time.time() < deadline:
+            time.sleep(0.001)
+        b = srv.submit(p1, abi.options(dct_precision=prec1, fixed_iterations=ITERS))
+        same(ref1, srv.wait(b))
+        same(ref0, srv.wait(a))
+
+
+def test_full_placement_with_stop_rules_in_a_batch():
+    p = synth.plain_1k(5)
+    ref = texo_amd.place(p)
+    with texo_amd.Server(max_batch=2) as srv:
+        ids = [srv.submit(p), srv.submit(synth.plain_1k(6)), srv.submit(p)]
+        got = [srv.wait(i) for i in ids]
+    same(ref, got[0])
+    same(ref, got[2])
+    assert got[0][1]["field_overflow"][0] <= 0.10
+
+
+def test_rejected_problem_fails_at_submit_and_poll_releases_results():
+    W, H, cap = problems.mixed_device_capacity()
+    bad = problems.density_problem(W, H, cap, [(0.0, 0.0, [(abi.KIND_MEMORY, 0.0, 0.0)])])
+    with texo_amd.Server() as srv:
+        with pytest.raises(abi.PlacementError) as e:
+            srv.submit(bad)
+        assert e.value.name == "InsufficientCapacity"
+        job = srv.submit(synth.plain_1k(3), abi.options(fixed_iterations=5))
+        res = None
+        deadline = time.time() + 60
+        while res is None and time.time() < deadline:
+            res = srv.poll(job)
+            time.sleep(0.001)
+        assert res is not None and res[1]["iterations"] == 5 and res[1]["elapsed_seconds"] > 0.0
+        rc = srv.lib.texo_gp_server_poll(srv.handle, job, None, None)
+        assert abi.STATUS[rc] == "InvalidArgument"  # released by the first poll

@@ -34,6 +34,7 @@ def load():
                            "(the placement path has no CPU fallback)")
     lib = C.CDLL(LIB_PATH)
     abi.declare_session_api(lib, "texo_gp")
+    abi.declare_server_api(lib, "texo_gp")
     P, d = C.POINTER, C.c_double
     lib.texo_gp_last_error.restype = C.c_char_p
     lib.texo_gp_last_error.argtypes = []
@@ -92,3 +93,71 @@ def place(problem: Problem, dct_precision: int = abi.DCT_FP32, fixed_iterations:
     if rc != 0:
         raise PlacementError(rep)
     return targets.reshape(-1, 2), rep.as_dict()
+
+
+class Server:
+    """Continuous-batching placement server on one GPU.
+
+    Jobs submitted here run in lockstep rounds with every other admitted job:
+    each round issues one launch per kernel family over the whole batch, and a
+    job's result is bit-identical to the same job placed alone.  `submit`
+    deep-copies and validates the problem on the calling thread; `wait` and
+    `poll` return `(targets[n_units, 2], report)` or raise `PlacementError`.
+    """
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_batch: int = 0):
+        self.lib = load()
+        self.handle = self.lib.texo_gp_server_create(device, hbm_budget_bytes, max_batch)
+        if not self.handle:
+            raise RuntimeError(f"texo_gp_server_create failed: {last_error()}")
+        self._units = {}
+
+    def submit(self, problem: Problem, opts: Optional[abi.texo_gp_options] = None) -> int:
+        opts = opts if opts is not None else options()
+        job = self.lib.texo_gp_server_submit(self.handle, C.byref(problem.struct()), C.byref(opts))
+        if job <= 0:
+            raise PlacementError.from_status(-job)
+        self._units[job] = problem.n_units
+        return job
+
+    def _result(self, fn, job: int):
+        targets = np.zeros(2 * self._units[job], np.uint32)
+        rep = abi.texo_gp_report()
+        rc = fn(self.handle, job, abi.ptr(targets, C.c_uint32), C.byref(rep))
+        if rc == abi.STATUS_CODE["NotReady"]:
+            return None
+        del self._units[job]
+        if rc != 0:
+            raise PlacementError(rep)
+        return targets.reshape(-1, 2), rep.as_dict()
+
+    def poll(self, job: int):
+        """None while the job runs, else its result (released by this call)."""
+        return self._result(self.lib.texo_gp_server_poll, job)
+
+    def wait(self, job: int):
+        return self._result(self.lib.texo_gp_server_wait, job)
+
+    def stats(self) -> dict:
+        st = abi.texo_gp_server_stats()
+        rc = self.lib.texo_gp_server_stats_get(self.handle, C.byref(st))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.texo_gp_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

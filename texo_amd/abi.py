@@ -51,6 +51,7 @@ STATUS = {
     900: "InvalidArgument", 901: "Device", 902: "OutOfMemory", 903: "Unsupported",
     904: "Cancelled", 905: "NotReady", 906: "Injected",
 }
+STATUS_CODE = {name: code for code, name in STATUS.items()}
 
 _p = C.POINTER
 
@@ -397,3 +398,31 @@ class PlacementError(RuntimeError):
         r = texo_gp_report()
         r.status = status
         return cls(r)
+
+
+class texo_gp_server_stats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+        ("failed", C.c_uint64), ("rounds", C.c_uint64), ("active", C.c_uint64),
+        ("pending", C.c_uint64), ("hbm_in_use", C.c_uint64), ("busy_seconds", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def declare_server_api(lib, prefix: str):
+    """Declares the continuous-batching server entry points of the product library."""
+    P = C.POINTER
+
+    def f(name, restype, *argtypes):
+        fn = getattr(lib, f"{prefix}_{name}")
+        fn.restype = restype
+        fn.argtypes = list(argtypes)
+
+    f("server_create", C.c_void_p, C.c_int32, C.c_uint64, C.c_uint32)
+    f("server_submit", C.c_int64, C.c_void_p, P(texo_gp_problem), P(texo_gp_options))
+    f("server_poll", C.c_int, C.c_void_p, C.c_int64, P(C.c_uint32), P(texo_gp_report))
+    f("server_wait", C.c_int, C.c_void_p, C.c_int64, P(C.c_uint32), P(texo_gp_report))
+    f("server_stats_get", C.c_int, C.c_void_p, P(texo_gp_server_stats))
+    f("server_destroy", None, C.c_void_p)

@@ -1,0 +1,581 @@
+// gp_api.hip — the C ABI of include/texo_gp.h: one-call placement
+// (eplace::place), evaluate-level sessions used by the parity tests, and the
+// continuous-batching server.
+
+#include "gp_host.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+using texo::Engine;
+using texo::Failure;
+using texo::Job;
+using texo::Phase;
+
+namespace {
+
+void put_failure(texo_gp_report *r, const Failure &f) {
+  if (!r) return;
+  r->status = f.status;
+  r->err_index = f.index;
+  r->err_index2 = f.index2;
+  r->err_kind = f.kind;
+  r->err_x = f.x;
+  r->err_y = f.y;
+  r->err_value[0] = f.v0;
+  r->err_value[1] = f.v1;
+}
+
+// Final report of a job (status, counts, HPWL and field overflows).
+void fill_report(Engine &e, Job &j, texo_gp_report *r) {
+  if (!r) return;
+  r->status = j.phase == Phase::Failed ? j.status : 0;
+  if (j.phase == Phase::Failed) put_failure(r, j.failure);
+  r->iterations = j.iterations;
+  r->evaluations = j.evaluations;
+  r->line_search_trials = j.trials;
+  r->area_rounds = j.area_rounds;
+  r->dct_precision = j.opt.precision;
+  r->fixed_iterations = j.opt.fixed_iterations;
+  r->gamma_base = j.opt.gamma_base;
+  r->final_wirelength = j.last_wa;
+  std::vector<double> x(2 * j.U);
+  if (j.U) {
+    if (j.empty) {
+      x = j.initial_xy;
+    } else {
+      HIP(hipMemcpyAsync(x.data(), j.d_vec[j.ix], 2 * j.U * sizeof(double), hipMemcpyDeviceToHost, e.stream()));
+      HIP(hipStreamSynchronize(e.stream()));
+    }
+  }
+  r->final_hpwl = j.hpwl(x);
+  r->n_fields = (uint32_t)j.F;
+  for (int f = 0; f < j.F && f < TEXO_GP_MAX_FIELDS; ++f) {
+    r->field_kind[f] = j.field_kind[f];
+    r->field_overflow[f] = j.started ? j.overflow(j.current, f) : 0.0;
+    r->field_energy[f] = j.current.energy[f];
+  }
+}
+
+// Targets of a finished job: the best checkpoint (eplace.rs:553), or the
+// rounded inputs when there is no density unit (eplace.rs:303-310).
+void copy_targets(Engine &e, Job &j, uint32_t *out) {
+  if (!out) return;
+  if (j.empty) {
+    for (size_t i = 0; i < 2 * j.U; ++i)
+      out[i] = texo::rounded_coordinate(j.initial_xy[i], (i & 1) ? j.H : j.W);
+    return;
+  }
+  HIP(hipMemcpyAsync(out, j.d_best, 2 * j.U * sizeof(uint32_t), hipMemcpyDeviceToHost, e.stream()));
+  HIP(hipStreamSynchronize(e.stream()));
+}
+
+void drive(Engine &e, Job &j) {
+  std::vector<Job *> one{&j};
+  while (j.phase != Phase::Done && j.phase != Phase::Failed && j.phase != Phase::Paused) e.round(one);
+}
+
+}  // namespace
+
+struct texo_gp_session {
+  std::unique_ptr<Engine> engine;
+  Job job;
+  ~texo_gp_session() {
+    if (engine) {
+      (void)hipSetDevice(engine->device());
+      engine->detach(&job);
+    }
+  }
+};
+
+extern "C" {
+
+const char *texo_gp_last_error(void) { return texo::g_last_error.c_str(); }
+
+texo_gp_session *texo_gp_session_create(const texo_gp_problem *problem, const texo_gp_options *options,
+                                        texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  try {
+    std::unique_ptr<texo_gp_session> s(new texo_gp_session());
+    s->job.load(problem, options);
+    s->engine.reset(new Engine(s->job.opt.device, 4));
+    s->engine->attach(&s->job);
+    return s.release();
+  } catch (const Failure &f) {
+    put_failure(report, f);
+    return nullptr;
+  } catch (const std::bad_alloc &) {
+    if (report) report->status = TEXO_GP_ERR_OUT_OF_MEMORY;
+    return nullptr;
+  }
+}
+
+void texo_gp_session_destroy(texo_gp_session *s) { delete s; }
+
+int texo_gp_session_sizes(const texo_gp_session *s, uint64_t *coordinates, uint32_t *fields, uint64_t *fillers,
+                          uint64_t *density_units) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  if (coordinates) *coordinates = s->job.C;
+  if (fields) *fields = (uint32_t)s->job.F;
+  if (fillers) *fillers = s->job.Fl;
+  if (density_units) *density_units = s->job.DU;
+  return 0;
+}
+
+int texo_gp_session_initial(const texo_gp_session *s, double *coordinates, double *lower, double *upper,
+                            uint32_t *field_kind, uint32_t *filler_field, double *filler_charge) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const Job &J = s->job;
+  if (coordinates) {
+    for (size_t u = 0; u < J.U; ++u) {
+      coordinates[2 * u] = texo::clampd(J.initial_xy[2 * u], J.lower[2 * u], J.upper[2 * u]);
+      coordinates[2 * u + 1] = texo::clampd(J.initial_xy[2 * u + 1], J.lower[2 * u + 1], J.upper[2 * u + 1]);
+    }
+    for (size_t k = 0; k < J.Fl; ++k) {
+      coordinates[2 * (J.U + k)] = J.fl_xy[2 * k];
+      coordinates[2 * (J.U + k) + 1] = J.fl_xy[2 * k + 1];
+    }
+  }
+  if (lower) std::copy(J.lower.begin(), J.lower.end(), lower);
+  if (upper) std::copy(J.upper.begin(), J.upper.end(), upper);
+  if (field_kind) std::copy(J.field_kind.begin(), J.field_kind.end(), field_kind);
+  for (size_t k = 0; k < J.Fl; ++k) {
+    if (filler_field) filler_field[k] = J.fl_field[k];
+    if (filler_charge) filler_charge[k] = J.fl_charge[k];
+  }
+  return 0;
+}
+
+int texo_gp_session_evaluate(texo_gp_session *s, const double *coordinates, double gamma, double *wa_value,
+                             double *wa_gradient, texo_gp_field_eval *fields, double *unit_gradient,
+                             double *filler_gradient, double *density, double *potential) {
+  if (!s || !coordinates) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &J = s->job;
+  try {
+    if (!(std::isfinite(gamma) && gamma > 0.0)) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    // validate_units_with_positions / validate_fillers_with_positions
+    // (electrostatic_placement.rs:485-582)
+    for (size_t i = 0; i < J.DU; ++i) {
+      const uint32_t u = J.du_unit[i];
+      if (!std::isfinite(coordinates[2 * u]) || !std::isfinite(coordinates[2 * u + 1]))
+        texo::fail(TEXO_GP_ERR_NON_FINITE_UNIT_COORDINATE, (int64_t)i, 0);
+    }
+    for (size_t k = 0; k < J.Fl; ++k)
+      if (!std::isfinite(coordinates[2 * (J.U + k)]) || !std::isfinite(coordinates[2 * (J.U + k) + 1]))
+        texo::fail(TEXO_GP_ERR_NON_FINITE_FILLER, (int64_t)k);
+    for (size_t u = 0; u < J.U; ++u)
+      if (!std::isfinite(coordinates[2 * u]) || !std::isfinite(coordinates[2 * u + 1]))
+        return TEXO_GP_ERR_INVALID_ARGUMENT;
+    Engine &E = *s->engine;
+    E.evaluate_once(&J, coordinates, gamma, density != nullptr);
+    const size_t FG = (size_t)J.F * J.G;
+    std::vector<float> phi(potential ? FG : 0);
+    hipStream_t st = E.stream();
+    if (wa_gradient && J.U)
+      HIP(hipMemcpyAsync(wa_gradient, J.d_wa_grad, 2 * J.U * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (unit_gradient && J.F && J.DU)
+      HIP(hipMemcpyAsync(unit_gradient, J.d_raw_units, 2 * (size_t)J.F * J.DU * sizeof(double), hipMemcpyDeviceToHost,
+                         st));
+    if (filler_gradient && J.Fl)
+      HIP(hipMemcpyAsync(filler_gradient, J.d_raw_fillers, 2 * J.Fl * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (density && FG) HIP(hipMemcpyAsync(density, J.d_density_out, FG * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (potential && FG) HIP(hipMemcpyAsync(phi.data(), J.slot[0].phi, FG * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+    const texo::EvalScalars &e = J.h_scalars->eval[2];
+    if (wa_value) *wa_value = e.wa_value;
+    for (int f = 0; f < J.F && fields; ++f) {
+      fields[f].energy = e.energy[f];
+      fields[f].normalized_overflow = J.overflow(e, f);
+      fields[f].net_charge = static_cast<double>(e.net_fx[f]) * texo::FX_INV;
+      fields[f].force_l1 = e.force_l1[f];
+      fields[f].real_area = J.real_area[f];
+      fields[f].filler_charge = (double)J.available[f] - J.real_area[f];
+    }
+    for (size_t i = 0; i < FG && potential; ++i) potential[i] = static_cast<double>(phi[i]);
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int texo_gp_session_poisson(texo_gp_session *s, const double *density, double *potential, double *energy) {
+  if (!s || !density || !potential || !energy) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &J = s->job;
+  try {
+    if (J.F == 0) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    Engine &E = *s->engine;
+    hipStream_t st = E.stream();
+    const size_t FG = (size_t)J.F * J.G;
+    std::vector<float> rho(FG), phi(FG);
+    for (size_t i = 0; i < FG; ++i) {
+      if (!std::isfinite(density[i])) return TEXO_GP_ERR_NON_FINITE_DENSITY;
+      rho[i] = static_cast<float>(density[i]);
+    }
+    if (J.p_slot == 0) J.p_valid = false;
+    HIP(hipMemcpyAsync(J.slot[0].rho, rho.data(), FG * sizeof(float), hipMemcpyHostToDevice, st));
+    texo::Task t = J.task(0, J.d_eval_coords, J.d_grad[0], nullptr, 1.0, false, &J.d_scalars->eval[2]);
+    std::vector<int2> gh, gw;
+    for (int b = 0; b < (int)((J.F * J.H + 31) / 32); ++b) gh.push_back(make_int2(0, b));
+    for (int b = 0; b < (int)((J.F * J.W + 31) / 32); ++b) gw.push_back(make_int2(0, b));
+    texo::Task *d_task = nullptr;
+    int2 *d_maps = nullptr;
+    HIP(hipMalloc(&d_task, sizeof(texo::Task)));
+    HIP(hipMalloc(&d_maps, (gh.size() + gw.size()) * sizeof(int2)));
+    HIP(hipMemcpy(d_task, &t, sizeof(t), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_maps, gh.data(), gh.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_maps + gh.size(), gw.data(), gw.size() * sizeof(int2), hipMemcpyHostToDevice));
+    texo::DctLaunch d{};
+    d.map_h = d_maps;
+    d.map_w = d_maps + gh.size();
+    d.row_blocks_h = (int)gh.size();
+    d.row_blocks_w = (int)gw.size();
+    d.W = (int)J.W;
+    d.H = (int)J.H;
+    d.precision = (int)J.opt.precision;
+    texo::launch_poisson(d_task, d, st);
+    std::vector<double> parts((size_t)J.nb_energy * texo::MAXF);
+    HIP(hipMemcpyAsync(parts.data(), J.slot[0].part_energy, parts.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP(hipMemcpyAsync(phi.data(), J.slot[0].phi, FG * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+    HIP(hipFree(d_task));
+    HIP(hipFree(d_maps));
+    for (size_t i = 0; i < FG; ++i) potential[i] = phi[i];
+    for (int f = 0; f < J.F; ++f) {
+      double e = 0.0;
+      for (int b = 0; b < J.nb_energy; ++b) e += parts[(size_t)b * texo::MAXF + f];
+      energy[f] = e;
+    }
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int texo_gp_session_run(texo_gp_session *s, uint32_t *out_targets_xy, texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  if (!s || !out_targets_xy) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const auto t0 = std::chrono::steady_clock::now();
+  try {
+    Job &J = s->job;
+    J.begin_run(false);
+    drive(*s->engine, J);
+    fill_report(*s->engine, J, report);
+    if (J.phase == Phase::Failed) return J.status;
+    copy_targets(*s->engine, J, out_targets_xy);
+    if (report) report->elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  } catch (const Failure &f) {
+    put_failure(report, f);
+    return f.status;
+  }
+}
+
+int texo_gp_session_iterate(texo_gp_session *s, uint32_t iterations, texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    Job &J = s->job;
+    if (J.empty) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    if (!J.started) J.begin_run(true);
+    J.pause_at = J.iterations + iterations;
+    if (J.phase == Phase::Paused) J.phase = Phase::IterStart;
+    drive(*s->engine, J);
+    fill_report(*s->engine, J, report);
+    return J.phase == Phase::Failed ? J.status : 0;
+  } catch (const Failure &f) {
+    put_failure(report, f);
+    return f.status;
+  }
+}
+
+int texo_gp_session_coordinates(const texo_gp_session *s, double *coordinates) {
+  if (!s || !coordinates) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const Job &J = s->job;
+  if (J.empty) {
+    std::copy(J.initial_xy.begin(), J.initial_xy.end(), coordinates);
+    return 0;
+  }
+  if (hipSetDevice(s->engine->device()) != hipSuccess) return TEXO_GP_ERR_DEVICE;
+  if (hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, s->engine->stream()) !=
+          hipSuccess ||
+      hipStreamSynchronize(s->engine->stream()) != hipSuccess)
+    return TEXO_GP_ERR_DEVICE;
+  return 0;
+}
+
+int texo_gp_session_rudy(texo_gp_session *s, const double *coordinates, double *horizontal, double *vertical) {
+  if (!s || !coordinates || !horizontal || !vertical) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &J = s->job;
+  try {
+    hipStream_t st = s->engine->stream();
+    HIP(hipMemcpyAsync(J.d_eval_coords, coordinates, J.C * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP(hipMemsetAsync(J.d_rudy_err, 0, sizeof(unsigned int), st));
+    texo::g_timer = &s->engine->timer;
+    texo::launch_rudy((int)J.n_nets, J.d_net_start, J.d_pin_unit, J.d_pin_xy, J.d_eval_coords, (int)J.W, (int)J.H,
+                      J.d_rudy_fx, J.d_rudy, J.d_rudy_err, st);
+    texo::g_timer = nullptr;
+    unsigned int err = 0;
+    HIP(hipMemcpyAsync(horizontal, J.d_rudy, J.G * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP(hipMemcpyAsync(vertical, J.d_rudy + J.G, J.G * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP(hipMemcpyAsync(&err, J.d_rudy_err, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+    return err ? TEXO_GP_ERR_INVALID_NORMALIZATION : 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int texo_gp_session_set_kernel_timing(texo_gp_session *s, const char *kernel, int enable) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  s->engine->timer.name = kernel ? kernel : "";
+  s->engine->timer.enabled = enable != 0;
+  s->engine->timer.used = 0;
+  return 0;
+}
+
+int texo_gp_session_kernel_timing(texo_gp_session *s, double *avg_ms, uint64_t *count) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  texo::KernelTimer &t = s->engine->timer;
+  if (hipStreamSynchronize(s->engine->stream()) != hipSuccess) return TEXO_GP_ERR_DEVICE;
+  if (hipDeviceSynchronize() != hipSuccess) return TEXO_GP_ERR_DEVICE;
+  double total = 0.0;
+  for (size_t i = 0; i < t.used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.events[i].first, t.events[i].second) != hipSuccess) return TEXO_GP_ERR_DEVICE;
+    total += ms;
+  }
+  if (avg_ms) *avg_ms = t.used ? total / (double)t.used : 0.0;
+  if (count) *count = t.used;
+  t.used = 0;
+  return 0;
+}
+
+int texo_gp_place(const texo_gp_problem *problem, const texo_gp_options *options, uint32_t *out_targets_xy,
+                  texo_gp_report *report) {
+  texo_gp_session *s = texo_gp_session_create(problem, options, report);
+  if (!s) return report ? report->status : TEXO_GP_ERR_INVALID_ARGUMENT;
+  const int rc = texo_gp_session_run(s, out_targets_xy, report);
+  texo_gp_session_destroy(s);
+  return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Continuous-batching server
+// ===========================================================================
+struct texo_gp_server {
+  struct Entry {
+    std::unique_ptr<Job> job;
+    std::chrono::steady_clock::time_point submitted;
+    bool done = false;
+    int status = 0;
+    std::vector<uint32_t> targets;
+    texo_gp_report report{};
+  };
+  std::unique_ptr<Engine> engine;
+  uint64_t budget = 0;
+  uint32_t max_batch = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<int64_t> pending;
+  std::unordered_map<int64_t, std::unique_ptr<Entry>> entries;
+  std::vector<int64_t> active;
+  int64_t next_id = 1;
+  bool stopping = false;
+  uint64_t hbm_in_use = 0;
+  texo_gp_server_stats stats{};
+  std::thread worker;
+
+  void loop();
+};
+
+void texo_gp_server::loop() {
+  (void)hipSetDevice(engine->device());
+  for (;;) {
+    std::vector<Job *> jobs;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] { return stopping || !pending.empty() || !active.empty(); });
+      if (stopping) return;
+      // admission in submission order, by HBM footprint and batch size
+      while (!pending.empty() && active.size() < max_batch) {
+        Entry &e = *entries[pending.front()];
+        const uint64_t need = e.job->device_bytes();
+        if (!active.empty() && hbm_in_use + need > budget) break;
+        const int64_t id = pending.front();
+        pending.pop_front();
+        lk.unlock();
+        bool ok = true;
+        try {
+          engine->attach(e.job.get());
+          e.job->begin_run(false);
+        } catch (const Failure &f) {
+          ok = false;
+          e.job->failure = f;
+          e.job->status = f.status;
+          e.job->phase = Phase::Failed;
+        }
+        lk.lock();
+        (void)ok;
+        active.push_back(id);
+        hbm_in_use += need;
+        stats.admitted += 1;
+      }
+      for (int64_t id : active) jobs.push_back(entries[id]->job.get());
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+      engine->round(jobs);
+    } catch (const Failure &f) {
+      for (Job *j : jobs) {
+        if (j->phase != Phase::Done && j->phase != Phase::Failed) {
+          j->failure = f;
+          j->status = f.status;
+          j->phase = Phase::Failed;
+        }
+      }
+    }
+    const double busy = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // retire finished jobs
+    std::unique_lock<std::mutex> lk(mu);
+    stats.rounds += 1;
+    stats.busy_seconds += busy;
+    std::vector<int64_t> still;
+    for (int64_t id : active) {
+      Entry &e = *entries[id];
+      Job &j = *e.job;
+      if (j.phase != Phase::Done && j.phase != Phase::Failed) {
+        still.push_back(id);
+        continue;
+      }
+      lk.unlock();
+      e.targets.assign(2 * j.U, 0);
+      try {
+        fill_report(*engine, j, &e.report);
+        if (j.phase == Phase::Done) copy_targets(*engine, j, e.targets.data());
+      } catch (const Failure &f) {
+        put_failure(&e.report, f);
+        j.status = f.status;
+        j.phase = Phase::Failed;
+      }
+      e.status = j.phase == Phase::Failed ? j.status : 0;
+      e.report.status = e.status;
+      e.report.elapsed_seconds =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
+      const uint64_t held = j.device_bytes();
+      engine->detach(&j);
+      e.job.reset();  // frees the job's HBM arena
+      lk.lock();
+      hbm_in_use -= std::min(hbm_in_use, held);
+      e.done = true;
+      if (e.status == 0)
+        stats.completed += 1;
+      else
+        stats.failed += 1;
+    }
+    active.swap(still);
+    cv_done.notify_all();
+  }
+}
+
+extern "C" {
+
+texo_gp_server *texo_gp_server_create(int32_t device, uint64_t hbm_budget_bytes, uint32_t max_batch) {
+  try {
+    std::unique_ptr<texo_gp_server> s(new texo_gp_server());
+    s->engine.reset(new Engine(device, 1024));
+    s->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)64 << 30;
+    s->max_batch = max_batch ? std::min<uint32_t>(max_batch, 1000) : 64;
+    s->worker = std::thread([p = s.get()] { p->loop(); });
+    return s.release();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+int64_t texo_gp_server_submit(texo_gp_server *s, const texo_gp_problem *problem, const texo_gp_options *options) {
+  if (!s) return -TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::unique_ptr<texo_gp_server::Entry> e(new texo_gp_server::Entry());
+  e->submitted = std::chrono::steady_clock::now();
+  e->job.reset(new Job());
+  try {
+    e->job->load(problem, options);  // deep copy + validation on the caller's thread
+  } catch (const Failure &f) {
+    return -(int64_t)f.status;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int64_t id = s->next_id++;
+  e->job->id = id;
+  s->entries[id] = std::move(e);
+  s->pending.push_back(id);
+  s->stats.submitted += 1;
+  s->cv_work.notify_all();
+  return id;
+}
+
+static int take_result(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_report *report) {
+  auto it = s->entries.find(id);
+  texo_gp_server::Entry &e = *it->second;
+  if (out) std::copy(e.targets.begin(), e.targets.end(), out);
+  if (report) *report = e.report;
+  const int status = e.status;
+  s->entries.erase(it);
+  return status;
+}
+
+int texo_gp_server_poll(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_report *report) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->entries.find(id);
+  if (it == s->entries.end()) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  if (!it->second->done) return TEXO_GP_ERR_NOT_READY;
+  return take_result(s, id, out, report);
+}
+
+int texo_gp_server_wait(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_report *report) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->entries.find(id);
+  if (it == s->entries.end()) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  s->cv_done.wait(lk, [&] {
+    auto e = s->entries.find(id);
+    return e == s->entries.end() || e->second->done;
+  });
+  // another waiter may have taken the result first
+  if (s->entries.find(id) == s->entries.end()) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  return take_result(s, id, out, report);
+}
+
+int texo_gp_server_stats_get(texo_gp_server *s, texo_gp_server_stats *stats) {
+  if (!s || !stats) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *stats = s->stats;
+  stats->active = s->active.size();
+  stats->pending = s->pending.size();
+  stats->hbm_in_use = s->hbm_in_use;
+  return 0;
+}
+
+void texo_gp_server_destroy(texo_gp_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->engine->device());
+  for (auto &kv : s->entries)
+    if (kv.second->job && kv.second->job->pool_index >= 0) s->engine->detach(kv.second->job.get());
+  s->entries.clear();
+  delete s;
+}
+
+}  // extern "C"

@@ -1,0 +1,915 @@
+// gp_engine.hip — lockstep round engine.  Every job is a state machine that
+// restates optimize_density_once (eplace.rs:439-582) around
+// dynamic_nesterov_step (global_placement.rs:121-335).  A round gathers, for
+// every job, the device work its next decision needs (preconditioner,
+// extrapolation, candidate step, one or two objective evaluations), issues it
+// as one launch per kernel family over all jobs, copies every job's FP64
+// scalars back with one transfer, synchronises once, and advances each job.
+//
+// Exactness of the reuse rules:
+//  * the density field depends only on coordinates and charges, so the origin
+//    re-evaluation of an iteration (new gamma and lambda) reuses the accepted
+//    candidate's field, and the initial evaluation reuses the probe's;
+//  * when the extrapolation factor is exactly 0, y == origin bit for bit and
+//    the reference reuses the origin evaluation (global_placement.rs:176-181).
+
+#include "gp_host.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <tuple>
+
+namespace texo {
+
+namespace {
+
+double half_nonzero(double v) {
+  const double h = v * 0.5;
+  return h == 0.0 ? v : h;
+}
+double rounding_margin(double a, double b) { return DBL_EPSILON * (1.0 + std::fabs(a) + std::fabs(b)); }
+double bits_to_double(unsigned long long b) {
+  double v;
+  std::memcpy(&v, &b, 8);
+  return v;
+}
+double kkt_of(const EvalScalars &s) { return bits_to_double(s.kkt_bits); }
+
+// checked_evaluation (global_placement.rs:562-586)
+void check_evaluation(const EvalScalars &s, double value) {
+  if (!std::isfinite(value)) fail(TEXO_GP_ERR_INVALID_OBJECTIVE);
+  if (s.nonfinite) fail(TEXO_GP_ERR_NON_FINITE_GRADIENT);
+}
+
+constexpr double INITIAL_DENSITY_WEIGHT = 1.0e-4;  // eplace.rs:126
+constexpr double AREA_RESET_DENSITY_WEIGHT = 0.1;  // eplace.rs:127
+constexpr double MULTIPLIER_ALPHA_HIGH = 1.06;     // eplace.rs:129
+constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
+
+__global__ void k_round_targets_raw(int n_units, const double *__restrict__ coords, int W, int H,
+                                    uint32_t *__restrict__ out) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= 2 * n_units) return;
+  const double extent = static_cast<double>(((i & 1) ? H : W) - 1);
+  double r = round(coords[i]);
+  if (!(r >= 0.0)) r = 0.0;
+  if (r > extent) r = extent;
+  out[i] = static_cast<uint32_t>(r);
+}
+
+}  // namespace
+
+void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s) {
+  if (n_units == 0) return;
+  hipLaunchKernelGGL(k_round_targets_raw, dim3((2 * n_units + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, n_units,
+                     coords, W, H, out);
+}
+
+// ---------------------------------------------------------------------------
+// Job run set-up
+// ---------------------------------------------------------------------------
+void Job::count_evaluation() {
+  evaluations += 1;
+  if (opt.inject && evaluations == opt.inject) fail(TEXO_GP_ERR_INJECTED);
+}
+
+void Job::begin_run(bool iterate_mode) {
+  started = true;
+  fixed_mode = iterate_mode || opt.fixed_iterations > 0;
+  no_stop = fixed_mode;
+  fixed_limit = (!iterate_mode && opt.fixed_iterations > 0) ? opt.fixed_iterations : UINT64_MAX;
+  has_prev_change = false;
+  prev_change = 0.0;
+  has_fixed_scales = false;
+  reset_after_area = false;
+  area_round = 0;
+  round_limit = empty ? 0 : area_round_limit();
+  begin_round();
+}
+
+void Job::begin_round() {
+  // place() (eplace.rs:357-368): AreaAdjustmentReady until the installed
+  // area change falls below one percent, then Final.
+  const bool adjust_area = has_routing && !fixed_mode;
+  area_ready = adjust_area && !(has_prev_change && prev_change < 0.01);
+  exhausted = false;
+  phase = empty ? Phase::Done : Phase::Probe;
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+Engine::Engine(int device, size_t pool_capacity) : device_(device) {
+  HIP(hipSetDevice(device));
+  // every stream keeps the default priority
+  HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIP(hipStreamCreateWithFlags(&wa_stream_, hipStreamNonBlocking));
+  HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+  HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+  HIP(hipMalloc(&d_pool_, pool_capacity * sizeof(JobScalars)));
+  HIP(hipHostMalloc(reinterpret_cast<void **>(&h_pool_), pool_capacity * sizeof(JobScalars), hipHostMallocDefault));
+  std::memset(h_pool_, 0, pool_capacity * sizeof(JobScalars));
+  HIP(hipMemset(d_pool_, 0, pool_capacity * sizeof(JobScalars)));
+  pool_used_.assign(pool_capacity, false);
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  (void)hipStreamSynchronize(stream_);
+  (void)hipStreamSynchronize(wa_stream_);
+  bases_.clear();
+  if (d_pool_) (void)hipFree(d_pool_);
+  if (h_pool_) (void)hipHostFree(h_pool_);
+  if (d_stage_) (void)hipFree(d_stage_);
+  if (h_stage_) (void)hipHostFree(h_stage_);
+  (void)hipEventDestroy(ev_fork_);
+  (void)hipEventDestroy(ev_join_);
+  (void)hipStreamDestroy(wa_stream_);
+  (void)hipStreamDestroy(stream_);
+}
+
+const Basis *Engine::basis(int W, int H) {
+  auto key = std::make_pair(W, H);
+  auto it = bases_.find(key);
+  if (it != bases_.end()) return it->second.get();
+  std::unique_ptr<Basis> b(new Basis());
+  b->build(W, H);
+  const Basis *r = b.get();
+  bases_[key] = std::move(b);
+  return r;
+}
+
+void Engine::attach(Job *job) {
+  size_t idx = 0;
+  while (idx < pool_used_.size() && pool_used_[idx]) ++idx;
+  if (idx == pool_used_.size()) fail(TEXO_GP_ERR_OUT_OF_MEMORY);
+  pool_used_[idx] = true;
+  job->pool_index = (int)idx;
+  job->d_scalars = d_pool_ + idx;
+  job->h_scalars = h_pool_ + idx;
+  if (!job->empty) job->upload(basis((int)job->W, (int)job->H), stream_);
+}
+
+void Engine::detach(Job *job) {
+  if (job->pool_index >= 0) pool_used_[job->pool_index] = false;
+  job->pool_index = -1;
+}
+
+void Engine::ensure_stage(size_t bytes) {
+  if (bytes <= stage_bytes_) return;
+  size_t cap = std::max<size_t>(bytes, 2 * stage_bytes_);
+  cap = (cap + 4095) & ~size_t(4095);
+  if (d_stage_) HIP(hipFree(d_stage_));
+  if (h_stage_) HIP(hipHostFree(h_stage_));
+  d_stage_ = nullptr;
+  h_stage_ = nullptr;
+  HIP(hipMalloc(reinterpret_cast<void **>(&d_stage_), cap));
+  HIP(hipHostMalloc(reinterpret_cast<void **>(&h_stage_), cap, hipHostMallocDefault));
+  stage_bytes_ = cap;
+}
+
+// Schedule decisions that need no device work run here, before a job's
+// requests for the next round are recorded.
+void Engine::plan(Job *j) {
+  j->evals.clear();
+  j->want_precond = j->want_extrapolate = j->want_candidate = false;
+  for (;;) {
+    switch (j->phase) {
+      case Phase::Probe: {
+        // density probe at gamma = base (eplace.rs:455)
+        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->opt.gamma_base, false, false, 0});
+        return;
+      }
+      case Phase::Init: {
+        // initial evaluation at the adaptive gamma (eplace.rs:457-458): same
+        // coordinates as the probe, whose density field slot 0 still holds
+        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->g0, true, false, 0});
+        return;
+      }
+      case Phase::IterStart: {
+        // loop head of optimize_density_once (eplace.rs:503-517)
+        if (j->no_stop) {
+          if (j->local_iterations >= j->fixed_limit) {
+            j->phase = Phase::RoundEnd;
+            continue;
+          }
+        } else {
+          if (j->stop_reached(j->current, j->area_ready)) {
+            j->phase = Phase::RoundEnd;
+            continue;
+          }
+          if (j->iterations >= j->opt.max_iterations) {
+            // stop_without_convergence (eplace.rs:989-999)
+            if (!j->best_effort) {
+              Failure f;
+              f.status = TEXO_GP_ERR_DID_NOT_CONVERGE;
+              f.index = (int64_t)j->iterations;
+              f.v0 = j->F > 0 ? j->overflow(j->current, 0) : 0.0;
+              f.v1 = j->F > 1 ? j->overflow(j->current, 1) : 0.0;
+              throw f;
+            }
+            j->exhausted = true;
+            j->phase = Phase::RoundEnd;
+            continue;
+          }
+        }
+        if (j->iterations >= j->pause_at) {
+          j->phase = Phase::Paused;
+          return;
+        }
+        j->gamma = j->adaptive_gamma(j->current);
+        j->iterations += 1;
+        j->local_iterations += 1;
+        if (!std::isfinite(j->momentum) || j->momentum < 1.0) fail(TEXO_GP_ERR_INVALID_MOMENTUM);
+        if (j->has_lipschitz && (!std::isfinite(j->lipschitz) || j->lipschitz <= 0.0))
+          fail(TEXO_GP_ERR_INVALID_LIPSCHITZ);
+        for (int f = 0; f < j->F; ++f)
+          if (!std::isfinite(j->mult[f]) || j->mult[f] < 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+        j->restart = false;
+        j->has_wl = j->has_lipschitz;
+        j->wl = j->lipschitz;
+        const double m = j->momentum;
+        j->next_momentum = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * m * m));
+        if (!std::isfinite(j->next_momentum)) fail(TEXO_GP_ERR_ITERATION_OVERFLOW);
+        j->beta = (m - 1.0) / j->next_momentum;
+        j->want_precond = true;
+        const int p = j->p_slot, q = 1 - p;
+        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, j->p_valid, false, p});
+        j->y_evaluated = j->beta != 0.0;
+        if (j->y_evaluated) {
+          j->want_extrapolate = true;
+          j->evals.push_back({q, j->d_vec[j->iy], j->d_grad[1], true, j->gamma, false, false, q});
+        }
+        return;
+      }
+      case Phase::Trial: {
+        // global_placement.rs:213-240
+        const double mmax = bits_to_double(j->h_scalars->metric.max_bits);
+        const double mmin = bits_to_double(j->h_scalars->metric.min_bits);
+        for (;;) {
+          j->trials += 1;
+          if (!std::isfinite(j->L * mmax) || !std::isfinite(j->L * mmin)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+          if (j->L * mmin == 0.0) {
+            j->L *= 2.0;
+            j->may_reduce = false;
+            if (!std::isfinite(j->L)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+            continue;
+          }
+          break;
+        }
+        j->want_candidate = true;
+        const int q = 1 - j->p_slot;
+        j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, q});
+        return;
+      }
+      case Phase::RoundEnd: {
+        // place() after optimize_density_once (eplace.rs:384-419)
+        if (!j->area_ready || j->exhausted) {
+          j->phase = Phase::Done;
+          return;
+        }
+        area_round(j);
+        continue;
+      }
+      default:
+        return;
+    }
+  }
+}
+
+// propose_area_adjustment / apply_area_adjustment (eplace.rs:627-819): RUDY
+// on the device, the order-dependent charge bookkeeping on the host.
+void Engine::area_round(Job *j) {
+  HIP(hipMemsetAsync(j->d_rudy_err, 0, sizeof(unsigned int), stream_));
+  launch_rudy((int)j->n_nets, j->d_net_start, j->d_pin_unit, j->d_pin_xy, j->d_vec[j->ix], (int)j->W, (int)j->H,
+              j->d_rudy_fx, j->d_rudy, j->d_rudy_err, stream_);
+  const size_t G = j->G, U = j->U;
+  std::vector<double> demand(2 * G), x(2 * U);
+  unsigned int err = 0;
+  HIP(hipMemcpyAsync(demand.data(), j->d_rudy, 2 * G * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP(hipMemcpyAsync(x.data(), j->d_vec[j->ix], 2 * U * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP(hipMemcpyAsync(&err, j->d_rudy_err, sizeof(unsigned int), hipMemcpyDeviceToHost, stream_));
+  HIP(hipStreamSynchronize(stream_));
+  if (err) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  std::map<uint32_t, double> clustering;
+  if (!j->controls.empty()) {
+    std::vector<std::pair<uint32_t, std::pair<double, double>>> regs;
+    for (size_t i = 0; i < j->DU; ++i) {
+      const uint32_t u = j->du_unit[i];
+      for (uint32_t m = j->du_mstart[i]; m < j->du_mstart[i + 1]; ++m)
+        if (j->field_kind[j->m_field[m]] == TEXO_KIND_REGISTER)
+          regs.push_back({j->m_cell[m], {x[2 * u] + j->m_off[2 * m], x[2 * u + 1] + j->m_off[2 * m + 1]}});
+    }
+    std::vector<double> areas;
+    if (!register_clustering(regs, j->controls, j->movable_instance_count, areas))
+      fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    for (size_t i = 0; i < regs.size(); ++i) clustering[regs[i].first] = areas[i];
+  }
+  // ContinuousRoutingDemand::utilization (eplace.rs:822-854)
+  auto utilization = [&](const double *d, const std::vector<uint32_t> &cap, uint32_t px, uint32_t py) {
+    const size_t idx = (size_t)py * j->W + px;
+    if (cap[idx] == 0) return d[idx] == 0.0 ? 0.0 : kSqrt2;
+    const double u = d[idx] / (double)cap[idx];
+    if (!(std::isfinite(u) && u >= 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    return u;
+  };
+  std::vector<AreaMember> members(j->M);
+  for (size_t i = 0; i < j->DU; ++i) {
+    const uint32_t u = j->du_unit[i];
+    for (uint32_t m = j->du_mstart[i]; m < j->du_mstart[i + 1]; ++m) {
+      const uint32_t px = rounded_coordinate(x[2 * u] + j->m_off[2 * m], j->W);
+      const uint32_t py = rounded_coordinate(x[2 * u + 1] + j->m_off[2 * m + 1], j->H);
+      const double h = std::min(utilization(demand.data(), j->routing_h, px, py), kSqrt2);
+      const double v = std::min(utilization(demand.data() + G, j->routing_v, px, py), kSqrt2);
+      const uint32_t kind = j->field_kind[j->m_field[m]];
+      // routability_adjusted_member_area (eplace.rs:741-758, instance_area.rs:64-84)
+      double area = j->m_charge[m];
+      if (is_lut(kind) || kind == TEXO_KIND_REGISTER) {
+        const double uu = std::max(h, v);
+        area = j->m_charge[m] * std::min(uu * uu, 2.0);
+      }
+      auto it = clustering.find(j->m_cell[m]);
+      members[m] = {kind, j->m_charge[m], area, it == clustering.end() ? 0.0 : it->second};
+    }
+  }
+  std::vector<std::pair<uint32_t, double>> fillers(j->Fl);
+  for (size_t k = 0; k < j->Fl; ++k) fillers[k] = {j->field_kind[j->fl_field[k]], j->fl_charge[k]};
+  std::vector<double> ma, fa;
+  const bool requires_reset = adjust_areas(members, fillers, ma, fa);
+  std::map<uint32_t, double> before, inc;
+  for (size_t m = 0; m < j->M; ++m) {
+    before[members[m].kind] += members[m].current;
+    inc[members[m].kind] += ma[m] - members[m].current;
+  }
+  double bt = 0.0, it = 0.0;
+  for (auto &kv : before) bt += kv.second;
+  for (auto &kv : inc) it += kv.second;
+  const double total_inc = it / bt;
+  if (!std::isfinite(total_inc)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  if (!requires_reset) {
+    j->has_prev_change = true;
+    j->prev_change = 0.0;
+    j->reset_after_area = true;
+    j->begin_round();
+    return;
+  }
+  if (j->area_round >= j->round_limit) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  j->apply_charges(ma, fa);
+  j->p_valid = false;
+  j->has_prev_change = true;
+  j->prev_change = total_inc;
+  j->reset_after_area = true;
+  j->area_round += 1;
+  j->area_rounds += 1;
+  j->begin_round();
+}
+
+namespace {
+
+// Restart from the origin (global_placement.rs:291-301): momentum 1, y = origin.
+void restart_from_origin(Job *j) {
+  j->restart = true;
+  j->next_momentum = 0.5 * (1.0 + std::sqrt(1.0 + 4.0));
+  j->y_is_origin = true;
+  j->y_s = j->origin_s;
+  j->y_value = j->origin_value;
+}
+
+void consider_best(Job *j, hipStream_t s) {
+  // PlacementCheckpoint::consider (eplace.rs:107-120): lexicographic total_cmp
+  const auto sc = j->excess(j->current);
+  const bool better = !j->has_best || sc.first < j->best_max || (sc.first == j->best_max && sc.second < j->best_total);
+  if (better) {
+    launch_round_targets((int)j->U, j->d_vec[j->ix], (int)j->W, (int)j->H, j->d_best, s);
+    j->best_max = sc.first;
+    j->best_total = sc.second;
+    j->has_best = true;
+  }
+}
+
+void iteration_end(Job *j, bool accepted, hipStream_t s) {
+  // eplace.rs:530-573
+  j->last_wa = j->current.wa_value;
+  j->validate_fields(j->current);
+  consider_best(j, s);
+  if (!j->no_stop) {
+    if (j->stop_reached(j->current, j->area_ready)) {
+      j->phase = Phase::RoundEnd;
+      return;
+    }
+    // stationary_target_is_fixed (eplace.rs:973-985)
+    if (accepted) {
+      j->has_prev_stationary = false;
+    } else {
+      std::vector<double> x(2 * j->U);
+      HIP(hipMemcpyAsync(x.data(), j->d_vec[j->ix], 2 * j->U * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP(hipStreamSynchronize(s));
+      std::vector<uint32_t> t(2 * j->U);
+      for (size_t i = 0; i < 2 * j->U; ++i) t[i] = rounded_coordinate(x[i], (i & 1) ? j->H : j->W);
+      const bool fixed = j->has_prev_stationary && j->prev_stationary == t;
+      j->prev_stationary = t;
+      j->has_prev_stationary = true;
+      if (fixed) {
+        if (!j->best_effort) {
+          Failure f;
+          f.status = TEXO_GP_ERR_DID_NOT_CONVERGE;
+          f.index = (int64_t)j->iterations;
+          throw f;
+        }
+        j->exhausted = true;
+        j->phase = Phase::RoundEnd;
+        return;
+      }
+    }
+  }
+  j->update_multipliers();
+  j->phase = Phase::IterStart;
+}
+
+void stationary(Job *j, bool set_l, double l, hipStream_t s) {
+  // NumericallyStationary return (global_placement.rs:185-208, 259-271)
+  HIP(hipMemcpyAsync(j->d_vec[j->ixp], j->d_vec[j->ix], j->C * sizeof(double), hipMemcpyDeviceToDevice, s));
+  j->momentum = 1.0;
+  if (set_l) {
+    j->has_lipschitz = true;
+    j->lipschitz = l;
+  }
+  j->current = j->origin_s;
+  iteration_end(j, false, s);
+}
+
+// y-stationarity decision after y is known (global_placement.rs:182-211).
+void decide_y(Job *j, hipStream_t s) {
+  for (;;) {
+    const double y_stat = kkt_of(j->y_s);
+    if (!std::isfinite(y_stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+    if (y_stat == 0.0) {
+      if (!j->y_is_origin) {
+        restart_from_origin(j);
+        continue;
+      }
+      stationary(j, false, 0.0, s);
+      return;
+    }
+    j->L = half_nonzero(j->has_wl ? j->wl : y_stat);
+    j->may_reduce = true;
+    j->phase = Phase::Trial;
+    return;
+  }
+}
+
+}  // namespace
+
+void Engine::advance(Job *j) {
+  const JobScalars &hs = *j->h_scalars;
+  switch (j->phase) {
+    case Phase::Probe: {
+      j->count_evaluation();
+      const EvalScalars probe = hs.eval[0];
+      j->validate_initial(probe);
+      j->g0 = j->adaptive_gamma(probe);
+      j->phase = Phase::Init;
+      return;
+    }
+    case Phase::Init: {
+      // eplace.rs:459-499
+      j->count_evaluation();
+      const EvalScalars init = hs.eval[0];
+      j->validate_initial(init);
+      if (j->has_fixed_scales) {
+        j->scales = j->fixed_scales;
+      } else {
+        j->scales.assign(j->F, 1.0);
+        for (int f = 0; f < j->F; ++f) j->scales[f] = std::max(init.energy[f], 1.0);
+        j->fixed_scales = j->scales;
+        j->has_fixed_scales = true;
+      }
+      j->validate_fields(init);
+      // DynamicNesterovState::new: previous = coordinates, momentum 1, no L
+      HIP(hipMemcpyAsync(j->d_vec[j->ixp], j->d_vec[j->ix], j->C * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+      j->momentum = 1.0;
+      j->has_lipschitz = false;
+      j->lipschitz = 0.0;
+      j->has_best = false;
+      j->current = init;
+      j->last_wa = init.wa_value;
+      consider_best(j, stream_);
+      j->has_prev_stationary = false;
+      j->p_slot = 0;
+      j->p_valid = true;
+      j->local_iterations = 0;
+      if (!j->no_stop && j->stop_reached(init, j->area_ready)) {
+        j->phase = Phase::RoundEnd;
+        return;
+      }
+      if (j->reset_after_area) {
+        // area_adjusted_density_multipliers / _step (eplace.rs:1183-1232)
+        const std::vector<double> s = j->subgradient(init);
+        const double wf = init.wa_l1;
+        double df = 0.0;
+        for (int f = 0; f < j->F; ++f) df += init.force_l1[f] * s[f];
+        if (!std::isfinite(wf) || wf <= 0.0 || !std::isfinite(df) || df <= 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+        const double scale = AREA_RESET_DENSITY_WEIGHT * wf / df;
+        bool any = false;
+        double norm = 0.0;
+        j->mult.assign(j->F, 0.0);
+        for (int f = 0; f < j->F; ++f) {
+          j->mult[f] = scale * s[f];
+          if (!(std::isfinite(j->mult[f]) && j->mult[f] >= 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+          any = any || j->mult[f] > 0.0;
+          norm += j->mult[f] * j->mult[f];
+        }
+        if (!any) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+        j->mstep = (MULTIPLIER_ALPHA_HIGH - 1.0) * std::sqrt(norm);
+        if (!(std::isfinite(j->mstep) && j->mstep > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+      } else {
+        // initial_density_multiplier (eplace.rs:1126-1148)
+        const double wf = init.wa_l1;
+        double df = 0.0;
+        for (int f = 0; f < j->F; ++f) df += init.raw_l1[f];
+        const double m0 = (wf == 0.0 || df == 0.0) ? 1.0 : INITIAL_DENSITY_WEIGHT * wf / df;
+        if (!(std::isfinite(m0) && m0 > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+        j->mult.assign(j->F, m0);
+        j->mstep = MULTIPLIER_ALPHA_HIGH - 1.0;
+      }
+      j->phase = Phase::IterStart;
+      return;
+    }
+    case Phase::IterStart: {
+      // origin (and y) evaluated (global_placement.rs:143-182)
+      if (hs.metric.nonfinite) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+      j->count_evaluation();
+      const int p = j->p_slot, q = 1 - p;
+      j->origin_s = hs.eval[p];
+      j->origin_value = j->combined_value(j->origin_s);
+      check_evaluation(j->origin_s, j->origin_value);
+      j->p_valid = true;
+      j->y_is_origin = true;
+      if (j->y_evaluated && hs.step.differs) {
+        j->y_is_origin = false;
+        j->count_evaluation();
+        j->y_s = hs.eval[q];
+        j->y_value = j->combined_value(j->y_s);
+        check_evaluation(j->y_s, j->y_value);
+      }
+      if (j->y_is_origin) {
+        j->y_s = j->origin_s;
+        j->y_value = j->origin_value;
+      }
+      decide_y(j, stream_);
+      return;
+    }
+    case Phase::Trial: {
+      const StepScalars st = hs.step;
+      if (!st.changed) {
+        // candidate == y (global_placement.rs:241-272)
+        if (j->may_reduce) {
+          const double smaller = half_nonzero(j->L);
+          if (smaller < j->L) {
+            j->L = smaller;
+            return;
+          }
+        }
+        if (!j->y_is_origin) {
+          j->has_wl = true;
+          j->wl = j->L;
+          restart_from_origin(j);
+          decide_y(j, stream_);
+          return;
+        }
+        stationary(j, true, j->L, stream_);
+        return;
+      }
+      j->count_evaluation();
+      const int q = 1 - j->p_slot;
+      const EvalScalars ce = hs.eval[q];
+      const double value = j->combined_value(ce);
+      check_evaluation(ce, value);
+      // majorizer and monotonicity (global_placement.rs:274-290)
+      const double upper = j->y_value + st.gdelta + 0.5 * j->L * st.msq;
+      if (!std::isfinite(upper)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+      const bool smooth = value <= upper + rounding_margin(value, upper);
+      const bool monotone = value <= j->origin_value + rounding_margin(value, j->origin_value);
+      if (smooth && monotone) {
+        const double stat = kkt_of(ce);
+        if (!std::isfinite(stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+        // previous <- origin, coordinates <- candidate (global_placement.rs:322-325)
+        const int old_prev = j->ixp;
+        j->ixp = j->ix;
+        j->ix = j->ic;
+        j->ic = old_prev;
+        j->momentum = j->next_momentum;
+        j->has_lipschitz = true;
+        j->lipschitz = j->L;
+        j->current = ce;
+        j->p_slot = q;
+        j->p_valid = true;
+        if (j->opt.trace)
+          std::fprintf(stderr,
+                       "TEXO_PNR_METRICS eplace iteration=%llu objective=%.9e stationarity=%.9e "
+                       "coordinate_change=%.9e line_search_trials=%llu gamma=%.9e wa=%.9e\n",
+                       (unsigned long long)j->iterations, value, stat, bits_to_double(st.change_bits),
+                       (unsigned long long)j->trials, j->gamma, ce.wa_value);
+        iteration_end(j, true, stream_);
+        return;
+      }
+      if (smooth && !j->restart) {
+        j->has_wl = true;
+        j->wl = j->L;
+        restart_from_origin(j);
+        decide_y(j, stream_);
+        return;
+      }
+      j->L *= 2.0;
+      j->may_reduce = false;
+      if (!std::isfinite(j->L)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+      return;
+    }
+    default:
+      return;
+  }
+}
+
+namespace {
+
+template <class T>
+size_t append(std::vector<char> &buf, const std::vector<T> &v) {
+  size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + v.size() * sizeof(T));
+  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
+}
+
+void add_blocks(std::vector<int2> &map, int task, size_t n, size_t per_block) {
+  const int nb = (int)((n + per_block - 1) / per_block);
+  for (int b = 0; b < nb; ++b) map.push_back(make_int2(task, b));
+}
+
+}  // namespace
+
+void Engine::round(const std::vector<Job *> &jobs) {
+  HIP(hipSetDevice(device_));
+  g_timer = &timer;
+  std::vector<Job *> active;
+  for (Job *j : jobs) {
+    if (j->phase == Phase::Done || j->phase == Phase::Failed || j->phase == Phase::Paused) continue;
+    try {
+      plan(j);
+      if (!j->evals.empty()) active.push_back(j);
+    } catch (const Failure &f) {
+      j->failure = f;
+      j->status = f.status;
+      j->phase = Phase::Failed;
+    }
+  }
+  if (active.empty()) {
+    g_timer = nullptr;
+    return;
+  }
+
+  // ---- tables ----
+  std::vector<OptTask> opt;
+  std::vector<int2> m_pre, m_ext, m_cand;
+  std::vector<int> l_pre, l_cand;
+  std::vector<Task> tasks;
+  std::vector<int2> m_dep, m_conv, m_gu, m_gf, m_wa, m_ug;
+  struct Group {
+    std::vector<int2> h, w;
+  };
+  std::map<std::tuple<int, int, int>, Group> groups;
+  for (Job *j : active) {
+    if (j->want_precond || j->want_extrapolate || j->want_candidate) {
+      OptTask o{};
+      o.n = j->C;
+      o.n_units = (int)j->U;
+      o.n_fillers = (int)j->Fl;
+      o.F = j->F;
+      o.DU = (int)j->DU;
+      o.lower = j->d_lower;
+      o.upper = j->d_upper;
+      o.wl_diag = j->d_wl_diag;
+      o.du_fcharge = j->d_du_fcharge;
+      o.fl_charge = j->d_fl_charge;
+      o.unit_du = j->d_unit_du;
+      o.fl_field = j->d_fl_field;
+      for (int f = 0; f < MAXF; ++f) o.lambda[f] = f < j->F ? j->mult[f] : 0.0;
+      o.metric = j->d_metric;
+      o.part_metric = j->d_part_metric;
+      o.metric_scalars = &j->d_scalars->metric;
+      o.x = j->d_vec[j->ix];
+      o.xp = j->d_vec[j->ixp];
+      o.y = j->d_vec[j->iy];
+      o.beta = j->beta;
+      o.yp = j->y_is_origin ? j->d_vec[j->ix] : j->d_vec[j->iy];
+      o.gy = j->y_is_origin ? j->d_grad[0] : j->d_grad[1];
+      o.origin = j->d_vec[j->ix];
+      o.cand = j->d_vec[j->ic];
+      o.lipschitz = j->L;
+      o.part_step = j->d_part_step;
+      o.step = &j->d_scalars->step;
+      o.W = (int)j->W;
+      o.H = (int)j->H;
+      const int oi = (int)opt.size();
+      opt.push_back(o);
+      if (j->want_precond) {
+        add_blocks(m_pre, oi, j->U + j->Fl, BLOCK);
+        l_pre.push_back(oi);
+      }
+      if (j->want_extrapolate) add_blocks(m_ext, oi, j->C, BLOCK);
+      if (j->want_candidate) {
+        add_blocks(m_cand, oi, j->C, BLOCK);
+        l_cand.push_back(oi);
+      }
+    }
+    for (const Job::EvalReq &r : j->evals) {
+      Task t = j->task(r.slot, r.coords, r.grad, r.metric ? j->d_metric : nullptr, r.gamma, r.skip,
+                       &j->d_scalars->eval[r.scalar_slot]);
+      if (r.parity) {
+        t.raw_units = j->d_raw_units;
+        t.raw_fillers = j->d_raw_fillers;
+        t.wa_grad_out = j->d_wa_grad;
+      }
+      const int ti = (int)tasks.size();
+      tasks.push_back(t);
+      if (j->F > 0 && !r.skip) {
+        add_blocks(m_dep, ti, j->M + j->Fl, BLOCK);
+        add_blocks(m_conv, ti, (size_t)j->F * j->G, BLOCK);
+        Group &g = groups[std::make_tuple((int)j->W, (int)j->H, (int)j->opt.precision)];
+        add_blocks(g.h, ti, (size_t)j->F * j->H, 32);
+        add_blocks(g.w, ti, (size_t)j->F * j->W, 32);
+      }
+      if (j->F > 0) {
+        for (int b = 0; b < j->nb_gu; ++b) m_gu.push_back(make_int2(ti, b));
+        if (j->Fl > 0)
+          for (int b = 0; b < j->nb_gf; ++b) m_gf.push_back(make_int2(ti, b));
+      }
+      for (int b = 0; b < j->nb_wa_large + j->nb_wa_small; ++b) m_wa.push_back(make_int2(ti, b));
+      for (int b = 0; b < j->nb_ug; ++b) m_ug.push_back(make_int2(ti, b));
+    }
+  }
+  std::vector<char> buf;
+  const size_t o_opt = append(buf, opt), o_tasks = append(buf, tasks);
+  const size_t o_pre = append(buf, m_pre), o_ext = append(buf, m_ext), o_cand = append(buf, m_cand);
+  const size_t o_lpre = append(buf, l_pre), o_lcand = append(buf, l_cand);
+  const size_t o_dep = append(buf, m_dep), o_conv = append(buf, m_conv), o_gu = append(buf, m_gu),
+               o_gf = append(buf, m_gf), o_wa = append(buf, m_wa), o_ug = append(buf, m_ug);
+  std::vector<std::pair<size_t, size_t>> o_groups;
+  for (auto &kv : groups) o_groups.push_back({append(buf, kv.second.h), append(buf, kv.second.w)});
+  ensure_stage(buf.size());
+  std::memcpy(h_stage_, buf.data(), buf.size());
+  HIP(hipMemcpyAsync(d_stage_, h_stage_, buf.size(), hipMemcpyHostToDevice, stream_));
+  auto dptr = [&](size_t off) { return d_stage_ + off; };
+  const Task *d_tasks = reinterpret_cast<const Task *>(dptr(o_tasks));
+
+  // ---- launches ----
+  OptMaps om{};
+  om.precond = reinterpret_cast<const int2 *>(dptr(o_pre));
+  om.extrapolate = reinterpret_cast<const int2 *>(dptr(o_ext));
+  om.candidate = reinterpret_cast<const int2 *>(dptr(o_cand));
+  om.targets = nullptr;
+  om.nb_precond = (int)m_pre.size();
+  om.nb_extrapolate = (int)m_ext.size();
+  om.nb_candidate = (int)m_cand.size();
+  om.nb_targets = 0;
+  om.precond_list = reinterpret_cast<const int *>(dptr(o_lpre));
+  om.candidate_list = reinterpret_cast<const int *>(dptr(o_lcand));
+  om.n_precond = (int)l_pre.size();
+  om.n_candidate = (int)l_cand.size();
+  if (!opt.empty()) launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, stream_);
+
+  Maps mp{};
+  mp.deposit = reinterpret_cast<const int2 *>(dptr(o_dep));
+  mp.convert = reinterpret_cast<const int2 *>(dptr(o_conv));
+  mp.gather_units = reinterpret_cast<const int2 *>(dptr(o_gu));
+  mp.gather_fillers = reinterpret_cast<const int2 *>(dptr(o_gf));
+  mp.wa = reinterpret_cast<const int2 *>(dptr(o_wa));
+  mp.unit_grad = reinterpret_cast<const int2 *>(dptr(o_ug));
+  mp.nb_deposit = (int)m_dep.size();
+  mp.nb_convert = (int)m_conv.size();
+  mp.nb_gather_units = (int)m_gu.size();
+  mp.nb_gather_fillers = (int)m_gf.size();
+  mp.nb_wa = (int)m_wa.size();
+  mp.nb_unit_grad = (int)m_ug.size();
+  mp.n_tasks = (int)tasks.size();
+  // WA depends only on coordinates: it runs on the second stream beside the
+  // density chain and joins before the combination.
+  HIP(hipEventRecord(ev_fork_, stream_));
+  HIP(hipStreamWaitEvent(wa_stream_, ev_fork_, 0));
+  launch_wa(d_tasks, mp, wa_stream_);
+  HIP(hipEventRecord(ev_join_, wa_stream_));
+  launch_density(d_tasks, mp, stream_);
+  size_t gi = 0;
+  for (auto &kv : groups) {
+    DctLaunch d{};
+    d.map_h = reinterpret_cast<const int2 *>(dptr(o_groups[gi].first));
+    d.map_w = reinterpret_cast<const int2 *>(dptr(o_groups[gi].second));
+    d.row_blocks_h = (int)kv.second.h.size();
+    d.row_blocks_w = (int)kv.second.w.size();
+    d.W = std::get<0>(kv.first);
+    d.H = std::get<1>(kv.first);
+    d.precision = std::get<2>(kv.first);
+    launch_poisson(d_tasks, d, stream_);
+    ++gi;
+  }
+  launch_gather(d_tasks, mp, stream_);
+  HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
+  launch_combine(d_tasks, mp, stream_);
+  HIP(hipGetLastError());
+  int hi = 0;
+  for (Job *j : active) hi = std::max(hi, j->pool_index);
+  HIP(hipMemcpyAsync(h_pool_, d_pool_, (size_t)(hi + 1) * sizeof(JobScalars), hipMemcpyDeviceToHost, stream_));
+  HIP(hipStreamSynchronize(stream_));
+
+  for (Job *j : active) {
+    try {
+      advance(j);
+    } catch (const Failure &f) {
+      j->failure = f;
+      j->status = f.status;
+      j->phase = Phase::Failed;
+    }
+  }
+  g_timer = nullptr;
+}
+
+void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_out) {
+  HIP(hipSetDevice(device_));
+  HIP(hipMemcpyAsync(j->d_eval_coords, coords, j->C * sizeof(double), hipMemcpyHostToDevice, stream_));
+  // slot 0 is overwritten: the optimizer's cached field there is stale now
+  if (j->p_slot == 0) j->p_valid = false;
+  const Phase saved = j->phase;
+  j->evals.clear();
+  j->evals.push_back({0, j->d_eval_coords, j->d_grad[0], false, g, false, true, 2});
+  j->want_precond = j->want_extrapolate = j->want_candidate = false;
+  std::vector<Job *> one{j};
+  // run exactly the planned request: mark the phase so plan() leaves it alone
+  j->phase = Phase::Paused;
+  std::vector<Task> tasks;
+  Task t = j->task(0, j->d_eval_coords, j->d_grad[0], nullptr, g, false, &j->d_scalars->eval[2]);
+  t.raw_units = j->d_raw_units;
+  t.raw_fillers = j->d_raw_fillers;
+  t.wa_grad_out = j->d_wa_grad;
+  if (density_out) t.density_out = j->d_density_out;
+  tasks.push_back(t);
+  std::vector<int2> m_dep, m_conv, m_gu, m_gf, m_wa, m_ug, gh, gw;
+  if (j->F > 0) {
+    add_blocks(m_dep, 0, j->M + j->Fl, BLOCK);
+    add_blocks(m_conv, 0, (size_t)j->F * j->G, BLOCK);
+    add_blocks(gh, 0, (size_t)j->F * j->H, 32);
+    add_blocks(gw, 0, (size_t)j->F * j->W, 32);
+    for (int b = 0; b < j->nb_gu; ++b) m_gu.push_back(make_int2(0, b));
+    if (j->Fl > 0)
+      for (int b = 0; b < j->nb_gf; ++b) m_gf.push_back(make_int2(0, b));
+  }
+  for (int b = 0; b < j->nb_wa_large + j->nb_wa_small; ++b) m_wa.push_back(make_int2(0, b));
+  for (int b = 0; b < j->nb_ug; ++b) m_ug.push_back(make_int2(0, b));
+  std::vector<char> buf;
+  const size_t o_tasks = append(buf, tasks), o_dep = append(buf, m_dep), o_conv = append(buf, m_conv),
+               o_gu = append(buf, m_gu), o_gf = append(buf, m_gf), o_wa = append(buf, m_wa), o_ug = append(buf, m_ug),
+               o_h = append(buf, gh), o_w = append(buf, gw);
+  ensure_stage(buf.size());
+  std::memcpy(h_stage_, buf.data(), buf.size());
+  HIP(hipMemcpyAsync(d_stage_, h_stage_, buf.size(), hipMemcpyHostToDevice, stream_));
+  const Task *d_tasks = reinterpret_cast<const Task *>(d_stage_ + o_tasks);
+  Maps mp{};
+  mp.deposit = reinterpret_cast<const int2 *>(d_stage_ + o_dep);
+  mp.convert = reinterpret_cast<const int2 *>(d_stage_ + o_conv);
+  mp.gather_units = reinterpret_cast<const int2 *>(d_stage_ + o_gu);
+  mp.gather_fillers = reinterpret_cast<const int2 *>(d_stage_ + o_gf);
+  mp.wa = reinterpret_cast<const int2 *>(d_stage_ + o_wa);
+  mp.unit_grad = reinterpret_cast<const int2 *>(d_stage_ + o_ug);
+  mp.nb_deposit = (int)m_dep.size();
+  mp.nb_convert = (int)m_conv.size();
+  mp.nb_gather_units = (int)m_gu.size();
+  mp.nb_gather_fillers = (int)m_gf.size();
+  mp.nb_wa = (int)m_wa.size();
+  mp.nb_unit_grad = (int)m_ug.size();
+  mp.n_tasks = 1;
+  g_timer = &timer;
+  launch_wa(d_tasks, mp, stream_);
+  launch_density(d_tasks, mp, stream_);
+  if (j->F > 0) {
+    DctLaunch d{};
+    d.map_h = reinterpret_cast<const int2 *>(d_stage_ + o_h);
+    d.map_w = reinterpret_cast<const int2 *>(d_stage_ + o_w);
+    d.row_blocks_h = (int)gh.size();
+    d.row_blocks_w = (int)gw.size();
+    d.W = (int)j->W;
+    d.H = (int)j->H;
+    d.precision = (int)j->opt.precision;
+    launch_poisson(d_tasks, d, stream_);
+  }
+  launch_gather(d_tasks, mp, stream_);
+  launch_combine(d_tasks, mp, stream_);
+  g_timer = nullptr;
+  HIP(hipGetLastError());
+  HIP(hipMemcpyAsync(h_pool_ + j->pool_index, d_pool_ + j->pool_index, sizeof(JobScalars), hipMemcpyDeviceToHost,
+                     stream_));
+  HIP(hipStreamSynchronize(stream_));
+  j->phase = saved;
+  j->evals.clear();
+}
+
+}  // namespace texo

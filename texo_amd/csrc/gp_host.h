@@ -1,0 +1,275 @@
+// gp_host.h — host-side types of the placement engine: jobs (one placement
+// problem each, with its device arena and optimizer state), the transform
+// bases shared per grid size, and the lockstep round engine.
+#pragma once
+
+#include "gp_common.h"
+#include "../../include/texo_gp.h"
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace texo {
+
+extern thread_local std::string g_last_error;
+
+// Typed failure carried to the ABI boundary (status + EplaceError payload).
+struct Failure {
+  int status;
+  int64_t index = -1, index2 = -1;
+  uint32_t kind = 0, x = 0, y = 0;
+  double v0 = 0.0, v1 = 0.0;
+};
+[[noreturn]] void fail(int status, int64_t index = -1, int64_t index2 = -1);
+void hip_check(hipError_t e, const char *what);
+#define HIP(x) ::texo::hip_check((x), #x)
+
+bool is_lut(uint32_t kind);
+double overflow_target(uint32_t kind);
+uint32_t rounded_coordinate(double v, uint32_t extent);
+double clampd(double v, double lo, double hi);
+
+struct KernelTimer {
+  std::string name;
+  bool enabled = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t used = 0;
+  ~KernelTimer();
+};
+
+// One allocation carved into 256-byte aligned buffers.
+class Arena {
+ public:
+  ~Arena();
+  template <class T>
+  void reserve(T *&p, size_t count) {
+    plan_.push_back({reinterpret_cast<void **>(&p), std::max<size_t>(count, 1) * sizeof(T)});
+  }
+  size_t commit();
+  size_t bytes() const { return bytes_; }
+
+ private:
+  struct Entry {
+    void **slot;
+    size_t bytes;
+  };
+  std::vector<Entry> plan_;
+  void *base_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+struct Options {
+  uint32_t precision = TEXO_GP_DCT_FP32;
+  uint32_t fixed_iterations = 0;
+  uint64_t max_iterations = 3000;  // eplace.rs:134
+  double gamma_base = 8.0;         // eplace.rs:133
+  bool trace = false;
+  int device = 0;
+  uint32_t inject = 0;
+};
+
+// DCT bases and 1/lambda for one (W, H, precision), shared by jobs.
+struct Basis {
+  int W = 0, H = 0;
+  Arena arena;
+  float *b32[4] = {};
+  uint16_t *bthi[4] = {}, *btlo[4] = {};
+  float *inv_lambda = nullptr;
+  void build(int W, int H);
+};
+
+struct Control {
+  uint64_t clock, lsr, ce;
+};
+
+// Scalars of one job read back by the host after every round.
+struct JobScalars {
+  EvalScalars eval[3];  // working slot 0, working slot 1, parity evaluation
+  StepScalars step;
+  MetricScalars metric;
+};
+
+struct AreaMember {
+  uint32_t kind;
+  double current, routability, clustering;
+};
+// clustering_areas (register_clustering.rs:17-81); false is the reference's None.
+bool register_clustering(const std::vector<std::pair<uint32_t, std::pair<double, double>>> &regs,
+                         const std::map<uint32_t, Control> &controls, uint64_t movable, std::vector<double> &areas);
+// adjust_instance_areas (instance_area.rs:157-372); returns requires_optimizer_reset.
+bool adjust_areas(const std::vector<AreaMember> &members, const std::vector<std::pair<uint32_t, double>> &fillers,
+                  std::vector<double> &member_out, std::vector<double> &filler_out);
+void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s);
+
+enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed };
+
+struct Job {
+  // ---- problem (host) ----
+  Options opt;
+  uint32_t W = 0, H = 0;
+  size_t G = 0, U = 0, DU = 0, M = 0, Fl = 0, C = 0, n_cells = 0;
+  int F = 0;
+  std::vector<uint32_t> field_kind;
+  std::vector<uint8_t> unit_fixed;
+  std::vector<double> initial_xy, lower, upper;
+  std::vector<uint32_t> du_unit, du_mstart, m_unit, m_cell;
+  std::vector<uint8_t> m_field;
+  std::vector<double> m_off, m_charge;
+  std::vector<uint8_t> fl_field;
+  std::vector<double> fl_charge, fl_xy;
+  std::vector<int32_t> unit_du;
+  std::map<uint32_t, std::vector<uint32_t>> capacity;  // after fixed occupancy
+  std::vector<size_t> available;                       // per field
+  std::vector<double> real_area, gamma_weight, wl_diag;
+  uint64_t movable_instance_count = 0;
+  bool empty = false, best_effort = false, has_routing = false;
+  std::vector<uint32_t> routing_h, routing_v;
+  std::map<uint32_t, Control> controls;
+  size_t n_nets = 0, n_arcs = 0, n_net_total = 0, n_pins_total = 0, first_arc_pin = 0;
+  std::vector<uint32_t> net_start, pin_unit, small_nets, large_nets, u_pstart, u_pins;
+  std::vector<double> pin_xy, net_weight;
+
+  void load(const texo_gp_problem *p, const texo_gp_options *o);
+  size_t device_bytes() const;  // arena footprint estimate for admission
+
+  // ---- device ----
+  Arena arena;
+  const Basis *basis = nullptr;
+  JobStatic js{};
+  long long *d_base = nullptr;
+  uint32_t *d_m_unit = nullptr, *d_du_unit = nullptr, *d_du_mstart = nullptr;
+  double2 *d_m_off = nullptr, *d_pin_xy = nullptr;
+  double *d_m_charge = nullptr, *d_fl_charge = nullptr, *d_du_fcharge = nullptr;
+  uint8_t *d_m_field = nullptr, *d_fl_field = nullptr;
+  uint32_t *d_net_start = nullptr, *d_pin_unit = nullptr, *d_small = nullptr, *d_large = nullptr;
+  uint32_t *d_u_pstart = nullptr, *d_u_pins = nullptr;
+  double *d_net_weight = nullptr, *d_wl_diag = nullptr;
+  int32_t *d_unit_du = nullptr;
+  double *d_lower = nullptr, *d_upper = nullptr, *d_metric = nullptr, *d_part_metric = nullptr, *d_part_step = nullptr;
+  double *d_vec[4] = {};   // coordinate buffers (rotated by index)
+  double *d_grad[3] = {};  // origin, y, candidate gradients
+  uint32_t *d_best = nullptr;
+  struct Slot {
+    long long *acc_m = nullptr, *acc_f = nullptr;
+    float *rho = nullptr, *phi = nullptr, *t0 = nullptr, *t1 = nullptr;
+    double2 *gfu = nullptr, *dpin = nullptr;
+    double *part_wa = nullptr, *part_energy = nullptr, *part_gu = nullptr, *part_gf = nullptr, *part_ug = nullptr;
+  } slot[2];
+  double *d_rudy = nullptr;
+  long long *d_rudy_fx = nullptr;
+  unsigned int *d_rudy_err = nullptr;
+  double *d_raw_units = nullptr, *d_raw_fillers = nullptr, *d_density_out = nullptr, *d_wa_grad = nullptr;
+  double *d_eval_coords = nullptr;
+  int nb_wa_small = 0, nb_wa_large = 0, nb_energy = 0, nb_gu = 0, nb_gf = 0, nb_ug = 0;
+
+  void upload(const Basis *b, hipStream_t s);
+  void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas);
+  Task task(int slot, const double *coords, double *grad, const double *metric, double gamma, bool skip,
+            EvalScalars *scalars) const;
+
+  // ---- engine bookkeeping ----
+  int64_t id = 0;
+  int pool_index = -1;
+  JobScalars *d_scalars = nullptr;  // in the engine pool
+  const JobScalars *h_scalars = nullptr;
+
+  // ---- optimizer / schedule state (eplace.rs:439-582, global_placement.rs:121-335) ----
+  Phase phase = Phase::Probe;
+  int status = 0;
+  Failure failure{};
+  uint64_t iterations = 0, evaluations = 0, trials = 0, area_rounds = 0;
+  uint64_t pause_at = UINT64_MAX;  // iterate(): stop when iterations reach this
+  bool no_stop = false;            // fixed-iteration or iterate mode: stop rules off
+  uint64_t fixed_limit = UINT64_MAX;
+  bool fixed_mode = false, area_ready = false, exhausted = false, started = false;
+  bool has_prev_change = false, reset_after_area = false, has_fixed_scales = false;
+  double prev_change = 0.0;
+  size_t round_limit = 0, area_round = 0;
+  uint64_t local_iterations = 0;
+  std::vector<double> scales, mult, fixed_scales;
+  double mstep = 0.0;
+  EvalScalars current{};
+  double last_wa = 0.0, gamma = 8.0, g0 = 8.0;
+  bool has_best = false;
+  double best_max = 0.0, best_total = 0.0;
+  bool want_targets = false;
+  std::vector<uint32_t> prev_stationary;
+  bool has_prev_stationary = false;
+  int ix = 0, ixp = 1, iy = 2, ic = 3;
+  int p_slot = 0;
+  bool p_valid = false;
+  double momentum = 1.0, lipschitz = 0.0;
+  bool has_lipschitz = false;
+  // one step's working state
+  bool restart = false, has_wl = false, y_is_origin = true, may_reduce = true, y_evaluated = false;
+  double wl = 0.0, L = 0.0, next_momentum = 1.0, beta = 0.0;
+  EvalScalars origin_s{}, y_s{};
+  double origin_value = 0.0, y_value = 0.0;
+  // requests planned for the next round
+  struct EvalReq {
+    int slot;
+    const double *coords;
+    double *grad;
+    bool metric;
+    double gamma;
+    bool skip;
+    bool parity;
+    int scalar_slot;
+  };
+  std::vector<EvalReq> evals;
+  bool want_precond = false, want_extrapolate = false, want_candidate = false;
+  std::vector<double> targets_out;  // unused placeholder for symmetry
+
+  // schedule helpers
+  double overflow(const EvalScalars &s, int f) const;
+  double combined_value(const EvalScalars &s) const;
+  double adaptive_gamma(const EvalScalars &s) const;
+  void validate_initial(const EvalScalars &s) const;
+  void validate_fields(const EvalScalars &s) const;
+  std::vector<double> subgradient(const EvalScalars &s) const;
+  bool stop_reached(const EvalScalars &s, bool area_ready) const;
+  std::pair<double, double> excess(const EvalScalars &s) const;
+  size_t area_round_limit() const;
+  void update_multipliers();
+  double hpwl(const std::vector<double> &x) const;
+  void begin_run(bool iterate_mode);
+  void begin_round();
+  void count_evaluation();
+};
+
+// Lockstep round engine: one launch per kernel family covers every active job.
+class Engine {
+ public:
+  explicit Engine(int device, size_t pool_capacity = 256);
+  ~Engine();
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  const Basis *basis(int W, int H);
+  void attach(Job *job);  // upload and reserve a scalar slot
+  void detach(Job *job);  // release the slot (job memory is freed by the job)
+  // Runs one round for every job in `jobs` whose phase needs the device.
+  void round(const std::vector<Job *> &jobs);
+  // Parity evaluation of one job at host coordinates.
+  void evaluate_once(Job *job, const double *coords, double gamma, bool density_out);
+  KernelTimer timer;
+
+ private:
+  void plan(Job *job);
+  void advance(Job *job);
+  void area_round(Job *job);
+  int device_;
+  hipStream_t stream_ = nullptr, wa_stream_ = nullptr;
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+  std::map<std::pair<int, int>, std::unique_ptr<Basis>> bases_;
+  JobScalars *d_pool_ = nullptr, *h_pool_ = nullptr;
+  std::vector<bool> pool_used_;
+  char *h_stage_ = nullptr, *d_stage_ = nullptr;
+  size_t stage_bytes_ = 0;
+  void ensure_stage(size_t bytes);
+};
+
+}  // namespace texo

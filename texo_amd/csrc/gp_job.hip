@@ -1,0 +1,983 @@
+// gp_job.hip — one placement job on the host: problem set-up
+// (eplace.rs:255-336, electrostatic_placement.rs:139-240,
+// analytical_placement.rs:116-231), its device arena, and the schedule
+// arithmetic of optimize_density_once (eplace.rs:1111-1415) and the area
+// rounds (instance_area.rs, register_clustering.rs).
+
+#include "gp_host.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <set>
+
+namespace texo {
+
+thread_local std::string g_last_error;
+thread_local KernelTimer *g_timer = nullptr;
+
+namespace {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
+constexpr double MULTIPLIER_ALPHA_LOW = 1.05;   // eplace.rs:128
+constexpr double MULTIPLIER_ALPHA_HIGH = 1.06;  // eplace.rs:129
+constexpr double AREA_ADJUSTMENT_OVERFLOW_TARGET = 0.15;  // eplace.rs:130
+constexpr uint32_t LARGE_NET_PINS = 16;  // nets above this use one wavefront
+}  // namespace
+
+[[noreturn]] void fail(int status, int64_t index, int64_t index2) {
+  Failure f;
+  f.status = status;
+  f.index = index;
+  f.index2 = index2;
+  throw f;
+}
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    fail(e == hipErrorOutOfMemory ? TEXO_GP_ERR_OUT_OF_MEMORY : TEXO_GP_ERR_DEVICE);
+  }
+}
+
+bool is_lut(uint32_t k) { return k >= 0x100u && k < 0x200u; }
+static bool exchangeable(uint32_t k) { return is_lut(k) || k == TEXO_KIND_REGISTER || k == TEXO_KIND_MEMORY; }
+
+// eplace.rs:1300-1308
+double overflow_target(uint32_t kind) {
+  if (kind == TEXO_KIND_MEMORY || kind == TEXO_KIND_DSP) return 0.20;
+  if (is_lut(kind) || kind == TEXO_KIND_REGISTER) return 0.10;
+  return 0.0;
+}
+
+// crates/texo-pnr/src/lib.rs:4566-4568 (f64::round: half away from zero)
+uint32_t rounded_coordinate(double v, uint32_t extent) {
+  double r = std::round(v);
+  const double hi = static_cast<double>(extent - 1);
+  if (!(r >= 0.0)) r = 0.0;
+  if (r > hi) r = hi;
+  return static_cast<uint32_t>(r);
+}
+
+double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+KernelTimer::~KernelTimer() {
+  for (auto &e : events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+}
+
+void timer_begin(KernelTimer *t, const char *name, hipStream_t s) {
+  if (!t || !t->enabled || t->name != name) return;
+  if (t->used == t->events.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    t->events.push_back({a, b});
+  }
+  (void)hipEventRecord(t->events[t->used].first, s);
+}
+
+void timer_end(KernelTimer *t, const char *name, hipStream_t s) {
+  if (!t || !t->enabled || t->name != name || t->used >= t->events.size()) return;
+  (void)hipEventRecord(t->events[t->used].second, s);
+  t->used += 1;
+}
+
+Arena::~Arena() {
+  if (base_) (void)hipFree(base_);
+}
+
+size_t Arena::commit() {
+  size_t total = 0;
+  for (auto &e : plan_) total += (e.bytes + 255) & ~size_t(255);
+  HIP(hipMalloc(&base_, total));
+  char *c = static_cast<char *>(base_);
+  for (auto &e : plan_) {
+    *e.slot = c;
+    c += (e.bytes + 255) & ~size_t(255);
+  }
+  bytes_ = total;
+  plan_.clear();
+  return total;
+}
+
+// Quadratic B-spline deposit on the host, used once per job for the smoothed
+// capacity background (electrostatic_placement.rs:173-190, 900-916).
+static void host_deposit(std::vector<double> &grid, size_t W, size_t H, double x, double y, double q) {
+  auto axis = [](size_t L, double c, size_t idx[3], double w[3]) {
+    const double maximum = static_cast<double>(L - 1);
+    c = clampd(c, 0.0, maximum);
+    const double cc = std::min(std::floor(c + 0.5), maximum);
+    const size_t center = static_cast<size_t>(cc);
+    const double o = c - cc;
+    idx[0] = center == 0 ? 0 : center - 1;
+    idx[1] = center;
+    idx[2] = std::min(center + 1, L - 1);
+    w[0] = 0.5 * ((0.5 - o) * (0.5 - o));
+    w[1] = 0.75 - o * o;
+    w[2] = 0.5 * ((0.5 + o) * (0.5 + o));
+  };
+  size_t ix[3], iy[3];
+  double wx[3], wy[3];
+  axis(W, x, ix, wx);
+  axis(H, y, iy, wy);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) grid[iy[r] * W + ix[c]] += q * wx[c] * wy[r];
+}
+
+// Poisson2d::new (electrostatic_placement.rs:675-714) and the orthonormal DCT
+// bases (:616-672) as dense matrices: FP32, and BF16 hi/lo splits transposed.
+void Basis::build(int w, int h) {
+  W = w;
+  H = h;
+  const size_t dims[4] = {(size_t)W, (size_t)H, (size_t)H, (size_t)W};
+  for (int j = 0; j < 4; ++j) {
+    arena.reserve(b32[j], dims[j] * dims[j]);
+    arena.reserve(bthi[j], dims[j] * dims[j]);
+    arena.reserve(btlo[j], dims[j] * dims[j]);
+  }
+  arena.reserve(inv_lambda, (size_t)W * H);
+  arena.commit();
+  auto basis = [](size_t n) {
+    std::vector<double> b(n * n);
+    for (size_t k = 0; k < n; ++k) {
+      const double s = k == 0 ? std::sqrt(1.0 / (double)n) : std::sqrt(2.0 / (double)n);
+      for (size_t x = 0; x < n; ++x) b[k * n + x] = s * std::cos(kPi * (double)k * ((double)x + 0.5) / (double)n);
+    }
+    return b;
+  };
+  const std::vector<double> cw = basis(W), ch = basis(H);
+  auto bf16 = [](float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  };
+  auto bf16f = [](uint16_t h) {
+    const uint32_t u = static_cast<uint32_t>(h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  // operand j: 0 = C_W^T, 1 = C_H^T, 2 = C_H, 3 = C_W  (row-major K x N)
+  for (int j = 0; j < 4; ++j) {
+    const size_t n = dims[j];
+    const std::vector<double> &c = (j == 0 || j == 3) ? cw : ch;
+    const bool transpose = j < 2;
+    std::vector<float> f32(n * n);
+    std::vector<uint16_t> thi(n * n), tlo(n * n);
+    for (size_t r = 0; r < n; ++r)
+      for (size_t k = 0; k < n; ++k) {
+        const double v = transpose ? c[k * n + r] : c[r * n + k];  // element (K index r, N index k)
+        const float fv = static_cast<float>(v);
+        f32[r * n + k] = fv;
+        const uint16_t hi = bf16(fv);
+        const uint16_t lo = bf16(static_cast<float>(v - (double)bf16f(hi)));
+        thi[k * n + r] = hi;  // transposed: (N index k, K index r)
+        tlo[k * n + r] = lo;
+      }
+    HIP(hipMemcpy(b32[j], f32.data(), n * n * sizeof(float), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(bthi[j], thi.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(btlo[j], tlo.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
+  }
+  std::vector<float> il((size_t)W * H);
+  for (int ky = 0; ky < H; ++ky) {
+    const double sy = std::sin(kPi * (double)ky / (2.0 * (double)H));
+    for (int kx = 0; kx < W; ++kx) {
+      const double sx = std::sin(kPi * (double)kx / (2.0 * (double)W));
+      const double lam = 4.0 * sx * sx + 4.0 * sy * sy;
+      il[(size_t)ky * W + kx] = (kx == 0 && ky == 0) ? 0.0f : static_cast<float>(1.0 / lam);
+    }
+  }
+  HIP(hipMemcpy(inv_lambda, il.data(), il.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
+  if (!p || p->abi_version != TEXO_GP_ABI_VERSION) fail(TEXO_GP_ERR_INVALID_ARGUMENT);
+  if (o) {
+    opt.precision = o->dct_precision;
+    opt.fixed_iterations = o->fixed_iterations;
+    if (o->max_iterations) opt.max_iterations = o->max_iterations;
+    if (o->gamma_base > 0.0) opt.gamma_base = o->gamma_base;
+    opt.trace = o->trace != 0;
+    opt.device = o->device;
+    opt.inject = o->inject_failure_at_evaluation;
+  }
+  if (opt.precision > TEXO_GP_DCT_FP16_SCALED) fail(TEXO_GP_ERR_INVALID_ARGUMENT);
+  if (opt.precision == TEXO_GP_DCT_FP16_SCALED) {
+    g_last_error = "FP16-scaled transform is not available in this build";
+    fail(TEXO_GP_ERR_UNSUPPORTED);
+  }
+  W = p->width;
+  H = p->height;
+  if (W == 0 || H == 0) fail(TEXO_GP_ERR_EMPTY_GRID);
+  if ((uint64_t)W * H > (1ull << 26)) fail(TEXO_GP_ERR_GRID_TOO_LARGE);
+  G = (size_t)W * H;
+  U = p->n_units;
+  n_cells = p->n_cells;
+  best_effort = p->best_effort != 0;
+  unit_fixed.assign(p->unit_fixed, p->unit_fixed + U);
+  initial_xy.assign(p->unit_initial_xy, p->unit_initial_xy + 2 * U);
+  for (size_t c = 0; c < n_cells; ++c)
+    if (p->cell_unit[c] >= U) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)c);
+
+  // Density units: movable units with exchangeable members (eplace.rs:261-301)
+  std::set<uint32_t> kinds;
+  unit_du.assign(U, -1);
+  for (size_t u = 0; u < U; ++u) {
+    const uint32_t b = p->unit_member_start[u], e = p->unit_member_start[u + 1];
+    if (unit_fixed[u] || b == e) continue;
+    unit_du[u] = static_cast<int32_t>(du_unit.size());
+    du_unit.push_back(static_cast<uint32_t>(u));
+    du_mstart.push_back(static_cast<uint32_t>(m_unit.size()));
+    for (uint32_t m = b; m < e; ++m) {
+      if (!exchangeable(p->member_kind[m]) || p->member_cell[m] >= n_cells) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)m);
+      kinds.insert(p->member_kind[m]);
+      m_unit.push_back(static_cast<uint32_t>(u));
+      m_cell.push_back(p->member_cell[m]);
+      m_field.push_back(0);  // filled below
+      m_off.push_back(p->member_offset_xy[2 * m]);
+      m_off.push_back(p->member_offset_xy[2 * m + 1]);
+      m_charge.push_back(1.0);
+    }
+  }
+  du_mstart.push_back(static_cast<uint32_t>(m_unit.size()));
+  DU = du_unit.size();
+  M = m_unit.size();
+  field_kind.assign(kinds.begin(), kinds.end());
+  F = static_cast<int>(field_kind.size());
+  if (F > MAXF) fail(TEXO_GP_ERR_UNSUPPORTED);
+  {
+    size_t k = 0;
+    for (size_t u = 0; u < U; ++u) {
+      const uint32_t b = p->unit_member_start[u], e = p->unit_member_start[u + 1];
+      if (unit_fixed[u] || b == e) continue;
+      for (uint32_t m = b; m < e; ++m)
+        m_field[k++] = static_cast<uint8_t>(std::find(field_kind.begin(), field_kind.end(), p->member_kind[m]) -
+                                            field_kind.begin());
+    }
+  }
+  for (size_t c = 0; c < n_cells; ++c)
+    if (!unit_fixed[p->cell_unit[c]]) movable_instance_count += 1;
+
+  // Hypergraph (analytical_placement.rs:116-168): nets then timed arcs.
+  n_nets = p->n_nets;
+  n_arcs = p->n_arcs;
+  n_net_total = n_nets + n_arcs;
+  net_start.assign(n_net_total + 1, 0);
+  const size_t P = n_nets ? p->net_pin_start[n_nets] : 0;
+  for (size_t n = 0; n < n_nets; ++n) {
+    if (p->net_pin_start[n + 1] < p->net_pin_start[n] + 2) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)n);
+    net_start[n + 1] = p->net_pin_start[n + 1];
+  }
+  first_arc_pin = P;
+  n_pins_total = P + 2 * n_arcs;
+  for (size_t a = 0; a < n_arcs; ++a) net_start[n_nets + a + 1] = static_cast<uint32_t>(P + 2 * (a + 1));
+  std::vector<uint32_t> pin_cell(n_pins_total);
+  for (size_t i = 0; i < P; ++i) pin_cell[i] = p->pin_cell[i];
+  net_weight.assign(n_net_total, 1.0);
+  for (size_t a = 0; a < n_arcs; ++a) {
+    if (p->arc_extra_weight[a] == 0) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)a);
+    pin_cell[P + 2 * a] = p->arc_cells[2 * a];
+    pin_cell[P + 2 * a + 1] = p->arc_cells[2 * a + 1];
+    const uint64_t w = p->arc_extra_weight[a];
+    net_weight[n_nets + a] = (double)(uint32_t)(w >> 32) * 4294967296.0 + (double)(uint32_t)(w & 0xffffffffu);
+  }
+  pin_unit.resize(n_pins_total);
+  pin_xy.resize(2 * n_pins_total);
+  for (size_t i = 0; i < n_pins_total; ++i) {
+    const uint32_t c = pin_cell[i];
+    if (c >= n_cells) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)i);
+    if (p->cell_fixed[c]) {
+      pin_unit[i] = NO_UNIT;
+      pin_xy[2 * i] = p->cell_fixed_xy[2 * c];
+      pin_xy[2 * i + 1] = p->cell_fixed_xy[2 * c + 1];
+    } else {
+      pin_unit[i] = p->cell_unit[c];
+      pin_xy[2 * i] = p->cell_offset_xy[2 * c];
+      pin_xy[2 * i + 1] = p->cell_offset_xy[2 * c + 1];
+    }
+  }
+  for (size_t n = 0; n < n_net_total; ++n)
+    (net_start[n + 1] - net_start[n] > LARGE_NET_PINS ? large_nets : small_nets).push_back(static_cast<uint32_t>(n));
+  // unit -> incident movable pins (pin order; nets before arcs)
+  u_pstart.assign(U + 1, 0);
+  for (size_t i = 0; i < n_pins_total; ++i)
+    if (pin_unit[i] != NO_UNIT) u_pstart[pin_unit[i] + 1] += 1;
+  for (size_t u = 0; u < U; ++u) u_pstart[u + 1] += u_pstart[u];
+  u_pins.assign(u_pstart[U], 0);
+  {
+    std::vector<uint32_t> fill(u_pstart.begin(), u_pstart.end() - 1);
+    for (size_t i = 0; i < n_pins_total; ++i)
+      if (pin_unit[i] != NO_UNIT) u_pins[fill[pin_unit[i]]++] = static_cast<uint32_t>(i);
+  }
+  // wirelength_preconditioner (analytical_placement.rs:177-211) and
+  // baseline_cell_incidence_weights (:219-231)
+  wl_diag.assign(U, 0.0);
+  std::vector<double> cell_weight(n_cells, 0.0);
+  {
+    std::vector<uint32_t> units, cells;
+    for (size_t n = 0; n < n_nets; ++n) {
+      units.clear();
+      cells.clear();
+      for (uint32_t q = net_start[n]; q < net_start[n + 1]; ++q) {
+        units.push_back(p->cell_unit[pin_cell[q]]);
+        cells.push_back(pin_cell[q]);
+      }
+      const double w = 1.0 / (double)(uint32_t)(net_start[n + 1] - net_start[n] - 1);
+      std::sort(units.begin(), units.end());
+      units.erase(std::unique(units.begin(), units.end()), units.end());
+      if (units.size() >= 2)
+        for (uint32_t u : units) wl_diag[u] += w;
+      std::sort(cells.begin(), cells.end());
+      cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+      for (uint32_t c : cells) cell_weight[c] += w;
+    }
+    for (size_t a = 0; a < n_arcs; ++a) {
+      const uint32_t du_ = p->cell_unit[p->arc_cells[2 * a]], su = p->cell_unit[p->arc_cells[2 * a + 1]];
+      if (du_ == su) continue;
+      wl_diag[du_] += net_weight[n_nets + a];
+      wl_diag[su] += net_weight[n_nets + a];
+    }
+  }
+  has_routing = p->routing_horizontal && p->routing_vertical;
+  if (has_routing) {
+    routing_h.assign(p->routing_horizontal, p->routing_horizontal + G);
+    routing_v.assign(p->routing_vertical, p->routing_vertical + G);
+  }
+  for (size_t i = 0; i < p->n_register_controls; ++i)
+    controls[p->register_cell[i]] = {p->register_clock_lsr[2 * i], p->register_clock_lsr[2 * i + 1],
+                                     p->register_ce[i]};
+
+  if (DU == 0) {  // eplace.rs:303-310: no density unit, targets are the rounded inputs
+    empty = true;
+    C = 2 * U;
+    lower.resize(C);
+    upper.resize(C);
+    for (size_t u = 0; u < U; ++u) {
+      lower[2 * u] = p->unit_bounds[4 * u];
+      lower[2 * u + 1] = p->unit_bounds[4 * u + 1];
+      upper[2 * u] = p->unit_bounds[4 * u + 2];
+      upper[2 * u + 1] = p->unit_bounds[4 * u + 3];
+    }
+    return;
+  }
+
+  // DensityModel::new (electrostatic_placement.rs:139-196)
+  for (size_t k = 0; k < p->n_capacity_kinds; ++k)
+    capacity[p->capacity_kind[k]] = std::vector<uint32_t>(p->capacity + k * G, p->capacity + (k + 1) * G);
+  for (size_t i = 0; i < p->n_fixed_occupancy; ++i) {
+    const uint32_t kind = p->fixed_occupancy_kind[i], x = p->fixed_occupancy_xy[2 * i],
+                   y = p->fixed_occupancy_xy[2 * i + 1];
+    if (x >= W || y >= H) {
+      Failure f;
+      f.status = TEXO_GP_ERR_POINT_OUTSIDE_GRID;
+      f.x = x;
+      f.y = y;
+      throw f;
+    }
+    auto it = capacity.find(kind);
+    if (it == capacity.end() || it->second[(size_t)y * W + x] == 0) {
+      Failure f;
+      f.status = TEXO_GP_ERR_FIXED_CAPACITY_EXHAUSTED;
+      f.kind = kind;
+      f.x = x;
+      f.y = y;
+      throw f;
+    }
+    it->second[(size_t)y * W + x] -= 1;
+  }
+
+  // initial_fillers (electrostatic_placement.rs:198-240) with unit checks
+  for (size_t i = 0; i < DU; ++i) {
+    const uint32_t u = du_unit[i];
+    for (uint32_t m = du_mstart[i]; m < du_mstart[i + 1]; ++m) {
+      if (!std::isfinite(initial_xy[2 * u]) || !std::isfinite(initial_xy[2 * u + 1]) || !std::isfinite(m_off[2 * m]) ||
+          !std::isfinite(m_off[2 * m + 1]))
+        fail(TEXO_GP_ERR_NON_FINITE_UNIT_COORDINATE, (int64_t)i, (int64_t)(m - du_mstart[i]));
+    }
+  }
+  available.assign(F, 0);
+  real_area.assign(F, 0.0);
+  std::vector<size_t> counts(F, 0);
+  for (size_t m = 0; m < M; ++m) {
+    counts[m_field[m]] += 1;
+    real_area[m_field[m]] += m_charge[m];
+  }
+  for (int f = 0; f < F; ++f) {
+    auto it = capacity.find(field_kind[f]);
+    size_t total = 0;
+    if (it != capacity.end())
+      for (uint32_t c : it->second) total += c;
+    available[f] = total;
+    if (counts[f] > total) {
+      Failure e;
+      e.status = TEXO_GP_ERR_INSUFFICIENT_CAPACITY;
+      e.kind = field_kind[f];
+      e.v0 = (double)counts[f];
+      e.v1 = (double)total;
+      throw e;
+    }
+    const double avail = (double)total;
+    if (!std::isfinite(real_area[f]) || real_area[f] < 0.0 || real_area[f] > avail + 1.0e-9) {
+      Failure e;
+      e.status = TEXO_GP_ERR_INSUFFICIENT_AREA;
+      e.kind = field_kind[f];
+      e.v0 = real_area[f];
+      e.v1 = avail;
+      throw e;
+    }
+    const double filler_area = avail - real_area[f];
+    if (filler_area == 0.0) continue;
+    std::vector<size_t> occupied;
+    for (size_t i = 0; i < G; ++i)
+      if (it->second[i] != 0) occupied.push_back(i);
+    double assigned = 0.0;
+    for (size_t ord = 0; ord < occupied.size(); ++ord) {
+      const size_t index = occupied[ord];
+      const double charge = ord + 1 == occupied.size() ? filler_area - assigned
+                                                       : filler_area * (double)it->second[index] / avail;
+      assigned += charge;
+      fl_field.push_back(static_cast<uint8_t>(f));
+      fl_charge.push_back(charge);
+      fl_xy.push_back((double)(index % W));
+      fl_xy.push_back((double)(index / W));
+    }
+  }
+  Fl = fl_field.size();
+  C = 2 * (U + Fl);
+  // density_gamma_weights (eplace.rs:1424-1439)
+  gamma_weight.assign(F, 0.0);
+  for (size_t m = 0; m < M; ++m) gamma_weight[m_field[m]] += cell_weight[m_cell[m]];
+  if (controls.size() != p->n_register_controls) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+
+  // initial_coordinates_and_bounds (eplace.rs:1441-1468), then clamp
+  lower.resize(C);
+  upper.resize(C);
+  for (size_t u = 0; u < U; ++u) {
+    lower[2 * u] = p->unit_bounds[4 * u];
+    lower[2 * u + 1] = p->unit_bounds[4 * u + 1];
+    upper[2 * u] = p->unit_bounds[4 * u + 2];
+    upper[2 * u + 1] = p->unit_bounds[4 * u + 3];
+  }
+  for (size_t j = 0; j < Fl; ++j) {
+    lower[2 * (U + j)] = 0.0;
+    lower[2 * (U + j) + 1] = 0.0;
+    upper[2 * (U + j)] = (double)(W - 1);
+    upper[2 * (U + j) + 1] = (double)(H - 1);
+  }
+}
+
+
+// Device footprint of one job (static arrays, two working slots, optimizer
+// vectors), used for admission by HBM footprint.
+size_t Job::device_bytes() const {
+  const size_t FG = (size_t)std::max(F, 1) * G;
+  size_t b = FG * 8 + M * (4 + 16 + 8 + 1) + DU * 8 + (size_t)std::max(F, 1) * DU * 8 + Fl * 9;
+  b += (n_net_total + 1) * 4 + n_pins_total * (4 + 16) + n_net_total * 8 + (small_nets.size() + large_nets.size()) * 4;
+  b += (U + 1) * 4 + u_pins.size() * 4 + U * 12;
+  b += C * 8 * (3 + 4 + 3) + 2 * U * 4;
+  b += 2 * (FG * (8 + 8 + 4 * 4) + (size_t)std::max(F, 1) * DU * 16 + n_pins_total * 16);
+  b += 2 * G * 16 + FG * 8 + 2 * U * 8 + C * 8;
+  return b + 64 * 4096;
+}
+
+void Job::upload(const Basis *b, hipStream_t s) {
+  basis = b;
+  const size_t FG = (size_t)std::max(F, 1) * G;
+  auto blocks = [](size_t n, size_t per) { return std::max<size_t>(1, (n + per - 1) / per); };
+  nb_wa_small = small_nets.empty() ? 0 : (int)blocks(small_nets.size(), BLOCK);
+  nb_wa_large = large_nets.empty() ? 0 : (int)blocks(large_nets.size(), BLOCK / 64);
+  nb_energy = F > 0 ? poisson_energy_blocks((int)W, (int)H, F) : 0;
+  nb_gu = (int)blocks(DU, BLOCK);
+  nb_gf = (int)blocks(Fl, BLOCK);
+  nb_ug = (int)blocks(U, BLOCK);
+  arena.reserve(d_base, FG);
+  arena.reserve(d_m_unit, M);
+  arena.reserve(d_m_off, M);
+  arena.reserve(d_m_charge, M);
+  arena.reserve(d_m_field, M);
+  arena.reserve(d_du_unit, DU);
+  arena.reserve(d_du_mstart, DU + 1);
+  arena.reserve(d_du_fcharge, (size_t)std::max(F, 1) * DU);
+  arena.reserve(d_fl_field, Fl);
+  arena.reserve(d_fl_charge, Fl);
+  arena.reserve(d_net_start, n_net_total + 1);
+  arena.reserve(d_pin_unit, n_pins_total);
+  arena.reserve(d_pin_xy, n_pins_total);
+  arena.reserve(d_net_weight, n_net_total);
+  arena.reserve(d_small, small_nets.size());
+  arena.reserve(d_large, large_nets.size());
+  arena.reserve(d_u_pstart, U + 1);
+  arena.reserve(d_u_pins, u_pins.size());
+  arena.reserve(d_wl_diag, U);
+  arena.reserve(d_unit_du, U);
+  arena.reserve(d_lower, C);
+  arena.reserve(d_upper, C);
+  arena.reserve(d_metric, C);
+  arena.reserve(d_part_metric, blocks(U + Fl, BLOCK) * 3);
+  arena.reserve(d_part_step, blocks(C, BLOCK) * 4);
+  for (auto &v : d_vec) arena.reserve(v, C);
+  for (auto &g : d_grad) arena.reserve(g, C);
+  arena.reserve(d_best, 2 * U);
+  for (auto &sl : slot) {
+    arena.reserve(sl.acc_m, FG);
+    arena.reserve(sl.acc_f, FG);
+    arena.reserve(sl.rho, FG);
+    arena.reserve(sl.phi, FG);
+    arena.reserve(sl.t0, FG);
+    arena.reserve(sl.t1, FG);
+    arena.reserve(sl.gfu, (size_t)std::max(F, 1) * DU);
+    arena.reserve(sl.dpin, n_pins_total);
+    arena.reserve(sl.part_wa, (size_t)nb_wa_small + nb_wa_large);
+    arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
+    arena.reserve(sl.part_gu, (size_t)nb_gu * 2 * MAXF);
+    arena.reserve(sl.part_gf, (size_t)nb_gf * GF_STRIDE);
+    arena.reserve(sl.part_ug, (size_t)nb_ug * UG_STRIDE);
+  }
+  arena.reserve(d_rudy, 2 * G);
+  arena.reserve(d_rudy_fx, 2 * G);
+  arena.reserve(d_rudy_err, 1);
+  arena.reserve(d_raw_units, (size_t)std::max(F, 1) * DU * 2);
+  arena.reserve(d_raw_fillers, 2 * Fl);
+  arena.reserve(d_density_out, FG);
+  arena.reserve(d_wa_grad, 2 * U);
+  arena.reserve(d_eval_coords, C);
+  arena.commit();
+
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    if (bytes) HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  };
+  // fixed-point background: -smoothed capacity per field
+  std::vector<long long> base(FG, 0);
+  for (int f = 0; f < F; ++f) {
+    std::vector<double> smooth(G, 0.0);
+    auto it = capacity.find(field_kind[f]);
+    if (it != capacity.end())
+      for (size_t i = 0; i < G; ++i)
+        if (it->second[i]) host_deposit(smooth, W, H, (double)(i % W), (double)(i / W), (double)it->second[i]);
+    for (size_t i = 0; i < G; ++i) base[(size_t)f * G + i] = std::llrint(-smooth[i] * FX_SCALE);
+  }
+  up(d_base, base.data(), FG * sizeof(long long));
+  up(d_m_unit, m_unit.data(), M * 4);
+  up(d_m_off, m_off.data(), M * 16);
+  up(d_m_field, m_field.data(), M);
+  up(d_du_unit, du_unit.data(), DU * 4);
+  up(d_du_mstart, du_mstart.data(), (DU + 1) * 4);
+  up(d_fl_field, fl_field.data(), Fl);
+  up(d_net_start, net_start.data(), (n_net_total + 1) * 4);
+  up(d_pin_unit, pin_unit.data(), n_pins_total * 4);
+  up(d_pin_xy, pin_xy.data(), n_pins_total * 16);
+  up(d_net_weight, net_weight.data(), n_net_total * 8);
+  up(d_small, small_nets.data(), small_nets.size() * 4);
+  up(d_large, large_nets.data(), large_nets.size() * 4);
+  up(d_u_pstart, u_pstart.data(), (U + 1) * 4);
+  up(d_u_pins, u_pins.data(), u_pins.size() * 4);
+  up(d_wl_diag, wl_diag.data(), U * 8);
+  up(d_unit_du, unit_du.data(), U * 4);
+  up(d_lower, lower.data(), C * 8);
+  up(d_upper, upper.data(), C * 8);
+  std::vector<double> x0(C);
+  for (size_t u = 0; u < U; ++u) {
+    x0[2 * u] = initial_xy[2 * u];
+    x0[2 * u + 1] = initial_xy[2 * u + 1];
+  }
+  for (size_t k = 0; k < Fl; ++k) {
+    x0[2 * (U + k)] = fl_xy[2 * k];
+    x0[2 * (U + k) + 1] = fl_xy[2 * k + 1];
+  }
+  for (size_t i = 0; i < C; ++i) x0[i] = clampd(x0[i], lower[i], upper[i]);
+  up(d_vec[ix], x0.data(), C * 8);
+  up(d_vec[ixp], x0.data(), C * 8);
+  HIP(hipStreamSynchronize(s));  // host vectors above are pageable and local
+  apply_charges(std::vector<double>(m_charge), std::vector<double>(fl_charge));
+
+  js = JobStatic{};
+  js.W = (int)W;
+  js.H = (int)H;
+  js.F = F;
+  js.n_members = (int)M;
+  js.n_fillers = (int)Fl;
+  js.n_units = (int)U;
+  js.n_density_units = (int)DU;
+  js.m_unit = d_m_unit;
+  js.m_off = d_m_off;
+  js.m_charge = d_m_charge;
+  js.m_field = d_m_field;
+  js.du_unit = d_du_unit;
+  js.du_mstart = d_du_mstart;
+  js.fl_field = d_fl_field;
+  js.fl_charge = d_fl_charge;
+  js.base_fx = d_base;
+  js.unit_du = d_unit_du;
+  js.lower = d_lower;
+  js.upper = d_upper;
+  js.n_small = (int)small_nets.size();
+  js.n_large = (int)large_nets.size();
+  js.small_nets = d_small;
+  js.large_nets = d_large;
+  js.net_start = d_net_start;
+  js.net_weight = d_net_weight;
+  js.pin_unit = d_pin_unit;
+  js.pin_xy = d_pin_xy;
+  js.u_pstart = d_u_pstart;
+  js.u_pins = d_u_pins;
+  js.first_arc_pin = (uint32_t)first_arc_pin;
+  for (int j = 0; j < 4; ++j) {
+    js.b32[j] = b->b32[j];
+    js.bthi[j] = b->bthi[j];
+    js.btlo[j] = b->btlo[j];
+  }
+  js.inv_lambda = b->inv_lambda;
+}
+
+// Installs member and filler charges (apply_area_adjustment, eplace.rs:793-819).
+void Job::apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas) {
+  m_charge = member_areas;
+  fl_charge = filler_areas;
+  std::vector<double> fcharge((size_t)std::max(F, 1) * DU, 0.0);
+  for (size_t i = 0; i < DU; ++i)
+    for (uint32_t m = du_mstart[i]; m < du_mstart[i + 1]; ++m) fcharge[(size_t)m_field[m] * DU + i] += m_charge[m];
+  real_area.assign(F, 0.0);
+  for (size_t m = 0; m < M; ++m) real_area[m_field[m]] += m_charge[m];
+  if (M) HIP(hipMemcpy(d_m_charge, m_charge.data(), M * 8, hipMemcpyHostToDevice));
+  if (Fl) HIP(hipMemcpy(d_fl_charge, fl_charge.data(), Fl * 8, hipMemcpyHostToDevice));
+  if (DU) HIP(hipMemcpy(d_du_fcharge, fcharge.data(), fcharge.size() * 8, hipMemcpyHostToDevice));
+}
+
+Task Job::task(int sl, const double *coords, double *grad, const double *metric, double g, bool skip,
+               EvalScalars *scalars) const {
+  Task t{};
+  t.j = js;
+  const Slot &w = slot[sl];
+  t.acc_m = w.acc_m;
+  t.acc_f = w.acc_f;
+  t.rho = w.rho;
+  t.phi = w.phi;
+  t.t0 = w.t0;
+  t.t1 = w.t1;
+  t.gfu = w.gfu;
+  t.dpin = w.dpin;
+  t.part_wa = w.part_wa;
+  t.part_energy = w.part_energy;
+  t.part_gu = w.part_gu;
+  t.part_gf = w.part_gf;
+  t.part_ug = w.part_ug;
+  t.nb_wa_small = nb_wa_small;
+  t.nb_wa_large = nb_wa_large;
+  t.nb_energy = F > 0 ? nb_energy : 0;
+  t.nb_gu = F > 0 ? nb_gu : 0;
+  t.nb_gf = F > 0 && Fl > 0 ? nb_gf : 0;
+  t.nb_ug = nb_ug;
+  t.coords = coords;
+  t.grad = grad;
+  t.metric = metric;
+  t.gamma = g;
+  t.skip_density = skip ? 1 : 0;
+  for (int f = 0; f < MAXF; ++f) {
+    t.lambda[f] = f < (int)mult.size() ? mult[f] : 0.0;
+    t.scale[f] = f < (int)scales.size() ? scales[f] : 1.0;
+  }
+  t.scalars = scalars;
+  return t;
+}
+
+double Job::overflow(const EvalScalars &s, int f) const {
+  return static_cast<double>(s.over_fx[f]) * FX_INV / real_area[f];
+}
+
+double Job::combined_value(const EvalScalars &s) const {
+  // combined_objective (eplace.rs:1022-1044) with
+  // augmented_density_value_and_coefficient (eplace.rs:1111-1124)
+  double v = s.wa_value;
+  for (int f = 0; f < F; ++f) {
+    const double e = s.energy[f] / scales[f];
+    v += mult[f] * scales[f] * (e + 0.5 * AUGMENTED_DENSITY_BETA * e * e);
+  }
+  return v;
+}
+
+double Job::adaptive_gamma(const EvalScalars &s) const {
+  // field_wirelength_gamma + weighted_gamma_mean (eplace.rs:1325-1395)
+  double ws = 0.0, wsum = 0.0;
+  const double base = opt.gamma_base;
+  for (int f = 0; f < F; ++f) {
+    const double o = overflow(s, f), target = overflow_target(field_kind[f]);
+    if (!std::isfinite(o) || o < 0.0 || !(target >= 0.0 && target < 1.0) || !(base > 0.0))
+      fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    const double slope = 2.0 / (1.0 - target);
+    const double gamma = base * std::pow(10.0, slope * o + (1.0 - slope));
+    if (!(std::isfinite(gamma) && gamma > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    const double w = gamma_weight[f];
+    ws += gamma * w;
+    wsum += w;
+  }
+  const double mean = wsum == 0.0 ? base : ws / wsum;
+  if (!(std::isfinite(mean) && mean > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return mean;
+}
+
+void Job::validate_initial(const EvalScalars &s) const {
+  bool ok = std::isfinite(s.wa_value) && s.nonfinite == 0;
+  for (int f = 0; f < F; ++f)
+    ok &= std::isfinite(s.energy[f]) && s.energy[f] >= 0.0 && std::isfinite(overflow(s, f)) &&
+          std::isfinite(s.force_l1[f]) && s.force_l1[f] >= 0.0;
+  if (!ok) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+}
+
+void Job::validate_fields(const EvalScalars &s) const {
+  for (int f = 0; f < F; ++f)
+    if (!std::isfinite(s.energy[f]) || !std::isfinite(scales[f]) || scales[f] <= 0.0)
+      fail(TEXO_GP_ERR_DENSITY_FIELD_CHANGED);
+}
+
+std::vector<double> Job::subgradient(const EvalScalars &s) const {
+  validate_fields(s);
+  std::vector<double> v(F);
+  for (int f = 0; f < F; ++f) {
+    const double e = s.energy[f] / scales[f];
+    v[f] = e + 0.5 * AUGMENTED_DENSITY_BETA * e * e;
+    if (!(std::isfinite(v[f]) && v[f] >= 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  }
+  return v;
+}
+
+bool Job::stop_reached(const EvalScalars &s, bool area_ready) const {
+  for (int f = 0; f < F; ++f) {
+    const uint32_t k = field_kind[f];
+    if (area_ready) {
+      if ((is_lut(k) || k == TEXO_KIND_REGISTER) && !(overflow(s, f) <= AREA_ADJUSTMENT_OVERFLOW_TARGET)) return false;
+    } else if (!(overflow(s, f) <= overflow_target(k))) {
+      return false;
+    }
+  }
+  return true;
+}
+
+std::pair<double, double> Job::excess(const EvalScalars &s) const {
+  double mx = 0.0, total = 0.0;
+  for (int f = 0; f < F; ++f) {
+    const double e = std::max(overflow(s, f) - overflow_target(field_kind[f]), 0.0);
+    mx = std::max(mx, e);
+    total += e;
+  }
+  return {mx, total};
+}
+
+double Job::hpwl(const std::vector<double> &x) const {
+  double total = 0.0;
+  for (size_t n = 0; n < n_nets; ++n) {
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (uint32_t p = net_start[n]; p < net_start[n + 1]; ++p) {
+      double px = pin_xy[2 * p], py = pin_xy[2 * p + 1];
+      if (pin_unit[p] != NO_UNIT) {
+        px += x[2 * pin_unit[p]];
+        py += x[2 * pin_unit[p] + 1];
+      }
+      xmin = std::min(xmin, px);
+      xmax = std::max(xmax, px);
+      ymin = std::min(ymin, py);
+      ymax = std::max(ymax, py);
+    }
+    total += (xmax - xmin) + (ymax - ymin);
+  }
+  return total;
+}
+
+size_t Job::area_round_limit() const {
+  double initial = 0.0, avail = 0.0;
+  for (double q : m_charge) initial += q;
+  for (int f = 0; f < F; ++f) avail += (double)(uint32_t)available[f];
+  if (!(std::isfinite(initial) && initial > 0.0 && avail >= initial)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  double t = initial;
+  size_t rounds = 0;
+  while (t < avail) {
+    t *= 1.01;
+    if (!std::isfinite(t)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    rounds += 1;
+  }
+  return rounds + 1;
+}
+
+void Job::update_multipliers() {
+  const std::vector<double> s = subgradient(current);
+  double n = 0.0;
+  for (double v : s) n += v * v;
+  n = std::sqrt(n);
+  if (n != 0.0) {
+    if (!std::isfinite(n) || !std::isfinite(mstep) || mstep <= 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    for (int f = 0; f < F; ++f) {
+      mult[f] += mstep * s[f] / n;
+      if (!std::isfinite(mult[f]) || mult[f] < 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    }
+  }
+  validate_fields(current);
+  double en = 0.0;
+  for (int f = 0; f < F; ++f) {
+    const double e = current.energy[f] / scales[f];
+    en += e * e;
+  }
+  const double lg = std::log(AUGMENTED_DENSITY_BETA * std::sqrt(en) + 1.0);
+  if (!std::isfinite(lg) || lg < 0.0) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  const double growth = MULTIPLIER_ALPHA_LOW + (1.0 - 1.0 / (1.0 + lg)) * (MULTIPLIER_ALPHA_HIGH - MULTIPLIER_ALPHA_LOW);
+  if (!(std::isfinite(growth) && growth >= MULTIPLIER_ALPHA_LOW && growth <= MULTIPLIER_ALPHA_HIGH))
+    fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  const double next = mstep * growth;
+  if (!(std::isfinite(next) && next > 0.0)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  mstep = next;
+}
+
+// Area-round helpers (instance_area.rs:157-372, register_clustering.rs:17-96).
+double soft_division_ceiling(double value, double divisor) {
+  const double q = value / divisor;
+  const double fl = std::floor(q);
+  if (q - fl < 1.0 / divisor) return value + (1.0 - divisor) * fl;
+  return std::ceil(q);
+}
+
+// clustering_areas (register_clustering.rs:17-81) for registers given as
+// (cell, x, y); false reproduces the reference's None.
+bool register_clustering(const std::vector<std::pair<uint32_t, std::pair<double, double>>> &regs,
+                         const std::map<uint32_t, Control> &controls, uint64_t movable, std::vector<double> &areas) {
+  areas.assign(regs.size(), 0.0);
+  if (regs.empty() || movable == 0) return true;
+  if (movable > 0xffffffffull) return false;
+  const double sigma = std::sqrt(1.0e-5 * (double)(uint32_t)movable);
+  if (!std::isfinite(sigma) || sigma <= 0.0) return false;
+  const double half = 2.5 * sigma, scale = sigma * kSqrt2;
+  std::map<std::pair<uint64_t, uint64_t>, std::vector<size_t>> groups;
+  for (size_t i = 0; i < regs.size(); ++i) {
+    auto it = controls.find(regs[i].first);
+    if (it == controls.end()) return false;
+    groups[{it->second.clock, it->second.lsr}].push_back(i);
+  }
+  auto prob = [&](double lo, double hi, double mean) {
+    return clampd(0.5 * (std::erf((hi - mean) / scale) - std::erf((lo - mean) / scale)), 0.0, 1.0);
+  };
+  for (size_t r = 0; r < regs.size(); ++r) {
+    const Control &focal = controls.at(regs[r].first);
+    std::map<uint64_t, double> expected;
+    const double fx = regs[r].second.first, fy = regs[r].second.second;
+    for (size_t o : groups[{focal.clock, focal.lsr}]) {
+      const double px = prob(fx - half, fx + half, regs[o].second.first);
+      const double py = prob(fy - half, fy + half, regs[o].second.second);
+      expected[controls.at(regs[o].first).ce] += px * py;
+    }
+    const double ef = expected.count(focal.ce) ? expected[focal.ce] : 0.0;
+    if (!(std::isfinite(ef) && ef > 0.0)) return false;
+    double total = 0.0;
+    for (auto &kv : expected) total += soft_division_ceiling(kv.second, 2.0);
+    const double area = 8.0 / ef * soft_division_ceiling(ef, 2.0) / total * soft_division_ceiling(total, 4.0);
+    if (!(std::isfinite(area) && area >= 0.0)) return false;
+    areas[r] = area;
+  }
+  return true;
+}
+
+
+
+void add_checked(std::map<uint32_t, double> &t, uint32_t k, double v) {
+  double &x = t[k];
+  x += v;
+  if (!std::isfinite(x)) fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+}
+
+// adjust_instance_areas (instance_area.rs:157-253) with
+// adjusted_filler_areas (:293-332) and validate_postconditions (:334-367).
+bool adjust_areas(const std::vector<AreaMember> &members, const std::vector<std::pair<uint32_t, double>> &fillers,
+                  std::vector<double> &member_out, std::vector<double> &filler_out) {
+  for (size_t i = 0; i < members.size(); ++i) {
+    const AreaMember &m = members[i];
+    if (!std::isfinite(m.current) || m.current < 0.0 || !std::isfinite(m.routability) || m.routability < 0.0 ||
+        !std::isfinite(m.clustering) || m.clustering < 0.0)
+      fail(TEXO_GP_ERR_INVALID_MEMBER_AREA, (int64_t)i);
+  }
+  for (size_t i = 0; i < fillers.size(); ++i)
+    if (!std::isfinite(fillers[i].second) || fillers[i].second < 0.0) fail(TEXO_GP_ERR_INVALID_FILLER_AREA, (int64_t)i);
+  const size_t n = members.size();
+  std::vector<double> requested(n), delta(n);
+  std::map<uint32_t, double> desired, avail, scales, target, actual;
+  for (size_t i = 0; i < n; ++i) {
+    // pin_area is supplied as 0.0 by the caller (eplace.rs:708-710)
+    requested[i] = std::max(std::max(std::max(members[i].routability, 0.0), members[i].clustering), members[i].current);
+    delta[i] = requested[i] - members[i].current;
+    add_checked(desired, members[i].kind, delta[i]);
+  }
+  for (auto &f : fillers) add_checked(avail, f.first, f.second);
+  for (auto &kv : desired) {
+    const double a = avail.count(kv.first) ? avail[kv.first] : 0.0;
+    scales[kv.first] = kv.second == 0.0 ? 1.0 : std::min(a / kv.second, 1.0);
+    target[kv.first] = std::min(kv.second, a);
+  }
+  std::map<uint32_t, size_t> last;
+  for (size_t i = 0; i < n; ++i) last[members[i].kind] = i;
+  member_out.assign(n, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t k = members[i].kind;
+    const double scale = scales[k];
+    double increase;
+    if (last[k] == i) {
+      const double assigned = actual.count(k) ? actual[k] : 0.0;
+      increase = clampd(target[k] - assigned, 0.0, delta[i]);
+    } else {
+      increase = delta[i] * scale;
+    }
+    const double adjusted = scale == 1.0 ? requested[i] : members[i].current + increase;
+    if (!std::isfinite(adjusted)) fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+    add_checked(actual, k, adjusted - members[i].current);
+    member_out[i] = adjusted;
+  }
+  bool all_zero = true;
+  for (auto &kv : actual) all_zero = all_zero && kv.second == 0.0;
+  filler_out.assign(fillers.size(), 0.0);
+  if (all_zero) {
+    for (size_t i = 0; i < fillers.size(); ++i) filler_out[i] = fillers[i].second;
+  } else {
+    std::map<uint32_t, double> remaining, assigned;
+    for (auto &kv : avail) remaining[kv.first] = std::max(kv.second - (actual.count(kv.first) ? actual[kv.first] : 0.0), 0.0);
+    std::map<uint32_t, size_t> lastf;
+    for (size_t i = 0; i < fillers.size(); ++i) lastf[fillers[i].first] = i;
+    for (size_t i = 0; i < fillers.size(); ++i) {
+      const uint32_t k = fillers[i].first;
+      const double cur = fillers[i].second, total = avail[k], tgt = remaining[k];
+      double area;
+      if (lastf[k] == i)
+        area = std::max(tgt - (assigned.count(k) ? assigned[k] : 0.0), 0.0);
+      else if (total == 0.0)
+        area = 0.0;
+      else
+        area = cur * tgt / total;
+      const double tol = 64.0 * DBL_EPSILON * std::max(std::fabs(cur), 1.0);
+      if (!std::isfinite(area) || area > cur + tol) fail(TEXO_GP_ERR_ARITHMETIC_OVERFLOW);
+      area = std::min(area, cur);
+      add_checked(assigned, k, area);
+      filler_out[i] = area;
+    }
+  }
+  std::map<uint32_t, double> before, after;
+  for (size_t i = 0; i < n; ++i) {
+    if (member_out[i] < members[i].current || !std::isfinite(member_out[i])) fail(TEXO_GP_ERR_POSTCONDITION_VIOLATION);
+    add_checked(before, members[i].kind, members[i].current);
+    add_checked(after, members[i].kind, member_out[i]);
+  }
+  for (size_t i = 0; i < fillers.size(); ++i) {
+    if (filler_out[i] < 0.0 || filler_out[i] > fillers[i].second || !std::isfinite(filler_out[i]))
+      fail(TEXO_GP_ERR_POSTCONDITION_VIOLATION);
+    add_checked(before, fillers[i].first, fillers[i].second);
+    add_checked(after, fillers[i].first, filler_out[i]);
+  }
+  const double terms = std::max((double)(uint32_t)(n + fillers.size()), 1.0);
+  for (auto &kv : before) {
+    const double a = after.count(kv.first) ? after[kv.first] : 0.0;
+    if (std::fabs(kv.second - a) > 64.0 * DBL_EPSILON * terms * std::max(std::fabs(kv.second), 1.0))
+      fail(TEXO_GP_ERR_POSTCONDITION_VIOLATION);
+  }
+  bool reset = false;
+  for (auto &kv : actual) reset = reset || kv.second > 0.0;
+  return reset;
+}
+
+
+}  // namespace texo

@@ -1,20 +1,20 @@
 // gp_kernels.hip — scatter/gather and elementwise kernels of the electrostatic
 // global-placement loop for gfx950 (64-lane wavefronts).
 //
-// Every kernel here is HBM/latency bound; reductions are deterministic:
-// integer sums use int64 atomics (exact, order-free), maxima use atomicMax on
-// the bit patterns of non-negative doubles (order-free), and FP64 sums go
-// through per-block partials reduced in a fixed order by a single block.
+// Every kernel covers a table of tasks through a block map; each block works
+// for exactly one task.  All are HBM/latency bound.  Reductions are
+// deterministic: integer sums use int64 atomics (exact, order-free), FP64 sums
+// and maxima go through per-block partials that one block per task reduces
+// in a fixed order.  No kernel uses same-address atomics across blocks.
 
 #include "gp_common.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace texo {
 
 namespace {
-
-constexpr int BLOCK = 256;
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -29,8 +29,7 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-// Block-wide sum of K values per thread; thread 0 writes out[0..K).  The
-// reduction tree is fixed by the block shape, so results are reproducible.
+// Block-wide sum of K values per thread; thread 0 writes out[0..K).
 template <int K>
 __device__ void block_sum(double (&v)[K], double *out) {
   __shared__ double s[K][BLOCK / 64];
@@ -43,23 +42,13 @@ __device__ void block_sum(double (&v)[K], double *out) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) / 64;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double acc = 0.0;
-      for (int w = 0; w < nw; ++w) acc += s[k][w];
-      out[k] = acc;
-    }
+    for (int k = 0; k < K; ++k) out[k] = ((s[k][0] + s[k][1]) + s[k][2]) + s[k][3];
   }
 }
 
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *p, double v) {
-  atomicMax(p, static_cast<unsigned long long>(__double_as_longlong(v)));
-}
-
 // Block-wide maximum of a non-negative value and OR of a flag; thread 0
-// writes out[0] = max, out[1] = flag (as 0.0 / 1.0).  Used instead of
-// same-address atomics, which serialise across thousands of waves.
+// writes out[0] = max and out[1] = flag as 0.0 / 1.0.
 __device__ void block_max_flag(double v, bool flag, double *out) {
   __shared__ double s[BLOCK / 64];
   __shared__ int f[BLOCK / 64];
@@ -72,454 +61,14 @@ __device__ void block_max_flag(double v, bool flag, double *out) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double m = 0.0;
-    int a = 0;
-    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
-      m = fmax(m, s[w]);
-      a |= f[w];
-    }
-    out[0] = m;
-    out[1] = a ? 1.0 : 0.0;
+    out[0] = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+    out[1] = (f[0] | f[1] | f[2] | f[3]) ? 1.0 : 0.0;
   }
 }
 
-// kkt_component (global_placement.rs:675-689) divided by sqrt(metric).
-__device__ __forceinline__ double kkt_scaled(double g, double x, double lo, double hi, double m) {
-  const bool equal = __double_as_longlong(lo) == __double_as_longlong(hi);
-  if (equal || (x <= lo && g > 0.0) || (x >= hi && g < 0.0)) return 0.0;
-  return fabs(g) / sqrt(m);
-}
-
-// ---------------------------------------------------------------------------
-// Density deposit (electrostatic_placement.rs:296-332, 900-916).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLOCK) k_deposit_init(int total, const long long *__restrict__ base,
-                                                        long long *__restrict__ acc_m,
-                                                        long long *__restrict__ acc_f, EvalScalars *scalars) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i < total) {
-    acc_m[i] = base[i];
-    acc_f[i] = 0;
-  }
-  if (i == 0) {
-    for (int f = 0; f < MAXF; ++f) {
-      scalars->over_fx[f] = 0;
-      scalars->net_fx[f] = 0;
-    }
-    scalars->kkt_bits = 0;
-    scalars->nonfinite = 0;
-  }
-}
-
-__global__ void k_zero_eval(EvalScalars *scalars) {
-  for (int f = 0; f < MAXF; ++f) {
-    scalars->over_fx[f] = 0;
-    scalars->net_fx[f] = 0;
-  }
-  scalars->kkt_bits = 0;
-  scalars->nonfinite = 0;
-}
-
-__global__ void __launch_bounds__(BLOCK) k_deposit(DensityArgs a, const double *__restrict__ coords) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  double x, y, q;
-  int f;
-  long long *acc;
-  if (i < a.n_members) {
-    const uint32_t u = a.m_unit[i];
-    const double2 off = a.m_off[i];
-    x = coords[2 * (size_t)u] + off.x;
-    y = coords[2 * (size_t)u + 1] + off.y;
-    q = a.m_charge[i];
-    f = a.m_field[i];
-    acc = a.acc_m;
-  } else if (i < a.n_members + a.n_fillers) {
-    const int j = i - a.n_members;
-    const size_t c = 2 * ((size_t)a.n_units + j);
-    x = coords[c];
-    y = coords[c + 1];
-    q = a.fl_charge[j];
-    f = a.fl_field[j];
-    acc = a.acc_f;
-  } else {
-    return;
-  }
-  Axis ax, ay;
-  bspline_axis(a.W, x, ax);
-  bspline_axis(a.H, y, ay);
-  unsigned long long *g =
-      reinterpret_cast<unsigned long long *>(acc + (size_t)f * (size_t)a.W * (size_t)a.H);
-  const int ix[3] = {ax.i0, ax.i1, ax.i2};
-  const int iy[3] = {ay.i0, ay.i1, ay.i2};
-  const double wx[3] = {ax.w0, ax.w1, ax.w2};
-  const double wy[3] = {ay.w0, ay.w1, ay.w2};
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double v = q * wx[c] * wy[r];
-      const long long fx = __double2ll_rn(v * FX_SCALE);
-      atomicAdd(g + (size_t)iy[r] * a.W + ix[c], static_cast<unsigned long long>(fx));
-    }
-  }
-}
-
-// rho = members + fillers (fixed point) -> FP32 transform input; physical
-// overflow (before fillers) and net charge as exact int64 sums.
-__global__ void __launch_bounds__(BLOCK) k_density_convert(int G, int F, const long long *__restrict__ acc_m,
-                                                           const long long *__restrict__ acc_f,
-                                                           float *__restrict__ rho, double *__restrict__ density_out,
-                                                           EvalScalars *scalars) {
-  __shared__ unsigned long long s_over[MAXF], s_net[MAXF];
-  if (threadIdx.x < MAXF) {
-    s_over[threadIdx.x] = 0;
-    s_net[threadIdx.x] = 0;
-  }
-  __syncthreads();
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i < F * G) {
-    const int f = i / G;
-    const long long m = acc_m[i];
-    const long long t = m + acc_f[i];
-    const double v = static_cast<double>(t) * FX_INV;
-    rho[i] = static_cast<float>(v);
-    if (density_out) density_out[i] = v;
-    if (m > 0) atomicAdd(&s_over[f], static_cast<unsigned long long>(m));
-    atomicAdd(&s_net[f], static_cast<unsigned long long>(t));
-  }
-  __syncthreads();
-  if (threadIdx.x < F) {
-    atomicAdd(reinterpret_cast<unsigned long long *>(&scalars->over_fx[threadIdx.x]), s_over[threadIdx.x]);
-    atomicAdd(reinterpret_cast<unsigned long long *>(&scalars->net_fx[threadIdx.x]), s_net[threadIdx.x]);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Field gradient gather (electrostatic_placement.rs:338-374, 919-941).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void load_coefficients(const GatherArgs &g, int F, double *coef) {
-  // Each block re-reduces the energy partials of the transform in the same
-  // fixed order, then forms lambda (1 + beta E/E0) (eplace.rs:1111-1124).
-  if (threadIdx.x < F) {
-    const int f = threadIdx.x;
-    double e = 0.0;
-    for (int b = 0; b < g.nb_energy; ++b) e += g.energy_partials[b * MAXF + f];
-    const double normalized = e / g.scale[f];
-    coef[f] = g.lambda[f] * (1.0 + AUGMENTED_DENSITY_BETA * normalized);
-  }
-  __syncthreads();
-}
-
-__device__ __forceinline__ void sample(const float *__restrict__ phi, int W, int H, double x, double y, double &gx,
-                                       double &gy) {
-  Axis ax, ay;
-  bspline_axis(W, x, ax);
-  bspline_axis(H, y, ay);
-  const int ix[3] = {ax.i0, ax.i1, ax.i2};
-  const int iy[3] = {ay.i0, ay.i1, ay.i2};
-  const double wx[3] = {ax.w0, ax.w1, ax.w2};
-  const double wy[3] = {ay.w0, ay.w1, ay.w2};
-  const double dx[3] = {ax.d0, ax.d1, ax.d2};
-  const double dy[3] = {ay.d0, ay.d1, ay.d2};
-  gx = 0.0;
-  gy = 0.0;
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double p = static_cast<double>(phi[(size_t)iy[r] * W + ix[c]]);
-      gx += dx[c] * wy[r] * p;
-      gy += wx[c] * dy[r] * p;
-    }
-  }
-}
-
-__global__ void __launch_bounds__(BLOCK) k_gather_units(DensityArgs a, GatherArgs g, const double *__restrict__ coords,
-                                                        double2 *__restrict__ gfu, double *__restrict__ raw_units,
-                                                        double *__restrict__ partials) {
-  __shared__ double coef[MAXF];
-  load_coefficients(g, a.F, coef);
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  const size_t G = (size_t)a.W * a.H;
-  double red[2 * MAXF];
-#pragma unroll
-  for (int k = 0; k < 2 * MAXF; ++k) red[k] = 0.0;
-  if (i < a.n_density_units) {
-    const uint32_t u = a.du_unit[i];
-    const double ox = coords[2 * (size_t)u], oy = coords[2 * (size_t)u + 1];
-    double gxf[MAXF], gyf[MAXF];
-#pragma unroll
-    for (int f = 0; f < MAXF; ++f) gxf[f] = gyf[f] = 0.0;
-    for (uint32_t m = a.du_mstart[i]; m < a.du_mstart[i + 1]; ++m) {
-      const int f = a.m_field[m];
-      const double2 off = a.m_off[m];
-      const double q = a.m_charge[m];
-      double gx, gy;
-      sample(g.phi + f * G, a.W, a.H, ox + off.x, oy + off.y, gx, gy);
-#pragma unroll
-      for (int k = 0; k < MAXF; ++k) {
-        if (k == f) {
-          red[k] += q * (fabs(gx) + fabs(gy));
-          gxf[k] += q * gx;
-          gyf[k] += q * gy;
-        }
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < MAXF; ++f) {
-      if (f < a.F) {
-        red[MAXF + f] = fabs(gxf[f]) + fabs(gyf[f]);
-        gfu[(size_t)f * a.n_density_units + i] = make_double2(coef[f] * gxf[f], coef[f] * gyf[f]);
-        if (raw_units) {
-          raw_units[2 * ((size_t)f * a.n_density_units + i)] = gxf[f];
-          raw_units[2 * ((size_t)f * a.n_density_units + i) + 1] = gyf[f];
-        }
-      }
-    }
-  }
-  block_sum<2 * MAXF>(red, partials + (size_t)blockIdx.x * 2 * MAXF);
-}
-
-__global__ void __launch_bounds__(BLOCK) k_gather_fillers(DensityArgs a, GatherArgs g,
-                                                          const double *__restrict__ coords,
-                                                          double *__restrict__ raw_fillers, double *__restrict__ grad,
-                                                          const double *__restrict__ metric,
-                                                          const double *__restrict__ lower,
-                                                          const double *__restrict__ upper,
-                                                          double *__restrict__ partials, EvalScalars *scalars) {
-  __shared__ double coef[MAXF];
-  load_coefficients(g, a.F, coef);
-  const int j = blockIdx.x * BLOCK + threadIdx.x;
-  const size_t G = (size_t)a.W * a.H;
-  double red[2 * MAXF];
-#pragma unroll
-  for (int k = 0; k < 2 * MAXF; ++k) red[k] = 0.0;
-  double kkt = 0.0;
-  bool bad = false;
-  if (j < a.n_fillers) {
-    const size_t c = 2 * ((size_t)a.n_units + j);
-    const double x = coords[c], y = coords[c + 1];
-    const int f = a.fl_field[j];
-    const double q = a.fl_charge[j];
-    double gx, gy;
-    sample(g.phi + f * G, a.W, a.H, x, y, gx, gy);
-    const double rx = q * gx, ry = q * gy;
-#pragma unroll
-    for (int k = 0; k < MAXF; ++k) {
-      if (k == f) {
-        red[k] = q * (fabs(gx) + fabs(gy));
-        red[MAXF + k] = fabs(rx) + fabs(ry);
-      }
-    }
-    if (raw_fillers) {
-      raw_fillers[2 * (size_t)j] = rx;
-      raw_fillers[2 * (size_t)j + 1] = ry;
-    }
-    const double vx = 0.0 + coef[f] * rx, vy = 0.0 + coef[f] * ry;
-    grad[c] = vx;
-    grad[c + 1] = vy;
-    bad = !isfinite(vx) || !isfinite(vy);
-    if (metric) {
-      kkt = fmax(kkt_scaled(vx, x, lower[c], upper[c], metric[c]),
-                 kkt_scaled(vy, y, lower[c + 1], upper[c + 1], metric[c + 1]));
-    }
-  }
-  block_sum<2 * MAXF>(red, partials + (size_t)blockIdx.x * GF_STRIDE);
-  block_max_flag(kkt, bad, partials + (size_t)blockIdx.x * GF_STRIDE + 2 * MAXF);
-}
-
-// ---------------------------------------------------------------------------
-// Weighted-average wirelength (analytical_placement.rs:576-703), FP64.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double2 pin_coordinate(const WaArgs &a, const double *__restrict__ coords, uint32_t p) {
-  const uint32_t u = a.pin_unit[p];
-  const double2 v = a.pin_xy[p];
-  if (u == NO_UNIT) return v;
-  return make_double2(coords[2 * (size_t)u] + v.x, coords[2 * (size_t)u + 1] + v.y);
-}
-
-__device__ void wa_thread_nets(const WaArgs &a, const double *__restrict__ coords, double gamma,
-                               double *__restrict__ partials, int block) {
-  const int t = block * BLOCK + threadIdx.x;
-  double value[1] = {0.0};
-  if (t < a.n_small) {
-    const uint32_t net = a.small_nets[t];
-    const uint32_t b = a.net_start[net], e = a.net_start[net + 1];
-    const double w = a.net_weight[net];
-    double2 first = pin_coordinate(a, coords, b);
-    double xmin = first.x, xmax = first.x, ymin = first.y, ymax = first.y;
-    for (uint32_t p = b + 1; p < e; ++p) {
-      const double2 c = pin_coordinate(a, coords, p);
-      xmin = fmin(xmin, c.x);
-      xmax = fmax(xmax, c.x);
-      ymin = fmin(ymin, c.y);
-      ymax = fmax(ymax, c.y);
-    }
-    double psx = 0, pdx = 0, nsx = 0, ndx = 0, psy = 0, pdy = 0, nsy = 0, ndy = 0;
-    for (uint32_t p = b; p < e; ++p) {
-      const double2 c = pin_coordinate(a, coords, p);
-      const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
-      const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
-      const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
-      const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
-      const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
-      psx += ex;
-      pdx += ex * dxm;
-      nsx += enx;
-      ndx += enx * dxn;
-      psy += ey;
-      pdy += ey * dym;
-      nsy += eny;
-      ndy += eny * dyn;
-    }
-    const double pmx = pdx / psx, nmx = ndx / nsx, pmy = pdy / psy, nmy = ndy / nsy;
-    for (uint32_t p = b; p < e; ++p) {
-      const double2 c = pin_coordinate(a, coords, p);
-      const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
-      const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
-      const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
-      const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
-      const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
-      const double ddx = (ex / psx) * (1.0 + (dxm - pmx) / gamma) - (enx / nsx) * (1.0 - (dxn - nmx) / gamma);
-      const double ddy = (ey / psy) * (1.0 + (dym - pmy) / gamma) - (eny / nsy) * (1.0 - (dyn - nmy) / gamma);
-      a.dpin[p] = make_double2(w * ddx, w * ddy);
-    }
-    value[0] = w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
-  }
-  block_sum<1>(value, partials + blockIdx.x);
-}
-
-// One wavefront per net for high-fanout nets.
-__device__ void wa_wave_nets(const WaArgs &a, const double *__restrict__ coords, double gamma,
-                             double *__restrict__ partials, int block) {
-  const int lane = threadIdx.x & 63;
-  const int t = block * (BLOCK / 64) + (threadIdx.x >> 6);
-  double value[1] = {0.0};
-  if (t < a.n_large) {
-    const uint32_t net = a.large_nets[t];
-    const uint32_t b = a.net_start[net], e = a.net_start[net + 1];
-    const double w = a.net_weight[net];
-    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-    for (uint32_t p = b + lane; p < e; p += 64) {
-      const double2 c = pin_coordinate(a, coords, p);
-      xmin = fmin(xmin, c.x);
-      xmax = fmax(xmax, c.x);
-      ymin = fmin(ymin, c.y);
-      ymax = fmax(ymax, c.y);
-    }
-    xmin = wave_min(xmin);
-    xmax = wave_max(xmax);
-    ymin = wave_min(ymin);
-    ymax = wave_max(ymax);
-    double psx = 0, pdx = 0, nsx = 0, ndx = 0, psy = 0, pdy = 0, nsy = 0, ndy = 0;
-    for (uint32_t p = b + lane; p < e; p += 64) {
-      const double2 c = pin_coordinate(a, coords, p);
-      const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
-      const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
-      const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
-      const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
-      const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
-      psx += ex;
-      pdx += ex * dxm;
-      nsx += enx;
-      ndx += enx * dxn;
-      psy += ey;
-      pdy += ey * dym;
-      nsy += eny;
-      ndy += eny * dyn;
-    }
-    psx = wave_sum(psx);
-    pdx = wave_sum(pdx);
-    nsx = wave_sum(nsx);
-    ndx = wave_sum(ndx);
-    psy = wave_sum(psy);
-    pdy = wave_sum(pdy);
-    nsy = wave_sum(nsy);
-    ndy = wave_sum(ndy);
-    const double pmx = pdx / psx, nmx = ndx / nsx, pmy = pdy / psy, nmy = ndy / nsy;
-    for (uint32_t p = b + lane; p < e; p += 64) {
-      const double2 c = pin_coordinate(a, coords, p);
-      const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
-      const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
-      const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
-      const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
-      const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
-      const double ddx = (ex / psx) * (1.0 + (dxm - pmx) / gamma) - (enx / nsx) * (1.0 - (dxn - nmx) / gamma);
-      const double ddy = (ey / psy) * (1.0 + (dym - pmy) / gamma) - (eny / nsy) * (1.0 - (dyn - nmy) / gamma);
-      a.dpin[p] = make_double2(w * ddx, w * ddy);
-    }
-    if (lane == 0) value[0] = w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
-  }
-  block_sum<1>(value, partials + blockIdx.x);
-}
-
-// One launch for both shapes: the first `wave_blocks` blocks take the
-// high-fanout nets (long serial chains, scheduled first), the rest take one
-// small net per thread.  Block b writes partial b.
-__global__ void __launch_bounds__(BLOCK) k_wa(WaArgs a, const double *__restrict__ coords, double gamma,
-                                              double *__restrict__ partials, int wave_blocks) {
-  if ((int)blockIdx.x < wave_blocks)
-    wa_wave_nets(a, coords, gamma, partials, blockIdx.x);
-  else
-    wa_thread_nets(a, coords, gamma, partials, blockIdx.x - wave_blocks);
-}
-
-// Combined unit gradient (combined_objective, eplace.rs:1022-1044): WA
-// baseline pins, then timed-arc pins, then each field in order.
-__global__ void __launch_bounds__(BLOCK) k_unit_gradient(WaArgs a, int F, int DU, const double2 *__restrict__ gfu,
-                                                         const double *__restrict__ coords,
-                                                         const double *__restrict__ metric,
-                                                         const double *__restrict__ lower,
-                                                         const double *__restrict__ upper, double *__restrict__ grad,
-                                                         double *__restrict__ wa_grad_out,
-                                                         double *__restrict__ partials, EvalScalars *scalars) {
-  const int u = blockIdx.x * BLOCK + threadIdx.x;
-  double l1[1] = {0.0};
-  double kkt = 0.0;
-  bool bad = false;
-  if (u < a.n_units) {
-    double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
-    for (uint32_t k = a.u_pstart[u]; k < a.u_pstart[u + 1]; ++k) {
-      const uint32_t p = a.u_pins[k];
-      const double2 d = a.dpin[p];
-      if (p < a.first_arc_pin) {
-        sx += d.x;
-        sy += d.y;
-      } else {
-        tx += d.x;
-        ty += d.y;
-      }
-    }
-    double gx = sx + tx, gy = sy + ty;
-    l1[0] = fabs(gx) + fabs(gy);
-    if (wa_grad_out) {
-      wa_grad_out[2 * (size_t)u] = gx;
-      wa_grad_out[2 * (size_t)u + 1] = gy;
-    }
-    const int du = a.unit_du[u];
-    if (du >= 0) {
-      for (int f = 0; f < F; ++f) {
-        const double2 v = gfu[(size_t)f * DU + du];
-        gx += v.x;
-        gy += v.y;
-      }
-    }
-    const size_t c = 2 * (size_t)u;
-    grad[c] = gx;
-    grad[c + 1] = gy;
-    bad = !isfinite(gx) || !isfinite(gy);
-    if (metric) {
-      kkt = fmax(kkt_scaled(gx, coords[c], lower[c], upper[c], metric[c]),
-                 kkt_scaled(gy, coords[c + 1], lower[c + 1], upper[c + 1], metric[c + 1]));
-    }
-  }
-  block_sum<1>(l1, partials + (size_t)blockIdx.x * UG_STRIDE);
-  block_max_flag(kkt, bad, partials + (size_t)blockIdx.x * UG_STRIDE + 1);
-}
-
-// Single-block reductions of per-block partials.  Each thread accumulates a
-// fixed strided subset, waves reduce by shuffles and four wave results are
-// combined in wave order: a fixed order, so results are reproducible.
+// Single-block reductions of per-block partials: each thread accumulates a
+// fixed strided subset, waves reduce by shuffles, four wave results combine
+// in wave order.  The order is fixed, so results are reproducible.
 template <int K>
 __device__ void sum_partials(const double *__restrict__ p, int count, double (&out)[K], int stride, int offset) {
   __shared__ double s[K][BLOCK / 64];
@@ -562,76 +111,485 @@ __device__ void max_partials(const double *__restrict__ p, int count, double (&o
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(BLOCK) k_finalize_eval(FinalizeArgs a, EvalScalars *scalars) {
-  double v1[1];
-  sum_partials<1>(a.wa_partials, a.nb_wa, v1, 1, 0);
-  if (threadIdx.x == 0) scalars->wa_value = v1[0];
-  sum_partials<1>(a.ug_partials, a.nb_ug, v1, UG_STRIDE, 0);
-  if (threadIdx.x == 0) scalars->wa_l1 = v1[0];
-  double gu[2 * MAXF], gf[2 * MAXF], mu[2], mf[2];
-  sum_partials<2 * MAXF>(a.gu_partials, a.nb_gu, gu, 2 * MAXF, 0);
-  sum_partials<2 * MAXF>(a.gf_partials, a.nb_gf, gf, GF_STRIDE, 0);
-  max_partials<2>(a.ug_partials, a.nb_ug, mu, UG_STRIDE, 1);
-  max_partials<2>(a.gf_partials, a.nb_gf, mf, GF_STRIDE, 2 * MAXF);
-  if (threadIdx.x < MAXF) {
-    // the energy uses the same serial order as the coefficient computed in
-    // the gather kernels, so both agree bit for bit
-    const int f = threadIdx.x;
-    double e = 0.0;
-    for (int b = 0; b < a.nb_energy; ++b) e += a.energy_partials[b * MAXF + f];
-    scalars->energy[f] = e;
-    scalars->force_l1[f] = gu[f] + gf[f];
-    scalars->raw_l1[f] = gu[MAXF + f] + gf[MAXF + f];
+// kkt_component (global_placement.rs:675-689) divided by sqrt(metric).
+__device__ __forceinline__ double kkt_scaled(double g, double x, double lo, double hi, double m) {
+  const bool equal = __double_as_longlong(lo) == __double_as_longlong(hi);
+  if (equal || (x <= lo && g > 0.0) || (x >= hi && g < 0.0)) return 0.0;
+  return fabs(g) / sqrt(m);
+}
+
+// ---------------------------------------------------------------------------
+// Density deposit (electrostatic_placement.rs:296-332, 900-916).
+// ---------------------------------------------------------------------------
+// acc_m = -smoothed capacity, acc_f = 0; the task's int64 sums restart.
+__global__ void __launch_bounds__(BLOCK) k_deposit_init(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const Task &t = tasks[bm.x];
+  const int total = t.j.F * t.j.W * t.j.H;
+  const int i = bm.y * BLOCK + threadIdx.x;
+  if (i < total) {
+    t.acc_m[i] = t.j.base_fx[i];
+    t.acc_f[i] = 0;
   }
-  if (threadIdx.x == 0) {
-    const double kkt = fmax(mu[0], mf[0]);
-    scalars->kkt_bits = static_cast<unsigned long long>(__double_as_longlong(kkt));
-    scalars->nonfinite = (mu[1] > 0.0 || mf[1] > 0.0) ? 1u : 0u;
+  if (bm.y == 0 && threadIdx.x < MAXF) {
+    t.scalars->over_fx[threadIdx.x] = 0;
+    t.scalars->net_fx[threadIdx.x] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const Task &t = tasks[bm.x];
+  const JobStatic &a = t.j;
+  const int i = bm.y * BLOCK + threadIdx.x;
+  double x, y, q;
+  int f;
+  long long *acc;
+  if (i < a.n_members) {
+    const uint32_t u = a.m_unit[i];
+    const double2 off = a.m_off[i];
+    x = t.coords[2 * (size_t)u] + off.x;
+    y = t.coords[2 * (size_t)u + 1] + off.y;
+    q = a.m_charge[i];
+    f = a.m_field[i];
+    acc = t.acc_m;
+  } else if (i < a.n_members + a.n_fillers) {
+    const int k = i - a.n_members;
+    const size_t c = 2 * ((size_t)a.n_units + k);
+    x = t.coords[c];
+    y = t.coords[c + 1];
+    q = a.fl_charge[k];
+    f = a.fl_field[k];
+    acc = t.acc_f;
+  } else {
+    return;
+  }
+  Axis ax, ay;
+  bspline_axis(a.W, x, ax);
+  bspline_axis(a.H, y, ay);
+  unsigned long long *g = reinterpret_cast<unsigned long long *>(acc + (size_t)f * (size_t)a.W * (size_t)a.H);
+  const int ix[3] = {ax.i0, ax.i1, ax.i2};
+  const int iy[3] = {ay.i0, ay.i1, ay.i2};
+  const double wx[3] = {ax.w0, ax.w1, ax.w2};
+  const double wy[3] = {ay.w0, ay.w1, ay.w2};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      // charge * column.weight * row.weight, quantised before the add
+      const double v = q * wx[c] * wy[r];
+      atomicAdd(g + (size_t)iy[r] * a.W + ix[c], static_cast<unsigned long long>(__double2ll_rn(v * FX_SCALE)));
+    }
+  }
+}
+
+// rho = members + fillers -> FP32 transform input; physical overflow (before
+// fillers) and net charge as exact int64 sums.
+__global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+  __shared__ unsigned long long s_over[MAXF], s_net[MAXF];
+  const int2 bm = map[blockIdx.x];
+  const Task &t = tasks[bm.x];
+  if (threadIdx.x < MAXF) {
+    s_over[threadIdx.x] = 0;
+    s_net[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  const int G = t.j.W * t.j.H;
+  const int i = bm.y * BLOCK + threadIdx.x;
+  if (i < t.j.F * G) {
+    const int f = i / G;
+    const long long m = t.acc_m[i];
+    const long long v = m + t.acc_f[i];
+    const double d = static_cast<double>(v) * FX_INV;
+    t.rho[i] = static_cast<float>(d);
+    if (t.density_out) t.density_out[i] = d;
+    if (m > 0) atomicAdd(&s_over[f], static_cast<unsigned long long>(m));
+    atomicAdd(&s_net[f], static_cast<unsigned long long>(v));
+  }
+  __syncthreads();
+  if (threadIdx.x < t.j.F) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->over_fx[threadIdx.x]), s_over[threadIdx.x]);
+    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->net_fx[threadIdx.x]), s_net[threadIdx.x]);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Preconditioner and Nesterov elementwise steps.
+// Field gradient gather (electrostatic_placement.rs:338-374, 919-941).
 // ---------------------------------------------------------------------------
-// diagonal_preconditioner (eplace.rs:1046-1100)
-__global__ void __launch_bounds__(BLOCK) k_preconditioner(int n_units, int n_fillers, int F, int DU,
-                                                          const double *__restrict__ wl_diag,
-                                                          const int32_t *__restrict__ unit_du,
-                                                          const double *__restrict__ du_fcharge,
-                                                          const uint8_t *__restrict__ fl_field,
-                                                          const double *__restrict__ fl_charge, double l0, double l1,
-                                                          double l2, double l3, double *__restrict__ metric,
-                                                          double *__restrict__ partials) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  const double lambda[MAXF] = {l0, l1, l2, l3};
+// Each block re-reduces the transform's energy partials in one fixed serial
+// order and forms lambda (1 + beta E/E0) (eplace.rs:1111-1124).
+__device__ __forceinline__ void load_coefficients(const Task &t, double *coef) {
+  if (threadIdx.x < t.j.F) {
+    const int f = threadIdx.x;
+    double e = 0.0;
+    for (int b = 0; b < t.nb_energy; ++b) e += t.part_energy[b * MAXF + f];
+    coef[f] = t.lambda[f] * (1.0 + AUGMENTED_DENSITY_BETA * (e / t.scale[f]));
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void sample(const float *__restrict__ phi, int W, int H, double x, double y, double &gx,
+                                       double &gy) {
+  Axis ax, ay;
+  bspline_axis(W, x, ax);
+  bspline_axis(H, y, ay);
+  const int ix[3] = {ax.i0, ax.i1, ax.i2};
+  const int iy[3] = {ay.i0, ay.i1, ay.i2};
+  const double wx[3] = {ax.w0, ax.w1, ax.w2};
+  const double wy[3] = {ay.w0, ay.w1, ay.w2};
+  const double dx[3] = {ax.d0, ax.d1, ax.d2};
+  const double dy[3] = {ay.d0, ay.d1, ay.d2};
+  gx = 0.0;
+  gy = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double p = static_cast<double>(phi[(size_t)iy[r] * W + ix[c]]);
+      gx += dx[c] * wy[r] * p;
+      gy += wx[c] * dy[r] * p;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_gather_units(const Task *__restrict__ tasks,
+                                                        const int2 *__restrict__ map) {
+  __shared__ double coef[MAXF];
+  const int2 bm = map[blockIdx.x];
+  const Task &t = tasks[bm.x];
+  const JobStatic &a = t.j;
+  load_coefficients(t, coef);
+  const int i = bm.y * BLOCK + threadIdx.x;
+  const size_t G = (size_t)a.W * a.H;
+  double red[2 * MAXF];
+#pragma unroll
+  for (int k = 0; k < 2 * MAXF; ++k) red[k] = 0.0;
+  if (i < a.n_density_units) {
+    const uint32_t u = a.du_unit[i];
+    const double ox = t.coords[2 * (size_t)u], oy = t.coords[2 * (size_t)u + 1];
+    double gxf[MAXF], gyf[MAXF];
+#pragma unroll
+    for (int f = 0; f < MAXF; ++f) gxf[f] = gyf[f] = 0.0;
+    for (uint32_t m = a.du_mstart[i]; m < a.du_mstart[i + 1]; ++m) {
+      const int f = a.m_field[m];
+      const double2 off = a.m_off[m];
+      const double q = a.m_charge[m];
+      double gx, gy;
+      sample(t.phi + f * G, a.W, a.H, ox + off.x, oy + off.y, gx, gy);
+#pragma unroll
+      for (int k = 0; k < MAXF; ++k) {
+        if (k == f) {
+          red[k] += q * (fabs(gx) + fabs(gy));
+          gxf[k] += q * gx;
+          gyf[k] += q * gy;
+        }
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < MAXF; ++f) {
+      if (f < a.F) {
+        red[MAXF + f] = fabs(gxf[f]) + fabs(gyf[f]);
+        t.gfu[(size_t)f * a.n_density_units + i] = make_double2(coef[f] * gxf[f], coef[f] * gyf[f]);
+        if (t.raw_units) {
+          t.raw_units[2 * ((size_t)f * a.n_density_units + i)] = gxf[f];
+          t.raw_units[2 * ((size_t)f * a.n_density_units + i) + 1] = gyf[f];
+        }
+      }
+    }
+  }
+  block_sum<2 * MAXF>(red, t.part_gu + (size_t)bm.y * 2 * MAXF);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_gather_fillers(const Task *__restrict__ tasks,
+                                                          const int2 *__restrict__ map) {
+  __shared__ double coef[MAXF];
+  const int2 bm = map[blockIdx.x];
+  const Task &t = tasks[bm.x];
+  const JobStatic &a = t.j;
+  load_coefficients(t, coef);
+  const int k = bm.y * BLOCK + threadIdx.x;
+  const size_t G = (size_t)a.W * a.H;
+  double red[2 * MAXF];
+#pragma unroll
+  for (int q = 0; q < 2 * MAXF; ++q) red[q] = 0.0;
+  double kkt = 0.0;
+  bool bad = false;
+  if (k < a.n_fillers) {
+    const size_t c = 2 * ((size_t)a.n_units + k);
+    const double x = t.coords[c], y = t.coords[c + 1];
+    const int f = a.fl_field[k];
+    const double q = a.fl_charge[k];
+    double gx, gy;
+    sample(t.phi + f * G, a.W, a.H, x, y, gx, gy);
+    const double rx = q * gx, ry = q * gy;
+#pragma unroll
+    for (int p = 0; p < MAXF; ++p) {
+      if (p == f) {
+        red[p] = q * (fabs(gx) + fabs(gy));
+        red[MAXF + p] = fabs(rx) + fabs(ry);
+      }
+    }
+    if (t.raw_fillers) {
+      t.raw_fillers[2 * (size_t)k] = rx;
+      t.raw_fillers[2 * (size_t)k + 1] = ry;
+    }
+    // the combined gradient of a filler starts at 0.0 (eplace.rs:1031-1041)
+    const double vx = 0.0 + coef[f] * rx, vy = 0.0 + coef[f] * ry;
+    t.grad[c] = vx;
+    t.grad[c + 1] = vy;
+    bad = !isfinite(vx) || !isfinite(vy);
+    if (t.metric) {
+      kkt = fmax(kkt_scaled(vx, x, a.lower[c], a.upper[c], t.metric[c]),
+                 kkt_scaled(vy, y, a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
+    }
+  }
+  block_sum<2 * MAXF>(red, t.part_gf + (size_t)bm.y * GF_STRIDE);
+  block_max_flag(kkt, bad, t.part_gf + (size_t)bm.y * GF_STRIDE + 2 * MAXF);
+}
+
+// ---------------------------------------------------------------------------
+// Weighted-average wirelength (analytical_placement.rs:576-703), FP64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 pin_coordinate(const JobStatic &a, const double *__restrict__ coords, uint32_t p) {
+  const uint32_t u = a.pin_unit[p];
+  const double2 v = a.pin_xy[p];
+  if (u == NO_UNIT) return v;
+  return make_double2(coords[2 * (size_t)u] + v.x, coords[2 * (size_t)u + 1] + v.y);
+}
+
+struct WaSums {
+  double psx, pdx, nsx, ndx, psy, pdy, nsy, ndy;
+};
+
+__device__ __forceinline__ void wa_accumulate(double2 c, double xmin, double xmax, double ymin, double ymax,
+                                              double gamma, WaSums &s) {
+  const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
+  // exactly 1.0 where the delta is zero (analytical_placement.rs:666-678)
+  const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
+  const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
+  const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
+  const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
+  s.psx += ex;
+  s.pdx += ex * dxm;
+  s.nsx += enx;
+  s.ndx += enx * dxn;
+  s.psy += ey;
+  s.pdy += ey * dym;
+  s.nsy += eny;
+  s.ndy += eny * dyn;
+}
+
+__device__ __forceinline__ double2 wa_derivative(double2 c, double xmin, double xmax, double ymin, double ymax,
+                                                 double gamma, const WaSums &s, double pmx, double nmx, double pmy,
+                                                 double nmy) {
+  const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
+  const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
+  const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
+  const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
+  const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
+  const double ddx = (ex / s.psx) * (1.0 + (dxm - pmx) / gamma) - (enx / s.nsx) * (1.0 - (dxn - nmx) / gamma);
+  const double ddy = (ey / s.psy) * (1.0 + (dym - pmy) / gamma) - (eny / s.nsy) * (1.0 - (dyn - nmy) / gamma);
+  return make_double2(ddx, ddy);
+}
+
+// Thread per net (<= 16 pins).
+__device__ double wa_thread_net(const Task &t, int local) {
+  const JobStatic &a = t.j;
+  if (local >= a.n_small) return 0.0;
+  const uint32_t net = a.small_nets[local];
+  const uint32_t b = a.net_start[net], e = a.net_start[net + 1];
+  const double w = a.net_weight[net];
+  const double gamma = t.gamma;
+  const double2 first = pin_coordinate(a, t.coords, b);
+  double xmin = first.x, xmax = first.x, ymin = first.y, ymax = first.y;
+  for (uint32_t p = b + 1; p < e; ++p) {
+    const double2 c = pin_coordinate(a, t.coords, p);
+    xmin = fmin(xmin, c.x);
+    xmax = fmax(xmax, c.x);
+    ymin = fmin(ymin, c.y);
+    ymax = fmax(ymax, c.y);
+  }
+  WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t p = b; p < e; ++p) wa_accumulate(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s);
+  const double pmx = s.pdx / s.psx, nmx = s.ndx / s.nsx, pmy = s.pdy / s.psy, nmy = s.ndy / s.nsy;
+  for (uint32_t p = b; p < e; ++p) {
+    const double2 d =
+        wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
+    t.dpin[p] = make_double2(w * d.x, w * d.y);
+  }
+  return w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
+}
+
+// One wavefront per high-fanout net; lane 0 returns the value.
+__device__ double wa_wave_net(const Task &t, int local) {
+  const JobStatic &a = t.j;
+  const int lane = threadIdx.x & 63;
+  if (local >= a.n_large) return 0.0;
+  const uint32_t net = a.large_nets[local];
+  const uint32_t b = a.net_start[net], e = a.net_start[net + 1];
+  const double w = a.net_weight[net];
+  const double gamma = t.gamma;
+  double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+  for (uint32_t p = b + lane; p < e; p += 64) {
+    const double2 c = pin_coordinate(a, t.coords, p);
+    xmin = fmin(xmin, c.x);
+    xmax = fmax(xmax, c.x);
+    ymin = fmin(ymin, c.y);
+    ymax = fmax(ymax, c.y);
+  }
+  xmin = wave_min(xmin);
+  xmax = wave_max(xmax);
+  ymin = wave_min(ymin);
+  ymax = wave_max(ymax);
+  WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t p = b + lane; p < e; p += 64)
+    wa_accumulate(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s);
+  s.psx = wave_sum(s.psx);
+  s.pdx = wave_sum(s.pdx);
+  s.nsx = wave_sum(s.nsx);
+  s.ndx = wave_sum(s.ndx);
+  s.psy = wave_sum(s.psy);
+  s.pdy = wave_sum(s.pdy);
+  s.nsy = wave_sum(s.nsy);
+  s.ndy = wave_sum(s.ndy);
+  const double pmx = s.pdx / s.psx, nmx = s.ndx / s.nsx, pmy = s.pdy / s.psy, nmy = s.ndy / s.nsy;
+  for (uint32_t p = b + lane; p < e; p += 64) {
+    const double2 d =
+        wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
+    t.dpin[p] = make_double2(w * d.x, w * d.y);
+  }
+  return lane == 0 ? w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy) : 0.0;
+}
+
+// Per task: local blocks [0, nb_wa_large) take high-fanout nets (long serial
+// chains, scheduled first), the rest one small net per thread.
+__global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const Task &t = tasks[bm.x];
+  double value[1];
+  if (bm.y < t.nb_wa_large)
+    value[0] = wa_wave_net(t, bm.y * (BLOCK / 64) + (threadIdx.x >> 6));
+  else
+    value[0] = wa_thread_net(t, (bm.y - t.nb_wa_large) * BLOCK + threadIdx.x);
+  block_sum<1>(value, t.part_wa + bm.y);
+}
+
+// Combined unit gradient (combined_objective, eplace.rs:1022-1044): WA
+// baseline pins, then timed-arc pins, then each field in order.
+__global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict__ tasks,
+                                                         const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const Task &t = tasks[bm.x];
+  const JobStatic &a = t.j;
+  const int u = bm.y * BLOCK + threadIdx.x;
+  double l1[1] = {0.0};
+  double kkt = 0.0;
+  bool bad = false;
+  if (u < a.n_units) {
+    double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
+    for (uint32_t k = a.u_pstart[u]; k < a.u_pstart[u + 1]; ++k) {
+      const uint32_t p = a.u_pins[k];
+      const double2 d = t.dpin[p];
+      if (p < a.first_arc_pin) {
+        sx += d.x;
+        sy += d.y;
+      } else {
+        tx += d.x;
+        ty += d.y;
+      }
+    }
+    double gx = sx + tx, gy = sy + ty;
+    l1[0] = fabs(gx) + fabs(gy);
+    if (t.wa_grad_out) {
+      t.wa_grad_out[2 * (size_t)u] = gx;
+      t.wa_grad_out[2 * (size_t)u + 1] = gy;
+    }
+    const int du = a.unit_du[u];
+    if (du >= 0) {
+      for (int f = 0; f < a.F; ++f) {
+        const double2 v = t.gfu[(size_t)f * a.n_density_units + du];
+        gx += v.x;
+        gy += v.y;
+      }
+    }
+    const size_t c = 2 * (size_t)u;
+    t.grad[c] = gx;
+    t.grad[c + 1] = gy;
+    bad = !isfinite(gx) || !isfinite(gy);
+    if (t.metric) {
+      kkt = fmax(kkt_scaled(gx, t.coords[c], a.lower[c], a.upper[c], t.metric[c]),
+                 kkt_scaled(gy, t.coords[c + 1], a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
+    }
+  }
+  block_sum<1>(l1, t.part_ug + (size_t)bm.y * UG_STRIDE);
+  block_max_flag(kkt, bad, t.part_ug + (size_t)bm.y * UG_STRIDE + 1);
+}
+
+// One block per task: every FP64 scalar of the evaluation.
+__global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tasks) {
+  const Task &t = tasks[blockIdx.x];
+  EvalScalars *sc = t.scalars;
+  double v1[1];
+  sum_partials<1>(t.part_wa, t.nb_wa_small + t.nb_wa_large, v1, 1, 0);
+  if (threadIdx.x == 0) sc->wa_value = v1[0];
+  sum_partials<1>(t.part_ug, t.nb_ug, v1, UG_STRIDE, 0);
+  if (threadIdx.x == 0) sc->wa_l1 = v1[0];
+  double gu[2 * MAXF], gf[2 * MAXF], mu[2], mf[2];
+  sum_partials<2 * MAXF>(t.part_gu, t.nb_gu, gu, 2 * MAXF, 0);
+  sum_partials<2 * MAXF>(t.part_gf, t.nb_gf, gf, GF_STRIDE, 0);
+  max_partials<2>(t.part_ug, t.nb_ug, mu, UG_STRIDE, 1);
+  max_partials<2>(t.part_gf, t.nb_gf, mf, GF_STRIDE, 2 * MAXF);
+  if (threadIdx.x < MAXF) {
+    // same serial order as the coefficient in the gather kernels
+    const int f = threadIdx.x;
+    double e = 0.0;
+    for (int b = 0; b < t.nb_energy; ++b) e += t.part_energy[b * MAXF + f];
+    sc->energy[f] = f < t.j.F ? e : 0.0;
+    sc->force_l1[f] = gu[f] + gf[f];
+    sc->raw_l1[f] = gu[MAXF + f] + gf[MAXF + f];
+  }
+  if (threadIdx.x == 0) {
+    sc->kkt_bits = static_cast<unsigned long long>(__double_as_longlong(fmax(mu[0], mf[0])));
+    sc->nonfinite = (mu[1] > 0.0 || mf[1] > 0.0) ? 1u : 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Optimizer elementwise kernels.
+// ---------------------------------------------------------------------------
+// diagonal_preconditioner (eplace.rs:1046-1100); per-block max/min/flag.
+__global__ void __launch_bounds__(BLOCK) k_preconditioner(const OptTask *__restrict__ tasks,
+                                                          const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const OptTask &t = tasks[bm.x];
+  const int i = bm.y * BLOCK + threadIdx.x;
   double d = 0.0;
   bool valid = false;
-  if (i < n_units) {
-    d = wl_diag[i];
-    const int du = unit_du[i];
+  if (i < t.n_units) {
+    d = t.wl_diag[i];
+    const int du = t.unit_du[i];
     if (du >= 0) {
-      for (int f = 0; f < F; ++f) {
-        const double charge = du_fcharge[(size_t)f * DU + du];
-        if (charge != 0.0) d += lambda[f] * charge;
+      for (int f = 0; f < t.F; ++f) {
+        const double charge = t.du_fcharge[(size_t)f * t.DU + du];
+        if (charge != 0.0) d += t.lambda[f] * charge;
       }
     }
     valid = true;
-  } else if (i < n_units + n_fillers) {
-    const int j = i - n_units;
-    d = 0.0 + lambda[fl_field[j]] * fl_charge[j];
+  } else if (i < t.n_units + t.n_fillers) {
+    const int k = i - t.n_units;
+    d = 0.0 + t.lambda[t.fl_field[k]] * t.fl_charge[k];
     valid = true;
   }
   double lo = INFINITY, hi = 0.0;
   bool bad = false;
   if (valid) {
     bad = !isfinite(d) || d < 0.0;
-    d = fmax(d, 1.0);
-    metric[2 * (size_t)i] = d;
-    metric[2 * (size_t)i + 1] = d;
+    d = fmax(d, 1.0);  // elfPlace eq. (16) floor
+    t.metric[2 * (size_t)i] = d;
+    t.metric[2 * (size_t)i + 1] = d;
     lo = d;
     hi = d;
   }
-  // block max / -min / flag, reduced by k_metric_finalize
   __shared__ double smax[BLOCK / 64], smin[BLOCK / 64];
   __shared__ int sbad[BLOCK / 64];
   lo = wave_min(lo);
@@ -645,112 +603,114 @@ __global__ void __launch_bounds__(BLOCK) k_preconditioner(int n_units, int n_fil
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double mx = 0.0, mn = INFINITY;
-    int b = 0;
-    for (int w = 0; w < BLOCK / 64; ++w) {
-      mx = fmax(mx, smax[w]);
-      mn = fmin(mn, smin[w]);
-      b |= sbad[w];
-    }
-    partials[3 * (size_t)blockIdx.x] = mx;
-    partials[3 * (size_t)blockIdx.x + 1] = mn;
-    partials[3 * (size_t)blockIdx.x + 2] = b ? 1.0 : 0.0;
+    double *p = t.part_metric + 3 * (size_t)bm.y;
+    p[0] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+    p[1] = fmin(fmin(smin[0], smin[1]), fmin(smin[2], smin[3]));
+    p[2] = (sbad[0] | sbad[1] | sbad[2] | sbad[3]) ? 1.0 : 0.0;
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_metric_finalize(const double *__restrict__ partials, int nb,
-                                                           MetricScalars *ms) {
-  __shared__ double smax[BLOCK], smin[BLOCK], sbad[BLOCK];
+__global__ void __launch_bounds__(BLOCK) k_metric_finalize(const OptTask *__restrict__ tasks,
+                                                           const int *__restrict__ list) {
+  const OptTask &t = tasks[list[blockIdx.x]];
+  const int nb = (t.n_units + t.n_fillers + BLOCK - 1) / BLOCK;
+  __shared__ double smax[BLOCK / 64], smin[BLOCK / 64], sbad[BLOCK / 64];
   double mx = 0.0, mn = INFINITY, b = 0.0;
   for (int i = threadIdx.x; i < nb; i += BLOCK) {
-    mx = fmax(mx, partials[3 * (size_t)i]);
-    mn = fmin(mn, partials[3 * (size_t)i + 1]);
-    b = fmax(b, partials[3 * (size_t)i + 2]);
+    mx = fmax(mx, t.part_metric[3 * (size_t)i]);
+    mn = fmin(mn, t.part_metric[3 * (size_t)i + 1]);
+    b = fmax(b, t.part_metric[3 * (size_t)i + 2]);
   }
-  smax[threadIdx.x] = mx;
-  smin[threadIdx.x] = mn;
-  sbad[threadIdx.x] = b;
+  mx = wave_max(mx);
+  mn = wave_min(mn);
+  b = wave_max(b);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    smax[wave] = mx;
+    smin[wave] = mn;
+    sbad[wave] = b;
+  }
   __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
-      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
-      sbad[threadIdx.x] = fmax(sbad[threadIdx.x], sbad[threadIdx.x + w]);
-    }
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    ms->max_bits = static_cast<unsigned long long>(__double_as_longlong(smax[0]));
-    ms->min_bits = static_cast<unsigned long long>(__double_as_longlong(smin[0]));
-    ms->nonfinite = sbad[0] > 0.0 ? 1u : 0u;
+    t.metric_scalars->max_bits = static_cast<unsigned long long>(
+        __double_as_longlong(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]))));
+    t.metric_scalars->min_bits = static_cast<unsigned long long>(
+        __double_as_longlong(fmin(fmin(smin[0], smin[1]), fmin(smin[2], smin[3]))));
+    t.metric_scalars->nonfinite = fmax(fmax(sbad[0], sbad[1]), fmax(sbad[2], sbad[3])) > 0.0 ? 1u : 0u;
+    t.step->differs = 0u;  // reset before this round's extrapolation
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_extrapolate(size_t n, const double *__restrict__ x,
-                                                       const double *__restrict__ xp,
-                                                       const double *__restrict__ lower,
-                                                       const double *__restrict__ upper, double beta,
-                                                       double *__restrict__ y, unsigned int *differs) {
-  const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(BLOCK) k_extrapolate(const OptTask *__restrict__ tasks,
+                                                       const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const OptTask &t = tasks[bm.x];
+  const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
   bool changed = false;
-  if (i < n) {
-    const double cur = x[i];
-    double v = cur + beta * (cur - xp[i]);
-    const double lo = lower[i], hi = upper[i];
+  if (i < t.n) {
+    const double cur = t.x[i];
+    double v = cur + t.beta * (cur - t.xp[i]);
+    const double lo = t.lower[i], hi = t.upper[i];
     v = v < lo ? lo : (v > hi ? hi : v);
-    y[i] = v;
+    t.y[i] = v;
     changed = v != cur;
   }
-  if (__syncthreads_or(changed) && threadIdx.x == 0) *differs = 1u;
+  // one plain store per block that saw a change (no atomics)
+  if (__syncthreads_or(changed) && threadIdx.x == 0) t.step->differs = 1u;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_candidate(size_t n, const double *__restrict__ y,
-                                                     const double *__restrict__ gy,
-                                                     const double *__restrict__ metric, double lipschitz,
-                                                     const double *__restrict__ lower,
-                                                     const double *__restrict__ upper,
-                                                     const double *__restrict__ origin, double *__restrict__ cand,
-                                                     double *__restrict__ partials) {
-  const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(BLOCK) k_candidate(const OptTask *__restrict__ tasks,
+                                                     const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const OptTask &t = tasks[bm.x];
+  const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
   double red[2] = {0.0, 0.0};
   double change = 0.0;
   bool changed = false;
-  if (i < n) {
-    const double m = metric[i];
-    const double yi = y[i], g = gy[i];
-    double c = yi - g / (lipschitz * m);
-    const double lo = lower[i], hi = upper[i];
+  if (i < t.n) {
+    const double m = t.metric[i];
+    const double yi = t.yp[i], g = t.gy[i];
+    double c = yi - g / (t.lipschitz * m);
+    const double lo = t.lower[i], hi = t.upper[i];
     c = c < lo ? lo : (c > hi ? hi : c);
-    cand[i] = c;
+    t.cand[i] = c;
     changed = c != yi;
     const double d = c - yi;
     red[0] = g * d;
     red[1] = m * d * d;
-    change = fabs(c - origin[i]);
+    change = fabs(c - t.origin[i]);
   }
-  block_sum<2>(red, partials + 4 * (size_t)blockIdx.x);
-  block_max_flag(change, changed, partials + 4 * (size_t)blockIdx.x + 2);
+  block_sum<2>(red, t.part_step + 4 * (size_t)bm.y);
+  block_max_flag(change, changed, t.part_step + 4 * (size_t)bm.y + 2);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_step_finalize(const double *__restrict__ partials, int nb,
-                                                         StepScalars *step) {
+__global__ void __launch_bounds__(BLOCK) k_step_finalize(const OptTask *__restrict__ tasks,
+                                                         const int *__restrict__ list) {
+  const OptTask &t = tasks[list[blockIdx.x]];
+  const int nb = (int)((t.n + BLOCK - 1) / BLOCK);
   double v[2], m[2];
-  sum_partials<2>(partials, nb, v, 4, 0);
-  max_partials<2>(partials, nb, m, 4, 2);
+  sum_partials<2>(t.part_step, nb, v, 4, 0);
+  max_partials<2>(t.part_step, nb, m, 4, 2);
   if (threadIdx.x == 0) {
-    step->gdelta = v[0];
-    step->msq = v[1];
-    step->change_bits = static_cast<unsigned long long>(__double_as_longlong(m[0]));
-    step->changed = m[1] > 0.0 ? 1u : 0u;
+    t.step->gdelta = v[0];
+    t.step->msq = v[1];
+    t.step->change_bits = static_cast<unsigned long long>(__double_as_longlong(m[0]));
+    t.step->changed = m[1] > 0.0 ? 1u : 0u;
   }
 }
 
-__global__ void k_zero_step(StepScalars *step, unsigned int *flag) {
-  step->gdelta = 0.0;
-  step->msq = 0.0;
-  step->change_bits = 0;
-  step->changed = 0;
-  if (flag) *flag = 0;
+// rounded_targets_from_coordinates (eplace.rs:954-971, lib.rs:4566-4568)
+__global__ void __launch_bounds__(BLOCK) k_round_targets(const OptTask *__restrict__ tasks,
+                                                         const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  const OptTask &t = tasks[bm.x];
+  const int i = bm.y * BLOCK + threadIdx.x;
+  if (i >= 2 * t.n_units) return;
+  const double extent = static_cast<double>(((i & 1) ? t.H : t.W) - 1);
+  double r = round(t.round_src[i]);
+  if (!(r >= 0.0)) r = 0.0;
+  if (r > extent) r = extent;
+  t.targets[i] = static_cast<uint32_t>(r);
 }
 
 // ---------------------------------------------------------------------------
@@ -758,9 +718,9 @@ __global__ void k_zero_step(StepScalars *step, unsigned int *flag) {
 // bounding box, then stride over the candidate tiles.  The reference scans
 // every tile and keeps those whose overlap with [min-1/2, max+1/2] is
 // positive; that set lies inside t in (min-1, max+1), so the kernel visits
-// floor(min-1)..ceil(max+1) and applies the reference's own overlap formula,
-// which keeps the tile ranges and overlap values bit-exact.  Demand is
-// accumulated in 2^-40 fixed point so the sums are run-to-run identical.
+// floor(min-1)..ceil(max+1) and applies the reference's overlap formula,
+// which keeps tile ranges and overlaps bit-exact.  Demand accumulates in
+// 2^-40 fixed point so sums are run-to-run identical.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double tile_overlap(int t, double rlo, double rhi) {
   const double tmin = static_cast<double>(t) - 0.5, tmax = static_cast<double>(t) + 0.5;
@@ -800,7 +760,7 @@ __global__ void __launch_bounds__(BLOCK) k_rudy(int n_nets, const uint32_t *__re
   xmax = wave_max(xmax);
   ymin = wave_min(ymin);
   ymax = wave_max(ymax);
-  if (__all(same)) return;
+  if (__all(same)) return;  // a net inside one unit is not routing demand
   if (!isfinite(xmin) || !isfinite(xmax) || !isfinite(ymin) || !isfinite(ymax)) {
     if (lane == 0) *error = 1u;
     return;
@@ -809,10 +769,10 @@ __global__ void __launch_bounds__(BLOCK) k_rudy(int n_nets, const uint32_t *__re
   const double area = (dx + 1.0) * (dy + 1.0);
   const double hden = dx / area, vden = dy / area;
   const double rx0 = xmin - 0.5, rx1 = xmax + 0.5, ry0 = ymin - 0.5, ry1 = ymax + 0.5;
-  int x0 = static_cast<int>(fmax(floor(xmin - 1.0), 0.0));
-  int x1 = static_cast<int>(fmin(ceil(xmax + 1.0), static_cast<double>(W - 1)));
-  int y0 = static_cast<int>(fmax(floor(ymin - 1.0), 0.0));
-  int y1 = static_cast<int>(fmin(ceil(ymax + 1.0), static_cast<double>(H - 1)));
+  const int x0 = static_cast<int>(fmax(floor(xmin - 1.0), 0.0));
+  const int x1 = static_cast<int>(fmin(ceil(xmax + 1.0), static_cast<double>(W - 1)));
+  const int y0 = static_cast<int>(fmax(floor(ymin - 1.0), 0.0));
+  const int y1 = static_cast<int>(fmin(ceil(ymax + 1.0), static_cast<double>(H - 1)));
   const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
   if (nx <= 0 || ny <= 0) return;
   const long long tiles = (long long)nx * ny;
@@ -843,18 +803,6 @@ __global__ void __launch_bounds__(BLOCK) k_fx_to_double(size_t n, const long lon
   if (i < n) out[i] = static_cast<double>(fx[i]) * FX_INV;
 }
 
-// rounded_targets_from_coordinates (eplace.rs:954-971, lib.rs:4566-4568)
-__global__ void __launch_bounds__(BLOCK) k_round_targets(int n_units, const double *__restrict__ coords, int W, int H,
-                                                         uint32_t *__restrict__ out) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= 2 * n_units) return;
-  const double extent = static_cast<double>(((i & 1) ? H : W) - 1);
-  double r = round(coords[i]);
-  if (!(r >= 0.0)) r = 0.0;
-  if (r > extent) r = extent;
-  out[i] = static_cast<uint32_t>(r);
-}
-
 int blocks_for(size_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
 
 }  // namespace
@@ -862,92 +810,51 @@ int blocks_for(size_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-void launch_zero_eval(EvalScalars *scalars, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_eval, dim3(1), dim3(1), 0, s, scalars);
-}
-
-void launch_deposit(const DensityArgs &a, const double *coords, EvalScalars *scalars, hipStream_t s) {
-  const int total = a.F * a.W * a.H;
-  hipLaunchKernelGGL(k_deposit_init, dim3(blocks_for(total)), dim3(BLOCK), 0, s, total, a.base_fx, a.acc_m, a.acc_f,
-                     scalars);
-  const int n = a.n_members + a.n_fillers;
-  if (n > 0) {
+void launch_density(const Task *tasks, const Maps &m, hipStream_t s) {
+  if (m.nb_convert > 0) hipLaunchKernelGGL(k_deposit_init, dim3(m.nb_convert), dim3(BLOCK), 0, s, tasks, m.convert);
+  if (m.nb_deposit > 0) {
     timer_begin(g_timer, "deposit", s);
-    hipLaunchKernelGGL(k_deposit, dim3(blocks_for(n)), dim3(BLOCK), 0, s, a, coords);
+    hipLaunchKernelGGL(k_deposit, dim3(m.nb_deposit), dim3(BLOCK), 0, s, tasks, m.deposit);
     timer_end(g_timer, "deposit", s);
   }
+  if (m.nb_convert > 0) hipLaunchKernelGGL(k_convert, dim3(m.nb_convert), dim3(BLOCK), 0, s, tasks, m.convert);
 }
 
-void launch_density_convert(const DensityArgs &a, float *rho, double *density_out, EvalScalars *scalars,
-                            hipStream_t s) {
-  const int G = a.W * a.H;
-  hipLaunchKernelGGL(k_density_convert, dim3(blocks_for((size_t)a.F * G)), dim3(BLOCK), 0, s, G, a.F,
-                     a.acc_m, a.acc_f, rho, density_out, scalars);
+void launch_gather(const Task *tasks, const Maps &m, hipStream_t s) {
+  if (m.nb_gather_units > 0) {
+    timer_begin(g_timer, "gather", s);
+    hipLaunchKernelGGL(k_gather_units, dim3(m.nb_gather_units), dim3(BLOCK), 0, s, tasks, m.gather_units);
+    timer_end(g_timer, "gather", s);
+  }
+  if (m.nb_gather_fillers > 0)
+    hipLaunchKernelGGL(k_gather_fillers, dim3(m.nb_gather_fillers), dim3(BLOCK), 0, s, tasks, m.gather_fillers);
 }
 
-void launch_gather(const DensityArgs &a, const GatherArgs &g, const double *coords, double2 *gfu, double *raw_units,
-                   double *raw_fillers, double *grad, const double *metric, const double *lower, const double *upper,
-                   double *partials_units, double *partials_fillers, EvalScalars *scalars, int *nb_units,
-                   int *nb_fillers, hipStream_t s) {
-  *nb_units = std::max(1, blocks_for(a.n_density_units));
-  *nb_fillers = std::max(1, blocks_for(a.n_fillers));
-  timer_begin(g_timer, "gather", s);
-  hipLaunchKernelGGL(k_gather_units, dim3(*nb_units), dim3(BLOCK), 0, s, a, g, coords, gfu, raw_units,
-                     partials_units);
-  timer_end(g_timer, "gather", s);
-  hipLaunchKernelGGL(k_gather_fillers, dim3(*nb_fillers), dim3(BLOCK), 0, s, a, g, coords, raw_fillers, grad, metric,
-                     lower, upper, partials_fillers, scalars);
-}
-
-void launch_wa(const WaArgs &a, const double *coords, double gamma, double *partials, int *nb, hipStream_t s) {
-  const int nbs = a.n_small > 0 ? blocks_for(a.n_small) : 0;
-  const int nbl = a.n_large > 0 ? (a.n_large + BLOCK / 64 - 1) / (BLOCK / 64) : 0;
-  *nb = nbs + nbl;
-  if (*nb == 0) return;
+void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
+  if (m.nb_wa <= 0) return;
   timer_begin(g_timer, "wa", s);
-  hipLaunchKernelGGL(k_wa, dim3(*nb), dim3(BLOCK), 0, s, a, coords, gamma, partials, nbl);
+  hipLaunchKernelGGL(k_wa, dim3(m.nb_wa), dim3(BLOCK), 0, s, tasks, m.wa);
   timer_end(g_timer, "wa", s);
 }
 
-void launch_unit_gradient(const WaArgs &a, int F, int DU, const double2 *gfu, const double *coords,
-                          const double *metric, const double *lower, const double *upper, double *grad,
-                          double *wa_grad_out, double *partials, EvalScalars *scalars, int *nb, hipStream_t s) {
-  *nb = std::max(1, blocks_for(a.n_units));
-  hipLaunchKernelGGL(k_unit_gradient, dim3(*nb), dim3(BLOCK), 0, s, a, F, DU, gfu, coords, metric, lower, upper,
-                     grad, wa_grad_out, partials, scalars);
+void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
+  if (m.nb_unit_grad > 0)
+    hipLaunchKernelGGL(k_unit_gradient, dim3(m.nb_unit_grad), dim3(BLOCK), 0, s, tasks, m.unit_grad);
+  if (m.n_tasks > 0) hipLaunchKernelGGL(k_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
-void launch_finalize_eval(const FinalizeArgs &a, EvalScalars *scalars, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize_eval, dim3(1), dim3(BLOCK), 0, s, a, scalars);
-}
-
-void launch_preconditioner(int n_units, int n_fillers, int F, int DU, const double *wl_diag, const int32_t *unit_du,
-                           const double *du_fcharge, const uint8_t *fl_field, const double *fl_charge,
-                           const double *lambda, double *metric, double *partials, MetricScalars *ms, hipStream_t s) {
-  const int n = n_units + n_fillers;
-  const int nb = std::max(1, blocks_for(n));
-  hipLaunchKernelGGL(k_preconditioner, dim3(nb), dim3(BLOCK), 0, s, n_units, n_fillers, F, DU, wl_diag, unit_du,
-                     du_fcharge, fl_field, fl_charge, lambda[0], lambda[1], lambda[2], lambda[3], metric, partials);
-  hipLaunchKernelGGL(k_metric_finalize, dim3(1), dim3(BLOCK), 0, s, partials, nb, ms);
-}
-
-void launch_extrapolate(size_t n, const double *x, const double *xp, const double *lower, const double *upper,
-                        double beta, double *y, unsigned int *differs, hipStream_t s) {
-  hipLaunchKernelGGL(k_extrapolate, dim3(std::max(1, blocks_for(n))), dim3(BLOCK), 0, s, n, x, xp, lower, upper, beta,
-                     y, differs);
-}
-
-void launch_candidate(size_t n, const double *y, const double *gy, const double *metric, double lipschitz,
-                      const double *lower, const double *upper, const double *origin, double *cand, double *partials,
-                      StepScalars *step, hipStream_t s) {
-  const int nb = std::max(1, blocks_for(n));
-  hipLaunchKernelGGL(k_candidate, dim3(nb), dim3(BLOCK), 0, s, n, y, gy, metric, lipschitz, lower, upper, origin,
-                     cand, partials);
-  hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(BLOCK), 0, s, partials, nb, step);
-}
-
-void launch_zero_step(StepScalars *step, unsigned int *flag, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_step, dim3(1), dim3(1), 0, s, step, flag);
+void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
+  if (m.nb_precond > 0) {
+    hipLaunchKernelGGL(k_preconditioner, dim3(m.nb_precond), dim3(BLOCK), 0, s, tasks, m.precond);
+    hipLaunchKernelGGL(k_metric_finalize, dim3(m.n_precond), dim3(BLOCK), 0, s, tasks, m.precond_list);
+  }
+  if (m.nb_extrapolate > 0)
+    hipLaunchKernelGGL(k_extrapolate, dim3(m.nb_extrapolate), dim3(BLOCK), 0, s, tasks, m.extrapolate);
+  if (m.nb_candidate > 0) {
+    hipLaunchKernelGGL(k_candidate, dim3(m.nb_candidate), dim3(BLOCK), 0, s, tasks, m.candidate);
+    hipLaunchKernelGGL(k_step_finalize, dim3(m.n_candidate), dim3(BLOCK), 0, s, tasks, m.candidate_list);
+  }
+  if (m.nb_targets > 0) hipLaunchKernelGGL(k_round_targets, dim3(m.nb_targets), dim3(BLOCK), 0, s, tasks, m.targets);
 }
 
 void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, const double2 *pin_xy,
@@ -960,12 +867,6 @@ void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit
                      fx + G, error);
   timer_end(g_timer, "rudy", s);
   hipLaunchKernelGGL(k_fx_to_double, dim3(std::max(1, blocks_for(2 * G))), dim3(BLOCK), 0, s, 2 * G, fx, out);
-}
-
-void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s) {
-  if (n_units == 0) return;
-  hipLaunchKernelGGL(k_round_targets, dim3(blocks_for(2 * (size_t)n_units)), dim3(BLOCK), 0, s, n_units, coords, W, H,
-                     out);
 }
 
 }  // namespace texo
