@@ -21,6 +21,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "GP iters/s at 84k cells, 256\u00b2 bins; jobs/s + p99 latency, MFMA vs FP32 plain"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
@@ -139,7 +140,7 @@ def main():
                          f"{dt:.1f} s)"}
 
     line = {
-        "metric": "GP iters/s at 84k cells, 256^2 bins",
+        "metric": METRIC,
         "value": main_run["it_per_s"],
         "unit": "GP iterations/s",
         "n_gpus": 1,

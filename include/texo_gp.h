@@ -322,6 +322,36 @@ int texo_gp_session_kernel_timing(texo_gp_session *session, double *avg_ms,
 int texo_gp_session_iterate(texo_gp_session *session, uint32_t iterations,
                             texo_gp_report *report);
 
+/* ---- lockstep batches: multi-start placement ----------------------------- */
+/* No counterpart in the reference (it places one design per call).  A batch
+ * owns one GPU and runs every job it holds in lockstep rounds, one launch per
+ * kernel family over all jobs: multi-start seeds, timing-weight variants or
+ * unrelated designs of any grid size.  Each job's trajectory and result are
+ * bit-identical to the same job run alone. */
+typedef struct texo_gp_batch texo_gp_batch;
+
+texo_gp_batch *texo_gp_batch_create(int32_t device);
+/* Returns the job's index (>= 0), or -(status) with `report` filled. */
+int64_t texo_gp_batch_add(texo_gp_batch *batch, const texo_gp_problem *problem,
+                          const texo_gp_options *options, texo_gp_report *report);
+/* Full placement of every job (eplace::place each); returns once every job
+ * has finished or failed.  Per-job outcomes come from texo_gp_batch_result. */
+int texo_gp_batch_run(texo_gp_batch *batch);
+/* `iterations` more GP iterations of every job with the stop rules off, as
+ * texo_gp_session_iterate. */
+int texo_gp_batch_iterate(texo_gp_batch *batch, uint32_t iterations);
+/* Status of job `job`; targets (best checkpoint, after run) and report. */
+int texo_gp_batch_result(texo_gp_batch *batch, uint64_t job, uint32_t *out_targets_xy,
+                         texo_gp_report *report);
+/* Sizes of job `job`, as texo_gp_session_sizes. */
+int texo_gp_batch_sizes(const texo_gp_batch *batch, uint64_t job, uint64_t *coordinates, uint32_t *fields,
+                        uint64_t *fillers, uint64_t *density_units);
+int texo_gp_batch_coordinates(texo_gp_batch *batch, uint64_t job, double *coordinates);
+/* Kernel timing over the batch's launches (one launch covers every job). */
+int texo_gp_batch_set_kernel_timing(texo_gp_batch *batch, const char *kernel, int enable);
+int texo_gp_batch_kernel_timing(texo_gp_batch *batch, double *avg_ms, uint64_t *count);
+void texo_gp_batch_destroy(texo_gp_batch *batch);
+
 /* ---- serving: continuous batching of many placement jobs --------------- */
 /* No counterpart in the reference (it places one design per call).  A server
  * owns one GPU and a worker thread that runs every admitted job in lockstep

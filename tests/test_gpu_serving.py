@@ -92,3 +92,32 @@ def test_rejected_problem_fails_at_submit_and_poll_releases_results():
         assert res is not None and res[1]["iterations"] == 5 and res[1]["elapsed_seconds"] > 0.0
         rc = srv.lib.texo_gp_server_poll(srv.handle, job, None, None)
         assert abi.STATUS[rc] == "InvalidArgument"  # released by the first poll
+
+
+def test_lockstep_batch_iterates_bit_identically_to_sessions():
+    jobs = mix()
+    with texo_amd.Batch() as batch:
+        for p, prec in jobs:
+            batch.add(p, abi.options(dct_precision=prec))
+        batch.iterate(25)
+        batch.iterate(15)
+        got = [batch.coordinates(i) for i in range(len(jobs))]
+        reps = [batch.result(i, targets=False)[1] for i in range(len(jobs))]
+    for (p, prec), x, rep in zip(jobs, got, reps):
+        s = texo_amd.Session(p, abi.options(dct_precision=prec))
+        r = s.iterate(40)
+        assert np.array_equal(s.coordinates(), x)
+        assert r.evaluations == rep["evaluations"] and r.final_hpwl == rep["final_hpwl"]
+        s.close()
+
+
+def test_lockstep_batch_run_matches_place():
+    jobs = [synth.plain_1k(7), synth.plain_1k(8)]
+    ref = [texo_amd.place(p) for p in jobs]
+    with texo_amd.Batch() as batch:
+        for p in jobs:
+            batch.add(p)
+        batch.run()
+        got = [batch.result(i) for i in range(len(jobs))]
+    for a, b in zip(ref, got):
+        same(a, b)

@@ -35,6 +35,7 @@ def load():
     lib = C.CDLL(LIB_PATH)
     abi.declare_session_api(lib, "texo_gp")
     abi.declare_server_api(lib, "texo_gp")
+    abi.declare_batch_api(lib, "texo_gp")
     P, d = C.POINTER, C.c_double
     lib.texo_gp_last_error.restype = C.c_char_p
     lib.texo_gp_last_error.argtypes = []
@@ -148,6 +149,90 @@ class Server:
     def close(self):
         if getattr(self, "handle", None):
             self.lib.texo_gp_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Lockstep batch on one GPU: every job advances in the same rounds, one
+    launch per kernel family covering all of them (multi-start seeds,
+    timing-weight variants, unrelated designs).  Each job's trajectory is
+    bit-identical to the same job run alone."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.device = device
+        self.handle = self.lib.texo_gp_batch_create(device)
+        if not self.handle:
+            raise RuntimeError(f"texo_gp_batch_create failed: {last_error()}")
+        self.problems = []
+
+    def add(self, problem: Problem, opts: Optional[abi.texo_gp_options] = None) -> int:
+        opts = opts if opts is not None else options(device=self.device)
+        rep = abi.texo_gp_report()
+        idx = self.lib.texo_gp_batch_add(self.handle, C.byref(problem.struct()), C.byref(opts), C.byref(rep))
+        if idx < 0:
+            raise PlacementError(rep)
+        self.problems.append(problem)
+        return idx
+
+    def run(self):
+        rc = self.lib.texo_gp_batch_run(self.handle)
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+
+    def iterate(self, iterations: int):
+        rc = self.lib.texo_gp_batch_iterate(self.handle, iterations)
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+
+    def result(self, job: int, targets: bool = True):
+        """(targets[n_units, 2] or None, report dict); raises PlacementError for a failed job."""
+        t = np.zeros(2 * self.problems[job].n_units, np.uint32) if targets else None
+        rep = abi.texo_gp_report()
+        rc = self.lib.texo_gp_batch_result(self.handle, job, abi.ptr(t, C.c_uint32), C.byref(rep))
+        if rc != 0:
+            raise PlacementError(rep)
+        return (t.reshape(-1, 2) if t is not None else None), rep.as_dict()
+
+    def sizes(self, job: int) -> dict:
+        n, f, fl, du = C.c_uint64(), C.c_uint32(), C.c_uint64(), C.c_uint64()
+        rc = self.lib.texo_gp_batch_sizes(self.handle, job, C.byref(n), C.byref(f), C.byref(fl), C.byref(du))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return {"coordinates": n.value, "fields": f.value, "fillers": fl.value, "density_units": du.value}
+
+    def coordinates(self, job: int) -> np.ndarray:
+        x = np.zeros(self.sizes(job)["coordinates"])
+        rc = self.lib.texo_gp_batch_coordinates(self.handle, job, abi.ptr(x, C.c_double))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return x
+
+    def set_kernel_timing(self, kernel: Optional[str]):
+        self.lib.texo_gp_batch_set_kernel_timing(self.handle, (kernel or "").encode(), 1 if kernel else 0)
+
+    def kernel_timing(self):
+        avg, count = C.c_double(), C.c_uint64()
+        rc = self.lib.texo_gp_batch_kernel_timing(self.handle, C.byref(avg), C.byref(count))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return avg.value, count.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.texo_gp_batch_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

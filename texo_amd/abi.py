@@ -426,3 +426,24 @@ def declare_server_api(lib, prefix: str):
     f("server_wait", C.c_int, C.c_void_p, C.c_int64, P(C.c_uint32), P(texo_gp_report))
     f("server_stats_get", C.c_int, C.c_void_p, P(texo_gp_server_stats))
     f("server_destroy", None, C.c_void_p)
+
+
+def declare_batch_api(lib, prefix: str):
+    """Declares the lockstep batch entry points of the product library."""
+    P = C.POINTER
+
+    def f(name, restype, *argtypes):
+        fn = getattr(lib, f"{prefix}_{name}")
+        fn.restype = restype
+        fn.argtypes = list(argtypes)
+
+    f("batch_create", C.c_void_p, C.c_int32)
+    f("batch_add", C.c_int64, C.c_void_p, P(texo_gp_problem), P(texo_gp_options), P(texo_gp_report))
+    f("batch_run", C.c_int, C.c_void_p)
+    f("batch_iterate", C.c_int, C.c_void_p, C.c_uint32)
+    f("batch_result", C.c_int, C.c_void_p, C.c_uint64, P(C.c_uint32), P(texo_gp_report))
+    f("batch_sizes", C.c_int, C.c_void_p, C.c_uint64, P(C.c_uint64), P(C.c_uint32), P(C.c_uint64), P(C.c_uint64))
+    f("batch_coordinates", C.c_int, C.c_void_p, C.c_uint64, P(C.c_double))
+    f("batch_set_kernel_timing", C.c_int, C.c_void_p, C.c_char_p, C.c_int)
+    f("batch_kernel_timing", C.c_int, C.c_void_p, P(C.c_double), P(C.c_uint64))
+    f("batch_destroy", None, C.c_void_p)

@@ -81,6 +81,29 @@ void drive(Engine &e, Job &j) {
   while (j.phase != Phase::Done && j.phase != Phase::Failed && j.phase != Phase::Paused) e.round(one);
 }
 
+int timing_set(Engine &e, const char *kernel, int enable) {
+  e.timer.name = kernel ? kernel : "";
+  e.timer.enabled = enable != 0;
+  e.timer.used = 0;
+  return 0;
+}
+
+// Average duration of the named kernel's launches since the last call.
+int timing_get(Engine &e, double *avg_ms, uint64_t *count) {
+  texo::KernelTimer &t = e.timer;
+  if (hipSetDevice(e.device()) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return TEXO_GP_ERR_DEVICE;
+  double total = 0.0;
+  for (size_t i = 0; i < t.used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.events[i].first, t.events[i].second) != hipSuccess) return TEXO_GP_ERR_DEVICE;
+    total += ms;
+  }
+  if (avg_ms) *avg_ms = t.used ? total / (double)t.used : 0.0;
+  if (count) *count = t.used;
+  t.used = 0;
+  return 0;
+}
+
 }  // namespace
 
 struct texo_gp_session {
@@ -333,27 +356,12 @@ int texo_gp_session_rudy(texo_gp_session *s, const double *coordinates, double *
 
 int texo_gp_session_set_kernel_timing(texo_gp_session *s, const char *kernel, int enable) {
   if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
-  s->engine->timer.name = kernel ? kernel : "";
-  s->engine->timer.enabled = enable != 0;
-  s->engine->timer.used = 0;
-  return 0;
+  return timing_set(*s->engine, kernel, enable);
 }
 
 int texo_gp_session_kernel_timing(texo_gp_session *s, double *avg_ms, uint64_t *count) {
   if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
-  texo::KernelTimer &t = s->engine->timer;
-  if (hipStreamSynchronize(s->engine->stream()) != hipSuccess) return TEXO_GP_ERR_DEVICE;
-  if (hipDeviceSynchronize() != hipSuccess) return TEXO_GP_ERR_DEVICE;
-  double total = 0.0;
-  for (size_t i = 0; i < t.used; ++i) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, t.events[i].first, t.events[i].second) != hipSuccess) return TEXO_GP_ERR_DEVICE;
-    total += ms;
-  }
-  if (avg_ms) *avg_ms = t.used ? total / (double)t.used : 0.0;
-  if (count) *count = t.used;
-  t.used = 0;
-  return 0;
+  return timing_get(*s->engine, avg_ms, count);
 }
 
 int texo_gp_place(const texo_gp_problem *problem, const texo_gp_options *options, uint32_t *out_targets_xy,
@@ -364,6 +372,147 @@ int texo_gp_place(const texo_gp_problem *problem, const texo_gp_options *options
   texo_gp_session_destroy(s);
   return rc;
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// Lockstep batches
+// ===========================================================================
+struct texo_gp_batch {
+  std::unique_ptr<Engine> engine;
+  std::vector<std::unique_ptr<Job>> jobs;
+  ~texo_gp_batch() {
+    if (!engine) return;
+    (void)hipSetDevice(engine->device());
+    for (auto &j : jobs) engine->detach(j.get());
+    jobs.clear();
+  }
+  void drive_all() {
+    std::vector<Job *> all;
+    for (auto &j : jobs) all.push_back(j.get());
+    for (;;) {
+      bool busy = false;
+      for (Job *j : all) busy = busy || (j->phase != Phase::Done && j->phase != Phase::Failed && j->phase != Phase::Paused);
+      if (!busy) return;
+      engine->round(all);
+    }
+  }
+};
+
+extern "C" {
+
+texo_gp_batch *texo_gp_batch_create(int32_t device) {
+  try {
+    std::unique_ptr<texo_gp_batch> b(new texo_gp_batch());
+    b->engine.reset(new Engine(device, 1024));
+    return b.release();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+int64_t texo_gp_batch_add(texo_gp_batch *b, const texo_gp_problem *problem, const texo_gp_options *options,
+                          texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  if (!b) return -(int64_t)TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::unique_ptr<Job> j(new Job());
+  try {
+    j->load(problem, options);
+    if (j->opt.device != b->engine->device()) texo::fail(TEXO_GP_ERR_INVALID_ARGUMENT);
+    b->engine->attach(j.get());
+  } catch (const Failure &f) {
+    put_failure(report, f);
+    if (j->pool_index >= 0) b->engine->detach(j.get());
+    return -(int64_t)f.status;
+  }
+  j->id = (int64_t)b->jobs.size();
+  b->jobs.push_back(std::move(j));
+  return (int64_t)b->jobs.size() - 1;
+}
+
+int texo_gp_batch_run(texo_gp_batch *b) {
+  if (!b) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    for (auto &j : b->jobs)
+      if (j->phase != Phase::Failed) j->begin_run(false);
+    b->drive_all();
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int texo_gp_batch_iterate(texo_gp_batch *b, uint32_t iterations) {
+  if (!b) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    for (auto &j : b->jobs) {
+      if (j->empty || j->phase == Phase::Failed || j->phase == Phase::Done) continue;
+      if (!j->started) j->begin_run(true);
+      j->pause_at = j->iterations + iterations;
+      if (j->phase == Phase::Paused) j->phase = Phase::IterStart;
+    }
+    b->drive_all();
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
+int texo_gp_batch_result(texo_gp_batch *b, uint64_t job, uint32_t *out_targets_xy, texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  if (!b || job >= b->jobs.size()) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &j = *b->jobs[job];
+  try {
+    fill_report(*b->engine, j, report);
+    if (j.phase == Phase::Failed) return j.status;
+    if (out_targets_xy) {
+      if (j.phase != Phase::Done) return TEXO_GP_ERR_NOT_READY;
+      copy_targets(*b->engine, j, out_targets_xy);
+    }
+    return 0;
+  } catch (const Failure &f) {
+    put_failure(report, f);
+    return f.status;
+  }
+}
+
+int texo_gp_batch_sizes(const texo_gp_batch *b, uint64_t job, uint64_t *coordinates, uint32_t *fields,
+                        uint64_t *fillers, uint64_t *density_units) {
+  if (!b || job >= b->jobs.size()) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const Job &J = *b->jobs[job];
+  if (coordinates) *coordinates = J.C;
+  if (fields) *fields = (uint32_t)J.F;
+  if (fillers) *fillers = J.Fl;
+  if (density_units) *density_units = J.DU;
+  return 0;
+}
+
+int texo_gp_batch_coordinates(texo_gp_batch *b, uint64_t job, double *coordinates) {
+  if (!b || job >= b->jobs.size() || !coordinates) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const Job &J = *b->jobs[job];
+  if (J.empty) {
+    std::copy(J.initial_xy.begin(), J.initial_xy.end(), coordinates);
+    return 0;
+  }
+  if (hipSetDevice(b->engine->device()) != hipSuccess ||
+      hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, b->engine->stream()) !=
+          hipSuccess ||
+      hipStreamSynchronize(b->engine->stream()) != hipSuccess)
+    return TEXO_GP_ERR_DEVICE;
+  return 0;
+}
+
+int texo_gp_batch_set_kernel_timing(texo_gp_batch *b, const char *kernel, int enable) {
+  if (!b) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  return timing_set(*b->engine, kernel, enable);
+}
+
+int texo_gp_batch_kernel_timing(texo_gp_batch *b, double *avg_ms, uint64_t *count) {
+  if (!b) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  return timing_get(*b->engine, avg_ms, count);
+}
+
+void texo_gp_batch_destroy(texo_gp_batch *b) { delete b; }
 
 }  // extern "C"
 
