@@ -1,12 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark: GP iterations/s of the electrostatic global-placement hot loop.
+"""Benchmark of the electrostatic global-placement hot loop on one MI355X.
 
-One step = one GP iteration (one dynamic_nesterov_step of
-optimize_density_once, crates/texo-pnr/src/eplace.rs:503-574) on the
-84k-cell / 256x256-bin ECP5-85-scale workload, all inputs resident in HBM.
-The stop rule is disabled so exactly K iterations are timed after W warm-up
-iterations (the session's one-time set-up and first evaluations are inside the
-warm-up).  Prints one JSON line.
+Headline: GP iterations/s on the 84k-cell / 256x256-bin ECP5-85-scale
+workload.  One step = one GP iteration (one dynamic_nesterov_step of
+optimize_density_once, crates/texo-pnr/src/eplace.rs:503-574) with all
+inputs resident in HBM.  The stop rules are off so exactly K iterations are
+timed after W warm-up iterations (set-up and the first evaluations fall in
+the warm-up); the library synchronises its streams before each timed call
+returns, on both sides of the timed region.
+
+Forms measured on the same workload:
+  * plain FP32: one job, FP32 vector-ALU DCT;
+  * MFMA single: one job, split-precision MFMA DCT;
+  * MFMA batched (the reported value): B multi-start placements of the same
+    design advanced in lockstep, one launch per kernel family covering all B;
+    value = B * K / elapsed.
+Serving (mixed sizes, 1k-84k cells, Poisson arrivals): full placements
+through the continuous-batching server versus the same server admitting one
+job at a time with the plain FP32 DCT; p50/p99 latency and jobs/s.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -21,18 +34,19 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "GP iters/s at 84k cells, 256\u00b2 bins; jobs/s + p99 latency, MFMA vs FP32 plain"
+METRIC = "GP iters/s at 84k cells, 256² bins; jobs/s + p99 latency, MFMA vs FP32 plain"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+FP16_PEAK_TFLOPS = 2500.0
 
 
-def algorithmic_bytes(kernel: str, sess, problem) -> float:
-    """Compulsory HBM bytes of one launch of `kernel` (each input read once,
-    each output written once, at storage precision)."""
+def algorithmic_bytes(kernel: str, sizes: dict, problem) -> float:
+    """Compulsory HBM bytes of one job's share of one launch of `kernel`
+    (each input read once, each output written once, at storage precision)."""
     M = problem.n_members
-    Fl = sess.n_fillers
-    F, G = sess.n_fields, problem.width * problem.height
+    Fl = sizes["fillers"]
+    F = sizes["fields"]
     if kernel == "deposit":
         # coordinates (16 B), member offset/charge/field/unit (29 B) or filler
         # charge/field (9 B), 9 int64 read-modify-writes (144 B)
@@ -43,16 +57,21 @@ def algorithmic_bytes(kernel: str, sess, problem) -> float:
         # per pin: unit 4 + offset 16 + coordinates 16 + derivative 16; per net: start 4 + weight 8 + id 4
         return pins * 52 + nets * 16
     if kernel == "gather":
-        DU = sess.n_density_units
+        DU = sizes["density_units"]
         # per unit: unit id, member range, coordinates; per member 3x3 phi taps (36 B FP32) + offset/charge/field;
         # outputs coef*G per field (16 B) per unit
         return DU * (4 + 8 + 16 + 16 * F) + M * (36 + 25)
     raise ValueError(kernel)
 
 
-def dct_flops(sess, problem) -> float:
-    W, H, F = problem.width, problem.height, sess.n_fields
+def dct_flops(sizes: dict, problem) -> float:
+    W, H, F = problem.width, problem.height, sizes["fields"]
     return F * 4.0 * W * H * (W + H)
+
+
+def progress(msg: str):
+    """Progress on stderr; stdout carries only the result line."""
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(problem, iterations: int):
@@ -69,99 +88,182 @@ def cpu_baseline(problem, iterations: int):
     return iterations / dt, dt
 
 
+def run_single(problem, precision, steps, warmup):
+    import texo_amd
+    from texo_amd import abi
+    sess = texo_amd.Session(problem, abi.options(dct_precision=precision))
+    sess.iterate(warmup)
+    rep0 = sess.iterate(0)
+    t0 = time.perf_counter()
+    rep = sess.iterate(steps)
+    dt = time.perf_counter() - t0
+    out = {"it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+           "evaluations_per_iteration": (rep.evaluations - rep0.evaluations) / steps,
+           "trials_per_iteration": (rep.line_search_trials - rep0.line_search_trials) / steps,
+           "hpwl": rep.final_hpwl, "overflow": list(rep.field_overflow[:rep.n_fields])}
+    sess.close()
+    return out
+
+
+def run_batched(problems, precision, steps, warmup, timing_kernel):
+    import texo_amd
+    from texo_amd import abi
+    b = texo_amd.Batch()
+    for p in problems:
+        b.add(p, abi.options(dct_precision=precision))
+    b.iterate(warmup)
+    r0 = [b.result(i, targets=False)[1] for i in range(len(problems))]
+    b.set_kernel_timing(timing_kernel)
+    t0 = time.perf_counter()
+    b.iterate(steps)
+    dt = time.perf_counter() - t0
+    avg_ms, launches = b.kernel_timing()
+    r1 = [b.result(i, targets=False)[1] for i in range(len(problems))]
+    sizes = b.sizes(0)
+    n = len(problems)
+    out = {"it_per_s": n * steps / dt, "per_job_it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+           "kernel_avg_ms": avg_ms, "kernel_launches": launches,
+           "evaluations_per_iteration": sum(b1["evaluations"] - a["evaluations"] for a, b1 in zip(r0, r1)) / (n * steps),
+           "trials_per_iteration": sum(b1["line_search_trials"] - a["line_search_trials"]
+                                       for a, b1 in zip(r0, r1)) / (n * steps),
+           "hpwl": [r["final_hpwl"] for r in r1]}
+    b.close()
+    return out, sizes
+
+
+def run_serving(stream, precision, max_batch, at_rate: bool):
+    """Submits the stream to a server (at its arrival times, or all at once)
+    and waits for every job.  Latency = submission to completion."""
+    import texo_amd
+    from texo_amd import abi
+    with texo_amd.Server(max_batch=max_batch) as srv:
+        t_start = time.perf_counter()
+        ids = []
+        for arrival, p in stream:
+            if at_rate:
+                while True:
+                    ahead = arrival - (time.perf_counter() - t_start)
+                    if ahead <= 0:
+                        break
+                    time.sleep(min(ahead, 0.002))
+            ids.append(srv.submit(p, abi.options(dct_precision=precision)))
+        lat, failed = [], 0
+        for i in ids:
+            _, rep = srv.wait(i, raise_on_error=False)
+            lat.append(rep["elapsed_seconds"])
+            failed += rep["status"] != "OK"
+        makespan = time.perf_counter() - t_start
+        stats = srv.stats()
+    lat = np.array(lat)
+    return {"jobs": len(stream), "jobs_per_s": len(stream) / makespan, "makespan_s": makespan,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "failed": int(failed), "rounds": stats["rounds"], "busy_s": stats["busy_seconds"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="ecp5_85k")
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--batch", type=int, default=8, help="multi-start jobs in the batched form")
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3", "fp16_scaled"])
     ap.add_argument("--roofline-kernel", default="deposit", choices=["deposit", "wa", "gather", "dct"])
+    ap.add_argument("--serving-jobs", type=int, default=256)
+    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compare", action="store_true", help="also time the other DCT precision")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build measures one GPU: the placement loop of one job runs on one "
+                                   "device, and runs on several GPUs are not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import texo_amd
+    import texo_amd  # noqa: F401  (fails loudly when the HIP library is missing)
     from texo_amd import abi, synth
 
-    problem = synth.CONFIGS[args.config]()
-    prec = {"fp32": abi.DCT_FP32, "bf16x3": abi.DCT_BF16X3}
+    prec = {"fp32": abi.DCT_FP32, "bf16x3": abi.DCT_BF16X3, "fp16_scaled": abi.DCT_FP16_SCALED}
+    mfma = prec[args.precision]
+    base = synth.ecp5_85k()
 
-    def run(precision: int, timing_kernel: str):
-        sess = texo_amd.Session(problem, abi.options(dct_precision=precision))
-        sess.iterate(args.warmup)
-        sess.set_kernel_timing(timing_kernel)
-        rep0 = sess.iterate(0)
-        t0 = time.perf_counter()
-        rep = sess.iterate(args.steps)  # returns after a stream synchronisation
-        dt = time.perf_counter() - t0
-        avg_ms, launches = sess.kernel_timing()
-        evals = rep.evaluations - rep0.evaluations
-        trials = rep.line_search_trials - rep0.line_search_trials
-        out = {"seconds": dt, "it_per_s": args.steps / dt, "kernel_avg_ms": avg_ms, "kernel_launches": launches,
-               "evaluations": evals, "trials": trials, "hpwl": rep.final_hpwl,
-               "overflow": list(rep.field_overflow[:rep.n_fields])}
-        return sess, out
-
-    sess, main_run = run(prec[args.precision], args.roofline_kernel)
-    other = None
-    if args.compare:
-        alt = "fp32" if args.precision == "bf16x3" else "bf16x3"
-        s2, other = run(prec[alt], "dct")
-        other["precision"] = alt
-        s2.close()
+    plain = run_single(base, abi.DCT_FP32, args.steps, args.warmup)
+    progress(f"plain fp32 single job: {plain['it_per_s']:.1f} it/s")
+    single = run_single(base, mfma, args.steps, args.warmup)
+    progress(f"{args.precision} single job: {single['it_per_s']:.1f} it/s")
+    starts = [synth.multistart(base, s) for s in range(args.batch)]
+    batched, sizes = run_batched(starts, mfma, args.steps, args.warmup, args.roofline_kernel)
+    progress(f"{args.precision} batch of {args.batch}: {batched['it_per_s']:.1f} it/s")
 
     k = args.roofline_kernel
-    avg_s = main_run["kernel_avg_ms"] * 1e-3
+    avg_s = batched["kernel_avg_ms"] * 1e-3
     if k == "dct":
-        flops = dct_flops(sess, problem)
+        flops = args.batch * dct_flops(sizes, base)
         achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
-        peak = BF16_PEAK_TFLOPS if args.precision == "bf16x3" else FP32_PEAK_TFLOPS
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                    "frac": achieved / peak, "traffic": None, "kernel": "dct (4 stages)",
-                    "algorithmic_per_launch": flops}
+        peak = {"fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS, "fp16_scaled": FP16_PEAK_TFLOPS}[args.precision]
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                    "traffic": None, "kernel": "dct (4 stages)", "algorithmic_per_launch": flops,
+                    "avg_launch_ms": batched["kernel_avg_ms"]}
     else:
-        nbytes = algorithmic_bytes(k, sess, problem)
+        nbytes = args.batch * algorithmic_bytes(k, sizes, base)
         achieved = nbytes / avg_s / 1e9 if avg_s > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": k,
-                    "algorithmic_per_launch": nbytes, "avg_launch_ms": main_run["kernel_avg_ms"]}
+                    "algorithmic_per_launch": nbytes, "avg_launch_ms": batched["kernel_avg_ms"],
+                    "jobs_per_launch": args.batch}
+
+    serving = None
+    if not args.no_serving:
+        stream = synth.serving_stream(args.serving_jobs, rate=args.serving_rate)
+        serving = {"jobs": args.serving_jobs, "arrival_rate_jobs_per_s": args.serving_rate,
+                   "cells": "log-uniform 1k-84k, grid side nextpow2(sqrt(cells/4)) in 64-256, "
+                            "weight exponents {2,4,6}"}
+        for name, precision, max_batch, at_rate in (("batched_mfma_at_rate", mfma, 256, True),
+                                                    ("batched_mfma_saturated", mfma, 256, False),
+                                                    ("one_at_a_time_fp32_at_rate", abi.DCT_FP32, 1, True),
+                                                    ("one_at_a_time_fp32_saturated", abi.DCT_FP32, 1, False)):
+            serving[name] = run_serving(stream, precision, max_batch, at_rate)
+            progress(f"serving {name}: {serving[name]}")
+        serving["saturated_jobs_per_s_ratio"] = (serving["batched_mfma_saturated"]["jobs_per_s"]
+                                                 / serving["one_at_a_time_fp32_saturated"]["jobs_per_s"])
 
     cpu = None
     if not args.no_cpu:
-        v, dt = cpu_baseline(problem, args.cpu_iterations)
+        progress("cpu baseline")
+        v, dt = cpu_baseline(base, args.cpu_iterations)
         cpu = {"value": v, "unit": "GP iterations/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_iterations} GP iterations of {args.config} (FP64 C++ restatement, "
-                         f"{dt:.1f} s)"}
+               "sample": f"{args.cpu_iterations} GP iterations of the 84k-cell / 256x256 workload, one job "
+                         f"(FP64 C++ restatement of the reference loop, 1 thread, {dt:.1f} s)"}
 
     line = {
         "metric": METRIC,
-        "value": main_run["it_per_s"],
+        "value": batched["it_per_s"],
         "unit": "GP iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * main_run["seconds"] / args.steps,
+        "ms_per_step": batched["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp64 coords/reductions, int64 fixed-point density, "
-                 + ("bf16x3 MFMA" if args.precision == "bf16x3" else "fp32") + " DCT",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: {problem.meta['cells']} cells, {problem.width}x{problem.height} bins, "
-                               f"{problem.meta['nets']} nets, {problem.meta['arcs']} timed arcs",
-                   "precision": args.precision, "evaluations_per_iteration":
-                       main_run["evaluations"] / args.steps, "trials_per_iteration": main_run["trials"] / args.steps},
+        "config": {"workload": f"ecp5_85k: {base.meta['cells']} cells, {base.width}x{base.height} bins, "
+                               f"{base.meta['nets']} nets, {base.meta['arcs']} timed arcs; "
+                               f"{args.batch} multi-start placements in lockstep",
+                   "arithmetic": "f64 coordinates, wirelength and reductions; int64 fixed-point density; "
+                                 f"{args.precision} Poisson transform",
+                   "batch": args.batch, "precision": args.precision,
+                   "evaluations_per_iteration": batched["evaluations_per_iteration"],
+                   "trials_per_iteration": batched["trials_per_iteration"]},
+        "forms": {"fp32_plain_single_it_per_s": plain["it_per_s"],
+                  f"{args.precision}_single_it_per_s": single["it_per_s"],
+                  f"{args.precision}_batched_it_per_s": batched["it_per_s"],
+                  "batched_per_job_it_per_s": batched["per_job_it_per_s"],
+                  "batched_vs_fp32_plain": batched["it_per_s"] / plain["it_per_s"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
-    if other:
-        line["compare"] = {other["precision"]: other["it_per_s"]}
     print(json.dumps(line), flush=True)
 
 

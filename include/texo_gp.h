@@ -110,7 +110,8 @@ enum texo_gp_status {
 enum texo_gp_dct_precision {
   TEXO_GP_DCT_FP32 = 0,       /* plain FP32 contraction */
   TEXO_GP_DCT_BF16X3 = 1,     /* split hi/lo BF16 on MFMA, FP32 accumulate */
-  TEXO_GP_DCT_FP16_SCALED = 2 /* per-field scaled FP16 on MFMA, FP32 acc */
+  TEXO_GP_DCT_FP16_SCALED = 2 /* split hi/lo FP16 on MFMA with power-of-two
+                                 scaling per 32-row block, FP32 accumulate */
 };
 
 /* Flattened placement problem.  Every pointer is borrowed for the call.

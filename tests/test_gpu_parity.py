@@ -31,7 +31,8 @@ def pair(problem, precision=abi.DCT_FP32):
 
 
 @pytest.mark.parametrize("name", ["plain_1k", "ecp5_25k"])
-@pytest.mark.parametrize("precision,field_tol", [(abi.DCT_FP32, 1e-5), (abi.DCT_BF16X3, 1e-3)])
+@pytest.mark.parametrize("precision,field_tol", [(abi.DCT_FP32, 1e-5), (abi.DCT_BF16X3, 1e-3),
+                                                 (abi.DCT_FP16_SCALED, 1e-5)])
 def test_evaluate_matches_oracle(name, precision, field_tol):
     print(f"evaluate parity {name} precision={precision}", flush=True)
     p = synth.CONFIGS[name]()
@@ -59,7 +60,8 @@ def test_evaluate_matches_oracle(name, precision, field_tol):
 
 
 @pytest.mark.parametrize("W,H", [(7, 5), (11, 1), (127, 96), (64, 64), (256, 256)])
-@pytest.mark.parametrize("precision,tol", [(abi.DCT_FP32, 2e-6), (abi.DCT_BF16X3, 2e-4)])
+@pytest.mark.parametrize("precision,tol", [(abi.DCT_FP32, 2e-6), (abi.DCT_BF16X3, 2e-4),
+                                           (abi.DCT_FP16_SCALED, 4e-6)])
 def test_poisson_matches_oracle(W, H, precision, tol):
     print(f"poisson parity {W}x{H} precision={precision}", flush=True)
     # density pattern of fast_prime_grid_solver_matches_orthonormal_matrix_oracle
@@ -73,6 +75,9 @@ def test_poisson_matches_oracle(W, H, precision, tol):
     e = C.c_double()
     oracle.load().oracle_poisson_solve(W, H, abi.ptr(density, C.c_double), abi.ptr(ref, C.c_double), C.byref(e))
     scale = max(np.abs(ref).max(), 1.0)
+    print(f"max error / max |phi| = {np.abs(phi.reshape(-1) - ref).max() / scale:.3e}, "
+          f"relative L2 = {rel(phi.reshape(-1), ref):.3e}, energy {abs(energy[0] - e.value) / abs(e.value):.3e}",
+          flush=True)
     assert np.abs(phi.reshape(-1) - ref).max() <= tol * scale * 10
     assert abs(energy[0] - e.value) <= tol * abs(e.value)
     g.close()
@@ -176,3 +181,21 @@ def test_place_with_reference_stop_rules_matches_oracle():
     assert abs(rg["final_hpwl"] - rep.final_hpwl) <= 0.005 * rep.final_hpwl
     assert tg.shape == (p.n_units, 2)
     assert np.all(tg[:, 0] < p.width) and np.all(tg[:, 1] < p.height)
+
+
+@pytest.mark.parametrize("precision", [abi.DCT_FP32, abi.DCT_FP16_SCALED])
+def test_small_serving_job_converges_like_the_oracle(precision):
+    # 1,430 cells on 64x64 bins: the FP64 oracle stops at 123 iterations with
+    # the register overflow just under its 0.10 target; the transform must be
+    # accurate enough to reach the same stop (BF16x3 is not: it plateaus above)
+    p = synth.serving_job(103, 1430, 4)
+    tg, rg = texo_amd.place(p, dct_precision=precision)
+    to = np.zeros(2 * p.n_units, np.uint32)
+    rep = abi.texo_gp_report()
+    import ctypes as C
+    rc = oracle.load().oracle_place(C.byref(p.struct()), C.byref(abi.options()), abi.ptr(to, C.c_uint32),
+                                    C.byref(rep))
+    assert rc == 0
+    print("gpu", rg, "oracle", rep.as_dict(), flush=True)
+    assert max(rg["field_overflow"]) <= 0.10
+    assert abs(rg["final_hpwl"] - rep.final_hpwl) <= 0.01 * rep.final_hpwl

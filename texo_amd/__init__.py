@@ -121,7 +121,7 @@ class Server:
         self._units[job] = problem.n_units
         return job
 
-    def _result(self, fn, job: int):
+    def _result(self, fn, job: int, raise_on_error: bool):
         targets = np.zeros(2 * self._units[job], np.uint32)
         rep = abi.texo_gp_report()
         rc = fn(self.handle, job, abi.ptr(targets, C.c_uint32), C.byref(rep))
@@ -129,15 +129,18 @@ class Server:
             return None
         del self._units[job]
         if rc != 0:
-            raise PlacementError(rep)
+            if raise_on_error:
+                raise PlacementError(rep)
+            return None, rep.as_dict()
         return targets.reshape(-1, 2), rep.as_dict()
 
-    def poll(self, job: int):
-        """None while the job runs, else its result (released by this call)."""
-        return self._result(self.lib.texo_gp_server_poll, job)
+    def poll(self, job: int, raise_on_error: bool = True):
+        """None while the job runs, else its result (released by this call).
+        With raise_on_error=False a failed job gives (None, report)."""
+        return self._result(self.lib.texo_gp_server_poll, job, raise_on_error)
 
-    def wait(self, job: int):
-        return self._result(self.lib.texo_gp_server_wait, job)
+    def wait(self, job: int, raise_on_error: bool = True):
+        return self._result(self.lib.texo_gp_server_wait, job, raise_on_error)
 
     def stats(self) -> dict:
         st = abi.texo_gp_server_stats()

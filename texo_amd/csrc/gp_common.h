@@ -114,6 +114,9 @@ struct JobStatic {
   const float *b32[4];        // C_W^T, C_H^T, C_H, C_W (row-major K x N)
   const uint16_t *bthi[4];    // transposed BF16 hi/lo splits (N x K)
   const uint16_t *btlo[4];
+  const uint16_t *fthi[4];    // transposed FP16 hi/lo splits of basis * 2^s (N x K)
+  const uint16_t *ftlo[4];
+  float f16_scale[4];         // 2^s of each FP16 basis
   const float *inv_lambda;    // [H*W], ky*W + kx
 };
 

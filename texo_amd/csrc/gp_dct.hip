@@ -23,6 +23,7 @@ namespace {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -36,6 +37,32 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_to_float(unsigned short h) {
   return __uint_as_float(static_cast<unsigned int>(h) << 16);
+}
+
+// Power of two that brings max |a| into [2^14, 2^15), the top of FP16's
+// range, so that the hi/lo split keeps 22 significant bits.
+__device__ __forceinline__ float pow2_scale(float amax) {
+  if (!(amax > 0.f)) return 1.f;
+  int e = 0;
+  (void)frexpf(amax, &e);
+  e = e < -100 ? -100 : e;
+  return ldexpf(1.f, 15 - e);
+}
+
+__device__ __forceinline__ void split_f16(float x, _Float16 &hi, _Float16 &lo) {
+  hi = static_cast<_Float16>(x);
+  lo = static_cast<_Float16>(x - static_cast<float>(hi));  // x - hi is exact in FP32
+}
+
+__device__ __forceinline__ half4_t half4_of(uint2 v) {
+  half4_t h;
+  __builtin_memcpy(&h, &v, 8);
+  return h;
+}
+
+__device__ __forceinline__ float wave_max_f(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
 }
 
 // Stage geometry resolved per block from the task.
@@ -150,24 +177,60 @@ __device__ __forceinline__ void write_energy(const Stage &e, int local_rb, doubl
 }
 
 // ---------------------------------------------------------------------------
-// BF16x3 split-precision MFMA, fast shape (W, H multiples of 32): one
-// wavefront per 32x32 tile; A (FP32 activations) is split on the fly into
-// hi + lo BF16, B (static basis) is pre-split and pre-transposed so each
+// Fast shape (W, H multiples of 32): one 256-thread block per 32x32 output
+// tile, K split over its four wavefronts (wave w takes the 16-wide k-steps
+// w, w+4, ...).  The four partial tiles meet in LDS and are summed in wave
+// order, so results do not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int SPLIT = 4;
+constexpr int FAST_THREADS = 64 * SPLIT;
+
+// Sums the per-wave partial tiles red[w][32][33] and runs the epilogue:
+// thread t owns column t % 32 and rows 4 (t / 32) .. 4 (t / 32) + 3.
+__device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][33], int m0, int n0, int local_rb) {
+  const int col = threadIdx.x & 31, rg = threadIdx.x >> 5;
+  float4_t v;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = 4 * rg + q;
+    v[q] = ((red[0][row][col] + red[1][row][col]) + red[2][row][col]) + red[3][row][col];
+  }
+  double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
+  store4(e, m0 + 4 * rg, n0 + col, v, energy);
+  if (!e.energy) return;
+  __shared__ double se[SPLIT][MAXF];
+#pragma unroll
+  for (int k = 0; k < MAXF; ++k) energy[k] = wave_sum(energy[k]);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) se[wave][k] = energy[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < MAXF) {
+    const int k = threadIdx.x;
+    e.energy[((size_t)local_rb * gridDim.x + blockIdx.x) * MAXF + k] = ((se[0][k] + se[1][k]) + se[2][k]) + se[3][k];
+  }
+}
+
+// BF16x3 split-precision MFMA.  A (FP32 activations) is split on the fly
+// into hi + lo BF16, B (static basis) is pre-split and pre-transposed so each
 // fragment is one 8-byte load.  Each 16x16x16 step issues hi*hi + hi*lo +
 // lo*hi with FP32 accumulation.
 // v_mfma_f32_16x16x16_bf16 operand layout (64 lanes): A lane l holds
 // A[l%16][4*(l/16) + i], B lane l holds B[4*(l/16) + i][l%16], and the
 // accumulator lane l holds D[4*(l/16) + i][l%16], i = 0..3.
-// ---------------------------------------------------------------------------
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_bf16_fast(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+__global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__restrict__ tasks,
+                                                                 const int2 *__restrict__ map) {
+  __shared__ float red[SPLIT][32][33];
   const int2 bm = map[blockIdx.y];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.bthi[S]);
   const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.btlo[S]);
   const int K = e.K;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
   const int q = lane >> 4, r = lane & 15;
   const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
@@ -180,8 +243,8 @@ __global__ void __launch_bounds__(64) k_gemm_bf16_fast(const Task *__restrict__ 
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll 2
+  for (int k0 = 16 * wave; k0 < K; k0 += 16 * SPLIT) {
     const float4 av[2] = {*reinterpret_cast<const float4 *>(a0 + k0), *reinterpret_cast<const float4 *>(a1 + k0)};
     const uint2 bhv[2] = {bh0[k0 / 4], bh1[k0 / 4]};
     const uint2 blv[2] = {bl0[k0 / 4], bl1[k0 / 4]};
@@ -209,60 +272,155 @@ __global__ void __launch_bounds__(64) k_gemm_bf16_fast(const Task *__restrict__ 
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo[i], bhi[j], acc[i][j], 0, 0, 0);
       }
   }
-  double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) store4(e, m0 + 16 * i + 4 * q, n0 + 16 * j + r, acc[i][j], energy);
-  write_energy(e, bm.y, energy);
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) red[wave][16 * i + 4 * q + s][16 * j + r] = acc[i][j][s];
+  __syncthreads();
+  fast_epilogue(e, red, m0, n0, bm.y);
 }
 
-// FP32 vector-ALU, fast shape: one wavefront per 32x32 tile, 4x4 outputs per
-// lane, K staged through LDS in chunks of 32.
+// FP16-scaled split precision on MFMA.  A pre-pass finds max |A| over the
+// block's 32 rows; A is scaled by a power of two into the top of FP16's
+// range and split into hi + lo FP16 (22 significant bits); the basis is
+// pre-scaled, pre-split and pre-transposed.  hi*hi + hi*lo + lo*hi run on
+// v_mfma_f32_16x16x16_f16 with FP32 accumulation; the product of the two
+// power-of-two scales is divided out exactly before the epilogue.
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_f32_fast(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  __shared__ float As[32][33];
-  __shared__ float Bs[32][36];
+__global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__restrict__ tasks,
+                                                                 const int2 *__restrict__ map) {
+  __shared__ float red[SPLIT][32][33];
+  __shared__ float smax[SPLIT];
+  const int2 bm = map[blockIdx.y];
+  const Task &t = tasks[bm.x];
+  const Stage e = stage_of<S>(t);
+  const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.fthi[S]);
+  const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.ftlo[S]);
+  const int K = e.K;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int q = lane >> 4, r = lane & 15;
+  const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
+  const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
+  float amax = 0.f;
+  for (int k0 = 16 * wave; k0 < K; k0 += 16 * SPLIT) {
+    const float4 u = *reinterpret_cast<const float4 *>(a0 + k0);
+    const float4 v = *reinterpret_cast<const float4 *>(a1 + k0);
+    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(u.x), fabsf(u.y)), fmaxf(fabsf(u.z), fabsf(u.w))));
+    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+  }
+  amax = wave_max_f(amax);
+  if (lane == 0) smax[wave] = amax;
+  __syncthreads();
+  const float sa = pow2_scale(fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3])));
+  const float unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
+  const size_t k4 = (size_t)K / 4;
+  const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;
+  const uint2 *bl0 = Btl + (size_t)(n0 + r) * k4 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k4 + q;
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int k0 = 16 * wave; k0 < K; k0 += 16 * SPLIT) {
+    const float4 av[2] = {*reinterpret_cast<const float4 *>(a0 + k0), *reinterpret_cast<const float4 *>(a1 + k0)};
+    half4_t ahi[2], alo[2], bhi[2], blo[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float f4[4] = {av[i].x * sa, av[i].y * sa, av[i].z * sa, av[i].w * sa};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        _Float16 h, l;
+        split_f16(f4[s], h, l);
+        ahi[i][s] = h;
+        alo[i][s] = l;
+      }
+    }
+    bhi[0] = half4_of(bh0[k0 / 4]);
+    bhi[1] = half4_of(bh1[k0 / 4]);
+    blo[0] = half4_of(bl0[k0 / 4]);
+    blo[1] = half4_of(bl1[k0 / 4]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], blo[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo[i], bhi[j], acc[i][j], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) red[wave][16 * i + 4 * q + s][16 * j + r] = acc[i][j][s] * unscale;
+  __syncthreads();
+  fast_epilogue(e, red, m0, n0, bm.y);
+}
+
+// FP32 vector-ALU: each wave owns the 32-wide k-chunks w, w+4, ..., stages
+// them through its own LDS tiles and accumulates 4x4 outputs per lane.
+template <int S>
+__global__ void __launch_bounds__(FAST_THREADS) k_gemm_f32_fast(const Task *__restrict__ tasks,
+                                                                const int2 *__restrict__ map) {
+  __shared__ float As[SPLIT][32][33];
+  __shared__ float Bs[SPLIT][32][36];
+  __shared__ float red[SPLIT][32][33];
   const int2 bm = map[blockIdx.y];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   const float *B = t.j.b32[S];
   const int K = e.K;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tr = lane >> 3, tc = lane & 7;
   const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
   float acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  // every wave runs the same number of rounds so the block barriers match
+  const int chunks = K / 32, rounds = (chunks + SPLIT - 1) / SPLIT;
+  for (int c = 0; c < rounds; ++c) {
+    const int chunk = wave + SPLIT * c;
+    const bool active = chunk < chunks;
+    const int k0 = 32 * chunk;
+    if (active) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int idx = lane + 64 * s;
-      const int rr = idx >> 3, cc = (idx & 7) * 4;
-      const float4 a = *reinterpret_cast<const float4 *>(e.A + (size_t)(m0 + rr) * K + k0 + cc);
-      As[rr][cc] = a.x;
-      As[rr][cc + 1] = a.y;
-      As[rr][cc + 2] = a.z;
-      As[rr][cc + 3] = a.w;
-      *reinterpret_cast<float4 *>(&Bs[rr][cc]) = *reinterpret_cast<const float4 *>(B + (size_t)(k0 + rr) * e.N + n0 + cc);
+      for (int s = 0; s < 4; ++s) {
+        const int idx = lane + 64 * s;
+        const int rr = idx >> 3, cc = (idx & 7) * 4;
+        const float4 a = *reinterpret_cast<const float4 *>(e.A + (size_t)(m0 + rr) * K + k0 + cc);
+        As[wave][rr][cc] = a.x;
+        As[wave][rr][cc + 1] = a.y;
+        As[wave][rr][cc + 2] = a.z;
+        As[wave][rr][cc + 3] = a.w;
+        *reinterpret_cast<float4 *>(&Bs[wave][rr][cc]) =
+            *reinterpret_cast<const float4 *>(B + (size_t)(k0 + rr) * e.N + n0 + cc);
+      }
     }
     __syncthreads();
+    if (active) {
 #pragma unroll 8
-    for (int kk = 0; kk < 32; ++kk) {
-      const float4 b = *reinterpret_cast<const float4 *>(&Bs[kk][4 * tc]);
-      const float bv[4] = {b.x, b.y, b.z, b.w};
+      for (int kk = 0; kk < 32; ++kk) {
+        const float4 b = *reinterpret_cast<const float4 *>(&Bs[wave][kk][4 * tc]);
+        const float bv[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float a = As[4 * tr + i][kk];
+        for (int i = 0; i < 4; ++i) {
+          const float a = As[wave][4 * tr + i][kk];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a, bv[j], acc[i][j]);
+          for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a, bv[j], acc[i][j]);
+        }
       }
     }
     __syncthreads();
   }
-  double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    store4(e, m0 + 4 * tr, n0 + 4 * tc + j, float4_t{acc[0][j], acc[1][j], acc[2][j], acc[3][j]}, energy);
-  write_energy(e, bm.y, energy);
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red[wave][4 * tr + i][4 * tc + j] = acc[i][j];
+  __syncthreads();
+  fast_epilogue(e, red, m0, n0, bm.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -319,6 +477,73 @@ __global__ void __launch_bounds__(64) k_gemm_bf16_any(const Task *__restrict__ t
       for (int s = 0; s < 4; ++s) {
         const int row = m0 + 16 * i + 4 * q + s, col = n0 + 16 * j + r;
         if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j][s], energy);
+      }
+  write_energy(e, bm.y, energy);
+}
+
+// FP16-scaled split precision, any grid size (bounds-checked 32x32 tiles).
+template <int S>
+__global__ void __launch_bounds__(64) k_gemm_f16s_any(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.y];
+  const Task &t = tasks[bm.x];
+  const Stage e = stage_of<S>(t);
+  const unsigned short *Bth = t.j.fthi[S], *Btl = t.j.ftlo[S];
+  const int K = e.K;
+  const int lane = threadIdx.x;
+  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int q = lane >> 4, r = lane & 15;
+  float amax = 0.f;
+  for (int idx = lane; idx < 32 * K; idx += 64) {
+    const int row = m0 + idx / K;
+    if (row < e.M) amax = fmaxf(amax, fabsf(e.A[(size_t)row * K + idx % K]));
+  }
+  const float sa = pow2_scale(wave_max_f(amax));
+  const float unscale = 1.f / (sa * t.j.f16_scale[S]);
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    half4_t ahi[2], alo[2], bhi[2], blo[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int row = m0 + 16 * i + r, col = n0 + 16 * i + r;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int k = k0 + 4 * q + s;
+        const float v = (row < e.M && k < K) ? e.A[(size_t)row * K + k] * sa : 0.f;
+        _Float16 h, l;
+        split_f16(v, h, l);
+        ahi[i][s] = h;
+        alo[i][s] = l;
+        const bool in = col < e.N && k < K;
+        unsigned short bh = in ? Bth[(size_t)col * K + k] : 0, bl = in ? Btl[(size_t)col * K + k] : 0;
+        _Float16 fh, fl;
+        __builtin_memcpy(&fh, &bh, 2);
+        __builtin_memcpy(&fl, &bl, 2);
+        bhi[i][s] = fh;
+        blo[i][s] = fl;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], blo[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo[i], bhi[j], acc[i][j], 0, 0, 0);
+      }
+  }
+  double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int row = m0 + 16 * i + 4 * q + s, col = n0 + 16 * j + r;
+        if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j][s] * unscale, energy);
       }
   write_energy(e, bm.y, energy);
 }
@@ -381,12 +606,17 @@ void launch_stage(const Task *tasks, const DctLaunch &d, bool fast, hipStream_t 
   const int2 *map = h_rows ? d.map_h : d.map_w;
   if (d.precision == 1) {
     if (fast)
-      hipLaunchKernelGGL(k_gemm_bf16_fast<S>, grid, dim3(64), 0, s, tasks, map);
+      hipLaunchKernelGGL(k_gemm_bf16_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map);
     else
       hipLaunchKernelGGL(k_gemm_bf16_any<S>, grid, dim3(64), 0, s, tasks, map);
+  } else if (d.precision == 2) {
+    if (fast)
+      hipLaunchKernelGGL(k_gemm_f16s_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map);
+    else
+      hipLaunchKernelGGL(k_gemm_f16s_any<S>, grid, dim3(64), 0, s, tasks, map);
   } else {
     if (fast)
-      hipLaunchKernelGGL(k_gemm_f32_fast<S>, grid, dim3(64), 0, s, tasks, map);
+      hipLaunchKernelGGL(k_gemm_f32_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map);
     else
       hipLaunchKernelGGL(k_gemm_f32_any<S>, grid, dim3(64), 0, s, tasks, map);
   }

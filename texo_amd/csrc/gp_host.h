@@ -78,6 +78,8 @@ struct Basis {
   Arena arena;
   float *b32[4] = {};
   uint16_t *bthi[4] = {}, *btlo[4] = {};
+  uint16_t *fthi[4] = {}, *ftlo[4] = {};
+  float f16_scale[4] = {};
   float *inv_lambda = nullptr;
   void build(int W, int H);
 };

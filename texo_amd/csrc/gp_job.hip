@@ -138,6 +138,8 @@ void Basis::build(int w, int h) {
     arena.reserve(b32[j], dims[j] * dims[j]);
     arena.reserve(bthi[j], dims[j] * dims[j]);
     arena.reserve(btlo[j], dims[j] * dims[j]);
+    arena.reserve(fthi[j], dims[j] * dims[j]);
+    arena.reserve(ftlo[j], dims[j] * dims[j]);
   }
   arena.reserve(inv_lambda, (size_t)W * H);
   arena.commit();
@@ -167,7 +169,20 @@ void Basis::build(int w, int h) {
     const std::vector<double> &c = (j == 0 || j == 3) ? cw : ch;
     const bool transpose = j < 2;
     std::vector<float> f32(n * n);
-    std::vector<uint16_t> thi(n * n), tlo(n * n);
+    std::vector<uint16_t> thi(n * n), tlo(n * n), fhi(n * n), flo(n * n);
+    // FP16 splits of basis * 2^s with max |basis| * 2^s in [2^14, 2^15)
+    double bmax = 0.0;
+    for (size_t i = 0; i < n * n; ++i) bmax = std::max(bmax, std::fabs(c[i]));
+    int e = 0;
+    (void)std::frexp(bmax, &e);
+    const float scale = std::ldexp(1.0f, 15 - e);
+    f16_scale[j] = scale;
+    auto f16_bits = [](float f) {
+      const _Float16 h = static_cast<_Float16>(f);
+      uint16_t u;
+      std::memcpy(&u, &h, 2);
+      return u;
+    };
     for (size_t r = 0; r < n; ++r)
       for (size_t k = 0; k < n; ++k) {
         const double v = transpose ? c[k * n + r] : c[r * n + k];  // element (K index r, N index k)
@@ -177,10 +192,16 @@ void Basis::build(int w, int h) {
         const uint16_t lo = bf16(static_cast<float>(v - (double)bf16f(hi)));
         thi[k * n + r] = hi;  // transposed: (N index k, K index r)
         tlo[k * n + r] = lo;
+        const double sv = v * static_cast<double>(scale);
+        const float h = static_cast<float>(static_cast<_Float16>(static_cast<float>(sv)));
+        fhi[k * n + r] = f16_bits(h);
+        flo[k * n + r] = f16_bits(static_cast<float>(sv - static_cast<double>(h)));
       }
     HIP(hipMemcpy(b32[j], f32.data(), n * n * sizeof(float), hipMemcpyHostToDevice));
     HIP(hipMemcpy(bthi[j], thi.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP(hipMemcpy(btlo[j], tlo.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(fthi[j], fhi.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(ftlo[j], flo.data(), n * n * sizeof(uint16_t), hipMemcpyHostToDevice));
   }
   std::vector<float> il((size_t)W * H);
   for (int ky = 0; ky < H; ++ky) {
@@ -206,10 +227,6 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
     opt.inject = o->inject_failure_at_evaluation;
   }
   if (opt.precision > TEXO_GP_DCT_FP16_SCALED) fail(TEXO_GP_ERR_INVALID_ARGUMENT);
-  if (opt.precision == TEXO_GP_DCT_FP16_SCALED) {
-    g_last_error = "FP16-scaled transform is not available in this build";
-    fail(TEXO_GP_ERR_UNSUPPORTED);
-  }
   W = p->width;
   H = p->height;
   if (W == 0 || H == 0) fail(TEXO_GP_ERR_EMPTY_GRID);
@@ -629,6 +646,9 @@ void Job::upload(const Basis *b, hipStream_t s) {
     js.b32[j] = b->b32[j];
     js.bthi[j] = b->bthi[j];
     js.btlo[j] = b->btlo[j];
+    js.fthi[j] = b->fthi[j];
+    js.ftlo[j] = b->ftlo[j];
+    js.f16_scale[j] = b->f16_scale[j];
   }
   js.inv_lambda = b->inv_lambda;
 }

@@ -348,3 +348,32 @@ def serving_job(seed: int, cells: int, weight_exponent: int) -> abi.Problem:
 
 
 CONFIGS = {"plain_1k": plain_1k, "ecp5_25k": ecp5_25k, "ecp5_85k": ecp5_85k}
+
+
+def multistart(problem: abi.Problem, start: int) -> abi.Problem:
+    """The same design with the movable units' initial coordinates redrawn
+    (Gaussian around the die centre, sigma = side/12, clipped to each unit's
+    bounds) from seed `start`: one start of a multi-start placement."""
+    import dataclasses
+    rng = np.random.default_rng(1_000_003 * (start + 1))
+    W, H = problem.width, problem.height
+    movable = problem.unit_fixed == 0
+    init = problem.unit_initial_xy.reshape(-1, 2).copy()
+    b = problem.unit_bounds.reshape(-1, 4)
+    sigma = max(W, H) / 12.0
+    draw = (np.array([W - 1, H - 1]) / 2.0) + rng.normal(0.0, sigma, size=(int(movable.sum()), 2))
+    init[movable] = np.clip(draw, b[movable, :2], b[movable, 2:])
+    meta = dict(problem.meta, start=start)
+    return dataclasses.replace(problem, unit_initial_xy=init.reshape(-1), meta=meta)
+
+
+def serving_stream(n_jobs: int, seed: int = 2026, rate: float = 20.0, min_cells: int = 1000, max_cells: int = 84000):
+    """Mixed-size serving stream: cell counts log-uniform in [min, max],
+    weight exponents from {2, 4, 6}, per-job seeds, Poisson arrivals at `rate`
+    jobs/s.  Returns [(arrival_seconds, problem)]."""
+    rng = np.random.default_rng(seed)
+    cells = np.exp(rng.uniform(math.log(min_cells), math.log(max_cells), n_jobs)).astype(int)
+    exps = rng.choice([2, 4, 6], size=n_jobs)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
+    arrivals -= arrivals[0]
+    return [(float(arrivals[i]), serving_job(seed * 1000 + i, int(cells[i]), int(exps[i]))) for i in range(n_jobs)]
