@@ -42,25 +42,32 @@ FP16_PEAK_TFLOPS = 2500.0
 
 
 def algorithmic_bytes(kernel: str, sizes: dict, problem) -> float:
-    """Compulsory HBM bytes of one job's share of one launch of `kernel`
-    (each input read once, each output written once, at storage precision)."""
+    """Compulsory HBM bytes of one job's share of one launch of `kernel`: each
+    input read once and each output written once, at storage precision.
+    Gathered unit coordinates count once per coordinate, not once per use."""
     M = problem.n_members
     Fl = sizes["fillers"]
     F = sizes["fields"]
+    G = problem.width * problem.height
+    C = sizes["coordinates"]
+    U = problem.n_units
+    DU = sizes["density_units"]
+    pins = problem.pin_cell.size + 2 * problem.arc_extra_weight.size
+    nets = problem.n_nets + problem.arc_extra_weight.size
     if kernel == "deposit":
-        # coordinates (16 B), member offset/charge/field/unit (29 B) or filler
-        # charge/field (9 B), 9 int64 read-modify-writes (144 B)
-        return M * (16 + 29 + 144) + Fl * (16 + 9 + 144)
+        # coordinates (8 B each); per charge: coordinate index 4, offset 16,
+        # charge 8, layer 1; int64 accumulators of members and fillers per
+        # field and bin: initial value read 8 + sum written 8
+        return C * 8 + (M + Fl) * 29 + F * G * 2 * 16
     if kernel == "wa":
-        pins = problem.pin_cell.size + 2 * problem.arc_extra_weight.size
-        nets = problem.n_nets + problem.arc_extra_weight.size
-        # per pin: unit 4 + offset 16 + coordinates 16 + derivative 16; per net: start 4 + weight 8 + id 4
-        return pins * 52 + nets * 16
+        # per pin: unit 4 + offset/fixed xy 16 + derivative written 16; unit
+        # coordinates 16; per net: start 4 + weight 8 + id 4
+        return pins * 36 + U * 16 + nets * 16
     if kernel == "gather":
-        DU = sizes["density_units"]
-        # per unit: unit id, member range, coordinates; per member 3x3 phi taps (36 B FP32) + offset/charge/field;
-        # outputs coef*G per field (16 B) per unit
-        return DU * (4 + 8 + 16 + 16 * F) + M * (36 + 25)
+        # potential F*G FP32; per density unit: unit id 4, member range 4,
+        # coordinates 16, scaled field gradient written 16 per field; per
+        # member: offset 16, charge 8, field 1
+        return F * G * 4 + DU * (24 + 16 * F) + M * 25
     raise ValueError(kernel)
 
 
@@ -173,7 +180,20 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-batched", action="store_true", help="time the batched form only (for profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (reference reading for the HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the placement loop of one job runs on one "
                                    "device, and runs on several GPUs are not part of it", "n_gpus": args.gpus}))
@@ -186,10 +206,13 @@ def main():
     mfma = prec[args.precision]
     base = synth.ecp5_85k()
 
-    plain = run_single(base, abi.DCT_FP32, args.steps, args.warmup)
-    progress(f"plain fp32 single job: {plain['it_per_s']:.1f} it/s")
-    single = run_single(base, mfma, args.steps, args.warmup)
-    progress(f"{args.precision} single job: {single['it_per_s']:.1f} it/s")
+    if args.only_batched:
+        plain = single = {"it_per_s": float("nan")}
+    else:
+        plain = run_single(base, abi.DCT_FP32, args.steps, args.warmup)
+        progress(f"plain fp32 single job: {plain['it_per_s']:.1f} it/s")
+        single = run_single(base, mfma, args.steps, args.warmup)
+        progress(f"{args.precision} single job: {single['it_per_s']:.1f} it/s")
     starts = [synth.multistart(base, s) for s in range(args.batch)]
     batched, sizes = run_batched(starts, mfma, args.steps, args.warmup, args.roofline_kernel)
     progress(f"{args.precision} batch of {args.batch}: {batched['it_per_s']:.1f} it/s")
@@ -212,7 +235,7 @@ def main():
                     "jobs_per_launch": args.batch}
 
     serving = None
-    if not args.no_serving:
+    if not args.no_serving and not args.only_batched:
         stream = synth.serving_stream(args.serving_jobs, rate=args.serving_rate)
         serving = {"jobs": args.serving_jobs, "arrival_rate_jobs_per_s": args.serving_rate,
                    "cells": "log-uniform 1k-84k, grid side nextpow2(sqrt(cells/4)) in 64-256, "

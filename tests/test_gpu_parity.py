@@ -199,3 +199,28 @@ def test_small_serving_job_converges_like_the_oracle(precision):
     print("gpu", rg, "oracle", rep.as_dict(), flush=True)
     assert max(rg["field_overflow"]) <= 0.10
     assert abs(rg["final_hpwl"] - rep.final_hpwl) <= 0.01 * rep.final_hpwl
+
+
+@pytest.mark.parametrize("name", ["plain_1k", "ecp5_25k"])
+def test_density_is_identical_for_any_charge_order(name):
+    # A fresh session deposits in input order (blocks span the die: HBM
+    # atomics); after iterations the charges are sorted by tile (LDS windows).
+    # The int64 sums must agree bit for bit, and with the oracle.
+    p = synth.CONFIGS[name]()
+    fresh = texo_amd.Session(p)
+    moved = texo_amd.Session(p)
+    moved.iterate(9)
+    o = abi.Session(oracle.load(), "oracle", p)
+    rng = np.random.default_rng(3)
+    for x in (moved.coordinates(), np.clip(fresh.initial + rng.normal(0, 2.0, fresh.initial.size), fresh.lower,
+                                           fresh.upper)):
+        a = fresh.evaluate(x, 4.0, want_grids=True)
+        b = moved.evaluate(x, 4.0, want_grids=True)
+        r = o.evaluate(x, 4.0, want_grids=True)
+        assert np.array_equal(a["density"], b["density"])
+        assert np.array_equal(a["overflow"], b["overflow"])
+        charges = p.n_members + fresh.n_fillers
+        assert np.abs(a["density"] - r["density"]).max() <= 9 * charges * FX_QUANTUM
+    fresh.close()
+    moved.close()
+    o.close()

@@ -58,6 +58,19 @@ struct MetricScalars {
   unsigned int pad;
 };
 
+constexpr int SORT_TILE_SHIFT = 3;  // charges are binned by 8x8-bin tiles
+constexpr int DEPOSIT_CHUNK = 2 * BLOCK;  // charges per deposit block
+
+// Centre bin of the quadratic B-spline stencil (electrostatic_placement.rs:950-976);
+// a NaN coordinate lands on the last bin, as in bspline_axis.
+__device__ __forceinline__ int bin_center(int length, double coordinate) {
+  const double maximum = static_cast<double>(length - 1);
+  const double c = coordinate < 0.0 ? 0.0 : (coordinate > maximum ? maximum : coordinate);
+  double cc = floor(c + 0.5);
+  cc = cc < maximum ? cc : maximum;
+  return static_cast<int>(cc);
+}
+
 // Quadratic B-spline axis stencil (electrostatic_placement.rs:950-976).
 struct Axis {
   int i0, i1, i2;
@@ -99,6 +112,14 @@ struct JobStatic {
   const double *fl_charge;    // filler charge
   const long long *base_fx;   // [F*G] -smoothed capacity, fixed point
   const int32_t *unit_du;     // unit -> density unit or -1
+  const uint32_t *perm;       // deposit order of the charges (members, then fillers)
+  int tiles_x, n_keys;        // 8x8-bin tiles of the charge sort
+  // charges in deposit order: coordinate index (unit, or n_units + filler),
+  // offset from it, charge, layer = 2 field + (1 for a filler)
+  const uint32_t *s_ci;
+  const double2 *s_off;
+  const double *s_q;
+  const uint8_t *s_layer;
   const double *lower, *upper;
   // hypergraph: nets then timed arcs as weighted 2-pin nets
   int n_small, n_large;
@@ -142,6 +163,27 @@ struct Task {
   double *raw_units, *raw_fillers, *density_out, *wa_grad_out;
 };
 
+// Spatial sort of one job's charges by 8x8-bin tile of their stencil centre
+// (counting sort).  The order only steers which charges share a deposit
+// block; the integer density sums do not depend on it.
+struct SortTask {
+  JobStatic j;
+  const double *coords;
+  uint32_t *count;  // [n_keys + 1]
+  uint32_t *perm;   // [members + fillers]
+  uint32_t *s_ci;   // the charges in the new order
+  double2 *s_off;
+  double *s_q;
+  uint8_t *s_layer;
+};
+struct SortMaps {
+  const int2 *charges, *keys;
+  int nb_charges, nb_keys;
+  const int *list;  // one task per block of the scan
+  int n_list;
+};
+void launch_sort(const SortTask *tasks, const SortMaps &m, hipStream_t s);
+
 // One optimizer elementwise request of one job.
 struct OptTask {
   size_t n;  // coordinates
@@ -170,7 +212,10 @@ struct OptTask {
   uint32_t *targets;
 };
 
-// Block maps of one round: entry b = (task index, local block index).
+// Block maps of one round: entry b = (task index, local block index).  The
+// host interleaves each map so that every XCD (blocks are dispatched to the
+// eight XCDs round-robin) receives a contiguous range of a task's blocks and
+// its L2 caches one job's arrays; entries with task -1 are padding.
 struct Maps {
   const int2 *deposit, *convert, *gather_units, *gather_fillers, *wa, *unit_grad;
   int nb_deposit, nb_convert, nb_gather_units, nb_gather_fillers, nb_wa, nb_unit_grad;

@@ -47,6 +47,9 @@ constexpr double INITIAL_DENSITY_WEIGHT = 1.0e-4;  // eplace.rs:126
 constexpr double AREA_RESET_DENSITY_WEIGHT = 0.1;  // eplace.rs:127
 constexpr double MULTIPLIER_ALPHA_HIGH = 1.06;     // eplace.rs:129
 constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
+// Iterations between re-sorts of a job's charges for the deposit; the order
+// only affects speed, never the density sums.
+constexpr uint64_t SORT_PERIOD = 8;
 
 __global__ void k_round_targets_raw(int n_units, const double *__restrict__ coords, int W, int H,
                                     uint32_t *__restrict__ out) {
@@ -174,11 +177,12 @@ void Engine::ensure_stage(size_t bytes) {
 // requests for the next round are recorded.
 void Engine::plan(Job *j) {
   j->evals.clear();
-  j->want_precond = j->want_extrapolate = j->want_candidate = false;
+  j->want_precond = j->want_extrapolate = j->want_candidate = j->want_sort = false;
   for (;;) {
     switch (j->phase) {
       case Phase::Probe: {
         // density probe at gamma = base (eplace.rs:455)
+        j->want_sort = true;
         j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->opt.gamma_base, false, false, 0});
         return;
       }
@@ -222,6 +226,7 @@ void Engine::plan(Job *j) {
         j->gamma = j->adaptive_gamma(j->current);
         j->iterations += 1;
         j->local_iterations += 1;
+        j->want_sort = j->iterations % SORT_PERIOD == 0;
         if (!std::isfinite(j->momentum) || j->momentum < 1.0) fail(TEXO_GP_ERR_INVALID_MOMENTUM);
         if (j->has_lipschitz && (!std::isfinite(j->lipschitz) || j->lipschitz <= 0.0))
           fail(TEXO_GP_ERR_INVALID_LIPSCHITZ);
@@ -647,6 +652,26 @@ void add_blocks(std::vector<int2> &map, int task, size_t n, size_t per_block) {
   for (int b = 0; b < nb; ++b) map.push_back(make_int2(task, b));
 }
 
+// Workgroups are dispatched to the XCDs round-robin (block b -> XCD b % 8).
+// Reorders a block map so that XCD x runs the contiguous slice
+// [x * per, (x + 1) * per) of the task-ordered list: a slice holds one job
+// (or one spatial part of a job), so each XCD's L2 caches one job's
+// coordinates and fields instead of all of them.  Padding entries carry
+// task -1.  Only speed depends on the dispatch order.
+constexpr int XCDS = 8;
+std::vector<int2> xcd_interleave(const std::vector<int2> &blocks) {
+  const size_t n = blocks.size();
+  if (n < 2 * XCDS) return blocks;
+  const size_t per = (n + XCDS - 1) / XCDS;
+  std::vector<int2> out(XCDS * per, make_int2(-1, -1));
+  for (size_t x = 0; x < XCDS; ++x)
+    for (size_t i = 0; i < per; ++i) {
+      const size_t src = x * per + i;
+      if (src < n) out[XCDS * i + x] = blocks[src];
+    }
+  return out;
+}
+
 }  // namespace
 
 void Engine::round(const std::vector<Job *> &jobs) {
@@ -671,6 +696,9 @@ void Engine::round(const std::vector<Job *> &jobs) {
 
   // ---- tables ----
   std::vector<OptTask> opt;
+  std::vector<SortTask> sorts;
+  std::vector<int2> m_sort_c, m_sort_k;
+  std::vector<int> l_sort;
   std::vector<int2> m_pre, m_ext, m_cand;
   std::vector<int> l_pre, l_cand;
   std::vector<Task> tasks;
@@ -680,6 +708,13 @@ void Engine::round(const std::vector<Job *> &jobs) {
   };
   std::map<std::tuple<int, int, int>, Group> groups;
   for (Job *j : active) {
+    if (j->want_sort && j->F > 0 && j->M + j->Fl > 0) {
+      const int si = (int)sorts.size();
+      sorts.push_back(j->sort_task());
+      add_blocks(m_sort_c, si, j->M + j->Fl, BLOCK);
+      add_blocks(m_sort_k, si, (size_t)j->n_keys + 1, BLOCK);
+      l_sort.push_back(si);
+    }
     if (j->want_precond || j->want_extrapolate || j->want_candidate) {
       OptTask o{};
       o.n = j->C;
@@ -734,7 +769,7 @@ void Engine::round(const std::vector<Job *> &jobs) {
       const int ti = (int)tasks.size();
       tasks.push_back(t);
       if (j->F > 0 && !r.skip) {
-        add_blocks(m_dep, ti, j->M + j->Fl, BLOCK);
+        add_blocks(m_dep, ti, j->M + j->Fl, DEPOSIT_CHUNK);
         add_blocks(m_conv, ti, (size_t)j->F * j->G, BLOCK);
         Group &g = groups[std::make_tuple((int)j->W, (int)j->H, (int)j->opt.precision)];
         add_blocks(g.h, ti, (size_t)j->F * j->H, 32);
@@ -749,10 +784,14 @@ void Engine::round(const std::vector<Job *> &jobs) {
       for (int b = 0; b < j->nb_ug; ++b) m_ug.push_back(make_int2(ti, b));
     }
   }
+  for (auto *m : {&m_sort_c, &m_sort_k, &m_pre, &m_ext, &m_cand, &m_dep, &m_conv, &m_gu, &m_gf, &m_wa, &m_ug})
+    *m = xcd_interleave(*m);
   std::vector<char> buf;
   const size_t o_opt = append(buf, opt), o_tasks = append(buf, tasks);
   const size_t o_pre = append(buf, m_pre), o_ext = append(buf, m_ext), o_cand = append(buf, m_cand);
   const size_t o_lpre = append(buf, l_pre), o_lcand = append(buf, l_cand);
+  const size_t o_sorts = append(buf, sorts), o_sort_c = append(buf, m_sort_c), o_sort_k = append(buf, m_sort_k),
+               o_lsort = append(buf, l_sort);
   const size_t o_dep = append(buf, m_dep), o_conv = append(buf, m_conv), o_gu = append(buf, m_gu),
                o_gf = append(buf, m_gf), o_wa = append(buf, m_wa), o_ug = append(buf, m_ug);
   std::vector<std::pair<size_t, size_t>> o_groups;
@@ -778,6 +817,16 @@ void Engine::round(const std::vector<Job *> &jobs) {
   om.n_precond = (int)l_pre.size();
   om.n_candidate = (int)l_cand.size();
   if (!opt.empty()) launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, stream_);
+  if (!sorts.empty()) {
+    SortMaps sm{};
+    sm.charges = reinterpret_cast<const int2 *>(dptr(o_sort_c));
+    sm.keys = reinterpret_cast<const int2 *>(dptr(o_sort_k));
+    sm.nb_charges = (int)m_sort_c.size();
+    sm.nb_keys = (int)m_sort_k.size();
+    sm.list = reinterpret_cast<const int *>(dptr(o_lsort));
+    sm.n_list = (int)l_sort.size();
+    launch_sort(reinterpret_cast<const SortTask *>(dptr(o_sorts)), sm, stream_);
+  }
 
   Maps mp{};
   mp.deposit = reinterpret_cast<const int2 *>(dptr(o_dep));
@@ -855,7 +904,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   tasks.push_back(t);
   std::vector<int2> m_dep, m_conv, m_gu, m_gf, m_wa, m_ug, gh, gw;
   if (j->F > 0) {
-    add_blocks(m_dep, 0, j->M + j->Fl, BLOCK);
+    add_blocks(m_dep, 0, j->M + j->Fl, DEPOSIT_CHUNK);
     add_blocks(m_conv, 0, (size_t)j->F * j->G, BLOCK);
     add_blocks(gh, 0, (size_t)j->F * j->H, 32);
     add_blocks(gw, 0, (size_t)j->F * j->W, 32);
@@ -865,6 +914,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   }
   for (int b = 0; b < j->nb_wa_large + j->nb_wa_small; ++b) m_wa.push_back(make_int2(0, b));
   for (int b = 0; b < j->nb_ug; ++b) m_ug.push_back(make_int2(0, b));
+  for (auto *m : {&m_dep, &m_conv, &m_gu, &m_gf, &m_wa, &m_ug}) *m = xcd_interleave(*m);
   std::vector<char> buf;
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, m_dep), o_conv = append(buf, m_conv),
                o_gu = append(buf, m_gu), o_gf = append(buf, m_gf), o_wa = append(buf, m_wa), o_ug = append(buf, m_ug),

@@ -166,12 +166,18 @@ struct Job {
   unsigned int *d_rudy_err = nullptr;
   double *d_raw_units = nullptr, *d_raw_fillers = nullptr, *d_density_out = nullptr, *d_wa_grad = nullptr;
   double *d_eval_coords = nullptr;
+  uint32_t *d_perm = nullptr, *d_sort_count = nullptr, *d_s_ci = nullptr;
+  double2 *d_s_off = nullptr;
+  double *d_s_q = nullptr;
+  uint8_t *d_s_layer = nullptr;
+  int tiles_x = 0, n_keys = 0;
   int nb_wa_small = 0, nb_wa_large = 0, nb_energy = 0, nb_gu = 0, nb_gf = 0, nb_ug = 0;
 
   void upload(const Basis *b, hipStream_t s);
   void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas);
   Task task(int slot, const double *coords, double *grad, const double *metric, double gamma, bool skip,
             EvalScalars *scalars) const;
+  SortTask sort_task() const;
 
   // ---- engine bookkeeping ----
   int64_t id = 0;
@@ -223,7 +229,7 @@ struct Job {
     int scalar_slot;
   };
   std::vector<EvalReq> evals;
-  bool want_precond = false, want_extrapolate = false, want_candidate = false;
+  bool want_precond = false, want_extrapolate = false, want_candidate = false, want_sort = false;
   std::vector<double> targets_out;  // unused placeholder for symmetry
 
   // schedule helpers

@@ -498,6 +498,7 @@ size_t Job::device_bytes() const {
   b += C * 8 * (3 + 4 + 3) + 2 * U * 4;
   b += 2 * (FG * (8 + 8 + 4 * 4) + (size_t)std::max(F, 1) * DU * 16 + n_pins_total * 16);
   b += 2 * G * 16 + FG * 8 + 2 * U * 8 + C * 8;
+  b += (M + Fl) * (4 + 4 + 16 + 8 + 1) + ((G >> (2 * SORT_TILE_SHIFT)) + W + H + 1) * 4;
   return b + 64 * 4096;
 }
 
@@ -511,6 +512,8 @@ void Job::upload(const Basis *b, hipStream_t s) {
   nb_gu = (int)blocks(DU, BLOCK);
   nb_gf = (int)blocks(Fl, BLOCK);
   nb_ug = (int)blocks(U, BLOCK);
+  tiles_x = (int)((W + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
+  n_keys = tiles_x * (int)((H + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
   arena.reserve(d_base, FG);
   arena.reserve(d_m_unit, M);
   arena.reserve(d_m_off, M);
@@ -562,6 +565,12 @@ void Job::upload(const Basis *b, hipStream_t s) {
   arena.reserve(d_density_out, FG);
   arena.reserve(d_wa_grad, 2 * U);
   arena.reserve(d_eval_coords, C);
+  arena.reserve(d_perm, M + Fl);
+  arena.reserve(d_s_ci, M + Fl);
+  arena.reserve(d_s_off, M + Fl);
+  arena.reserve(d_s_q, M + Fl);
+  arena.reserve(d_s_layer, M + Fl);
+  arena.reserve(d_sort_count, (size_t)n_keys + 1);
   arena.commit();
 
   auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -629,6 +638,13 @@ void Job::upload(const Basis *b, hipStream_t s) {
   js.fl_charge = d_fl_charge;
   js.base_fx = d_base;
   js.unit_du = d_unit_du;
+  js.perm = d_perm;
+  js.s_ci = d_s_ci;
+  js.s_off = d_s_off;
+  js.s_q = d_s_q;
+  js.s_layer = d_s_layer;
+  js.tiles_x = tiles_x;
+  js.n_keys = n_keys;
   js.lower = d_lower;
   js.upper = d_upper;
   js.n_small = (int)small_nets.size();
@@ -665,6 +681,34 @@ void Job::apply_charges(const std::vector<double> &member_areas, const std::vect
   if (M) HIP(hipMemcpy(d_m_charge, m_charge.data(), M * 8, hipMemcpyHostToDevice));
   if (Fl) HIP(hipMemcpy(d_fl_charge, fl_charge.data(), Fl * 8, hipMemcpyHostToDevice));
   if (DU) HIP(hipMemcpy(d_du_fcharge, fcharge.data(), fcharge.size() * 8, hipMemcpyHostToDevice));
+  // deposit order restarts as the input order with the new charges; the
+  // next sort reorders it
+  const size_t n = M + Fl;
+  std::vector<uint32_t> perm(n), ci(n);
+  std::vector<double> off(2 * n, 0.0), q(n);
+  std::vector<uint8_t> layer(n);
+  for (size_t i = 0; i < M; ++i) {
+    perm[i] = (uint32_t)i;
+    ci[i] = m_unit[i];
+    off[2 * i] = m_off[2 * i];
+    off[2 * i + 1] = m_off[2 * i + 1];
+    q[i] = m_charge[i];
+    layer[i] = (uint8_t)(2 * m_field[i]);
+  }
+  for (size_t k = 0; k < Fl; ++k) {
+    const size_t i = M + k;
+    perm[i] = (uint32_t)i;
+    ci[i] = (uint32_t)(U + k);
+    q[i] = fl_charge[k];
+    layer[i] = (uint8_t)(2 * fl_field[k] + 1);
+  }
+  if (n) {
+    HIP(hipMemcpy(d_perm, perm.data(), n * 4, hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_s_ci, ci.data(), n * 4, hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_s_off, off.data(), n * 16, hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_s_q, q.data(), n * 8, hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_s_layer, layer.data(), n, hipMemcpyHostToDevice));
+  }
 }
 
 Task Job::task(int sl, const double *coords, double *grad, const double *metric, double g, bool skip,
@@ -701,6 +745,19 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
     t.scale[f] = f < (int)scales.size() ? scales[f] : 1.0;
   }
   t.scalars = scalars;
+  return t;
+}
+
+SortTask Job::sort_task() const {
+  SortTask t{};
+  t.j = js;
+  t.coords = d_vec[ix];
+  t.count = d_sort_count;
+  t.perm = d_perm;
+  t.s_ci = d_s_ci;
+  t.s_off = d_s_off;
+  t.s_q = d_s_q;
+  t.s_layer = d_s_layer;
   return t;
 }
 

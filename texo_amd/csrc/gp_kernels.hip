@@ -124,6 +124,7 @@ __device__ __forceinline__ double kkt_scaled(double g, double x, double lo, doub
 // acc_m = -smoothed capacity, acc_f = 0; the task's int64 sums restart.
 __global__ void __launch_bounds__(BLOCK) k_deposit_init(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const Task &t = tasks[bm.x];
   const int total = t.j.F * t.j.W * t.j.H;
   const int i = bm.y * BLOCK + threadIdx.x;
@@ -137,48 +138,228 @@ __global__ void __launch_bounds__(BLOCK) k_deposit_init(const Task *__restrict__
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  const Task &t = tasks[bm.x];
-  const JobStatic &a = t.j;
-  const int i = bm.y * BLOCK + threadIdx.x;
+// Position and charge of one deposit charge.
+struct Charge {
   double x, y, q;
   int f;
-  long long *acc;
+  bool filler;
+};
+
+// One block deposits DEP_CHUNK consecutive charges of the sort order.  The
+// block first takes the bounding box of its stencils; when the box (times
+// the fields it spans, times members/fillers) fits the LDS window, the
+// charges accumulate there with LDS int64 atomics and the block adds each
+// touched window cell to HBM with one atomic; otherwise every stencil cell
+// goes to HBM directly.  Either way each cell receives the same int64
+// quanta, so the density is bit-identical whatever the order or the path.
+constexpr int DEP_PER_THREAD = DEPOSIT_CHUNK / BLOCK;
+constexpr int DEP_CHUNK = DEPOSIT_CHUNK;
+constexpr int DEP_WINDOW = 6144;  // int64 cells (48 KB)
+
+__global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+  __shared__ unsigned long long win[DEP_WINDOW];
+  __shared__ int bb[6];  // x0, x1, y0, y1, f0, f1
+  const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+  const Task &t = tasks[bm.x];
+  const JobStatic &a = t.j;
+  const int total = a.n_members + a.n_fillers;
+  const size_t G = (size_t)a.W * (size_t)a.H;
+  if (threadIdx.x == 0) {
+    bb[0] = bb[2] = bb[4] = 0x7fffffff;
+    bb[1] = bb[3] = bb[5] = -1;
+  }
+  __syncthreads();
+  Axis ax[DEP_PER_THREAD], ay[DEP_PER_THREAD];
+  double q[DEP_PER_THREAD];
+  int f[DEP_PER_THREAD];
+  bool filler[DEP_PER_THREAD], valid[DEP_PER_THREAD];
+  int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};  // x, y, field
+#pragma unroll
+  for (int k = 0; k < DEP_PER_THREAD; ++k) {
+    const int i = bm.y * DEP_CHUNK + k * BLOCK + threadIdx.x;
+    valid[k] = i < total;
+    if (valid[k]) {
+      // a filler's offset is +0.0, which leaves its coordinate unchanged
+      const uint32_t ci = a.s_ci[i];
+      const double2 off = a.s_off[i];
+      const int layer = a.s_layer[i];
+      const Charge c = {t.coords[2 * (size_t)ci] + off.x, t.coords[2 * (size_t)ci + 1] + off.y, a.s_q[i], layer >> 1,
+                        (layer & 1) != 0};
+      bspline_axis(a.W, c.x, ax[k]);
+      bspline_axis(a.H, c.y, ay[k]);
+      q[k] = c.q;
+      f[k] = c.f;
+      filler[k] = c.filler;
+      lo[0] = min(lo[0], ax[k].i0);
+      hi[0] = max(hi[0], ax[k].i2);
+      lo[1] = min(lo[1], ay[k].i0);
+      hi[1] = max(hi[1], ay[k].i2);
+      lo[2] = min(lo[2], c.f);
+      hi[2] = max(hi[2], c.f);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], off, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], off, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0 && hi[0] >= 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&bb[2 * d], lo[d]);
+      atomicMax(&bb[2 * d + 1], hi[d]);
+    }
+  }
+  __syncthreads();
+  const int x0 = bb[0], y0 = bb[2], f0 = bb[4];
+  const int nx = bb[1] - x0 + 1, ny = bb[3] - y0 + 1, nf = bb[5] - f0 + 1;
+  if (bb[1] < 0) return;  // no charge in this block (uniform)
+  const int cells = nx * ny * nf * 2;
+  if (cells <= DEP_WINDOW) {
+    for (int idx = threadIdx.x; idx < cells; idx += BLOCK) win[idx] = 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < DEP_PER_THREAD; ++k) {
+      if (!valid[k]) continue;
+      const int ix[3] = {ax[k].i0, ax[k].i1, ax[k].i2};
+      const int iy[3] = {ay[k].i0, ay[k].i1, ay[k].i2};
+      const double wx[3] = {ax[k].w0, ax[k].w1, ax[k].w2};
+      const double wy[3] = {ay[k].w0, ay[k].w1, ay[k].w2};
+      const int layer = 2 * (f[k] - f0) + (filler[k] ? 1 : 0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          // charge * column.weight * row.weight, quantised before the add
+          const double v = q[k] * wx[c] * wy[r];
+          atomicAdd(&win[((size_t)layer * ny + (iy[r] - y0)) * nx + (ix[c] - x0)],
+                    static_cast<unsigned long long>(__double2ll_rn(v * FX_SCALE)));
+        }
+      }
+    }
+    __syncthreads();
+    const int plane = nx * ny;
+    for (int idx = threadIdx.x; idx < cells; idx += BLOCK) {
+      const unsigned long long v = win[idx];
+      if (v == 0ull) continue;
+      const int layer = idx / plane, rem = idx - layer * plane;
+      const int y = y0 + rem / nx, x = x0 + rem % nx;
+      long long *acc = (layer & 1) ? t.acc_f : t.acc_m;
+      atomicAdd(reinterpret_cast<unsigned long long *>(acc + (size_t)(f0 + (layer >> 1)) * G + (size_t)y * a.W + x),
+                v);
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < DEP_PER_THREAD; ++k) {
+    if (!valid[k]) continue;
+    unsigned long long *g =
+        reinterpret_cast<unsigned long long *>((filler[k] ? t.acc_f : t.acc_m) + (size_t)f[k] * G);
+    const int ix[3] = {ax[k].i0, ax[k].i1, ax[k].i2};
+    const int iy[3] = {ay[k].i0, ay[k].i1, ay[k].i2};
+    const double wx[3] = {ax[k].w0, ax[k].w1, ax[k].w2};
+    const double wy[3] = {ay[k].w0, ay[k].w1, ay[k].w2};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double v = q[k] * wx[c] * wy[r];
+        atomicAdd(g + (size_t)iy[r] * a.W + ix[c], static_cast<unsigned long long>(__double2ll_rn(v * FX_SCALE)));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Counting sort of charges by 8x8-bin tile (clear, count, scan, scatter).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int charge_key(const JobStatic &a, const double *__restrict__ coords, int i) {
+  double x, y;
   if (i < a.n_members) {
     const uint32_t u = a.m_unit[i];
     const double2 off = a.m_off[i];
-    x = t.coords[2 * (size_t)u] + off.x;
-    y = t.coords[2 * (size_t)u + 1] + off.y;
-    q = a.m_charge[i];
-    f = a.m_field[i];
-    acc = t.acc_m;
-  } else if (i < a.n_members + a.n_fillers) {
-    const int k = i - a.n_members;
-    const size_t c = 2 * ((size_t)a.n_units + k);
-    x = t.coords[c];
-    y = t.coords[c + 1];
-    q = a.fl_charge[k];
-    f = a.fl_field[k];
-    acc = t.acc_f;
+    x = coords[2 * (size_t)u] + off.x;
+    y = coords[2 * (size_t)u + 1] + off.y;
   } else {
-    return;
+    const size_t o = 2 * ((size_t)a.n_units + (i - a.n_members));
+    x = coords[o];
+    y = coords[o + 1];
   }
-  Axis ax, ay;
-  bspline_axis(a.W, x, ax);
-  bspline_axis(a.H, y, ay);
-  unsigned long long *g = reinterpret_cast<unsigned long long *>(acc + (size_t)f * (size_t)a.W * (size_t)a.H);
-  const int ix[3] = {ax.i0, ax.i1, ax.i2};
-  const int iy[3] = {ay.i0, ay.i1, ay.i2};
-  const double wx[3] = {ax.w0, ax.w1, ax.w2};
-  const double wy[3] = {ay.w0, ay.w1, ay.w2};
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      // charge * column.weight * row.weight, quantised before the add
-      const double v = q * wx[c] * wy[r];
-      atomicAdd(g + (size_t)iy[r] * a.W + ix[c], static_cast<unsigned long long>(__double2ll_rn(v * FX_SCALE)));
+  const int cx = bin_center(a.W, x), cy = bin_center(a.H, y);
+  return (cy >> SORT_TILE_SHIFT) * a.tiles_x + (cx >> SORT_TILE_SHIFT);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sort_clear(const SortTask *__restrict__ tasks,
+                                                      const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+  const SortTask &t = tasks[bm.x];
+  const int i = bm.y * BLOCK + threadIdx.x;
+  if (i <= t.j.n_keys) t.count[i] = 0u;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sort_count(const SortTask *__restrict__ tasks,
+                                                      const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+  const SortTask &t = tasks[bm.x];
+  const int i = bm.y * BLOCK + threadIdx.x;
+  if (i < t.j.n_members + t.j.n_fillers) atomicAdd(&t.count[charge_key(t.j, t.coords, i)], 1u);
+}
+
+// Exclusive scan of the key counts; one block per task.
+__global__ void __launch_bounds__(BLOCK) k_sort_scan(const SortTask *__restrict__ tasks,
+                                                     const int *__restrict__ list) {
+  __shared__ uint32_t part[BLOCK];
+  const SortTask &t = tasks[list[blockIdx.x]];
+  const int n = t.j.n_keys;
+  const int per = (n + BLOCK - 1) / BLOCK;
+  const int b = threadIdx.x * per, e = min(n, b + per);
+  uint32_t sum = 0;
+  for (int i = b; i < e; ++i) sum += t.count[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int k = 0; k < BLOCK; ++k) {
+      const uint32_t v = part[k];
+      part[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  uint32_t run = part[threadIdx.x];
+  for (int i = b; i < e; ++i) {
+    const uint32_t v = t.count[i];
+    t.count[i] = run;
+    run += v;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sort_scatter(const SortTask *__restrict__ tasks,
+                                                        const int2 *__restrict__ map) {
+  const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+  const SortTask &t = tasks[bm.x];
+  const int i = bm.y * BLOCK + threadIdx.x;
+  if (i < t.j.n_members + t.j.n_fillers) {
+    const JobStatic &a = t.j;
+    const uint32_t pos = atomicAdd(&t.count[charge_key(a, t.coords, i)], 1u);
+    t.perm[pos] = static_cast<uint32_t>(i);
+    if (i < a.n_members) {
+      t.s_ci[pos] = a.m_unit[i];
+      t.s_off[pos] = a.m_off[i];
+      t.s_q[pos] = a.m_charge[i];
+      t.s_layer[pos] = static_cast<uint8_t>(2 * a.m_field[i]);
+    } else {
+      const int k = i - a.n_members;
+      t.s_ci[pos] = static_cast<uint32_t>(a.n_units + k);
+      t.s_off[pos] = make_double2(0.0, 0.0);
+      t.s_q[pos] = a.fl_charge[k];
+      t.s_layer[pos] = static_cast<uint8_t>(2 * a.fl_field[k] + 1);
     }
   }
 }
@@ -188,6 +369,7 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
 __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
   __shared__ unsigned long long s_over[MAXF], s_net[MAXF];
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const Task &t = tasks[bm.x];
   if (threadIdx.x < MAXF) {
     s_over[threadIdx.x] = 0;
@@ -256,6 +438,7 @@ __global__ void __launch_bounds__(BLOCK) k_gather_units(const Task *__restrict__
                                                         const int2 *__restrict__ map) {
   __shared__ double coef[MAXF];
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   load_coefficients(t, coef);
@@ -304,6 +487,7 @@ __global__ void __launch_bounds__(BLOCK) k_gather_fillers(const Task *__restrict
                                                           const int2 *__restrict__ map) {
   __shared__ double coef[MAXF];
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   load_coefficients(t, coef);
@@ -465,6 +649,7 @@ __device__ double wa_wave_net(const Task &t, int local) {
 // chains, scheduled first), the rest one small net per thread.
 __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const Task &t = tasks[bm.x];
   double value[1];
   if (bm.y < t.nb_wa_large)
@@ -479,6 +664,7 @@ __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, co
 __global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict__ tasks,
                                                          const int2 *__restrict__ map) {
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   const int u = bm.y * BLOCK + threadIdx.x;
@@ -561,6 +747,7 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tas
 __global__ void __launch_bounds__(BLOCK) k_preconditioner(const OptTask *__restrict__ tasks,
                                                           const int2 *__restrict__ map) {
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const OptTask &t = tasks[bm.x];
   const int i = bm.y * BLOCK + threadIdx.x;
   double d = 0.0;
@@ -644,6 +831,7 @@ __global__ void __launch_bounds__(BLOCK) k_metric_finalize(const OptTask *__rest
 __global__ void __launch_bounds__(BLOCK) k_extrapolate(const OptTask *__restrict__ tasks,
                                                        const int2 *__restrict__ map) {
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const OptTask &t = tasks[bm.x];
   const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
   bool changed = false;
@@ -662,6 +850,7 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(const OptTask *__restrict
 __global__ void __launch_bounds__(BLOCK) k_candidate(const OptTask *__restrict__ tasks,
                                                      const int2 *__restrict__ map) {
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const OptTask &t = tasks[bm.x];
   const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
   double red[2] = {0.0, 0.0};
@@ -703,6 +892,7 @@ __global__ void __launch_bounds__(BLOCK) k_step_finalize(const OptTask *__restri
 __global__ void __launch_bounds__(BLOCK) k_round_targets(const OptTask *__restrict__ tasks,
                                                          const int2 *__restrict__ map) {
   const int2 bm = map[blockIdx.x];
+  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
   const OptTask &t = tasks[bm.x];
   const int i = bm.y * BLOCK + threadIdx.x;
   if (i >= 2 * t.n_units) return;
@@ -818,6 +1008,14 @@ void launch_density(const Task *tasks, const Maps &m, hipStream_t s) {
     timer_end(g_timer, "deposit", s);
   }
   if (m.nb_convert > 0) hipLaunchKernelGGL(k_convert, dim3(m.nb_convert), dim3(BLOCK), 0, s, tasks, m.convert);
+}
+
+void launch_sort(const SortTask *tasks, const SortMaps &m, hipStream_t s) {
+  if (m.nb_charges <= 0) return;
+  hipLaunchKernelGGL(k_sort_clear, dim3(m.nb_keys), dim3(BLOCK), 0, s, tasks, m.keys);
+  hipLaunchKernelGGL(k_sort_count, dim3(m.nb_charges), dim3(BLOCK), 0, s, tasks, m.charges);
+  hipLaunchKernelGGL(k_sort_scan, dim3(m.n_list), dim3(BLOCK), 0, s, tasks, m.list);
+  hipLaunchKernelGGL(k_sort_scatter, dim3(m.nb_charges), dim3(BLOCK), 0, s, tasks, m.charges);
 }
 
 void launch_gather(const Task *tasks, const Maps &m, hipStream_t s) {
