@@ -317,8 +317,6 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
       pin_xy[2 * i + 1] = p->cell_offset_xy[2 * c + 1];
     }
   }
-  for (size_t n = 0; n < n_net_total; ++n)
-    (net_start[n + 1] - net_start[n] > LARGE_NET_PINS ? large_nets : small_nets).push_back(static_cast<uint32_t>(n));
   // unit -> incident movable pins (pin order; nets before arcs)
   u_pstart.assign(U + 1, 0);
   for (size_t i = 0; i < n_pins_total; ++i)
@@ -358,6 +356,46 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
       wl_diag[du_] += net_weight[n_nets + a];
       wl_diag[su] += net_weight[n_nets + a];
     }
+  }
+  // Device layout of the nets: small nets (at most LARGE_NET_PINS pins)
+  // grouped by degree, then large nets longest first, then the timed arcs.
+  // Lanes of a wavefront take consecutive small nets, so a wave's pin loops
+  // have one length and its pins are contiguous.  Pins are renumbered with
+  // their nets; each unit still lists its pins in the input net order, which
+  // fixes the summation order of its gradient.
+  {
+    std::vector<uint32_t> order(n_nets);
+    for (size_t n = 0; n < n_nets; ++n) order[n] = (uint32_t)n;
+    auto degree = [&](uint32_t n) { return net_start[n + 1] - net_start[n]; };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      const uint32_t da = degree(a), db = degree(b);
+      const bool la = da > LARGE_NET_PINS, lb = db > LARGE_NET_PINS;
+      if (la != lb) return lb;
+      return la ? da > db : da < db;
+    });
+    std::vector<uint32_t> start2(n_net_total + 1, 0), unit2(n_pins_total), new_pin(n_pins_total);
+    std::vector<double> xy2(2 * n_pins_total), w2(n_net_total);
+    uint32_t q = 0;
+    for (size_t k = 0; k < n_net_total; ++k) {
+      const uint32_t n = k < n_nets ? order[k] : (uint32_t)k;
+      w2[k] = net_weight[n];
+      for (uint32_t i = net_start[n]; i < net_start[n + 1]; ++i, ++q) {
+        new_pin[i] = q;
+        unit2[q] = pin_unit[i];
+        xy2[2 * q] = pin_xy[2 * i];
+        xy2[2 * q + 1] = pin_xy[2 * i + 1];
+      }
+      start2[k + 1] = q;
+      if (k >= n_nets || degree(n) <= LARGE_NET_PINS)
+        small_nets.push_back((uint32_t)k);
+      else
+        large_nets.push_back((uint32_t)k);
+    }
+    for (auto &i : u_pins) i = new_pin[i];
+    net_start.swap(start2);
+    pin_unit.swap(unit2);
+    pin_xy.swap(xy2);
+    net_weight.swap(w2);
   }
   has_routing = p->routing_horizontal && p->routing_vertical;
   if (has_routing) {
