@@ -26,6 +26,33 @@ constexpr int BLOCK = 256;
 constexpr int GF_STRIDE = 2 * MAXF + 2;  // filler gather partials: sums, kkt max, flag
 constexpr int UG_STRIDE = 3;             // unit gradient partials: wa l1, kkt max, flag
 
+// Block plan of one launch: task t owns logical blocks [start[t], start[t+1]).
+// Workgroups are dispatched to the eight XCDs round-robin; with per > 0 the
+// grid has 8 * per blocks and XCD x runs the logical slice [x per, (x+1) per),
+// so a slice holds one job (or one spatial part of one) and each XCD's L2
+// caches that job's arrays.  Only speed depends on the dispatch order.
+struct BlockPlan {
+  const int *start;  // [n_tasks + 1]
+  int n_tasks, total, per;
+};
+constexpr int XCDS = 8;
+inline int plan_grid(const BlockPlan &p) { return p.per > 0 ? XCDS * p.per : p.total; }
+__device__ __forceinline__ bool plan_block(const BlockPlan &p, int2 &bm) {
+  const int b = blockIdx.x;
+  const int lb = p.per > 0 ? (b % XCDS) * p.per + b / XCDS : b;
+  if (lb >= p.total) return false;
+  int lo = 0, hi = p.n_tasks - 1;  // last task whose first block is <= lb
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.start[mid] <= lb)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  bm = make_int2(lo, lb - p.start[lo]);
+  return true;
+}
+
 // Reductions finished by k_finalize for one objective evaluation.
 struct EvalScalars {
   double wa_value;              // WA wirelength incl. timed arcs
@@ -59,7 +86,12 @@ struct MetricScalars {
 };
 
 constexpr int SORT_TILE_SHIFT = 3;  // charges are binned by 8x8-bin tiles
-constexpr int DEPOSIT_CHUNK = 2 * BLOCK;  // charges per deposit block
+constexpr int DEP_TILE = 1 << SORT_TILE_SHIFT;
+// A tile's deposit window reaches DEP_HALO bins past the tile: the stencil
+// (one bin) plus two bins of movement since the charges were last sorted.
+constexpr int DEP_HALO = 3;
+constexpr int DEP_WIN = DEP_TILE + 2 * DEP_HALO;
+constexpr int DEP_WIN2 = DEP_WIN * DEP_WIN;
 
 // Centre bin of the quadratic B-spline stencil (electrostatic_placement.rs:950-976);
 // a NaN coordinate lands on the last bin, as in bspline_axis.
@@ -114,6 +146,7 @@ struct JobStatic {
   const int32_t *unit_du;     // unit -> density unit or -1
   const uint32_t *perm;       // deposit order of the charges (members, then fillers)
   int tiles_x, n_keys;        // 8x8-bin tiles of the charge sort
+  const uint32_t *key_end;    // [n_keys] end of each tile's charges in deposit order
   // charges in deposit order: coordinate index (unit, or n_units + filler),
   // offset from it, charge, layer = 2 field + (1 for a filler)
   const uint32_t *s_ci;
@@ -148,6 +181,8 @@ struct Task {
   long long *acc_m, *acc_f;    // [F*G]
   float *rho, *phi, *t0, *t1;  // [F*G]
   double2 *gfu;                // [F*DU] coefficient-scaled field gradient per density unit
+  long long *partial;          // [n_keys][2F][DEP_WIN2] per-tile deposit windows
+  uint32_t *tile_flag;         // [n_keys] 1 where the tile's window was written
   double2 *dpin;               // [pins] weighted WA derivative
   double *part_wa, *part_energy, *part_gu, *part_gf, *part_ug;
   int nb_wa_small, nb_wa_large, nb_energy, nb_gu, nb_gf, nb_ug;
@@ -177,8 +212,7 @@ struct SortTask {
   uint8_t *s_layer;
 };
 struct SortMaps {
-  const int2 *charges, *keys;
-  int nb_charges, nb_keys;
+  BlockPlan charges, keys;
   const int *list;  // one task per block of the scan
   int n_list;
 };
@@ -206,19 +240,10 @@ struct OptTask {
   double lipschitz;
   double *part_step;  // [blocks][4] candidate, [blocks] extrapolation flag
   StepScalars *step;
-  // rounding of unit targets into `targets`
-  const double *round_src;
-  int W, H;
-  uint32_t *targets;
 };
 
-// Block maps of one round: entry b = (task index, local block index).  The
-// host interleaves each map so that every XCD (blocks are dispatched to the
-// eight XCDs round-robin) receives a contiguous range of a task's blocks and
-// its L2 caches one job's arrays; entries with task -1 are padding.
 struct Maps {
-  const int2 *deposit, *convert, *gather_units, *gather_fillers, *wa, *unit_grad;
-  int nb_deposit, nb_convert, nb_gather_units, nb_gather_fillers, nb_wa, nb_unit_grad;
+  BlockPlan deposit, convert, gather_units, gather_fillers, wa, unit_grad;
   int n_tasks;
 };
 
@@ -229,8 +254,7 @@ void launch_combine(const Task *tasks, const Maps &m, hipStream_t s);
 
 // Optimizer maps; each finalize list holds one task index per block.
 struct OptMaps {
-  const int2 *precond, *extrapolate, *candidate, *targets;
-  int nb_precond, nb_extrapolate, nb_candidate, nb_targets;
+  BlockPlan precond, extrapolate, candidate;
   const int *precond_list, *candidate_list;
   int n_precond, n_candidate;
 };

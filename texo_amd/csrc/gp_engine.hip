@@ -64,6 +64,39 @@ __global__ void k_round_targets_raw(int n_units, const double *__restrict__ coor
 
 }  // namespace
 
+namespace {
+
+template <class T>
+size_t append(std::vector<char> &buf, const std::vector<T> &v) {
+  size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + v.size() * sizeof(T));
+  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
+}
+
+void add_blocks(std::vector<int2> &map, int task, size_t n, size_t per_block) {
+  const int nb = (int)((n + per_block - 1) / per_block);
+  for (int b = 0; b < nb; ++b) map.push_back(make_int2(task, b));
+}
+
+// Blocks per task of one launch family, as prefix sums (see BlockPlan).
+struct Plan {
+  std::vector<int> start{0};
+  void add(size_t n, size_t per_block) { start.push_back(start.back() + (int)((n + per_block - 1) / per_block)); }
+  void none() { start.push_back(start.back()); }
+  int total() const { return start.back(); }
+  BlockPlan device(const char *stage, size_t offset) const {
+    BlockPlan b{};
+    b.start = reinterpret_cast<const int *>(stage + offset);
+    b.n_tasks = (int)start.size() - 1;
+    b.total = total();
+    b.per = b.total >= 2 * XCDS ? (b.total + XCDS - 1) / XCDS : 0;
+    return b;
+  }
+};
+
+}  // namespace
+
 void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s) {
   if (n_units == 0) return;
   hipLaunchKernelGGL(k_round_targets_raw, dim3((2 * n_units + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, n_units,
@@ -152,7 +185,10 @@ void Engine::attach(Job *job) {
   job->pool_index = (int)idx;
   job->d_scalars = d_pool_ + idx;
   job->h_scalars = h_pool_ + idx;
-  if (!job->empty) job->upload(basis((int)job->W, (int)job->H), stream_);
+  if (!job->empty) {
+    job->upload(basis((int)job->W, (int)job->H), stream_);
+    sort_now(job);
+  }
 }
 
 void Engine::detach(Job *job) {
@@ -171,6 +207,30 @@ void Engine::ensure_stage(size_t bytes) {
   HIP(hipMalloc(reinterpret_cast<void **>(&d_stage_), cap));
   HIP(hipHostMalloc(reinterpret_cast<void **>(&h_stage_), cap, hipHostMallocDefault));
   stage_bytes_ = cap;
+}
+
+void Engine::sort_now(Job *j) {
+  if (j->F == 0 || j->M + j->Fl == 0) return;
+  HIP(hipSetDevice(device_));
+  std::vector<SortTask> tasks{j->sort_task()};
+  Plan pc, pk;
+  pc.add(j->M + j->Fl, BLOCK);
+  pk.add((size_t)j->n_keys + 1, BLOCK);
+  std::vector<int> list{0};
+  std::vector<char> buf;
+  const size_t o_t = append(buf, tasks), o_c = append(buf, pc.start), o_k = append(buf, pk.start),
+               o_l = append(buf, list);
+  ensure_stage(buf.size());
+  std::memcpy(h_stage_, buf.data(), buf.size());
+  HIP(hipMemcpyAsync(d_stage_, h_stage_, buf.size(), hipMemcpyHostToDevice, stream_));
+  SortMaps sm{};
+  sm.charges = pc.device(d_stage_, o_c);
+  sm.keys = pk.device(d_stage_, o_k);
+  sm.list = reinterpret_cast<const int *>(d_stage_ + o_l);
+  sm.n_list = 1;
+  launch_sort(reinterpret_cast<const SortTask *>(d_stage_ + o_t), sm, stream_);
+  HIP(hipGetLastError());
+  HIP(hipStreamSynchronize(stream_));
 }
 
 // Schedule decisions that need no device work run here, before a job's
@@ -637,42 +697,6 @@ void Engine::advance(Job *j) {
   }
 }
 
-namespace {
-
-template <class T>
-size_t append(std::vector<char> &buf, const std::vector<T> &v) {
-  size_t off = (buf.size() + 255) & ~size_t(255);
-  buf.resize(off + v.size() * sizeof(T));
-  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
-  return off;
-}
-
-void add_blocks(std::vector<int2> &map, int task, size_t n, size_t per_block) {
-  const int nb = (int)((n + per_block - 1) / per_block);
-  for (int b = 0; b < nb; ++b) map.push_back(make_int2(task, b));
-}
-
-// Workgroups are dispatched to the XCDs round-robin (block b -> XCD b % 8).
-// Reorders a block map so that XCD x runs the contiguous slice
-// [x * per, (x + 1) * per) of the task-ordered list: a slice holds one job
-// (or one spatial part of a job), so each XCD's L2 caches one job's
-// coordinates and fields instead of all of them.  Padding entries carry
-// task -1.  Only speed depends on the dispatch order.
-constexpr int XCDS = 8;
-std::vector<int2> xcd_interleave(const std::vector<int2> &blocks) {
-  const size_t n = blocks.size();
-  if (n < 2 * XCDS) return blocks;
-  const size_t per = (n + XCDS - 1) / XCDS;
-  std::vector<int2> out(XCDS * per, make_int2(-1, -1));
-  for (size_t x = 0; x < XCDS; ++x)
-    for (size_t i = 0; i < per; ++i) {
-      const size_t src = x * per + i;
-      if (src < n) out[XCDS * i + x] = blocks[src];
-    }
-  return out;
-}
-
-}  // namespace
 
 void Engine::round(const std::vector<Job *> &jobs) {
   HIP(hipSetDevice(device_));
@@ -697,23 +721,20 @@ void Engine::round(const std::vector<Job *> &jobs) {
   // ---- tables ----
   std::vector<OptTask> opt;
   std::vector<SortTask> sorts;
-  std::vector<int2> m_sort_c, m_sort_k;
-  std::vector<int> l_sort;
-  std::vector<int2> m_pre, m_ext, m_cand;
-  std::vector<int> l_pre, l_cand;
+  std::vector<int> l_sort, l_pre, l_cand;
+  Plan p_sort_c, p_sort_k, p_pre, p_ext, p_cand;
   std::vector<Task> tasks;
-  std::vector<int2> m_dep, m_conv, m_gu, m_gf, m_wa, m_ug;
+  Plan p_dep, p_conv, p_gu, p_gf, p_wa, p_ug;
   struct Group {
     std::vector<int2> h, w;
   };
   std::map<std::tuple<int, int, int>, Group> groups;
   for (Job *j : active) {
     if (j->want_sort && j->F > 0 && j->M + j->Fl > 0) {
-      const int si = (int)sorts.size();
+      l_sort.push_back((int)sorts.size());
       sorts.push_back(j->sort_task());
-      add_blocks(m_sort_c, si, j->M + j->Fl, BLOCK);
-      add_blocks(m_sort_k, si, (size_t)j->n_keys + 1, BLOCK);
-      l_sort.push_back(si);
+      p_sort_c.add(j->M + j->Fl, BLOCK);
+      p_sort_k.add((size_t)j->n_keys + 1, BLOCK);
     }
     if (j->want_precond || j->want_extrapolate || j->want_candidate) {
       OptTask o{};
@@ -744,18 +765,23 @@ void Engine::round(const std::vector<Job *> &jobs) {
       o.lipschitz = j->L;
       o.part_step = j->d_part_step;
       o.step = &j->d_scalars->step;
-      o.W = (int)j->W;
-      o.H = (int)j->H;
       const int oi = (int)opt.size();
       opt.push_back(o);
       if (j->want_precond) {
-        add_blocks(m_pre, oi, j->U + j->Fl, BLOCK);
+        p_pre.add(j->U + j->Fl, BLOCK);
         l_pre.push_back(oi);
+      } else {
+        p_pre.none();
       }
-      if (j->want_extrapolate) add_blocks(m_ext, oi, j->C, BLOCK);
+      if (j->want_extrapolate)
+        p_ext.add(j->C, BLOCK);
+      else
+        p_ext.none();
       if (j->want_candidate) {
-        add_blocks(m_cand, oi, j->C, BLOCK);
+        p_cand.add(j->C, BLOCK);
         l_cand.push_back(oi);
+      } else {
+        p_cand.none();
       }
     }
     for (const Job::EvalReq &r : j->evals) {
@@ -768,32 +794,39 @@ void Engine::round(const std::vector<Job *> &jobs) {
       }
       const int ti = (int)tasks.size();
       tasks.push_back(t);
-      if (j->F > 0 && !r.skip) {
-        add_blocks(m_dep, ti, j->M + j->Fl, DEPOSIT_CHUNK);
-        add_blocks(m_conv, ti, (size_t)j->F * j->G, BLOCK);
+      const bool density = j->F > 0 && !r.skip;
+      if (density) {
+        p_dep.add((size_t)j->n_keys, 1);
+        p_conv.add((size_t)j->F * j->G, BLOCK);
         Group &g = groups[std::make_tuple((int)j->W, (int)j->H, (int)j->opt.precision)];
         add_blocks(g.h, ti, (size_t)j->F * j->H, 32);
         add_blocks(g.w, ti, (size_t)j->F * j->W, 32);
+      } else {
+        p_dep.none();
+        p_conv.none();
       }
       if (j->F > 0) {
-        for (int b = 0; b < j->nb_gu; ++b) m_gu.push_back(make_int2(ti, b));
+        p_gu.add((size_t)j->nb_gu, 1);
         if (j->Fl > 0)
-          for (int b = 0; b < j->nb_gf; ++b) m_gf.push_back(make_int2(ti, b));
+          p_gf.add((size_t)j->nb_gf, 1);
+        else
+          p_gf.none();
+      } else {
+        p_gu.none();
+        p_gf.none();
       }
-      for (int b = 0; b < j->nb_wa_large + j->nb_wa_small; ++b) m_wa.push_back(make_int2(ti, b));
-      for (int b = 0; b < j->nb_ug; ++b) m_ug.push_back(make_int2(ti, b));
+      p_wa.add((size_t)(j->nb_wa_large + j->nb_wa_small), 1);
+      p_ug.add((size_t)j->nb_ug, 1);
     }
   }
-  for (auto *m : {&m_sort_c, &m_sort_k, &m_pre, &m_ext, &m_cand, &m_dep, &m_conv, &m_gu, &m_gf, &m_wa, &m_ug})
-    *m = xcd_interleave(*m);
   std::vector<char> buf;
-  const size_t o_opt = append(buf, opt), o_tasks = append(buf, tasks);
-  const size_t o_pre = append(buf, m_pre), o_ext = append(buf, m_ext), o_cand = append(buf, m_cand);
-  const size_t o_lpre = append(buf, l_pre), o_lcand = append(buf, l_cand);
-  const size_t o_sorts = append(buf, sorts), o_sort_c = append(buf, m_sort_c), o_sort_k = append(buf, m_sort_k),
-               o_lsort = append(buf, l_sort);
-  const size_t o_dep = append(buf, m_dep), o_conv = append(buf, m_conv), o_gu = append(buf, m_gu),
-               o_gf = append(buf, m_gf), o_wa = append(buf, m_wa), o_ug = append(buf, m_ug);
+  const size_t o_opt = append(buf, opt), o_tasks = append(buf, tasks), o_sorts = append(buf, sorts);
+  const size_t o_lpre = append(buf, l_pre), o_lcand = append(buf, l_cand), o_lsort = append(buf, l_sort);
+  const size_t o_psc = append(buf, p_sort_c.start), o_psk = append(buf, p_sort_k.start),
+               o_ppre = append(buf, p_pre.start), o_pext = append(buf, p_ext.start),
+               o_pcand = append(buf, p_cand.start);
+  const size_t o_pdep = append(buf, p_dep.start), o_pconv = append(buf, p_conv.start), o_pgu = append(buf, p_gu.start),
+               o_pgf = append(buf, p_gf.start), o_pwa = append(buf, p_wa.start), o_pug = append(buf, p_ug.start);
   std::vector<std::pair<size_t, size_t>> o_groups;
   for (auto &kv : groups) o_groups.push_back({append(buf, kv.second.h), append(buf, kv.second.w)});
   ensure_stage(buf.size());
@@ -803,44 +836,33 @@ void Engine::round(const std::vector<Job *> &jobs) {
   const Task *d_tasks = reinterpret_cast<const Task *>(dptr(o_tasks));
 
   // ---- launches ----
-  OptMaps om{};
-  om.precond = reinterpret_cast<const int2 *>(dptr(o_pre));
-  om.extrapolate = reinterpret_cast<const int2 *>(dptr(o_ext));
-  om.candidate = reinterpret_cast<const int2 *>(dptr(o_cand));
-  om.targets = nullptr;
-  om.nb_precond = (int)m_pre.size();
-  om.nb_extrapolate = (int)m_ext.size();
-  om.nb_candidate = (int)m_cand.size();
-  om.nb_targets = 0;
-  om.precond_list = reinterpret_cast<const int *>(dptr(o_lpre));
-  om.candidate_list = reinterpret_cast<const int *>(dptr(o_lcand));
-  om.n_precond = (int)l_pre.size();
-  om.n_candidate = (int)l_cand.size();
-  if (!opt.empty()) launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, stream_);
+  if (!opt.empty()) {
+    OptMaps om{};
+    om.precond = p_pre.device(d_stage_, o_ppre);
+    om.extrapolate = p_ext.device(d_stage_, o_pext);
+    om.candidate = p_cand.device(d_stage_, o_pcand);
+    om.precond_list = reinterpret_cast<const int *>(dptr(o_lpre));
+    om.candidate_list = reinterpret_cast<const int *>(dptr(o_lcand));
+    om.n_precond = (int)l_pre.size();
+    om.n_candidate = (int)l_cand.size();
+    launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, stream_);
+  }
   if (!sorts.empty()) {
     SortMaps sm{};
-    sm.charges = reinterpret_cast<const int2 *>(dptr(o_sort_c));
-    sm.keys = reinterpret_cast<const int2 *>(dptr(o_sort_k));
-    sm.nb_charges = (int)m_sort_c.size();
-    sm.nb_keys = (int)m_sort_k.size();
+    sm.charges = p_sort_c.device(d_stage_, o_psc);
+    sm.keys = p_sort_k.device(d_stage_, o_psk);
     sm.list = reinterpret_cast<const int *>(dptr(o_lsort));
     sm.n_list = (int)l_sort.size();
     launch_sort(reinterpret_cast<const SortTask *>(dptr(o_sorts)), sm, stream_);
   }
 
   Maps mp{};
-  mp.deposit = reinterpret_cast<const int2 *>(dptr(o_dep));
-  mp.convert = reinterpret_cast<const int2 *>(dptr(o_conv));
-  mp.gather_units = reinterpret_cast<const int2 *>(dptr(o_gu));
-  mp.gather_fillers = reinterpret_cast<const int2 *>(dptr(o_gf));
-  mp.wa = reinterpret_cast<const int2 *>(dptr(o_wa));
-  mp.unit_grad = reinterpret_cast<const int2 *>(dptr(o_ug));
-  mp.nb_deposit = (int)m_dep.size();
-  mp.nb_convert = (int)m_conv.size();
-  mp.nb_gather_units = (int)m_gu.size();
-  mp.nb_gather_fillers = (int)m_gf.size();
-  mp.nb_wa = (int)m_wa.size();
-  mp.nb_unit_grad = (int)m_ug.size();
+  mp.deposit = p_dep.device(d_stage_, o_pdep);
+  mp.convert = p_conv.device(d_stage_, o_pconv);
+  mp.gather_units = p_gu.device(d_stage_, o_pgu);
+  mp.gather_fillers = p_gf.device(d_stage_, o_pgf);
+  mp.wa = p_wa.device(d_stage_, o_pwa);
+  mp.unit_grad = p_ug.device(d_stage_, o_pug);
   mp.n_tasks = (int)tasks.size();
   // WA depends only on coordinates: it runs on the second stream beside the
   // density chain and joins before the combination.
@@ -902,40 +924,41 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   t.wa_grad_out = j->d_wa_grad;
   if (density_out) t.density_out = j->d_density_out;
   tasks.push_back(t);
-  std::vector<int2> m_dep, m_conv, m_gu, m_gf, m_wa, m_ug, gh, gw;
+  Plan p_dep, p_conv, p_gu, p_gf, p_wa, p_ug;
+  std::vector<int2> gh, gw;
   if (j->F > 0) {
-    add_blocks(m_dep, 0, j->M + j->Fl, DEPOSIT_CHUNK);
-    add_blocks(m_conv, 0, (size_t)j->F * j->G, BLOCK);
+    p_dep.add((size_t)j->n_keys, 1);
+    p_conv.add((size_t)j->F * j->G, BLOCK);
     add_blocks(gh, 0, (size_t)j->F * j->H, 32);
     add_blocks(gw, 0, (size_t)j->F * j->W, 32);
-    for (int b = 0; b < j->nb_gu; ++b) m_gu.push_back(make_int2(0, b));
+    p_gu.add((size_t)j->nb_gu, 1);
     if (j->Fl > 0)
-      for (int b = 0; b < j->nb_gf; ++b) m_gf.push_back(make_int2(0, b));
+      p_gf.add((size_t)j->nb_gf, 1);
+    else
+      p_gf.none();
+  } else {
+    p_dep.none();
+    p_conv.none();
+    p_gu.none();
+    p_gf.none();
   }
-  for (int b = 0; b < j->nb_wa_large + j->nb_wa_small; ++b) m_wa.push_back(make_int2(0, b));
-  for (int b = 0; b < j->nb_ug; ++b) m_ug.push_back(make_int2(0, b));
-  for (auto *m : {&m_dep, &m_conv, &m_gu, &m_gf, &m_wa, &m_ug}) *m = xcd_interleave(*m);
+  p_wa.add((size_t)(j->nb_wa_large + j->nb_wa_small), 1);
+  p_ug.add((size_t)j->nb_ug, 1);
   std::vector<char> buf;
-  const size_t o_tasks = append(buf, tasks), o_dep = append(buf, m_dep), o_conv = append(buf, m_conv),
-               o_gu = append(buf, m_gu), o_gf = append(buf, m_gf), o_wa = append(buf, m_wa), o_ug = append(buf, m_ug),
-               o_h = append(buf, gh), o_w = append(buf, gw);
+  const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),
+               o_gu = append(buf, p_gu.start), o_gf = append(buf, p_gf.start), o_wa = append(buf, p_wa.start),
+               o_ug = append(buf, p_ug.start), o_h = append(buf, gh), o_w = append(buf, gw);
   ensure_stage(buf.size());
   std::memcpy(h_stage_, buf.data(), buf.size());
   HIP(hipMemcpyAsync(d_stage_, h_stage_, buf.size(), hipMemcpyHostToDevice, stream_));
   const Task *d_tasks = reinterpret_cast<const Task *>(d_stage_ + o_tasks);
   Maps mp{};
-  mp.deposit = reinterpret_cast<const int2 *>(d_stage_ + o_dep);
-  mp.convert = reinterpret_cast<const int2 *>(d_stage_ + o_conv);
-  mp.gather_units = reinterpret_cast<const int2 *>(d_stage_ + o_gu);
-  mp.gather_fillers = reinterpret_cast<const int2 *>(d_stage_ + o_gf);
-  mp.wa = reinterpret_cast<const int2 *>(d_stage_ + o_wa);
-  mp.unit_grad = reinterpret_cast<const int2 *>(d_stage_ + o_ug);
-  mp.nb_deposit = (int)m_dep.size();
-  mp.nb_convert = (int)m_conv.size();
-  mp.nb_gather_units = (int)m_gu.size();
-  mp.nb_gather_fillers = (int)m_gf.size();
-  mp.nb_wa = (int)m_wa.size();
-  mp.nb_unit_grad = (int)m_ug.size();
+  mp.deposit = p_dep.device(d_stage_, o_dep);
+  mp.convert = p_conv.device(d_stage_, o_conv);
+  mp.gather_units = p_gu.device(d_stage_, o_gu);
+  mp.gather_fillers = p_gf.device(d_stage_, o_gf);
+  mp.wa = p_wa.device(d_stage_, o_wa);
+  mp.unit_grad = p_ug.device(d_stage_, o_ug);
   mp.n_tasks = 1;
   g_timer = &timer;
   launch_wa(d_tasks, mp, stream_);

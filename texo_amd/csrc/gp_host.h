@@ -159,6 +159,8 @@ struct Job {
     long long *acc_m = nullptr, *acc_f = nullptr;
     float *rho = nullptr, *phi = nullptr, *t0 = nullptr, *t1 = nullptr;
     double2 *gfu = nullptr, *dpin = nullptr;
+    long long *partial = nullptr;
+    uint32_t *tile_flag = nullptr;
     double *part_wa = nullptr, *part_energy = nullptr, *part_gu = nullptr, *part_gf = nullptr, *part_ug = nullptr;
   } slot[2];
   double *d_rudy = nullptr;
@@ -259,6 +261,7 @@ class Engine {
   const Basis *basis(int W, int H);
   void attach(Job *job);  // upload and reserve a scalar slot
   void detach(Job *job);  // release the slot (job memory is freed by the job)
+  void sort_now(Job *job);  // sorts the job's charges by tile at its current coordinates
   // Runs one round for every job in `jobs` whose phase needs the device.
   void round(const std::vector<Job *> &jobs);
   // Parity evaluation of one job at host coordinates.

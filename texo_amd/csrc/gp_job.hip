@@ -537,6 +537,8 @@ size_t Job::device_bytes() const {
   b += 2 * (FG * (8 + 8 + 4 * 4) + (size_t)std::max(F, 1) * DU * 16 + n_pins_total * 16);
   b += 2 * G * 16 + FG * 8 + 2 * U * 8 + C * 8;
   b += (M + Fl) * (4 + 4 + 16 + 8 + 1) + ((G >> (2 * SORT_TILE_SHIFT)) + W + H + 1) * 4;
+  const size_t keys = (size_t)((W + DEP_TILE - 1) / DEP_TILE) * ((H + DEP_TILE - 1) / DEP_TILE);
+  b += 2 * keys * (2 * (size_t)std::max(F, 1) * DEP_WIN2 * 8 + 4);
   return b + 64 * 4096;
 }
 
@@ -588,6 +590,8 @@ void Job::upload(const Basis *b, hipStream_t s) {
     arena.reserve(sl.t0, FG);
     arena.reserve(sl.t1, FG);
     arena.reserve(sl.gfu, (size_t)std::max(F, 1) * DU);
+    arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
+    arena.reserve(sl.tile_flag, (size_t)n_keys);
     arena.reserve(sl.dpin, n_pins_total);
     arena.reserve(sl.part_wa, (size_t)nb_wa_small + nb_wa_large);
     arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
@@ -614,6 +618,11 @@ void Job::upload(const Basis *b, hipStream_t s) {
   auto up = [&](void *dst, const void *src, size_t bytes) {
     if (bytes) HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
   };
+  // the deposit accumulators start at zero and k_convert leaves them at zero
+  for (auto &sl : slot) {
+    HIP(hipMemsetAsync(sl.acc_m, 0, FG * sizeof(long long), s));
+    HIP(hipMemsetAsync(sl.acc_f, 0, FG * sizeof(long long), s));
+  }
   // fixed-point background: -smoothed capacity per field
   std::vector<long long> base(FG, 0);
   for (int f = 0; f < F; ++f) {
@@ -655,6 +664,34 @@ void Job::upload(const Basis *b, hipStream_t s) {
   for (size_t i = 0; i < C; ++i) x0[i] = clampd(x0[i], lower[i], upper[i]);
   up(d_vec[ix], x0.data(), C * 8);
   up(d_vec[ixp], x0.data(), C * 8);
+  {
+    // input order until the engine sorts the charges by tile
+    const size_t n = M + Fl;
+    std::vector<uint32_t> perm(n), ci(n);
+    std::vector<double> off(2 * n, 0.0), q(n);
+    std::vector<uint8_t> layer(n);
+    for (size_t i = 0; i < M; ++i) {
+      perm[i] = (uint32_t)i;
+      ci[i] = m_unit[i];
+      off[2 * i] = m_off[2 * i];
+      off[2 * i + 1] = m_off[2 * i + 1];
+      q[i] = m_charge[i];
+      layer[i] = (uint8_t)(2 * m_field[i]);
+    }
+    for (size_t k = 0; k < Fl; ++k) {
+      const size_t i = M + k;
+      perm[i] = (uint32_t)i;
+      ci[i] = (uint32_t)(U + k);
+      q[i] = fl_charge[k];
+      layer[i] = (uint8_t)(2 * fl_field[k] + 1);
+    }
+    up(d_perm, perm.data(), n * 4);
+    up(d_s_ci, ci.data(), n * 4);
+    up(d_s_off, off.data(), n * 16);
+    up(d_s_q, q.data(), n * 8);
+    up(d_s_layer, layer.data(), n);
+    HIP(hipStreamSynchronize(s));  // host vectors above are pageable and local
+  }
   HIP(hipStreamSynchronize(s));  // host vectors above are pageable and local
   apply_charges(std::vector<double>(m_charge), std::vector<double>(fl_charge));
 
@@ -683,6 +720,7 @@ void Job::upload(const Basis *b, hipStream_t s) {
   js.s_layer = d_s_layer;
   js.tiles_x = tiles_x;
   js.n_keys = n_keys;
+  js.key_end = d_sort_count;
   js.lower = d_lower;
   js.upper = d_upper;
   js.n_small = (int)small_nets.size();
@@ -719,33 +757,14 @@ void Job::apply_charges(const std::vector<double> &member_areas, const std::vect
   if (M) HIP(hipMemcpy(d_m_charge, m_charge.data(), M * 8, hipMemcpyHostToDevice));
   if (Fl) HIP(hipMemcpy(d_fl_charge, fl_charge.data(), Fl * 8, hipMemcpyHostToDevice));
   if (DU) HIP(hipMemcpy(d_du_fcharge, fcharge.data(), fcharge.size() * 8, hipMemcpyHostToDevice));
-  // deposit order restarts as the input order with the new charges; the
-  // next sort reorders it
+  // the sorted charge copies follow the new charges; the deposit order stays
   const size_t n = M + Fl;
-  std::vector<uint32_t> perm(n), ci(n);
-  std::vector<double> off(2 * n, 0.0), q(n);
-  std::vector<uint8_t> layer(n);
-  for (size_t i = 0; i < M; ++i) {
-    perm[i] = (uint32_t)i;
-    ci[i] = m_unit[i];
-    off[2 * i] = m_off[2 * i];
-    off[2 * i + 1] = m_off[2 * i + 1];
-    q[i] = m_charge[i];
-    layer[i] = (uint8_t)(2 * m_field[i]);
-  }
-  for (size_t k = 0; k < Fl; ++k) {
-    const size_t i = M + k;
-    perm[i] = (uint32_t)i;
-    ci[i] = (uint32_t)(U + k);
-    q[i] = fl_charge[k];
-    layer[i] = (uint8_t)(2 * fl_field[k] + 1);
-  }
   if (n) {
-    HIP(hipMemcpy(d_perm, perm.data(), n * 4, hipMemcpyHostToDevice));
-    HIP(hipMemcpy(d_s_ci, ci.data(), n * 4, hipMemcpyHostToDevice));
-    HIP(hipMemcpy(d_s_off, off.data(), n * 16, hipMemcpyHostToDevice));
+    std::vector<uint32_t> perm(n);
+    HIP(hipMemcpy(perm.data(), d_perm, n * 4, hipMemcpyDeviceToHost));
+    std::vector<double> q(n);
+    for (size_t i = 0; i < n; ++i) q[i] = perm[i] < M ? m_charge[perm[i]] : fl_charge[perm[i] - M];
     HIP(hipMemcpy(d_s_q, q.data(), n * 8, hipMemcpyHostToDevice));
-    HIP(hipMemcpy(d_s_layer, layer.data(), n, hipMemcpyHostToDevice));
   }
 }
 
@@ -761,6 +780,8 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.t0 = w.t0;
   t.t1 = w.t1;
   t.gfu = w.gfu;
+  t.partial = w.partial;
+  t.tile_flag = w.tile_flag;
   t.dpin = w.dpin;
   t.part_wa = w.part_wa;
   t.part_energy = w.part_energy;

@@ -1,7 +1,7 @@
 // gp_kernels.hip — scatter/gather and elementwise kernels of the electrostatic
 // global-placement loop for gfx950 (64-lane wavefronts).
 //
-// Every kernel covers a table of tasks through a block map; each block works
+// Every kernel covers a table of tasks through a block plan; each block works
 // for exactly one task.  All are HBM/latency bound.  Reductions are
 // deterministic: integer sums use int64 atomics (exact, order-free), FP64 sums
 // and maxima go through per-block partials that one block per task reduces
@@ -121,156 +121,81 @@ __device__ __forceinline__ double kkt_scaled(double g, double x, double lo, doub
 // ---------------------------------------------------------------------------
 // Density deposit (electrostatic_placement.rs:296-332, 900-916).
 // ---------------------------------------------------------------------------
-// acc_m = -smoothed capacity, acc_f = 0; the task's int64 sums restart.
-__global__ void __launch_bounds__(BLOCK) k_deposit_init(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+// Deposit, one block per 8x8-bin tile of the last charge sort.  The block
+// walks the tile's charges (contiguous in the sorted arrays) and adds each
+// stencil cell's int64 quantum into an LDS window covering the tile plus
+// DEP_HALO bins, one layer per (field, members | fillers); it then stores the
+// whole window to the tile's partial with plain stores.  A charge that moved
+// so far since the sort that its stencil leaves the window adds its quanta to
+// the HBM accumulators with atomics instead.  k_convert adds, for every bin,
+// the accumulators and the windows that cover it.  The quanta are the same
+// as a sequential deposit's and int64 addition is exact, so the density does
+// not depend on the order, the tiling or which path a charge took.
+__global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ tasks, BlockPlan plan) {
+  __shared__ unsigned long long win[2 * MAXF * DEP_WIN2];
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
-  const int total = t.j.F * t.j.W * t.j.H;
-  const int i = bm.y * BLOCK + threadIdx.x;
-  if (i < total) {
-    t.acc_m[i] = t.j.base_fx[i];
-    t.acc_f[i] = 0;
-  }
-  if (bm.y == 0 && threadIdx.x < MAXF) {
+  const JobStatic &a = t.j;
+  const int key = bm.y;
+  if (key == 0 && threadIdx.x < MAXF) {
+    // the task's int64 sums restart (k_convert adds to them)
     t.scalars->over_fx[threadIdx.x] = 0;
     t.scalars->net_fx[threadIdx.x] = 0;
   }
-}
-
-// Position and charge of one deposit charge.
-struct Charge {
-  double x, y, q;
-  int f;
-  bool filler;
-};
-
-// One block deposits DEP_CHUNK consecutive charges of the sort order.  The
-// block first takes the bounding box of its stencils; when the box (times
-// the fields it spans, times members/fillers) fits the LDS window, the
-// charges accumulate there with LDS int64 atomics and the block adds each
-// touched window cell to HBM with one atomic; otherwise every stencil cell
-// goes to HBM directly.  Either way each cell receives the same int64
-// quanta, so the density is bit-identical whatever the order or the path.
-constexpr int DEP_PER_THREAD = DEPOSIT_CHUNK / BLOCK;
-constexpr int DEP_CHUNK = DEPOSIT_CHUNK;
-constexpr int DEP_WINDOW = 6144;  // int64 cells (48 KB)
-
-__global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  __shared__ unsigned long long win[DEP_WINDOW];
-  __shared__ int bb[6];  // x0, x1, y0, y1, f0, f1
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
-  const Task &t = tasks[bm.x];
-  const JobStatic &a = t.j;
-  const int total = a.n_members + a.n_fillers;
+  const uint32_t begin = key > 0 ? a.key_end[key - 1] : 0u, end = a.key_end[key];
+  if (begin == end) {
+    if (threadIdx.x == 0) t.tile_flag[key] = 0u;
+    return;
+  }
+  const int cells = 2 * a.F * DEP_WIN2;
+  for (int idx = threadIdx.x; idx < cells; idx += BLOCK) win[idx] = 0ull;
+  __syncthreads();
+  const int x0 = (key % a.tiles_x) * DEP_TILE - DEP_HALO, y0 = (key / a.tiles_x) * DEP_TILE - DEP_HALO;
   const size_t G = (size_t)a.W * (size_t)a.H;
-  if (threadIdx.x == 0) {
-    bb[0] = bb[2] = bb[4] = 0x7fffffff;
-    bb[1] = bb[3] = bb[5] = -1;
-  }
-  __syncthreads();
-  Axis ax[DEP_PER_THREAD], ay[DEP_PER_THREAD];
-  double q[DEP_PER_THREAD];
-  int f[DEP_PER_THREAD];
-  bool filler[DEP_PER_THREAD], valid[DEP_PER_THREAD];
-  int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};  // x, y, field
-#pragma unroll
-  for (int k = 0; k < DEP_PER_THREAD; ++k) {
-    const int i = bm.y * DEP_CHUNK + k * BLOCK + threadIdx.x;
-    valid[k] = i < total;
-    if (valid[k]) {
-      // a filler's offset is +0.0, which leaves its coordinate unchanged
-      const uint32_t ci = a.s_ci[i];
-      const double2 off = a.s_off[i];
-      const int layer = a.s_layer[i];
-      const Charge c = {t.coords[2 * (size_t)ci] + off.x, t.coords[2 * (size_t)ci + 1] + off.y, a.s_q[i], layer >> 1,
-                        (layer & 1) != 0};
-      bspline_axis(a.W, c.x, ax[k]);
-      bspline_axis(a.H, c.y, ay[k]);
-      q[k] = c.q;
-      f[k] = c.f;
-      filler[k] = c.filler;
-      lo[0] = min(lo[0], ax[k].i0);
-      hi[0] = max(hi[0], ax[k].i2);
-      lo[1] = min(lo[1], ay[k].i0);
-      hi[1] = max(hi[1], ay[k].i2);
-      lo[2] = min(lo[2], c.f);
-      hi[2] = max(hi[2], c.f);
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    for (int off = 32; off > 0; off >>= 1) {
-      lo[d] = min(lo[d], __shfl_xor(lo[d], off, 64));
-      hi[d] = max(hi[d], __shfl_xor(hi[d], off, 64));
-    }
-  }
-  if ((threadIdx.x & 63) == 0 && hi[0] >= 0) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      atomicMin(&bb[2 * d], lo[d]);
-      atomicMax(&bb[2 * d + 1], hi[d]);
-    }
-  }
-  __syncthreads();
-  const int x0 = bb[0], y0 = bb[2], f0 = bb[4];
-  const int nx = bb[1] - x0 + 1, ny = bb[3] - y0 + 1, nf = bb[5] - f0 + 1;
-  if (bb[1] < 0) return;  // no charge in this block (uniform)
-  const int cells = nx * ny * nf * 2;
-  if (cells <= DEP_WINDOW) {
-    for (int idx = threadIdx.x; idx < cells; idx += BLOCK) win[idx] = 0ull;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < DEP_PER_THREAD; ++k) {
-      if (!valid[k]) continue;
-      const int ix[3] = {ax[k].i0, ax[k].i1, ax[k].i2};
-      const int iy[3] = {ay[k].i0, ay[k].i1, ay[k].i2};
-      const double wx[3] = {ax[k].w0, ax[k].w1, ax[k].w2};
-      const double wy[3] = {ay[k].w0, ay[k].w1, ay[k].w2};
-      const int layer = 2 * (f[k] - f0) + (filler[k] ? 1 : 0);
+  for (uint32_t i = begin + threadIdx.x; i < end; i += BLOCK) {
+    // a filler's offset is +0.0, which leaves its coordinate unchanged
+    const uint32_t ci = a.s_ci[i];
+    const double2 off = a.s_off[i];
+    const int layer = a.s_layer[i];
+    const double q = a.s_q[i];
+    Axis ax, ay;
+    bspline_axis(a.W, t.coords[2 * (size_t)ci] + off.x, ax);
+    bspline_axis(a.H, t.coords[2 * (size_t)ci + 1] + off.y, ay);
+    const int ix[3] = {ax.i0, ax.i1, ax.i2};
+    const int iy[3] = {ay.i0, ay.i1, ay.i2};
+    const double wx[3] = {ax.w0, ax.w1, ax.w2};
+    const double wy[3] = {ay.w0, ay.w1, ay.w2};
+    const bool inside = ax.i0 >= x0 && ax.i2 < x0 + DEP_WIN && ay.i0 >= y0 && ay.i2 < y0 + DEP_WIN;
+    if (inside) {
+      unsigned long long *w = win + (size_t)layer * DEP_WIN2;
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           // charge * column.weight * row.weight, quantised before the add
-          const double v = q[k] * wx[c] * wy[r];
-          atomicAdd(&win[((size_t)layer * ny + (iy[r] - y0)) * nx + (ix[c] - x0)],
+          const double v = q * wx[c] * wy[r];
+          atomicAdd(w + (iy[r] - y0) * DEP_WIN + (ix[c] - x0),
                     static_cast<unsigned long long>(__double2ll_rn(v * FX_SCALE)));
         }
       }
-    }
-    __syncthreads();
-    const int plane = nx * ny;
-    for (int idx = threadIdx.x; idx < cells; idx += BLOCK) {
-      const unsigned long long v = win[idx];
-      if (v == 0ull) continue;
-      const int layer = idx / plane, rem = idx - layer * plane;
-      const int y = y0 + rem / nx, x = x0 + rem % nx;
-      long long *acc = (layer & 1) ? t.acc_f : t.acc_m;
-      atomicAdd(reinterpret_cast<unsigned long long *>(acc + (size_t)(f0 + (layer >> 1)) * G + (size_t)y * a.W + x),
-                v);
-    }
-    return;
-  }
+    } else {
+      unsigned long long *g =
+          reinterpret_cast<unsigned long long *>(((layer & 1) ? t.acc_f : t.acc_m) + (size_t)(layer >> 1) * G);
 #pragma unroll
-  for (int k = 0; k < DEP_PER_THREAD; ++k) {
-    if (!valid[k]) continue;
-    unsigned long long *g =
-        reinterpret_cast<unsigned long long *>((filler[k] ? t.acc_f : t.acc_m) + (size_t)f[k] * G);
-    const int ix[3] = {ax[k].i0, ax[k].i1, ax[k].i2};
-    const int iy[3] = {ay[k].i0, ay[k].i1, ay[k].i2};
-    const double wx[3] = {ax[k].w0, ax[k].w1, ax[k].w2};
-    const double wy[3] = {ay[k].w0, ay[k].w1, ay[k].w2};
+      for (int r = 0; r < 3; ++r) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double v = q[k] * wx[c] * wy[r];
-        atomicAdd(g + (size_t)iy[r] * a.W + ix[c], static_cast<unsigned long long>(__double2ll_rn(v * FX_SCALE)));
+        for (int c = 0; c < 3; ++c) {
+          const double v = q * wx[c] * wy[r];
+          atomicAdd(g + (size_t)iy[r] * a.W + ix[c], static_cast<unsigned long long>(__double2ll_rn(v * FX_SCALE)));
+        }
       }
     }
   }
+  __syncthreads();
+  long long *out = t.partial + (size_t)key * cells;
+  for (int idx = threadIdx.x; idx < cells; idx += BLOCK) out[idx] = static_cast<long long>(win[idx]);
+  if (threadIdx.x == 0) t.tile_flag[key] = 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -293,18 +218,18 @@ __device__ __forceinline__ int charge_key(const JobStatic &a, const double *__re
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sort_clear(const SortTask *__restrict__ tasks,
-                                                      const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+                                                      BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const SortTask &t = tasks[bm.x];
   const int i = bm.y * BLOCK + threadIdx.x;
   if (i <= t.j.n_keys) t.count[i] = 0u;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sort_count(const SortTask *__restrict__ tasks,
-                                                      const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+                                                      BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const SortTask &t = tasks[bm.x];
   const int i = bm.y * BLOCK + threadIdx.x;
   if (i < t.j.n_members + t.j.n_fillers) atomicAdd(&t.count[charge_key(t.j, t.coords, i)], 1u);
@@ -340,9 +265,9 @@ __global__ void __launch_bounds__(BLOCK) k_sort_scan(const SortTask *__restrict_
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sort_scatter(const SortTask *__restrict__ tasks,
-                                                        const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+                                                        BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const SortTask &t = tasks[bm.x];
   const int i = bm.y * BLOCK + threadIdx.x;
   if (i < t.j.n_members + t.j.n_fillers) {
@@ -365,33 +290,76 @@ __global__ void __launch_bounds__(BLOCK) k_sort_scatter(const SortTask *__restri
 }
 
 // rho = members + fillers -> FP32 transform input; physical overflow (before
-// fillers) and net charge as exact int64 sums.
-__global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  __shared__ unsigned long long s_over[MAXF], s_net[MAXF];
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+// fillers) and net charge as exact int64 sums.  Each bin adds the background
+// (-smoothed capacity, members only), its HBM accumulators and the deposit
+// windows of the (at most 2 x 2) tiles whose window covers it, and clears
+// the accumulators it read, which are zero again for the next deposit.
+__device__ __forceinline__ void tile_range(int v, int tiles, int &lo, int &hi) {
+  // tiles t with t*DEP_TILE - DEP_HALO <= v < t*DEP_TILE - DEP_HALO + DEP_WIN
+  const int span = DEP_WIN - DEP_HALO - 1;  // v - span <= t*DEP_TILE <= v + DEP_HALO
+  lo = v > span ? (v - span + DEP_TILE - 1) / DEP_TILE : 0;
+  hi = min((v + DEP_HALO) / DEP_TILE, tiles - 1);
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ tasks, BlockPlan plan) {
+  __shared__ long long s_over[BLOCK / 64][MAXF], s_net[BLOCK / 64][MAXF];
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
-  if (threadIdx.x < MAXF) {
-    s_over[threadIdx.x] = 0;
-    s_net[threadIdx.x] = 0;
-  }
-  __syncthreads();
-  const int G = t.j.W * t.j.H;
+  const JobStatic &a = t.j;
+  const int G = a.W * a.H;
   const int i = bm.y * BLOCK + threadIdx.x;
-  if (i < t.j.F * G) {
-    const int f = i / G;
-    const long long m = t.acc_m[i];
-    const long long v = m + t.acc_f[i];
+  int f = -1;
+  long long over = 0, net = 0;
+  if (i < a.F * G) {
+    f = i / G;
+    const int cell = i - f * G, y = cell / a.W, x = cell - y * a.W;
+    const long long am = t.acc_m[i], af = t.acc_f[i];
+    if (am != 0) t.acc_m[i] = 0;
+    if (af != 0) t.acc_f[i] = 0;
+    long long m = a.base_fx[i] + am, fl = af;
+    int tx0, tx1, ty0, ty1;
+    tile_range(x, a.tiles_x, tx0, tx1);
+    tile_range(y, a.n_keys / a.tiles_x, ty0, ty1);
+    const int cells = 2 * a.F * DEP_WIN2;
+    for (int ty = ty0; ty <= ty1; ++ty)
+      for (int tx = tx0; tx <= tx1; ++tx) {
+        const int key = ty * a.tiles_x + tx;
+        if (!t.tile_flag[key]) continue;
+        const long long *w = t.partial + (size_t)key * cells;
+        const int o = (y - (ty * DEP_TILE - DEP_HALO)) * DEP_WIN + (x - (tx * DEP_TILE - DEP_HALO));
+        m += w[(2 * f) * DEP_WIN2 + o];
+        fl += w[(2 * f + 1) * DEP_WIN2 + o];
+      }
+    const long long v = m + fl;
     const double d = static_cast<double>(v) * FX_INV;
     t.rho[i] = static_cast<float>(d);
     if (t.density_out) t.density_out[i] = d;
-    if (m > 0) atomicAdd(&s_over[f], static_cast<unsigned long long>(m));
-    atomicAdd(&s_net[f], static_cast<unsigned long long>(v));
+    over = m > 0 ? m : 0;
+    net = v;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < MAXF; ++k) {
+    const long long o = wave_sum_ll(f == k ? over : 0);
+    const long long n = wave_sum_ll(f == k ? net : 0);
+    if (lane == 0) {
+      s_over[wave][k] = o;
+      s_net[wave][k] = n;
+    }
   }
   __syncthreads();
-  if (threadIdx.x < t.j.F) {
-    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->over_fx[threadIdx.x]), s_over[threadIdx.x]);
-    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->net_fx[threadIdx.x]), s_net[threadIdx.x]);
+  if (threadIdx.x < a.F) {
+    const int k = threadIdx.x;
+    const long long o = s_over[0][k] + s_over[1][k] + s_over[2][k] + s_over[3][k];
+    const long long n = s_net[0][k] + s_net[1][k] + s_net[2][k] + s_net[3][k];
+    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->over_fx[k]), static_cast<unsigned long long>(o));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->net_fx[k]), static_cast<unsigned long long>(n));
   }
 }
 
@@ -435,10 +403,10 @@ __device__ __forceinline__ void sample(const float *__restrict__ phi, int W, int
 }
 
 __global__ void __launch_bounds__(BLOCK) k_gather_units(const Task *__restrict__ tasks,
-                                                        const int2 *__restrict__ map) {
+                                                        BlockPlan plan) {
   __shared__ double coef[MAXF];
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   load_coefficients(t, coef);
@@ -484,10 +452,10 @@ __global__ void __launch_bounds__(BLOCK) k_gather_units(const Task *__restrict__
 }
 
 __global__ void __launch_bounds__(BLOCK) k_gather_fillers(const Task *__restrict__ tasks,
-                                                          const int2 *__restrict__ map) {
+                                                          BlockPlan plan) {
   __shared__ double coef[MAXF];
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   load_coefficients(t, coef);
@@ -647,9 +615,9 @@ __device__ double wa_wave_net(const Task &t, int local) {
 
 // Per task: local blocks [0, nb_wa_large) take high-fanout nets (long serial
 // chains, scheduled first), the rest one small net per thread.
-__global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+__global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
   double value[1];
   if (bm.y < t.nb_wa_large)
@@ -662,9 +630,9 @@ __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, co
 // Combined unit gradient (combined_objective, eplace.rs:1022-1044): WA
 // baseline pins, then timed-arc pins, then each field in order.
 __global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict__ tasks,
-                                                         const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+                                                         BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   const int u = bm.y * BLOCK + threadIdx.x;
@@ -745,9 +713,9 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tas
 // ---------------------------------------------------------------------------
 // diagonal_preconditioner (eplace.rs:1046-1100); per-block max/min/flag.
 __global__ void __launch_bounds__(BLOCK) k_preconditioner(const OptTask *__restrict__ tasks,
-                                                          const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+                                                          BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const OptTask &t = tasks[bm.x];
   const int i = bm.y * BLOCK + threadIdx.x;
   double d = 0.0;
@@ -829,9 +797,9 @@ __global__ void __launch_bounds__(BLOCK) k_metric_finalize(const OptTask *__rest
 }
 
 __global__ void __launch_bounds__(BLOCK) k_extrapolate(const OptTask *__restrict__ tasks,
-                                                       const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+                                                       BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const OptTask &t = tasks[bm.x];
   const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
   bool changed = false;
@@ -848,9 +816,9 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(const OptTask *__restrict
 }
 
 __global__ void __launch_bounds__(BLOCK) k_candidate(const OptTask *__restrict__ tasks,
-                                                     const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
+                                                     BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
   const OptTask &t = tasks[bm.x];
   const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
   double red[2] = {0.0, 0.0};
@@ -886,21 +854,6 @@ __global__ void __launch_bounds__(BLOCK) k_step_finalize(const OptTask *__restri
     t.step->change_bits = static_cast<unsigned long long>(__double_as_longlong(m[0]));
     t.step->changed = m[1] > 0.0 ? 1u : 0u;
   }
-}
-
-// rounded_targets_from_coordinates (eplace.rs:954-971, lib.rs:4566-4568)
-__global__ void __launch_bounds__(BLOCK) k_round_targets(const OptTask *__restrict__ tasks,
-                                                         const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.x];
-  if (bm.x < 0) return;  // padding entry of an XCD-interleaved map
-  const OptTask &t = tasks[bm.x];
-  const int i = bm.y * BLOCK + threadIdx.x;
-  if (i >= 2 * t.n_units) return;
-  const double extent = static_cast<double>(((i & 1) ? t.H : t.W) - 1);
-  double r = round(t.round_src[i]);
-  if (!(r >= 0.0)) r = 0.0;
-  if (r > extent) r = extent;
-  t.targets[i] = static_cast<uint32_t>(r);
 }
 
 // ---------------------------------------------------------------------------
@@ -1001,58 +954,58 @@ int blocks_for(size_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
 // Launchers.
 // ---------------------------------------------------------------------------
 void launch_density(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.nb_convert > 0) hipLaunchKernelGGL(k_deposit_init, dim3(m.nb_convert), dim3(BLOCK), 0, s, tasks, m.convert);
-  if (m.nb_deposit > 0) {
+  if (m.deposit.total > 0) {
     timer_begin(g_timer, "deposit", s);
-    hipLaunchKernelGGL(k_deposit, dim3(m.nb_deposit), dim3(BLOCK), 0, s, tasks, m.deposit);
+    hipLaunchKernelGGL(k_deposit, dim3(plan_grid(m.deposit)), dim3(BLOCK), 0, s, tasks, m.deposit);
     timer_end(g_timer, "deposit", s);
   }
-  if (m.nb_convert > 0) hipLaunchKernelGGL(k_convert, dim3(m.nb_convert), dim3(BLOCK), 0, s, tasks, m.convert);
+  if (m.convert.total > 0)
+    hipLaunchKernelGGL(k_convert, dim3(plan_grid(m.convert)), dim3(BLOCK), 0, s, tasks, m.convert);
 }
 
 void launch_sort(const SortTask *tasks, const SortMaps &m, hipStream_t s) {
-  if (m.nb_charges <= 0) return;
-  hipLaunchKernelGGL(k_sort_clear, dim3(m.nb_keys), dim3(BLOCK), 0, s, tasks, m.keys);
-  hipLaunchKernelGGL(k_sort_count, dim3(m.nb_charges), dim3(BLOCK), 0, s, tasks, m.charges);
+  if (m.charges.total <= 0) return;
+  hipLaunchKernelGGL(k_sort_clear, dim3(plan_grid(m.keys)), dim3(BLOCK), 0, s, tasks, m.keys);
+  hipLaunchKernelGGL(k_sort_count, dim3(plan_grid(m.charges)), dim3(BLOCK), 0, s, tasks, m.charges);
   hipLaunchKernelGGL(k_sort_scan, dim3(m.n_list), dim3(BLOCK), 0, s, tasks, m.list);
-  hipLaunchKernelGGL(k_sort_scatter, dim3(m.nb_charges), dim3(BLOCK), 0, s, tasks, m.charges);
+  hipLaunchKernelGGL(k_sort_scatter, dim3(plan_grid(m.charges)), dim3(BLOCK), 0, s, tasks, m.charges);
 }
 
 void launch_gather(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.nb_gather_units > 0) {
+  if (m.gather_units.total > 0) {
     timer_begin(g_timer, "gather", s);
-    hipLaunchKernelGGL(k_gather_units, dim3(m.nb_gather_units), dim3(BLOCK), 0, s, tasks, m.gather_units);
+    hipLaunchKernelGGL(k_gather_units, dim3(plan_grid(m.gather_units)), dim3(BLOCK), 0, s, tasks, m.gather_units);
     timer_end(g_timer, "gather", s);
   }
-  if (m.nb_gather_fillers > 0)
-    hipLaunchKernelGGL(k_gather_fillers, dim3(m.nb_gather_fillers), dim3(BLOCK), 0, s, tasks, m.gather_fillers);
+  if (m.gather_fillers.total > 0)
+    hipLaunchKernelGGL(k_gather_fillers, dim3(plan_grid(m.gather_fillers)), dim3(BLOCK), 0, s, tasks,
+                       m.gather_fillers);
 }
 
 void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.nb_wa <= 0) return;
+  if (m.wa.total <= 0) return;
   timer_begin(g_timer, "wa", s);
-  hipLaunchKernelGGL(k_wa, dim3(m.nb_wa), dim3(BLOCK), 0, s, tasks, m.wa);
+  hipLaunchKernelGGL(k_wa, dim3(plan_grid(m.wa)), dim3(BLOCK), 0, s, tasks, m.wa);
   timer_end(g_timer, "wa", s);
 }
 
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.nb_unit_grad > 0)
-    hipLaunchKernelGGL(k_unit_gradient, dim3(m.nb_unit_grad), dim3(BLOCK), 0, s, tasks, m.unit_grad);
+  if (m.unit_grad.total > 0)
+    hipLaunchKernelGGL(k_unit_gradient, dim3(plan_grid(m.unit_grad)), dim3(BLOCK), 0, s, tasks, m.unit_grad);
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
 void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
-  if (m.nb_precond > 0) {
-    hipLaunchKernelGGL(k_preconditioner, dim3(m.nb_precond), dim3(BLOCK), 0, s, tasks, m.precond);
+  if (m.precond.total > 0) {
+    hipLaunchKernelGGL(k_preconditioner, dim3(plan_grid(m.precond)), dim3(BLOCK), 0, s, tasks, m.precond);
     hipLaunchKernelGGL(k_metric_finalize, dim3(m.n_precond), dim3(BLOCK), 0, s, tasks, m.precond_list);
   }
-  if (m.nb_extrapolate > 0)
-    hipLaunchKernelGGL(k_extrapolate, dim3(m.nb_extrapolate), dim3(BLOCK), 0, s, tasks, m.extrapolate);
-  if (m.nb_candidate > 0) {
-    hipLaunchKernelGGL(k_candidate, dim3(m.nb_candidate), dim3(BLOCK), 0, s, tasks, m.candidate);
+  if (m.extrapolate.total > 0)
+    hipLaunchKernelGGL(k_extrapolate, dim3(plan_grid(m.extrapolate)), dim3(BLOCK), 0, s, tasks, m.extrapolate);
+  if (m.candidate.total > 0) {
+    hipLaunchKernelGGL(k_candidate, dim3(plan_grid(m.candidate)), dim3(BLOCK), 0, s, tasks, m.candidate);
     hipLaunchKernelGGL(k_step_finalize, dim3(m.n_candidate), dim3(BLOCK), 0, s, tasks, m.candidate_list);
   }
-  if (m.nb_targets > 0) hipLaunchKernelGGL(k_round_targets, dim3(m.nb_targets), dim3(BLOCK), 0, s, tasks, m.targets);
 }
 
 void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, const double2 *pin_xy,
