@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8, help="multi-start jobs in the batched form")
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3", "fp16_scaled"])
+    ap.add_argument("--precision", default="fp16_scaled", choices=["fp32", "bf16x3", "fp16_scaled"])
     ap.add_argument("--roofline-kernel", default="deposit", choices=["deposit", "wa", "gather", "dct"])
     ap.add_argument("--serving-jobs", type=int, default=256)
     ap.add_argument("--serving-rate", type=float, default=20.0)

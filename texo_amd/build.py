@@ -59,6 +59,13 @@ def build(verbose: bool = False) -> str:
     cmd = [hipcc(), "-shared", "--offload-arch=gfx950", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # static archive of the same objects, for hosts that link the library
+    # into their own binary (a Rust -sys crate); each object carries its own
+    # gfx950 code object, so no device link step is needed
+    archive = os.path.join(HERE, "libtexo_gp.a")
+    if os.path.exists(archive):
+        os.remove(archive)
+    subprocess.run(["ar", "rcs", archive, *objs], check=True)
     return LIB
 
 

@@ -387,14 +387,19 @@ struct texo_gp_batch {
     for (auto &j : jobs) engine->detach(j.get());
     jobs.clear();
   }
+  // Rounds until every job is paused, done or failed.  With two or more jobs
+  // they alternate in two groups so the host plans one group while the GPU
+  // runs the other's round; each job's arithmetic is unchanged.
   void drive_all() {
-    std::vector<Job *> all;
-    for (auto &j : jobs) all.push_back(j.get());
-    for (;;) {
-      bool busy = false;
-      for (Job *j : all) busy = busy || (j->phase != Phase::Done && j->phase != Phase::Failed && j->phase != Phase::Paused);
-      if (!busy) return;
-      engine->round(all);
+    std::vector<Job *> group[2];
+    for (size_t k = 0; k < jobs.size(); ++k) group[jobs.size() > 1 ? k % 2 : 0].push_back(jobs[k].get());
+    bool live[2] = {engine->launch(0, group[0]), engine->launch(1, group[1])};
+    while (live[0] || live[1]) {
+      for (int c = 0; c < 2; ++c) {
+        if (!live[c]) continue;
+        engine->finish(c);
+        live[c] = engine->launch(c, group[c]);
+      }
     }
   }
 };
@@ -543,16 +548,25 @@ struct texo_gp_server {
   std::thread worker;
 
   void loop();
+  void retire(std::vector<int64_t> &group);
 };
 
 void texo_gp_server::loop() {
   (void)hipSetDevice(engine->device());
+  // Active jobs alternate between two groups whose rounds are pipelined: the
+  // host finishes and re-plans one group while the GPU runs the other's.
+  std::vector<int64_t> group[2];
+  bool live[2] = {false, false};
   for (;;) {
-    std::vector<Job *> jobs;
     {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait(lk, [&] { return stopping || !pending.empty() || !active.empty(); });
-      if (stopping) return;
+      if (stopping) {
+        lk.unlock();
+        for (int c = 0; c < 2; ++c)
+          if (live[c]) engine->finish(c);
+        return;
+      }
       // admission in submission order, by HBM footprint and batch size
       while (!pending.empty() && active.size() < max_batch) {
         Entry &e = *entries[pending.front()];
@@ -561,77 +575,105 @@ void texo_gp_server::loop() {
         const int64_t id = pending.front();
         pending.pop_front();
         lk.unlock();
-        bool ok = true;
         try {
           engine->attach(e.job.get());
           e.job->begin_run(false);
         } catch (const Failure &f) {
-          ok = false;
           e.job->failure = f;
           e.job->status = f.status;
           e.job->phase = Phase::Failed;
         }
         lk.lock();
-        (void)ok;
         active.push_back(id);
+        group[group[0].size() <= group[1].size() ? 0 : 1].push_back(id);
         hbm_in_use += need;
         stats.admitted += 1;
       }
-      for (int64_t id : active) jobs.push_back(entries[id]->job.get());
     }
-    const auto t0 = std::chrono::steady_clock::now();
-    try {
-      engine->round(jobs);
-    } catch (const Failure &f) {
-      for (Job *j : jobs) {
-        if (j->phase != Phase::Done && j->phase != Phase::Failed) {
-          j->failure = f;
-          j->status = f.status;
-          j->phase = Phase::Failed;
+    for (int c = 0; c < 2; ++c) {
+      if (live[c]) {
+        const auto t0 = std::chrono::steady_clock::now();
+        try {
+          engine->finish(c);
+        } catch (const Failure &f) {
+          for (int64_t id : group[c]) {
+            Job &j = *entries[id]->job;
+            if (j.phase != Phase::Done && j.phase != Phase::Failed) {
+              j.failure = f;
+              j.status = f.status;
+              j.phase = Phase::Failed;
+            }
+          }
         }
+        live[c] = false;
+        std::lock_guard<std::mutex> lk(mu);
+        stats.rounds += 1;
+        stats.busy_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       }
-    }
-    const double busy = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    // retire finished jobs
-    std::unique_lock<std::mutex> lk(mu);
-    stats.rounds += 1;
-    stats.busy_seconds += busy;
-    std::vector<int64_t> still;
-    for (int64_t id : active) {
-      Entry &e = *entries[id];
-      Job &j = *e.job;
-      if (j.phase != Phase::Done && j.phase != Phase::Failed) {
-        still.push_back(id);
-        continue;
+      retire(group[c]);
+      std::vector<Job *> jobs;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        for (int64_t id : group[c]) jobs.push_back(entries[id]->job.get());
       }
-      lk.unlock();
-      e.targets.assign(2 * j.U, 0);
       try {
-        fill_report(*engine, j, &e.report);
-        if (j.phase == Phase::Done) copy_targets(*engine, j, e.targets.data());
+        live[c] = engine->launch(c, jobs);
       } catch (const Failure &f) {
-        put_failure(&e.report, f);
-        j.status = f.status;
-        j.phase = Phase::Failed;
+        for (Job *j : jobs) {
+          if (j->phase != Phase::Done && j->phase != Phase::Failed) {
+            j->failure = f;
+            j->status = f.status;
+            j->phase = Phase::Failed;
+          }
+        }
+        live[c] = false;
       }
-      e.status = j.phase == Phase::Failed ? j.status : 0;
-      e.report.status = e.status;
-      e.report.elapsed_seconds =
-          std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
-      const uint64_t held = j.device_bytes();
-      engine->detach(&j);
-      e.job.reset();  // frees the job's HBM arena
-      lk.lock();
-      hbm_in_use -= std::min(hbm_in_use, held);
-      e.done = true;
-      if (e.status == 0)
-        stats.completed += 1;
-      else
-        stats.failed += 1;
+      if (!live[c]) retire(group[c]);
     }
-    active.swap(still);
-    cv_done.notify_all();
   }
+}
+
+// Moves finished jobs of `group` out of the active set with their results.
+void texo_gp_server::retire(std::vector<int64_t> &group) {
+  std::vector<int64_t> still;
+  for (int64_t id : group) {
+    Entry *ep;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      ep = entries[id].get();
+    }
+    Entry &e = *ep;
+    Job &j = *e.job;
+    if (j.phase != Phase::Done && j.phase != Phase::Failed) {
+      still.push_back(id);
+      continue;
+    }
+    e.targets.assign(2 * j.U, 0);
+    try {
+      fill_report(*engine, j, &e.report);
+      if (j.phase == Phase::Done) copy_targets(*engine, j, e.targets.data());
+    } catch (const Failure &f) {
+      put_failure(&e.report, f);
+      j.status = f.status;
+      j.phase = Phase::Failed;
+    }
+    e.status = j.phase == Phase::Failed ? j.status : 0;
+    e.report.status = e.status;
+    e.report.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
+    const uint64_t held = j.device_bytes();
+    engine->detach(&j);
+    e.job.reset();  // frees the job's HBM arena
+    std::lock_guard<std::mutex> lk(mu);
+    hbm_in_use -= std::min(hbm_in_use, held);
+    e.done = true;
+    if (e.status == 0)
+      stats.completed += 1;
+    else
+      stats.failed += 1;
+    active.erase(std::find(active.begin(), active.end(), id));
+  }
+  group.swap(still);
+  cv_done.notify_all();
 }
 
 extern "C" {

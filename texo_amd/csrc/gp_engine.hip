@@ -97,6 +97,21 @@ struct Plan {
 
 }  // namespace
 
+// Gathers the scalar blocks of a round's jobs into one contiguous buffer for
+// a single device-to-host copy.
+__global__ void k_collect_scalars(const int *__restrict__ index, const JobScalars *__restrict__ pool,
+                                  JobScalars *__restrict__ out) {
+  constexpr int WORDS = (int)(sizeof(JobScalars) / sizeof(unsigned long long));
+  static_assert(sizeof(JobScalars) % sizeof(unsigned long long) == 0, "JobScalars is a whole number of words");
+  const unsigned long long *src = reinterpret_cast<const unsigned long long *>(pool + index[blockIdx.x]);
+  unsigned long long *dst = reinterpret_cast<unsigned long long *>(out + blockIdx.x);
+  for (int w = threadIdx.x; w < WORDS; w += blockDim.x) dst[w] = src[w];
+}
+
+void launch_collect_scalars(const int *index, int n, const JobScalars *pool, JobScalars *out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_collect_scalars, dim3(n), dim3(64), 0, s, index, pool, out);
+}
+
 void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s) {
   if (n_units == 0) return;
   hipLaunchKernelGGL(k_round_targets_raw, dim3((2 * n_units + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, n_units,
@@ -149,6 +164,7 @@ Engine::Engine(int device, size_t pool_capacity) : device_(device) {
   std::memset(h_pool_, 0, pool_capacity * sizeof(JobScalars));
   HIP(hipMemset(d_pool_, 0, pool_capacity * sizeof(JobScalars)));
   pool_used_.assign(pool_capacity, false);
+  for (RoundCtx &c : ctx_) HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
 }
 
 Engine::~Engine() {
@@ -158,8 +174,11 @@ Engine::~Engine() {
   bases_.clear();
   if (d_pool_) (void)hipFree(d_pool_);
   if (h_pool_) (void)hipHostFree(h_pool_);
-  if (d_stage_) (void)hipFree(d_stage_);
-  if (h_stage_) (void)hipHostFree(h_stage_);
+  for (RoundCtx &c : ctx_) {
+    free_ctx(c);
+    if (c.done) (void)hipEventDestroy(c.done);
+  }
+  free_ctx(aux_);
   (void)hipEventDestroy(ev_fork_);
   (void)hipEventDestroy(ev_join_);
   (void)hipStreamDestroy(wa_stream_);
@@ -196,17 +215,39 @@ void Engine::detach(Job *job) {
   job->pool_index = -1;
 }
 
-void Engine::ensure_stage(size_t bytes) {
-  if (bytes <= stage_bytes_) return;
-  size_t cap = std::max<size_t>(bytes, 2 * stage_bytes_);
+void Engine::ensure_stage(RoundCtx &c, size_t bytes) {
+  if (bytes <= c.stage_bytes) return;
+  size_t cap = std::max<size_t>(bytes, 2 * c.stage_bytes);
   cap = (cap + 4095) & ~size_t(4095);
-  if (d_stage_) HIP(hipFree(d_stage_));
-  if (h_stage_) HIP(hipHostFree(h_stage_));
-  d_stage_ = nullptr;
-  h_stage_ = nullptr;
-  HIP(hipMalloc(reinterpret_cast<void **>(&d_stage_), cap));
-  HIP(hipHostMalloc(reinterpret_cast<void **>(&h_stage_), cap, hipHostMallocDefault));
-  stage_bytes_ = cap;
+  if (c.d_stage) HIP(hipFree(c.d_stage));
+  if (c.h_stage) HIP(hipHostFree(c.h_stage));
+  c.d_stage = nullptr;
+  c.h_stage = nullptr;
+  HIP(hipMalloc(reinterpret_cast<void **>(&c.d_stage), cap));
+  HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_stage), cap, hipHostMallocDefault));
+  c.stage_bytes = cap;
+}
+
+void Engine::ensure_compact(RoundCtx &c, size_t n) {
+  if (n <= c.compact_cap) return;
+  const size_t cap = std::max<size_t>(n, 2 * c.compact_cap);
+  if (c.d_compact) HIP(hipFree(c.d_compact));
+  if (c.h_compact) HIP(hipHostFree(c.h_compact));
+  c.d_compact = nullptr;
+  c.h_compact = nullptr;
+  HIP(hipMalloc(reinterpret_cast<void **>(&c.d_compact), cap * sizeof(JobScalars)));
+  HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_compact), cap * sizeof(JobScalars), hipHostMallocDefault));
+  c.compact_cap = cap;
+}
+
+void Engine::free_ctx(RoundCtx &c) {
+  if (c.d_stage) (void)hipFree(c.d_stage);
+  if (c.h_stage) (void)hipHostFree(c.h_stage);
+  if (c.d_compact) (void)hipFree(c.d_compact);
+  if (c.h_compact) (void)hipHostFree(c.h_compact);
+  c.d_stage = c.h_stage = nullptr;
+  c.d_compact = c.h_compact = nullptr;
+  c.stage_bytes = c.compact_cap = 0;
 }
 
 void Engine::sort_now(Job *j) {
@@ -220,15 +261,17 @@ void Engine::sort_now(Job *j) {
   std::vector<char> buf;
   const size_t o_t = append(buf, tasks), o_c = append(buf, pc.start), o_k = append(buf, pk.start),
                o_l = append(buf, list);
-  ensure_stage(buf.size());
-  std::memcpy(h_stage_, buf.data(), buf.size());
-  HIP(hipMemcpyAsync(d_stage_, h_stage_, buf.size(), hipMemcpyHostToDevice, stream_));
+  // wait for any round in flight before reusing the set-up staging buffer
+  HIP(hipStreamSynchronize(stream_));
+  ensure_stage(aux_, buf.size());
+  std::memcpy(aux_.h_stage, buf.data(), buf.size());
+  HIP(hipMemcpyAsync(aux_.d_stage, aux_.h_stage, buf.size(), hipMemcpyHostToDevice, stream_));
   SortMaps sm{};
-  sm.charges = pc.device(d_stage_, o_c);
-  sm.keys = pk.device(d_stage_, o_k);
-  sm.list = reinterpret_cast<const int *>(d_stage_ + o_l);
+  sm.charges = pc.device(aux_.d_stage, o_c);
+  sm.keys = pk.device(aux_.d_stage, o_k);
+  sm.list = reinterpret_cast<const int *>(aux_.d_stage + o_l);
   sm.n_list = 1;
-  launch_sort(reinterpret_cast<const SortTask *>(d_stage_ + o_t), sm, stream_);
+  launch_sort(reinterpret_cast<const SortTask *>(aux_.d_stage + o_t), sm, stream_);
   HIP(hipGetLastError());
   HIP(hipStreamSynchronize(stream_));
 }
@@ -699,9 +742,15 @@ void Engine::advance(Job *j) {
 
 
 void Engine::round(const std::vector<Job *> &jobs) {
+  if (launch(0, jobs)) finish(0);
+}
+
+bool Engine::launch(int c, const std::vector<Job *> &jobs) {
+  RoundCtx &cx = ctx_[c];
   HIP(hipSetDevice(device_));
   g_timer = &timer;
-  std::vector<Job *> active;
+  std::vector<Job *> &active = cx.active;
+  active.clear();
   for (Job *j : jobs) {
     if (j->phase == Phase::Done || j->phase == Phase::Failed || j->phase == Phase::Paused) continue;
     try {
@@ -715,7 +764,7 @@ void Engine::round(const std::vector<Job *> &jobs) {
   }
   if (active.empty()) {
     g_timer = nullptr;
-    return;
+    return false;
   }
 
   // ---- tables ----
@@ -829,9 +878,14 @@ void Engine::round(const std::vector<Job *> &jobs) {
                o_pgf = append(buf, p_gf.start), o_pwa = append(buf, p_wa.start), o_pug = append(buf, p_ug.start);
   std::vector<std::pair<size_t, size_t>> o_groups;
   for (auto &kv : groups) o_groups.push_back({append(buf, kv.second.h), append(buf, kv.second.w)});
-  ensure_stage(buf.size());
-  std::memcpy(h_stage_, buf.data(), buf.size());
-  HIP(hipMemcpyAsync(d_stage_, h_stage_, buf.size(), hipMemcpyHostToDevice, stream_));
+  std::vector<int> pool_index;
+  for (Job *j : active) pool_index.push_back(j->pool_index);
+  const size_t o_pool = append(buf, pool_index);
+  ensure_stage(cx, buf.size());
+  ensure_compact(cx, active.size());
+  std::memcpy(cx.h_stage, buf.data(), buf.size());
+  HIP(hipMemcpyAsync(cx.d_stage, cx.h_stage, buf.size(), hipMemcpyHostToDevice, stream_));
+  char *const d_stage_ = cx.d_stage;
   auto dptr = [&](size_t off) { return d_stage_ + off; };
   const Task *d_tasks = reinterpret_cast<const Task *>(dptr(o_tasks));
 
@@ -887,13 +941,27 @@ void Engine::round(const std::vector<Job *> &jobs) {
   launch_gather(d_tasks, mp, stream_);
   HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
   launch_combine(d_tasks, mp, stream_);
+  launch_collect_scalars(reinterpret_cast<const int *>(dptr(o_pool)), (int)active.size(), d_pool_, cx.d_compact,
+                         stream_);
   HIP(hipGetLastError());
-  int hi = 0;
-  for (Job *j : active) hi = std::max(hi, j->pool_index);
-  HIP(hipMemcpyAsync(h_pool_, d_pool_, (size_t)(hi + 1) * sizeof(JobScalars), hipMemcpyDeviceToHost, stream_));
-  HIP(hipStreamSynchronize(stream_));
+  HIP(hipMemcpyAsync(cx.h_compact, cx.d_compact, active.size() * sizeof(JobScalars), hipMemcpyDeviceToHost,
+                     stream_));
+  HIP(hipEventRecord(cx.done, stream_));
+  cx.pending = true;
+  g_timer = nullptr;
+  return true;
+}
 
-  for (Job *j : active) {
+void Engine::finish(int c) {
+  RoundCtx &cx = ctx_[c];
+  if (!cx.pending) return;
+  HIP(hipSetDevice(device_));
+  HIP(hipEventSynchronize(cx.done));
+  cx.pending = false;
+  g_timer = &timer;
+  for (size_t k = 0; k < cx.active.size(); ++k) {
+    Job *j = cx.active[k];
+    j->h_scalars = cx.h_compact + k;
     try {
       advance(j);
     } catch (const Failure &f) {
@@ -948,9 +1016,11 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),
                o_gu = append(buf, p_gu.start), o_gf = append(buf, p_gf.start), o_wa = append(buf, p_wa.start),
                o_ug = append(buf, p_ug.start), o_h = append(buf, gh), o_w = append(buf, gw);
-  ensure_stage(buf.size());
-  std::memcpy(h_stage_, buf.data(), buf.size());
-  HIP(hipMemcpyAsync(d_stage_, h_stage_, buf.size(), hipMemcpyHostToDevice, stream_));
+  HIP(hipStreamSynchronize(stream_));
+  ensure_stage(aux_, buf.size());
+  std::memcpy(aux_.h_stage, buf.data(), buf.size());
+  HIP(hipMemcpyAsync(aux_.d_stage, aux_.h_stage, buf.size(), hipMemcpyHostToDevice, stream_));
+  char *const d_stage_ = aux_.d_stage;
   const Task *d_tasks = reinterpret_cast<const Task *>(d_stage_ + o_tasks);
   Maps mp{};
   mp.deposit = p_dep.device(d_stage_, o_dep);
@@ -981,6 +1051,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   HIP(hipMemcpyAsync(h_pool_ + j->pool_index, d_pool_ + j->pool_index, sizeof(JobScalars), hipMemcpyDeviceToHost,
                      stream_));
   HIP(hipStreamSynchronize(stream_));
+  j->h_scalars = h_pool_ + j->pool_index;
   j->phase = saved;
   j->evals.clear();
 }

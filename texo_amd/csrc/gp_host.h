@@ -106,6 +106,7 @@ bool register_clustering(const std::vector<std::pair<uint32_t, std::pair<double,
 bool adjust_areas(const std::vector<AreaMember> &members, const std::vector<std::pair<uint32_t, double>> &fillers,
                   std::vector<double> &member_out, std::vector<double> &filler_out);
 void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s);
+void launch_collect_scalars(const int *index, int n, const JobScalars *pool, JobScalars *out, hipStream_t s);
 
 enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed };
 
@@ -264,6 +265,13 @@ class Engine {
   void sort_now(Job *job);  // sorts the job's charges by tile at its current coordinates
   // Runs one round for every job in `jobs` whose phase needs the device.
   void round(const std::vector<Job *> &jobs);
+  // Pipelined rounds: launch() plans one round for `jobs` in context c (0 or
+  // 1), issues it and returns (false when no job needs the device); finish(c)
+  // waits for it and advances its jobs.  With two job groups in the two
+  // contexts the host plans one group while the GPU runs the other's round.
+  bool launch(int c, const std::vector<Job *> &jobs);
+  void finish(int c);
+  bool pending(int c) const { return ctx_[c].pending; }
   // Parity evaluation of one job at host coordinates.
   void evaluate_once(Job *job, const double *coords, double gamma, bool density_out);
   KernelTimer timer;
@@ -278,9 +286,21 @@ class Engine {
   std::map<std::pair<int, int>, std::unique_ptr<Basis>> bases_;
   JobScalars *d_pool_ = nullptr, *h_pool_ = nullptr;
   std::vector<bool> pool_used_;
-  char *h_stage_ = nullptr, *d_stage_ = nullptr;
-  size_t stage_bytes_ = 0;
-  void ensure_stage(size_t bytes);
+  // Staging of one round: the task tables go up in one copy, the active
+  // jobs' scalars come back in one copy (gathered into `d_compact`).
+  struct RoundCtx {
+    char *h_stage = nullptr, *d_stage = nullptr;
+    size_t stage_bytes = 0;
+    JobScalars *d_compact = nullptr, *h_compact = nullptr;
+    size_t compact_cap = 0;
+    hipEvent_t done = nullptr;
+    std::vector<Job *> active;
+    bool pending = false;
+  };
+  RoundCtx ctx_[2], aux_;  // aux_: set-up sorts and parity evaluations
+  void ensure_stage(RoundCtx &c, size_t bytes);
+  void ensure_compact(RoundCtx &c, size_t n);
+  void free_ctx(RoundCtx &c);
 };
 
 }  // namespace texo

@@ -111,7 +111,7 @@ def make_problem(*, seed: int, side: int, n_lut: int, n_ff: int, n_carry_units: 
                  n_io: int, n_nets: int, degree: str, degree_p: float, degree_cap: Optional[int],
                  high_fanout: int = 0, high_fanout_range=(256, 512), weight_exponent: Optional[int] = None,
                  routing_capacity: Optional[int] = None, register_controls: bool = False,
-                 name: str = "synthetic") -> abi.Problem:
+                 region_lut_sites: Optional[float] = None, name: str = "synthetic") -> abi.Problem:
     rng = np.random.default_rng(seed)
     W = H = side
     G = W * H
@@ -154,6 +154,13 @@ def make_problem(*, seed: int, side: int, n_lut: int, n_ff: int, n_carry_units: 
     # memory columns
     mem_cols = [W // 3, (2 * W) // 3] if n_memory else []
     usable = np.ones((H, W), bool)
+    if region_lut_sites is not None:
+        # logic capacity confined to a central square holding about
+        # `region_lut_sites` LUT sites per bin at the 0.9 target density
+        s = min(side, int(math.ceil(math.sqrt((n_lut + 2 * n_carry_units) / (0.9 * region_lut_sites)))) + 1)
+        lo = (side - s) // 2
+        usable[:] = False
+        usable[lo:lo + s, lo:lo + s] = True
     for x in mem_cols:
         usable[:, x] = False
     mem_xy = np.zeros((n_memory, 2))
@@ -334,7 +341,8 @@ def ecp5_85k(seed: int = 2, weight_exponent: int = 4, routing_capacity: int = 8,
 def serving_job(seed: int, cells: int, weight_exponent: int) -> abi.Problem:
     """One job of the mixed-size serving stream: grid side =
     clamp(nextpow2(sqrt(cells/4)), 64, 256), the ECP5 cell mix, nets
-    1.2 per cell of degree 2 + Geometric(0.4)."""
+    1.2 per cell of degree 2 + Geometric(0.4); logic capacity in a central
+    square at about 2 LUT sites per bin (a design occupies part of the die)."""
     side = 1 << max(0, math.ceil(math.log2(max(1.0, math.sqrt(cells / 4.0)))))
     side = min(max(side, 64), 256)
     n_carry_units = max(0, int(cells * 0.01))
@@ -344,7 +352,7 @@ def serving_job(seed: int, cells: int, weight_exponent: int) -> abi.Problem:
     return make_problem(seed=seed, side=side, n_lut=n_lut, n_ff=n_ff, n_carry_units=n_carry_units,
                         n_memory=n_memory, n_io=0, n_nets=int(cells * 1.2), degree="geometric",
                         degree_p=0.4, degree_cap=64, weight_exponent=weight_exponent,
-                        name=f"job{seed}")
+                        region_lut_sites=2.0, name=f"job{seed}")
 
 
 CONFIGS = {"plain_1k": plain_1k, "ecp5_25k": ecp5_25k, "ecp5_85k": ecp5_85k}
