@@ -387,12 +387,16 @@ struct texo_gp_batch {
     for (auto &j : jobs) engine->detach(j.get());
     jobs.clear();
   }
-  // Rounds until every job is paused, done or failed.  With two or more jobs
-  // they alternate in two groups so the host plans one group while the GPU
-  // runs the other's round; each job's arithmetic is unchanged.
+  // Rounds until every job is paused, done or failed.  When the jobs are large
+  // enough to fill the GPU in two halves they alternate in two groups, so the
+  // host plans one group while the GPU runs the other's round; each job's
+  // arithmetic is unchanged.
   void drive_all() {
+    size_t charges = 0;
+    for (auto &j : jobs) charges += j->M + j->Fl;
+    const bool split = jobs.size() > 1 && charges >= texo::PIPELINE_MIN_CHARGES;
     std::vector<Job *> group[2];
-    for (size_t k = 0; k < jobs.size(); ++k) group[jobs.size() > 1 ? k % 2 : 0].push_back(jobs[k].get());
+    for (size_t k = 0; k < jobs.size(); ++k) group[split ? k % 2 : 0].push_back(jobs[k].get());
     bool live[2] = {engine->launch(0, group[0]), engine->launch(1, group[1])};
     while (live[0] || live[1]) {
       for (int c = 0; c < 2; ++c) {
@@ -544,6 +548,7 @@ struct texo_gp_server {
   int64_t next_id = 1;
   bool stopping = false;
   uint64_t hbm_in_use = 0;
+  size_t charges_in_flight = 0;
   texo_gp_server_stats stats{};
   std::thread worker;
 
@@ -585,7 +590,9 @@ void texo_gp_server::loop() {
         }
         lk.lock();
         active.push_back(id);
-        group[group[0].size() <= group[1].size() ? 0 : 1].push_back(id);
+        charges_in_flight += e.job->M + e.job->Fl;
+        const bool split = charges_in_flight >= texo::PIPELINE_MIN_CHARGES;
+        group[split && group[1].size() < group[0].size() ? 1 : 0].push_back(id);
         hbm_in_use += need;
         stats.admitted += 1;
       }
@@ -661,10 +668,12 @@ void texo_gp_server::retire(std::vector<int64_t> &group) {
     e.report.status = e.status;
     e.report.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
     const uint64_t held = j.device_bytes();
+    const size_t charges = j.M + j.Fl;
     engine->detach(&j);
     e.job.reset();  // frees the job's HBM arena
     std::lock_guard<std::mutex> lk(mu);
     hbm_in_use -= std::min(hbm_in_use, held);
+    charges_in_flight -= std::min(charges_in_flight, charges);
     e.done = true;
     if (e.status == 0)
       stats.completed += 1;

@@ -23,8 +23,8 @@ constexpr double FX_INV = 1.0 / 1099511627776.0;
 constexpr uint32_t NO_UNIT = 0xffffffffu;          // fixed pin marker
 constexpr double AUGMENTED_DENSITY_BETA = 2000.0;  // eplace.rs:125
 constexpr int BLOCK = 256;
-constexpr int GF_STRIDE = 2 * MAXF + 2;  // filler gather partials: sums, kkt max, flag
-constexpr int UG_STRIDE = 3;             // unit gradient partials: wa l1, kkt max, flag
+constexpr int GF_STRIDE = 2 * MAXF + 2;  // filler partials: force l1, raw l1 per field, kkt max, flag
+constexpr int UG_STRIDE = 2 * MAXF + 3;  // unit partials: wa l1, force l1, raw l1 per field, kkt max, flag
 
 // Block plan of one launch: task t owns logical blocks [start[t], start[t+1]).
 // Workgroups are dispatched to the eight XCDs round-robin; with per > 0 the
@@ -180,12 +180,13 @@ struct Task {
   // working slot
   long long *acc_m, *acc_f;    // [F*G]
   float *rho, *phi, *t0, *t1;  // [F*G]
-  double2 *gfu;                // [F*DU] coefficient-scaled field gradient per density unit
+  double2 *samp;               // [M+Fl] field sample (d/dx, d/dy of the potential) per charge
+  double *field_coef;          // [2*MAXF] lambda (1 + beta E/E0) per field, then E per field
   long long *partial;          // [n_keys][2F][DEP_WIN2] per-tile deposit windows
   uint32_t *tile_flag;         // [n_keys] 1 where the tile's window was written
   double2 *dpin;               // [pins] weighted WA derivative
-  double *part_wa, *part_energy, *part_gu, *part_gf, *part_ug;
-  int nb_wa_small, nb_wa_large, nb_energy, nb_gu, nb_gf, nb_ug;
+  double *part_wa, *part_energy, *part_gf, *part_ug;
+  int nb_wa_small, nb_wa_large, nb_energy, nb_gf, nb_ug;
   // request
   const double *coords;
   double *grad;
@@ -243,12 +244,12 @@ struct OptTask {
 };
 
 struct Maps {
-  BlockPlan deposit, convert, gather_units, gather_fillers, wa, unit_grad;
+  BlockPlan deposit, convert, sample, fillers, wa, unit_grad;
   int n_tasks;
 };
 
 void launch_density(const Task *tasks, const Maps &m, hipStream_t s);
-void launch_gather(const Task *tasks, const Maps &m, hipStream_t s);
+void launch_gather(const Task *tasks, const Maps &m, hipStream_t s);  // coefficients, samples, fillers
 void launch_wa(const Task *tasks, const Maps &m, hipStream_t s);
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s);
 

@@ -773,7 +773,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   std::vector<int> l_sort, l_pre, l_cand;
   Plan p_sort_c, p_sort_k, p_pre, p_ext, p_cand;
   std::vector<Task> tasks;
-  Plan p_dep, p_conv, p_gu, p_gf, p_wa, p_ug;
+  Plan p_dep, p_conv, p_sample, p_gf, p_wa, p_ug;
   struct Group {
     std::vector<int2> h, w;
   };
@@ -844,26 +844,23 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
       const int ti = (int)tasks.size();
       tasks.push_back(t);
       const bool density = j->F > 0 && !r.skip;
+      // a skipped task reuses the slot's field and samples (same point)
       if (density) {
         p_dep.add((size_t)j->n_keys, 1);
         p_conv.add((size_t)j->F * j->G, BLOCK);
+        p_sample.add((size_t)j->n_keys, 1);
         Group &g = groups[std::make_tuple((int)j->W, (int)j->H, (int)j->opt.precision)];
         add_blocks(g.h, ti, (size_t)j->F * j->H, 32);
         add_blocks(g.w, ti, (size_t)j->F * j->W, 32);
       } else {
         p_dep.none();
         p_conv.none();
+        p_sample.none();
       }
-      if (j->F > 0) {
-        p_gu.add((size_t)j->nb_gu, 1);
-        if (j->Fl > 0)
-          p_gf.add((size_t)j->nb_gf, 1);
-        else
-          p_gf.none();
-      } else {
-        p_gu.none();
+      if (j->F > 0 && j->Fl > 0)
+        p_gf.add((size_t)j->nb_gf, 1);
+      else
         p_gf.none();
-      }
       p_wa.add((size_t)(j->nb_wa_large + j->nb_wa_small), 1);
       p_ug.add((size_t)j->nb_ug, 1);
     }
@@ -874,7 +871,8 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   const size_t o_psc = append(buf, p_sort_c.start), o_psk = append(buf, p_sort_k.start),
                o_ppre = append(buf, p_pre.start), o_pext = append(buf, p_ext.start),
                o_pcand = append(buf, p_cand.start);
-  const size_t o_pdep = append(buf, p_dep.start), o_pconv = append(buf, p_conv.start), o_pgu = append(buf, p_gu.start),
+  const size_t o_pdep = append(buf, p_dep.start), o_pconv = append(buf, p_conv.start),
+               o_pgu = append(buf, p_sample.start),
                o_pgf = append(buf, p_gf.start), o_pwa = append(buf, p_wa.start), o_pug = append(buf, p_ug.start);
   std::vector<std::pair<size_t, size_t>> o_groups;
   for (auto &kv : groups) o_groups.push_back({append(buf, kv.second.h), append(buf, kv.second.w)});
@@ -913,8 +911,8 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   Maps mp{};
   mp.deposit = p_dep.device(d_stage_, o_pdep);
   mp.convert = p_conv.device(d_stage_, o_pconv);
-  mp.gather_units = p_gu.device(d_stage_, o_pgu);
-  mp.gather_fillers = p_gf.device(d_stage_, o_pgf);
+  mp.sample = p_sample.device(d_stage_, o_pgu);
+  mp.fillers = p_gf.device(d_stage_, o_pgf);
   mp.wa = p_wa.device(d_stage_, o_pwa);
   mp.unit_grad = p_ug.device(d_stage_, o_pug);
   mp.n_tasks = (int)tasks.size();
@@ -992,14 +990,14 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   t.wa_grad_out = j->d_wa_grad;
   if (density_out) t.density_out = j->d_density_out;
   tasks.push_back(t);
-  Plan p_dep, p_conv, p_gu, p_gf, p_wa, p_ug;
+  Plan p_dep, p_conv, p_sample, p_gf, p_wa, p_ug;
   std::vector<int2> gh, gw;
   if (j->F > 0) {
     p_dep.add((size_t)j->n_keys, 1);
     p_conv.add((size_t)j->F * j->G, BLOCK);
     add_blocks(gh, 0, (size_t)j->F * j->H, 32);
     add_blocks(gw, 0, (size_t)j->F * j->W, 32);
-    p_gu.add((size_t)j->nb_gu, 1);
+    p_sample.add((size_t)j->n_keys, 1);
     if (j->Fl > 0)
       p_gf.add((size_t)j->nb_gf, 1);
     else
@@ -1007,14 +1005,14 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   } else {
     p_dep.none();
     p_conv.none();
-    p_gu.none();
+    p_sample.none();
     p_gf.none();
   }
   p_wa.add((size_t)(j->nb_wa_large + j->nb_wa_small), 1);
   p_ug.add((size_t)j->nb_ug, 1);
   std::vector<char> buf;
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),
-               o_gu = append(buf, p_gu.start), o_gf = append(buf, p_gf.start), o_wa = append(buf, p_wa.start),
+               o_gu = append(buf, p_sample.start), o_gf = append(buf, p_gf.start), o_wa = append(buf, p_wa.start),
                o_ug = append(buf, p_ug.start), o_h = append(buf, gh), o_w = append(buf, gw);
   HIP(hipStreamSynchronize(stream_));
   ensure_stage(aux_, buf.size());
@@ -1025,8 +1023,8 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   Maps mp{};
   mp.deposit = p_dep.device(d_stage_, o_dep);
   mp.convert = p_conv.device(d_stage_, o_conv);
-  mp.gather_units = p_gu.device(d_stage_, o_gu);
-  mp.gather_fillers = p_gf.device(d_stage_, o_gf);
+  mp.sample = p_sample.device(d_stage_, o_gu);
+  mp.fillers = p_gf.device(d_stage_, o_gf);
   mp.wa = p_wa.device(d_stage_, o_wa);
   mp.unit_grad = p_ug.device(d_stage_, o_ug);
   mp.n_tasks = 1;

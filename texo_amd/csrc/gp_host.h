@@ -110,6 +110,11 @@ void launch_collect_scalars(const int *index, int n, const JobScalars *pool, Job
 
 enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed };
 
+// Below this many charges (members + fillers) in flight a round's kernels do
+// not fill the GPU, and splitting the jobs into two pipelined groups costs
+// more than the host time it hides.
+constexpr size_t PIPELINE_MIN_CHARGES = 4u << 20;
+
 struct Job {
   // ---- problem (host) ----
   Options opt;
@@ -159,10 +164,11 @@ struct Job {
   struct Slot {
     long long *acc_m = nullptr, *acc_f = nullptr;
     float *rho = nullptr, *phi = nullptr, *t0 = nullptr, *t1 = nullptr;
-    double2 *gfu = nullptr, *dpin = nullptr;
+    double2 *samp = nullptr, *dpin = nullptr;
+    double *field_coef = nullptr;
     long long *partial = nullptr;
     uint32_t *tile_flag = nullptr;
-    double *part_wa = nullptr, *part_energy = nullptr, *part_gu = nullptr, *part_gf = nullptr, *part_ug = nullptr;
+    double *part_wa = nullptr, *part_energy = nullptr, *part_gf = nullptr, *part_ug = nullptr;
   } slot[2];
   double *d_rudy = nullptr;
   long long *d_rudy_fx = nullptr;
@@ -174,7 +180,7 @@ struct Job {
   double *d_s_q = nullptr;
   uint8_t *d_s_layer = nullptr;
   int tiles_x = 0, n_keys = 0;
-  int nb_wa_small = 0, nb_wa_large = 0, nb_energy = 0, nb_gu = 0, nb_gf = 0, nb_ug = 0;
+  int nb_wa_small = 0, nb_wa_large = 0, nb_energy = 0, nb_gf = 0, nb_ug = 0;
 
   void upload(const Basis *b, hipStream_t s);
   void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas);

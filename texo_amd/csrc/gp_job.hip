@@ -534,7 +534,7 @@ size_t Job::device_bytes() const {
   b += (n_net_total + 1) * 4 + n_pins_total * (4 + 16) + n_net_total * 8 + (small_nets.size() + large_nets.size()) * 4;
   b += (U + 1) * 4 + u_pins.size() * 4 + U * 12;
   b += C * 8 * (3 + 4 + 3) + 2 * U * 4;
-  b += 2 * (FG * (8 + 8 + 4 * 4) + (size_t)std::max(F, 1) * DU * 16 + n_pins_total * 16);
+  b += 2 * (FG * (8 + 8 + 4 * 4) + (M + Fl) * 16 + n_pins_total * 16);
   b += 2 * G * 16 + FG * 8 + 2 * U * 8 + C * 8;
   b += (M + Fl) * (4 + 4 + 16 + 8 + 1) + ((G >> (2 * SORT_TILE_SHIFT)) + W + H + 1) * 4;
   const size_t keys = (size_t)((W + DEP_TILE - 1) / DEP_TILE) * ((H + DEP_TILE - 1) / DEP_TILE);
@@ -549,7 +549,6 @@ void Job::upload(const Basis *b, hipStream_t s) {
   nb_wa_small = small_nets.empty() ? 0 : (int)blocks(small_nets.size(), BLOCK);
   nb_wa_large = large_nets.empty() ? 0 : (int)blocks(large_nets.size(), BLOCK / 64);
   nb_energy = F > 0 ? poisson_energy_blocks((int)W, (int)H, F) : 0;
-  nb_gu = (int)blocks(DU, BLOCK);
   nb_gf = (int)blocks(Fl, BLOCK);
   nb_ug = (int)blocks(U, BLOCK);
   tiles_x = (int)((W + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
@@ -589,13 +588,13 @@ void Job::upload(const Basis *b, hipStream_t s) {
     arena.reserve(sl.phi, FG);
     arena.reserve(sl.t0, FG);
     arena.reserve(sl.t1, FG);
-    arena.reserve(sl.gfu, (size_t)std::max(F, 1) * DU);
+    arena.reserve(sl.samp, M + Fl);
+    arena.reserve(sl.field_coef, 2 * MAXF);
     arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
     arena.reserve(sl.tile_flag, (size_t)n_keys);
     arena.reserve(sl.dpin, n_pins_total);
     arena.reserve(sl.part_wa, (size_t)nb_wa_small + nb_wa_large);
     arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
-    arena.reserve(sl.part_gu, (size_t)nb_gu * 2 * MAXF);
     arena.reserve(sl.part_gf, (size_t)nb_gf * GF_STRIDE);
     arena.reserve(sl.part_ug, (size_t)nb_ug * UG_STRIDE);
   }
@@ -779,19 +778,18 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.phi = w.phi;
   t.t0 = w.t0;
   t.t1 = w.t1;
-  t.gfu = w.gfu;
+  t.samp = w.samp;
+  t.field_coef = w.field_coef;
   t.partial = w.partial;
   t.tile_flag = w.tile_flag;
   t.dpin = w.dpin;
   t.part_wa = w.part_wa;
   t.part_energy = w.part_energy;
-  t.part_gu = w.part_gu;
   t.part_gf = w.part_gf;
   t.part_ug = w.part_ug;
   t.nb_wa_small = nb_wa_small;
   t.nb_wa_large = nb_wa_large;
   t.nb_energy = F > 0 ? nb_energy : 0;
-  t.nb_gu = F > 0 ? nb_gu : 0;
   t.nb_gf = F > 0 && Fl > 0 ? nb_gf : 0;
   t.nb_ug = nb_ug;
   t.coords = coords;

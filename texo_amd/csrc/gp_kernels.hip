@@ -366,23 +366,26 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
 // ---------------------------------------------------------------------------
 // Field gradient gather (electrostatic_placement.rs:338-374, 919-941).
 // ---------------------------------------------------------------------------
-// Each block re-reduces the transform's energy partials in one fixed serial
-// order and forms lambda (1 + beta E/E0) (eplace.rs:1111-1124).
-__device__ __forceinline__ void load_coefficients(const Task &t, double *coef) {
-  if (threadIdx.x < t.j.F) {
+// One block per task: E_f = 1/2 <rho_bar, phi> from the transform's
+// partials (fixed order) and the coefficient lambda (1 + beta E/E0)
+// (eplace.rs:1111-1124).
+__global__ void __launch_bounds__(BLOCK) k_coefficients(const Task *__restrict__ tasks) {
+  const Task &t = tasks[blockIdx.x];
+  double e[MAXF];
+  sum_partials<MAXF>(t.part_energy, t.nb_energy, e, MAXF, 0);
+  if (threadIdx.x < MAXF) {
     const int f = threadIdx.x;
-    double e = 0.0;
-    for (int b = 0; b < t.nb_energy; ++b) e += t.part_energy[b * MAXF + f];
-    coef[f] = t.lambda[f] * (1.0 + AUGMENTED_DENSITY_BETA * (e / t.scale[f]));
+    const double ef = f < t.j.F ? e[f] : 0.0;
+    t.field_coef[f] = t.lambda[f] * (1.0 + AUGMENTED_DENSITY_BETA * (ef / t.scale[f]));
+    t.field_coef[MAXF + f] = ef;
   }
-  __syncthreads();
 }
 
-__device__ __forceinline__ void sample(const float *__restrict__ phi, int W, int H, double x, double y, double &gx,
-                                       double &gy) {
-  Axis ax, ay;
-  bspline_axis(W, x, ax);
-  bspline_axis(H, y, ay);
+// d/dx, d/dy of the quadratic B-spline interpolant of phi at (x, y)
+// (sample_quadratic_bspline_with_gradient, electrostatic_placement.rs:919-941).
+// `at(ix, iy)` returns phi at a bin.
+template <class At>
+__device__ __forceinline__ void sample_field(const Axis &ax, const Axis &ay, At at, double &gx, double &gy) {
   const int ix[3] = {ax.i0, ax.i1, ax.i2};
   const int iy[3] = {ay.i0, ay.i1, ay.i2};
   const double wx[3] = {ax.w0, ax.w1, ax.w2};
@@ -395,72 +398,63 @@ __device__ __forceinline__ void sample(const float *__restrict__ phi, int W, int
   for (int r = 0; r < 3; ++r) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double p = static_cast<double>(phi[(size_t)iy[r] * W + ix[c]]);
+      const double p = static_cast<double>(at(ix[c], iy[r]));
       gx += dx[c] * wy[r] * p;
       gy += wx[c] * dy[r] * p;
     }
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_gather_units(const Task *__restrict__ tasks,
-                                                        BlockPlan plan) {
-  __shared__ double coef[MAXF];
+// One block per 8x8-bin tile of the charge sort: the potential of the
+// tile's window (tile + DEP_HALO) is staged in LDS and every charge of the
+// tile samples its field there (from HBM when it moved out of the window).
+// The sample goes to the charge's input index, so the per-unit and
+// per-filler sums below keep the reference's order.
+__global__ void __launch_bounds__(BLOCK) k_sample(const Task *__restrict__ tasks, BlockPlan plan) {
+  __shared__ float win[MAXF * DEP_WIN2];
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
-  load_coefficients(t, coef);
-  const int i = bm.y * BLOCK + threadIdx.x;
-  const size_t G = (size_t)a.W * a.H;
-  double red[2 * MAXF];
-#pragma unroll
-  for (int k = 0; k < 2 * MAXF; ++k) red[k] = 0.0;
-  if (i < a.n_density_units) {
-    const uint32_t u = a.du_unit[i];
-    const double ox = t.coords[2 * (size_t)u], oy = t.coords[2 * (size_t)u + 1];
-    double gxf[MAXF], gyf[MAXF];
-#pragma unroll
-    for (int f = 0; f < MAXF; ++f) gxf[f] = gyf[f] = 0.0;
-    for (uint32_t m = a.du_mstart[i]; m < a.du_mstart[i + 1]; ++m) {
-      const int f = a.m_field[m];
-      const double2 off = a.m_off[m];
-      const double q = a.m_charge[m];
-      double gx, gy;
-      sample(t.phi + f * G, a.W, a.H, ox + off.x, oy + off.y, gx, gy);
-#pragma unroll
-      for (int k = 0; k < MAXF; ++k) {
-        if (k == f) {
-          red[k] += q * (fabs(gx) + fabs(gy));
-          gxf[k] += q * gx;
-          gyf[k] += q * gy;
-        }
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < MAXF; ++f) {
-      if (f < a.F) {
-        red[MAXF + f] = fabs(gxf[f]) + fabs(gyf[f]);
-        t.gfu[(size_t)f * a.n_density_units + i] = make_double2(coef[f] * gxf[f], coef[f] * gyf[f]);
-        if (t.raw_units) {
-          t.raw_units[2 * ((size_t)f * a.n_density_units + i)] = gxf[f];
-          t.raw_units[2 * ((size_t)f * a.n_density_units + i) + 1] = gyf[f];
-        }
-      }
-    }
+  const int key = bm.y;
+  const uint32_t begin = key > 0 ? a.key_end[key - 1] : 0u, end = a.key_end[key];
+  if (begin == end) return;
+  const int x0 = (key % a.tiles_x) * DEP_TILE - DEP_HALO, y0 = (key / a.tiles_x) * DEP_TILE - DEP_HALO;
+  const size_t G = (size_t)a.W * (size_t)a.H;
+  for (int idx = threadIdx.x; idx < a.F * DEP_WIN2; idx += BLOCK) {
+    const int f = idx / DEP_WIN2, r = (idx - f * DEP_WIN2) / DEP_WIN, c = idx % DEP_WIN;
+    const int gx = x0 + c, gy = y0 + r;
+    win[idx] = (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) ? t.phi[f * G + (size_t)gy * a.W + gx] : 0.f;
   }
-  block_sum<2 * MAXF>(red, t.part_gu + (size_t)bm.y * 2 * MAXF);
+  __syncthreads();
+  for (uint32_t i = begin + threadIdx.x; i < end; i += BLOCK) {
+    // a filler's offset is +0.0, which leaves its coordinate unchanged
+    const uint32_t ci = a.s_ci[i];
+    const double2 off = a.s_off[i];
+    const int f = a.s_layer[i] >> 1;
+    Axis ax, ay;
+    bspline_axis(a.W, t.coords[2 * (size_t)ci] + off.x, ax);
+    bspline_axis(a.H, t.coords[2 * (size_t)ci + 1] + off.y, ay);
+    double gx, gy;
+    if (ax.i0 >= x0 && ax.i2 < x0 + DEP_WIN && ay.i0 >= y0 && ay.i2 < y0 + DEP_WIN) {
+      const float *w = win + f * DEP_WIN2;
+      sample_field(ax, ay, [&](int x, int y) { return w[(y - y0) * DEP_WIN + (x - x0)]; }, gx, gy);
+    } else {
+      const float *phi = t.phi + f * G;
+      sample_field(ax, ay, [&](int x, int y) { return phi[(size_t)y * a.W + x]; }, gx, gy);
+    }
+    t.samp[a.perm[i]] = make_double2(gx, gy);
+  }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_gather_fillers(const Task *__restrict__ tasks,
-                                                          BlockPlan plan) {
-  __shared__ double coef[MAXF];
+// Thread per filler, input order: combined filler gradient, force and raw
+// l1 sums, KKT term.
+__global__ void __launch_bounds__(BLOCK) k_filler_gradient(const Task *__restrict__ tasks, BlockPlan plan) {
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
-  load_coefficients(t, coef);
   const int k = bm.y * BLOCK + threadIdx.x;
-  const size_t G = (size_t)a.W * a.H;
   double red[2 * MAXF];
 #pragma unroll
   for (int q = 0; q < 2 * MAXF; ++q) red[q] = 0.0;
@@ -468,11 +462,10 @@ __global__ void __launch_bounds__(BLOCK) k_gather_fillers(const Task *__restrict
   bool bad = false;
   if (k < a.n_fillers) {
     const size_t c = 2 * ((size_t)a.n_units + k);
-    const double x = t.coords[c], y = t.coords[c + 1];
     const int f = a.fl_field[k];
     const double q = a.fl_charge[k];
-    double gx, gy;
-    sample(t.phi + f * G, a.W, a.H, x, y, gx, gy);
+    const double2 g = t.samp[(size_t)a.n_members + k];
+    const double gx = g.x, gy = g.y;
     const double rx = q * gx, ry = q * gy;
 #pragma unroll
     for (int p = 0; p < MAXF; ++p) {
@@ -486,13 +479,14 @@ __global__ void __launch_bounds__(BLOCK) k_gather_fillers(const Task *__restrict
       t.raw_fillers[2 * (size_t)k + 1] = ry;
     }
     // the combined gradient of a filler starts at 0.0 (eplace.rs:1031-1041)
-    const double vx = 0.0 + coef[f] * rx, vy = 0.0 + coef[f] * ry;
+    const double coef = t.field_coef[f];
+    const double vx = 0.0 + coef * rx, vy = 0.0 + coef * ry;
     t.grad[c] = vx;
     t.grad[c + 1] = vy;
     bad = !isfinite(vx) || !isfinite(vy);
     if (t.metric) {
-      kkt = fmax(kkt_scaled(vx, x, a.lower[c], a.upper[c], t.metric[c]),
-                 kkt_scaled(vy, y, a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
+      kkt = fmax(kkt_scaled(vx, t.coords[c], a.lower[c], a.upper[c], t.metric[c]),
+                 kkt_scaled(vy, t.coords[c + 1], a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
     }
   }
   block_sum<2 * MAXF>(red, t.part_gf + (size_t)bm.y * GF_STRIDE);
@@ -544,13 +538,57 @@ __device__ __forceinline__ double2 wa_derivative(double2 c, double xmin, double 
   return make_double2(ddx, ddy);
 }
 
-// Thread per net (<= 16 pins).
+// Net of exactly D pins: every pin coordinate is loaded once, up front, and
+// kept in registers for the three passes.  Same arithmetic, in the same
+// order, as the general loop below.
+template <int D>
+__device__ __forceinline__ double wa_fixed_net(const JobStatic &a, const Task &t, uint32_t b, double w) {
+  const double gamma = t.gamma;
+  double2 c[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) c[k] = pin_coordinate(a, t.coords, b + k);
+  double xmin = c[0].x, xmax = c[0].x, ymin = c[0].y, ymax = c[0].y;
+#pragma unroll
+  for (int k = 1; k < D; ++k) {
+    xmin = fmin(xmin, c[k].x);
+    xmax = fmax(xmax, c[k].x);
+    ymin = fmin(ymin, c[k].y);
+    ymax = fmax(ymax, c[k].y);
+  }
+  WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < D; ++k) wa_accumulate(c[k], xmin, xmax, ymin, ymax, gamma, s);
+  const double pmx = s.pdx / s.psx, nmx = s.ndx / s.nsx, pmy = s.pdy / s.psy, nmy = s.ndy / s.nsy;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double2 d = wa_derivative(c[k], xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
+    t.dpin[b + k] = make_double2(w * d.x, w * d.y);
+  }
+  return w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
+}
+
+// Thread per net (<= 16 pins).  Small nets are grouped by degree, so the
+// lanes of a wavefront take the same branch below.
 __device__ double wa_thread_net(const Task &t, int local) {
   const JobStatic &a = t.j;
   if (local >= a.n_small) return 0.0;
   const uint32_t net = a.small_nets[local];
   const uint32_t b = a.net_start[net], e = a.net_start[net + 1];
   const double w = a.net_weight[net];
+  switch (e - b) {
+    case 2:
+      return wa_fixed_net<2>(a, t, b, w);
+    case 3:
+      return wa_fixed_net<3>(a, t, b, w);
+    case 4:
+      return wa_fixed_net<4>(a, t, b, w);
+    case 5:
+      return wa_fixed_net<5>(a, t, b, w);
+    case 6:
+      return wa_fixed_net<6>(a, t, b, w);
+    default:
+      break;
+  }
   const double gamma = t.gamma;
   const double2 first = pin_coordinate(a, t.coords, b);
   double xmin = first.x, xmax = first.x, ymin = first.y, ymax = first.y;
@@ -628,15 +666,18 @@ __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, Bl
 }
 
 // Combined unit gradient (combined_objective, eplace.rs:1022-1044): WA
-// baseline pins, then timed-arc pins, then each field in order.
-__global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict__ tasks,
-                                                         BlockPlan plan) {
+// baseline pins, then timed-arc pins, then each field in order, the field
+// term being coefficient x the sum of the unit's member samples x charge in
+// member order (electrostatic_placement.rs:338-374).
+__global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict__ tasks, BlockPlan plan) {
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   const int u = bm.y * BLOCK + threadIdx.x;
-  double l1[1] = {0.0};
+  double red[1 + 2 * MAXF];
+#pragma unroll
+  for (int k = 0; k < 1 + 2 * MAXF; ++k) red[k] = 0.0;
   double kkt = 0.0;
   bool bad = false;
   if (u < a.n_units) {
@@ -653,17 +694,41 @@ __global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict_
       }
     }
     double gx = sx + tx, gy = sy + ty;
-    l1[0] = fabs(gx) + fabs(gy);
+    red[0] = fabs(gx) + fabs(gy);
     if (t.wa_grad_out) {
       t.wa_grad_out[2 * (size_t)u] = gx;
       t.wa_grad_out[2 * (size_t)u + 1] = gy;
     }
     const int du = a.unit_du[u];
     if (du >= 0) {
-      for (int f = 0; f < a.F; ++f) {
-        const double2 v = t.gfu[(size_t)f * a.n_density_units + du];
-        gx += v.x;
-        gy += v.y;
+      double gxf[MAXF], gyf[MAXF];
+#pragma unroll
+      for (int f = 0; f < MAXF; ++f) gxf[f] = gyf[f] = 0.0;
+      for (uint32_t m = a.du_mstart[du]; m < a.du_mstart[du + 1]; ++m) {
+        const int f = a.m_field[m];
+        const double q = a.m_charge[m];
+        const double2 g = t.samp[m];
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+          if (k == f) {
+            red[1 + k] += q * (fabs(g.x) + fabs(g.y));
+            gxf[k] += q * g.x;
+            gyf[k] += q * g.y;
+          }
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < MAXF; ++f) {
+        if (f < a.F) {
+          red[1 + MAXF + f] = fabs(gxf[f]) + fabs(gyf[f]);
+          const double coef = t.field_coef[f];
+          gx += coef * gxf[f];
+          gy += coef * gyf[f];
+          if (t.raw_units) {
+            t.raw_units[2 * ((size_t)f * a.n_density_units + du)] = gxf[f];
+            t.raw_units[2 * ((size_t)f * a.n_density_units + du) + 1] = gyf[f];
+          }
+        }
       }
     }
     const size_t c = 2 * (size_t)u;
@@ -675,8 +740,8 @@ __global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict_
                  kkt_scaled(gy, t.coords[c + 1], a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
     }
   }
-  block_sum<1>(l1, t.part_ug + (size_t)bm.y * UG_STRIDE);
-  block_max_flag(kkt, bad, t.part_ug + (size_t)bm.y * UG_STRIDE + 1);
+  block_sum<1 + 2 * MAXF>(red, t.part_ug + (size_t)bm.y * UG_STRIDE);
+  block_max_flag(kkt, bad, t.part_ug + (size_t)bm.y * UG_STRIDE + 1 + 2 * MAXF);
 }
 
 // One block per task: every FP64 scalar of the evaluation.
@@ -686,21 +751,17 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tas
   double v1[1];
   sum_partials<1>(t.part_wa, t.nb_wa_small + t.nb_wa_large, v1, 1, 0);
   if (threadIdx.x == 0) sc->wa_value = v1[0];
-  sum_partials<1>(t.part_ug, t.nb_ug, v1, UG_STRIDE, 0);
-  if (threadIdx.x == 0) sc->wa_l1 = v1[0];
-  double gu[2 * MAXF], gf[2 * MAXF], mu[2], mf[2];
-  sum_partials<2 * MAXF>(t.part_gu, t.nb_gu, gu, 2 * MAXF, 0);
+  double ug[1 + 2 * MAXF], gf[2 * MAXF], mu[2], mf[2];
+  sum_partials<1 + 2 * MAXF>(t.part_ug, t.nb_ug, ug, UG_STRIDE, 0);
   sum_partials<2 * MAXF>(t.part_gf, t.nb_gf, gf, GF_STRIDE, 0);
-  max_partials<2>(t.part_ug, t.nb_ug, mu, UG_STRIDE, 1);
+  max_partials<2>(t.part_ug, t.nb_ug, mu, UG_STRIDE, 1 + 2 * MAXF);
   max_partials<2>(t.part_gf, t.nb_gf, mf, GF_STRIDE, 2 * MAXF);
+  if (threadIdx.x == 0) sc->wa_l1 = ug[0];
   if (threadIdx.x < MAXF) {
-    // same serial order as the coefficient in the gather kernels
     const int f = threadIdx.x;
-    double e = 0.0;
-    for (int b = 0; b < t.nb_energy; ++b) e += t.part_energy[b * MAXF + f];
-    sc->energy[f] = f < t.j.F ? e : 0.0;
-    sc->force_l1[f] = gu[f] + gf[f];
-    sc->raw_l1[f] = gu[MAXF + f] + gf[MAXF + f];
+    sc->energy[f] = f < t.j.F ? t.field_coef[MAXF + f] : 0.0;
+    sc->force_l1[f] = ug[1 + f] + gf[f];
+    sc->raw_l1[f] = ug[1 + MAXF + f] + gf[MAXF + f];
   }
   if (threadIdx.x == 0) {
     sc->kkt_bits = static_cast<unsigned long long>(__double_as_longlong(fmax(mu[0], mf[0])));
@@ -972,14 +1033,15 @@ void launch_sort(const SortTask *tasks, const SortMaps &m, hipStream_t s) {
 }
 
 void launch_gather(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.gather_units.total > 0) {
+  if (m.n_tasks <= 0) return;
+  hipLaunchKernelGGL(k_coefficients, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
+  if (m.sample.total > 0) {
     timer_begin(g_timer, "gather", s);
-    hipLaunchKernelGGL(k_gather_units, dim3(plan_grid(m.gather_units)), dim3(BLOCK), 0, s, tasks, m.gather_units);
+    hipLaunchKernelGGL(k_sample, dim3(plan_grid(m.sample)), dim3(BLOCK), 0, s, tasks, m.sample);
     timer_end(g_timer, "gather", s);
   }
-  if (m.gather_fillers.total > 0)
-    hipLaunchKernelGGL(k_gather_fillers, dim3(plan_grid(m.gather_fillers)), dim3(BLOCK), 0, s, tasks,
-                       m.gather_fillers);
+  if (m.fillers.total > 0)
+    hipLaunchKernelGGL(k_filler_gradient, dim3(plan_grid(m.fillers)), dim3(BLOCK), 0, s, tasks, m.fillers);
 }
 
 void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
