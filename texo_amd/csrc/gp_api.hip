@@ -242,6 +242,16 @@ int texo_gp_session_poisson(texo_gp_session *s, const double *density, double *p
     }
     if (J.p_slot == 0) J.p_valid = false;
     HIP(hipMemcpyAsync(J.slot[0].rho, rho.data(), FG * sizeof(float), hipMemcpyHostToDevice, st));
+    // input range of the FP16-scaled transform (k_convert records it in a full evaluation)
+    std::vector<unsigned int> amax(4 * texo::MAXF, 0u);
+    for (size_t i = 0; i < FG; ++i) {
+      const float a = std::fabs(rho[i]);
+      unsigned int bits;
+      std::memcpy(&bits, &a, 4);
+      amax[i / J.G] = std::max(amax[i / J.G], bits);
+    }
+    HIP(hipMemcpyAsync(J.slot[0].amax, amax.data(), amax.size() * 4, hipMemcpyHostToDevice, st));
+    HIP(hipStreamSynchronize(st));
     texo::Task t = J.task(0, J.d_eval_coords, J.d_grad[0], nullptr, 1.0, false, &J.d_scalars->eval[2]);
     std::vector<int2> gh, gw;
     for (int b = 0; b < (int)((J.F * J.H + 31) / 32); ++b) gh.push_back(make_int2(0, b));

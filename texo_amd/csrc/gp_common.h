@@ -36,9 +36,8 @@ struct BlockPlan {
   int n_tasks, total, per;
 };
 constexpr int XCDS = 8;
-inline int plan_grid(const BlockPlan &p) { return p.per > 0 ? XCDS * p.per : p.total; }
-__device__ __forceinline__ bool plan_block(const BlockPlan &p, int2 &bm) {
-  const int b = blockIdx.x;
+__host__ __device__ inline int plan_grid(const BlockPlan &p) { return p.per > 0 ? XCDS * p.per : p.total; }
+__device__ __forceinline__ bool plan_block_at(const BlockPlan &p, int b, int2 &bm) {
   const int lb = p.per > 0 ? (b % XCDS) * p.per + b / XCDS : b;
   if (lb >= p.total) return false;
   int lo = 0, hi = p.n_tasks - 1;  // last task whose first block is <= lb
@@ -52,6 +51,7 @@ __device__ __forceinline__ bool plan_block(const BlockPlan &p, int2 &bm) {
   bm = make_int2(lo, lb - p.start[lo]);
   return true;
 }
+__device__ __forceinline__ bool plan_block(const BlockPlan &p, int2 &bm) { return plan_block_at(p, blockIdx.x, bm); }
 
 // Reductions finished by k_finalize for one objective evaluation.
 struct EvalScalars {
@@ -183,6 +183,7 @@ struct Task {
   double2 *samp;               // [M+Fl] field sample (d/dx, d/dy of the potential) per charge
   double *field_coef;          // [2*MAXF] lambda (1 + beta E/E0) per field, then E per field
   long long *partial;          // [n_keys][2F][DEP_WIN2] per-tile deposit windows
+  unsigned int *amax;          // [4][MAXF] max |input| of each transform stage per field (float bits)
   uint32_t *tile_flag;         // [n_keys] 1 where the tile's window was written
   double2 *dpin;               // [pins] weighted WA derivative
   double *part_wa, *part_energy, *part_gf, *part_ug;
@@ -239,8 +240,9 @@ struct OptTask {
   const double *yp, *gy, *origin;
   double *cand;
   double lipschitz;
-  double *part_step;  // [blocks][4] candidate, [blocks] extrapolation flag
+  double *part_step;  // [blocks][4]: candidate sums, change max, changed flag; extrapolation flag in [3]
   StepScalars *step;
+  int do_precond, do_extrapolate, do_candidate;  // the work this round asks of the job
 };
 
 struct Maps {
@@ -256,8 +258,7 @@ void launch_combine(const Task *tasks, const Maps &m, hipStream_t s);
 // Optimizer maps; each finalize list holds one task index per block.
 struct OptMaps {
   BlockPlan precond, extrapolate, candidate;
-  const int *precond_list, *candidate_list;
-  int n_precond, n_candidate;
+  int n_tasks;  // one finalize block per task
 };
 void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s);
 

@@ -69,6 +69,7 @@ __device__ __forceinline__ float wave_max_f(float v) {
 struct Stage {
   const float *A;
   float *out;
+  unsigned int *amax_out;  // max |out| per field for the next stage (fast path), or null
   int M, K, N, R;  // rows of this task, inner, columns, rows per field
   bool transposed;
   const float *inv_lambda;
@@ -83,6 +84,7 @@ __device__ __forceinline__ Stage stage_of(const Task &t) {
   s.W = W;
   s.inv_lambda = nullptr;
   s.energy = nullptr;
+  s.amax_out = S < 3 ? t.amax + (S + 1) * MAXF : nullptr;
   if constexpr (S == 0) {
     s.A = t.rho;
     s.out = t.t0;
@@ -140,7 +142,8 @@ __device__ __forceinline__ void store_element(const Stage &e, int row, int col, 
 
 // rows row0..row0+3 of column col; row0 % 4 == 0 and R % 4 == 0 keep the
 // four rows inside one field
-__device__ __forceinline__ void store4(const Stage &e, int row0, int col, float4_t v, double (&energy)[MAXF]) {
+// Stores rows row0..row0+3 of column col; `v` leaves holding the stored values.
+__device__ __forceinline__ void store4(const Stage &e, int row0, int col, float4_t &v, double (&energy)[MAXF]) {
   const int f = row0 / e.R, r0 = row0 - f * e.R;
   if (e.inv_lambda) {
     const float4 lam = *reinterpret_cast<const float4 *>(e.inv_lambda + (size_t)col * e.W + r0);
@@ -197,6 +200,19 @@ __device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][3
   }
   double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
   store4(e, m0 + 4 * rg, n0 + col, v, energy);
+  if (e.amax_out) {
+    // the 32 rows of a fast-shape tile lie in one field; v now holds the
+    // stored values (after the spectral scaling in stage 1)
+    __shared__ unsigned int sm[SPLIT];
+    float mx = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
+    mx = wave_max_f(mx);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = __float_as_uint(mx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned int b = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
+      if (b) atomicMax(e.amax_out + m0 / e.R, b);
+    }
+  }
   if (!e.energy) return;
   __shared__ double se[SPLIT][MAXF];
 #pragma unroll
@@ -282,17 +298,16 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__r
   fast_epilogue(e, red, m0, n0, bm.y);
 }
 
-// FP16-scaled split precision on MFMA.  A pre-pass finds max |A| over the
-// block's 32 rows; A is scaled by a power of two into the top of FP16's
-// range and split into hi + lo FP16 (22 significant bits); the basis is
-// pre-scaled, pre-split and pre-transposed.  hi*hi + hi*lo + lo*hi run on
+// FP16-scaled split precision on MFMA.  The producer of A (k_convert, or the
+// previous stage's epilogue) recorded max |A| per field; A is scaled by a
+// power of two into the top of FP16's range and split into hi + lo FP16 (22
+// significant bits); the basis is pre-scaled, pre-split and pre-transposed.  hi*hi + hi*lo + lo*hi run on
 // v_mfma_f32_16x16x16_f16 with FP32 accumulation; the product of the two
 // power-of-two scales is divided out exactly before the epilogue.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__restrict__ tasks,
                                                                  const int2 *__restrict__ map) {
   __shared__ float red[SPLIT][32][33];
-  __shared__ float smax[SPLIT];
   const int2 bm = map[blockIdx.y];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
@@ -304,17 +319,8 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
   const int q = lane >> 4, r = lane & 15;
   const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
   const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
-  float amax = 0.f;
-  for (int k0 = 16 * wave; k0 < K; k0 += 16 * SPLIT) {
-    const float4 u = *reinterpret_cast<const float4 *>(a0 + k0);
-    const float4 v = *reinterpret_cast<const float4 *>(a1 + k0);
-    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(u.x), fabsf(u.y)), fmaxf(fabsf(u.z), fabsf(u.w))));
-    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
-  }
-  amax = wave_max_f(amax);
-  if (lane == 0) smax[wave] = amax;
-  __syncthreads();
-  const float sa = pow2_scale(fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3])));
+  // max |A| over the 32 rows' field, recorded by the producer of A
+  const float sa = pow2_scale(__uint_as_float(t.amax[S * MAXF + m0 / e.R]));
   const float unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
   const size_t k4 = (size_t)K / 4;
   const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;

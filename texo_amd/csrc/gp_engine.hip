@@ -814,6 +814,9 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
       o.lipschitz = j->L;
       o.part_step = j->d_part_step;
       o.step = &j->d_scalars->step;
+      o.do_precond = j->want_precond;
+      o.do_extrapolate = j->want_extrapolate;
+      o.do_candidate = j->want_candidate;
       const int oi = (int)opt.size();
       opt.push_back(o);
       if (j->want_precond) {
@@ -893,10 +896,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
     om.precond = p_pre.device(d_stage_, o_ppre);
     om.extrapolate = p_ext.device(d_stage_, o_pext);
     om.candidate = p_cand.device(d_stage_, o_pcand);
-    om.precond_list = reinterpret_cast<const int *>(dptr(o_lpre));
-    om.candidate_list = reinterpret_cast<const int *>(dptr(o_lcand));
-    om.n_precond = (int)l_pre.size();
-    om.n_candidate = (int)l_cand.size();
+    om.n_tasks = (int)opt.size();
     launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, stream_);
   }
   if (!sorts.empty()) {

@@ -167,6 +167,7 @@ struct Job {
     double2 *samp = nullptr, *dpin = nullptr;
     double *field_coef = nullptr;
     long long *partial = nullptr;
+    unsigned int *amax = nullptr;
     uint32_t *tile_flag = nullptr;
     double *part_wa = nullptr, *part_energy = nullptr, *part_gf = nullptr, *part_ug = nullptr;
   } slot[2];

@@ -592,6 +592,7 @@ void Job::upload(const Basis *b, hipStream_t s) {
     arena.reserve(sl.field_coef, 2 * MAXF);
     arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
     arena.reserve(sl.tile_flag, (size_t)n_keys);
+    arena.reserve(sl.amax, 4 * MAXF);
     arena.reserve(sl.dpin, n_pins_total);
     arena.reserve(sl.part_wa, (size_t)nb_wa_small + nb_wa_large);
     arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
@@ -781,6 +782,7 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.samp = w.samp;
   t.field_coef = w.field_coef;
   t.partial = w.partial;
+  t.amax = w.amax;
   t.tile_flag = w.tile_flag;
   t.dpin = w.dpin;
   t.part_wa = w.part_wa;

@@ -138,10 +138,14 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   const int key = bm.y;
-  if (key == 0 && threadIdx.x < MAXF) {
-    // the task's int64 sums restart (k_convert adds to them)
-    t.scalars->over_fx[threadIdx.x] = 0;
-    t.scalars->net_fx[threadIdx.x] = 0;
+  if (key == 0 && threadIdx.x < 4 * MAXF) {
+    // the task's int64 sums and transform maxima restart (k_convert and the
+    // transform stages add to them)
+    if (threadIdx.x < MAXF) {
+      t.scalars->over_fx[threadIdx.x] = 0;
+      t.scalars->net_fx[threadIdx.x] = 0;
+    }
+    t.amax[threadIdx.x] = 0u;
   }
   const uint32_t begin = key > 0 ? a.key_end[key - 1] : 0u, end = a.key_end[key];
   if (begin == end) {
@@ -308,6 +312,7 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
 
 __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ tasks, BlockPlan plan) {
   __shared__ long long s_over[BLOCK / 64][MAXF], s_net[BLOCK / 64][MAXF];
+  __shared__ unsigned int s_max[BLOCK / 64][MAXF];
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
@@ -316,6 +321,7 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
   const int i = bm.y * BLOCK + threadIdx.x;
   int f = -1;
   long long over = 0, net = 0;
+  unsigned int amax = 0u;  // |rho| as float bits: the FP16-scaled transform's input range
   if (i < a.F * G) {
     f = i / G;
     const int cell = i - f * G, y = cell / a.W, x = cell - y * a.W;
@@ -338,19 +344,24 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
       }
     const long long v = m + fl;
     const double d = static_cast<double>(v) * FX_INV;
-    t.rho[i] = static_cast<float>(d);
+    const float r = static_cast<float>(d);
+    t.rho[i] = r;
     if (t.density_out) t.density_out[i] = d;
     over = m > 0 ? m : 0;
     net = v;
+    amax = __float_as_uint(fabsf(r));
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < MAXF; ++k) {
     const long long o = wave_sum_ll(f == k ? over : 0);
     const long long n = wave_sum_ll(f == k ? net : 0);
+    unsigned int mx = f == k ? amax : 0u;
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned int)__shfl_xor((int)mx, off, 64));
     if (lane == 0) {
       s_over[wave][k] = o;
       s_net[wave][k] = n;
+      s_max[wave][k] = mx;
     }
   }
   __syncthreads();
@@ -360,6 +371,8 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
     const long long n = s_net[0][k] + s_net[1][k] + s_net[2][k] + s_net[3][k];
     atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->over_fx[k]), static_cast<unsigned long long>(o));
     atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->net_fx[k]), static_cast<unsigned long long>(n));
+    const unsigned int mx = max(max(s_max[0][k], s_max[1][k]), max(s_max[2][k], s_max[3][k]));
+    if (mx) atomicMax(&t.amax[k], mx);
   }
 }
 
@@ -772,13 +785,12 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tas
 // ---------------------------------------------------------------------------
 // Optimizer elementwise kernels.
 // ---------------------------------------------------------------------------
+// Optimizer elementwise work of one round in one launch: preconditioner
+// blocks, then extrapolation blocks, then candidate blocks (different jobs,
+// or different vectors of one job, so they do not depend on each other).
 // diagonal_preconditioner (eplace.rs:1046-1100); per-block max/min/flag.
-__global__ void __launch_bounds__(BLOCK) k_preconditioner(const OptTask *__restrict__ tasks,
-                                                          BlockPlan plan) {
-  int2 bm;
-  if (!plan_block(plan, bm)) return;
-  const OptTask &t = tasks[bm.x];
-  const int i = bm.y * BLOCK + threadIdx.x;
+__device__ void opt_preconditioner(const OptTask &t, int lb) {
+  const int i = lb * BLOCK + threadIdx.x;
   double d = 0.0;
   bool valid = false;
   if (i < t.n_units) {
@@ -819,50 +831,16 @@ __global__ void __launch_bounds__(BLOCK) k_preconditioner(const OptTask *__restr
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double *p = t.part_metric + 3 * (size_t)bm.y;
+    double *p = t.part_metric + 3 * (size_t)lb;
     p[0] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
     p[1] = fmin(fmin(smin[0], smin[1]), fmin(smin[2], smin[3]));
     p[2] = (sbad[0] | sbad[1] | sbad[2] | sbad[3]) ? 1.0 : 0.0;
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_metric_finalize(const OptTask *__restrict__ tasks,
-                                                           const int *__restrict__ list) {
-  const OptTask &t = tasks[list[blockIdx.x]];
-  const int nb = (t.n_units + t.n_fillers + BLOCK - 1) / BLOCK;
-  __shared__ double smax[BLOCK / 64], smin[BLOCK / 64], sbad[BLOCK / 64];
-  double mx = 0.0, mn = INFINITY, b = 0.0;
-  for (int i = threadIdx.x; i < nb; i += BLOCK) {
-    mx = fmax(mx, t.part_metric[3 * (size_t)i]);
-    mn = fmin(mn, t.part_metric[3 * (size_t)i + 1]);
-    b = fmax(b, t.part_metric[3 * (size_t)i + 2]);
-  }
-  mx = wave_max(mx);
-  mn = wave_min(mn);
-  b = wave_max(b);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    smax[wave] = mx;
-    smin[wave] = mn;
-    sbad[wave] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    t.metric_scalars->max_bits = static_cast<unsigned long long>(
-        __double_as_longlong(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]))));
-    t.metric_scalars->min_bits = static_cast<unsigned long long>(
-        __double_as_longlong(fmin(fmin(smin[0], smin[1]), fmin(smin[2], smin[3]))));
-    t.metric_scalars->nonfinite = fmax(fmax(sbad[0], sbad[1]), fmax(sbad[2], sbad[3])) > 0.0 ? 1u : 0u;
-    t.step->differs = 0u;  // reset before this round's extrapolation
-  }
-}
-
-__global__ void __launch_bounds__(BLOCK) k_extrapolate(const OptTask *__restrict__ tasks,
-                                                       BlockPlan plan) {
-  int2 bm;
-  if (!plan_block(plan, bm)) return;
-  const OptTask &t = tasks[bm.x];
-  const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
+// y = clamp(x + beta (x - xp)); per-block flag "y differs from x".
+__device__ void opt_extrapolate(const OptTask &t, int lb) {
+  const size_t i = (size_t)lb * BLOCK + threadIdx.x;
   bool changed = false;
   if (i < t.n) {
     const double cur = t.x[i];
@@ -872,16 +850,13 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(const OptTask *__restrict
     t.y[i] = v;
     changed = v != cur;
   }
-  // one plain store per block that saw a change (no atomics)
-  if (__syncthreads_or(changed) && threadIdx.x == 0) t.step->differs = 1u;
+  const bool any = __syncthreads_or(changed);
+  if (threadIdx.x == 0) t.part_step[4 * (size_t)lb + 3] = any ? 1.0 : 0.0;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_candidate(const OptTask *__restrict__ tasks,
-                                                     BlockPlan plan) {
-  int2 bm;
-  if (!plan_block(plan, bm)) return;
-  const OptTask &t = tasks[bm.x];
-  const size_t i = (size_t)bm.y * BLOCK + threadIdx.x;
+// c = clamp(yp - gy / (L m)); per-block g.(c - y), sum m (c - y)^2, max |c - x|, changed.
+__device__ void opt_candidate(const OptTask &t, int lb) {
+  const size_t i = (size_t)lb * BLOCK + threadIdx.x;
   double red[2] = {0.0, 0.0};
   double change = 0.0;
   bool changed = false;
@@ -898,22 +873,75 @@ __global__ void __launch_bounds__(BLOCK) k_candidate(const OptTask *__restrict__
     red[1] = m * d * d;
     change = fabs(c - t.origin[i]);
   }
-  block_sum<2>(red, t.part_step + 4 * (size_t)bm.y);
-  block_max_flag(change, changed, t.part_step + 4 * (size_t)bm.y + 2);
+  block_sum<2>(red, t.part_step + 4 * (size_t)lb);
+  block_max_flag(change, changed, t.part_step + 4 * (size_t)lb + 2);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_step_finalize(const OptTask *__restrict__ tasks,
-                                                         const int *__restrict__ list) {
-  const OptTask &t = tasks[list[blockIdx.x]];
+__global__ void __launch_bounds__(BLOCK) k_optimizer(const OptTask *__restrict__ tasks, OptMaps m) {
+  const int g0 = plan_grid(m.precond), g1 = plan_grid(m.extrapolate);
+  int b = blockIdx.x;
+  int2 bm;
+  if (b < g0) {
+    if (plan_block_at(m.precond, b, bm)) opt_preconditioner(tasks[bm.x], bm.y);
+    return;
+  }
+  b -= g0;
+  if (b < g1) {
+    if (plan_block_at(m.extrapolate, b, bm)) opt_extrapolate(tasks[bm.x], bm.y);
+    return;
+  }
+  b -= g1;
+  if (plan_block_at(m.candidate, b, bm)) opt_candidate(tasks[bm.x], bm.y);
+}
+
+// One block per task: the metric extremes (line search denominator checks,
+// global_placement.rs:215-232), the extrapolation flag, and the candidate
+// step's reductions, each in a fixed order.
+__global__ void __launch_bounds__(BLOCK) k_optimizer_finalize(const OptTask *__restrict__ tasks) {
+  const OptTask &t = tasks[blockIdx.x];
+  if (t.do_precond) {
+    const int nb = (t.n_units + t.n_fillers + BLOCK - 1) / BLOCK;
+    __shared__ double smax[BLOCK / 64], smin[BLOCK / 64], sbad[BLOCK / 64];
+    double mx = 0.0, mn = INFINITY, bad = 0.0;
+    for (int i = threadIdx.x; i < nb; i += BLOCK) {
+      mx = fmax(mx, t.part_metric[3 * (size_t)i]);
+      mn = fmin(mn, t.part_metric[3 * (size_t)i + 1]);
+      bad = fmax(bad, t.part_metric[3 * (size_t)i + 2]);
+    }
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    bad = wave_max(bad);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+      smax[wave] = mx;
+      smin[wave] = mn;
+      sbad[wave] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      t.metric_scalars->max_bits = static_cast<unsigned long long>(
+          __double_as_longlong(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]))));
+      t.metric_scalars->min_bits = static_cast<unsigned long long>(
+          __double_as_longlong(fmin(fmin(smin[0], smin[1]), fmin(smin[2], smin[3]))));
+      t.metric_scalars->nonfinite = fmax(fmax(sbad[0], sbad[1]), fmax(sbad[2], sbad[3])) > 0.0 ? 1u : 0u;
+    }
+  }
   const int nb = (int)((t.n + BLOCK - 1) / BLOCK);
-  double v[2], m[2];
-  sum_partials<2>(t.part_step, nb, v, 4, 0);
-  max_partials<2>(t.part_step, nb, m, 4, 2);
-  if (threadIdx.x == 0) {
-    t.step->gdelta = v[0];
-    t.step->msq = v[1];
-    t.step->change_bits = static_cast<unsigned long long>(__double_as_longlong(m[0]));
-    t.step->changed = m[1] > 0.0 ? 1u : 0u;
+  if (t.do_extrapolate) {
+    double f[1];
+    max_partials<1>(t.part_step, nb, f, 4, 3);
+    if (threadIdx.x == 0) t.step->differs = f[0] > 0.0 ? 1u : 0u;
+  }
+  if (t.do_candidate) {
+    double v[2], mm[2];
+    sum_partials<2>(t.part_step, nb, v, 4, 0);
+    max_partials<2>(t.part_step, nb, mm, 4, 2);
+    if (threadIdx.x == 0) {
+      t.step->gdelta = v[0];
+      t.step->msq = v[1];
+      t.step->change_bits = static_cast<unsigned long long>(__double_as_longlong(mm[0]));
+      t.step->changed = mm[1] > 0.0 ? 1u : 0u;
+    }
   }
 }
 
@@ -1058,16 +1086,9 @@ void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
 }
 
 void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
-  if (m.precond.total > 0) {
-    hipLaunchKernelGGL(k_preconditioner, dim3(plan_grid(m.precond)), dim3(BLOCK), 0, s, tasks, m.precond);
-    hipLaunchKernelGGL(k_metric_finalize, dim3(m.n_precond), dim3(BLOCK), 0, s, tasks, m.precond_list);
-  }
-  if (m.extrapolate.total > 0)
-    hipLaunchKernelGGL(k_extrapolate, dim3(plan_grid(m.extrapolate)), dim3(BLOCK), 0, s, tasks, m.extrapolate);
-  if (m.candidate.total > 0) {
-    hipLaunchKernelGGL(k_candidate, dim3(plan_grid(m.candidate)), dim3(BLOCK), 0, s, tasks, m.candidate);
-    hipLaunchKernelGGL(k_step_finalize, dim3(m.n_candidate), dim3(BLOCK), 0, s, tasks, m.candidate_list);
-  }
+  const int grid = plan_grid(m.precond) + plan_grid(m.extrapolate) + plan_grid(m.candidate);
+  if (grid > 0) hipLaunchKernelGGL(k_optimizer, dim3(grid), dim3(BLOCK), 0, s, tasks, m);
+  if (m.n_tasks > 0) hipLaunchKernelGGL(k_optimizer_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
 void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, const double2 *pin_xy,
