@@ -162,8 +162,8 @@ struct JobStatic {
   const uint32_t *pin_unit;   // NO_UNIT for fixed pins
   const double2 *pin_xy;      // offset, or fixed coordinate
   const uint32_t *u_pstart;   // unit -> incident movable pins
-  const uint32_t *u_pins;
-  uint32_t first_arc_pin;
+  const uint32_t *pin_slot;  // pin -> its entry in the unit-ordered derivative array (NO_UNIT: fixed pin)
+  const uint32_t *u_nbase;   // unit -> number of its net pins (its timed-arc pins follow them)
   // transform bases (shared by every job with this grid)
   const float *b32[4];        // C_W^T, C_H^T, C_H, C_W (row-major K x N)
   const uint16_t *bthi[4];    // transposed BF16 hi/lo splits (N x K)
@@ -185,7 +185,7 @@ struct Task {
   long long *partial;          // [n_keys][2F][DEP_WIN2] per-tile deposit windows
   unsigned int *amax;          // [4][MAXF] max |input| of each transform stage per field (float bits)
   uint32_t *tile_flag;         // [n_keys] 1 where the tile's window was written
-  double2 *dpin;               // [pins] weighted WA derivative
+  double2 *dpin;               // [movable pins] weighted WA derivative, grouped by unit
   double *part_wa, *part_energy, *part_gf, *part_ug;
   int nb_wa_small, nb_wa_large, nb_energy, nb_gf, nb_ug;
   // request
@@ -206,8 +206,9 @@ struct Task {
 struct SortTask {
   JobStatic j;
   const double *coords;
-  uint32_t *count;  // [n_keys + 1]
-  uint32_t *perm;   // [members + fillers]
+  uint32_t *count;        // [n_keys + 1]
+  const uint32_t *order;  // the previous deposit order (walked so that a block's charges share tiles)
+  uint32_t *perm;         // [members + fillers] the new order
   uint32_t *s_ci;   // the charges in the new order
   double2 *s_off;
   double *s_q;

@@ -138,7 +138,7 @@ struct Job {
   std::vector<uint32_t> routing_h, routing_v;
   std::map<uint32_t, Control> controls;
   size_t n_nets = 0, n_arcs = 0, n_net_total = 0, n_pins_total = 0, first_arc_pin = 0;
-  std::vector<uint32_t> net_start, pin_unit, small_nets, large_nets, u_pstart, u_pins;
+  std::vector<uint32_t> net_start, pin_unit, small_nets, large_nets, u_pstart, u_pins, pin_slot, u_nbase;
   std::vector<double> pin_xy, net_weight;
 
   void load(const texo_gp_problem *p, const texo_gp_options *o);
@@ -154,7 +154,7 @@ struct Job {
   double *d_m_charge = nullptr, *d_fl_charge = nullptr, *d_du_fcharge = nullptr;
   uint8_t *d_m_field = nullptr, *d_fl_field = nullptr;
   uint32_t *d_net_start = nullptr, *d_pin_unit = nullptr, *d_small = nullptr, *d_large = nullptr;
-  uint32_t *d_u_pstart = nullptr, *d_u_pins = nullptr;
+  uint32_t *d_u_pstart = nullptr, *d_pin_slot = nullptr, *d_u_nbase = nullptr;
   double *d_net_weight = nullptr, *d_wl_diag = nullptr;
   int32_t *d_unit_du = nullptr;
   double *d_lower = nullptr, *d_upper = nullptr, *d_metric = nullptr, *d_part_metric = nullptr, *d_part_step = nullptr;
@@ -176,7 +176,7 @@ struct Job {
   unsigned int *d_rudy_err = nullptr;
   double *d_raw_units = nullptr, *d_raw_fillers = nullptr, *d_density_out = nullptr, *d_wa_grad = nullptr;
   double *d_eval_coords = nullptr;
-  uint32_t *d_perm = nullptr, *d_sort_count = nullptr, *d_s_ci = nullptr;
+  uint32_t *d_perm = nullptr, *d_perm_next = nullptr, *d_sort_count = nullptr, *d_s_ci = nullptr;
   double2 *d_s_off = nullptr;
   double *d_s_q = nullptr;
   uint8_t *d_s_layer = nullptr;
@@ -187,7 +187,7 @@ struct Job {
   void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas);
   Task task(int slot, const double *coords, double *grad, const double *metric, double gamma, bool skip,
             EvalScalars *scalars) const;
-  SortTask sort_task() const;
+  SortTask sort_task();  // next sort; the new order becomes the job's order
 
   // ---- engine bookkeeping ----
   int64_t id = 0;

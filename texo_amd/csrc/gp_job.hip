@@ -392,6 +392,16 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
         large_nets.push_back((uint32_t)k);
     }
     for (auto &i : u_pins) i = new_pin[i];
+    // each pin's derivative goes straight to its unit's entry (nets first,
+    // then timed arcs, each in input order), so the per-unit sums read
+    // contiguously in the reference's order
+    pin_slot.assign(n_pins_total, NO_UNIT);
+    u_nbase.assign(U, 0);
+    for (size_t u = 0; u < U; ++u)
+      for (uint32_t k = u_pstart[u]; k < u_pstart[u + 1]; ++k) {
+        pin_slot[u_pins[k]] = k;
+        if (u_pins[k] < first_arc_pin) u_nbase[u] += 1;
+      }
     net_start.swap(start2);
     pin_unit.swap(unit2);
     pin_xy.swap(xy2);
@@ -532,11 +542,11 @@ size_t Job::device_bytes() const {
   const size_t FG = (size_t)std::max(F, 1) * G;
   size_t b = FG * 8 + M * (4 + 16 + 8 + 1) + DU * 8 + (size_t)std::max(F, 1) * DU * 8 + Fl * 9;
   b += (n_net_total + 1) * 4 + n_pins_total * (4 + 16) + n_net_total * 8 + (small_nets.size() + large_nets.size()) * 4;
-  b += (U + 1) * 4 + u_pins.size() * 4 + U * 12;
+  b += (U + 1) * 4 + n_pins_total * 4 + U * 16;
   b += C * 8 * (3 + 4 + 3) + 2 * U * 4;
   b += 2 * (FG * (8 + 8 + 4 * 4) + (M + Fl) * 16 + n_pins_total * 16);
   b += 2 * G * 16 + FG * 8 + 2 * U * 8 + C * 8;
-  b += (M + Fl) * (4 + 4 + 16 + 8 + 1) + ((G >> (2 * SORT_TILE_SHIFT)) + W + H + 1) * 4;
+  b += (M + Fl) * (4 + 4 + 4 + 16 + 8 + 1) + ((G >> (2 * SORT_TILE_SHIFT)) + W + H + 1) * 4;
   const size_t keys = (size_t)((W + DEP_TILE - 1) / DEP_TILE) * ((H + DEP_TILE - 1) / DEP_TILE);
   b += 2 * keys * (2 * (size_t)std::max(F, 1) * DEP_WIN2 * 8 + 4);
   return b + 64 * 4096;
@@ -570,7 +580,8 @@ void Job::upload(const Basis *b, hipStream_t s) {
   arena.reserve(d_small, small_nets.size());
   arena.reserve(d_large, large_nets.size());
   arena.reserve(d_u_pstart, U + 1);
-  arena.reserve(d_u_pins, u_pins.size());
+  arena.reserve(d_pin_slot, n_pins_total);
+  arena.reserve(d_u_nbase, U);
   arena.reserve(d_wl_diag, U);
   arena.reserve(d_unit_du, U);
   arena.reserve(d_lower, C);
@@ -593,7 +604,7 @@ void Job::upload(const Basis *b, hipStream_t s) {
     arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
     arena.reserve(sl.tile_flag, (size_t)n_keys);
     arena.reserve(sl.amax, 4 * MAXF);
-    arena.reserve(sl.dpin, n_pins_total);
+    arena.reserve(sl.dpin, u_pins.size());
     arena.reserve(sl.part_wa, (size_t)nb_wa_small + nb_wa_large);
     arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
     arena.reserve(sl.part_gf, (size_t)nb_gf * GF_STRIDE);
@@ -608,6 +619,7 @@ void Job::upload(const Basis *b, hipStream_t s) {
   arena.reserve(d_wa_grad, 2 * U);
   arena.reserve(d_eval_coords, C);
   arena.reserve(d_perm, M + Fl);
+  arena.reserve(d_perm_next, M + Fl);
   arena.reserve(d_s_ci, M + Fl);
   arena.reserve(d_s_off, M + Fl);
   arena.reserve(d_s_q, M + Fl);
@@ -647,7 +659,8 @@ void Job::upload(const Basis *b, hipStream_t s) {
   up(d_small, small_nets.data(), small_nets.size() * 4);
   up(d_large, large_nets.data(), large_nets.size() * 4);
   up(d_u_pstart, u_pstart.data(), (U + 1) * 4);
-  up(d_u_pins, u_pins.data(), u_pins.size() * 4);
+  up(d_pin_slot, pin_slot.data(), n_pins_total * 4);
+  up(d_u_nbase, u_nbase.data(), U * 4);
   up(d_wl_diag, wl_diag.data(), U * 8);
   up(d_unit_du, unit_du.data(), U * 4);
   up(d_lower, lower.data(), C * 8);
@@ -732,8 +745,8 @@ void Job::upload(const Basis *b, hipStream_t s) {
   js.pin_unit = d_pin_unit;
   js.pin_xy = d_pin_xy;
   js.u_pstart = d_u_pstart;
-  js.u_pins = d_u_pins;
-  js.first_arc_pin = (uint32_t)first_arc_pin;
+  js.pin_slot = d_pin_slot;
+  js.u_nbase = d_u_nbase;
   for (int j = 0; j < 4; ++j) {
     js.b32[j] = b->b32[j];
     js.bthi[j] = b->bthi[j];
@@ -807,12 +820,16 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   return t;
 }
 
-SortTask Job::sort_task() const {
+SortTask Job::sort_task() {
   SortTask t{};
   t.j = js;
   t.coords = d_vec[ix];
   t.count = d_sort_count;
-  t.perm = d_perm;
+  t.order = d_perm;
+  t.perm = d_perm_next;
+  // work enqueued after the sort reads the new order
+  std::swap(d_perm, d_perm_next);
+  js.perm = d_perm;
   t.s_ci = d_s_ci;
   t.s_off = d_s_off;
   t.s_q = d_s_q;

@@ -230,13 +230,51 @@ __global__ void __launch_bounds__(BLOCK) k_sort_clear(const SortTask *__restrict
   if (i <= t.j.n_keys) t.count[i] = 0u;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_sort_count(const SortTask *__restrict__ tasks,
-                                                      BlockPlan plan) {
+// Charges are walked in the previous deposit order, so the charges of one
+// block mostly share a few tiles: the block counts them in an LDS table
+// spanning its key range and adds each nonzero count to HBM once (one
+// charge at a time when the range is too wide).
+constexpr int SORT_LDS_KEYS = 1024;
+
+__device__ __forceinline__ void block_key_range(int key, bool valid, int &kmin, int &kmax) {
+  __shared__ int s_lo[BLOCK / 64], s_hi[BLOCK / 64];
+  int lo = valid ? key : 0x7fffffff, hi = valid ? key : -1;
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off, 64));
+    hi = max(hi, __shfl_xor(hi, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_lo[wave] = lo;
+    s_hi[wave] = hi;
+  }
+  __syncthreads();
+  kmin = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+  kmax = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sort_count(const SortTask *__restrict__ tasks, BlockPlan plan) {
+  __shared__ uint32_t cnt[SORT_LDS_KEYS];
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const SortTask &t = tasks[bm.x];
   const int i = bm.y * BLOCK + threadIdx.x;
-  if (i < t.j.n_members + t.j.n_fillers) atomicAdd(&t.count[charge_key(t.j, t.coords, i)], 1u);
+  const bool valid = i < t.j.n_members + t.j.n_fillers;
+  const int key = valid ? charge_key(t.j, t.coords, (int)t.order[i]) : 0;
+  int kmin, kmax;
+  block_key_range(key, valid, kmin, kmax);
+  if (kmax < 0) return;
+  if (kmax - kmin >= SORT_LDS_KEYS) {
+    if (valid) atomicAdd(&t.count[key], 1u);
+    return;
+  }
+  const int span = kmax - kmin + 1;
+  for (int k = threadIdx.x; k < span; k += BLOCK) cnt[k] = 0u;
+  __syncthreads();
+  if (valid) atomicAdd(&cnt[key - kmin], 1u);
+  __syncthreads();
+  for (int k = threadIdx.x; k < span; k += BLOCK)
+    if (cnt[k]) atomicAdd(&t.count[kmin + k], cnt[k]);
 }
 
 // Exclusive scan of the key counts; one block per task.
@@ -268,28 +306,48 @@ __global__ void __launch_bounds__(BLOCK) k_sort_scan(const SortTask *__restrict_
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_sort_scatter(const SortTask *__restrict__ tasks,
-                                                        BlockPlan plan) {
+// Each block reserves one range per key it holds (one HBM atomic per key),
+// then places its charges at LDS-assigned ranks inside the range.
+__global__ void __launch_bounds__(BLOCK) k_sort_scatter(const SortTask *__restrict__ tasks, BlockPlan plan) {
+  __shared__ uint32_t cnt[SORT_LDS_KEYS], base[SORT_LDS_KEYS];
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const SortTask &t = tasks[bm.x];
+  const JobStatic &a = t.j;
   const int i = bm.y * BLOCK + threadIdx.x;
-  if (i < t.j.n_members + t.j.n_fillers) {
-    const JobStatic &a = t.j;
-    const uint32_t pos = atomicAdd(&t.count[charge_key(a, t.coords, i)], 1u);
-    t.perm[pos] = static_cast<uint32_t>(i);
-    if (i < a.n_members) {
-      t.s_ci[pos] = a.m_unit[i];
-      t.s_off[pos] = a.m_off[i];
-      t.s_q[pos] = a.m_charge[i];
-      t.s_layer[pos] = static_cast<uint8_t>(2 * a.m_field[i]);
-    } else {
-      const int k = i - a.n_members;
-      t.s_ci[pos] = static_cast<uint32_t>(a.n_units + k);
-      t.s_off[pos] = make_double2(0.0, 0.0);
-      t.s_q[pos] = a.fl_charge[k];
-      t.s_layer[pos] = static_cast<uint8_t>(2 * a.fl_field[k] + 1);
-    }
+  const bool valid = i < a.n_members + a.n_fillers;
+  const int c = valid ? (int)t.order[i] : 0;
+  const int key = valid ? charge_key(a, t.coords, c) : 0;
+  int kmin, kmax;
+  block_key_range(key, valid, kmin, kmax);
+  if (kmax < 0) return;
+  uint32_t pos = 0;
+  if (kmax - kmin >= SORT_LDS_KEYS) {
+    if (valid) pos = atomicAdd(&t.count[key], 1u);
+  } else {
+    const int span = kmax - kmin + 1;
+    for (int k = threadIdx.x; k < span; k += BLOCK) cnt[k] = 0u;
+    __syncthreads();
+    const uint32_t rank = valid ? atomicAdd(&cnt[key - kmin], 1u) : 0u;
+    __syncthreads();
+    for (int k = threadIdx.x; k < span; k += BLOCK)
+      if (cnt[k]) base[k] = atomicAdd(&t.count[kmin + k], cnt[k]);
+    __syncthreads();
+    pos = valid ? base[key - kmin] + rank : 0u;
+  }
+  if (!valid) return;
+  t.perm[pos] = static_cast<uint32_t>(c);
+  if (c < a.n_members) {
+    t.s_ci[pos] = a.m_unit[c];
+    t.s_off[pos] = a.m_off[c];
+    t.s_q[pos] = a.m_charge[c];
+    t.s_layer[pos] = static_cast<uint8_t>(2 * a.m_field[c]);
+  } else {
+    const int k = c - a.n_members;
+    t.s_ci[pos] = static_cast<uint32_t>(a.n_units + k);
+    t.s_off[pos] = make_double2(0.0, 0.0);
+    t.s_q[pos] = a.fl_charge[k];
+    t.s_layer[pos] = static_cast<uint8_t>(2 * a.fl_field[k] + 1);
   }
 }
 
@@ -568,14 +626,34 @@ __device__ __forceinline__ double wa_fixed_net(const JobStatic &a, const Task &t
     ymin = fmin(ymin, c[k].y);
     ymax = fmax(ymax, c[k].y);
   }
+  // the four exponentials of each pin are computed once and reused by the
+  // derivative pass (the values are the ones wa_derivative would recompute)
+  double ex[D], enx[D], ey[D], eny[D];
   WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < D; ++k) wa_accumulate(c[k], xmin, xmax, ymin, ymax, gamma, s);
+  for (int k = 0; k < D; ++k) {
+    const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
+    ex[k] = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
+    enx[k] = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
+    ey[k] = dym == 0.0 ? 1.0 : exp(dym / gamma);
+    eny[k] = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
+    s.psx += ex[k];
+    s.pdx += ex[k] * dxm;
+    s.nsx += enx[k];
+    s.ndx += enx[k] * dxn;
+    s.psy += ey[k];
+    s.pdy += ey[k] * dym;
+    s.nsy += eny[k];
+    s.ndy += eny[k] * dyn;
+  }
   const double pmx = s.pdx / s.psx, nmx = s.ndx / s.nsx, pmy = s.pdy / s.psy, nmy = s.ndy / s.nsy;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    const double2 d = wa_derivative(c[k], xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
-    t.dpin[b + k] = make_double2(w * d.x, w * d.y);
+    const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
+    const double ddx = (ex[k] / s.psx) * (1.0 + (dxm - pmx) / gamma) - (enx[k] / s.nsx) * (1.0 - (dxn - nmx) / gamma);
+    const double ddy = (ey[k] / s.psy) * (1.0 + (dym - pmy) / gamma) - (eny[k] / s.nsy) * (1.0 - (dyn - nmy) / gamma);
+    const uint32_t slot = a.pin_slot[b + k];
+    if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * ddx, w * ddy);
   }
   return w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
 }
@@ -618,7 +696,8 @@ __device__ double wa_thread_net(const Task &t, int local) {
   for (uint32_t p = b; p < e; ++p) {
     const double2 d =
         wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
-    t.dpin[p] = make_double2(w * d.x, w * d.y);
+    const uint32_t slot = a.pin_slot[p];
+    if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * d.x, w * d.y);
   }
   return w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
 }
@@ -659,7 +738,8 @@ __device__ double wa_wave_net(const Task &t, int local) {
   for (uint32_t p = b + lane; p < e; p += 64) {
     const double2 d =
         wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
-    t.dpin[p] = make_double2(w * d.x, w * d.y);
+    const uint32_t slot = a.pin_slot[p];
+    if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * d.x, w * d.y);
   }
   return lane == 0 ? w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy) : 0.0;
 }
@@ -695,16 +775,16 @@ __global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict_
   bool bad = false;
   if (u < a.n_units) {
     double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
-    for (uint32_t k = a.u_pstart[u]; k < a.u_pstart[u + 1]; ++k) {
-      const uint32_t p = a.u_pins[k];
-      const double2 d = t.dpin[p];
-      if (p < a.first_arc_pin) {
-        sx += d.x;
-        sy += d.y;
-      } else {
-        tx += d.x;
-        ty += d.y;
-      }
+    const uint32_t pb = a.u_pstart[u], pe = a.u_pstart[u + 1], pm = pb + a.u_nbase[u];
+    for (uint32_t k = pb; k < pm; ++k) {  // net pins
+      const double2 d = t.dpin[k];
+      sx += d.x;
+      sy += d.y;
+    }
+    for (uint32_t k = pm; k < pe; ++k) {  // timed-arc pins
+      const double2 d = t.dpin[k];
+      tx += d.x;
+      ty += d.y;
     }
     double gx = sx + tx, gy = sy + ty;
     red[0] = fabs(gx) + fabs(gy);
