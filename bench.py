@@ -138,22 +138,35 @@ def run_batched(problems, precision, steps, warmup, timing_kernel):
     return out, sizes
 
 
-def run_serving(stream, precision, max_batch, at_rate: bool):
-    """Submits the stream to a server (at its arrival times, or all at once)
-    and waits for every job.  Latency = submission to completion."""
+def run_serving(stream, precision, max_batch, at_rate: bool, clients: int):
+    """Submits the stream to a server from `clients` threads (job i from
+    thread i % clients, at its arrival time or as soon as the thread can) and
+    waits for every job.  Each submission loads and validates its problem on
+    the submitting thread, as a serving front end would.  Latency =
+    submission to completion."""
+    import threading
     import texo_amd
     from texo_amd import abi
     with texo_amd.Server(max_batch=max_batch) as srv:
+        ids = [None] * len(stream)
         t_start = time.perf_counter()
-        ids = []
-        for arrival, p in stream:
-            if at_rate:
-                while True:
-                    ahead = arrival - (time.perf_counter() - t_start)
-                    if ahead <= 0:
-                        break
-                    time.sleep(min(ahead, 0.002))
-            ids.append(srv.submit(p, abi.options(dct_precision=precision)))
+
+        def client(k):
+            for i in range(k, len(stream), clients):
+                arrival, p = stream[i]
+                if at_rate:
+                    while True:
+                        ahead = arrival - (time.perf_counter() - t_start)
+                        if ahead <= 0:
+                            break
+                        time.sleep(min(ahead, 0.002))
+                ids[i] = srv.submit(p, abi.options(dct_precision=precision))
+
+        threads = [threading.Thread(target=client, args=(k,)) for k in range(clients)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
         lat, failed = [], 0
         for i in ids:
             _, rep = srv.wait(i, raise_on_error=False)
@@ -162,9 +175,10 @@ def run_serving(stream, precision, max_batch, at_rate: bool):
         makespan = time.perf_counter() - t_start
         stats = srv.stats()
     lat = np.array(lat)
-    return {"jobs": len(stream), "jobs_per_s": len(stream) / makespan, "makespan_s": makespan,
+    return {"jobs": len(stream), "clients": clients, "jobs_per_s": len(stream) / makespan, "makespan_s": makespan,
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "failed": int(failed), "rounds": stats["rounds"], "busy_s": stats["busy_seconds"]}
+            "failed": int(failed), "rounds": stats["rounds"], "busy_s": stats["busy_seconds"],
+            "admit_s": stats["admit_seconds"], "retire_s": stats["retire_seconds"]}
 
 
 def main():
@@ -177,6 +191,7 @@ def main():
     ap.add_argument("--roofline-kernel", default="deposit", choices=["deposit", "wa", "gather", "dct"])
     ap.add_argument("--serving-jobs", type=int, default=256)
     ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-clients", type=int, default=4, help="threads submitting the serving stream")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -239,12 +254,12 @@ def main():
         stream = synth.serving_stream(args.serving_jobs, rate=args.serving_rate)
         serving = {"jobs": args.serving_jobs, "arrival_rate_jobs_per_s": args.serving_rate,
                    "cells": "log-uniform 1k-84k, grid side nextpow2(sqrt(cells/4)) in 64-256, "
-                            "weight exponents {2,4,6}"}
+                            "weight exponents {2,4,6}", "clients": args.serving_clients}
         for name, precision, max_batch, at_rate in (("batched_mfma_at_rate", mfma, 256, True),
                                                     ("batched_mfma_saturated", mfma, 256, False),
                                                     ("one_at_a_time_fp32_at_rate", abi.DCT_FP32, 1, True),
                                                     ("one_at_a_time_fp32_saturated", abi.DCT_FP32, 1, False)):
-            serving[name] = run_serving(stream, precision, max_batch, at_rate)
+            serving[name] = run_serving(stream, precision, max_batch, at_rate, args.serving_clients)
             progress(f"serving {name}: {serving[name]}")
         serving["saturated_jobs_per_s_ratio"] = (serving["batched_mfma_saturated"]["jobs_per_s"]
                                                  / serving["one_at_a_time_fp32_saturated"]["jobs_per_s"])

@@ -370,6 +370,8 @@ typedef struct texo_gp_server_stats {
   uint64_t active, pending; /* jobs on the GPU / waiting for admission */
   uint64_t hbm_in_use;      /* bytes held by admitted jobs */
   double busy_seconds;      /* wall time spent in rounds */
+  double admit_seconds;     /* server-thread time admitting jobs (arena, upload enqueue) */
+  double retire_seconds;    /* server-thread time reading back and releasing finished jobs */
 } texo_gp_server_stats;
 
 texo_gp_server *texo_gp_server_create(int32_t device, uint64_t hbm_budget_bytes, uint32_t max_batch);

@@ -405,6 +405,7 @@ class texo_gp_server_stats(C.Structure):
         ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
         ("failed", C.c_uint64), ("rounds", C.c_uint64), ("active", C.c_uint64),
         ("pending", C.c_uint64), ("hbm_in_use", C.c_uint64), ("busy_seconds", C.c_double),
+        ("admit_seconds", C.c_double), ("retire_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -32,8 +32,9 @@ void put_failure(texo_gp_report *r, const Failure &f) {
   r->err_value[1] = f.v1;
 }
 
-// Final report of a job (status, counts, HPWL and field overflows).
-void fill_report(Engine &e, Job &j, texo_gp_report *r) {
+// Final report of a job (status, counts, field overflows; HPWL of `x` when
+// given).
+void fill_report_at(Job &j, const std::vector<double> *x, texo_gp_report *r) {
   if (!r) return;
   r->status = j.phase == Phase::Failed ? j.status : 0;
   if (j.phase == Phase::Failed) put_failure(r, j.failure);
@@ -45,6 +46,17 @@ void fill_report(Engine &e, Job &j, texo_gp_report *r) {
   r->fixed_iterations = j.opt.fixed_iterations;
   r->gamma_base = j.opt.gamma_base;
   r->final_wirelength = j.last_wa;
+  if (x) r->final_hpwl = j.hpwl(*x);
+  r->n_fields = (uint32_t)j.F;
+  for (int f = 0; f < j.F && f < TEXO_GP_MAX_FIELDS; ++f) {
+    r->field_kind[f] = j.field_kind[f];
+    r->field_overflow[f] = j.started ? j.overflow(j.current, f) : 0.0;
+    r->field_energy[f] = j.current.energy[f];
+  }
+}
+
+void fill_report(Engine &e, Job &j, texo_gp_report *r) {
+  if (!r) return;
   std::vector<double> x(2 * j.U);
   if (j.U) {
     if (j.empty) {
@@ -54,13 +66,7 @@ void fill_report(Engine &e, Job &j, texo_gp_report *r) {
       HIP(hipStreamSynchronize(e.stream()));
     }
   }
-  r->final_hpwl = j.hpwl(x);
-  r->n_fields = (uint32_t)j.F;
-  for (int f = 0; f < j.F && f < TEXO_GP_MAX_FIELDS; ++f) {
-    r->field_kind[f] = j.field_kind[f];
-    r->field_overflow[f] = j.started ? j.overflow(j.current, f) : 0.0;
-    r->field_energy[f] = j.current.energy[f];
-  }
+  fill_report_at(j, &x, r);
 }
 
 // Targets of a finished job: the best checkpoint (eplace.rs:553), or the
@@ -436,7 +442,7 @@ int64_t texo_gp_batch_add(texo_gp_batch *b, const texo_gp_problem *problem, cons
   if (!b) return -(int64_t)TEXO_GP_ERR_INVALID_ARGUMENT;
   std::unique_ptr<Job> j(new Job());
   try {
-    j->load(problem, options);
+    j->load(problem, options, b->engine->image_pool());
     if (j->opt.device != b->engine->device()) texo::fail(TEXO_GP_ERR_INVALID_ARGUMENT);
     b->engine->attach(j.get());
   } catch (const Failure &f) {
@@ -540,12 +546,21 @@ void texo_gp_batch_destroy(texo_gp_batch *b) { delete b; }
 // ===========================================================================
 struct texo_gp_server {
   struct Entry {
-    std::unique_ptr<Job> job;
+    std::unique_ptr<Job> job;  // host part kept until the result is taken
     std::chrono::steady_clock::time_point submitted;
     bool done = false;
     int status = 0;
     std::vector<uint32_t> targets;
+    std::vector<double> x;  // final coordinates, for the report's HPWL
     texo_gp_report report{};
+    void *readback = nullptr;  // pinned landing block of the result copies
+    size_t readback_cap = 0;
+  };
+  // Result copies of the jobs one retire() found finished, enqueued on the
+  // compute stream after their last work; `ev` marks their landing.
+  struct Readback {
+    hipEvent_t ev;
+    std::vector<int64_t> ids;
   };
   std::unique_ptr<Engine> engine;
   uint64_t budget = 0;
@@ -561,9 +576,17 @@ struct texo_gp_server {
   size_t charges_in_flight = 0;
   texo_gp_server_stats stats{};
   std::thread worker;
+  std::deque<Readback> readbacks;  // server thread only
+  std::vector<hipEvent_t> spare_events;
 
   void loop();
   void retire(std::vector<int64_t> &group);
+  void collect(bool block);
+  ~texo_gp_server() {
+    for (auto &r : readbacks)
+      if (r.ev) (void)hipEventDestroy(r.ev);
+    for (hipEvent_t ev : spare_events) (void)hipEventDestroy(ev);
+  }
 };
 
 void texo_gp_server::loop() {
@@ -573,13 +596,20 @@ void texo_gp_server::loop() {
   std::vector<int64_t> group[2];
   bool live[2] = {false, false};
   for (;;) {
+    collect(false);
     {
       std::unique_lock<std::mutex> lk(mu);
       cv_work.wait(lk, [&] { return stopping || !pending.empty() || !active.empty(); });
       if (stopping) {
         lk.unlock();
         for (int c = 0; c < 2; ++c)
-          if (live[c]) engine->finish(c);
+          if (live[c]) {
+            try {
+              engine->finish(c);
+            } catch (const Failure &) {
+            }
+          }
+        collect(true);
         return;
       }
       // admission in submission order, by HBM footprint and batch size
@@ -590,6 +620,7 @@ void texo_gp_server::loop() {
         const int64_t id = pending.front();
         pending.pop_front();
         lk.unlock();
+        const auto t0 = std::chrono::steady_clock::now();
         try {
           engine->attach(e.job.get());
           e.job->begin_run(false);
@@ -599,6 +630,7 @@ void texo_gp_server::loop() {
           e.job->phase = Phase::Failed;
         }
         lk.lock();
+        stats.admit_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         active.push_back(id);
         charges_in_flight += e.job->M + e.job->Fl;
         const bool split = charges_in_flight >= texo::PIPELINE_MIN_CHARGES;
@@ -647,12 +679,20 @@ void texo_gp_server::loop() {
       }
       if (!live[c]) retire(group[c]);
     }
+    // nothing in flight to overlap with: wait for the pending results
+    if (!live[0] && !live[1]) collect(true);
   }
 }
 
-// Moves finished jobs of `group` out of the active set with their results.
+// Takes the finished jobs out of `group` and enqueues their result copies
+// on the compute stream, after the work the last round's advance left there
+// for them; collect() completes them once the copies have landed, so the
+// host never waits here for the round in flight.
 void texo_gp_server::retire(std::vector<int64_t> &group) {
+  const auto t0 = std::chrono::steady_clock::now();
   std::vector<int64_t> still;
+  Readback rb{nullptr, {}};
+  hipStream_t st = nullptr;
   for (int64_t id : group) {
     Entry *ep;
     {
@@ -665,34 +705,125 @@ void texo_gp_server::retire(std::vector<int64_t> &group) {
       still.push_back(id);
       continue;
     }
-    e.targets.assign(2 * j.U, 0);
-    try {
-      fill_report(*engine, j, &e.report);
-      if (j.phase == Phase::Done) copy_targets(*engine, j, e.targets.data());
-    } catch (const Failure &f) {
-      put_failure(&e.report, f);
-      j.status = f.status;
-      j.phase = Phase::Failed;
+    if (!j.empty && j.arena.base() && j.U) {
+      try {
+        if (!st) st = engine->stream();
+        e.readback = engine->image_pool()->acquire(j.U * 24, &e.readback_cap);
+        char *dst = static_cast<char *>(e.readback);
+        HIP(hipMemcpyAsync(dst, j.d_vec[j.ix], j.U * 16, hipMemcpyDeviceToHost, st));
+        if (j.phase == Phase::Done) HIP(hipMemcpyAsync(dst + j.U * 16, j.d_best, j.U * 8, hipMemcpyDeviceToHost, st));
+      } catch (const Failure &f) {
+        j.failure = f;
+        j.status = f.status;
+        j.phase = Phase::Failed;
+      }
     }
-    e.status = j.phase == Phase::Failed ? j.status : 0;
-    e.report.status = e.status;
-    e.report.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
-    const uint64_t held = j.device_bytes();
-    const size_t charges = j.M + j.Fl;
-    engine->detach(&j);
-    e.job.reset();  // frees the job's HBM arena
-    std::lock_guard<std::mutex> lk(mu);
-    hbm_in_use -= std::min(hbm_in_use, held);
-    charges_in_flight -= std::min(charges_in_flight, charges);
-    e.done = true;
-    if (e.status == 0)
-      stats.completed += 1;
-    else
-      stats.failed += 1;
-    active.erase(std::find(active.begin(), active.end(), id));
+    rb.ids.push_back(id);
   }
   group.swap(still);
+  if (!rb.ids.empty()) {
+    hipEvent_t ev = nullptr;
+    if (!spare_events.empty()) {
+      ev = spare_events.back();
+      spare_events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      ev = nullptr;
+    }
+    if (ev && hipEventRecord(ev, engine->stream()) != hipSuccess) {
+      (void)hipEventDestroy(ev);
+      ev = nullptr;
+    }
+    if (!ev) (void)hipStreamSynchronize(engine->stream());  // no marker: land them now
+    rb.ev = ev;
+    readbacks.push_back(std::move(rb));
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  stats.retire_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Completes the retired jobs whose result copies have landed (all of them
+// when `block`): results into the entries, device memory back to the pools.
+// The HPWL of the report and the release of the job's host data happen on
+// the thread that takes the result.
+void texo_gp_server::collect(bool block) {
+  const auto t0 = std::chrono::steady_clock::now();
+  bool any = false;
+  while (!readbacks.empty()) {
+    Readback &rb = readbacks.front();
+    const hipError_t q = !rb.ev ? hipSuccess : block ? hipEventSynchronize(rb.ev) : hipEventQuery(rb.ev);
+    if (q == hipErrorNotReady) break;
+    for (int64_t id : rb.ids) {
+      Entry *ep;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        ep = entries[id].get();
+      }
+      Entry &e = *ep;
+      Job &j = *e.job;
+      e.targets.assign(2 * j.U, 0);
+      if (q != hipSuccess && j.phase == Phase::Done) {
+        j.status = TEXO_GP_ERR_DEVICE;
+        j.phase = Phase::Failed;
+      }
+      if (j.empty || !e.readback) {
+        e.x = j.initial_xy;
+        if (j.empty && j.phase == Phase::Done)
+          for (size_t i = 0; i < 2 * j.U; ++i)
+            e.targets[i] = texo::rounded_coordinate(j.initial_xy[i], (i & 1) ? j.H : j.W);
+      } else {
+        const char *src = static_cast<const char *>(e.readback);
+        e.x.resize(2 * j.U);
+        std::memcpy(e.x.data(), src, j.U * 16);
+        if (j.phase == Phase::Done) std::memcpy(e.targets.data(), src + j.U * 16, j.U * 8);
+        engine->image_pool()->release(e.readback, e.readback_cap);
+        e.readback = nullptr;
+      }
+      fill_report_at(j, nullptr, &e.report);
+      e.status = j.phase == Phase::Failed ? j.status : 0;
+      e.report.status = e.status;
+      e.report.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
+      const uint64_t held = j.device_bytes();
+      const size_t charges = j.M + j.Fl;
+      engine->detach(&j);
+      j.arena.release();
+      j.release_image();
+      std::lock_guard<std::mutex> lk(mu);
+      hbm_in_use -= std::min(hbm_in_use, held);
+      charges_in_flight -= std::min(charges_in_flight, charges);
+      if (e.status == 0)
+        stats.completed += 1;
+      else
+        stats.failed += 1;
+      active.erase(std::find(active.begin(), active.end(), id));
+      e.done = true;
+    }
+    if (rb.ev) spare_events.push_back(rb.ev);
+    readbacks.pop_front();
+    any = true;
+  }
+  if (!any) return;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    stats.retire_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
   cv_done.notify_all();
+}
+
+// Moves a finished entry out of the table and fills the caller's buffers;
+// the HPWL and the release of the job's host data run outside the lock.
+static int take_result(texo_gp_server *s, std::unique_lock<std::mutex> &lk, int64_t id, uint32_t *out,
+                       texo_gp_report *report) {
+  auto it = s->entries.find(id);
+  std::unique_ptr<texo_gp_server::Entry> e = std::move(it->second);
+  s->entries.erase(it);
+  lk.unlock();
+  if (e->job) {
+    e->report.final_hpwl = e->job->hpwl(e->x);
+    e->job.reset();
+  }
+  if (out) std::copy(e->targets.begin(), e->targets.end(), out);
+  if (report) *report = e->report;
+  return e->status;
 }
 
 extern "C" {
@@ -716,7 +847,8 @@ int64_t texo_gp_server_submit(texo_gp_server *s, const texo_gp_problem *problem,
   e->submitted = std::chrono::steady_clock::now();
   e->job.reset(new Job());
   try {
-    e->job->load(problem, options);  // deep copy + validation on the caller's thread
+    // deep copy, validation and upload image on the caller's thread
+    e->job->load(problem, options, s->engine->image_pool());
   } catch (const Failure &f) {
     return -(int64_t)f.status;
   }
@@ -730,23 +862,13 @@ int64_t texo_gp_server_submit(texo_gp_server *s, const texo_gp_problem *problem,
   return id;
 }
 
-static int take_result(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_report *report) {
-  auto it = s->entries.find(id);
-  texo_gp_server::Entry &e = *it->second;
-  if (out) std::copy(e.targets.begin(), e.targets.end(), out);
-  if (report) *report = e.report;
-  const int status = e.status;
-  s->entries.erase(it);
-  return status;
-}
-
 int texo_gp_server_poll(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_report *report) {
   if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> lk(s->mu);
+  std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->entries.find(id);
   if (it == s->entries.end()) return TEXO_GP_ERR_INVALID_ARGUMENT;
   if (!it->second->done) return TEXO_GP_ERR_NOT_READY;
-  return take_result(s, id, out, report);
+  return take_result(s, lk, id, out, report);
 }
 
 int texo_gp_server_wait(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_report *report) {
@@ -760,7 +882,7 @@ int texo_gp_server_wait(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_re
   });
   // another waiter may have taken the result first
   if (s->entries.find(id) == s->entries.end()) return TEXO_GP_ERR_INVALID_ARGUMENT;
-  return take_result(s, id, out, report);
+  return take_result(s, lk, id, out, report);
 }
 
 int texo_gp_server_stats_get(texo_gp_server *s, texo_gp_server_stats *stats) {

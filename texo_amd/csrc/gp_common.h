@@ -35,6 +35,19 @@ struct BlockPlan {
   const int *start;  // [n_tasks + 1]
   int n_tasks, total, per;
 };
+// WA lane groups: a net of D pins gets the power-of-two number of lanes that
+// gives each lane at most WA_PINS_PER_LANE pins (one lane up to that many);
+// nets of more than WA_LOOP_PINS pins get a wavefront that loops.
+constexpr int WA_PINS_PER_LANE = 4;
+constexpr int WA_LOOP_PINS = 64 * WA_PINS_PER_LANE;
+constexpr int WA_MAX_SEGMENTS = 16;
+__host__ __device__ inline int wa_lanes(uint32_t pins) {
+  if (pins > (uint32_t)WA_LOOP_PINS) return 0;
+  int g = 1;
+  while ((uint32_t)g * WA_PINS_PER_LANE < pins) g <<= 1;
+  return g;
+}
+
 constexpr int XCDS = 8;
 __host__ __device__ inline int plan_grid(const BlockPlan &p) { return p.per > 0 ? XCDS * p.per : p.total; }
 __device__ __forceinline__ bool plan_block_at(const BlockPlan &p, int b, int2 &bm) {
@@ -154,9 +167,11 @@ struct JobStatic {
   const double *s_q;
   const uint8_t *s_layer;
   const double *lower, *upper;
-  // hypergraph: nets then timed arcs as weighted 2-pin nets
-  int n_small, n_large;
-  const uint32_t *small_nets, *large_nets;
+  // hypergraph: nets then timed arcs as weighted 2-pin nets; the nets in
+  // WA segments: {first net, nets, lanes per net (0: one wavefront looping
+  // over a long net), first wavefront}
+  int n_wa_seg;
+  const int4 *wa_seg;
   const uint32_t *net_start;
   const double *net_weight;
   const uint32_t *pin_unit;   // NO_UNIT for fixed pins
@@ -187,7 +202,7 @@ struct Task {
   uint32_t *tile_flag;         // [n_keys] 1 where the tile's window was written
   double2 *dpin;               // [movable pins] weighted WA derivative, grouped by unit
   double *part_wa, *part_energy, *part_gf, *part_ug;
-  int nb_wa_small, nb_wa_large, nb_energy, nb_gf, nb_ug;
+  int nb_wa, nb_energy, nb_gf, nb_ug;
   // request
   const double *coords;
   double *grad;

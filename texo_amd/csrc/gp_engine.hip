@@ -157,8 +157,12 @@ Engine::Engine(int device, size_t pool_capacity) : device_(device) {
   // every stream keeps the default priority
   HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP(hipStreamCreateWithFlags(&wa_stream_, hipStreamNonBlocking));
+  HIP(hipStreamCreateWithFlags(&io_stream_, hipStreamNonBlocking));
   HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+  HIP(hipEventCreateWithFlags(&ev_upload_, hipEventDisableTiming));
+  blocks_ = std::make_shared<BlockPool>(BLOCK_POOL_KEEP, false);
+  images_ = std::make_shared<BlockPool>(IMAGE_POOL_KEEP, true);
   HIP(hipMalloc(&d_pool_, pool_capacity * sizeof(JobScalars)));
   HIP(hipHostMalloc(reinterpret_cast<void **>(&h_pool_), pool_capacity * sizeof(JobScalars), hipHostMallocDefault));
   std::memset(h_pool_, 0, pool_capacity * sizeof(JobScalars));
@@ -171,6 +175,7 @@ Engine::~Engine() {
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
   (void)hipStreamSynchronize(wa_stream_);
+  (void)hipStreamSynchronize(io_stream_);
   bases_.clear();
   if (d_pool_) (void)hipFree(d_pool_);
   if (h_pool_) (void)hipHostFree(h_pool_);
@@ -181,6 +186,8 @@ Engine::~Engine() {
   free_ctx(aux_);
   (void)hipEventDestroy(ev_fork_);
   (void)hipEventDestroy(ev_join_);
+  (void)hipEventDestroy(ev_upload_);
+  (void)hipStreamDestroy(io_stream_);
   (void)hipStreamDestroy(wa_stream_);
   (void)hipStreamDestroy(stream_);
 }
@@ -205,9 +212,17 @@ void Engine::attach(Job *job) {
   job->d_scalars = d_pool_ + idx;
   job->h_scalars = h_pool_ + idx;
   if (!job->empty) {
-    job->upload(basis((int)job->W, (int)job->H), stream_);
-    sort_now(job);
+    // the first round sorts the charges by tile (the probe evaluation)
+    job->upload(basis((int)job->W, (int)job->H), blocks_, io_stream_);
+    HIP(hipEventRecord(ev_upload_, io_stream_));
+    uploads_pending_ = true;
   }
+}
+
+void Engine::await_uploads() {
+  if (!uploads_pending_) return;
+  HIP(hipStreamWaitEvent(stream_, ev_upload_, 0));
+  uploads_pending_ = false;
 }
 
 void Engine::detach(Job *job) {
@@ -253,6 +268,8 @@ void Engine::free_ctx(RoundCtx &c) {
 void Engine::sort_now(Job *j) {
   if (j->F == 0 || j->M + j->Fl == 0) return;
   HIP(hipSetDevice(device_));
+  await_uploads();
+  j->sorted = true;
   std::vector<SortTask> tasks{j->sort_task()};
   Plan pc, pk;
   pc.add(j->M + j->Fl, BLOCK);
@@ -390,6 +407,7 @@ void Engine::plan(Job *j) {
 // propose_area_adjustment / apply_area_adjustment (eplace.rs:627-819): RUDY
 // on the device, the order-dependent charge bookkeeping on the host.
 void Engine::area_round(Job *j) {
+  await_uploads();
   HIP(hipMemsetAsync(j->d_rudy_err, 0, sizeof(unsigned int), stream_));
   launch_rudy((int)j->n_nets, j->d_net_start, j->d_pin_unit, j->d_pin_xy, j->d_vec[j->ix], (int)j->W, (int)j->H,
               j->d_rudy_fx, j->d_rudy, j->d_rudy_err, stream_);
@@ -464,7 +482,7 @@ void Engine::area_round(Job *j) {
     return;
   }
   if (j->area_round >= j->round_limit) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
-  j->apply_charges(ma, fa);
+  j->apply_charges(ma, fa, stream_);
   j->p_valid = false;
   j->has_prev_change = true;
   j->prev_change = total_inc;
@@ -780,6 +798,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   std::map<std::tuple<int, int, int>, Group> groups;
   for (Job *j : active) {
     if (j->want_sort && j->F > 0 && j->M + j->Fl > 0) {
+      j->sorted = true;
       l_sort.push_back((int)sorts.size());
       sorts.push_back(j->sort_task());
       p_sort_c.add(j->M + j->Fl, BLOCK);
@@ -864,7 +883,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
         p_gf.add((size_t)j->nb_gf, 1);
       else
         p_gf.none();
-      p_wa.add((size_t)(j->nb_wa_large + j->nb_wa_small), 1);
+      p_wa.add((size_t)j->nb_wa, 1);
       p_ug.add((size_t)j->nb_ug, 1);
     }
   }
@@ -885,6 +904,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   ensure_stage(cx, buf.size());
   ensure_compact(cx, active.size());
   std::memcpy(cx.h_stage, buf.data(), buf.size());
+  await_uploads();
   HIP(hipMemcpyAsync(cx.d_stage, cx.h_stage, buf.size(), hipMemcpyHostToDevice, stream_));
   char *const d_stage_ = cx.d_stage;
   auto dptr = [&](size_t off) { return d_stage_ + off; };
@@ -959,6 +979,7 @@ void Engine::finish(int c) {
   g_timer = &timer;
   for (size_t k = 0; k < cx.active.size(); ++k) {
     Job *j = cx.active[k];
+    j->release_image();  // the round ran after the job's upload
     j->h_scalars = cx.h_compact + k;
     try {
       advance(j);
@@ -973,6 +994,8 @@ void Engine::finish(int c) {
 
 void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_out) {
   HIP(hipSetDevice(device_));
+  if (!j->sorted) sort_now(j);
+  await_uploads();
   HIP(hipMemcpyAsync(j->d_eval_coords, coords, j->C * sizeof(double), hipMemcpyHostToDevice, stream_));
   // slot 0 is overwritten: the optimizer's cached field there is stale now
   if (j->p_slot == 0) j->p_valid = false;
@@ -1008,7 +1031,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
     p_sample.none();
     p_gf.none();
   }
-  p_wa.add((size_t)(j->nb_wa_large + j->nb_wa_small), 1);
+  p_wa.add((size_t)j->nb_wa, 1);
   p_ug.add((size_t)j->nb_ug, 1);
   std::vector<char> buf;
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -41,25 +42,54 @@ struct KernelTimer {
   ~KernelTimer();
 };
 
+// Blocks kept for reuse once a job ends: device blocks for the job arenas
+// (hipFree waits for the whole device) and pinned host blocks for the upload
+// images (pinning is slow).  A serving loop that admits and retires jobs
+// continuously then neither allocates nor frees after it has warmed up.  At
+// most `keep_bytes` of unused blocks are held.  Thread-safe.
+class BlockPool {
+ public:
+  BlockPool(size_t keep_bytes, bool pinned_host) : keep_(keep_bytes), host_(pinned_host) {}
+  ~BlockPool();
+  void *acquire(size_t bytes, size_t *capacity);
+  void release(void *p, size_t capacity);
+
+ private:
+  void free_block(void *p);
+  std::mutex mu_;
+  std::multimap<size_t, void *> free_;  // capacity -> block
+  size_t kept_ = 0, keep_;
+  bool host_;
+};
+
 // One allocation carved into 256-byte aligned buffers.
 class Arena {
  public:
-  ~Arena();
+  ~Arena() { release(); }
+  // Plans a buffer of `count` elements; returns its byte offset in the arena.
   template <class T>
-  void reserve(T *&p, size_t count) {
-    plan_.push_back({reinterpret_cast<void **>(&p), std::max<size_t>(count, 1) * sizeof(T)});
+  size_t reserve(T *&p, size_t count) {
+    const size_t at = planned_;
+    plan_.push_back({reinterpret_cast<void **>(&p), at});
+    planned_ += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+    return at;
   }
-  size_t commit();
-  size_t bytes() const { return bytes_; }
+  size_t planned() const { return planned_; }
+  // Allocates the planned bytes (from `pool` when given) and sets every
+  // reserved pointer.
+  void commit(std::shared_ptr<BlockPool> pool = nullptr);
+  void release();  // returns the memory (to its pool); pointers dangle after
+  char *base() const { return static_cast<char *>(base_); }
 
  private:
   struct Entry {
     void **slot;
-    size_t bytes;
+    size_t offset;
   };
   std::vector<Entry> plan_;
+  std::shared_ptr<BlockPool> pool_;
   void *base_ = nullptr;
-  size_t bytes_ = 0;
+  size_t planned_ = 0, capacity_ = 0;
 };
 
 struct Options {
@@ -115,6 +145,11 @@ enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed
 // more than the host time it hides.
 constexpr size_t PIPELINE_MIN_CHARGES = 4u << 20;
 
+// Unused device blocks / pinned image blocks an engine keeps for the jobs it
+// admits next.
+constexpr size_t BLOCK_POOL_KEEP = size_t(16) << 30;
+constexpr size_t IMAGE_POOL_KEEP = size_t(4) << 30;
+
 struct Job {
   // ---- problem (host) ----
   Options opt;
@@ -138,11 +173,24 @@ struct Job {
   std::vector<uint32_t> routing_h, routing_v;
   std::map<uint32_t, Control> controls;
   size_t n_nets = 0, n_arcs = 0, n_net_total = 0, n_pins_total = 0, first_arc_pin = 0;
-  std::vector<uint32_t> net_start, pin_unit, small_nets, large_nets, u_pstart, u_pins, pin_slot, u_nbase;
+  std::vector<uint32_t> net_start, pin_unit, u_pstart, u_pins, pin_slot, u_nbase;
+  std::vector<int4> wa_seg;  // WA segments (JobStatic::wa_seg)
+  int wa_waves = 0;          // wavefronts of one WA evaluation
   std::vector<double> pin_xy, net_weight;
+  std::vector<double> h_du_fcharge;  // charge per field and density unit
+  // Upload image: the bytes of the arena's leading block of arrays that
+  // start with data, built by load() on the submitting thread (in a pinned
+  // block of `image_pool` when given) and copied to the device in one piece.
+  char *image = nullptr;
+  size_t image_bytes = 0, image_cap = 0, zero_begin = 0, zero_end = 0;
+  std::shared_ptr<BlockPool> image_pool;
+  std::unique_ptr<char[]> image_heap;
 
-  void load(const texo_gp_problem *p, const texo_gp_options *o);
-  size_t device_bytes() const;  // arena footprint estimate for admission
+  ~Job() { release_image(); }
+  void load(const texo_gp_problem *p, const texo_gp_options *o,
+            const std::shared_ptr<BlockPool> &image_pool = nullptr);
+  size_t device_bytes() const { return arena.planned(); }  // for admission
+  void release_image();  // once the upload has completed
 
   // ---- device ----
   Arena arena;
@@ -153,7 +201,8 @@ struct Job {
   double2 *d_m_off = nullptr, *d_pin_xy = nullptr;
   double *d_m_charge = nullptr, *d_fl_charge = nullptr, *d_du_fcharge = nullptr;
   uint8_t *d_m_field = nullptr, *d_fl_field = nullptr;
-  uint32_t *d_net_start = nullptr, *d_pin_unit = nullptr, *d_small = nullptr, *d_large = nullptr;
+  uint32_t *d_net_start = nullptr, *d_pin_unit = nullptr;
+  int4 *d_wa_seg = nullptr;
   uint32_t *d_u_pstart = nullptr, *d_pin_slot = nullptr, *d_u_nbase = nullptr;
   double *d_net_weight = nullptr, *d_wl_diag = nullptr;
   int32_t *d_unit_du = nullptr;
@@ -181,10 +230,16 @@ struct Job {
   double *d_s_q = nullptr;
   uint8_t *d_s_layer = nullptr;
   int tiles_x = 0, n_keys = 0;
-  int nb_wa_small = 0, nb_wa_large = 0, nb_energy = 0, nb_gf = 0, nb_ug = 0;
+  int nb_wa = 0, nb_energy = 0, nb_gf = 0, nb_ug = 0;
 
-  void upload(const Basis *b, hipStream_t s);
-  void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas);
+  // Allocates the arena (from `pool` when given) and enqueues the image copy
+  // and the accumulator clear on `s`; the image stays alive until
+  // release_image(), so nothing waits here.
+  void upload(const Basis *b, const std::shared_ptr<BlockPool> &pool, hipStream_t s);
+  // New member / filler charges: host state, then the device copies on `s`
+  // (waited for before returning).
+  void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas, hipStream_t s);
+  bool sorted = false;  // charges ordered by tile at least once
   Task task(int slot, const double *coords, double *grad, const double *metric, double gamma, bool skip,
             EvalScalars *scalars) const;
   SortTask sort_task();  // next sort; the new order becomes the job's order
@@ -254,6 +309,8 @@ struct Job {
   size_t area_round_limit() const;
   void update_multipliers();
   double hpwl(const std::vector<double> &x) const;
+  void stage(const std::shared_ptr<BlockPool> &pool);
+  void set_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas);
   void begin_run(bool iterate_mode);
   void begin_round();
   void count_evaluation();
@@ -265,9 +322,17 @@ class Engine {
   explicit Engine(int device, size_t pool_capacity = 256);
   ~Engine();
   int device() const { return device_; }
-  hipStream_t stream() const { return stream_; }
+  // Compute stream, ordered after every upload enqueued so far.
+  hipStream_t stream() {
+    if (uploads_pending_ && hipStreamWaitEvent(stream_, ev_upload_, 0) == hipSuccess) uploads_pending_ = false;
+    return stream_;
+  }
+  // Stream of uploads and result read-backs: work there waits for no round.
+  hipStream_t io_stream() const { return io_stream_; }
+  // Pinned host blocks for upload images (usable from any thread).
+  const std::shared_ptr<BlockPool> &image_pool() const { return images_; }
   const Basis *basis(int W, int H);
-  void attach(Job *job);  // upload and reserve a scalar slot
+  void attach(Job *job);  // enqueue the upload and reserve a scalar slot
   void detach(Job *job);  // release the slot (job memory is freed by the job)
   void sort_now(Job *job);  // sorts the job's charges by tile at its current coordinates
   // Runs one round for every job in `jobs` whose phase needs the device.
@@ -287,9 +352,12 @@ class Engine {
   void plan(Job *job);
   void advance(Job *job);
   void area_round(Job *job);
+  void await_uploads();  // orders the compute stream after the enqueued uploads
   int device_;
-  hipStream_t stream_ = nullptr, wa_stream_ = nullptr;
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+  hipStream_t stream_ = nullptr, wa_stream_ = nullptr, io_stream_ = nullptr;
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_upload_ = nullptr;
+  bool uploads_pending_ = false;
+  std::shared_ptr<BlockPool> blocks_, images_;
   std::map<std::pair<int, int>, std::unique_ptr<Basis>> bases_;
   JobScalars *d_pool_ = nullptr, *h_pool_ = nullptr;
   std::vector<bool> pool_used_;

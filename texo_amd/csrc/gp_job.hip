@@ -86,46 +86,76 @@ void timer_end(KernelTimer *t, const char *name, hipStream_t s) {
   t->used += 1;
 }
 
-Arena::~Arena() {
-  if (base_) (void)hipFree(base_);
+BlockPool::~BlockPool() {
+  for (auto &b : free_) free_block(b.second);
 }
 
-size_t Arena::commit() {
-  size_t total = 0;
-  for (auto &e : plan_) total += (e.bytes + 255) & ~size_t(255);
-  HIP(hipMalloc(&base_, total));
-  char *c = static_cast<char *>(base_);
-  for (auto &e : plan_) {
-    *e.slot = c;
-    c += (e.bytes + 255) & ~size_t(255);
+void BlockPool::free_block(void *p) {
+  if (host_)
+    (void)hipHostFree(p);
+  else
+    (void)hipFree(p);
+}
+
+// Blocks come in four size classes per power of two (at most 25% unused),
+// and a request takes the smallest free block of at least its class that is
+// no more than twice as large.
+void *BlockPool::acquire(size_t bytes, size_t *capacity) {
+  size_t cls = std::max<size_t>(bytes, 1u << 20);
+  const int top = 63 - __builtin_clzll(cls);
+  const size_t step = size_t(1) << (top - 2);
+  cls = (cls + step - 1) & ~(step - 1);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.lower_bound(cls);
+    if (it != free_.end() && it->first <= 2 * cls) {
+      void *p = it->second;
+      *capacity = it->first;
+      kept_ -= it->first;
+      free_.erase(it);
+      return p;
+    }
   }
-  bytes_ = total;
-  plan_.clear();
-  return total;
+  void *p = nullptr;
+  if (host_)
+    HIP(hipHostMalloc(&p, cls, hipHostMallocDefault));
+  else
+    HIP(hipMalloc(&p, cls));
+  *capacity = cls;
+  return p;
 }
 
-// Quadratic B-spline deposit on the host, used once per job for the smoothed
-// capacity background (electrostatic_placement.rs:173-190, 900-916).
-static void host_deposit(std::vector<double> &grid, size_t W, size_t H, double x, double y, double q) {
-  auto axis = [](size_t L, double c, size_t idx[3], double w[3]) {
-    const double maximum = static_cast<double>(L - 1);
-    c = clampd(c, 0.0, maximum);
-    const double cc = std::min(std::floor(c + 0.5), maximum);
-    const size_t center = static_cast<size_t>(cc);
-    const double o = c - cc;
-    idx[0] = center == 0 ? 0 : center - 1;
-    idx[1] = center;
-    idx[2] = std::min(center + 1, L - 1);
-    w[0] = 0.5 * ((0.5 - o) * (0.5 - o));
-    w[1] = 0.75 - o * o;
-    w[2] = 0.5 * ((0.5 + o) * (0.5 + o));
-  };
-  size_t ix[3], iy[3];
-  double wx[3], wy[3];
-  axis(W, x, ix, wx);
-  axis(H, y, iy, wy);
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) grid[iy[r] * W + ix[c]] += q * wx[c] * wy[r];
+void BlockPool::release(void *p, size_t capacity) {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (kept_ + capacity <= keep_) {
+      free_.emplace(capacity, p);
+      kept_ += capacity;
+      return;
+    }
+  }
+  free_block(p);
+}
+
+void Arena::release() {
+  if (!base_) return;
+  if (pool_)
+    pool_->release(base_, capacity_);
+  else
+    (void)hipFree(base_);
+  base_ = nullptr;
+  pool_.reset();
+}
+
+void Arena::commit(std::shared_ptr<BlockPool> pool) {
+  if (pool) {
+    base_ = pool->acquire(planned_, &capacity_);
+    pool_ = std::move(pool);
+  } else {
+    HIP(hipMalloc(&base_, planned_));
+    capacity_ = planned_;
+  }
+  for (auto &e : plan_) *e.slot = static_cast<char *>(base_) + e.offset;
 }
 
 // Poisson2d::new (electrostatic_placement.rs:675-714) and the orthonormal DCT
@@ -215,7 +245,7 @@ void Basis::build(int w, int h) {
   HIP(hipMemcpy(inv_lambda, il.data(), il.size() * sizeof(float), hipMemcpyHostToDevice));
 }
 
-void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
+void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::shared_ptr<BlockPool> &pool) {
   if (!p || p->abi_version != TEXO_GP_ABI_VERSION) fail(TEXO_GP_ERR_INVALID_ARGUMENT);
   if (o) {
     opt.precision = o->dct_precision;
@@ -241,8 +271,16 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
     if (p->cell_unit[c] >= U) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)c);
 
   // Density units: movable units with exchangeable members (eplace.rs:261-301)
-  std::set<uint32_t> kinds;
+  std::vector<uint8_t> kind_seen(1u << 16, 0);
   unit_du.assign(U, -1);
+  {
+    size_t members = 0;
+    for (size_t u = 0; u < U; ++u)
+      if (!unit_fixed[u]) members += p->unit_member_start[u + 1] - p->unit_member_start[u];
+    m_unit.reserve(members);
+    m_cell.reserve(members);
+    m_off.reserve(2 * members);
+  }
   for (size_t u = 0; u < U; ++u) {
     const uint32_t b = p->unit_member_start[u], e = p->unit_member_start[u + 1];
     if (unit_fixed[u] || b == e) continue;
@@ -251,29 +289,32 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
     du_mstart.push_back(static_cast<uint32_t>(m_unit.size()));
     for (uint32_t m = b; m < e; ++m) {
       if (!exchangeable(p->member_kind[m]) || p->member_cell[m] >= n_cells) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)m);
-      kinds.insert(p->member_kind[m]);
+      kind_seen[p->member_kind[m]] = 1;
       m_unit.push_back(static_cast<uint32_t>(u));
       m_cell.push_back(p->member_cell[m]);
-      m_field.push_back(0);  // filled below
       m_off.push_back(p->member_offset_xy[2 * m]);
       m_off.push_back(p->member_offset_xy[2 * m + 1]);
-      m_charge.push_back(1.0);
     }
   }
   du_mstart.push_back(static_cast<uint32_t>(m_unit.size()));
   DU = du_unit.size();
   M = m_unit.size();
-  field_kind.assign(kinds.begin(), kinds.end());
+  m_charge.assign(M, 1.0);
+  // fields in ascending kind order (the reference's BTreeMap over ResourceKind)
+  for (uint32_t k = 0; k < kind_seen.size(); ++k)
+    if (kind_seen[k]) {
+      kind_seen[k] = static_cast<uint8_t>(field_kind.size());
+      field_kind.push_back(k);
+    }
   F = static_cast<int>(field_kind.size());
   if (F > MAXF) fail(TEXO_GP_ERR_UNSUPPORTED);
+  m_field.resize(M);
   {
     size_t k = 0;
     for (size_t u = 0; u < U; ++u) {
       const uint32_t b = p->unit_member_start[u], e = p->unit_member_start[u + 1];
       if (unit_fixed[u] || b == e) continue;
-      for (uint32_t m = b; m < e; ++m)
-        m_field[k++] = static_cast<uint8_t>(std::find(field_kind.begin(), field_kind.end(), p->member_kind[m]) -
-                                            field_kind.begin());
+      for (uint32_t m = b; m < e; ++m) m_field[k++] = kind_seen[p->member_kind[m]];
     }
   }
   for (size_t c = 0; c < n_cells; ++c)
@@ -333,22 +374,32 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
   wl_diag.assign(U, 0.0);
   std::vector<double> cell_weight(n_cells, 0.0);
   {
-    std::vector<uint32_t> units, cells;
+    // distinct units / cells of a net by stamping them with the net's number:
+    // each distinct one receives the net's weight once, nets in input order
+    std::vector<uint64_t> seen_u(U, UINT64_MAX), seen_c(n_cells, UINT64_MAX);
     for (size_t n = 0; n < n_nets; ++n) {
-      units.clear();
-      cells.clear();
-      for (uint32_t q = net_start[n]; q < net_start[n + 1]; ++q) {
-        units.push_back(p->cell_unit[pin_cell[q]]);
-        cells.push_back(pin_cell[q]);
-      }
+      const uint64_t first = 2 * (uint64_t)n, second = first + 1;
       const double w = 1.0 / (double)(uint32_t)(net_start[n + 1] - net_start[n] - 1);
-      std::sort(units.begin(), units.end());
-      units.erase(std::unique(units.begin(), units.end()), units.end());
-      if (units.size() >= 2)
-        for (uint32_t u : units) wl_diag[u] += w;
-      std::sort(cells.begin(), cells.end());
-      cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
-      for (uint32_t c : cells) cell_weight[c] += w;
+      uint32_t distinct = 0;
+      for (uint32_t q = net_start[n]; q < net_start[n + 1]; ++q) {
+        const uint32_t c = pin_cell[q], u = p->cell_unit[c];
+        if (seen_u[u] != first) {
+          seen_u[u] = first;
+          distinct += 1;
+        }
+        if (seen_c[c] != first) {
+          seen_c[c] = first;
+          cell_weight[c] += w;
+        }
+      }
+      if (distinct >= 2)
+        for (uint32_t q = net_start[n]; q < net_start[n + 1]; ++q) {
+          const uint32_t u = p->cell_unit[pin_cell[q]];
+          if (seen_u[u] == first) {
+            seen_u[u] = second;
+            wl_diag[u] += w;
+          }
+        }
     }
     for (size_t a = 0; a < n_arcs; ++a) {
       const uint32_t du_ = p->cell_unit[p->arc_cells[2 * a]], su = p->cell_unit[p->arc_cells[2 * a + 1]];
@@ -364,15 +415,27 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
   // their nets; each unit still lists its pins in the input net order, which
   // fixes the summation order of its gradient.
   {
-    std::vector<uint32_t> order(n_nets);
-    for (size_t n = 0; n < n_nets; ++n) order[n] = (uint32_t)n;
     auto degree = [&](uint32_t n) { return net_start[n + 1] - net_start[n]; };
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-      const uint32_t da = degree(a), db = degree(b);
-      const bool la = da > LARGE_NET_PINS, lb = db > LARGE_NET_PINS;
-      if (la != lb) return lb;
-      return la ? da > db : da < db;
-    });
+    // stable counting sort of the small nets by degree, then the large nets
+    // stably by descending degree
+    std::vector<uint32_t> order(n_nets), large;
+    {
+      std::vector<uint32_t> at(LARGE_NET_PINS + 2, 0);
+      for (size_t n = 0; n < n_nets; ++n) {
+        const uint32_t d = degree((uint32_t)n);
+        if (d <= LARGE_NET_PINS)
+          at[d + 1] += 1;
+        else
+          large.push_back((uint32_t)n);
+      }
+      for (size_t d = 1; d < at.size(); ++d) at[d] += at[d - 1];
+      for (size_t n = 0; n < n_nets; ++n) {
+        const uint32_t d = degree((uint32_t)n);
+        if (d <= LARGE_NET_PINS) order[at[d]++] = (uint32_t)n;
+      }
+      std::stable_sort(large.begin(), large.end(), [&](uint32_t a, uint32_t b) { return degree(a) > degree(b); });
+      std::copy(large.begin(), large.end(), order.begin() + (n_nets - large.size()));
+    }
     std::vector<uint32_t> start2(n_net_total + 1, 0), unit2(n_pins_total), new_pin(n_pins_total);
     std::vector<double> xy2(2 * n_pins_total), w2(n_net_total);
     uint32_t q = 0;
@@ -386,11 +449,26 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
         xy2[2 * q + 1] = pin_xy[2 * i + 1];
       }
       start2[k + 1] = q;
-      if (k >= n_nets || degree(n) <= LARGE_NET_PINS)
-        small_nets.push_back((uint32_t)k);
-      else
-        large_nets.push_back((uint32_t)k);
     }
+    // WA segments: runs of consecutive nets with one lane-group size; the
+    // looping wavefronts (longest nets) are scheduled first
+    std::vector<int4> runs;
+    for (size_t k = 0; k < n_net_total; ++k) {
+      const int g = wa_lanes(start2[k + 1] - start2[k]);
+      if (!runs.empty() && runs.back().z == g && (size_t)(runs.back().x + runs.back().y) == k)
+        runs.back().y += 1;
+      else
+        runs.push_back(make_int4((int)k, 1, g, 0));
+    }
+    std::stable_partition(runs.begin(), runs.end(), [](const int4 &r) { return r.z == 0; });
+    if (runs.size() > (size_t)WA_MAX_SEGMENTS) fail(TEXO_GP_ERR_UNSUPPORTED);
+    int waves = 0;
+    for (int4 &r : runs) {
+      r.w = waves;
+      waves += r.z == 0 ? r.y : (int)(((int64_t)r.y * r.z + 63) / 64);
+    }
+    wa_seg = runs;
+    wa_waves = waves;
     for (auto &i : u_pins) i = new_pin[i];
     // each pin's derivative goes straight to its unit's entry (nets first,
     // then timed arcs, each in input order), so the per-unit sums read
@@ -533,180 +611,16 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o) {
     upper[2 * (U + j)] = (double)(W - 1);
     upper[2 * (U + j) + 1] = (double)(H - 1);
   }
+  set_charges(std::vector<double>(m_charge), std::vector<double>(fl_charge));
+  stage(pool);
 }
 
 
-// Device footprint of one job (static arrays, two working slots, optimizer
-// vectors), used for admission by HBM footprint.
-size_t Job::device_bytes() const {
-  const size_t FG = (size_t)std::max(F, 1) * G;
-  size_t b = FG * 8 + M * (4 + 16 + 8 + 1) + DU * 8 + (size_t)std::max(F, 1) * DU * 8 + Fl * 9;
-  b += (n_net_total + 1) * 4 + n_pins_total * (4 + 16) + n_net_total * 8 + (small_nets.size() + large_nets.size()) * 4;
-  b += (U + 1) * 4 + n_pins_total * 4 + U * 16;
-  b += C * 8 * (3 + 4 + 3) + 2 * U * 4;
-  b += 2 * (FG * (8 + 8 + 4 * 4) + (M + Fl) * 16 + n_pins_total * 16);
-  b += 2 * G * 16 + FG * 8 + 2 * U * 8 + C * 8;
-  b += (M + Fl) * (4 + 4 + 4 + 16 + 8 + 1) + ((G >> (2 * SORT_TILE_SHIFT)) + W + H + 1) * 4;
-  const size_t keys = (size_t)((W + DEP_TILE - 1) / DEP_TILE) * ((H + DEP_TILE - 1) / DEP_TILE);
-  b += 2 * keys * (2 * (size_t)std::max(F, 1) * DEP_WIN2 * 8 + 4);
-  return b + 64 * 4096;
-}
-
-void Job::upload(const Basis *b, hipStream_t s) {
+void Job::upload(const Basis *b, const std::shared_ptr<BlockPool> &pool, hipStream_t s) {
   basis = b;
-  const size_t FG = (size_t)std::max(F, 1) * G;
-  auto blocks = [](size_t n, size_t per) { return std::max<size_t>(1, (n + per - 1) / per); };
-  nb_wa_small = small_nets.empty() ? 0 : (int)blocks(small_nets.size(), BLOCK);
-  nb_wa_large = large_nets.empty() ? 0 : (int)blocks(large_nets.size(), BLOCK / 64);
-  nb_energy = F > 0 ? poisson_energy_blocks((int)W, (int)H, F) : 0;
-  nb_gf = (int)blocks(Fl, BLOCK);
-  nb_ug = (int)blocks(U, BLOCK);
-  tiles_x = (int)((W + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
-  n_keys = tiles_x * (int)((H + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
-  arena.reserve(d_base, FG);
-  arena.reserve(d_m_unit, M);
-  arena.reserve(d_m_off, M);
-  arena.reserve(d_m_charge, M);
-  arena.reserve(d_m_field, M);
-  arena.reserve(d_du_unit, DU);
-  arena.reserve(d_du_mstart, DU + 1);
-  arena.reserve(d_du_fcharge, (size_t)std::max(F, 1) * DU);
-  arena.reserve(d_fl_field, Fl);
-  arena.reserve(d_fl_charge, Fl);
-  arena.reserve(d_net_start, n_net_total + 1);
-  arena.reserve(d_pin_unit, n_pins_total);
-  arena.reserve(d_pin_xy, n_pins_total);
-  arena.reserve(d_net_weight, n_net_total);
-  arena.reserve(d_small, small_nets.size());
-  arena.reserve(d_large, large_nets.size());
-  arena.reserve(d_u_pstart, U + 1);
-  arena.reserve(d_pin_slot, n_pins_total);
-  arena.reserve(d_u_nbase, U);
-  arena.reserve(d_wl_diag, U);
-  arena.reserve(d_unit_du, U);
-  arena.reserve(d_lower, C);
-  arena.reserve(d_upper, C);
-  arena.reserve(d_metric, C);
-  arena.reserve(d_part_metric, blocks(U + Fl, BLOCK) * 3);
-  arena.reserve(d_part_step, blocks(C, BLOCK) * 4);
-  for (auto &v : d_vec) arena.reserve(v, C);
-  for (auto &g : d_grad) arena.reserve(g, C);
-  arena.reserve(d_best, 2 * U);
-  for (auto &sl : slot) {
-    arena.reserve(sl.acc_m, FG);
-    arena.reserve(sl.acc_f, FG);
-    arena.reserve(sl.rho, FG);
-    arena.reserve(sl.phi, FG);
-    arena.reserve(sl.t0, FG);
-    arena.reserve(sl.t1, FG);
-    arena.reserve(sl.samp, M + Fl);
-    arena.reserve(sl.field_coef, 2 * MAXF);
-    arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
-    arena.reserve(sl.tile_flag, (size_t)n_keys);
-    arena.reserve(sl.amax, 4 * MAXF);
-    arena.reserve(sl.dpin, u_pins.size());
-    arena.reserve(sl.part_wa, (size_t)nb_wa_small + nb_wa_large);
-    arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
-    arena.reserve(sl.part_gf, (size_t)nb_gf * GF_STRIDE);
-    arena.reserve(sl.part_ug, (size_t)nb_ug * UG_STRIDE);
-  }
-  arena.reserve(d_rudy, 2 * G);
-  arena.reserve(d_rudy_fx, 2 * G);
-  arena.reserve(d_rudy_err, 1);
-  arena.reserve(d_raw_units, (size_t)std::max(F, 1) * DU * 2);
-  arena.reserve(d_raw_fillers, 2 * Fl);
-  arena.reserve(d_density_out, FG);
-  arena.reserve(d_wa_grad, 2 * U);
-  arena.reserve(d_eval_coords, C);
-  arena.reserve(d_perm, M + Fl);
-  arena.reserve(d_perm_next, M + Fl);
-  arena.reserve(d_s_ci, M + Fl);
-  arena.reserve(d_s_off, M + Fl);
-  arena.reserve(d_s_q, M + Fl);
-  arena.reserve(d_s_layer, M + Fl);
-  arena.reserve(d_sort_count, (size_t)n_keys + 1);
-  arena.commit();
-
-  auto up = [&](void *dst, const void *src, size_t bytes) {
-    if (bytes) HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-  };
-  // the deposit accumulators start at zero and k_convert leaves them at zero
-  for (auto &sl : slot) {
-    HIP(hipMemsetAsync(sl.acc_m, 0, FG * sizeof(long long), s));
-    HIP(hipMemsetAsync(sl.acc_f, 0, FG * sizeof(long long), s));
-  }
-  // fixed-point background: -smoothed capacity per field
-  std::vector<long long> base(FG, 0);
-  for (int f = 0; f < F; ++f) {
-    std::vector<double> smooth(G, 0.0);
-    auto it = capacity.find(field_kind[f]);
-    if (it != capacity.end())
-      for (size_t i = 0; i < G; ++i)
-        if (it->second[i]) host_deposit(smooth, W, H, (double)(i % W), (double)(i / W), (double)it->second[i]);
-    for (size_t i = 0; i < G; ++i) base[(size_t)f * G + i] = std::llrint(-smooth[i] * FX_SCALE);
-  }
-  up(d_base, base.data(), FG * sizeof(long long));
-  up(d_m_unit, m_unit.data(), M * 4);
-  up(d_m_off, m_off.data(), M * 16);
-  up(d_m_field, m_field.data(), M);
-  up(d_du_unit, du_unit.data(), DU * 4);
-  up(d_du_mstart, du_mstart.data(), (DU + 1) * 4);
-  up(d_fl_field, fl_field.data(), Fl);
-  up(d_net_start, net_start.data(), (n_net_total + 1) * 4);
-  up(d_pin_unit, pin_unit.data(), n_pins_total * 4);
-  up(d_pin_xy, pin_xy.data(), n_pins_total * 16);
-  up(d_net_weight, net_weight.data(), n_net_total * 8);
-  up(d_small, small_nets.data(), small_nets.size() * 4);
-  up(d_large, large_nets.data(), large_nets.size() * 4);
-  up(d_u_pstart, u_pstart.data(), (U + 1) * 4);
-  up(d_pin_slot, pin_slot.data(), n_pins_total * 4);
-  up(d_u_nbase, u_nbase.data(), U * 4);
-  up(d_wl_diag, wl_diag.data(), U * 8);
-  up(d_unit_du, unit_du.data(), U * 4);
-  up(d_lower, lower.data(), C * 8);
-  up(d_upper, upper.data(), C * 8);
-  std::vector<double> x0(C);
-  for (size_t u = 0; u < U; ++u) {
-    x0[2 * u] = initial_xy[2 * u];
-    x0[2 * u + 1] = initial_xy[2 * u + 1];
-  }
-  for (size_t k = 0; k < Fl; ++k) {
-    x0[2 * (U + k)] = fl_xy[2 * k];
-    x0[2 * (U + k) + 1] = fl_xy[2 * k + 1];
-  }
-  for (size_t i = 0; i < C; ++i) x0[i] = clampd(x0[i], lower[i], upper[i]);
-  up(d_vec[ix], x0.data(), C * 8);
-  up(d_vec[ixp], x0.data(), C * 8);
-  {
-    // input order until the engine sorts the charges by tile
-    const size_t n = M + Fl;
-    std::vector<uint32_t> perm(n), ci(n);
-    std::vector<double> off(2 * n, 0.0), q(n);
-    std::vector<uint8_t> layer(n);
-    for (size_t i = 0; i < M; ++i) {
-      perm[i] = (uint32_t)i;
-      ci[i] = m_unit[i];
-      off[2 * i] = m_off[2 * i];
-      off[2 * i + 1] = m_off[2 * i + 1];
-      q[i] = m_charge[i];
-      layer[i] = (uint8_t)(2 * m_field[i]);
-    }
-    for (size_t k = 0; k < Fl; ++k) {
-      const size_t i = M + k;
-      perm[i] = (uint32_t)i;
-      ci[i] = (uint32_t)(U + k);
-      q[i] = fl_charge[k];
-      layer[i] = (uint8_t)(2 * fl_field[k] + 1);
-    }
-    up(d_perm, perm.data(), n * 4);
-    up(d_s_ci, ci.data(), n * 4);
-    up(d_s_off, off.data(), n * 16);
-    up(d_s_q, q.data(), n * 8);
-    up(d_s_layer, layer.data(), n);
-    HIP(hipStreamSynchronize(s));  // host vectors above are pageable and local
-  }
-  HIP(hipStreamSynchronize(s));  // host vectors above are pageable and local
-  apply_charges(std::vector<double>(m_charge), std::vector<double>(fl_charge));
+  arena.commit(pool);
+  HIP(hipMemcpyAsync(arena.base(), image, image_bytes, hipMemcpyHostToDevice, s));
+  HIP(hipMemsetAsync(arena.base() + zero_begin, 0, zero_end - zero_begin, s));
 
   js = JobStatic{};
   js.W = (int)W;
@@ -736,10 +650,8 @@ void Job::upload(const Basis *b, hipStream_t s) {
   js.key_end = d_sort_count;
   js.lower = d_lower;
   js.upper = d_upper;
-  js.n_small = (int)small_nets.size();
-  js.n_large = (int)large_nets.size();
-  js.small_nets = d_small;
-  js.large_nets = d_large;
+  js.n_wa_seg = (int)wa_seg.size();
+  js.wa_seg = d_wa_seg;
   js.net_start = d_net_start;
   js.net_weight = d_net_weight;
   js.pin_unit = d_pin_unit;
@@ -759,25 +671,213 @@ void Job::upload(const Basis *b, hipStream_t s) {
 }
 
 // Installs member and filler charges (apply_area_adjustment, eplace.rs:793-819).
-void Job::apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas) {
+void Job::set_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas) {
   m_charge = member_areas;
   fl_charge = filler_areas;
-  std::vector<double> fcharge((size_t)std::max(F, 1) * DU, 0.0);
+  h_du_fcharge.assign((size_t)std::max(F, 1) * DU, 0.0);
   for (size_t i = 0; i < DU; ++i)
-    for (uint32_t m = du_mstart[i]; m < du_mstart[i + 1]; ++m) fcharge[(size_t)m_field[m] * DU + i] += m_charge[m];
+    for (uint32_t m = du_mstart[i]; m < du_mstart[i + 1]; ++m) h_du_fcharge[(size_t)m_field[m] * DU + i] += m_charge[m];
   real_area.assign(F, 0.0);
   for (size_t m = 0; m < M; ++m) real_area[m_field[m]] += m_charge[m];
-  if (M) HIP(hipMemcpy(d_m_charge, m_charge.data(), M * 8, hipMemcpyHostToDevice));
-  if (Fl) HIP(hipMemcpy(d_fl_charge, fl_charge.data(), Fl * 8, hipMemcpyHostToDevice));
-  if (DU) HIP(hipMemcpy(d_du_fcharge, fcharge.data(), fcharge.size() * 8, hipMemcpyHostToDevice));
+}
+
+void Job::apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas,
+                        hipStream_t s) {
+  set_charges(member_areas, filler_areas);
+  if (M) HIP(hipMemcpyAsync(d_m_charge, m_charge.data(), M * 8, hipMemcpyHostToDevice, s));
+  if (Fl) HIP(hipMemcpyAsync(d_fl_charge, fl_charge.data(), Fl * 8, hipMemcpyHostToDevice, s));
+  if (DU) HIP(hipMemcpyAsync(d_du_fcharge, h_du_fcharge.data(), h_du_fcharge.size() * 8, hipMemcpyHostToDevice, s));
   // the sorted charge copies follow the new charges; the deposit order stays
   const size_t n = M + Fl;
+  std::vector<uint32_t> perm(n);
+  std::vector<double> q(n);
   if (n) {
-    std::vector<uint32_t> perm(n);
-    HIP(hipMemcpy(perm.data(), d_perm, n * 4, hipMemcpyDeviceToHost));
-    std::vector<double> q(n);
+    HIP(hipMemcpyAsync(perm.data(), d_perm, n * 4, hipMemcpyDeviceToHost, s));
+    HIP(hipStreamSynchronize(s));
     for (size_t i = 0; i < n; ++i) q[i] = perm[i] < M ? m_charge[perm[i]] : fl_charge[perm[i] - M];
-    HIP(hipMemcpy(d_s_q, q.data(), n * 8, hipMemcpyHostToDevice));
+    HIP(hipMemcpyAsync(d_s_q, q.data(), n * 8, hipMemcpyHostToDevice, s));
+  }
+  HIP(hipStreamSynchronize(s));
+}
+
+void Job::release_image() {
+  if (image_pool && image) image_pool->release(image, image_cap);
+  image_pool.reset();
+  image_heap.reset();
+  image = nullptr;
+}
+
+// Arena plan and upload image of one job.  The arrays that start with data
+// lead the arena, in one block that the image fills byte for byte: the
+// problem arrays, the fixed-point background (-smoothed capacity per field),
+// the clamped start coordinates (twice: current and previous iterate) and the
+// charges in input order with their sorted-copy fields.  The deposit
+// accumulators follow (cleared once at upload; k_convert leaves them at
+// zero), then the working arrays.
+void Job::stage(const std::shared_ptr<BlockPool> &pool) {
+  const size_t FG = (size_t)std::max(F, 1) * G, n = M + Fl;
+  auto blocks = [](size_t count, size_t per) { return std::max<size_t>(1, (count + per - 1) / per); };
+  nb_wa = wa_waves == 0 ? 0 : (int)blocks((size_t)wa_waves, BLOCK / 64);
+  nb_energy = F > 0 ? poisson_energy_blocks((int)W, (int)H, F) : 0;
+  nb_gf = (int)blocks(Fl, BLOCK);
+  nb_ug = (int)blocks(U, BLOCK);
+  tiles_x = (int)((W + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
+  n_keys = tiles_x * (int)((H + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
+
+  struct Copy {
+    size_t at;
+    const void *src;
+    size_t bytes;
+  };
+  std::vector<Copy> copies;
+  auto data = [&](auto *&ptr, size_t count, const void *src) {
+    const size_t at = arena.reserve(ptr, count);
+    if (src) copies.push_back({at, src, count * sizeof(*ptr)});
+    return at;
+  };
+  const size_t o_base = data(d_base, FG, nullptr);
+  data(d_m_unit, M, m_unit.data());
+  data(d_m_off, M, m_off.data());
+  data(d_m_charge, M, m_charge.data());
+  data(d_m_field, M, m_field.data());
+  data(d_du_unit, DU, du_unit.data());
+  data(d_du_mstart, DU + 1, du_mstart.data());
+  data(d_du_fcharge, (size_t)std::max(F, 1) * DU, h_du_fcharge.data());
+  data(d_fl_field, Fl, fl_field.data());
+  data(d_fl_charge, Fl, fl_charge.data());
+  data(d_net_start, n_net_total + 1, net_start.data());
+  data(d_pin_unit, n_pins_total, pin_unit.data());
+  data(d_pin_xy, n_pins_total, pin_xy.data());
+  data(d_net_weight, n_net_total, net_weight.data());
+  data(d_wa_seg, wa_seg.size(), wa_seg.data());
+  data(d_u_pstart, U + 1, u_pstart.data());
+  data(d_pin_slot, n_pins_total, pin_slot.data());
+  data(d_u_nbase, U, u_nbase.data());
+  data(d_wl_diag, U, wl_diag.data());
+  data(d_unit_du, U, unit_du.data());
+  data(d_lower, C, lower.data());
+  data(d_upper, C, upper.data());
+  const size_t o_x0 = data(d_vec[ix], C, nullptr), o_x1 = data(d_vec[ixp], C, nullptr);
+  const size_t o_perm = data(d_perm, n, nullptr), o_ci = data(d_s_ci, n, nullptr);
+  const size_t o_off = data(d_s_off, n, nullptr), o_q = data(d_s_q, n, nullptr);
+  const size_t o_layer = data(d_s_layer, n, nullptr);
+  image_bytes = arena.planned();
+  zero_begin = image_bytes;
+  for (auto &sl : slot) {
+    arena.reserve(sl.acc_m, FG);
+    arena.reserve(sl.acc_f, FG);
+  }
+  zero_end = arena.planned();
+  arena.reserve(d_metric, C);
+  arena.reserve(d_part_metric, blocks(U + Fl, BLOCK) * 3);
+  arena.reserve(d_part_step, blocks(C, BLOCK) * 4);
+  for (int k = 0; k < 4; ++k)
+    if (k != ix && k != ixp) arena.reserve(d_vec[k], C);
+  for (auto &g : d_grad) arena.reserve(g, C);
+  arena.reserve(d_best, 2 * U);
+  for (auto &sl : slot) {
+    arena.reserve(sl.rho, FG);
+    arena.reserve(sl.phi, FG);
+    arena.reserve(sl.t0, FG);
+    arena.reserve(sl.t1, FG);
+    arena.reserve(sl.samp, n);
+    arena.reserve(sl.field_coef, 2 * MAXF);
+    arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
+    arena.reserve(sl.tile_flag, (size_t)n_keys);
+    arena.reserve(sl.amax, 4 * MAXF);
+    arena.reserve(sl.dpin, u_pins.size());
+    arena.reserve(sl.part_wa, (size_t)nb_wa);
+    arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
+    arena.reserve(sl.part_gf, (size_t)nb_gf * GF_STRIDE);
+    arena.reserve(sl.part_ug, (size_t)nb_ug * UG_STRIDE);
+  }
+  arena.reserve(d_rudy, 2 * G);
+  arena.reserve(d_rudy_fx, 2 * G);
+  arena.reserve(d_rudy_err, 1);
+  arena.reserve(d_raw_units, (size_t)std::max(F, 1) * DU * 2);
+  arena.reserve(d_raw_fillers, 2 * Fl);
+  arena.reserve(d_density_out, FG);
+  arena.reserve(d_wa_grad, 2 * U);
+  arena.reserve(d_eval_coords, C);
+  arena.reserve(d_perm_next, n);
+  arena.reserve(d_sort_count, (size_t)n_keys + 1);
+
+  if (pool) {
+    image = static_cast<char *>(pool->acquire(image_bytes, &image_cap));
+    image_pool = pool;
+  } else {
+    image_heap.reset(new char[image_bytes]);
+    image = image_heap.get();
+  }
+  for (const Copy &c : copies)
+    if (c.bytes) std::memcpy(image + c.at, c.src, c.bytes);
+
+  // Smoothed capacity (electrostatic_placement.rs:173-190, 900-916): a
+  // capacity point sits on a bin centre, so its spline weights are 1/8,
+  // 3/4, 1/8 per axis (clamped at the edges): the smoothing is one pass per
+  // axis.  Every partial sum is a small multiple of 1/64, exact in FP64, so
+  // this equals depositing the points one by one in any order.
+  long long *base = reinterpret_cast<long long *>(image + o_base);
+  std::fill(base, base + FG, 0ll);
+  std::vector<double> row(G), smooth(G);
+  for (int f = 0; f < F; ++f) {
+    auto it = capacity.find(field_kind[f]);
+    if (it == capacity.end()) continue;
+    const uint32_t *cap = it->second.data();
+    std::fill(row.begin(), row.end(), 0.0);
+    std::fill(smooth.begin(), smooth.end(), 0.0);
+    for (size_t y = 0; y < H; ++y)
+      for (size_t x = 0; x < W; ++x) {
+        const double q = (double)cap[y * W + x];
+        if (q == 0.0) continue;
+        double *r = row.data() + y * W;
+        r[x == 0 ? 0 : x - 1] += 0.125 * q;
+        r[x] += 0.75 * q;
+        r[std::min<size_t>(x + 1, W - 1)] += 0.125 * q;
+      }
+    for (size_t y = 0; y < H; ++y) {
+      const double *r = row.data() + y * W;
+      double *lo = smooth.data() + (y == 0 ? 0 : y - 1) * W, *mid = smooth.data() + y * W,
+             *hi = smooth.data() + std::min<size_t>(y + 1, H - 1) * W;
+      for (size_t x = 0; x < W; ++x) {
+        lo[x] += 0.125 * r[x];
+        mid[x] += 0.75 * r[x];
+        hi[x] += 0.125 * r[x];
+      }
+    }
+    for (size_t i = 0; i < G; ++i) base[(size_t)f * G + i] = std::llrint(-smooth[i] * FX_SCALE);
+  }
+  double *x0 = reinterpret_cast<double *>(image + o_x0);
+  for (size_t u = 0; u < U; ++u) {
+    x0[2 * u] = initial_xy[2 * u];
+    x0[2 * u + 1] = initial_xy[2 * u + 1];
+  }
+  for (size_t k = 0; k < Fl; ++k) {
+    x0[2 * (U + k)] = fl_xy[2 * k];
+    x0[2 * (U + k) + 1] = fl_xy[2 * k + 1];
+  }
+  for (size_t i = 0; i < C; ++i) x0[i] = clampd(x0[i], lower[i], upper[i]);
+  std::memcpy(image + o_x1, x0, C * sizeof(double));
+  // input order until the engine sorts the charges by tile
+  uint32_t *perm = reinterpret_cast<uint32_t *>(image + o_perm), *ci = reinterpret_cast<uint32_t *>(image + o_ci);
+  double *off = reinterpret_cast<double *>(image + o_off), *q = reinterpret_cast<double *>(image + o_q);
+  uint8_t *layer = reinterpret_cast<uint8_t *>(image + o_layer);
+  for (size_t i = 0; i < M; ++i) {
+    perm[i] = (uint32_t)i;
+    ci[i] = m_unit[i];
+    off[2 * i] = m_off[2 * i];
+    off[2 * i + 1] = m_off[2 * i + 1];
+    q[i] = m_charge[i];
+    layer[i] = (uint8_t)(2 * m_field[i]);
+  }
+  for (size_t k = 0; k < Fl; ++k) {
+    const size_t i = M + k;
+    perm[i] = (uint32_t)i;
+    ci[i] = (uint32_t)(U + k);
+    off[2 * i] = 0.0;
+    off[2 * i + 1] = 0.0;
+    q[i] = fl_charge[k];
+    layer[i] = (uint8_t)(2 * fl_field[k] + 1);
   }
 }
 
@@ -802,8 +902,7 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.part_energy = w.part_energy;
   t.part_gf = w.part_gf;
   t.part_ug = w.part_ug;
-  t.nb_wa_small = nb_wa_small;
-  t.nb_wa_large = nb_wa_large;
+  t.nb_wa = nb_wa;
   t.nb_energy = F > 0 ? nb_energy : 0;
   t.nb_gf = F > 0 && Fl > 0 ? nb_gf : 0;
   t.nb_ug = nb_ug;

@@ -578,14 +578,16 @@ struct WaSums {
   double psx, pdx, nsx, ndx, psy, pdy, nsy, ndy;
 };
 
+// The reference divides by gamma; these multiply by its reciprocal `ig`
+// (a relative difference of one rounding in each exponent's argument).
 __device__ __forceinline__ void wa_accumulate(double2 c, double xmin, double xmax, double ymin, double ymax,
-                                              double gamma, WaSums &s) {
+                                              double ig, WaSums &s) {
   const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
   // exactly 1.0 where the delta is zero (analytical_placement.rs:666-678)
-  const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
-  const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
-  const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
-  const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
+  const double ex = dxm == 0.0 ? 1.0 : exp(dxm * ig);
+  const double enx = dxn == 0.0 ? 1.0 : exp(-dxn * ig);
+  const double ey = dym == 0.0 ? 1.0 : exp(dym * ig);
+  const double eny = dyn == 0.0 ? 1.0 : exp(-dyn * ig);
   s.psx += ex;
   s.pdx += ex * dxm;
   s.nsx += enx;
@@ -596,121 +598,129 @@ __device__ __forceinline__ void wa_accumulate(double2 c, double xmin, double xma
   s.ndy += eny * dyn;
 }
 
+// r = reciprocals of the four exponential sums (psx, nsx, psy, nsy)
 __device__ __forceinline__ double2 wa_derivative(double2 c, double xmin, double xmax, double ymin, double ymax,
-                                                 double gamma, const WaSums &s, double pmx, double nmx, double pmy,
+                                                 double ig, const double4 &r, double pmx, double nmx, double pmy,
                                                  double nmy) {
   const double dxm = c.x - xmax, dxn = c.x - xmin, dym = c.y - ymax, dyn = c.y - ymin;
-  const double ex = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
-  const double enx = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
-  const double ey = dym == 0.0 ? 1.0 : exp(dym / gamma);
-  const double eny = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
-  const double ddx = (ex / s.psx) * (1.0 + (dxm - pmx) / gamma) - (enx / s.nsx) * (1.0 - (dxn - nmx) / gamma);
-  const double ddy = (ey / s.psy) * (1.0 + (dym - pmy) / gamma) - (eny / s.nsy) * (1.0 - (dyn - nmy) / gamma);
+  const double ex = dxm == 0.0 ? 1.0 : exp(dxm * ig);
+  const double enx = dxn == 0.0 ? 1.0 : exp(-dxn * ig);
+  const double ey = dym == 0.0 ? 1.0 : exp(dym * ig);
+  const double eny = dyn == 0.0 ? 1.0 : exp(-dyn * ig);
+  const double ddx = (ex * r.x) * (1.0 + (dxm - pmx) * ig) - (enx * r.y) * (1.0 - (dxn - nmx) * ig);
+  const double ddy = (ey * r.z) * (1.0 + (dym - pmy) * ig) - (eny * r.w) * (1.0 - (dyn - nmy) * ig);
   return make_double2(ddx, ddy);
 }
 
-// Net of exactly D pins: every pin coordinate is loaded once, up front, and
-// kept in registers for the three passes.  Same arithmetic, in the same
-// order, as the general loop below.
-template <int D>
-__device__ __forceinline__ double wa_fixed_net(const JobStatic &a, const Task &t, uint32_t b, double w) {
-  const double gamma = t.gamma;
-  double2 c[D];
+// Butterfly reductions inside aligned groups of G lanes.
+template <int G>
+__device__ __forceinline__ double group_min(double v) {
 #pragma unroll
-  for (int k = 0; k < D; ++k) c[k] = pin_coordinate(a, t.coords, b + k);
-  double xmin = c[0].x, xmax = c[0].x, ymin = c[0].y, ymax = c[0].y;
+  for (int o = G / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <int G>
+__device__ __forceinline__ double group_max(double v) {
 #pragma unroll
-  for (int k = 1; k < D; ++k) {
-    xmin = fmin(xmin, c[k].x);
-    xmax = fmax(xmax, c[k].x);
-    ymin = fmin(ymin, c[k].y);
-    ymax = fmax(ymax, c[k].y);
+  for (int o = G / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One net per aligned group of G lanes (a WA segment of G-lane nets).  Lane
+// r of the group holds pins b + r + G k (k < WA_PINS_PER_LANE) in
+// registers: every pin coordinate is loaded once and its four exponentials
+// are computed once; the group combines extremes and sums by butterflies.
+template <int G>
+__device__ double wa_group_net(const JobStatic &a, const Task &t, const int4 &seg, int wave) {
+  constexpr int K = WA_PINS_PER_LANE;
+  const int lane = threadIdx.x & 63, r = lane & (G - 1);
+  const int local = (wave - seg.w) * (64 / G) + lane / G;
+  const bool valid = local < seg.y;
+  uint32_t b = 0, e = 0;
+  double w = 0.0;
+  if (valid) {
+    const uint32_t net = (uint32_t)(seg.x + local);
+    b = a.net_start[net];
+    e = a.net_start[net + 1];
+    w = a.net_weight[net];
   }
-  // the four exponentials of each pin are computed once and reused by the
-  // derivative pass (the values are the ones wa_derivative would recompute)
-  double ex[D], enx[D], ey[D], eny[D];
+  const double ig = 1.0 / t.gamma;
+  double2 c[K];
+  bool in[K];
+  double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t p = b + (uint32_t)(r + G * k);
+    in[k] = p < e;
+    c[k] = in[k] ? pin_coordinate(a, t.coords, p) : make_double2(0.0, 0.0);
+    if (in[k]) {
+      xmin = fmin(xmin, c[k].x);
+      xmax = fmax(xmax, c[k].x);
+      ymin = fmin(ymin, c[k].y);
+      ymax = fmax(ymax, c[k].y);
+    }
+  }
+  xmin = group_min<G>(xmin);
+  xmax = group_max<G>(xmax);
+  ymin = group_min<G>(ymin);
+  ymax = group_max<G>(ymax);
+  double ex[K], enx[K], ey[K], eny[K];
   WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
+  for (int k = 0; k < K; ++k) {
     const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
-    ex[k] = dxm == 0.0 ? 1.0 : exp(dxm / gamma);
-    enx[k] = dxn == 0.0 ? 1.0 : exp(-dxn / gamma);
-    ey[k] = dym == 0.0 ? 1.0 : exp(dym / gamma);
-    eny[k] = dyn == 0.0 ? 1.0 : exp(-dyn / gamma);
-    s.psx += ex[k];
-    s.pdx += ex[k] * dxm;
-    s.nsx += enx[k];
-    s.ndx += enx[k] * dxn;
-    s.psy += ey[k];
-    s.pdy += ey[k] * dym;
-    s.nsy += eny[k];
-    s.ndy += eny[k] * dyn;
+    // exactly 1.0 where the delta is zero (analytical_placement.rs:666-678)
+    ex[k] = !in[k] ? 0.0 : dxm == 0.0 ? 1.0 : exp(dxm * ig);
+    enx[k] = !in[k] ? 0.0 : dxn == 0.0 ? 1.0 : exp(-dxn * ig);
+    ey[k] = !in[k] ? 0.0 : dym == 0.0 ? 1.0 : exp(dym * ig);
+    eny[k] = !in[k] ? 0.0 : dyn == 0.0 ? 1.0 : exp(-dyn * ig);
+    if (in[k]) {
+      s.psx += ex[k];
+      s.pdx += ex[k] * dxm;
+      s.nsx += enx[k];
+      s.ndx += enx[k] * dxn;
+      s.psy += ey[k];
+      s.pdy += ey[k] * dym;
+      s.nsy += eny[k];
+      s.ndy += eny[k] * dyn;
+    }
   }
-  const double pmx = s.pdx / s.psx, nmx = s.ndx / s.nsx, pmy = s.pdy / s.psy, nmy = s.ndy / s.nsy;
+  s.psx = group_sum<G>(s.psx);
+  s.pdx = group_sum<G>(s.pdx);
+  s.nsx = group_sum<G>(s.nsx);
+  s.ndx = group_sum<G>(s.ndx);
+  s.psy = group_sum<G>(s.psy);
+  s.pdy = group_sum<G>(s.pdy);
+  s.nsy = group_sum<G>(s.nsy);
+  s.ndy = group_sum<G>(s.ndy);
+  const double4 rs = make_double4(1.0 / s.psx, 1.0 / s.nsx, 1.0 / s.psy, 1.0 / s.nsy);
+  const double pmx = s.pdx * rs.x, nmx = s.ndx * rs.y, pmy = s.pdy * rs.z, nmy = s.ndy * rs.w;
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
+  for (int k = 0; k < K; ++k) {
+    if (!in[k]) continue;
+    const uint32_t p = b + (uint32_t)(r + G * k);
     const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
-    const double ddx = (ex[k] / s.psx) * (1.0 + (dxm - pmx) / gamma) - (enx[k] / s.nsx) * (1.0 - (dxn - nmx) / gamma);
-    const double ddy = (ey[k] / s.psy) * (1.0 + (dym - pmy) / gamma) - (eny[k] / s.nsy) * (1.0 - (dyn - nmy) / gamma);
-    const uint32_t slot = a.pin_slot[b + k];
+    const double ddx = (ex[k] * rs.x) * (1.0 + (dxm - pmx) * ig) - (enx[k] * rs.y) * (1.0 - (dxn - nmx) * ig);
+    const double ddy = (ey[k] * rs.z) * (1.0 + (dym - pmy) * ig) - (eny[k] * rs.w) * (1.0 - (dyn - nmy) * ig);
+    const uint32_t slot = a.pin_slot[p];
     if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * ddx, w * ddy);
   }
-  return w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
+  return valid && r == 0 ? w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy) : 0.0;
 }
 
-// Thread per net (<= 16 pins).  Small nets are grouped by degree, so the
-// lanes of a wavefront take the same branch below.
-__device__ double wa_thread_net(const Task &t, int local) {
-  const JobStatic &a = t.j;
-  if (local >= a.n_small) return 0.0;
-  const uint32_t net = a.small_nets[local];
-  const uint32_t b = a.net_start[net], e = a.net_start[net + 1];
-  const double w = a.net_weight[net];
-  switch (e - b) {
-    case 2:
-      return wa_fixed_net<2>(a, t, b, w);
-    case 3:
-      return wa_fixed_net<3>(a, t, b, w);
-    case 4:
-      return wa_fixed_net<4>(a, t, b, w);
-    case 5:
-      return wa_fixed_net<5>(a, t, b, w);
-    case 6:
-      return wa_fixed_net<6>(a, t, b, w);
-    default:
-      break;
-  }
-  const double gamma = t.gamma;
-  const double2 first = pin_coordinate(a, t.coords, b);
-  double xmin = first.x, xmax = first.x, ymin = first.y, ymax = first.y;
-  for (uint32_t p = b + 1; p < e; ++p) {
-    const double2 c = pin_coordinate(a, t.coords, p);
-    xmin = fmin(xmin, c.x);
-    xmax = fmax(xmax, c.x);
-    ymin = fmin(ymin, c.y);
-    ymax = fmax(ymax, c.y);
-  }
-  WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t p = b; p < e; ++p) wa_accumulate(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s);
-  const double pmx = s.pdx / s.psx, nmx = s.ndx / s.nsx, pmy = s.pdy / s.psy, nmy = s.ndy / s.nsy;
-  for (uint32_t p = b; p < e; ++p) {
-    const double2 d =
-        wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
-    const uint32_t slot = a.pin_slot[p];
-    if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * d.x, w * d.y);
-  }
-  return w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy);
-}
-
-// One wavefront per high-fanout net; lane 0 returns the value.
-__device__ double wa_wave_net(const Task &t, int local) {
-  const JobStatic &a = t.j;
+// One wavefront per net of more than WA_LOOP_PINS pins, looping over its
+// pins in each pass; lane 0 returns the value.
+__device__ double wa_loop_net(const JobStatic &a, const Task &t, uint32_t net) {
   const int lane = threadIdx.x & 63;
-  if (local >= a.n_large) return 0.0;
-  const uint32_t net = a.large_nets[local];
   const uint32_t b = a.net_start[net], e = a.net_start[net + 1];
   const double w = a.net_weight[net];
-  const double gamma = t.gamma;
+  const double ig = 1.0 / t.gamma;
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
   for (uint32_t p = b + lane; p < e; p += 64) {
     const double2 c = pin_coordinate(a, t.coords, p);
@@ -725,7 +735,7 @@ __device__ double wa_wave_net(const Task &t, int local) {
   ymax = wave_max(ymax);
   WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t p = b + lane; p < e; p += 64)
-    wa_accumulate(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s);
+    wa_accumulate(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, ig, s);
   s.psx = wave_sum(s.psx);
   s.pdx = wave_sum(s.pdx);
   s.nsx = wave_sum(s.nsx);
@@ -734,27 +744,58 @@ __device__ double wa_wave_net(const Task &t, int local) {
   s.pdy = wave_sum(s.pdy);
   s.nsy = wave_sum(s.nsy);
   s.ndy = wave_sum(s.ndy);
-  const double pmx = s.pdx / s.psx, nmx = s.ndx / s.nsx, pmy = s.pdy / s.psy, nmy = s.ndy / s.nsy;
+  const double4 r = make_double4(1.0 / s.psx, 1.0 / s.nsx, 1.0 / s.psy, 1.0 / s.nsy);
+  const double pmx = s.pdx * r.x, nmx = s.ndx * r.y, pmy = s.pdy * r.z, nmy = s.ndy * r.w;
   for (uint32_t p = b + lane; p < e; p += 64) {
-    const double2 d =
-        wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, gamma, s, pmx, nmx, pmy, nmy);
+    const double2 d = wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, ig, r, pmx, nmx, pmy, nmy);
     const uint32_t slot = a.pin_slot[p];
     if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * d.x, w * d.y);
   }
   return lane == 0 ? w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy) : 0.0;
 }
 
-// Per task: local blocks [0, nb_wa_large) take high-fanout nets (long serial
-// chains, scheduled first), the rest one small net per thread.
+// Per task: each wavefront takes part of one WA segment (looping nets first).
 __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, BlockPlan plan) {
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
-  double value[1];
-  if (bm.y < t.nb_wa_large)
-    value[0] = wa_wave_net(t, bm.y * (BLOCK / 64) + (threadIdx.x >> 6));
-  else
-    value[0] = wa_thread_net(t, (bm.y - t.nb_wa_large) * BLOCK + threadIdx.x);
+  const JobStatic &a = t.j;
+  const int wave = bm.y * (BLOCK / 64) + (threadIdx.x >> 6);
+  int4 seg = make_int4(0, 0, -1, 0);  // past the last segment: no net
+  for (int k = 0; k < a.n_wa_seg; ++k) {
+    const int4 sk = a.wa_seg[k];
+    const int n = sk.z == 0 ? sk.y : (sk.y * sk.z + 63) / 64;
+    if (wave >= sk.w && wave < sk.w + n) seg = sk;
+  }
+  double value[1] = {0.0};
+  switch (seg.z) {
+    case 0:
+      value[0] = wa_loop_net(a, t, (uint32_t)(seg.x + (wave - seg.w)));
+      break;
+    case 1:
+      value[0] = wa_group_net<1>(a, t, seg, wave);
+      break;
+    case 2:
+      value[0] = wa_group_net<2>(a, t, seg, wave);
+      break;
+    case 4:
+      value[0] = wa_group_net<4>(a, t, seg, wave);
+      break;
+    case 8:
+      value[0] = wa_group_net<8>(a, t, seg, wave);
+      break;
+    case 16:
+      value[0] = wa_group_net<16>(a, t, seg, wave);
+      break;
+    case 32:
+      value[0] = wa_group_net<32>(a, t, seg, wave);
+      break;
+    case 64:
+      value[0] = wa_group_net<64>(a, t, seg, wave);
+      break;
+    default:
+      break;
+  }
   block_sum<1>(value, t.part_wa + bm.y);
 }
 
@@ -842,7 +883,7 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tas
   const Task &t = tasks[blockIdx.x];
   EvalScalars *sc = t.scalars;
   double v1[1];
-  sum_partials<1>(t.part_wa, t.nb_wa_small + t.nb_wa_large, v1, 1, 0);
+  sum_partials<1>(t.part_wa, t.nb_wa, v1, 1, 0);
   if (threadIdx.x == 0) sc->wa_value = v1[0];
   double ug[1 + 2 * MAXF], gf[2 * MAXF], mu[2], mf[2];
   sum_partials<1 + 2 * MAXF>(t.part_ug, t.nb_ug, ug, UG_STRIDE, 0);
