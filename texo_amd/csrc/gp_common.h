@@ -262,12 +262,12 @@ struct OptTask {
 };
 
 struct Maps {
-  BlockPlan deposit, convert, sample, fillers, wa, unit_grad;
+  BlockPlan deposit, convert, sample, wa, gradient;
   int n_tasks;
 };
 
 void launch_density(const Task *tasks, const Maps &m, hipStream_t s);
-void launch_gather(const Task *tasks, const Maps &m, hipStream_t s);  // coefficients, samples, fillers
+void launch_gather(const Task *tasks, const Maps &m, hipStream_t s);  // coefficients and field samples
 void launch_wa(const Task *tasks, const Maps &m, hipStream_t s);
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s);
 

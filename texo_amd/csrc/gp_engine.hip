@@ -246,19 +246,19 @@ void Engine::ensure_stage(RoundCtx &c, size_t bytes) {
 void Engine::ensure_compact(RoundCtx &c, size_t n) {
   if (n <= c.compact_cap) return;
   const size_t cap = std::max<size_t>(n, 2 * c.compact_cap);
-  if (c.d_compact) HIP(hipFree(c.d_compact));
   if (c.h_compact) HIP(hipHostFree(c.h_compact));
   c.d_compact = nullptr;
   c.h_compact = nullptr;
-  HIP(hipMalloc(reinterpret_cast<void **>(&c.d_compact), cap * sizeof(JobScalars)));
-  HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_compact), cap * sizeof(JobScalars), hipHostMallocDefault));
+  // the collecting kernel writes straight into coherent pinned host memory
+  HIP(hipHostMalloc(reinterpret_cast<void **>(&c.h_compact), cap * sizeof(JobScalars),
+                    hipHostMallocMapped | hipHostMallocCoherent));
+  HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&c.d_compact), c.h_compact, 0));
   c.compact_cap = cap;
 }
 
 void Engine::free_ctx(RoundCtx &c) {
   if (c.d_stage) (void)hipFree(c.d_stage);
   if (c.h_stage) (void)hipHostFree(c.h_stage);
-  if (c.d_compact) (void)hipFree(c.d_compact);
   if (c.h_compact) (void)hipHostFree(c.h_compact);
   c.d_stage = c.h_stage = nullptr;
   c.d_compact = c.h_compact = nullptr;
@@ -791,7 +791,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   std::vector<int> l_sort, l_pre, l_cand;
   Plan p_sort_c, p_sort_k, p_pre, p_ext, p_cand;
   std::vector<Task> tasks;
-  Plan p_dep, p_conv, p_sample, p_gf, p_wa, p_ug;
+  Plan p_dep, p_conv, p_sample, p_wa, p_gr;
   struct Group {
     std::vector<int2> h, w;
   };
@@ -877,14 +877,14 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
       } else {
         p_dep.none();
         p_conv.none();
-        p_sample.none();
+        // one sample block still computes the field coefficients
+        if (j->F > 0)
+          p_sample.add(1, 1);
+        else
+          p_sample.none();
       }
-      if (j->F > 0 && j->Fl > 0)
-        p_gf.add((size_t)j->nb_gf, 1);
-      else
-        p_gf.none();
       p_wa.add((size_t)j->nb_wa, 1);
-      p_ug.add((size_t)j->nb_ug, 1);
+      p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
     }
   }
   std::vector<char> buf;
@@ -894,8 +894,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
                o_ppre = append(buf, p_pre.start), o_pext = append(buf, p_ext.start),
                o_pcand = append(buf, p_cand.start);
   const size_t o_pdep = append(buf, p_dep.start), o_pconv = append(buf, p_conv.start),
-               o_pgu = append(buf, p_sample.start),
-               o_pgf = append(buf, p_gf.start), o_pwa = append(buf, p_wa.start), o_pug = append(buf, p_ug.start);
+               o_pgu = append(buf, p_sample.start), o_pwa = append(buf, p_wa.start), o_pgr = append(buf, p_gr.start);
   std::vector<std::pair<size_t, size_t>> o_groups;
   for (auto &kv : groups) o_groups.push_back({append(buf, kv.second.h), append(buf, kv.second.w)});
   std::vector<int> pool_index;
@@ -932,9 +931,8 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   mp.deposit = p_dep.device(d_stage_, o_pdep);
   mp.convert = p_conv.device(d_stage_, o_pconv);
   mp.sample = p_sample.device(d_stage_, o_pgu);
-  mp.fillers = p_gf.device(d_stage_, o_pgf);
   mp.wa = p_wa.device(d_stage_, o_pwa);
-  mp.unit_grad = p_ug.device(d_stage_, o_pug);
+  mp.gradient = p_gr.device(d_stage_, o_pgr);
   mp.n_tasks = (int)tasks.size();
   // WA depends only on coordinates: it runs on the second stream beside the
   // density chain and joins before the combination.
@@ -962,8 +960,6 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   launch_collect_scalars(reinterpret_cast<const int *>(dptr(o_pool)), (int)active.size(), d_pool_, cx.d_compact,
                          stream_);
   HIP(hipGetLastError());
-  HIP(hipMemcpyAsync(cx.h_compact, cx.d_compact, active.size() * sizeof(JobScalars), hipMemcpyDeviceToHost,
-                     stream_));
   HIP(hipEventRecord(cx.done, stream_));
   cx.pending = true;
   g_timer = nullptr;
@@ -974,7 +970,12 @@ void Engine::finish(int c) {
   RoundCtx &cx = ctx_[c];
   if (!cx.pending) return;
   HIP(hipSetDevice(device_));
-  HIP(hipEventSynchronize(cx.done));
+  // the next round's planning waits on this: poll rather than sleep
+  for (;;) {
+    const hipError_t q = hipEventQuery(cx.done);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) HIP(q);
+  }
   cx.pending = false;
   g_timer = &timer;
   for (size_t k = 0; k < cx.active.size(); ++k) {
@@ -1013,7 +1014,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   t.wa_grad_out = j->d_wa_grad;
   if (density_out) t.density_out = j->d_density_out;
   tasks.push_back(t);
-  Plan p_dep, p_conv, p_sample, p_gf, p_wa, p_ug;
+  Plan p_dep, p_conv, p_sample, p_wa, p_gr;
   std::vector<int2> gh, gw;
   if (j->F > 0) {
     p_dep.add((size_t)j->n_keys, 1);
@@ -1021,22 +1022,17 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
     add_blocks(gh, 0, (size_t)j->F * j->H, 32);
     add_blocks(gw, 0, (size_t)j->F * j->W, 32);
     p_sample.add((size_t)j->n_keys, 1);
-    if (j->Fl > 0)
-      p_gf.add((size_t)j->nb_gf, 1);
-    else
-      p_gf.none();
   } else {
     p_dep.none();
     p_conv.none();
     p_sample.none();
-    p_gf.none();
   }
   p_wa.add((size_t)j->nb_wa, 1);
-  p_ug.add((size_t)j->nb_ug, 1);
+  p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
   std::vector<char> buf;
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),
-               o_gu = append(buf, p_sample.start), o_gf = append(buf, p_gf.start), o_wa = append(buf, p_wa.start),
-               o_ug = append(buf, p_ug.start), o_h = append(buf, gh), o_w = append(buf, gw);
+               o_gu = append(buf, p_sample.start), o_wa = append(buf, p_wa.start), o_gr = append(buf, p_gr.start),
+               o_h = append(buf, gh), o_w = append(buf, gw);
   HIP(hipStreamSynchronize(stream_));
   ensure_stage(aux_, buf.size());
   std::memcpy(aux_.h_stage, buf.data(), buf.size());
@@ -1047,9 +1043,8 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   mp.deposit = p_dep.device(d_stage_, o_dep);
   mp.convert = p_conv.device(d_stage_, o_conv);
   mp.sample = p_sample.device(d_stage_, o_gu);
-  mp.fillers = p_gf.device(d_stage_, o_gf);
   mp.wa = p_wa.device(d_stage_, o_wa);
-  mp.unit_grad = p_ug.device(d_stage_, o_ug);
+  mp.gradient = p_gr.device(d_stage_, o_gr);
   mp.n_tasks = 1;
   g_timer = &timer;
   launch_wa(d_tasks, mp, stream_);

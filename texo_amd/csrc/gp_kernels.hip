@@ -440,8 +440,7 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
 // One block per task: E_f = 1/2 <rho_bar, phi> from the transform's
 // partials (fixed order) and the coefficient lambda (1 + beta E/E0)
 // (eplace.rs:1111-1124).
-__global__ void __launch_bounds__(BLOCK) k_coefficients(const Task *__restrict__ tasks) {
-  const Task &t = tasks[blockIdx.x];
+__device__ void task_coefficients(const Task &t) {
   double e[MAXF];
   sum_partials<MAXF>(t.part_energy, t.nb_energy, e, MAXF, 0);
   if (threadIdx.x < MAXF) {
@@ -481,6 +480,9 @@ __device__ __forceinline__ void sample_field(const Axis &ax, const Axis &ay, At 
 // tile samples its field there (from HBM when it moved out of the window).
 // The sample goes to the charge's input index, so the per-unit and
 // per-filler sums below keep the reference's order.
+// Block 0 of each task also sums the energy partials into the field
+// coefficients (a task whose slot already holds its field has only that
+// block: its coefficients still follow the current multipliers).
 __global__ void __launch_bounds__(BLOCK) k_sample(const Task *__restrict__ tasks, BlockPlan plan) {
   __shared__ float win[MAXF * DEP_WIN2];
   int2 bm;
@@ -488,6 +490,8 @@ __global__ void __launch_bounds__(BLOCK) k_sample(const Task *__restrict__ tasks
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   const int key = bm.y;
+  if (key == 0) task_coefficients(t);
+  if (t.skip_density) return;
   const uint32_t begin = key > 0 ? a.key_end[key - 1] : 0u, end = a.key_end[key];
   if (begin == end) return;
   const int x0 = (key % a.tiles_x) * DEP_TILE - DEP_HALO, y0 = (key / a.tiles_x) * DEP_TILE - DEP_HALO;
@@ -520,12 +524,9 @@ __global__ void __launch_bounds__(BLOCK) k_sample(const Task *__restrict__ tasks
 
 // Thread per filler, input order: combined filler gradient, force and raw
 // l1 sums, KKT term.
-__global__ void __launch_bounds__(BLOCK) k_filler_gradient(const Task *__restrict__ tasks, BlockPlan plan) {
-  int2 bm;
-  if (!plan_block(plan, bm)) return;
-  const Task &t = tasks[bm.x];
+__device__ void filler_gradient_block(const Task &t, int lb) {
   const JobStatic &a = t.j;
-  const int k = bm.y * BLOCK + threadIdx.x;
+  const int k = lb * BLOCK + threadIdx.x;
   double red[2 * MAXF];
 #pragma unroll
   for (int q = 0; q < 2 * MAXF; ++q) red[q] = 0.0;
@@ -560,8 +561,8 @@ __global__ void __launch_bounds__(BLOCK) k_filler_gradient(const Task *__restric
                  kkt_scaled(vy, t.coords[c + 1], a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
     }
   }
-  block_sum<2 * MAXF>(red, t.part_gf + (size_t)bm.y * GF_STRIDE);
-  block_max_flag(kkt, bad, t.part_gf + (size_t)bm.y * GF_STRIDE + 2 * MAXF);
+  block_sum<2 * MAXF>(red, t.part_gf + (size_t)lb * GF_STRIDE);
+  block_max_flag(kkt, bad, t.part_gf + (size_t)lb * GF_STRIDE + 2 * MAXF);
 }
 
 // ---------------------------------------------------------------------------
@@ -803,12 +804,9 @@ __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, Bl
 // baseline pins, then timed-arc pins, then each field in order, the field
 // term being coefficient x the sum of the unit's member samples x charge in
 // member order (electrostatic_placement.rs:338-374).
-__global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict__ tasks, BlockPlan plan) {
-  int2 bm;
-  if (!plan_block(plan, bm)) return;
-  const Task &t = tasks[bm.x];
+__device__ void unit_gradient_block(const Task &t, int lb) {
   const JobStatic &a = t.j;
-  const int u = bm.y * BLOCK + threadIdx.x;
+  const int u = lb * BLOCK + threadIdx.x;
   double red[1 + 2 * MAXF];
 #pragma unroll
   for (int k = 0; k < 1 + 2 * MAXF; ++k) red[k] = 0.0;
@@ -874,34 +872,82 @@ __global__ void __launch_bounds__(BLOCK) k_unit_gradient(const Task *__restrict_
                  kkt_scaled(gy, t.coords[c + 1], a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
     }
   }
-  block_sum<1 + 2 * MAXF>(red, t.part_ug + (size_t)bm.y * UG_STRIDE);
-  block_max_flag(kkt, bad, t.part_ug + (size_t)bm.y * UG_STRIDE + 1 + 2 * MAXF);
+  block_sum<1 + 2 * MAXF>(red, t.part_ug + (size_t)lb * UG_STRIDE);
+  block_max_flag(kkt, bad, t.part_ug + (size_t)lb * UG_STRIDE + 1 + 2 * MAXF);
 }
 
-// One block per task: every FP64 scalar of the evaluation.
-__global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tasks) {
-  const Task &t = tasks[blockIdx.x];
-  EvalScalars *sc = t.scalars;
-  double v1[1];
-  sum_partials<1>(t.part_wa, t.nb_wa, v1, 1, 0);
-  if (threadIdx.x == 0) sc->wa_value = v1[0];
-  double ug[1 + 2 * MAXF], gf[2 * MAXF], mu[2], mf[2];
-  sum_partials<1 + 2 * MAXF>(t.part_ug, t.nb_ug, ug, UG_STRIDE, 0);
-  sum_partials<2 * MAXF>(t.part_gf, t.nb_gf, gf, GF_STRIDE, 0);
-  max_partials<2>(t.part_ug, t.nb_ug, mu, UG_STRIDE, 1 + 2 * MAXF);
-  max_partials<2>(t.part_gf, t.nb_gf, mf, GF_STRIDE, 2 * MAXF);
-  if (threadIdx.x == 0) sc->wa_l1 = ug[0];
-  if (threadIdx.x < MAXF) {
-    const int f = threadIdx.x;
-    sc->energy[f] = f < t.j.F ? t.field_coef[MAXF + f] : 0.0;
-    sc->force_l1[f] = ug[1 + f] + gf[f];
-    sc->raw_l1[f] = ug[1 + MAXF + f] + gf[MAXF + f];
+// Every FP64 scalar of one evaluation from its partials (one block).
+__device__ void finalize_task(const Task &t) {
+  // one pass over the three partial arrays; each sum and maximum combines
+  // in the order of sum_partials / max_partials
+  constexpr int NS = 1 + (1 + 2 * MAXF) + 2 * MAXF, NM = 4;
+  __shared__ double sh[NS + NM][BLOCK / 64];
+  double acc[NS + NM];
+#pragma unroll
+  for (int k = 0; k < NS + NM; ++k) acc[k] = 0.0;
+  const int n = max(t.nb_wa, max(t.nb_ug, t.nb_gf));
+  for (int b = threadIdx.x; b < n; b += BLOCK) {
+    if (b < t.nb_wa) acc[0] += t.part_wa[b];
+    if (b < t.nb_ug) {
+      const double *p = t.part_ug + (size_t)b * UG_STRIDE;
+#pragma unroll
+      for (int k = 0; k < 1 + 2 * MAXF; ++k) acc[1 + k] += p[k];
+      acc[NS] = fmax(acc[NS], p[1 + 2 * MAXF]);
+      acc[NS + 1] = fmax(acc[NS + 1], p[2 + 2 * MAXF]);
+    }
+    if (b < t.nb_gf) {
+      const double *p = t.part_gf + (size_t)b * GF_STRIDE;
+#pragma unroll
+      for (int k = 0; k < 2 * MAXF; ++k) acc[2 + 2 * MAXF + k] += p[k];
+      acc[NS + 2] = fmax(acc[NS + 2], p[2 * MAXF]);
+      acc[NS + 3] = fmax(acc[NS + 3], p[2 * MAXF + 1]);
+    }
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NS + NM; ++k) {
+    acc[k] = k < NS ? wave_sum(acc[k]) : wave_max(acc[k]);
+    if (lane == 0) sh[k][wave] = acc[k];
+  }
+  __syncthreads();
+  double r[NS + NM];
+#pragma unroll
+  for (int k = 0; k < NS + NM; ++k)
+    r[k] = k < NS ? ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3]
+                  : fmax(fmax(sh[k][0], sh[k][1]), fmax(sh[k][2], sh[k][3]));
+  // r: wa value, unit sums (wa l1, force l1 x MAXF, raw l1 x MAXF), filler
+  // sums (force l1 x MAXF, raw l1 x MAXF), then unit / filler kkt and flag
   if (threadIdx.x == 0) {
-    sc->kkt_bits = static_cast<unsigned long long>(__double_as_longlong(fmax(mu[0], mf[0])));
-    sc->nonfinite = (mu[1] > 0.0 || mf[1] > 0.0) ? 1u : 0u;
+    EvalScalars *sc = t.scalars;
+    sc->wa_value = r[0];
+    sc->wa_l1 = r[1];
+#pragma unroll
+    for (int f = 0; f < MAXF; ++f) {
+      sc->energy[f] = f < t.j.F ? t.field_coef[MAXF + f] : 0.0;
+      sc->force_l1[f] = r[2 + f] + r[2 + 2 * MAXF + f];
+      sc->raw_l1[f] = r[2 + MAXF + f] + r[2 + 3 * MAXF + f];
+    }
+    sc->kkt_bits = static_cast<unsigned long long>(__double_as_longlong(fmax(r[NS], r[NS + 2])));
+    sc->nonfinite = (r[NS + 1] > 0.0 || r[NS + 3] > 0.0) ? 1u : 0u;
   }
 }
+
+// Gradient combination of one evaluation in one launch: per task the unit
+// blocks, then the filler blocks.
+__global__ void __launch_bounds__(BLOCK) k_gradient(const Task *__restrict__ tasks, BlockPlan plan) {
+  int2 bm;
+  if (!plan_block(plan, bm)) return;
+  const Task &t = tasks[bm.x];
+  if (bm.y < t.nb_ug)
+    unit_gradient_block(t, bm.y);
+  else
+    filler_gradient_block(t, bm.y - t.nb_ug);
+}
+
+// One block per task: every FP64 scalar of the evaluation.  (A last-block
+// reduction inside k_gradient would need a device-scope fence per block,
+// which writes back the XCD's L2.)
+__global__ void __launch_bounds__(BLOCK) k_finalize(const Task *__restrict__ tasks) { finalize_task(tasks[blockIdx.x]); }
 
 // ---------------------------------------------------------------------------
 // Optimizer elementwise kernels.
@@ -1182,15 +1228,10 @@ void launch_sort(const SortTask *tasks, const SortMaps &m, hipStream_t s) {
 }
 
 void launch_gather(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.n_tasks <= 0) return;
-  hipLaunchKernelGGL(k_coefficients, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
-  if (m.sample.total > 0) {
-    timer_begin(g_timer, "gather", s);
-    hipLaunchKernelGGL(k_sample, dim3(plan_grid(m.sample)), dim3(BLOCK), 0, s, tasks, m.sample);
-    timer_end(g_timer, "gather", s);
-  }
-  if (m.fillers.total > 0)
-    hipLaunchKernelGGL(k_filler_gradient, dim3(plan_grid(m.fillers)), dim3(BLOCK), 0, s, tasks, m.fillers);
+  if (m.n_tasks <= 0 || m.sample.total <= 0) return;
+  timer_begin(g_timer, "gather", s);
+  hipLaunchKernelGGL(k_sample, dim3(plan_grid(m.sample)), dim3(BLOCK), 0, s, tasks, m.sample);
+  timer_end(g_timer, "gather", s);
 }
 
 void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
@@ -1201,8 +1242,8 @@ void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
 }
 
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.unit_grad.total > 0)
-    hipLaunchKernelGGL(k_unit_gradient, dim3(plan_grid(m.unit_grad)), dim3(BLOCK), 0, s, tasks, m.unit_grad);
+  if (m.gradient.total > 0)
+    hipLaunchKernelGGL(k_gradient, dim3(plan_grid(m.gradient)), dim3(BLOCK), 0, s, tasks, m.gradient);
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
