@@ -98,11 +98,12 @@ struct MetricScalars {
   unsigned int pad;
 };
 
-constexpr int SORT_TILE_SHIFT = 3;  // charges are binned by 8x8-bin tiles
+constexpr int SORT_TILE_SHIFT = 4;  // charges are binned by 16x16-bin tiles
 constexpr int DEP_TILE = 1 << SORT_TILE_SHIFT;
 // A tile's deposit window reaches DEP_HALO bins past the tile: the stencil
-// (one bin) plus two bins of movement since the charges were last sorted.
-constexpr int DEP_HALO = 3;
+// (one bin) plus one bin of movement since the charges were last sorted
+// (charges that moved further take the global-atomic path).
+constexpr int DEP_HALO = 2;
 constexpr int DEP_WIN = DEP_TILE + 2 * DEP_HALO;
 constexpr int DEP_WIN2 = DEP_WIN * DEP_WIN;
 
@@ -158,7 +159,7 @@ struct JobStatic {
   const long long *base_fx;   // [F*G] -smoothed capacity, fixed point
   const int32_t *unit_du;     // unit -> density unit or -1
   const uint32_t *perm;       // deposit order of the charges (members, then fillers)
-  int tiles_x, n_keys;        // 8x8-bin tiles of the charge sort
+  int tiles_x, n_keys;        // DEP_TILE x DEP_TILE-bin tiles of the charge sort
   const uint32_t *key_end;    // [n_keys] end of each tile's charges in deposit order
   // charges in deposit order: coordinate index (unit, or n_units + filler),
   // offset from it, charge, layer = 2 field + (1 for a filler)
@@ -215,7 +216,7 @@ struct Task {
   double *raw_units, *raw_fillers, *density_out, *wa_grad_out;
 };
 
-// Spatial sort of one job's charges by 8x8-bin tile of their stencil centre
+// Spatial sort of one job's charges by sort tile of their stencil centre
 // (counting sort).  The order only steers which charges share a deposit
 // block; the integer density sums do not depend on it.
 struct SortTask {

@@ -49,17 +49,29 @@ constexpr double MULTIPLIER_ALPHA_HIGH = 1.06;     // eplace.rs:129
 constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
 // Iterations between re-sorts of a job's charges for the deposit; the order
 // only affects speed, never the density sums.
-constexpr uint64_t SORT_PERIOD = 8;
+constexpr uint64_t SORT_PERIOD = 4;
 
-__global__ void k_round_targets_raw(int n_units, const double *__restrict__ coords, int W, int H,
-                                    uint32_t *__restrict__ out) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= 2 * n_units) return;
-  const double extent = static_cast<double>(((i & 1) ? H : W) - 1);
-  double r = round(coords[i]);
+// Rounded targets (rounded_coordinate, lib.rs:4566-4568) of up to
+// TARGET_BATCH jobs in one launch; the table travels as the kernel argument.
+struct TargetBatch {
+  const double *coords[TARGET_BATCH];
+  uint32_t *out[TARGET_BATCH];
+  int n[TARGET_BATCH];  // 2 x units
+  int W[TARGET_BATCH], H[TARGET_BATCH];
+  int block_end[TARGET_BATCH];  // prefix sums of the jobs' blocks
+  int count;
+};
+
+__global__ void k_round_targets(TargetBatch b) {
+  int k = 0;
+  while (k + 1 < b.count && (int)blockIdx.x >= b.block_end[k]) ++k;
+  const int i = ((int)blockIdx.x - (k ? b.block_end[k - 1] : 0)) * BLOCK + threadIdx.x;
+  if (i >= b.n[k]) return;
+  const double extent = static_cast<double>(((i & 1) ? b.H[k] : b.W[k]) - 1);
+  double r = round(b.coords[k][i]);
   if (!(r >= 0.0)) r = 0.0;
   if (r > extent) r = extent;
-  out[i] = static_cast<uint32_t>(r);
+  b.out[k][i] = static_cast<uint32_t>(r);
 }
 
 }  // namespace
@@ -112,10 +124,32 @@ void launch_collect_scalars(const int *index, int n, const JobScalars *pool, Job
   if (n > 0) hipLaunchKernelGGL(k_collect_scalars, dim3(n), dim3(64), 0, s, index, pool, out);
 }
 
-void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s) {
-  if (n_units == 0) return;
-  hipLaunchKernelGGL(k_round_targets_raw, dim3((2 * n_units + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, n_units,
-                     coords, W, H, out);
+// Rounds the pending best checkpoints of `jobs` (PlacementCheckpoint::consider
+// stores the rounded targets, eplace.rs:107-120) in launches of TARGET_BATCH.
+void launch_round_targets(const std::vector<Job *> &jobs, hipStream_t s) {
+  TargetBatch b{};
+  int blocks = 0;
+  auto flush = [&] {
+    if (b.count > 0) hipLaunchKernelGGL(k_round_targets, dim3(blocks), dim3(BLOCK), 0, s, b);
+    b = TargetBatch{};
+    blocks = 0;
+  };
+  for (Job *j : jobs) {
+    if (!j->best_src) continue;
+    if (j->U > 0) {
+      const int k = b.count++;
+      b.coords[k] = j->best_src;
+      b.out[k] = j->d_best;
+      b.n[k] = (int)(2 * j->U);
+      b.W[k] = (int)j->W;
+      b.H[k] = (int)j->H;
+      blocks += (int)((2 * j->U + BLOCK - 1) / BLOCK);
+      b.block_end[k] = blocks;
+      if (b.count == TARGET_BATCH) flush();
+    }
+    j->best_src = nullptr;
+  }
+  flush();
 }
 
 // ---------------------------------------------------------------------------
@@ -508,7 +542,9 @@ void consider_best(Job *j, hipStream_t s) {
   const auto sc = j->excess(j->current);
   const bool better = !j->has_best || sc.first < j->best_max || (sc.first == j->best_max && sc.second < j->best_total);
   if (better) {
-    launch_round_targets((int)j->U, j->d_vec[j->ix], (int)j->W, (int)j->H, j->d_best, s);
+    // rounded into d_best by the engine's batched launch once the round's
+    // jobs have advanced (before any later work on the stream)
+    j->best_src = j->d_vec[j->ix];
     j->best_max = sc.first;
     j->best_total = sc.second;
     j->has_best = true;
@@ -554,9 +590,16 @@ void iteration_end(Job *j, bool accepted, hipStream_t s) {
   j->phase = Phase::IterStart;
 }
 
+// Copies the current coordinates over the previous iterate, first rounding
+// a pending best checkpoint that still lives in the overwritten buffer.
+void copy_to_previous(Job *j, hipStream_t s) {
+  if (j->best_src && j->best_src == j->d_vec[j->ixp]) launch_round_targets(std::vector<Job *>{j}, s);
+  HIP(hipMemcpyAsync(j->d_vec[j->ixp], j->d_vec[j->ix], j->C * sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+
 void stationary(Job *j, bool set_l, double l, hipStream_t s) {
   // NumericallyStationary return (global_placement.rs:185-208, 259-271)
-  HIP(hipMemcpyAsync(j->d_vec[j->ixp], j->d_vec[j->ix], j->C * sizeof(double), hipMemcpyDeviceToDevice, s));
+  copy_to_previous(j, s);
   j->momentum = 1.0;
   if (set_l) {
     j->has_lipschitz = true;
@@ -614,7 +657,7 @@ void Engine::advance(Job *j) {
       }
       j->validate_fields(init);
       // DynamicNesterovState::new: previous = coordinates, momentum 1, no L
-      HIP(hipMemcpyAsync(j->d_vec[j->ixp], j->d_vec[j->ix], j->C * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+      copy_to_previous(j, stream_);
       j->momentum = 1.0;
       j->has_lipschitz = false;
       j->lipschitz = 0.0;
@@ -990,6 +1033,7 @@ void Engine::finish(int c) {
       j->phase = Phase::Failed;
     }
   }
+  launch_round_targets(cx.active, stream_);
   g_timer = nullptr;
 }
 

@@ -135,7 +135,10 @@ bool register_clustering(const std::vector<std::pair<uint32_t, std::pair<double,
 // adjust_instance_areas (instance_area.rs:157-372); returns requires_optimizer_reset.
 bool adjust_areas(const std::vector<AreaMember> &members, const std::vector<std::pair<uint32_t, double>> &fillers,
                   std::vector<double> &member_out, std::vector<double> &filler_out);
-void launch_round_targets(int n_units, const double *coords, int W, int H, uint32_t *out, hipStream_t s);
+struct Job;
+// Jobs per launch of the batched best-checkpoint rounding.
+constexpr int TARGET_BATCH = 64;
+void launch_round_targets(const std::vector<Job *> &jobs, hipStream_t s);
 void launch_collect_scalars(const int *index, int n, const JobScalars *pool, JobScalars *out, hipStream_t s);
 
 enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed };
@@ -268,6 +271,7 @@ struct Job {
   EvalScalars current{};
   double last_wa = 0.0, gamma = 8.0, g0 = 8.0;
   bool has_best = false;
+  const double *best_src = nullptr;  // coordinates of a new best, not yet rounded
   double best_max = 0.0, best_total = 0.0;
   bool want_targets = false;
   std::vector<uint32_t> prev_stationary;

@@ -121,8 +121,8 @@ __device__ __forceinline__ double kkt_scaled(double g, double x, double lo, doub
 // ---------------------------------------------------------------------------
 // Density deposit (electrostatic_placement.rs:296-332, 900-916).
 // ---------------------------------------------------------------------------
-// Deposit, one block per 8x8-bin tile of the last charge sort.  The block
-// walks the tile's charges (contiguous in the sorted arrays) and adds each
+// Deposit, one block per sort tile (DEP_TILE x DEP_TILE bins) of the last
+// charge sort.  The block walks the tile's charges (contiguous in the sorted arrays) and adds each
 // stencil cell's int64 quantum into an LDS window covering the tile plus
 // DEP_HALO bins, one layer per (field, members | fillers); it then stores the
 // whole window to the tile's partial with plain stores.  A charge that moved
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
 }
 
 // ---------------------------------------------------------------------------
-// Counting sort of charges by 8x8-bin tile (clear, count, scan, scatter).
+// Counting sort of charges by sort tile (clear, count, scan, scatter).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int charge_key(const JobStatic &a, const double *__restrict__ coords, int i) {
   double x, y;
@@ -475,7 +475,7 @@ __device__ __forceinline__ void sample_field(const Axis &ax, const Axis &ay, At 
   }
 }
 
-// One block per 8x8-bin tile of the charge sort: the potential of the
+// One block per sort tile of the charges: the potential of the
 // tile's window (tile + DEP_HALO) is staged in LDS and every charge of the
 // tile samples its field there (from HBM when it moved out of the window).
 // The sample goes to the charge's input index, so the per-unit and
