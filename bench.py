@@ -167,6 +167,7 @@ def run_serving(stream, precision, max_batch, at_rate: bool, clients: int):
             t.start()
         for t in threads:
             t.join()
+        submitted = time.perf_counter() - t_start
         lat, failed = [], 0
         for i in ids:
             _, rep = srv.wait(i, raise_on_error=False)
@@ -176,6 +177,7 @@ def run_serving(stream, precision, max_batch, at_rate: bool, clients: int):
         stats = srv.stats()
     lat = np.array(lat)
     return {"jobs": len(stream), "clients": clients, "jobs_per_s": len(stream) / makespan, "makespan_s": makespan,
+            "all_submitted_s": submitted,
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "failed": int(failed), "rounds": stats["rounds"], "busy_s": stats["busy_seconds"],
             "admit_s": stats["admit_seconds"], "retire_s": stats["retire_seconds"]}

@@ -77,6 +77,8 @@ struct EvalScalars {
   long long net_fx[MAXF];       // sum rho, fixed point
   unsigned long long kkt_bits;  // max metric KKT component (bits of a double >= 0)
   unsigned int nonfinite;       // any non-finite gradient entry
+  unsigned int moved;           // charges deposited outside their tile's window
+  unsigned int moved_acc;       // (counted by k_deposit, moved over by k_finalize)
   unsigned int pad;
 };
 

@@ -49,7 +49,10 @@ constexpr double MULTIPLIER_ALPHA_HIGH = 1.06;     // eplace.rs:129
 constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
 // Iterations between re-sorts of a job's charges for the deposit; the order
 // only affects speed, never the density sums.
-constexpr uint64_t SORT_PERIOD = 4;
+constexpr uint64_t SORT_PERIOD = 8;
+// ... or at the next iteration once more than 1/MOVED_RESORT of the charges
+// fell outside their tile's window (those take global atomics)
+constexpr size_t MOVED_RESORT = 64;
 
 // Rounded targets (rounded_coordinate, lib.rs:4566-4568) of up to
 // TARGET_BATCH jobs in one launch; the table travels as the kernel argument.
@@ -380,7 +383,8 @@ void Engine::plan(Job *j) {
         j->gamma = j->adaptive_gamma(j->current);
         j->iterations += 1;
         j->local_iterations += 1;
-        j->want_sort = j->iterations % SORT_PERIOD == 0;
+        // periodic re-sort, or sooner when many charges left their windows
+        j->want_sort = j->iterations % SORT_PERIOD == 0 || (size_t)j->moved * MOVED_RESORT > j->M + j->Fl;
         if (!std::isfinite(j->momentum) || j->momentum < 1.0) fail(TEXO_GP_ERR_INVALID_MOMENTUM);
         if (j->has_lipschitz && (!std::isfinite(j->lipschitz) || j->lipschitz <= 0.0))
           fail(TEXO_GP_ERR_INVALID_LIPSCHITZ);
@@ -1025,6 +1029,9 @@ void Engine::finish(int c) {
     Job *j = cx.active[k];
     j->release_image();  // the round ran after the job's upload
     j->h_scalars = cx.h_compact + k;
+    j->moved = 0;
+    for (const Job::EvalReq &r : j->evals)
+      if (!r.skip) j->moved = std::max(j->moved, j->h_scalars->eval[r.scalar_slot].moved);
     try {
       advance(j);
     } catch (const Failure &f) {

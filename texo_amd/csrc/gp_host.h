@@ -299,6 +299,7 @@ struct Job {
   };
   std::vector<EvalReq> evals;
   bool want_precond = false, want_extrapolate = false, want_candidate = false, want_sort = false;
+  uint32_t moved = 0;  // most charges any density evaluation of the last round deposited outside their window
   std::vector<double> targets_out;  // unused placeholder for symmetry
 
   // schedule helpers

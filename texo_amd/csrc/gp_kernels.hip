@@ -157,6 +157,7 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
   __syncthreads();
   const int x0 = (key % a.tiles_x) * DEP_TILE - DEP_HALO, y0 = (key / a.tiles_x) * DEP_TILE - DEP_HALO;
   const size_t G = (size_t)a.W * (size_t)a.H;
+  unsigned int moved = 0;
   for (uint32_t i = begin + threadIdx.x; i < end; i += BLOCK) {
     // a filler's offset is +0.0, which leaves its coordinate unchanged
     const uint32_t ci = a.s_ci[i];
@@ -184,6 +185,7 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
         }
       }
     } else {
+      moved += 1;
       unsigned long long *g =
           reinterpret_cast<unsigned long long *>(((layer & 1) ? t.acc_f : t.acc_m) + (size_t)(layer >> 1) * G);
 #pragma unroll
@@ -196,7 +198,15 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
       }
     }
   }
+  // charges outside the window (moved since the sort): one count per block
+  __shared__ unsigned int s_moved[BLOCK / 64];
+  for (int off = 32; off > 0; off >>= 1) moved += __shfl_xor(moved, off, 64);
+  if ((threadIdx.x & 63) == 0) s_moved[threadIdx.x >> 6] = moved;
   __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int n = s_moved[0] + s_moved[1] + s_moved[2] + s_moved[3];
+    if (n) atomicAdd(&t.scalars->moved_acc, n);
+  }
   long long *out = t.partial + (size_t)key * cells;
   for (int idx = threadIdx.x; idx < cells; idx += BLOCK) out[idx] = static_cast<long long>(win[idx]);
   if (threadIdx.x == 0) t.tile_flag[key] = 1u;
@@ -929,6 +939,8 @@ __device__ void finalize_task(const Task &t) {
     }
     sc->kkt_bits = static_cast<unsigned long long>(__double_as_longlong(fmax(r[NS], r[NS + 2])));
     sc->nonfinite = (r[NS + 1] > 0.0 || r[NS + 3] > 0.0) ? 1u : 0u;
+    sc->moved = sc->moved_acc;
+    sc->moved_acc = 0u;
   }
 }
 
