@@ -36,6 +36,24 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GP iters/s at 84k cells, 256² bins; jobs/s + p99 latency, MFMA vs FP32 plain"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)
+# on the batched workload, scaled by the copy-kernel calibration; written by
+# the profiling recipe next to the summaries it comes from
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+
+
+def measured_traffic(kernel: str, batch: int):
+    """HBM bytes per launch of `kernel` from the committed counter runs, when
+    they were taken on this batch size (else None)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or t.get("batch") != batch:
+        return None
+    return k["fetch_bytes"] + k["write_bytes"]
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 FP16_PEAK_TFLOPS = 2500.0
@@ -64,10 +82,12 @@ def algorithmic_bytes(kernel: str, sizes: dict, problem) -> float:
         # coordinates 16; per net: start 4 + weight 8 + id 4
         return pins * 36 + U * 16 + nets * 16
     if kernel == "gather":
-        # potential F*G FP32; per density unit: unit id 4, member range 4,
-        # coordinates 16, scaled field gradient written 16 per field; per
-        # member: offset 16, charge 8, field 1
-        return F * G * 4 + DU * (24 + 16 * F) + M * 25
+        # k_gradient: per unit pin range 4 + arc split 4 + density unit 4 +
+        # coordinates 16 + gradient written 16; per movable pin its WA
+        # derivative 16; per density unit its member range 4; per member field
+        # 1, charge 8, offset 16; the potential F*G FP32 once; per filler field
+        # 1, charge 8, coordinates 16, gradient written 16
+        return U * 44 + pins * 16 + DU * 4 + M * 25 + F * G * 4 + Fl * 41
     raise ValueError(kernel)
 
 
@@ -188,9 +208,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--batch", type=int, default=8, help="multi-start jobs in the batched form")
+    ap.add_argument("--batch", type=int, default=16, help="multi-start jobs in the batched form")
     ap.add_argument("--precision", default="fp16_scaled", choices=["fp32", "bf16x3", "fp16_scaled"])
-    ap.add_argument("--roofline-kernel", default="deposit", choices=["deposit", "wa", "gather", "dct"])
+    ap.add_argument("--roofline-kernel", default="wa", choices=["deposit", "wa", "gather", "dct"])
     ap.add_argument("--serving-jobs", type=int, default=256)
     ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--serving-clients", type=int, default=4, help="threads submitting the serving stream")
@@ -247,7 +267,7 @@ def main():
         nbytes = args.batch * algorithmic_bytes(k, sizes, base)
         achieved = nbytes / avg_s / 1e9 if avg_s > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": k,
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(k, args.batch), "kernel": k,
                     "algorithmic_per_launch": nbytes, "avg_launch_ms": batched["kernel_avg_ms"],
                     "jobs_per_launch": args.batch}
 

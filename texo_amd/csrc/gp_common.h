@@ -198,7 +198,6 @@ struct Task {
   // working slot
   long long *acc_m, *acc_f;    // [F*G]
   float *rho, *phi, *t0, *t1;  // [F*G]
-  double2 *samp;               // [M+Fl] field sample (d/dx, d/dy of the potential) per charge
   double *field_coef;          // [2*MAXF] lambda (1 + beta E/E0) per field, then E per field
   long long *partial;          // [n_keys][2F][DEP_WIN2] per-tile deposit windows
   unsigned int *amax;          // [4][MAXF] max |input| of each transform stage per field (float bits)
@@ -265,12 +264,12 @@ struct OptTask {
 };
 
 struct Maps {
-  BlockPlan deposit, convert, sample, wa, gradient;
+  BlockPlan deposit, convert, wa, gradient;
   int n_tasks;
 };
 
 void launch_density(const Task *tasks, const Maps &m, hipStream_t s);
-void launch_gather(const Task *tasks, const Maps &m, hipStream_t s);  // coefficients and field samples
+void launch_gather(const Task *tasks, const Maps &m, hipStream_t s);  // field coefficients
 void launch_wa(const Task *tasks, const Maps &m, hipStream_t s);
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s);
 

@@ -838,7 +838,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   std::vector<int> l_sort, l_pre, l_cand;
   Plan p_sort_c, p_sort_k, p_pre, p_ext, p_cand;
   std::vector<Task> tasks;
-  Plan p_dep, p_conv, p_sample, p_wa, p_gr;
+  Plan p_dep, p_conv, p_wa, p_gr;
   struct Group {
     std::vector<int2> h, w;
   };
@@ -913,22 +913,16 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
       const int ti = (int)tasks.size();
       tasks.push_back(t);
       const bool density = j->F > 0 && !r.skip;
-      // a skipped task reuses the slot's field and samples (same point)
+      // a skipped task reuses the slot's field (same point)
       if (density) {
         p_dep.add((size_t)j->n_keys, 1);
         p_conv.add((size_t)j->F * j->G, BLOCK);
-        p_sample.add((size_t)j->n_keys, 1);
         Group &g = groups[std::make_tuple((int)j->W, (int)j->H, (int)j->opt.precision)];
         add_blocks(g.h, ti, (size_t)j->F * j->H, 32);
         add_blocks(g.w, ti, (size_t)j->F * j->W, 32);
       } else {
         p_dep.none();
         p_conv.none();
-        // one sample block still computes the field coefficients
-        if (j->F > 0)
-          p_sample.add(1, 1);
-        else
-          p_sample.none();
       }
       p_wa.add((size_t)j->nb_wa, 1);
       p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
@@ -941,7 +935,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
                o_ppre = append(buf, p_pre.start), o_pext = append(buf, p_ext.start),
                o_pcand = append(buf, p_cand.start);
   const size_t o_pdep = append(buf, p_dep.start), o_pconv = append(buf, p_conv.start),
-               o_pgu = append(buf, p_sample.start), o_pwa = append(buf, p_wa.start), o_pgr = append(buf, p_gr.start);
+               o_pwa = append(buf, p_wa.start), o_pgr = append(buf, p_gr.start);
   std::vector<std::pair<size_t, size_t>> o_groups;
   for (auto &kv : groups) o_groups.push_back({append(buf, kv.second.h), append(buf, kv.second.w)});
   std::vector<int> pool_index;
@@ -977,7 +971,6 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   Maps mp{};
   mp.deposit = p_dep.device(d_stage_, o_pdep);
   mp.convert = p_conv.device(d_stage_, o_pconv);
-  mp.sample = p_sample.device(d_stage_, o_pgu);
   mp.wa = p_wa.device(d_stage_, o_pwa);
   mp.gradient = p_gr.device(d_stage_, o_pgr);
   mp.n_tasks = (int)tasks.size();
@@ -1065,24 +1058,22 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   t.wa_grad_out = j->d_wa_grad;
   if (density_out) t.density_out = j->d_density_out;
   tasks.push_back(t);
-  Plan p_dep, p_conv, p_sample, p_wa, p_gr;
+  Plan p_dep, p_conv, p_wa, p_gr;
   std::vector<int2> gh, gw;
   if (j->F > 0) {
     p_dep.add((size_t)j->n_keys, 1);
     p_conv.add((size_t)j->F * j->G, BLOCK);
     add_blocks(gh, 0, (size_t)j->F * j->H, 32);
     add_blocks(gw, 0, (size_t)j->F * j->W, 32);
-    p_sample.add((size_t)j->n_keys, 1);
   } else {
     p_dep.none();
     p_conv.none();
-    p_sample.none();
   }
   p_wa.add((size_t)j->nb_wa, 1);
   p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
   std::vector<char> buf;
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),
-               o_gu = append(buf, p_sample.start), o_wa = append(buf, p_wa.start), o_gr = append(buf, p_gr.start),
+               o_wa = append(buf, p_wa.start), o_gr = append(buf, p_gr.start),
                o_h = append(buf, gh), o_w = append(buf, gw);
   HIP(hipStreamSynchronize(stream_));
   ensure_stage(aux_, buf.size());
@@ -1093,7 +1084,6 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   Maps mp{};
   mp.deposit = p_dep.device(d_stage_, o_dep);
   mp.convert = p_conv.device(d_stage_, o_conv);
-  mp.sample = p_sample.device(d_stage_, o_gu);
   mp.wa = p_wa.device(d_stage_, o_wa);
   mp.gradient = p_gr.device(d_stage_, o_gr);
   mp.n_tasks = 1;

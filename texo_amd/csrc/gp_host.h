@@ -216,7 +216,7 @@ struct Job {
   struct Slot {
     long long *acc_m = nullptr, *acc_f = nullptr;
     float *rho = nullptr, *phi = nullptr, *t0 = nullptr, *t1 = nullptr;
-    double2 *samp = nullptr, *dpin = nullptr;
+    double2 *dpin = nullptr;
     double *field_coef = nullptr;
     long long *partial = nullptr;
     unsigned int *amax = nullptr;

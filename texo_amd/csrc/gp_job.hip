@@ -780,7 +780,6 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
     arena.reserve(sl.phi, FG);
     arena.reserve(sl.t0, FG);
     arena.reserve(sl.t1, FG);
-    arena.reserve(sl.samp, n);
     arena.reserve(sl.field_coef, 2 * MAXF);
     arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
     arena.reserve(sl.tile_flag, (size_t)n_keys);
@@ -892,7 +891,6 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.phi = w.phi;
   t.t0 = w.t0;
   t.t1 = w.t1;
-  t.samp = w.samp;
   t.field_coef = w.field_coef;
   t.partial = w.partial;
   t.amax = w.amax;

@@ -485,51 +485,9 @@ __device__ __forceinline__ void sample_field(const Axis &ax, const Axis &ay, At 
   }
 }
 
-// One block per sort tile of the charges: the potential of the
-// tile's window (tile + DEP_HALO) is staged in LDS and every charge of the
-// tile samples its field there (from HBM when it moved out of the window).
-// The sample goes to the charge's input index, so the per-unit and
-// per-filler sums below keep the reference's order.
-// Block 0 of each task also sums the energy partials into the field
-// coefficients (a task whose slot already holds its field has only that
-// block: its coefficients still follow the current multipliers).
-__global__ void __launch_bounds__(BLOCK) k_sample(const Task *__restrict__ tasks, BlockPlan plan) {
-  __shared__ float win[MAXF * DEP_WIN2];
-  int2 bm;
-  if (!plan_block(plan, bm)) return;
-  const Task &t = tasks[bm.x];
-  const JobStatic &a = t.j;
-  const int key = bm.y;
-  if (key == 0) task_coefficients(t);
-  if (t.skip_density) return;
-  const uint32_t begin = key > 0 ? a.key_end[key - 1] : 0u, end = a.key_end[key];
-  if (begin == end) return;
-  const int x0 = (key % a.tiles_x) * DEP_TILE - DEP_HALO, y0 = (key / a.tiles_x) * DEP_TILE - DEP_HALO;
-  const size_t G = (size_t)a.W * (size_t)a.H;
-  for (int idx = threadIdx.x; idx < a.F * DEP_WIN2; idx += BLOCK) {
-    const int f = idx / DEP_WIN2, r = (idx - f * DEP_WIN2) / DEP_WIN, c = idx % DEP_WIN;
-    const int gx = x0 + c, gy = y0 + r;
-    win[idx] = (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) ? t.phi[f * G + (size_t)gy * a.W + gx] : 0.f;
-  }
-  __syncthreads();
-  for (uint32_t i = begin + threadIdx.x; i < end; i += BLOCK) {
-    // a filler's offset is +0.0, which leaves its coordinate unchanged
-    const uint32_t ci = a.s_ci[i];
-    const double2 off = a.s_off[i];
-    const int f = a.s_layer[i] >> 1;
-    Axis ax, ay;
-    bspline_axis(a.W, t.coords[2 * (size_t)ci] + off.x, ax);
-    bspline_axis(a.H, t.coords[2 * (size_t)ci + 1] + off.y, ay);
-    double gx, gy;
-    if (ax.i0 >= x0 && ax.i2 < x0 + DEP_WIN && ay.i0 >= y0 && ay.i2 < y0 + DEP_WIN) {
-      const float *w = win + f * DEP_WIN2;
-      sample_field(ax, ay, [&](int x, int y) { return w[(y - y0) * DEP_WIN + (x - x0)]; }, gx, gy);
-    } else {
-      const float *phi = t.phi + f * G;
-      sample_field(ax, ay, [&](int x, int y) { return phi[(size_t)y * a.W + x]; }, gx, gy);
-    }
-    t.samp[a.perm[i]] = make_double2(gx, gy);
-  }
+// One block per task: the field coefficients of the evaluation.
+__global__ void __launch_bounds__(BLOCK) k_coefficients(const Task *__restrict__ tasks) {
+  task_coefficients(tasks[blockIdx.x]);
 }
 
 // Thread per filler, input order: combined filler gradient, force and raw
@@ -546,8 +504,13 @@ __device__ void filler_gradient_block(const Task &t, int lb) {
     const size_t c = 2 * ((size_t)a.n_units + k);
     const int f = a.fl_field[k];
     const double q = a.fl_charge[k];
-    const double2 g = t.samp[(size_t)a.n_members + k];
-    const double gx = g.x, gy = g.y;
+    // the field at the filler (a filler's offset is +0.0)
+    Axis ax, ay;
+    bspline_axis(a.W, t.coords[c] + 0.0, ax);
+    bspline_axis(a.H, t.coords[c + 1] + 0.0, ay);
+    const float *phi = t.phi + (size_t)f * ((size_t)a.W * a.H);
+    double gx, gy;
+    sample_field(ax, ay, [&](int x, int y) { return phi[(size_t)y * a.W + x]; }, gx, gy);
     const double rx = q * gx, ry = q * gy;
 #pragma unroll
     for (int p = 0; p < MAXF; ++p) {
@@ -846,10 +809,20 @@ __device__ void unit_gradient_block(const Task &t, int lb) {
       double gxf[MAXF], gyf[MAXF];
 #pragma unroll
       for (int f = 0; f < MAXF; ++f) gxf[f] = gyf[f] = 0.0;
+      const double ux = t.coords[2 * (size_t)u], uy = t.coords[2 * (size_t)u + 1];
+      const size_t G = (size_t)a.W * a.H;
       for (uint32_t m = a.du_mstart[du]; m < a.du_mstart[du + 1]; ++m) {
         const int f = a.m_field[m];
         const double q = a.m_charge[m];
-        const double2 g = t.samp[m];
+        // the field at the member (sample_quadratic_bspline_with_gradient,
+        // electrostatic_placement.rs:919-941)
+        const double2 off = a.m_off[m];
+        Axis ax, ay;
+        bspline_axis(a.W, ux + off.x, ax);
+        bspline_axis(a.H, uy + off.y, ay);
+        const float *phi = t.phi + (size_t)f * G;
+        double2 g;
+        sample_field(ax, ay, [&](int x, int y) { return phi[(size_t)y * a.W + x]; }, g.x, g.y);
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
           if (k == f) {
@@ -1240,10 +1213,7 @@ void launch_sort(const SortTask *tasks, const SortMaps &m, hipStream_t s) {
 }
 
 void launch_gather(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.n_tasks <= 0 || m.sample.total <= 0) return;
-  timer_begin(g_timer, "gather", s);
-  hipLaunchKernelGGL(k_sample, dim3(plan_grid(m.sample)), dim3(BLOCK), 0, s, tasks, m.sample);
-  timer_end(g_timer, "gather", s);
+  if (m.n_tasks > 0) hipLaunchKernelGGL(k_coefficients, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
 void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
@@ -1254,8 +1224,11 @@ void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
 }
 
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
-  if (m.gradient.total > 0)
+  if (m.gradient.total > 0) {
+    timer_begin(g_timer, "gather", s);
     hipLaunchKernelGGL(k_gradient, dim3(plan_grid(m.gradient)), dim3(BLOCK), 0, s, tasks, m.gradient);
+    timer_end(g_timer, "gather", s);
+  }
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
