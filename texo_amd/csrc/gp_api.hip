@@ -248,15 +248,17 @@ int texo_gp_session_poisson(texo_gp_session *s, const double *density, double *p
     }
     if (J.p_slot == 0) J.p_valid = false;
     HIP(hipMemcpyAsync(J.slot[0].rho, rho.data(), FG * sizeof(float), hipMemcpyHostToDevice, st));
-    // input range of the FP16-scaled transform (k_convert records it in a full evaluation)
-    std::vector<unsigned int> amax(4 * texo::MAXF, 0u);
+    // input range of the FP16-scaled transform, as the k_convert blocks of a
+    // full evaluation record it: max |rho| per block of BLOCK cells and field
+    std::vector<unsigned int> amax((size_t)J.nb_conv * texo::MAXF, 0u);
     for (size_t i = 0; i < FG; ++i) {
       const float a = std::fabs(rho[i]);
       unsigned int bits;
       std::memcpy(&bits, &a, 4);
-      amax[i / J.G] = std::max(amax[i / J.G], bits);
+      unsigned int &m = amax[(i / texo::BLOCK) * texo::MAXF + i / J.G];
+      m = std::max(m, bits);
     }
-    HIP(hipMemcpyAsync(J.slot[0].amax, amax.data(), amax.size() * 4, hipMemcpyHostToDevice, st));
+    HIP(hipMemcpyAsync(J.slot[0].amax_part, amax.data(), amax.size() * 4, hipMemcpyHostToDevice, st));
     HIP(hipStreamSynchronize(st));
     texo::Task t = J.task(0, J.d_eval_coords, J.d_grad[0], nullptr, 1.0, false, &J.d_scalars->eval[2]);
     std::vector<int2> gh, gw;

@@ -200,11 +200,15 @@ struct Task {
   float *rho, *phi, *t0, *t1;  // [F*G]
   double *field_coef;          // [2*MAXF] lambda (1 + beta E/E0) per field, then E per field
   long long *partial;          // [n_keys][2F][DEP_WIN2] per-tile deposit windows
-  unsigned int *amax;          // [4][MAXF] max |input| of each transform stage per field (float bits)
+  // max |value| (float bits) per producing block, for the FP16-scaled
+  // transform's input ranges: k_convert blocks [nb_conv][MAXF], then one
+  // per 32x32 output tile of transform stages 0, 1, 2 [3][nb_tile]
+  unsigned int *amax_part;
+  long long *part_conv;        // [nb_conv][2*MAXF] over / net fixed-point sums per k_convert block
   uint32_t *tile_flag;         // [n_keys] 1 where the tile's window was written
   double2 *dpin;               // [movable pins] weighted WA derivative, grouped by unit
   double *part_wa, *part_energy, *part_gf, *part_ug;
-  int nb_wa, nb_energy, nb_gf, nb_ug;
+  int nb_wa, nb_energy, nb_gf, nb_ug, nb_conv, nb_tile;
   // request
   const double *coords;
   double *grad;

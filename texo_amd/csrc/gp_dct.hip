@@ -69,7 +69,7 @@ __device__ __forceinline__ float wave_max_f(float v) {
 struct Stage {
   const float *A;
   float *out;
-  unsigned int *amax_out;  // max |out| per field for the next stage (fast path), or null
+  unsigned int *amax_out;  // max |out| per output tile, for the next stage (fast path), or null
   int M, K, N, R;  // rows of this task, inner, columns, rows per field
   bool transposed;
   const float *inv_lambda;
@@ -84,7 +84,7 @@ __device__ __forceinline__ Stage stage_of(const Task &t) {
   s.W = W;
   s.inv_lambda = nullptr;
   s.energy = nullptr;
-  s.amax_out = S < 3 ? t.amax + (S + 1) * MAXF : nullptr;
+  s.amax_out = S < 3 ? t.amax_part + (size_t)t.nb_conv * MAXF + (size_t)S * t.nb_tile : nullptr;
   if constexpr (S == 0) {
     s.A = t.rho;
     s.out = t.t0;
@@ -208,10 +208,7 @@ __device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][3
     mx = wave_max_f(mx);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = __float_as_uint(mx);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      const unsigned int b = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
-      if (b) atomicMax(e.amax_out + m0 / e.R, b);
-    }
+    if (threadIdx.x == 0) e.amax_out[(size_t)local_rb * gridDim.x + blockIdx.x] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
   }
   if (!e.energy) return;
   __shared__ double se[SPLIT][MAXF];
@@ -227,6 +224,36 @@ __device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][3
     const int k = threadIdx.x;
     e.energy[((size_t)local_rb * gridDim.x + blockIdx.x) * MAXF + k] = ((se[0][k] + se[1][k]) + se[2][k]) + se[3][k];
   }
+}
+
+// max |input| (float bits) of field f of stage S's input, from the partials
+// its producer wrote: the k_convert blocks covering field f for stage 0,
+// else the 32x32 output tiles of field f of stage S-1.  Block-wide.
+template <int S>
+__device__ unsigned int input_amax(const Task &t, int f) {
+  const int W = t.j.W, H = t.j.H;
+  const unsigned int *p;
+  int n, stride;
+  if constexpr (S == 0) {
+    const size_t G = (size_t)W * H;
+    const int lo = (int)(f * G / BLOCK), hi = (int)(((f + 1) * G + BLOCK - 1) / BLOCK);
+    p = t.amax_part + (size_t)lo * MAXF + f;
+    n = hi - lo;
+    stride = MAXF;
+  } else {
+    // producer stage S-1: rows per field R, columns N (both multiples of 32)
+    const int R = (S - 1) == 0 ? H : W, N = (S - 1) == 0 ? W : H;
+    n = (R / 32) * (N / 32);
+    p = t.amax_part + (size_t)t.nb_conv * MAXF + (size_t)(S - 1) * t.nb_tile + (size_t)f * n;
+    stride = 1;
+  }
+  unsigned int m = 0u;
+  for (int i = threadIdx.x; i < n; i += FAST_THREADS) m = max(m, p[(size_t)i * stride]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, off, 64));
+  __shared__ unsigned int sm[FAST_THREADS / 64];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  return max(max(sm[0], sm[1]), max(sm[2], sm[3]));
 }
 
 // BF16x3 split-precision MFMA.  A (FP32 activations) is split on the fly
@@ -319,8 +346,8 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
   const int q = lane >> 4, r = lane & 15;
   const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
   const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
-  // max |A| over the 32 rows' field, recorded by the producer of A
-  const float sa = pow2_scale(__uint_as_float(t.amax[S * MAXF + m0 / e.R]));
+  // max |A| over the 32 rows' field, from the partials of the producer of A
+  const float sa = pow2_scale(__uint_as_float(input_amax<S>(t, m0 / e.R)));
   const float unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
   const size_t k4 = (size_t)K / 4;
   const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;

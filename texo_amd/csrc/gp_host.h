@@ -219,7 +219,8 @@ struct Job {
     double2 *dpin = nullptr;
     double *field_coef = nullptr;
     long long *partial = nullptr;
-    unsigned int *amax = nullptr;
+    unsigned int *amax_part = nullptr;
+    long long *part_conv = nullptr;
     uint32_t *tile_flag = nullptr;
     double *part_wa = nullptr, *part_energy = nullptr, *part_gf = nullptr, *part_ug = nullptr;
   } slot[2];
@@ -233,7 +234,7 @@ struct Job {
   double *d_s_q = nullptr;
   uint8_t *d_s_layer = nullptr;
   int tiles_x = 0, n_keys = 0;
-  int nb_wa = 0, nb_energy = 0, nb_gf = 0, nb_ug = 0;
+  int nb_wa = 0, nb_energy = 0, nb_gf = 0, nb_ug = 0, nb_conv = 0, nb_tile = 0;
 
   // Allocates the arena (from `pool` when given) and enqueues the image copy
   // and the accumulator clear on `s`; the image stays alive until

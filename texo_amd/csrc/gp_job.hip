@@ -721,6 +721,8 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   nb_energy = F > 0 ? poisson_energy_blocks((int)W, (int)H, F) : 0;
   nb_gf = (int)blocks(Fl, BLOCK);
   nb_ug = (int)blocks(U, BLOCK);
+  nb_conv = (int)blocks(FG, BLOCK);
+  nb_tile = (int)std::max<size_t>(1, FG / 1024);
   tiles_x = (int)((W + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
   n_keys = tiles_x * (int)((H + (1u << SORT_TILE_SHIFT) - 1) >> SORT_TILE_SHIFT);
 
@@ -783,7 +785,8 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
     arena.reserve(sl.field_coef, 2 * MAXF);
     arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
     arena.reserve(sl.tile_flag, (size_t)n_keys);
-    arena.reserve(sl.amax, 4 * MAXF);
+    arena.reserve(sl.amax_part, (size_t)nb_conv * MAXF + 3 * (size_t)nb_tile);
+    arena.reserve(sl.part_conv, (size_t)nb_conv * 2 * MAXF);
     arena.reserve(sl.dpin, u_pins.size());
     arena.reserve(sl.part_wa, (size_t)nb_wa);
     arena.reserve(sl.part_energy, (size_t)std::max(nb_energy, 1) * MAXF);
@@ -893,7 +896,10 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.t1 = w.t1;
   t.field_coef = w.field_coef;
   t.partial = w.partial;
-  t.amax = w.amax;
+  t.amax_part = w.amax_part;
+  t.part_conv = w.part_conv;
+  t.nb_conv = F > 0 ? nb_conv : 0;
+  t.nb_tile = nb_tile;
   t.tile_flag = w.tile_flag;
   t.dpin = w.dpin;
   t.part_wa = w.part_wa;

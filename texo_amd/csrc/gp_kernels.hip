@@ -138,15 +138,6 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   const int key = bm.y;
-  if (key == 0 && threadIdx.x < 4 * MAXF) {
-    // the task's int64 sums and transform maxima restart (k_convert and the
-    // transform stages add to them)
-    if (threadIdx.x < MAXF) {
-      t.scalars->over_fx[threadIdx.x] = 0;
-      t.scalars->net_fx[threadIdx.x] = 0;
-    }
-    t.amax[threadIdx.x] = 0u;
-  }
   const uint32_t begin = key > 0 ? a.key_end[key - 1] : 0u, end = a.key_end[key];
   if (begin == end) {
     if (threadIdx.x == 0) t.tile_flag[key] = 0u;
@@ -433,14 +424,14 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
     }
   }
   __syncthreads();
-  if (threadIdx.x < a.F) {
+  // plain per-block partials (k_finalize sums them; the transform's first
+  // stage takes the maximum of its field's blocks)
+  if (threadIdx.x < MAXF) {
     const int k = threadIdx.x;
-    const long long o = s_over[0][k] + s_over[1][k] + s_over[2][k] + s_over[3][k];
-    const long long n = s_net[0][k] + s_net[1][k] + s_net[2][k] + s_net[3][k];
-    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->over_fx[k]), static_cast<unsigned long long>(o));
-    atomicAdd(reinterpret_cast<unsigned long long *>(&t.scalars->net_fx[k]), static_cast<unsigned long long>(n));
-    const unsigned int mx = max(max(s_max[0][k], s_max[1][k]), max(s_max[2][k], s_max[3][k]));
-    if (mx) atomicMax(&t.amax[k], mx);
+    long long *pc = t.part_conv + (size_t)bm.y * 2 * MAXF;
+    pc[k] = s_over[0][k] + s_over[1][k] + s_over[2][k] + s_over[3][k];
+    pc[MAXF + k] = s_net[0][k] + s_net[1][k] + s_net[2][k] + s_net[3][k];
+    t.amax_part[(size_t)bm.y * MAXF + k] = max(max(s_max[0][k], s_max[1][k]), max(s_max[2][k], s_max[3][k]));
   }
 }
 
@@ -898,6 +889,22 @@ __device__ void finalize_task(const Task &t) {
   for (int k = 0; k < NS + NM; ++k)
     r[k] = k < NS ? ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3]
                   : fmax(fmax(sh[k][0], sh[k][1]), fmax(sh[k][2], sh[k][3]));
+  // fixed-point density sums of the k_convert blocks (exact in any order)
+  long long cv[2 * MAXF];
+#pragma unroll
+  for (int k = 0; k < 2 * MAXF; ++k) cv[k] = 0;
+  for (int b = threadIdx.x; b < t.nb_conv; b += BLOCK) {
+    const long long *pc = t.part_conv + (size_t)b * 2 * MAXF;
+#pragma unroll
+    for (int k = 0; k < 2 * MAXF; ++k) cv[k] += pc[k];
+  }
+  __shared__ long long scv[2 * MAXF][BLOCK / 64];
+#pragma unroll
+  for (int k = 0; k < 2 * MAXF; ++k) {
+    for (int off = 32; off > 0; off >>= 1) cv[k] += __shfl_xor(cv[k], off, 64);
+    if (lane == 0) scv[k][wave] = cv[k];
+  }
+  __syncthreads();
   // r: wa value, unit sums (wa l1, force l1 x MAXF, raw l1 x MAXF), filler
   // sums (force l1 x MAXF, raw l1 x MAXF), then unit / filler kkt and flag
   if (threadIdx.x == 0) {
@@ -914,6 +921,15 @@ __device__ void finalize_task(const Task &t) {
     sc->nonfinite = (r[NS + 1] > 0.0 || r[NS + 3] > 0.0) ? 1u : 0u;
     sc->moved = sc->moved_acc;
     sc->moved_acc = 0u;
+    // with no density pass this round the slot's partials are those of the
+    // same point (the sums then repeat)
+    if (t.nb_conv > 0) {
+#pragma unroll
+      for (int k = 0; k < MAXF; ++k) {
+        sc->over_fx[k] = scv[k][0] + scv[k][1] + scv[k][2] + scv[k][3];
+        sc->net_fx[k] = scv[MAXF + k][0] + scv[MAXF + k][1] + scv[MAXF + k][2] + scv[MAXF + k][3];
+      }
+    }
   }
 }
 
