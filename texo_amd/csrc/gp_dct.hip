@@ -8,7 +8,7 @@
 //   2. St'_f = (Rt_f C_H^T) .* invLambda^T rows F*W, K = N = H, spectral epilogue
 //   3. T_f   = (St'_f C_H)^T               rows F*W, K = N = H, transposed store
 //   4. phi_f = T_f C_W                     rows F*H, K = N = W
-// blockIdx.y walks a map of (task, 32-row block); a tile never straddles two
+// Each tile's row block indexes a map of (task, 32-row block); a tile never straddles two
 // tasks.  The electrostatic energy 1/2 <rho_bar, phi> equals
 // 1/2 sum invLambda S^2 by Parseval (orthonormal basis); it is accumulated in
 // FP64 in the epilogue of step 2.  The DC coefficient has invLambda = 0, so
@@ -168,12 +168,30 @@ __device__ __forceinline__ void store4(const Stage &e, int row0, int col, float4
   }
 }
 
-__device__ __forceinline__ void write_energy(const Stage &e, int local_rb, double (&energy)[MAXF]) {
+// Tiles of one stage launch: gx column tiles x gy row blocks (map entries),
+// launched as a 1-D grid of XCDS * per blocks.  The hardware deals blocks
+// round-robin to the XCDs; the renumbering gives each XCD a contiguous run
+// of tiles, so the column tiles of one row block share an L2 and its input
+// rows are fetched from HBM once instead of once per XCD.
+struct TileGrid {
+  int gx, gy, per;
+};
+
+__device__ __forceinline__ bool tile_of(const TileGrid &g, int &bx, int &by) {
+  const int b = (int)blockIdx.x;
+  const int lb = (b % XCDS) * g.per + b / XCDS;
+  if (lb >= g.gx * g.gy) return false;
+  by = lb / g.gx;
+  bx = lb - by * g.gx;
+  return true;
+}
+
+__device__ __forceinline__ void write_energy(const Stage &e, int local_rb, int bx, int gx, double (&energy)[MAXF]) {
   if (!e.energy) return;
 #pragma unroll
   for (int k = 0; k < MAXF; ++k) energy[k] = wave_sum(energy[k]);
   if ((threadIdx.x & 63) == 0) {
-    double *p = e.energy + ((size_t)local_rb * gridDim.x + blockIdx.x) * MAXF;
+    double *p = e.energy + ((size_t)local_rb * gx + bx) * MAXF;
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) p[k] = energy[k];
   }
@@ -190,7 +208,8 @@ constexpr int FAST_THREADS = 64 * SPLIT;
 
 // Sums the per-wave partial tiles red[w][32][33] and runs the epilogue:
 // thread t owns column t % 32 and rows 4 (t / 32) .. 4 (t / 32) + 3.
-__device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][33], int m0, int n0, int local_rb) {
+__device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][33], int m0, int n0, int local_rb,
+                                              int bx, int gx) {
   const int col = threadIdx.x & 31, rg = threadIdx.x >> 5;
   float4_t v;
 #pragma unroll
@@ -208,7 +227,7 @@ __device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][3
     mx = wave_max_f(mx);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = __float_as_uint(mx);
     __syncthreads();
-    if (threadIdx.x == 0) e.amax_out[(size_t)local_rb * gridDim.x + blockIdx.x] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
+    if (threadIdx.x == 0) e.amax_out[(size_t)local_rb * gx + bx] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
   }
   if (!e.energy) return;
   __shared__ double se[SPLIT][MAXF];
@@ -222,7 +241,7 @@ __device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][3
   __syncthreads();
   if (threadIdx.x < MAXF) {
     const int k = threadIdx.x;
-    e.energy[((size_t)local_rb * gridDim.x + blockIdx.x) * MAXF + k] = ((se[0][k] + se[1][k]) + se[2][k]) + se[3][k];
+    e.energy[((size_t)local_rb * gx + bx) * MAXF + k] = ((se[0][k] + se[1][k]) + se[2][k]) + se[3][k];
   }
 }
 
@@ -265,16 +284,18 @@ __device__ unsigned int input_amax(const Task &t, int f) {
 // accumulator lane l holds D[4*(l/16) + i][l%16], i = 0..3.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__restrict__ tasks,
-                                                                 const int2 *__restrict__ map) {
+                                                                 const int2 *__restrict__ map, TileGrid g) {
   __shared__ float red[SPLIT][32][33];
-  const int2 bm = map[blockIdx.y];
+  int bx, by;
+  if (!tile_of(g, bx, by)) return;
+  const int2 bm = map[by];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.bthi[S]);
   const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.btlo[S]);
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = bm.y * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
   const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
@@ -322,7 +343,7 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__r
 #pragma unroll
       for (int s = 0; s < 4; ++s) red[wave][16 * i + 4 * q + s][16 * j + r] = acc[i][j][s];
   __syncthreads();
-  fast_epilogue(e, red, m0, n0, bm.y);
+  fast_epilogue(e, red, m0, n0, bm.y, bx, g.gx);
 }
 
 // FP16-scaled split precision on MFMA.  The producer of A (k_convert, or the
@@ -333,16 +354,18 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__r
 // power-of-two scales is divided out exactly before the epilogue.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__restrict__ tasks,
-                                                                 const int2 *__restrict__ map) {
+                                                                 const int2 *__restrict__ map, TileGrid g) {
   __shared__ float red[SPLIT][32][33];
-  const int2 bm = map[blockIdx.y];
+  int bx, by;
+  if (!tile_of(g, bx, by)) return;
+  const int2 bm = map[by];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.fthi[S]);
   const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.ftlo[S]);
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = bm.y * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
   const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
@@ -392,25 +415,123 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
 #pragma unroll
       for (int s = 0; s < 4; ++s) red[wave][16 * i + 4 * q + s][16 * j + r] = acc[i][j][s] * unscale;
   __syncthreads();
-  fast_epilogue(e, red, m0, n0, bm.y);
+  fast_epilogue(e, red, m0, n0, bm.y, bx, g.gx);
+}
+
+// FP16-scaled split MFMA over 32 rows x 128 columns per block, for K <=
+// WIDE_KMAX: the block splits its 32 rows of A (scaled by 2^s) into FP16
+// hi + lo once, into LDS, and each wave owns one 32x32 column tile over the
+// whole K, its B fragments coming from the pre-split basis in L2.  The
+// epilogue stores each wave's tile and writes the per-32x32-tile energy and
+// range partials of the 32-wide kernels.
+constexpr int WIDE_KMAX = 256;
+constexpr int WIDE_KP = WIDE_KMAX + 8;  // LDS row stride (halves)
+
+template <int S>
+__global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_wide(const Task *__restrict__ tasks,
+                                                                 const int2 *__restrict__ map, TileGrid g) {
+  __shared__ _Float16 Ahi[32][WIDE_KP], Alo[32][WIDE_KP];
+  int bx, by;
+  if (!tile_of(g, bx, by)) return;
+  const int2 bm = map[by];
+  const Task &t = tasks[bm.x];
+  const Stage e = stage_of<S>(t);
+  const int K = e.K;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = bm.y * 32, n0 = (bx * SPLIT + wave) * 32;
+  const int q = lane >> 4, r = lane & 15;
+  const float sa = pow2_scale(__uint_as_float(input_amax<S>(t, m0 / e.R)));
+  const float unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
+  // stage A: 32 rows x K, coalesced float4 loads
+  const int k4 = K / 4;
+  for (int idx = threadIdx.x; idx < 32 * k4; idx += FAST_THREADS) {
+    const int row = idx / k4, c = idx - row * k4;
+    const float4 v = *reinterpret_cast<const float4 *>(e.A + (size_t)(m0 + row) * K + 4 * c);
+    const float f4[4] = {v.x * sa, v.y * sa, v.z * sa, v.w * sa};
+    half4_t hi, lo;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      _Float16 h, l;
+      split_f16(f4[u], h, l);
+      hi[u] = h;
+      lo[u] = l;
+    }
+    *reinterpret_cast<half4_t *>(&Ahi[row][4 * c]) = hi;
+    *reinterpret_cast<half4_t *>(&Alo[row][4 * c]) = lo;
+  }
+  __syncthreads();
+  const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.fthi[S]);
+  const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.ftlo[S]);
+  const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;
+  const uint2 *bl0 = Btl + (size_t)(n0 + r) * k4 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k4 + q;
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const half4_t bhi[2] = {half4_of(bh0[k0 / 4]), half4_of(bh1[k0 / 4])};
+    const half4_t blo[2] = {half4_of(bl0[k0 / 4]), half4_of(bl1[k0 / 4])};
+    half4_t ahi[2], alo[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ahi[i] = *reinterpret_cast<const half4_t *>(&Ahi[16 * i + r][k0 + 4 * q]);
+      alo[i] = *reinterpret_cast<const half4_t *>(&Alo[16 * i + r][k0 + 4 * q]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], blo[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo[i], bhi[j], acc[i][j], 0, 0, 0);
+      }
+  }
+  // accumulator lane l holds D[4 (l / 16) + u][l % 16] of each 16x16 tile
+  double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
+  float mx = 0.f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      float4_t v = acc[i][j] * unscale;
+      store4(e, m0 + 16 * i + 4 * q, n0 + 16 * j + r, v, energy);
+      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+    }
+  const int gx32 = e.N / 32, tile = bm.y * gx32 + n0 / 32;
+  if (e.amax_out) {
+    mx = wave_max_f(mx);
+    if (lane == 0) e.amax_out[tile] = __float_as_uint(mx);
+  }
+  if (e.energy) {
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) energy[k] = wave_sum(energy[k]);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < MAXF; ++k) e.energy[(size_t)tile * MAXF + k] = energy[k];
+    }
+  }
 }
 
 // FP32 vector-ALU: each wave owns the 32-wide k-chunks w, w+4, ..., stages
 // them through its own LDS tiles and accumulates 4x4 outputs per lane.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f32_fast(const Task *__restrict__ tasks,
-                                                                const int2 *__restrict__ map) {
+                                                                const int2 *__restrict__ map, TileGrid g) {
   __shared__ float As[SPLIT][32][33];
   __shared__ float Bs[SPLIT][32][36];
   __shared__ float red[SPLIT][32][33];
-  const int2 bm = map[blockIdx.y];
+  int bx, by;
+  if (!tile_of(g, bx, by)) return;
+  const int2 bm = map[by];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   const float *B = t.j.b32[S];
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tr = lane >> 3, tc = lane & 7;
-  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = bm.y * 32, n0 = bx * 32;
   float acc[4][4] = {};
   // every wave runs the same number of rounds so the block barriers match
   const int chunks = K / 32, rounds = (chunks + SPLIT - 1) / SPLIT;
@@ -453,22 +574,25 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f32_fast(const Task *__re
 #pragma unroll
     for (int j = 0; j < 4; ++j) red[wave][4 * tr + i][4 * tc + j] = acc[i][j];
   __syncthreads();
-  fast_epilogue(e, red, m0, n0, bm.y);
+  fast_epilogue(e, red, m0, n0, bm.y, bx, g.gx);
 }
 
 // ---------------------------------------------------------------------------
 // Any grid size: 32x32 tiles with bounds checks (prime widths, 7x5 tests).
 // ---------------------------------------------------------------------------
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_bf16_any(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.y];
+__global__ void __launch_bounds__(64) k_gemm_bf16_any(const Task *__restrict__ tasks, const int2 *__restrict__ map,
+                                                 TileGrid g) {
+  int bx, by;
+  if (!tile_of(g, bx, by)) return;
+  const int2 bm = map[by];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   // the transposed split basis: element (k, n) at [n * K + k]
   const unsigned short *Bth = t.j.bthi[S], *Btl = t.j.btlo[S];
   const int K = e.K;
   const int lane = threadIdx.x;
-  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = bm.y * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   float4_t acc[2][2];
 #pragma unroll
@@ -511,19 +635,22 @@ __global__ void __launch_bounds__(64) k_gemm_bf16_any(const Task *__restrict__ t
         const int row = m0 + 16 * i + 4 * q + s, col = n0 + 16 * j + r;
         if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j][s], energy);
       }
-  write_energy(e, bm.y, energy);
+  write_energy(e, bm.y, bx, g.gx, energy);
 }
 
 // FP16-scaled split precision, any grid size (bounds-checked 32x32 tiles).
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_f16s_any(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
-  const int2 bm = map[blockIdx.y];
+__global__ void __launch_bounds__(64) k_gemm_f16s_any(const Task *__restrict__ tasks, const int2 *__restrict__ map,
+                                                 TileGrid g) {
+  int bx, by;
+  if (!tile_of(g, bx, by)) return;
+  const int2 bm = map[by];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   const unsigned short *Bth = t.j.fthi[S], *Btl = t.j.ftlo[S];
   const int K = e.K;
   const int lane = threadIdx.x;
-  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = bm.y * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   float amax = 0.f;
   for (int idx = lane; idx < 32 * K; idx += 64) {
@@ -578,21 +705,24 @@ __global__ void __launch_bounds__(64) k_gemm_f16s_any(const Task *__restrict__ t
         const int row = m0 + 16 * i + 4 * q + s, col = n0 + 16 * j + r;
         if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j][s] * unscale, energy);
       }
-  write_energy(e, bm.y, energy);
+  write_energy(e, bm.y, bx, g.gx, energy);
 }
 
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_f32_any(const Task *__restrict__ tasks, const int2 *__restrict__ map) {
+__global__ void __launch_bounds__(64) k_gemm_f32_any(const Task *__restrict__ tasks, const int2 *__restrict__ map,
+                                                 TileGrid g) {
   __shared__ float As[32][33];
   __shared__ float Bs[32][33];
-  const int2 bm = map[blockIdx.y];
+  int bx, by;
+  if (!tile_of(g, bx, by)) return;
+  const int2 bm = map[by];
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
   const float *B = t.j.b32[S];
   const int K = e.K;
   const int lane = threadIdx.x;
   const int tr = lane >> 3, tc = lane & 7;
-  const int m0 = bm.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = bm.y * 32, n0 = bx * 32;
   float acc[4][4] = {};
   for (int k0 = 0; k0 < K; k0 += 32) {
     for (int idx = lane; idx < 32 * 32; idx += 64) {
@@ -619,12 +749,16 @@ __global__ void __launch_bounds__(64) k_gemm_f32_any(const Task *__restrict__ ta
       const int row = m0 + 4 * tr + i, col = n0 + 4 * tc + j;
       if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j], energy);
     }
-  write_energy(e, bm.y, energy);
+  write_energy(e, bm.y, bx, g.gx, energy);
 }
 
 }  // namespace
 
 bool poisson_fast_shape(int W, int H) { return W % 32 == 0 && H % 32 == 0; }
+// both extents multiples of the wide kernel's 128 columns and within its LDS rows
+static bool poisson_wide_shape(int W, int H) {
+  return W % (32 * SPLIT) == 0 && H % (32 * SPLIT) == 0 && W <= WIDE_KMAX && H <= WIDE_KMAX;
+}
 
 int poisson_energy_blocks(int W, int H, int F) {
   // stage 2: rows F*W in 32-row blocks, columns H in 32-column blocks
@@ -635,23 +769,32 @@ template <int S>
 void launch_stage(const Task *tasks, const DctLaunch &d, bool fast, hipStream_t s) {
   constexpr bool h_rows = S == 0 || S == 3;  // stages with F*H rows
   const int N = h_rows ? d.W : d.H;
-  const dim3 grid((N + 31) / 32, h_rows ? d.row_blocks_h : d.row_blocks_w);
+  TileGrid g;
+  g.gx = (N + 31) / 32;
+  g.gy = h_rows ? d.row_blocks_h : d.row_blocks_w;
+  g.per = (g.gx * g.gy + XCDS - 1) / XCDS;
+  const dim3 grid(XCDS * g.per);
   const int2 *map = h_rows ? d.map_h : d.map_w;
   if (d.precision == 1) {
     if (fast)
-      hipLaunchKernelGGL(k_gemm_bf16_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map);
+      hipLaunchKernelGGL(k_gemm_bf16_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map, g);
     else
-      hipLaunchKernelGGL(k_gemm_bf16_any<S>, grid, dim3(64), 0, s, tasks, map);
+      hipLaunchKernelGGL(k_gemm_bf16_any<S>, grid, dim3(64), 0, s, tasks, map, g);
   } else if (d.precision == 2) {
-    if (fast)
-      hipLaunchKernelGGL(k_gemm_f16s_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map);
+    if (fast && poisson_wide_shape(d.W, d.H)) {
+      TileGrid w = g;
+      w.gx = N / (32 * SPLIT);
+      w.per = (w.gx * w.gy + XCDS - 1) / XCDS;
+      hipLaunchKernelGGL(k_gemm_f16s_wide<S>, dim3(XCDS * w.per), dim3(FAST_THREADS), 0, s, tasks, map, w);
+    } else if (fast)
+      hipLaunchKernelGGL(k_gemm_f16s_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map, g);
     else
-      hipLaunchKernelGGL(k_gemm_f16s_any<S>, grid, dim3(64), 0, s, tasks, map);
+      hipLaunchKernelGGL(k_gemm_f16s_any<S>, grid, dim3(64), 0, s, tasks, map, g);
   } else {
     if (fast)
-      hipLaunchKernelGGL(k_gemm_f32_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map);
+      hipLaunchKernelGGL(k_gemm_f32_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map, g);
     else
-      hipLaunchKernelGGL(k_gemm_f32_any<S>, grid, dim3(64), 0, s, tasks, map);
+      hipLaunchKernelGGL(k_gemm_f32_any<S>, grid, dim3(64), 0, s, tasks, map, g);
   }
 }
 
