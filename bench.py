@@ -213,9 +213,9 @@ def main():
     ap.add_argument("--roofline-kernel", default="wa", choices=["deposit", "wa", "gather", "dct"])
     ap.add_argument("--serving-jobs", type=int, default=256)
     ap.add_argument("--serving-rate", type=float, default=20.0)
-    ap.add_argument("--serving-clients", type=int, default=4, help="threads submitting the serving stream")
+    ap.add_argument("--serving-clients", type=int, default=8, help="threads submitting the serving stream")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--cpu-iterations", type=int, default=20)
+    ap.add_argument("--cpu-iterations", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="time the batched form only (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

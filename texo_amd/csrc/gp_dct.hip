@@ -425,6 +425,9 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
 // epilogue stores each wave's tile and writes the per-32x32-tile energy and
 // range partials of the 32-wide kernels.
 constexpr int WIDE_KMAX = 256;
+// Below this many wide tiles (two per CU) a launch leaves CUs idle and the
+// 32x32 split-K kernel is faster.
+constexpr int WIDE_MIN_TILES = 512;
 constexpr int WIDE_KP = WIDE_KMAX + 8;  // LDS row stride (halves)
 
 template <int S>
@@ -781,10 +784,10 @@ void launch_stage(const Task *tasks, const DctLaunch &d, bool fast, hipStream_t 
     else
       hipLaunchKernelGGL(k_gemm_bf16_any<S>, grid, dim3(64), 0, s, tasks, map, g);
   } else if (d.precision == 2) {
-    if (fast && poisson_wide_shape(d.W, d.H)) {
-      TileGrid w = g;
-      w.gx = N / (32 * SPLIT);
-      w.per = (w.gx * w.gy + XCDS - 1) / XCDS;
+    TileGrid w = g;
+    w.gx = N / (32 * SPLIT);
+    w.per = (w.gx * w.gy + XCDS - 1) / XCDS;
+    if (fast && poisson_wide_shape(d.W, d.H) && w.gx * w.gy >= WIDE_MIN_TILES) {
       hipLaunchKernelGGL(k_gemm_f16s_wide<S>, dim3(XCDS * w.per), dim3(FAST_THREADS), 0, s, tasks, map, w);
     } else if (fast)
       hipLaunchKernelGGL(k_gemm_f16s_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map, g);
