@@ -849,8 +849,10 @@ int64_t texo_gp_server_submit(texo_gp_server *s, const texo_gp_problem *problem,
   e->submitted = std::chrono::steady_clock::now();
   e->job.reset(new Job());
   try {
-    // deep copy, validation and upload image on the caller's thread
+    // deep copy, validation, upload image and (for a new grid size) the
+    // transform basis on the caller's thread
     e->job->load(problem, options, s->engine->image_pool());
+    if (!e->job->empty) s->engine->basis((int)e->job->W, (int)e->job->H);
   } catch (const Failure &f) {
     return -(int64_t)f.status;
   }

@@ -230,9 +230,11 @@ Engine::~Engine() {
 }
 
 const Basis *Engine::basis(int W, int H) {
+  std::lock_guard<std::mutex> lk(bases_mu_);
   auto key = std::make_pair(W, H);
   auto it = bases_.find(key);
   if (it != bases_.end()) return it->second.get();
+  HIP(hipSetDevice(device_));
   std::unique_ptr<Basis> b(new Basis());
   b->build(W, H);
   const Basis *r = b.get();
@@ -423,6 +425,9 @@ void Engine::plan(Job *j) {
           break;
         }
         j->want_candidate = true;
+        // many charges outside their windows at the last evaluation: sort
+        // again, at this candidate
+        j->want_sort = (size_t)j->moved * MOVED_RESORT > j->M + j->Fl;
         const int q = 1 - j->p_slot;
         j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, q});
         return;
@@ -845,9 +850,17 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   std::map<std::tuple<int, int, int>, Group> groups;
   for (Job *j : active) {
     if (j->want_sort && j->F > 0 && j->M + j->Fl > 0) {
+      // by tile at the coordinates of the round's first density evaluation
+      // (computed by the optimizer launch that precedes the sort)
+      const double *at = j->d_vec[j->ix];
+      for (const Job::EvalReq &r : j->evals)
+        if (!r.skip) {
+          at = r.coords;
+          break;
+        }
       j->sorted = true;
       l_sort.push_back((int)sorts.size());
-      sorts.push_back(j->sort_task());
+      sorts.push_back(j->sort_task(at));
       p_sort_c.add(j->M + j->Fl, BLOCK);
       p_sort_k.add((size_t)j->n_keys + 1, BLOCK);
     }

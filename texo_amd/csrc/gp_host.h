@@ -246,7 +246,9 @@ struct Job {
   bool sorted = false;  // charges ordered by tile at least once
   Task task(int slot, const double *coords, double *grad, const double *metric, double gamma, bool skip,
             EvalScalars *scalars) const;
-  SortTask sort_task();  // next sort; the new order becomes the job's order
+  // next sort, at `coords` (null: the current iterate); the new order
+  // becomes the job's order
+  SortTask sort_task(const double *coords = nullptr);
 
   // ---- engine bookkeeping ----
   int64_t id = 0;
@@ -337,7 +339,7 @@ class Engine {
   hipStream_t io_stream() const { return io_stream_; }
   // Pinned host blocks for upload images (usable from any thread).
   const std::shared_ptr<BlockPool> &image_pool() const { return images_; }
-  const Basis *basis(int W, int H);
+  const Basis *basis(int W, int H);  // built on first use; thread-safe
   void attach(Job *job);  // enqueue the upload and reserve a scalar slot
   void detach(Job *job);  // release the slot (job memory is freed by the job)
   void sort_now(Job *job);  // sorts the job's charges by tile at its current coordinates
@@ -364,7 +366,8 @@ class Engine {
   hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_upload_ = nullptr;
   bool uploads_pending_ = false;
   std::shared_ptr<BlockPool> blocks_, images_;
-  std::map<std::pair<int, int>, std::unique_ptr<Basis>> bases_;
+  std::map<std::pair<int, int>, std::unique_ptr<Basis>> bases_;  // never erased while the engine lives
+  std::mutex bases_mu_;
   JobScalars *d_pool_ = nullptr, *h_pool_ = nullptr;
   std::vector<bool> pool_used_;
   // Staging of one round: the task tables go up in one copy, the active

@@ -923,10 +923,10 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   return t;
 }
 
-SortTask Job::sort_task() {
+SortTask Job::sort_task(const double *coords) {
   SortTask t{};
   t.j = js;
-  t.coords = d_vec[ix];
+  t.coords = coords ? coords : d_vec[ix];
   t.count = d_sort_count;
   t.order = d_perm;
   t.perm = d_perm_next;

@@ -768,9 +768,26 @@ __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, Bl
 // baseline pins, then timed-arc pins, then each field in order, the field
 // term being coefficient x the sum of the unit's member samples x charge in
 // member order (electrostatic_placement.rs:338-374).
+// A wave's 64 units own one contiguous run of the unit-ordered derivative
+// array; the wave copies it into LDS with coalesced loads (when it fits) and
+// each lane then sums its unit's entries from there in the same order.
+constexpr int DPIN_STAGE = 384;
+
 __device__ void unit_gradient_block(const Task &t, int lb) {
   const JobStatic &a = t.j;
   const int u = lb * BLOCK + threadIdx.x;
+  __shared__ double2 s_dpin[BLOCK / 64][DPIN_STAGE];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int u0 = lb * BLOCK + wave * 64;
+  uint32_t w_begin = 0, w_end = 0;
+  if (u0 < a.n_units) {
+    w_begin = a.u_pstart[u0];
+    w_end = a.u_pstart[min(u0 + 64, a.n_units)];
+  }
+  const bool staged = w_end - w_begin <= (uint32_t)DPIN_STAGE;
+  if (staged)
+    for (uint32_t k = w_begin + lane; k < w_end; k += 64) s_dpin[wave][k - w_begin] = t.dpin[k];
+  __syncthreads();
   double red[1 + 2 * MAXF];
 #pragma unroll
   for (int k = 0; k < 1 + 2 * MAXF; ++k) red[k] = 0.0;
@@ -779,16 +796,22 @@ __device__ void unit_gradient_block(const Task &t, int lb) {
   if (u < a.n_units) {
     double sx = 0.0, sy = 0.0, tx = 0.0, ty = 0.0;
     const uint32_t pb = a.u_pstart[u], pe = a.u_pstart[u + 1], pm = pb + a.u_nbase[u];
-    for (uint32_t k = pb; k < pm; ++k) {  // net pins
-      const double2 d = t.dpin[k];
-      sx += d.x;
-      sy += d.y;
-    }
-    for (uint32_t k = pm; k < pe; ++k) {  // timed-arc pins
-      const double2 d = t.dpin[k];
-      tx += d.x;
-      ty += d.y;
-    }
+    auto pin_sums = [&](auto at) {
+      for (uint32_t k = pb; k < pm; ++k) {  // net pins
+        const double2 d = at(k);
+        sx += d.x;
+        sy += d.y;
+      }
+      for (uint32_t k = pm; k < pe; ++k) {  // timed-arc pins
+        const double2 d = at(k);
+        tx += d.x;
+        ty += d.y;
+      }
+    };
+    if (staged)
+      pin_sums([&](uint32_t k) { return s_dpin[wave][k - w_begin]; });
+    else
+      pin_sums([&](uint32_t k) { return t.dpin[k]; });
     double gx = sx + tx, gy = sy + ty;
     red[0] = fabs(gx) + fabs(gy);
     if (t.wa_grad_out) {
