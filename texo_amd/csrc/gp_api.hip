@@ -261,31 +261,35 @@ int texo_gp_session_poisson(texo_gp_session *s, const double *density, double *p
     HIP(hipMemcpyAsync(J.slot[0].amax_part, amax.data(), amax.size() * 4, hipMemcpyHostToDevice, st));
     HIP(hipStreamSynchronize(st));
     texo::Task t = J.task(0, J.d_eval_coords, J.d_grad[0], nullptr, 1.0, false, &J.d_scalars->eval[2]);
-    std::vector<int2> gh, gw;
-    for (int b = 0; b < (int)((J.F * J.H + 31) / 32); ++b) gh.push_back(make_int2(0, b));
-    for (int b = 0; b < (int)((J.F * J.W + 31) / 32); ++b) gw.push_back(make_int2(0, b));
-    texo::Task *d_task = nullptr;
-    int2 *d_maps = nullptr;
-    HIP(hipMalloc(&d_task, sizeof(texo::Task)));
-    HIP(hipMalloc(&d_maps, (gh.size() + gw.size()) * sizeof(int2)));
-    HIP(hipMemcpy(d_task, &t, sizeof(t), hipMemcpyHostToDevice));
-    HIP(hipMemcpy(d_maps, gh.data(), gh.size() * sizeof(int2), hipMemcpyHostToDevice));
-    HIP(hipMemcpy(d_maps + gh.size(), gw.data(), gw.size() * sizeof(int2), hipMemcpyHostToDevice));
+    // one task: its transform tiles as four one-entry block plans
+    const int F = J.F, W = (int)J.W, H = (int)J.H;
     texo::DctLaunch d{};
-    d.map_h = d_maps;
-    d.map_w = d_maps + gh.size();
-    d.row_blocks_h = (int)gh.size();
-    d.row_blocks_w = (int)gw.size();
-    d.W = (int)J.W;
-    d.H = (int)J.H;
     d.precision = (int)J.opt.precision;
+    d.fast = texo::poisson_fast_shape(W, H);
+    d.wide_ok = d.precision == TEXO_GP_DCT_FP16_SCALED && d.fast && texo::poisson_wide_shape(W, H);
+    const int tiles[4] = {((F * H + 31) / 32) * ((W + 31) / 32), ((F * W + 31) / 32) * ((H + 31) / 32),
+                          d.wide_ok ? (F * H / 32) * (W / 128) : 0, d.wide_ok ? (F * W / 32) * (H / 128) : 0};
+    const int starts[8] = {0, tiles[0], 0, tiles[1], 0, tiles[2], 0, tiles[3]};
+    texo::Task *d_task = nullptr;
+    int *d_starts = nullptr;
+    HIP(hipMalloc(&d_task, sizeof(texo::Task)));
+    HIP(hipMalloc(&d_starts, sizeof(starts)));
+    HIP(hipMemcpy(d_task, &t, sizeof(t), hipMemcpyHostToDevice));
+    HIP(hipMemcpy(d_starts, starts, sizeof(starts), hipMemcpyHostToDevice));
+    texo::BlockPlan *plans[4] = {&d.h, &d.w, &d.h_wide, &d.w_wide};
+    for (int k = 0; k < 4; ++k) {
+      plans[k]->start = d_starts + 2 * k;
+      plans[k]->n_tasks = 1;
+      plans[k]->total = tiles[k];
+      plans[k]->per = tiles[k] >= 2 * texo::XCDS ? (tiles[k] + texo::XCDS - 1) / texo::XCDS : 0;
+    }
     texo::launch_poisson(d_task, d, st);
     std::vector<double> parts((size_t)J.nb_energy * texo::MAXF);
     HIP(hipMemcpyAsync(parts.data(), J.slot[0].part_energy, parts.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP(hipMemcpyAsync(phi.data(), J.slot[0].phi, FG * sizeof(float), hipMemcpyDeviceToHost, st));
     HIP(hipStreamSynchronize(st));
     HIP(hipFree(d_task));
-    HIP(hipFree(d_maps));
+    HIP(hipFree(d_starts));
     for (size_t i = 0; i < FG; ++i) potential[i] = phi[i];
     for (int f = 0; f < J.F; ++f) {
       double e = 0.0;

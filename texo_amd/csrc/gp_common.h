@@ -292,15 +292,20 @@ void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit
 // Poisson transform (gp_dct.hip) of every task of one grid size.  map_h has
 // one entry per 32-row block of the F*H-row stages, map_w of the F*W-row
 // stages: (task, local row block).
+// One transform launch group: tasks of one precision and shape class.  The
+// plans count, per task, 32-row blocks x 32-column tiles (h: the stages with
+// F*H rows, w: those with F*W rows), and the same with 128-column tiles when
+// every task of the group fits the wide FP16-scaled kernel.
 struct DctLaunch {
-  const int2 *map_h, *map_w;
-  int row_blocks_h, row_blocks_w;
-  int W, H;
+  BlockPlan h, w, h_wide, w_wide;
   int precision;
+  bool fast;     // every task's W and H are multiples of 32
+  bool wide_ok;  // ... and of 128, at most 256
 };
 void launch_poisson(const Task *tasks, const DctLaunch &d, hipStream_t s);
 int poisson_energy_blocks(int W, int H, int F);  // per task
 bool poisson_fast_shape(int W, int H);
+bool poisson_wide_shape(int W, int H);
 
 // Kernel timing hook: the engine brackets launches of one named kernel.
 struct KernelTimer;

@@ -8,7 +8,7 @@
 //   2. St'_f = (Rt_f C_H^T) .* invLambda^T rows F*W, K = N = H, spectral epilogue
 //   3. T_f   = (St'_f C_H)^T               rows F*W, K = N = H, transposed store
 //   4. phi_f = T_f C_W                     rows F*H, K = N = W
-// Each tile's row block indexes a map of (task, 32-row block); a tile never straddles two
+// A launch covers the tiles of every task of one group (a block plan); a tile never straddles two
 // tasks.  The electrostatic energy 1/2 <rho_bar, phi> equals
 // 1/2 sum invLambda S^2 by Parseval (orthonormal basis); it is accumulated in
 // FP64 in the epilogue of step 2.  The DC coefficient has invLambda = 0, so
@@ -168,24 +168,6 @@ __device__ __forceinline__ void store4(const Stage &e, int row0, int col, float4
   }
 }
 
-// Tiles of one stage launch: gx column tiles x gy row blocks (map entries),
-// launched as a 1-D grid of XCDS * per blocks.  The hardware deals blocks
-// round-robin to the XCDs; the renumbering gives each XCD a contiguous run
-// of tiles, so the column tiles of one row block share an L2 and its input
-// rows are fetched from HBM once instead of once per XCD.
-struct TileGrid {
-  int gx, gy, per;
-};
-
-__device__ __forceinline__ bool tile_of(const TileGrid &g, int &bx, int &by) {
-  const int b = (int)blockIdx.x;
-  const int lb = (b % XCDS) * g.per + b / XCDS;
-  if (lb >= g.gx * g.gy) return false;
-  by = lb / g.gx;
-  bx = lb - by * g.gx;
-  return true;
-}
-
 __device__ __forceinline__ void write_energy(const Stage &e, int local_rb, int bx, int gx, double (&energy)[MAXF]) {
   if (!e.energy) return;
 #pragma unroll
@@ -284,18 +266,19 @@ __device__ unsigned int input_amax(const Task &t, int f) {
 // accumulator lane l holds D[4*(l/16) + i][l%16], i = 0..3.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__restrict__ tasks,
-                                                                 const int2 *__restrict__ map, TileGrid g) {
+                                                                 BlockPlan p) {
   __shared__ float red[SPLIT][32][33];
-  int bx, by;
-  if (!tile_of(g, bx, by)) return;
-  const int2 bm = map[by];
+  int2 bm;
+  if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
+  // the task's tiles: 32-row blocks x 32-column tiles, row-major
+  const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
   const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.bthi[S]);
   const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.btlo[S]);
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = bm.y * 32, n0 = bx * 32;
+  const int m0 = rb * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
   const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
@@ -343,7 +326,7 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__r
 #pragma unroll
       for (int s = 0; s < 4; ++s) red[wave][16 * i + 4 * q + s][16 * j + r] = acc[i][j][s];
   __syncthreads();
-  fast_epilogue(e, red, m0, n0, bm.y, bx, g.gx);
+  fast_epilogue(e, red, m0, n0, rb, bx, gx);
 }
 
 // FP16-scaled split precision on MFMA.  The producer of A (k_convert, or the
@@ -354,18 +337,19 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__r
 // power-of-two scales is divided out exactly before the epilogue.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__restrict__ tasks,
-                                                                 const int2 *__restrict__ map, TileGrid g) {
+                                                                 BlockPlan p) {
   __shared__ float red[SPLIT][32][33];
-  int bx, by;
-  if (!tile_of(g, bx, by)) return;
-  const int2 bm = map[by];
+  int2 bm;
+  if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
+  // the task's tiles: 32-row blocks x 32-column tiles, row-major
+  const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
   const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.fthi[S]);
   const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.ftlo[S]);
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = bm.y * 32, n0 = bx * 32;
+  const int m0 = rb * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
   const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
@@ -415,7 +399,7 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
 #pragma unroll
       for (int s = 0; s < 4; ++s) red[wave][16 * i + 4 * q + s][16 * j + r] = acc[i][j][s] * unscale;
   __syncthreads();
-  fast_epilogue(e, red, m0, n0, bm.y, bx, g.gx);
+  fast_epilogue(e, red, m0, n0, rb, bx, gx);
 }
 
 // FP16-scaled split MFMA over 32 rows x 128 columns per block, for K <=
@@ -432,16 +416,17 @@ constexpr int WIDE_KP = WIDE_KMAX + 8;  // LDS row stride (halves)
 
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_wide(const Task *__restrict__ tasks,
-                                                                 const int2 *__restrict__ map, TileGrid g) {
+                                                                 BlockPlan p) {
   __shared__ _Float16 Ahi[32][WIDE_KP], Alo[32][WIDE_KP];
-  int bx, by;
-  if (!tile_of(g, bx, by)) return;
-  const int2 bm = map[by];
+  int2 bm;
+  if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
+  // the task's wide tiles: 32-row blocks x 128-column tiles, row-major
+  const int gw = e.N / (32 * SPLIT), rb = bm.y / gw, bw = bm.y - rb * gw;
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = bm.y * 32, n0 = (bx * SPLIT + wave) * 32;
+  const int m0 = rb * 32, n0 = (bw * SPLIT + wave) * 32;
   const int q = lane >> 4, r = lane & 15;
   const float sa = pow2_scale(__uint_as_float(input_amax<S>(t, m0 / e.R)));
   const float unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
@@ -502,7 +487,7 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_wide(const Task *__r
       store4(e, m0 + 16 * i + 4 * q, n0 + 16 * j + r, v, energy);
       mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
     }
-  const int gx32 = e.N / 32, tile = bm.y * gx32 + n0 / 32;
+  const int gx32 = e.N / 32, tile = rb * gx32 + n0 / 32;
   if (e.amax_out) {
     mx = wave_max_f(mx);
     if (lane == 0) e.amax_out[tile] = __float_as_uint(mx);
@@ -521,20 +506,21 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_wide(const Task *__r
 // them through its own LDS tiles and accumulates 4x4 outputs per lane.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f32_fast(const Task *__restrict__ tasks,
-                                                                const int2 *__restrict__ map, TileGrid g) {
+                                                                BlockPlan p) {
   __shared__ float As[SPLIT][32][33];
   __shared__ float Bs[SPLIT][32][36];
   __shared__ float red[SPLIT][32][33];
-  int bx, by;
-  if (!tile_of(g, bx, by)) return;
-  const int2 bm = map[by];
+  int2 bm;
+  if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
+  // the task's tiles: 32-row blocks x 32-column tiles, row-major
+  const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
   const float *B = t.j.b32[S];
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tr = lane >> 3, tc = lane & 7;
-  const int m0 = bm.y * 32, n0 = bx * 32;
+  const int m0 = rb * 32, n0 = bx * 32;
   float acc[4][4] = {};
   // every wave runs the same number of rounds so the block barriers match
   const int chunks = K / 32, rounds = (chunks + SPLIT - 1) / SPLIT;
@@ -577,25 +563,25 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f32_fast(const Task *__re
 #pragma unroll
     for (int j = 0; j < 4; ++j) red[wave][4 * tr + i][4 * tc + j] = acc[i][j];
   __syncthreads();
-  fast_epilogue(e, red, m0, n0, bm.y, bx, g.gx);
+  fast_epilogue(e, red, m0, n0, rb, bx, gx);
 }
 
 // ---------------------------------------------------------------------------
 // Any grid size: 32x32 tiles with bounds checks (prime widths, 7x5 tests).
 // ---------------------------------------------------------------------------
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_bf16_any(const Task *__restrict__ tasks, const int2 *__restrict__ map,
-                                                 TileGrid g) {
-  int bx, by;
-  if (!tile_of(g, bx, by)) return;
-  const int2 bm = map[by];
+__global__ void __launch_bounds__(64) k_gemm_bf16_any(const Task *__restrict__ tasks, BlockPlan p) {
+  int2 bm;
+  if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
+  // the task's tiles: 32-row blocks x 32-column tiles, row-major
+  const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
   // the transposed split basis: element (k, n) at [n * K + k]
   const unsigned short *Bth = t.j.bthi[S], *Btl = t.j.btlo[S];
   const int K = e.K;
   const int lane = threadIdx.x;
-  const int m0 = bm.y * 32, n0 = bx * 32;
+  const int m0 = rb * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   float4_t acc[2][2];
 #pragma unroll
@@ -638,22 +624,22 @@ __global__ void __launch_bounds__(64) k_gemm_bf16_any(const Task *__restrict__ t
         const int row = m0 + 16 * i + 4 * q + s, col = n0 + 16 * j + r;
         if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j][s], energy);
       }
-  write_energy(e, bm.y, bx, g.gx, energy);
+  write_energy(e, rb, bx, gx, energy);
 }
 
 // FP16-scaled split precision, any grid size (bounds-checked 32x32 tiles).
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_f16s_any(const Task *__restrict__ tasks, const int2 *__restrict__ map,
-                                                 TileGrid g) {
-  int bx, by;
-  if (!tile_of(g, bx, by)) return;
-  const int2 bm = map[by];
+__global__ void __launch_bounds__(64) k_gemm_f16s_any(const Task *__restrict__ tasks, BlockPlan p) {
+  int2 bm;
+  if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
+  // the task's tiles: 32-row blocks x 32-column tiles, row-major
+  const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
   const unsigned short *Bth = t.j.fthi[S], *Btl = t.j.ftlo[S];
   const int K = e.K;
   const int lane = threadIdx.x;
-  const int m0 = bm.y * 32, n0 = bx * 32;
+  const int m0 = rb * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
   float amax = 0.f;
   for (int idx = lane; idx < 32 * K; idx += 64) {
@@ -708,24 +694,24 @@ __global__ void __launch_bounds__(64) k_gemm_f16s_any(const Task *__restrict__ t
         const int row = m0 + 16 * i + 4 * q + s, col = n0 + 16 * j + r;
         if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j][s] * unscale, energy);
       }
-  write_energy(e, bm.y, bx, g.gx, energy);
+  write_energy(e, rb, bx, gx, energy);
 }
 
 template <int S>
-__global__ void __launch_bounds__(64) k_gemm_f32_any(const Task *__restrict__ tasks, const int2 *__restrict__ map,
-                                                 TileGrid g) {
+__global__ void __launch_bounds__(64) k_gemm_f32_any(const Task *__restrict__ tasks, BlockPlan p) {
   __shared__ float As[32][33];
   __shared__ float Bs[32][33];
-  int bx, by;
-  if (!tile_of(g, bx, by)) return;
-  const int2 bm = map[by];
+  int2 bm;
+  if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
   const Stage e = stage_of<S>(t);
+  // the task's tiles: 32-row blocks x 32-column tiles, row-major
+  const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
   const float *B = t.j.b32[S];
   const int K = e.K;
   const int lane = threadIdx.x;
   const int tr = lane >> 3, tc = lane & 7;
-  const int m0 = bm.y * 32, n0 = bx * 32;
+  const int m0 = rb * 32, n0 = bx * 32;
   float acc[4][4] = {};
   for (int k0 = 0; k0 < K; k0 += 32) {
     for (int idx = lane; idx < 32 * 32; idx += 64) {
@@ -752,14 +738,14 @@ __global__ void __launch_bounds__(64) k_gemm_f32_any(const Task *__restrict__ ta
       const int row = m0 + 4 * tr + i, col = n0 + 4 * tc + j;
       if (row < e.M && col < e.N) store_element(e, row, col, acc[i][j], energy);
     }
-  write_energy(e, bm.y, bx, g.gx, energy);
+  write_energy(e, rb, bx, gx, energy);
 }
 
 }  // namespace
 
 bool poisson_fast_shape(int W, int H) { return W % 32 == 0 && H % 32 == 0; }
 // both extents multiples of the wide kernel's 128 columns and within its LDS rows
-static bool poisson_wide_shape(int W, int H) {
+bool poisson_wide_shape(int W, int H) {
   return W % (32 * SPLIT) == 0 && H % (32 * SPLIT) == 0 && W <= WIDE_KMAX && H <= WIDE_KMAX;
 }
 
@@ -769,46 +755,38 @@ int poisson_energy_blocks(int W, int H, int F) {
 }
 
 template <int S>
-void launch_stage(const Task *tasks, const DctLaunch &d, bool fast, hipStream_t s) {
+void launch_stage(const Task *tasks, const DctLaunch &d, bool wide, hipStream_t s) {
   constexpr bool h_rows = S == 0 || S == 3;  // stages with F*H rows
-  const int N = h_rows ? d.W : d.H;
-  TileGrid g;
-  g.gx = (N + 31) / 32;
-  g.gy = h_rows ? d.row_blocks_h : d.row_blocks_w;
-  g.per = (g.gx * g.gy + XCDS - 1) / XCDS;
-  const dim3 grid(XCDS * g.per);
-  const int2 *map = h_rows ? d.map_h : d.map_w;
+  const BlockPlan &p = wide ? (h_rows ? d.h_wide : d.w_wide) : (h_rows ? d.h : d.w);
+  const dim3 grid(plan_grid(p));
   if (d.precision == 1) {
-    if (fast)
-      hipLaunchKernelGGL(k_gemm_bf16_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map, g);
+    if (d.fast)
+      hipLaunchKernelGGL(k_gemm_bf16_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, p);
     else
-      hipLaunchKernelGGL(k_gemm_bf16_any<S>, grid, dim3(64), 0, s, tasks, map, g);
+      hipLaunchKernelGGL(k_gemm_bf16_any<S>, grid, dim3(64), 0, s, tasks, p);
   } else if (d.precision == 2) {
-    TileGrid w = g;
-    w.gx = N / (32 * SPLIT);
-    w.per = (w.gx * w.gy + XCDS - 1) / XCDS;
-    if (fast && poisson_wide_shape(d.W, d.H) && w.gx * w.gy >= WIDE_MIN_TILES) {
-      hipLaunchKernelGGL(k_gemm_f16s_wide<S>, dim3(XCDS * w.per), dim3(FAST_THREADS), 0, s, tasks, map, w);
-    } else if (fast)
-      hipLaunchKernelGGL(k_gemm_f16s_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map, g);
+    if (wide)
+      hipLaunchKernelGGL(k_gemm_f16s_wide<S>, grid, dim3(FAST_THREADS), 0, s, tasks, p);
+    else if (d.fast)
+      hipLaunchKernelGGL(k_gemm_f16s_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, p);
     else
-      hipLaunchKernelGGL(k_gemm_f16s_any<S>, grid, dim3(64), 0, s, tasks, map, g);
+      hipLaunchKernelGGL(k_gemm_f16s_any<S>, grid, dim3(64), 0, s, tasks, p);
   } else {
-    if (fast)
-      hipLaunchKernelGGL(k_gemm_f32_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, map, g);
+    if (d.fast)
+      hipLaunchKernelGGL(k_gemm_f32_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, p);
     else
-      hipLaunchKernelGGL(k_gemm_f32_any<S>, grid, dim3(64), 0, s, tasks, map, g);
+      hipLaunchKernelGGL(k_gemm_f32_any<S>, grid, dim3(64), 0, s, tasks, p);
   }
 }
 
 void launch_poisson(const Task *tasks, const DctLaunch &d, hipStream_t s) {
-  if (d.row_blocks_h <= 0) return;
+  if (d.h.total <= 0) return;
   timer_begin(g_timer, "dct", s);
-  const bool fast = poisson_fast_shape(d.W, d.H);
-  launch_stage<0>(tasks, d, fast, s);
-  launch_stage<1>(tasks, d, fast, s);
-  launch_stage<2>(tasks, d, fast, s);
-  launch_stage<3>(tasks, d, fast, s);
+  const bool wide = d.precision == 2 && d.wide_ok && d.h_wide.total >= WIDE_MIN_TILES;
+  launch_stage<0>(tasks, d, wide, s);
+  launch_stage<1>(tasks, d, wide, s);
+  launch_stage<2>(tasks, d, wide, s);
+  launch_stage<3>(tasks, d, wide, s);
   timer_end(g_timer, "dct", s);
 }
 

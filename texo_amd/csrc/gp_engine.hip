@@ -19,6 +19,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <array>
 #include <tuple>
 
 namespace texo {
@@ -89,11 +90,6 @@ size_t append(std::vector<char> &buf, const std::vector<T> &v) {
   return off;
 }
 
-void add_blocks(std::vector<int2> &map, int task, size_t n, size_t per_block) {
-  const int nb = (int)((n + per_block - 1) / per_block);
-  for (int b = 0; b < nb; ++b) map.push_back(make_int2(task, b));
-}
-
 // Blocks per task of one launch family, as prefix sums (see BlockPlan).
 struct Plan {
   std::vector<int> start{0};
@@ -109,6 +105,52 @@ struct Plan {
     return b;
   }
 };
+
+// Transform tiles of the density tasks of one launch group (precision and
+// shape class), one plan entry per task of the round.
+struct DctGroup {
+  Plan h, w, h_wide, w_wide;
+  int precision = 0;
+  bool fast = false, wide_ok = false;
+  void pad(size_t n_tasks) {
+    for (Plan *p : {&h, &w, &h_wide, &w_wide})
+      while (p->start.size() < n_tasks + 1) p->none();
+  }
+  void add(size_t task, const Job &j) {
+    pad(task);
+    const size_t F = (size_t)j.F, W = j.W, H = j.H;
+    h.add(((F * H + 31) / 32) * ((W + 31) / 32), 1);
+    w.add(((F * W + 31) / 32) * ((H + 31) / 32), 1);
+    if (wide_ok) {
+      h_wide.add((F * H / 32) * (W / 128), 1);
+      w_wide.add((F * W / 32) * (H / 128), 1);
+    } else {
+      h_wide.none();
+      w_wide.none();
+    }
+  }
+  DctLaunch device(const char *stage, const size_t (&offset)[4]) const {
+    DctLaunch d{};
+    d.h = h.device(stage, offset[0]);
+    d.w = w.device(stage, offset[1]);
+    d.h_wide = h_wide.device(stage, offset[2]);
+    d.w_wide = w_wide.device(stage, offset[3]);
+    d.precision = precision;
+    d.fast = fast;
+    d.wide_ok = wide_ok;
+    return d;
+  }
+};
+
+DctGroup &dct_group(std::map<std::tuple<int, bool, bool>, DctGroup> &groups, const Job &j) {
+  const bool fast = poisson_fast_shape((int)j.W, (int)j.H);
+  const bool wide = j.opt.precision == TEXO_GP_DCT_FP16_SCALED && fast && poisson_wide_shape((int)j.W, (int)j.H);
+  DctGroup &g = groups[std::make_tuple((int)j.opt.precision, fast, wide)];
+  g.precision = (int)j.opt.precision;
+  g.fast = fast;
+  g.wide_ok = wide;
+  return g;
+}
 
 }  // namespace
 
@@ -844,10 +886,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   Plan p_sort_c, p_sort_k, p_pre, p_ext, p_cand;
   std::vector<Task> tasks;
   Plan p_dep, p_conv, p_wa, p_gr;
-  struct Group {
-    std::vector<int2> h, w;
-  };
-  std::map<std::tuple<int, int, int>, Group> groups;
+  std::map<std::tuple<int, bool, bool>, DctGroup> groups;
   for (Job *j : active) {
     if (j->want_sort && j->F > 0 && j->M + j->Fl > 0) {
       // by tile at the coordinates of the round's first density evaluation
@@ -930,9 +969,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
       if (density) {
         p_dep.add((size_t)j->n_keys, 1);
         p_conv.add((size_t)j->F * j->G, BLOCK);
-        Group &g = groups[std::make_tuple((int)j->W, (int)j->H, (int)j->opt.precision)];
-        add_blocks(g.h, ti, (size_t)j->F * j->H, 32);
-        add_blocks(g.w, ti, (size_t)j->F * j->W, 32);
+        dct_group(groups, *j).add((size_t)ti, *j);
       } else {
         p_dep.none();
         p_conv.none();
@@ -949,8 +986,13 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
                o_pcand = append(buf, p_cand.start);
   const size_t o_pdep = append(buf, p_dep.start), o_pconv = append(buf, p_conv.start),
                o_pwa = append(buf, p_wa.start), o_pgr = append(buf, p_gr.start);
-  std::vector<std::pair<size_t, size_t>> o_groups;
-  for (auto &kv : groups) o_groups.push_back({append(buf, kv.second.h), append(buf, kv.second.w)});
+  std::vector<std::array<size_t, 4>> o_groups;
+  for (auto &kv : groups) {
+    DctGroup &g = kv.second;
+    g.pad(tasks.size());
+    o_groups.push_back({append(buf, g.h.start), append(buf, g.w.start), append(buf, g.h_wide.start),
+                        append(buf, g.w_wide.start)});
+  }
   std::vector<int> pool_index;
   for (Job *j : active) pool_index.push_back(j->pool_index);
   const size_t o_pool = append(buf, pool_index);
@@ -996,15 +1038,8 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   launch_density(d_tasks, mp, stream_);
   size_t gi = 0;
   for (auto &kv : groups) {
-    DctLaunch d{};
-    d.map_h = reinterpret_cast<const int2 *>(dptr(o_groups[gi].first));
-    d.map_w = reinterpret_cast<const int2 *>(dptr(o_groups[gi].second));
-    d.row_blocks_h = (int)kv.second.h.size();
-    d.row_blocks_w = (int)kv.second.w.size();
-    d.W = std::get<0>(kv.first);
-    d.H = std::get<1>(kv.first);
-    d.precision = std::get<2>(kv.first);
-    launch_poisson(d_tasks, d, stream_);
+    const size_t off[4] = {o_groups[gi][0], o_groups[gi][1], o_groups[gi][2], o_groups[gi][3]};
+    launch_poisson(d_tasks, kv.second.device(d_stage_, off), stream_);
     ++gi;
   }
   launch_gather(d_tasks, mp, stream_);
@@ -1072,12 +1107,11 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   if (density_out) t.density_out = j->d_density_out;
   tasks.push_back(t);
   Plan p_dep, p_conv, p_wa, p_gr;
-  std::vector<int2> gh, gw;
+  std::map<std::tuple<int, bool, bool>, DctGroup> groups;
   if (j->F > 0) {
     p_dep.add((size_t)j->n_keys, 1);
     p_conv.add((size_t)j->F * j->G, BLOCK);
-    add_blocks(gh, 0, (size_t)j->F * j->H, 32);
-    add_blocks(gw, 0, (size_t)j->F * j->W, 32);
+    dct_group(groups, *j).add(0, *j);
   } else {
     p_dep.none();
     p_conv.none();
@@ -1086,8 +1120,14 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
   std::vector<char> buf;
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),
-               o_wa = append(buf, p_wa.start), o_gr = append(buf, p_gr.start),
-               o_h = append(buf, gh), o_w = append(buf, gw);
+               o_wa = append(buf, p_wa.start), o_gr = append(buf, p_gr.start);
+  std::vector<std::array<size_t, 4>> o_groups;
+  for (auto &kv : groups) {
+    DctGroup &dg = kv.second;
+    dg.pad(tasks.size());
+    o_groups.push_back({append(buf, dg.h.start), append(buf, dg.w.start), append(buf, dg.h_wide.start),
+                        append(buf, dg.w_wide.start)});
+  }
   HIP(hipStreamSynchronize(stream_));
   ensure_stage(aux_, buf.size());
   std::memcpy(aux_.h_stage, buf.data(), buf.size());
@@ -1103,16 +1143,11 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   g_timer = &timer;
   launch_wa(d_tasks, mp, stream_);
   launch_density(d_tasks, mp, stream_);
-  if (j->F > 0) {
-    DctLaunch d{};
-    d.map_h = reinterpret_cast<const int2 *>(d_stage_ + o_h);
-    d.map_w = reinterpret_cast<const int2 *>(d_stage_ + o_w);
-    d.row_blocks_h = (int)gh.size();
-    d.row_blocks_w = (int)gw.size();
-    d.W = (int)j->W;
-    d.H = (int)j->H;
-    d.precision = (int)j->opt.precision;
-    launch_poisson(d_tasks, d, stream_);
+  size_t gi = 0;
+  for (auto &kv : groups) {
+    const size_t off[4] = {o_groups[gi][0], o_groups[gi][1], o_groups[gi][2], o_groups[gi][3]};
+    launch_poisson(d_tasks, kv.second.device(d_stage_, off), stream_);
+    ++gi;
   }
   launch_gather(d_tasks, mp, stream_);
   launch_combine(d_tasks, mp, stream_);
