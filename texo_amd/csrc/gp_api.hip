@@ -362,7 +362,7 @@ int texo_gp_session_rudy(texo_gp_session *s, const double *coordinates, double *
     HIP(hipMemcpyAsync(J.d_eval_coords, coordinates, J.C * sizeof(double), hipMemcpyHostToDevice, st));
     HIP(hipMemsetAsync(J.d_rudy_err, 0, sizeof(unsigned int), st));
     texo::g_timer = &s->engine->timer;
-    texo::launch_rudy((int)J.n_nets, J.d_net_start, J.d_pin_unit, J.d_pin_xy, J.d_eval_coords, (int)J.W, (int)J.H,
+    texo::launch_rudy((int)J.n_nets, J.d_net_start, J.d_pin_unit, J.js.pin_xy, J.d_eval_coords, (int)J.W, (int)J.H,
                       J.d_rudy_fx, J.d_rudy, J.d_rudy_err, st);
     texo::g_timer = nullptr;
     unsigned int err = 0;

@@ -31,6 +31,20 @@ constexpr int UG_STRIDE = 2 * MAXF + 3;  // unit partials: wa l1, force l1, raw 
 // grid has 8 * per blocks and XCD x runs the logical slice [x per, (x+1) per),
 // so a slice holds one job (or one spatial part of one) and each XCD's L2
 // caches that job's arrays.  Only speed depends on the dispatch order.
+// Per-element (x, y) offsets, stored as FP64 pairs or, when every value of
+// the job is exact in FP32, as FP32 pairs (half the bytes, the same values).
+struct OffArr {
+  const double2 *d;
+  const float2 *f;
+  __host__ __device__ __forceinline__ double2 operator[](size_t i) const {
+    if (f) {
+      const float2 v = f[i];
+      return make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
+    }
+    return d[i];
+  }
+};
+
 struct BlockPlan {
   const int *start;  // [n_tasks + 1]
   int n_tasks, total, per;
@@ -151,7 +165,7 @@ struct JobStatic {
   int W, H, F;
   int n_members, n_fillers, n_units, n_density_units;
   const uint32_t *m_unit;     // member -> unit
-  const double2 *m_off;       // member offset
+  OffArr m_off;               // member offset
   const double *m_charge;     // member charge
   const uint8_t *m_field;     // member field
   const uint32_t *du_unit;    // density unit -> unit
@@ -166,7 +180,7 @@ struct JobStatic {
   // charges in deposit order: coordinate index (unit, or n_units + filler),
   // offset from it, charge, layer = 2 field + (1 for a filler)
   const uint32_t *s_ci;
-  const double2 *s_off;
+  OffArr s_off;
   const double *s_q;
   const uint8_t *s_layer;
   const double *lower, *upper;
@@ -178,7 +192,7 @@ struct JobStatic {
   const uint32_t *net_start;
   const double *net_weight;
   const uint32_t *pin_unit;   // NO_UNIT for fixed pins
-  const double2 *pin_xy;      // offset, or fixed coordinate
+  OffArr pin_xy;              // offset, or fixed coordinate
   const uint32_t *u_pstart;   // unit -> incident movable pins
   const uint32_t *pin_slot;  // pin -> its entry in the unit-ordered derivative array (NO_UNIT: fixed pin)
   const uint32_t *u_nbase;   // unit -> number of its net pins (its timed-arc pins follow them)
@@ -232,6 +246,7 @@ struct SortTask {
   uint32_t *perm;         // [members + fillers] the new order
   uint32_t *s_ci;   // the charges in the new order
   double2 *s_off;
+  float2 *s_off32;  // instead of s_off for a job with FP32 offsets
   double *s_q;
   uint8_t *s_layer;
 };
@@ -286,7 +301,7 @@ void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s);
 
 // RUDY (eplace.rs:856-937): `fx` is [2G] scratch, `out` receives
 // horizontal[G] then vertical[G].
-void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, const double2 *pin_xy,
+void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, OffArr pin_xy,
                  const double *coords, int W, int H, long long *fx, double *out, unsigned int *error, hipStream_t s);
 
 // Poisson transform (gp_dct.hip) of every task of one grid size.  map_h has

@@ -494,7 +494,7 @@ void Engine::plan(Job *j) {
 void Engine::area_round(Job *j) {
   await_uploads();
   HIP(hipMemsetAsync(j->d_rudy_err, 0, sizeof(unsigned int), stream_));
-  launch_rudy((int)j->n_nets, j->d_net_start, j->d_pin_unit, j->d_pin_xy, j->d_vec[j->ix], (int)j->W, (int)j->H,
+  launch_rudy((int)j->n_nets, j->d_net_start, j->d_pin_unit, j->js.pin_xy, j->d_vec[j->ix], (int)j->W, (int)j->H,
               j->d_rudy_fx, j->d_rudy, j->d_rudy_err, stream_);
   const size_t G = j->G, U = j->U;
   std::vector<double> demand(2 * G), x(2 * U);

@@ -202,6 +202,8 @@ struct Job {
   long long *d_base = nullptr;
   uint32_t *d_m_unit = nullptr, *d_du_unit = nullptr, *d_du_mstart = nullptr;
   double2 *d_m_off = nullptr, *d_pin_xy = nullptr;
+  float2 *d_m_off32 = nullptr, *d_pin_xy32 = nullptr;  // instead, when off32
+  bool off32 = false;  // every member offset and pin offset / fixed coordinate is exact in FP32
   double *d_m_charge = nullptr, *d_fl_charge = nullptr, *d_du_fcharge = nullptr;
   uint8_t *d_m_field = nullptr, *d_fl_field = nullptr;
   uint32_t *d_net_start = nullptr, *d_pin_unit = nullptr;
@@ -231,6 +233,7 @@ struct Job {
   double *d_eval_coords = nullptr;
   uint32_t *d_perm = nullptr, *d_perm_next = nullptr, *d_sort_count = nullptr, *d_s_ci = nullptr;
   double2 *d_s_off = nullptr;
+  float2 *d_s_off32 = nullptr;
   double *d_s_q = nullptr;
   uint8_t *d_s_layer = nullptr;
   int tiles_x = 0, n_keys = 0;

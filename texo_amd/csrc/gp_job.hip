@@ -631,7 +631,7 @@ void Job::upload(const Basis *b, const std::shared_ptr<BlockPool> &pool, hipStre
   js.n_units = (int)U;
   js.n_density_units = (int)DU;
   js.m_unit = d_m_unit;
-  js.m_off = d_m_off;
+  js.m_off = OffArr{off32 ? nullptr : d_m_off, off32 ? d_m_off32 : nullptr};
   js.m_charge = d_m_charge;
   js.m_field = d_m_field;
   js.du_unit = d_du_unit;
@@ -642,7 +642,7 @@ void Job::upload(const Basis *b, const std::shared_ptr<BlockPool> &pool, hipStre
   js.unit_du = d_unit_du;
   js.perm = d_perm;
   js.s_ci = d_s_ci;
-  js.s_off = d_s_off;
+  js.s_off = OffArr{off32 ? nullptr : d_s_off, off32 ? d_s_off32 : nullptr};
   js.s_q = d_s_q;
   js.s_layer = d_s_layer;
   js.tiles_x = tiles_x;
@@ -655,7 +655,7 @@ void Job::upload(const Basis *b, const std::shared_ptr<BlockPool> &pool, hipStre
   js.net_start = d_net_start;
   js.net_weight = d_net_weight;
   js.pin_unit = d_pin_unit;
-  js.pin_xy = d_pin_xy;
+  js.pin_xy = OffArr{off32 ? nullptr : d_pin_xy, off32 ? d_pin_xy32 : nullptr};
   js.u_pstart = d_u_pstart;
   js.pin_slot = d_pin_slot;
   js.u_nbase = d_u_nbase;
@@ -739,7 +739,23 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   };
   const size_t o_base = data(d_base, FG, nullptr);
   data(d_m_unit, M, m_unit.data());
-  data(d_m_off, M, m_off.data());
+  // offsets as FP32 pairs when that is exact for all of them
+  auto fp32_exact = [](const std::vector<double> &v) {
+    for (double x : v)
+      if (!(static_cast<double>(static_cast<float>(x)) == x)) return false;
+    return true;
+  };
+  off32 = fp32_exact(m_off) && fp32_exact(pin_xy);
+  std::vector<float> m_off32, pin_xy32;
+  if (off32) {
+    m_off32.assign(m_off.begin(), m_off.end());
+    pin_xy32.assign(pin_xy.begin(), pin_xy.end());
+    data(d_m_off32, M, m_off32.data());
+    data(d_m_off, 0, nullptr);
+  } else {
+    data(d_m_off, M, m_off.data());
+    data(d_m_off32, 0, nullptr);
+  }
   data(d_m_charge, M, m_charge.data());
   data(d_m_field, M, m_field.data());
   data(d_du_unit, DU, du_unit.data());
@@ -749,7 +765,13 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   data(d_fl_charge, Fl, fl_charge.data());
   data(d_net_start, n_net_total + 1, net_start.data());
   data(d_pin_unit, n_pins_total, pin_unit.data());
-  data(d_pin_xy, n_pins_total, pin_xy.data());
+  if (off32) {
+    data(d_pin_xy32, n_pins_total, pin_xy32.data());
+    data(d_pin_xy, 0, nullptr);
+  } else {
+    data(d_pin_xy, n_pins_total, pin_xy.data());
+    data(d_pin_xy32, 0, nullptr);
+  }
   data(d_net_weight, n_net_total, net_weight.data());
   data(d_wa_seg, wa_seg.size(), wa_seg.data());
   data(d_u_pstart, U + 1, u_pstart.data());
@@ -761,7 +783,12 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   data(d_upper, C, upper.data());
   const size_t o_x0 = data(d_vec[ix], C, nullptr), o_x1 = data(d_vec[ixp], C, nullptr);
   const size_t o_perm = data(d_perm, n, nullptr), o_ci = data(d_s_ci, n, nullptr);
-  const size_t o_off = data(d_s_off, n, nullptr), o_q = data(d_s_q, n, nullptr);
+  const size_t o_off = off32 ? data(d_s_off32, n, nullptr) : data(d_s_off, n, nullptr);
+  if (off32)
+    data(d_s_off, 0, nullptr);
+  else
+    data(d_s_off32, 0, nullptr);
+  const size_t o_q = data(d_s_q, n, nullptr);
   const size_t o_layer = data(d_s_layer, n, nullptr);
   image_bytes = arena.planned();
   zero_begin = image_bytes;
@@ -862,13 +889,24 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   std::memcpy(image + o_x1, x0, C * sizeof(double));
   // input order until the engine sorts the charges by tile
   uint32_t *perm = reinterpret_cast<uint32_t *>(image + o_perm), *ci = reinterpret_cast<uint32_t *>(image + o_ci);
-  double *off = reinterpret_cast<double *>(image + o_off), *q = reinterpret_cast<double *>(image + o_q);
+  double *q = reinterpret_cast<double *>(image + o_q);
+  // sorted-copy offsets in the job's offset precision
+  auto put_off = [&](size_t i, double x, double y) {
+    if (off32) {
+      float *o = reinterpret_cast<float *>(image + o_off);
+      o[2 * i] = static_cast<float>(x);
+      o[2 * i + 1] = static_cast<float>(y);
+    } else {
+      double *o = reinterpret_cast<double *>(image + o_off);
+      o[2 * i] = x;
+      o[2 * i + 1] = y;
+    }
+  };
   uint8_t *layer = reinterpret_cast<uint8_t *>(image + o_layer);
   for (size_t i = 0; i < M; ++i) {
     perm[i] = (uint32_t)i;
     ci[i] = m_unit[i];
-    off[2 * i] = m_off[2 * i];
-    off[2 * i + 1] = m_off[2 * i + 1];
+    put_off(i, m_off[2 * i], m_off[2 * i + 1]);
     q[i] = m_charge[i];
     layer[i] = (uint8_t)(2 * m_field[i]);
   }
@@ -876,8 +914,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
     const size_t i = M + k;
     perm[i] = (uint32_t)i;
     ci[i] = (uint32_t)(U + k);
-    off[2 * i] = 0.0;
-    off[2 * i + 1] = 0.0;
+    put_off(i, 0.0, 0.0);
     q[i] = fl_charge[k];
     layer[i] = (uint8_t)(2 * fl_field[k] + 1);
   }
@@ -934,7 +971,8 @@ SortTask Job::sort_task(const double *coords) {
   std::swap(d_perm, d_perm_next);
   js.perm = d_perm;
   t.s_ci = d_s_ci;
-  t.s_off = d_s_off;
+  t.s_off = off32 ? nullptr : d_s_off;
+  t.s_off32 = off32 ? d_s_off32 : nullptr;
   t.s_q = d_s_q;
   t.s_layer = d_s_layer;
   return t;

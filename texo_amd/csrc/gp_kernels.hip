@@ -340,13 +340,19 @@ __global__ void __launch_bounds__(BLOCK) k_sort_scatter(const SortTask *__restri
   t.perm[pos] = static_cast<uint32_t>(c);
   if (c < a.n_members) {
     t.s_ci[pos] = a.m_unit[c];
-    t.s_off[pos] = a.m_off[c];
+    if (t.s_off32)
+      t.s_off32[pos] = a.m_off.f[c];
+    else
+      t.s_off[pos] = a.m_off.d[c];
     t.s_q[pos] = a.m_charge[c];
     t.s_layer[pos] = static_cast<uint8_t>(2 * a.m_field[c]);
   } else {
     const int k = c - a.n_members;
     t.s_ci[pos] = static_cast<uint32_t>(a.n_units + k);
-    t.s_off[pos] = make_double2(0.0, 0.0);
+    if (t.s_off32)
+      t.s_off32[pos] = make_float2(0.f, 0.f);
+    else
+      t.s_off[pos] = make_double2(0.0, 0.0);
     t.s_q[pos] = a.fl_charge[k];
     t.s_layer[pos] = static_cast<uint8_t>(2 * a.fl_field[k] + 1);
   }
@@ -1153,7 +1159,7 @@ __device__ __forceinline__ double tile_overlap(int t, double rlo, double rhi) {
 
 __global__ void __launch_bounds__(BLOCK) k_rudy(int n_nets, const uint32_t *__restrict__ net_start,
                                                 const uint32_t *__restrict__ pin_unit,
-                                                const double2 *__restrict__ pin_xy,
+                                                OffArr pin_xy,
                                                 const double *__restrict__ coords, int W, int H,
                                                 long long *__restrict__ hfx, long long *__restrict__ vfx,
                                                 unsigned int *error) {
@@ -1277,7 +1283,7 @@ void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_optimizer_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
-void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, const double2 *pin_xy,
+void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, OffArr pin_xy,
                  const double *coords, int W, int H, long long *fx, double *out, unsigned int *error, hipStream_t s) {
   const size_t G = (size_t)W * H;
   hipLaunchKernelGGL(k_fill_ll, dim3(std::max(1, blocks_for(2 * G))), dim3(BLOCK), 0, s, 2 * G, fx);
