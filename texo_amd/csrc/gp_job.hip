@@ -781,15 +781,11 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   data(d_unit_du, U, unit_du.data());
   data(d_lower, C, lower.data());
   data(d_upper, C, upper.data());
-  const size_t o_x0 = data(d_vec[ix], C, nullptr), o_x1 = data(d_vec[ixp], C, nullptr);
-  const size_t o_perm = data(d_perm, n, nullptr), o_ci = data(d_s_ci, n, nullptr);
-  const size_t o_off = off32 ? data(d_s_off32, n, nullptr) : data(d_s_off, n, nullptr);
-  if (off32)
-    data(d_s_off, 0, nullptr);
-  else
-    data(d_s_off32, 0, nullptr);
-  const size_t o_q = data(d_s_q, n, nullptr);
-  const size_t o_layer = data(d_s_layer, n, nullptr);
+  // the previous iterate starts as a copy of x (DynamicNesterovState::new,
+  // on the device); the sorted charge copies are first written by the sort
+  // that precedes any deposit, which walks the input order `perm`
+  const size_t o_x0 = data(d_vec[ix], C, nullptr);
+  const size_t o_perm = data(d_perm, n, nullptr);
   image_bytes = arena.planned();
   zero_begin = image_bytes;
   for (auto &sl : slot) {
@@ -801,7 +797,12 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   arena.reserve(d_part_metric, blocks(U + Fl, BLOCK) * 3);
   arena.reserve(d_part_step, blocks(C, BLOCK) * 4);
   for (int k = 0; k < 4; ++k)
-    if (k != ix && k != ixp) arena.reserve(d_vec[k], C);
+    if (k != ix) arena.reserve(d_vec[k], C);
+  arena.reserve(d_s_ci, n);
+  arena.reserve(d_s_off, off32 ? 0 : n);
+  arena.reserve(d_s_off32, off32 ? n : 0);
+  arena.reserve(d_s_q, n);
+  arena.reserve(d_s_layer, n);
   for (auto &g : d_grad) arena.reserve(g, C);
   arena.reserve(d_best, 2 * U);
   for (auto &sl : slot) {
@@ -886,38 +887,9 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
     x0[2 * (U + k) + 1] = fl_xy[2 * k + 1];
   }
   for (size_t i = 0; i < C; ++i) x0[i] = clampd(x0[i], lower[i], upper[i]);
-  std::memcpy(image + o_x1, x0, C * sizeof(double));
   // input order until the engine sorts the charges by tile
-  uint32_t *perm = reinterpret_cast<uint32_t *>(image + o_perm), *ci = reinterpret_cast<uint32_t *>(image + o_ci);
-  double *q = reinterpret_cast<double *>(image + o_q);
-  // sorted-copy offsets in the job's offset precision
-  auto put_off = [&](size_t i, double x, double y) {
-    if (off32) {
-      float *o = reinterpret_cast<float *>(image + o_off);
-      o[2 * i] = static_cast<float>(x);
-      o[2 * i + 1] = static_cast<float>(y);
-    } else {
-      double *o = reinterpret_cast<double *>(image + o_off);
-      o[2 * i] = x;
-      o[2 * i + 1] = y;
-    }
-  };
-  uint8_t *layer = reinterpret_cast<uint8_t *>(image + o_layer);
-  for (size_t i = 0; i < M; ++i) {
-    perm[i] = (uint32_t)i;
-    ci[i] = m_unit[i];
-    put_off(i, m_off[2 * i], m_off[2 * i + 1]);
-    q[i] = m_charge[i];
-    layer[i] = (uint8_t)(2 * m_field[i]);
-  }
-  for (size_t k = 0; k < Fl; ++k) {
-    const size_t i = M + k;
-    perm[i] = (uint32_t)i;
-    ci[i] = (uint32_t)(U + k);
-    put_off(i, 0.0, 0.0);
-    q[i] = fl_charge[k];
-    layer[i] = (uint8_t)(2 * fl_field[k] + 1);
-  }
+  uint32_t *perm = reinterpret_cast<uint32_t *>(image + o_perm);
+  for (size_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
 }
 
 Task Job::task(int sl, const double *coords, double *grad, const double *metric, double g, bool skip,
