@@ -309,13 +309,15 @@ int texo_gp_session_coordinates(const texo_gp_session *session, double *coordina
 int texo_gp_session_rudy(texo_gp_session *session, const double *coordinates,
                          double *horizontal, double *vertical);
 
-/* Kernel timing: when enabled, each launch of the named kernel on the
- * session stream is bracketed by HIP events; `avg_ms` is the mean duration
- * and `count` the number of launches since the last reset. */
+/* Kernel timing: when enabled, each launch of the named kernel ("deposit",
+ * "wa", "gather", "dct", "rudy") is bracketed by HIP events on the stream it
+ * runs on; `avg_ms` is the mean duration, `count` the number of launches
+ * since the last reset and `avg_tasks` (may be null) the mean number of
+ * evaluations one launch covered. */
 int texo_gp_session_set_kernel_timing(texo_gp_session *session, const char *kernel,
                                       int enable);
 int texo_gp_session_kernel_timing(texo_gp_session *session, double *avg_ms,
-                                  uint64_t *count);
+                                  uint64_t *count, double *avg_tasks);
 
 /* Runs exactly `iterations` Nesterov iterations of the hot loop
  * (optimize_density_once, eplace.rs:503-574) with the stop rule disabled:
@@ -348,9 +350,11 @@ int texo_gp_batch_result(texo_gp_batch *batch, uint64_t job, uint32_t *out_targe
 int texo_gp_batch_sizes(const texo_gp_batch *batch, uint64_t job, uint64_t *coordinates, uint32_t *fields,
                         uint64_t *fillers, uint64_t *density_units);
 int texo_gp_batch_coordinates(texo_gp_batch *batch, uint64_t job, double *coordinates);
-/* Kernel timing over the batch's launches (one launch covers every job). */
+/* Kernel timing over the batch's launches (one launch covers the
+ * evaluations of every job of one of the batch's two job groups). */
 int texo_gp_batch_set_kernel_timing(texo_gp_batch *batch, const char *kernel, int enable);
-int texo_gp_batch_kernel_timing(texo_gp_batch *batch, double *avg_ms, uint64_t *count);
+int texo_gp_batch_kernel_timing(texo_gp_batch *batch, double *avg_ms, uint64_t *count,
+                                double *avg_tasks);
 void texo_gp_batch_destroy(texo_gp_batch *batch);
 
 /* ---- serving: continuous batching of many placement jobs --------------- */
@@ -372,6 +376,8 @@ typedef struct texo_gp_server_stats {
   double busy_seconds;      /* wall time spent in rounds */
   double admit_seconds;     /* server-thread time admitting jobs (arena, upload enqueue) */
   double retire_seconds;    /* server-thread time reading back and releasing finished jobs */
+  double wait_seconds;      /* part of busy_seconds spent waiting for the GPU to end a round */
+  double launch_seconds;    /* server-thread time planning rounds and enqueueing their work */
 } texo_gp_server_stats;
 
 texo_gp_server *texo_gp_server_create(int32_t device, uint64_t hbm_budget_bytes, uint32_t max_batch);

@@ -43,7 +43,7 @@ def load():
     lib.texo_gp_session_poisson.restype = C.c_int
     lib.texo_gp_session_set_kernel_timing.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.texo_gp_session_set_kernel_timing.restype = C.c_int
-    lib.texo_gp_session_kernel_timing.argtypes = [C.c_void_p, P(d), P(C.c_uint64)]
+    lib.texo_gp_session_kernel_timing.argtypes = [C.c_void_p, P(d), P(C.c_uint64), P(d)]
     lib.texo_gp_session_kernel_timing.restype = C.c_int
     _lib = lib
     return lib
@@ -75,11 +75,12 @@ class Session(abi.Session):
         self.lib.texo_gp_session_set_kernel_timing(self.handle, (kernel or "").encode(), 1 if kernel else 0)
 
     def kernel_timing(self):
-        avg, count = C.c_double(), C.c_uint64()
-        rc = self.lib.texo_gp_session_kernel_timing(self.handle, C.byref(avg), C.byref(count))
+        """(average ms per launch, launches, average evaluations per launch)."""
+        avg, count, tasks = C.c_double(), C.c_uint64(), C.c_double()
+        rc = self.lib.texo_gp_session_kernel_timing(self.handle, C.byref(avg), C.byref(count), C.byref(tasks))
         if rc != 0:
             raise PlacementError.from_status(rc)
-        return avg.value, count.value
+        return avg.value, count.value, tasks.value
 
 
 def place(problem: Problem, dct_precision: int = abi.DCT_FP32, fixed_iterations: int = 0, trace: bool = False,
@@ -227,11 +228,12 @@ class Batch:
         self.lib.texo_gp_batch_set_kernel_timing(self.handle, (kernel or "").encode(), 1 if kernel else 0)
 
     def kernel_timing(self):
-        avg, count = C.c_double(), C.c_uint64()
-        rc = self.lib.texo_gp_batch_kernel_timing(self.handle, C.byref(avg), C.byref(count))
+        """(average ms per launch, launches, average evaluations per launch)."""
+        avg, count, tasks = C.c_double(), C.c_uint64(), C.c_double()
+        rc = self.lib.texo_gp_batch_kernel_timing(self.handle, C.byref(avg), C.byref(count), C.byref(tasks))
         if rc != 0:
             raise PlacementError.from_status(rc)
-        return avg.value, count.value
+        return avg.value, count.value, tasks.value
 
     def close(self):
         if getattr(self, "handle", None):

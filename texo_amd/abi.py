@@ -406,6 +406,7 @@ class texo_gp_server_stats(C.Structure):
         ("failed", C.c_uint64), ("rounds", C.c_uint64), ("active", C.c_uint64),
         ("pending", C.c_uint64), ("hbm_in_use", C.c_uint64), ("busy_seconds", C.c_double),
         ("admit_seconds", C.c_double), ("retire_seconds", C.c_double),
+        ("wait_seconds", C.c_double), ("launch_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -446,5 +447,5 @@ def declare_batch_api(lib, prefix: str):
     f("batch_sizes", C.c_int, C.c_void_p, C.c_uint64, P(C.c_uint64), P(C.c_uint32), P(C.c_uint64), P(C.c_uint64))
     f("batch_coordinates", C.c_int, C.c_void_p, C.c_uint64, P(C.c_double))
     f("batch_set_kernel_timing", C.c_int, C.c_void_p, C.c_char_p, C.c_int)
-    f("batch_kernel_timing", C.c_int, C.c_void_p, P(C.c_double), P(C.c_uint64))
+    f("batch_kernel_timing", C.c_int, C.c_void_p, P(C.c_double), P(C.c_uint64), P(C.c_double))
     f("batch_destroy", None, C.c_void_p)

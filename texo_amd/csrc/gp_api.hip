@@ -62,8 +62,9 @@ void fill_report(Engine &e, Job &j, texo_gp_report *r) {
     if (j.empty) {
       x = j.initial_xy;
     } else {
-      HIP(hipMemcpyAsync(x.data(), j.d_vec[j.ix], 2 * j.U * sizeof(double), hipMemcpyDeviceToHost, e.stream()));
-      HIP(hipStreamSynchronize(e.stream()));
+      const hipStream_t st = e.stream();
+      HIP(hipMemcpyAsync(x.data(), j.d_vec[j.ix], 2 * j.U * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP(hipStreamSynchronize(st));
     }
   }
   fill_report_at(j, &x, r);
@@ -78,8 +79,9 @@ void copy_targets(Engine &e, Job &j, uint32_t *out) {
       out[i] = texo::rounded_coordinate(j.initial_xy[i], (i & 1) ? j.H : j.W);
     return;
   }
-  HIP(hipMemcpyAsync(out, j.d_best, 2 * j.U * sizeof(uint32_t), hipMemcpyDeviceToHost, e.stream()));
-  HIP(hipStreamSynchronize(e.stream()));
+  const hipStream_t st = e.stream();
+  HIP(hipMemcpyAsync(out, j.d_best, 2 * j.U * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIP(hipStreamSynchronize(st));
 }
 
 void drive(Engine &e, Job &j) {
@@ -91,11 +93,13 @@ int timing_set(Engine &e, const char *kernel, int enable) {
   e.timer.name = kernel ? kernel : "";
   e.timer.enabled = enable != 0;
   e.timer.used = 0;
+  e.timer.tasks = 0.0;
   return 0;
 }
 
-// Average duration of the named kernel's launches since the last call.
-int timing_get(Engine &e, double *avg_ms, uint64_t *count) {
+// Average duration of the named kernel's launches since the last call, and
+// the average number of evaluations one launch covered.
+int timing_get(Engine &e, double *avg_ms, uint64_t *count, double *avg_tasks) {
   texo::KernelTimer &t = e.timer;
   if (hipSetDevice(e.device()) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return TEXO_GP_ERR_DEVICE;
   double total = 0.0;
@@ -106,7 +110,9 @@ int timing_get(Engine &e, double *avg_ms, uint64_t *count) {
   }
   if (avg_ms) *avg_ms = t.used ? total / (double)t.used : 0.0;
   if (count) *count = t.used;
+  if (avg_tasks) *avg_tasks = t.used ? t.tasks / (double)t.used : 0.0;
   t.used = 0;
+  t.tasks = 0.0;
   return 0;
 }
 
@@ -346,11 +352,14 @@ int texo_gp_session_coordinates(const texo_gp_session *s, double *coordinates) {
     std::copy(J.initial_xy.begin(), J.initial_xy.end(), coordinates);
     return 0;
   }
-  if (hipSetDevice(s->engine->device()) != hipSuccess) return TEXO_GP_ERR_DEVICE;
-  if (hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, s->engine->stream()) !=
-          hipSuccess ||
-      hipStreamSynchronize(s->engine->stream()) != hipSuccess)
+  try {
+    HIP(hipSetDevice(s->engine->device()));
+    const hipStream_t st = s->engine->stream();
+    HIP(hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+  } catch (const Failure &) {
     return TEXO_GP_ERR_DEVICE;
+  }
   return 0;
 }
 
@@ -381,9 +390,9 @@ int texo_gp_session_set_kernel_timing(texo_gp_session *s, const char *kernel, in
   return timing_set(*s->engine, kernel, enable);
 }
 
-int texo_gp_session_kernel_timing(texo_gp_session *s, double *avg_ms, uint64_t *count) {
+int texo_gp_session_kernel_timing(texo_gp_session *s, double *avg_ms, uint64_t *count, double *avg_tasks) {
   if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
-  return timing_get(*s->engine, avg_ms, count);
+  return timing_get(*s->engine, avg_ms, count, avg_tasks);
 }
 
 int texo_gp_place(const texo_gp_problem *problem, const texo_gp_options *options, uint32_t *out_targets_xy,
@@ -525,11 +534,14 @@ int texo_gp_batch_coordinates(texo_gp_batch *b, uint64_t job, double *coordinate
     std::copy(J.initial_xy.begin(), J.initial_xy.end(), coordinates);
     return 0;
   }
-  if (hipSetDevice(b->engine->device()) != hipSuccess ||
-      hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, b->engine->stream()) !=
-          hipSuccess ||
-      hipStreamSynchronize(b->engine->stream()) != hipSuccess)
+  try {
+    HIP(hipSetDevice(b->engine->device()));
+    const hipStream_t st = b->engine->stream();
+    HIP(hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+  } catch (const Failure &) {
     return TEXO_GP_ERR_DEVICE;
+  }
   return 0;
 }
 
@@ -538,9 +550,9 @@ int texo_gp_batch_set_kernel_timing(texo_gp_batch *b, const char *kernel, int en
   return timing_set(*b->engine, kernel, enable);
 }
 
-int texo_gp_batch_kernel_timing(texo_gp_batch *b, double *avg_ms, uint64_t *count) {
+int texo_gp_batch_kernel_timing(texo_gp_batch *b, double *avg_ms, uint64_t *count, double *avg_tasks) {
   if (!b) return TEXO_GP_ERR_INVALID_ARGUMENT;
-  return timing_get(*b->engine, avg_ms, count);
+  return timing_get(*b->engine, avg_ms, count, avg_tasks);
 }
 
 void texo_gp_batch_destroy(texo_gp_batch *b) { delete b; }
@@ -586,7 +598,7 @@ struct texo_gp_server {
   std::vector<hipEvent_t> spare_events;
 
   void loop();
-  void retire(std::vector<int64_t> &group);
+  void retire(int c, std::vector<int64_t> &group);
   void collect(bool block);
   ~texo_gp_server() {
     for (auto &r : readbacks)
@@ -639,8 +651,13 @@ void texo_gp_server::loop() {
         stats.admit_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         active.push_back(id);
         charges_in_flight += e.job->M + e.job->Fl;
+        // the two groups' rounds overlap on the GPU: once there is enough
+        // work for two, a new job joins the group with fewer charges
+        size_t in_group[2] = {0, 0};
+        for (int c = 0; c < 2; ++c)
+          for (int64_t g : group[c]) in_group[c] += entries[g]->job->M + entries[g]->job->Fl;
         const bool split = charges_in_flight >= texo::PIPELINE_MIN_CHARGES;
-        group[split && group[1].size() < group[0].size() ? 1 : 0].push_back(id);
+        group[split && in_group[1] < in_group[0] ? 1 : 0].push_back(id);
         hbm_in_use += need;
         stats.admitted += 1;
       }
@@ -664,13 +681,15 @@ void texo_gp_server::loop() {
         std::lock_guard<std::mutex> lk(mu);
         stats.rounds += 1;
         stats.busy_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        stats.wait_seconds += engine->last_wait_seconds();
       }
-      retire(group[c]);
+      retire(c, group[c]);
       std::vector<Job *> jobs;
       {
         std::lock_guard<std::mutex> lk(mu);
         for (int64_t id : group[c]) jobs.push_back(entries[id]->job.get());
       }
+      const auto t1 = std::chrono::steady_clock::now();
       try {
         live[c] = engine->launch(c, jobs);
       } catch (const Failure &f) {
@@ -683,7 +702,11 @@ void texo_gp_server::loop() {
         }
         live[c] = false;
       }
-      if (!live[c]) retire(group[c]);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        stats.launch_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+      }
+      if (!live[c]) retire(c, group[c]);
     }
     // nothing in flight to overlap with: wait for the pending results
     if (!live[0] && !live[1]) collect(true);
@@ -694,7 +717,7 @@ void texo_gp_server::loop() {
 // on the compute stream, after the work the last round's advance left there
 // for them; collect() completes them once the copies have landed, so the
 // host never waits here for the round in flight.
-void texo_gp_server::retire(std::vector<int64_t> &group) {
+void texo_gp_server::retire(int c, std::vector<int64_t> &group) {
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<int64_t> still;
   Readback rb{nullptr, {}};
@@ -713,7 +736,7 @@ void texo_gp_server::retire(std::vector<int64_t> &group) {
     }
     if (!j.empty && j.arena.base() && j.U) {
       try {
-        if (!st) st = engine->stream();
+        if (!st) st = engine->stream(c);
         e.readback = engine->image_pool()->acquire(j.U * 24, &e.readback_cap);
         char *dst = static_cast<char *>(e.readback);
         HIP(hipMemcpyAsync(dst, j.d_vec[j.ix], j.U * 16, hipMemcpyDeviceToHost, st));
@@ -735,11 +758,11 @@ void texo_gp_server::retire(std::vector<int64_t> &group) {
     } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
       ev = nullptr;
     }
-    if (ev && hipEventRecord(ev, engine->stream()) != hipSuccess) {
+    if (ev && hipEventRecord(ev, engine->stream(c)) != hipSuccess) {
       (void)hipEventDestroy(ev);
       ev = nullptr;
     }
-    if (!ev) (void)hipStreamSynchronize(engine->stream());  // no marker: land them now
+    if (!ev) (void)hipStreamSynchronize(engine->stream(c));  // no marker: land them now
     rb.ev = ev;
     readbacks.push_back(std::move(rb));
   }

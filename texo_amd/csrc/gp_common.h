@@ -285,6 +285,7 @@ struct OptTask {
 struct Maps {
   BlockPlan deposit, convert, wa, gradient;
   int n_tasks;
+  int n_density;  // tasks whose density is computed (not skipped)
 };
 
 void launch_density(const Task *tasks, const Maps &m, hipStream_t s);
@@ -316,6 +317,7 @@ struct DctLaunch {
   int precision;
   bool fast;     // every task's W and H are multiples of 32
   bool wide_ok;  // ... and of 128, at most 256
+  int n_tasks;   // tasks with tiles in this launch
 };
 void launch_poisson(const Task *tasks, const DctLaunch &d, hipStream_t s);
 int poisson_energy_blocks(int W, int H, int F);  // per task
@@ -325,7 +327,7 @@ bool poisson_wide_shape(int W, int H);
 // Kernel timing hook: the engine brackets launches of one named kernel.
 struct KernelTimer;
 void timer_begin(KernelTimer *t, const char *name, hipStream_t s);
-void timer_end(KernelTimer *t, const char *name, hipStream_t s);
+void timer_end(KernelTimer *t, const char *name, hipStream_t s, int tasks);  // tasks: evaluations the launch covers
 extern thread_local KernelTimer *g_timer;
 
 }  // namespace texo

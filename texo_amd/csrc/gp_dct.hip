@@ -787,7 +787,7 @@ void launch_poisson(const Task *tasks, const DctLaunch &d, hipStream_t s) {
   launch_stage<1>(tasks, d, wide, s);
   launch_stage<2>(tasks, d, wide, s);
   launch_stage<3>(tasks, d, wide, s);
-  timer_end(g_timer, "dct", s);
+  timer_end(g_timer, "dct", s, d.n_tasks);
 }
 
 }  // namespace texo

@@ -13,6 +13,7 @@
 //  * when the extrapolation factor is exactly 0, y == origin bit for bit and
 //    the reference reuses the origin evaluation (global_placement.rs:176-181).
 
+#include <chrono>
 #include "gp_host.h"
 
 #include <algorithm>
@@ -110,7 +111,7 @@ struct Plan {
 // shape class), one plan entry per task of the round.
 struct DctGroup {
   Plan h, w, h_wide, w_wide;
-  int precision = 0;
+  int precision = 0, n_tasks = 0;
   bool fast = false, wide_ok = false;
   void pad(size_t n_tasks) {
     for (Plan *p : {&h, &w, &h_wide, &w_wide})
@@ -118,6 +119,7 @@ struct DctGroup {
   }
   void add(size_t task, const Job &j) {
     pad(task);
+    n_tasks += 1;
     const size_t F = (size_t)j.F, W = j.W, H = j.H;
     h.add(((F * H + 31) / 32) * ((W + 31) / 32), 1);
     w.add(((F * W + 31) / 32) * ((H + 31) / 32), 1);
@@ -138,6 +140,7 @@ struct DctGroup {
     d.precision = precision;
     d.fast = fast;
     d.wide_ok = wide_ok;
+    d.n_tasks = n_tasks;
     return d;
   }
 };
@@ -234,12 +237,15 @@ void Job::begin_round() {
 Engine::Engine(int device, size_t pool_capacity) : device_(device) {
   HIP(hipSetDevice(device));
   // every stream keeps the default priority
-  HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  HIP(hipStreamCreateWithFlags(&wa_stream_, hipStreamNonBlocking));
+  for (RoundCtx &c : ctx_) {
+    HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIP(hipStreamCreateWithFlags(&c.wa_stream, hipStreamNonBlocking));
+    HIP(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
+    HIP(hipEventCreateWithFlags(&c.join, hipEventDisableTiming));
+  }
   HIP(hipStreamCreateWithFlags(&io_stream_, hipStreamNonBlocking));
-  HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
-  HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
   HIP(hipEventCreateWithFlags(&ev_upload_, hipEventDisableTiming));
+  HIP(hipEventCreateWithFlags(&ev_ctx1_, hipEventDisableTiming));
   blocks_ = std::make_shared<BlockPool>(BLOCK_POOL_KEEP, false);
   images_ = std::make_shared<BlockPool>(IMAGE_POOL_KEEP, true);
   HIP(hipMalloc(&d_pool_, pool_capacity * sizeof(JobScalars)));
@@ -252,8 +258,10 @@ Engine::Engine(int device, size_t pool_capacity) : device_(device) {
 
 Engine::~Engine() {
   (void)hipSetDevice(device_);
-  (void)hipStreamSynchronize(stream_);
-  (void)hipStreamSynchronize(wa_stream_);
+  for (RoundCtx &c : ctx_) {
+    (void)hipStreamSynchronize(c.stream);
+    (void)hipStreamSynchronize(c.wa_stream);
+  }
   (void)hipStreamSynchronize(io_stream_);
   bases_.clear();
   if (d_pool_) (void)hipFree(d_pool_);
@@ -263,12 +271,28 @@ Engine::~Engine() {
     if (c.done) (void)hipEventDestroy(c.done);
   }
   free_ctx(aux_);
-  (void)hipEventDestroy(ev_fork_);
-  (void)hipEventDestroy(ev_join_);
+  for (RoundCtx &c : ctx_) {
+    (void)hipEventDestroy(c.fork);
+    (void)hipEventDestroy(c.join);
+    (void)hipStreamDestroy(c.wa_stream);
+    (void)hipStreamDestroy(c.stream);
+  }
   (void)hipEventDestroy(ev_upload_);
+  (void)hipEventDestroy(ev_ctx1_);
   (void)hipStreamDestroy(io_stream_);
-  (void)hipStreamDestroy(wa_stream_);
-  (void)hipStreamDestroy(stream_);
+}
+
+hipStream_t Engine::stream() {
+  HIP(hipSetDevice(device_));
+  await_uploads(0);
+  HIP(hipEventRecord(ev_ctx1_, ctx_[1].stream));
+  HIP(hipStreamWaitEvent(ctx_[0].stream, ev_ctx1_, 0));
+  return ctx_[0].stream;
+}
+
+hipStream_t Engine::stream(int c) {
+  await_uploads(c);
+  return ctx_[c].stream;
 }
 
 const Basis *Engine::basis(int W, int H) {
@@ -296,14 +320,14 @@ void Engine::attach(Job *job) {
     // the first round sorts the charges by tile (the probe evaluation)
     job->upload(basis((int)job->W, (int)job->H), blocks_, io_stream_);
     HIP(hipEventRecord(ev_upload_, io_stream_));
-    uploads_pending_ = true;
+    for (RoundCtx &c : ctx_) c.upload_wait = true;
   }
 }
 
-void Engine::await_uploads() {
-  if (!uploads_pending_) return;
-  HIP(hipStreamWaitEvent(stream_, ev_upload_, 0));
-  uploads_pending_ = false;
+void Engine::await_uploads(int c) {
+  if (!ctx_[c].upload_wait) return;
+  HIP(hipStreamWaitEvent(ctx_[c].stream, ev_upload_, 0));
+  ctx_[c].upload_wait = false;
 }
 
 void Engine::detach(Job *job) {
@@ -349,7 +373,7 @@ void Engine::free_ctx(RoundCtx &c) {
 void Engine::sort_now(Job *j) {
   if (j->F == 0 || j->M + j->Fl == 0) return;
   HIP(hipSetDevice(device_));
-  await_uploads();
+  const hipStream_t st = stream();
   j->sorted = true;
   std::vector<SortTask> tasks{j->sort_task()};
   Plan pc, pk;
@@ -360,18 +384,18 @@ void Engine::sort_now(Job *j) {
   const size_t o_t = append(buf, tasks), o_c = append(buf, pc.start), o_k = append(buf, pk.start),
                o_l = append(buf, list);
   // wait for any round in flight before reusing the set-up staging buffer
-  HIP(hipStreamSynchronize(stream_));
+  HIP(hipStreamSynchronize(st));
   ensure_stage(aux_, buf.size());
   std::memcpy(aux_.h_stage, buf.data(), buf.size());
-  HIP(hipMemcpyAsync(aux_.d_stage, aux_.h_stage, buf.size(), hipMemcpyHostToDevice, stream_));
+  HIP(hipMemcpyAsync(aux_.d_stage, aux_.h_stage, buf.size(), hipMemcpyHostToDevice, st));
   SortMaps sm{};
   sm.charges = pc.device(aux_.d_stage, o_c);
   sm.keys = pk.device(aux_.d_stage, o_k);
   sm.list = reinterpret_cast<const int *>(aux_.d_stage + o_l);
   sm.n_list = 1;
-  launch_sort(reinterpret_cast<const SortTask *>(aux_.d_stage + o_t), sm, stream_);
+  launch_sort(reinterpret_cast<const SortTask *>(aux_.d_stage + o_t), sm, st);
   HIP(hipGetLastError());
-  HIP(hipStreamSynchronize(stream_));
+  HIP(hipStreamSynchronize(st));
 }
 
 // Schedule decisions that need no device work run here, before a job's
@@ -492,17 +516,17 @@ void Engine::plan(Job *j) {
 // propose_area_adjustment / apply_area_adjustment (eplace.rs:627-819): RUDY
 // on the device, the order-dependent charge bookkeeping on the host.
 void Engine::area_round(Job *j) {
-  await_uploads();
-  HIP(hipMemsetAsync(j->d_rudy_err, 0, sizeof(unsigned int), stream_));
+  await_uploads(cur_);
+  HIP(hipMemsetAsync(j->d_rudy_err, 0, sizeof(unsigned int), cur()));
   launch_rudy((int)j->n_nets, j->d_net_start, j->d_pin_unit, j->js.pin_xy, j->d_vec[j->ix], (int)j->W, (int)j->H,
-              j->d_rudy_fx, j->d_rudy, j->d_rudy_err, stream_);
+              j->d_rudy_fx, j->d_rudy, j->d_rudy_err, cur());
   const size_t G = j->G, U = j->U;
   std::vector<double> demand(2 * G), x(2 * U);
   unsigned int err = 0;
-  HIP(hipMemcpyAsync(demand.data(), j->d_rudy, 2 * G * sizeof(double), hipMemcpyDeviceToHost, stream_));
-  HIP(hipMemcpyAsync(x.data(), j->d_vec[j->ix], 2 * U * sizeof(double), hipMemcpyDeviceToHost, stream_));
-  HIP(hipMemcpyAsync(&err, j->d_rudy_err, sizeof(unsigned int), hipMemcpyDeviceToHost, stream_));
-  HIP(hipStreamSynchronize(stream_));
+  HIP(hipMemcpyAsync(demand.data(), j->d_rudy, 2 * G * sizeof(double), hipMemcpyDeviceToHost, cur()));
+  HIP(hipMemcpyAsync(x.data(), j->d_vec[j->ix], 2 * U * sizeof(double), hipMemcpyDeviceToHost, cur()));
+  HIP(hipMemcpyAsync(&err, j->d_rudy_err, sizeof(unsigned int), hipMemcpyDeviceToHost, cur()));
+  HIP(hipStreamSynchronize(cur()));
   if (err) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
   std::map<uint32_t, double> clustering;
   if (!j->controls.empty()) {
@@ -567,7 +591,7 @@ void Engine::area_round(Job *j) {
     return;
   }
   if (j->area_round >= j->round_limit) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
-  j->apply_charges(ma, fa, stream_);
+  j->apply_charges(ma, fa, cur());
   j->p_valid = false;
   j->has_prev_change = true;
   j->prev_change = total_inc;
@@ -708,14 +732,14 @@ void Engine::advance(Job *j) {
       }
       j->validate_fields(init);
       // DynamicNesterovState::new: previous = coordinates, momentum 1, no L
-      copy_to_previous(j, stream_);
+      copy_to_previous(j, cur());
       j->momentum = 1.0;
       j->has_lipschitz = false;
       j->lipschitz = 0.0;
       j->has_best = false;
       j->current = init;
       j->last_wa = init.wa_value;
-      consider_best(j, stream_);
+      consider_best(j, cur());
       j->has_prev_stationary = false;
       j->p_slot = 0;
       j->p_valid = true;
@@ -778,7 +802,7 @@ void Engine::advance(Job *j) {
         j->y_s = j->origin_s;
         j->y_value = j->origin_value;
       }
-      decide_y(j, stream_);
+      decide_y(j, cur());
       return;
     }
     case Phase::Trial: {
@@ -796,10 +820,10 @@ void Engine::advance(Job *j) {
           j->has_wl = true;
           j->wl = j->L;
           restart_from_origin(j);
-          decide_y(j, stream_);
+          decide_y(j, cur());
           return;
         }
-        stationary(j, true, j->L, stream_);
+        stationary(j, true, j->L, cur());
         return;
       }
       j->count_evaluation();
@@ -832,14 +856,14 @@ void Engine::advance(Job *j) {
                        "coordinate_change=%.9e line_search_trials=%llu gamma=%.9e wa=%.9e\n",
                        (unsigned long long)j->iterations, value, stat, bits_to_double(st.change_bits),
                        (unsigned long long)j->trials, j->gamma, ce.wa_value);
-        iteration_end(j, true, stream_);
+        iteration_end(j, true, cur());
         return;
       }
       if (smooth && !j->restart) {
         j->has_wl = true;
         j->wl = j->L;
         restart_from_origin(j);
-        decide_y(j, stream_);
+        decide_y(j, cur());
         return;
       }
       j->L *= 2.0;
@@ -860,6 +884,7 @@ void Engine::round(const std::vector<Job *> &jobs) {
 bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   RoundCtx &cx = ctx_[c];
   HIP(hipSetDevice(device_));
+  cur_ = c;
   g_timer = &timer;
   std::vector<Job *> &active = cx.active;
   active.clear();
@@ -885,6 +910,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   std::vector<int> l_sort, l_pre, l_cand;
   Plan p_sort_c, p_sort_k, p_pre, p_ext, p_cand;
   std::vector<Task> tasks;
+  int n_density = 0;
   Plan p_dep, p_conv, p_wa, p_gr;
   std::map<std::tuple<int, bool, bool>, DctGroup> groups;
   for (Job *j : active) {
@@ -967,6 +993,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
       const bool density = j->F > 0 && !r.skip;
       // a skipped task reuses the slot's field (same point)
       if (density) {
+        n_density += 1;
         p_dep.add((size_t)j->n_keys, 1);
         p_conv.add((size_t)j->F * j->G, BLOCK);
         dct_group(groups, *j).add((size_t)ti, *j);
@@ -999,8 +1026,8 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   ensure_stage(cx, buf.size());
   ensure_compact(cx, active.size());
   std::memcpy(cx.h_stage, buf.data(), buf.size());
-  await_uploads();
-  HIP(hipMemcpyAsync(cx.d_stage, cx.h_stage, buf.size(), hipMemcpyHostToDevice, stream_));
+  await_uploads(c);
+  HIP(hipMemcpyAsync(cx.d_stage, cx.h_stage, buf.size(), hipMemcpyHostToDevice, cx.stream));
   char *const d_stage_ = cx.d_stage;
   auto dptr = [&](size_t off) { return d_stage_ + off; };
   const Task *d_tasks = reinterpret_cast<const Task *>(dptr(o_tasks));
@@ -1012,7 +1039,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
     om.extrapolate = p_ext.device(d_stage_, o_pext);
     om.candidate = p_cand.device(d_stage_, o_pcand);
     om.n_tasks = (int)opt.size();
-    launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, stream_);
+    launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, cx.stream);
   }
   if (!sorts.empty()) {
     SortMaps sm{};
@@ -1020,7 +1047,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
     sm.keys = p_sort_k.device(d_stage_, o_psk);
     sm.list = reinterpret_cast<const int *>(dptr(o_lsort));
     sm.n_list = (int)l_sort.size();
-    launch_sort(reinterpret_cast<const SortTask *>(dptr(o_sorts)), sm, stream_);
+    launch_sort(reinterpret_cast<const SortTask *>(dptr(o_sorts)), sm, cx.stream);
   }
 
   Maps mp{};
@@ -1029,26 +1056,27 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   mp.wa = p_wa.device(d_stage_, o_pwa);
   mp.gradient = p_gr.device(d_stage_, o_pgr);
   mp.n_tasks = (int)tasks.size();
+  mp.n_density = n_density;
   // WA depends only on coordinates: it runs on the second stream beside the
   // density chain and joins before the combination.
-  HIP(hipEventRecord(ev_fork_, stream_));
-  HIP(hipStreamWaitEvent(wa_stream_, ev_fork_, 0));
-  launch_wa(d_tasks, mp, wa_stream_);
-  HIP(hipEventRecord(ev_join_, wa_stream_));
-  launch_density(d_tasks, mp, stream_);
+  HIP(hipEventRecord(cx.fork, cx.stream));
+  HIP(hipStreamWaitEvent(cx.wa_stream, cx.fork, 0));
+  launch_wa(d_tasks, mp, cx.wa_stream);
+  HIP(hipEventRecord(cx.join, cx.wa_stream));
+  launch_density(d_tasks, mp, cx.stream);
   size_t gi = 0;
   for (auto &kv : groups) {
     const size_t off[4] = {o_groups[gi][0], o_groups[gi][1], o_groups[gi][2], o_groups[gi][3]};
-    launch_poisson(d_tasks, kv.second.device(d_stage_, off), stream_);
+    launch_poisson(d_tasks, kv.second.device(d_stage_, off), cx.stream);
     ++gi;
   }
-  launch_gather(d_tasks, mp, stream_);
-  HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
-  launch_combine(d_tasks, mp, stream_);
+  launch_gather(d_tasks, mp, cx.stream);
+  HIP(hipStreamWaitEvent(cx.stream, cx.join, 0));
+  launch_combine(d_tasks, mp, cx.stream);
   launch_collect_scalars(reinterpret_cast<const int *>(dptr(o_pool)), (int)active.size(), d_pool_, cx.d_compact,
-                         stream_);
+                         cx.stream);
   HIP(hipGetLastError());
-  HIP(hipEventRecord(cx.done, stream_));
+  HIP(hipEventRecord(cx.done, cx.stream));
   cx.pending = true;
   g_timer = nullptr;
   return true;
@@ -1058,12 +1086,15 @@ void Engine::finish(int c) {
   RoundCtx &cx = ctx_[c];
   if (!cx.pending) return;
   HIP(hipSetDevice(device_));
+  cur_ = c;
   // the next round's planning waits on this: poll rather than sleep
+  const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
     const hipError_t q = hipEventQuery(cx.done);
     if (q == hipSuccess) break;
     if (q != hipErrorNotReady) HIP(q);
   }
+  last_wait_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   cx.pending = false;
   g_timer = &timer;
   for (size_t k = 0; k < cx.active.size(); ++k) {
@@ -1081,15 +1112,15 @@ void Engine::finish(int c) {
       j->phase = Phase::Failed;
     }
   }
-  launch_round_targets(cx.active, stream_);
+  launch_round_targets(cx.active, cx.stream);
   g_timer = nullptr;
 }
 
 void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_out) {
   HIP(hipSetDevice(device_));
   if (!j->sorted) sort_now(j);
-  await_uploads();
-  HIP(hipMemcpyAsync(j->d_eval_coords, coords, j->C * sizeof(double), hipMemcpyHostToDevice, stream_));
+  const hipStream_t st = stream();
+  HIP(hipMemcpyAsync(j->d_eval_coords, coords, j->C * sizeof(double), hipMemcpyHostToDevice, st));
   // slot 0 is overwritten: the optimizer's cached field there is stale now
   if (j->p_slot == 0) j->p_valid = false;
   const Phase saved = j->phase;
@@ -1128,10 +1159,10 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
     o_groups.push_back({append(buf, dg.h.start), append(buf, dg.w.start), append(buf, dg.h_wide.start),
                         append(buf, dg.w_wide.start)});
   }
-  HIP(hipStreamSynchronize(stream_));
+  HIP(hipStreamSynchronize(st));
   ensure_stage(aux_, buf.size());
   std::memcpy(aux_.h_stage, buf.data(), buf.size());
-  HIP(hipMemcpyAsync(aux_.d_stage, aux_.h_stage, buf.size(), hipMemcpyHostToDevice, stream_));
+  HIP(hipMemcpyAsync(aux_.d_stage, aux_.h_stage, buf.size(), hipMemcpyHostToDevice, st));
   char *const d_stage_ = aux_.d_stage;
   const Task *d_tasks = reinterpret_cast<const Task *>(d_stage_ + o_tasks);
   Maps mp{};
@@ -1140,22 +1171,23 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   mp.wa = p_wa.device(d_stage_, o_wa);
   mp.gradient = p_gr.device(d_stage_, o_gr);
   mp.n_tasks = 1;
+  mp.n_density = j->F > 0 ? 1 : 0;
   g_timer = &timer;
-  launch_wa(d_tasks, mp, stream_);
-  launch_density(d_tasks, mp, stream_);
+  launch_wa(d_tasks, mp, st);
+  launch_density(d_tasks, mp, st);
   size_t gi = 0;
   for (auto &kv : groups) {
     const size_t off[4] = {o_groups[gi][0], o_groups[gi][1], o_groups[gi][2], o_groups[gi][3]};
-    launch_poisson(d_tasks, kv.second.device(d_stage_, off), stream_);
+    launch_poisson(d_tasks, kv.second.device(d_stage_, off), st);
     ++gi;
   }
-  launch_gather(d_tasks, mp, stream_);
-  launch_combine(d_tasks, mp, stream_);
+  launch_gather(d_tasks, mp, st);
+  launch_combine(d_tasks, mp, st);
   g_timer = nullptr;
   HIP(hipGetLastError());
   HIP(hipMemcpyAsync(h_pool_ + j->pool_index, d_pool_ + j->pool_index, sizeof(JobScalars), hipMemcpyDeviceToHost,
-                     stream_));
-  HIP(hipStreamSynchronize(stream_));
+                     st));
+  HIP(hipStreamSynchronize(st));
   j->h_scalars = h_pool_ + j->pool_index;
   j->phase = saved;
   j->evals.clear();

@@ -39,6 +39,7 @@ struct KernelTimer {
   bool enabled = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   size_t used = 0;
+  double tasks = 0.0;  // evaluations covered by the timed launches
   ~KernelTimer();
 };
 
@@ -143,10 +144,11 @@ void launch_collect_scalars(const int *index, int n, const JobScalars *pool, Job
 
 enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed };
 
-// Below this many charges (members + fillers) in flight a round's kernels do
-// not fill the GPU, and splitting the jobs into two pipelined groups costs
-// more than the host time it hides.
-constexpr size_t PIPELINE_MIN_CHARGES = 4u << 20;
+// From this many charges (members + fillers) in flight the jobs are split
+// into two groups whose rounds overlap on the GPU (each context has its own
+// streams) while the host plans the other group; below it one group's
+// launches are too small to gain from the split.
+constexpr size_t PIPELINE_MIN_CHARGES = 1u << 20;
 
 // Unused device blocks / pinned image blocks an engine keeps for the jobs it
 // admits next.
@@ -306,7 +308,6 @@ struct Job {
   std::vector<EvalReq> evals;
   bool want_precond = false, want_extrapolate = false, want_candidate = false, want_sort = false;
   uint32_t moved = 0;  // most charges any density evaluation of the last round deposited outside their window
-  std::vector<double> targets_out;  // unused placeholder for symmetry
 
   // schedule helpers
   double overflow(const EvalScalars &s, int f) const;
@@ -334,10 +335,12 @@ class Engine {
   ~Engine();
   int device() const { return device_; }
   // Compute stream, ordered after every upload enqueued so far.
-  hipStream_t stream() {
-    if (uploads_pending_ && hipStreamWaitEvent(stream_, ev_upload_, 0) == hipSuccess) uploads_pending_ = false;
-    return stream_;
-  }
+  // The compute stream of context 0, ordered after the uploads and after
+  // all work issued so far on context 1: host reads and set-up work enqueued
+  // here see every job's latest state.
+  hipStream_t stream();
+  // The compute stream of context c (a server reads a group's results there).
+  hipStream_t stream(int c);
   // Stream of uploads and result read-backs: work there waits for no round.
   hipStream_t io_stream() const { return io_stream_; }
   // Pinned host blocks for upload images (usable from any thread).
@@ -350,11 +353,15 @@ class Engine {
   void round(const std::vector<Job *> &jobs);
   // Pipelined rounds: launch() plans one round for `jobs` in context c (0 or
   // 1), issues it and returns (false when no job needs the device); finish(c)
-  // waits for it and advances its jobs.  With two job groups in the two
-  // contexts the host plans one group while the GPU runs the other's round.
+  // waits for it and advances its jobs.  Each context has its own streams, so
+  // with two job groups in the two contexts the host plans one group while
+  // the GPU runs the other's round, and the two groups' rounds overlap on
+  // the GPU.  A job stays in one context while it runs.
   bool launch(int c, const std::vector<Job *> &jobs);
   void finish(int c);
   bool pending(int c) const { return ctx_[c].pending; }
+  // Seconds the last finish() spent waiting for the GPU to end the round.
+  double last_wait_seconds() const { return last_wait_; }
   // Parity evaluation of one job at host coordinates.
   void evaluate_once(Job *job, const double *coords, double gamma, bool density_out);
   KernelTimer timer;
@@ -363,11 +370,13 @@ class Engine {
   void plan(Job *job);
   void advance(Job *job);
   void area_round(Job *job);
-  void await_uploads();  // orders the compute stream after the enqueued uploads
+  void await_uploads(int c);  // orders context c's compute stream after the enqueued uploads
+  hipStream_t cur() const { return ctx_[cur_].stream; }  // stream of the context being planned or advanced
   int device_;
-  hipStream_t stream_ = nullptr, wa_stream_ = nullptr, io_stream_ = nullptr;
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_upload_ = nullptr;
-  bool uploads_pending_ = false;
+  hipStream_t io_stream_ = nullptr;
+  hipEvent_t ev_upload_ = nullptr, ev_ctx1_ = nullptr;
+  int cur_ = 0;
+  double last_wait_ = 0.0;
   std::shared_ptr<BlockPool> blocks_, images_;
   std::map<std::pair<int, int>, std::unique_ptr<Basis>> bases_;  // never erased while the engine lives
   std::mutex bases_mu_;
@@ -376,6 +385,9 @@ class Engine {
   // Staging of one round: the task tables go up in one copy, the active
   // jobs' scalars come back in one copy (gathered into `d_compact`).
   struct RoundCtx {
+    hipStream_t stream = nullptr, wa_stream = nullptr;  // compute stream; WA beside the density chain
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool upload_wait = false;  // uploads enqueued since the stream last waited for them
     char *h_stage = nullptr, *d_stage = nullptr;
     size_t stage_bytes = 0;
     JobScalars *d_compact = nullptr, *h_compact = nullptr;
@@ -384,7 +396,7 @@ class Engine {
     std::vector<Job *> active;
     bool pending = false;
   };
-  RoundCtx ctx_[2], aux_;  // aux_: set-up sorts and parity evaluations
+  RoundCtx ctx_[2], aux_;  // aux_: set-up sorts and parity evaluations (on context 0's stream)
   void ensure_stage(RoundCtx &c, size_t bytes);
   void ensure_compact(RoundCtx &c, size_t n);
   void free_ctx(RoundCtx &c);

@@ -80,10 +80,11 @@ void timer_begin(KernelTimer *t, const char *name, hipStream_t s) {
   (void)hipEventRecord(t->events[t->used].first, s);
 }
 
-void timer_end(KernelTimer *t, const char *name, hipStream_t s) {
+void timer_end(KernelTimer *t, const char *name, hipStream_t s, int tasks) {
   if (!t || !t->enabled || t->name != name || t->used >= t->events.size()) return;
   (void)hipEventRecord(t->events[t->used].second, s);
   t->used += 1;
+  t->tasks += tasks;
 }
 
 BlockPool::~BlockPool() {

@@ -1243,7 +1243,7 @@ void launch_density(const Task *tasks, const Maps &m, hipStream_t s) {
   if (m.deposit.total > 0) {
     timer_begin(g_timer, "deposit", s);
     hipLaunchKernelGGL(k_deposit, dim3(plan_grid(m.deposit)), dim3(BLOCK), 0, s, tasks, m.deposit);
-    timer_end(g_timer, "deposit", s);
+    timer_end(g_timer, "deposit", s, m.n_density);
   }
   if (m.convert.total > 0)
     hipLaunchKernelGGL(k_convert, dim3(plan_grid(m.convert)), dim3(BLOCK), 0, s, tasks, m.convert);
@@ -1265,14 +1265,14 @@ void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
   if (m.wa.total <= 0) return;
   timer_begin(g_timer, "wa", s);
   hipLaunchKernelGGL(k_wa, dim3(plan_grid(m.wa)), dim3(BLOCK), 0, s, tasks, m.wa);
-  timer_end(g_timer, "wa", s);
+  timer_end(g_timer, "wa", s, m.n_tasks);
 }
 
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
   if (m.gradient.total > 0) {
     timer_begin(g_timer, "gather", s);
     hipLaunchKernelGGL(k_gradient, dim3(plan_grid(m.gradient)), dim3(BLOCK), 0, s, tasks, m.gradient);
-    timer_end(g_timer, "gather", s);
+    timer_end(g_timer, "gather", s, m.n_tasks);
   }
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
@@ -1291,7 +1291,7 @@ void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit
   timer_begin(g_timer, "rudy", s);
   hipLaunchKernelGGL(k_rudy, dim3(nb), dim3(BLOCK), 0, s, n_nets, net_start, pin_unit, pin_xy, coords, W, H, fx,
                      fx + G, error);
-  timer_end(g_timer, "rudy", s);
+  timer_end(g_timer, "rudy", s, 1);
   hipLaunchKernelGGL(k_fx_to_double, dim3(std::max(1, blocks_for(2 * G))), dim3(BLOCK), 0, s, 2 * G, fx, out);
 }
 
