@@ -409,9 +409,10 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
 // epilogue stores each wave's tile and writes the per-32x32-tile energy and
 // range partials of the 32-wide kernels.
 constexpr int WIDE_KMAX = 256;
-// Below this many wide tiles (two per CU) a launch leaves CUs idle and the
-// 32x32 split-K kernel is faster.
-constexpr int WIDE_MIN_TILES = 512;
+// Below this many wide tiles (one per CU) a launch leaves CUs idle and the
+// 32x32 split-K kernel is faster; from one per CU the wide kernel wins, also
+// beside the other job group's concurrent launches.
+constexpr int WIDE_MIN_TILES = 256;
 constexpr int WIDE_KP = WIDE_KMAX + 8;  // LDS row stride (halves)
 
 template <int S>
