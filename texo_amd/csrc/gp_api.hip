@@ -419,18 +419,23 @@ struct texo_gp_batch {
     jobs.clear();
   }
   // Rounds until every job is paused, done or failed.  When the jobs are large
-  // enough to fill the GPU in two halves they alternate in two groups, so the
-  // host plans one group while the GPU runs the other's round; each job's
-  // arithmetic is unchanged.
+  // enough they are dealt into ROUND_CONTEXTS groups whose rounds overlap on
+  // the GPU while the host plans the next group's; each job's arithmetic is
+  // unchanged.
   void drive_all() {
     size_t charges = 0;
     for (auto &j : jobs) charges += j->M + j->Fl;
     const bool split = jobs.size() > 1 && charges >= texo::PIPELINE_MIN_CHARGES;
-    std::vector<Job *> group[2];
-    for (size_t k = 0; k < jobs.size(); ++k) group[split ? k % 2 : 0].push_back(jobs[k].get());
-    bool live[2] = {engine->launch(0, group[0]), engine->launch(1, group[1])};
-    while (live[0] || live[1]) {
-      for (int c = 0; c < 2; ++c) {
+    constexpr int NC = texo::ROUND_CONTEXTS;
+    std::vector<Job *> group[NC];
+    for (size_t k = 0; k < jobs.size(); ++k) group[split ? k % NC : 0].push_back(jobs[k].get());
+    bool live[NC];
+    for (int c = 0; c < NC; ++c) live[c] = engine->launch(c, group[c]);
+    for (;;) {
+      bool any = false;
+      for (int c = 0; c < NC; ++c) any = any || live[c];
+      if (!any) break;
+      for (int c = 0; c < NC; ++c) {
         if (!live[c]) continue;
         engine->finish(c);
         live[c] = engine->launch(c, group[c]);
@@ -609,10 +614,11 @@ struct texo_gp_server {
 
 void texo_gp_server::loop() {
   (void)hipSetDevice(engine->device());
-  // Active jobs alternate between two groups whose rounds are pipelined: the
-  // host finishes and re-plans one group while the GPU runs the other's.
-  std::vector<int64_t> group[2];
-  bool live[2] = {false, false};
+  // Active jobs are held in ROUND_CONTEXTS groups whose rounds are pipelined: the
+  // host finishes and re-plans one group while the GPU runs the others'.
+  constexpr int NC = texo::ROUND_CONTEXTS;
+  std::vector<int64_t> group[NC];
+  bool live[NC] = {};
   for (;;) {
     collect(false);
     {
@@ -620,7 +626,7 @@ void texo_gp_server::loop() {
       cv_work.wait(lk, [&] { return stopping || !pending.empty() || !active.empty(); });
       if (stopping) {
         lk.unlock();
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < NC; ++c)
           if (live[c]) {
             try {
               engine->finish(c);
@@ -651,18 +657,21 @@ void texo_gp_server::loop() {
         stats.admit_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         active.push_back(id);
         charges_in_flight += e.job->M + e.job->Fl;
-        // the two groups' rounds overlap on the GPU: once there is enough
-        // work for two, a new job joins the group with fewer charges
-        size_t in_group[2] = {0, 0};
-        for (int c = 0; c < 2; ++c)
+        // the groups' rounds overlap on the GPU: once there is enough work to
+        // split, a new job joins the group with the fewest charges
+        size_t in_group[NC] = {};
+        int least = 0;
+        for (int c = 0; c < NC; ++c) {
           for (int64_t g : group[c]) in_group[c] += entries[g]->job->M + entries[g]->job->Fl;
+          if (in_group[c] < in_group[least]) least = c;
+        }
         const bool split = charges_in_flight >= texo::PIPELINE_MIN_CHARGES;
-        group[split && in_group[1] < in_group[0] ? 1 : 0].push_back(id);
+        group[split ? least : 0].push_back(id);
         hbm_in_use += need;
         stats.admitted += 1;
       }
     }
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < NC; ++c) {
       if (live[c]) {
         const auto t0 = std::chrono::steady_clock::now();
         try {
@@ -709,7 +718,9 @@ void texo_gp_server::loop() {
       if (!live[c]) retire(c, group[c]);
     }
     // nothing in flight to overlap with: wait for the pending results
-    if (!live[0] && !live[1]) collect(true);
+    bool any_live = false;
+    for (int c = 0; c < NC; ++c) any_live = any_live || live[c];
+    if (!any_live) collect(true);
   }
 }
 

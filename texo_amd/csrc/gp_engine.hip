@@ -242,10 +242,10 @@ Engine::Engine(int device, size_t pool_capacity) : device_(device) {
     HIP(hipStreamCreateWithFlags(&c.wa_stream, hipStreamNonBlocking));
     HIP(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
     HIP(hipEventCreateWithFlags(&c.join, hipEventDisableTiming));
+    HIP(hipEventCreateWithFlags(&c.mark, hipEventDisableTiming));
   }
   HIP(hipStreamCreateWithFlags(&io_stream_, hipStreamNonBlocking));
   HIP(hipEventCreateWithFlags(&ev_upload_, hipEventDisableTiming));
-  HIP(hipEventCreateWithFlags(&ev_ctx1_, hipEventDisableTiming));
   blocks_ = std::make_shared<BlockPool>(BLOCK_POOL_KEEP, false);
   images_ = std::make_shared<BlockPool>(IMAGE_POOL_KEEP, true);
   HIP(hipMalloc(&d_pool_, pool_capacity * sizeof(JobScalars)));
@@ -274,19 +274,21 @@ Engine::~Engine() {
   for (RoundCtx &c : ctx_) {
     (void)hipEventDestroy(c.fork);
     (void)hipEventDestroy(c.join);
+    (void)hipEventDestroy(c.mark);
     (void)hipStreamDestroy(c.wa_stream);
     (void)hipStreamDestroy(c.stream);
   }
   (void)hipEventDestroy(ev_upload_);
-  (void)hipEventDestroy(ev_ctx1_);
   (void)hipStreamDestroy(io_stream_);
 }
 
 hipStream_t Engine::stream() {
   HIP(hipSetDevice(device_));
   await_uploads(0);
-  HIP(hipEventRecord(ev_ctx1_, ctx_[1].stream));
-  HIP(hipStreamWaitEvent(ctx_[0].stream, ev_ctx1_, 0));
+  for (int c = 1; c < ROUND_CONTEXTS; ++c) {
+    HIP(hipEventRecord(ctx_[c].mark, ctx_[c].stream));
+    HIP(hipStreamWaitEvent(ctx_[0].stream, ctx_[c].mark, 0));
+  }
   return ctx_[0].stream;
 }
 

@@ -149,6 +149,10 @@ enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed
 // streams) while the host plans the other group; below it one group's
 // launches are too small to gain from the split.
 constexpr size_t PIPELINE_MIN_CHARGES = 1u << 20;
+// Round contexts: job groups whose rounds overlap.  Two are fastest on the
+// 16-job lockstep batch and the serving stream (three and four groups make
+// each launch smaller than the overlap gains).
+constexpr int ROUND_CONTEXTS = 2;
 
 // Unused device blocks / pinned image blocks an engine keeps for the jobs it
 // admits next.
@@ -336,8 +340,8 @@ class Engine {
   int device() const { return device_; }
   // Compute stream, ordered after every upload enqueued so far.
   // The compute stream of context 0, ordered after the uploads and after
-  // all work issued so far on context 1: host reads and set-up work enqueued
-  // here see every job's latest state.
+  // all work issued so far on the other contexts: host reads and set-up work
+  // enqueued here see every job's latest state.
   hipStream_t stream();
   // The compute stream of context c (a server reads a group's results there).
   hipStream_t stream(int c);
@@ -351,12 +355,12 @@ class Engine {
   void sort_now(Job *job);  // sorts the job's charges by tile at its current coordinates
   // Runs one round for every job in `jobs` whose phase needs the device.
   void round(const std::vector<Job *> &jobs);
-  // Pipelined rounds: launch() plans one round for `jobs` in context c (0 or
-  // 1), issues it and returns (false when no job needs the device); finish(c)
-  // waits for it and advances its jobs.  Each context has its own streams, so
-  // with two job groups in the two contexts the host plans one group while
-  // the GPU runs the other's round, and the two groups' rounds overlap on
-  // the GPU.  A job stays in one context while it runs.
+  // Pipelined rounds: launch() plans one round for `jobs` in context c
+  // (< ROUND_CONTEXTS), issues it and returns (false when no job needs the
+  // device); finish(c) waits for it and advances its jobs.  Each context has
+  // its own streams, so with one job group per context the host plans one
+  // group while the GPU runs the others' rounds, and the groups' rounds
+  // overlap on the GPU.  A job stays in one context while it runs.
   bool launch(int c, const std::vector<Job *> &jobs);
   void finish(int c);
   bool pending(int c) const { return ctx_[c].pending; }
@@ -374,7 +378,7 @@ class Engine {
   hipStream_t cur() const { return ctx_[cur_].stream; }  // stream of the context being planned or advanced
   int device_;
   hipStream_t io_stream_ = nullptr;
-  hipEvent_t ev_upload_ = nullptr, ev_ctx1_ = nullptr;
+  hipEvent_t ev_upload_ = nullptr;
   int cur_ = 0;
   double last_wait_ = 0.0;
   std::shared_ptr<BlockPool> blocks_, images_;
@@ -386,7 +390,7 @@ class Engine {
   // jobs' scalars come back in one copy (gathered into `d_compact`).
   struct RoundCtx {
     hipStream_t stream = nullptr, wa_stream = nullptr;  // compute stream; WA beside the density chain
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, mark = nullptr;  // mark: joins this stream into context 0's
     bool upload_wait = false;  // uploads enqueued since the stream last waited for them
     char *h_stage = nullptr, *d_stage = nullptr;
     size_t stage_bytes = 0;
@@ -396,7 +400,7 @@ class Engine {
     std::vector<Job *> active;
     bool pending = false;
   };
-  RoundCtx ctx_[2], aux_;  // aux_: set-up sorts and parity evaluations (on context 0's stream)
+  RoundCtx ctx_[ROUND_CONTEXTS], aux_;  // aux_: set-up sorts and parity evaluations (on context 0's stream)
   void ensure_stage(RoundCtx &c, size_t bytes);
   void ensure_compact(RoundCtx &c, size_t n);
   void free_ctx(RoundCtx &c);
