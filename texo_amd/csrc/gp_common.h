@@ -299,6 +299,8 @@ struct OptMaps {
   int n_tasks;  // one finalize block per task
 };
 void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s);
+// The optimizer's scalars (read by the host only): may run beside the density chain.
+void launch_optimizer_finalize(const OptTask *tasks, const OptMaps &m, hipStream_t s);
 
 // RUDY (eplace.rs:856-937): `fx` is [2G] scratch, `out` receives
 // horizontal[G] then vertical[G].

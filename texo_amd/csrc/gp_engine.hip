@@ -1035,14 +1035,29 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   const Task *d_tasks = reinterpret_cast<const Task *>(dptr(o_tasks));
 
   // ---- launches ----
-  if (!opt.empty()) {
-    OptMaps om{};
-    om.precond = p_pre.device(d_stage_, o_ppre);
-    om.extrapolate = p_ext.device(d_stage_, o_pext);
-    om.candidate = p_cand.device(d_stage_, o_pcand);
-    om.n_tasks = (int)opt.size();
-    launch_optimizer(reinterpret_cast<const OptTask *>(dptr(o_opt)), om, cx.stream);
-  }
+  OptMaps om{};
+  om.precond = p_pre.device(d_stage_, o_ppre);
+  om.extrapolate = p_ext.device(d_stage_, o_pext);
+  om.candidate = p_cand.device(d_stage_, o_pcand);
+  om.n_tasks = (int)opt.size();
+  const OptTask *d_opt = reinterpret_cast<const OptTask *>(dptr(o_opt));
+  if (!opt.empty()) launch_optimizer(d_opt, om, cx.stream);
+  // The second stream forks after the optimizer's vectors: it reduces the
+  // optimizer's scalars (only the host reads them) and runs WA (it depends
+  // only on coordinates) beside the sort and the density chain, and joins
+  // before the combination.
+  Maps mp{};
+  mp.deposit = p_dep.device(d_stage_, o_pdep);
+  mp.convert = p_conv.device(d_stage_, o_pconv);
+  mp.wa = p_wa.device(d_stage_, o_pwa);
+  mp.gradient = p_gr.device(d_stage_, o_pgr);
+  mp.n_tasks = (int)tasks.size();
+  mp.n_density = n_density;
+  HIP(hipEventRecord(cx.fork, cx.stream));
+  HIP(hipStreamWaitEvent(cx.wa_stream, cx.fork, 0));
+  if (!opt.empty()) launch_optimizer_finalize(d_opt, om, cx.wa_stream);
+  launch_wa(d_tasks, mp, cx.wa_stream);
+  HIP(hipEventRecord(cx.join, cx.wa_stream));
   if (!sorts.empty()) {
     SortMaps sm{};
     sm.charges = p_sort_c.device(d_stage_, o_psc);
@@ -1051,20 +1066,6 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
     sm.n_list = (int)l_sort.size();
     launch_sort(reinterpret_cast<const SortTask *>(dptr(o_sorts)), sm, cx.stream);
   }
-
-  Maps mp{};
-  mp.deposit = p_dep.device(d_stage_, o_pdep);
-  mp.convert = p_conv.device(d_stage_, o_pconv);
-  mp.wa = p_wa.device(d_stage_, o_pwa);
-  mp.gradient = p_gr.device(d_stage_, o_pgr);
-  mp.n_tasks = (int)tasks.size();
-  mp.n_density = n_density;
-  // WA depends only on coordinates: it runs on the second stream beside the
-  // density chain and joins before the combination.
-  HIP(hipEventRecord(cx.fork, cx.stream));
-  HIP(hipStreamWaitEvent(cx.wa_stream, cx.fork, 0));
-  launch_wa(d_tasks, mp, cx.wa_stream);
-  HIP(hipEventRecord(cx.join, cx.wa_stream));
   launch_density(d_tasks, mp, cx.stream);
   size_t gi = 0;
   for (auto &kv : groups) {

@@ -888,8 +888,19 @@ __device__ void finalize_task(const Task &t) {
   double acc[NS + NM];
 #pragma unroll
   for (int k = 0; k < NS + NM; ++k) acc[k] = 0.0;
-  const int n = max(t.nb_wa, max(t.nb_ug, t.nb_gf));
+  // fixed-point density sums of the k_convert blocks (exact in any order)
+  long long cv[2 * MAXF];
+#pragma unroll
+  for (int k = 0; k < 2 * MAXF; ++k) cv[k] = 0;
+  // one pass over the four partial arrays, so that their loads are in
+  // flight together
+  const int n = max(max(t.nb_wa, t.nb_conv), max(t.nb_ug, t.nb_gf));
   for (int b = threadIdx.x; b < n; b += BLOCK) {
+    if (b < t.nb_conv) {
+      const long long *pc = t.part_conv + (size_t)b * 2 * MAXF;
+#pragma unroll
+      for (int k = 0; k < 2 * MAXF; ++k) cv[k] += pc[k];
+    }
     if (b < t.nb_wa) acc[0] += t.part_wa[b];
     if (b < t.nb_ug) {
       const double *p = t.part_ug + (size_t)b * UG_STRIDE;
@@ -918,15 +929,6 @@ __device__ void finalize_task(const Task &t) {
   for (int k = 0; k < NS + NM; ++k)
     r[k] = k < NS ? ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3]
                   : fmax(fmax(sh[k][0], sh[k][1]), fmax(sh[k][2], sh[k][3]));
-  // fixed-point density sums of the k_convert blocks (exact in any order)
-  long long cv[2 * MAXF];
-#pragma unroll
-  for (int k = 0; k < 2 * MAXF; ++k) cv[k] = 0;
-  for (int b = threadIdx.x; b < t.nb_conv; b += BLOCK) {
-    const long long *pc = t.part_conv + (size_t)b * 2 * MAXF;
-#pragma unroll
-    for (int k = 0; k < 2 * MAXF; ++k) cv[k] += pc[k];
-  }
   __shared__ long long scv[2 * MAXF][BLOCK / 64];
 #pragma unroll
   for (int k = 0; k < 2 * MAXF; ++k) {
@@ -1280,6 +1282,9 @@ void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
 void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
   const int grid = plan_grid(m.precond) + plan_grid(m.extrapolate) + plan_grid(m.candidate);
   if (grid > 0) hipLaunchKernelGGL(k_optimizer, dim3(grid), dim3(BLOCK), 0, s, tasks, m);
+}
+
+void launch_optimizer_finalize(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_optimizer_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
