@@ -218,6 +218,58 @@ def run_serving(stream, precision, max_batch, at_rate: bool, clients: int, warm:
             "launch_s": d["launch_seconds"]}
 
 
+def run_eco(n_requests: int, n_full: int, clients: int):
+    """The incremental stream on its resident design (250k cells, 512x512
+    bins, 300k nets): `n_requests` requests, each re-placing a connected 1-5%
+    of the units for 100 iterations with the rest fixed, interleaved with
+    `n_full` full placements of the design from multi-start coordinates
+    (reference stop rules), all submitted at once from `clients` threads;
+    BF16x3 batched against one at a time with FP32.  Latency = submission to
+    completion; requests are generated before the clock starts."""
+    import threading
+    import texo_amd
+    from texo_amd import abi, synth
+    design = synth.eco_design()
+    eco = synth.EcoStream(design)
+    jobs = [(eco.request(1000 + k), 100) for k in range(n_requests)]
+    step = max(1, n_requests // max(1, n_full))
+    for k in range(n_full):
+        jobs.insert(min(len(jobs), (k + 1) * step + k), (synth.multistart(design, k), 0))
+    out = {"design": {k: design.meta[k] for k in ("cells", "nets", "pins")} | {"bins": "512x512"},
+           "requests": n_requests, "full_jobs": n_full, "clients": clients,
+           "mean_movable_units": float(np.mean([p.meta.get("movable_units", 0) for p, it in jobs if it]))}
+    for name, precision, max_batch in (("batched_bf16x3", abi.DCT_BF16X3, 256), ("one_at_a_time_fp32", abi.DCT_FP32, 1)):
+        with texo_amd.Server(max_batch=max_batch) as srv:
+            ids = [None] * len(jobs)
+            t0 = time.perf_counter()
+
+            def client(k):
+                for i in range(k, len(jobs), clients):
+                    p, it = jobs[i]
+                    ids[i] = srv.submit(p, abi.options(dct_precision=precision, fixed_iterations=it))
+
+            threads = [threading.Thread(target=client, args=(k,)) for k in range(clients)]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+            lat = {100: [], 0: []}
+            failed = 0
+            for i, (p, it) in zip(ids, jobs):
+                _, rep = srv.wait(i, raise_on_error=False)
+                lat[it].append(rep["elapsed_seconds"])
+                failed += rep["status"] != "OK"
+            dt = time.perf_counter() - t0
+        r = {"jobs_per_s": len(jobs) / dt, "makespan_s": dt, "failed": failed,
+             "request_p50_s": float(np.percentile(lat[100], 50)), "request_p99_s": float(np.percentile(lat[100], 99))}
+        if lat[0]:
+            r["full_p50_s"] = float(np.percentile(lat[0], 50))
+        out[name] = r
+        progress(f"eco {name}: {r}")
+    out["jobs_per_s_ratio"] = out["batched_bf16x3"]["jobs_per_s"] / out["one_at_a_time_fp32"]["jobs_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +282,9 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--serving-clients", type=int, default=8, help="threads submitting the serving stream")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--eco-requests", type=int, default=0,
+                    help="also run the incremental stream with this many requests (0: skip)")
+    ap.add_argument("--eco-full", type=int, default=2, help="full multi-start jobs interleaved with the requests")
     ap.add_argument("--cpu-iterations", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="time the batched form only (for profiling)")
@@ -304,6 +359,10 @@ def main():
         serving["saturated_jobs_per_s_ratio"] = (serving["batched_mfma_saturated"]["jobs_per_s"]
                                                  / serving["one_at_a_time_fp32_saturated"]["jobs_per_s"])
 
+    eco = None
+    if args.eco_requests > 0:
+        eco = run_eco(args.eco_requests, args.eco_full, args.serving_clients)
+
     cpu = None
     if not args.no_cpu:
         progress("cpu baseline")
@@ -341,6 +400,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "eco": eco,
     }
     print(json.dumps(line), flush=True)
 

@@ -59,7 +59,7 @@ def test_evaluate_matches_oracle(name, precision, field_tol):
     o.close()
 
 
-@pytest.mark.parametrize("W,H", [(7, 5), (11, 1), (127, 96), (64, 64), (256, 256)])
+@pytest.mark.parametrize("W,H", [(7, 5), (11, 1), (127, 96), (64, 64), (256, 256), (512, 512)])
 @pytest.mark.parametrize("precision,tol", [(abi.DCT_FP32, 2e-6), (abi.DCT_BF16X3, 2e-4),
                                            (abi.DCT_FP16_SCALED, 4e-6)])
 def test_poisson_matches_oracle(W, H, precision, tol):

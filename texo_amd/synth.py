@@ -385,3 +385,179 @@ def serving_stream(n_jobs: int, seed: int = 2026, rate: float = 20.0, min_cells:
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
     arrivals -= arrivals[0]
     return [(float(arrivals[i]), serving_job(seed * 1000 + i, int(cells[i]), int(exps[i]))) for i in range(n_jobs)]
+
+
+# ---------------------------------------------------------------------------
+# Incremental (ECO) re-placement on a resident design
+# ---------------------------------------------------------------------------
+def _assign_sites(unit_xy: np.ndarray, site_bins: np.ndarray, W: int) -> np.ndarray:
+    """Legal sites for units: the units are cut into row bands by y (band k
+    holds as many units as row band k of the sites has sites) and matched to
+    the sites by x inside each band, so the placement keeps the units'
+    coarse 2-D order.  `site_bins` are row-major bin indices (one per site,
+    repeated by capacity), at least as many as units."""
+    n = unit_xy.shape[0]
+    out = np.zeros((n, 2))
+    if n == 0:
+        return out
+    pick = site_bins[np.linspace(0, site_bins.size - 1, n).round().astype(np.int64)]
+    sx, sy = pick % W, pick // W
+    order_y = np.argsort(unit_xy[:, 1], kind="stable")
+    rows, counts = np.unique(sy, return_counts=True)
+    start = 0
+    for row, cnt in zip(rows, counts):
+        units = order_y[start:start + cnt]
+        units = units[np.argsort(unit_xy[units, 0], kind="stable")]
+        xs = np.sort(sx[sy == row])
+        out[units, 0] = xs
+        out[units, 1] = row
+        start += cnt
+    return out
+
+
+def eco_design(seed: int = 5, cells: int = 250_000, side: int = 512, nets: int = 300_000,
+               high_fanout: int = 16) -> abi.Problem:
+    """The resident design of the incremental stream: the ECP5 cell mix
+    (70% LUT4, 28% FF, 2% of cells in two-LUT carry units) with fixed DP16KD
+    blocks, nets of degree 2 + Geometric(0.4) plus `high_fanout` nets of
+    256-512 pins, on side x side bins, with a legal placement as its
+    coordinates: every movable unit's members sit on sites of their kind, no
+    bin over its capacity (a carry unit takes a LUT site at (x, y) and one at
+    (x, y + 1))."""
+    n_carry_units = int(cells * 0.01)
+    n_ff = int(cells * 0.28)
+    n_lut = cells - n_ff - 2 * n_carry_units
+    n_memory = int(round(208 * cells / 84_000))
+    p = make_problem(seed=seed, side=side, n_lut=n_lut, n_ff=n_ff, n_carry_units=n_carry_units, n_memory=n_memory,
+                     n_io=0, n_nets=nets, degree="geometric", degree_p=0.4, degree_cap=64, high_fanout=high_fanout,
+                     name="eco_design")
+    W, H = p.width, p.height
+    G = W * H
+    kinds = list(p.capacity_kind)
+    lut_cap = p.capacity[kinds.index(abi.KIND_LUT(4)) * G:(kinds.index(abi.KIND_LUT(4)) + 1) * G].astype(np.int64)
+    ff_cap = p.capacity[kinds.index(abi.KIND_REGISTER) * G:(kinds.index(abi.KIND_REGISTER) + 1) * G].astype(np.int64)
+    init = p.unit_initial_xy.reshape(-1, 2).copy()
+    sites = init.copy()
+    carry = np.arange(n_lut + n_ff, n_lut + n_ff + n_carry_units)
+    # carry units on vertically adjacent LUT sites, pairs starting on even rows
+    b = np.arange(G - W)
+    pairs = b[(lut_cap[:G - W] > 0) & (lut_cap[W:] > 0) & ((b // W) % 2 == 0)]
+    if pairs.size < n_carry_units:
+        raise ValueError("not enough vertically adjacent LUT sites for the carry units")
+    sites[carry] = _assign_sites(init[carry], pairs, W)
+    used = (sites[carry, 1] * W + sites[carry, 0]).astype(np.int64)
+    np.subtract.at(lut_cap, used, 1)
+    np.subtract.at(lut_cap, used + W, 1)
+    lut_sites = np.repeat(np.arange(G), lut_cap)
+    ff_sites = np.repeat(np.arange(G), ff_cap)
+    lut_units, ff_units = np.arange(n_lut), np.arange(n_lut, n_lut + n_ff)
+    sites[lut_units] = _assign_sites(init[lut_units], lut_sites, W)
+    sites[ff_units] = _assign_sites(init[ff_units], ff_sites, W)
+    import dataclasses
+    meta = dict(p.meta, name="eco_design", resident=True)
+    return dataclasses.replace(p, unit_initial_xy=sites.reshape(-1), meta=meta)
+
+
+class EcoStream:
+    """Incremental requests on a resident design: each request re-places a
+    connected subset of the movable units (grown breadth-first over the nets
+    from a random unit) with every other unit fixed at its resident site, as
+    texo-pnr does for a unit with a single placement choice: its cells become
+    fixed coordinates and its members fixed occupancy that the density model
+    subtracts from the capacity (eplace.rs:263-271)."""
+
+    def __init__(self, design: abi.Problem):
+        self.design = design
+        p = design
+        U = p.unit_fixed.size
+        self.movable = np.flatnonzero(p.unit_fixed == 0)
+        deg = np.diff(p.net_pin_start.astype(np.int64))
+        net_of_pin = np.repeat(np.arange(deg.size), deg)
+        pin_unit = p.cell_unit[p.pin_cell].astype(np.int64)
+        # unit -> nets and net -> units (CSR)
+        order = np.argsort(pin_unit, kind="stable")
+        self.unit_start = np.zeros(U + 1, np.int64)
+        np.add.at(self.unit_start, pin_unit + 1, 1)
+        self.unit_start = np.cumsum(self.unit_start)
+        self.unit_nets = net_of_pin[order]
+        self.net_start = p.net_pin_start.astype(np.int64)
+        self.net_units = pin_unit
+
+    def subset(self, rng, size: int) -> np.ndarray:
+        """A connected set of `size` movable units, breadth-first from a
+        random movable unit (each level is complete before the next starts,
+        so any prefix of the order is connected); a start whose component
+        is smaller than `size` (a unit without nets) is drawn again."""
+        fixed = self.design.unit_fixed != 0
+        best = np.zeros(0, np.int64)
+        for _ in range(64):
+            seen = np.zeros(fixed.size, bool)
+            start = int(rng.choice(self.movable))
+            order = [np.array([start])]
+            seen[start] = True
+            frontier = order[0]
+            total = 1
+            while total < size and frontier.size:
+                nets = [self.unit_nets[self.unit_start[u]:self.unit_start[u + 1]] for u in frontier]
+                nets = np.unique(np.concatenate(nets))
+                if nets.size == 0:
+                    break
+                nxt = np.unique(np.concatenate([self.net_units[self.net_start[n]:self.net_start[n + 1]] for n in nets]))
+                nxt = nxt[~seen[nxt] & ~fixed[nxt]]
+                seen[nxt] = True
+                order.append(nxt)
+                total += nxt.size
+                frontier = nxt
+            found = np.concatenate(order)[:size]
+            if found.size >= size:
+                return found
+            if found.size > best.size:
+                best = found
+        return best
+
+    def request(self, seed: int, fraction: Optional[float] = None) -> abi.Problem:
+        """One incremental request: a connected subset of 1-5% of the movable
+        units (or `fraction`) is ripped up and re-placed: it starts clumped
+        around its resident centroid (a Gaussian of sigma = side/12 for the
+        side of a square holding the subset at density 0.9, clipped to each
+        unit's bounds, as the initial coordinates of a full placement are);
+        every other unit is fixed at its resident site."""
+        import dataclasses
+        rng = np.random.default_rng(seed)
+        p = self.design
+        frac = float(rng.uniform(0.01, 0.05)) if fraction is None else fraction
+        keep = self.subset(rng, max(1, int(round(frac * self.movable.size))))
+        U = p.unit_fixed.size
+        fix = np.zeros(U, bool)
+        fix[self.movable] = True
+        fix[keep] = False
+        xy = p.unit_initial_xy.reshape(-1, 2)
+        init = xy.copy()
+        sigma = math.sqrt(keep.size / 0.9) / 12.0
+        b = p.unit_bounds.reshape(-1, 4)
+        init[keep] = np.clip(xy[keep].mean(axis=0) + rng.normal(0.0, sigma, size=(keep.size, 2)), b[keep, :2],
+                             b[keep, 2:])
+        unit_fixed = p.unit_fixed.copy()
+        unit_fixed[fix] = 1
+        bounds = p.unit_bounds.reshape(-1, 4).copy()
+        bounds[fix, :2] = xy[fix]
+        bounds[fix, 2:] = xy[fix]
+        cell_fixed = p.cell_fixed.copy()
+        cell_xy = p.cell_fixed_xy.reshape(-1, 2).copy()
+        cells = np.flatnonzero(fix[p.cell_unit])
+        cell_fixed[cells] = 1
+        cell_xy[cells] = xy[p.cell_unit[cells]] + p.cell_offset_xy.reshape(-1, 2)[cells]
+        # members of the newly fixed units occupy their sites
+        ms = p.unit_member_start.astype(np.int64)
+        counts = np.diff(ms)
+        owner_all = np.repeat(np.arange(U), counts)
+        members = np.flatnonzero(fix[owner_all])
+        owner = owner_all[members]
+        occ_xy = (xy[owner] + p.member_offset_xy.reshape(-1, 2)[members]).astype(np.uint32)
+        occ_kind = np.concatenate([p.fixed_occupancy_kind, p.member_kind[members]])
+        occ_xy = np.concatenate([p.fixed_occupancy_xy.reshape(-1), occ_xy.reshape(-1)])
+        meta = dict(p.meta, name=f"eco{seed}", eco_seed=seed, movable_units=int(keep.size))
+        return dataclasses.replace(p, unit_fixed=unit_fixed, unit_initial_xy=init.reshape(-1),
+                                   unit_bounds=bounds.reshape(-1), cell_fixed=cell_fixed,
+                                   cell_fixed_xy=cell_xy.reshape(-1), fixed_occupancy_kind=occ_kind,
+                                   fixed_occupancy_xy=occ_xy, meta=meta)
