@@ -266,13 +266,15 @@ int texo_gp_session_poisson(texo_gp_session *s, const double *density, double *p
     }
     HIP(hipMemcpyAsync(J.slot[0].amax_part, amax.data(), amax.size() * 4, hipMemcpyHostToDevice, st));
     HIP(hipStreamSynchronize(st));
-    texo::Task t = J.task(0, J.d_eval_coords, J.d_grad[0], nullptr, 1.0, false, &J.d_scalars->eval[2]);
+    // transform only: no WA runs
+    texo::Task t = J.task(0, J.d_eval_coords, J.d_grad[0], nullptr, 1.0, false, &J.d_scalars->eval[2], false);
     // one task: its transform tiles as four one-entry block plans
     const int F = J.F, W = (int)J.W, H = (int)J.H;
     texo::DctLaunch d{};
     d.precision = (int)J.opt.precision;
     d.fast = texo::poisson_fast_shape(W, H);
     d.wide_ok = d.precision == TEXO_GP_DCT_FP16_SCALED && d.fast && texo::poisson_wide_shape(W, H);
+    d.n_tasks = 1;
     const int tiles[4] = {((F * H + 31) / 32) * ((W + 31) / 32), ((F * W + 31) / 32) * ((H + 31) / 32),
                           d.wide_ok ? (F * H / 32) * (W / 128) : 0, d.wide_ok ? (F * W / 32) * (H / 128) : 0};
     const int starts[8] = {0, tiles[0], 0, tiles[1], 0, tiles[2], 0, tiles[3]};

@@ -54,7 +54,7 @@ struct BlockPlan {
 // nets of more than WA_LOOP_PINS pins get a wavefront that loops.
 constexpr int WA_PINS_PER_LANE = 4;
 constexpr int WA_LOOP_PINS = 64 * WA_PINS_PER_LANE;
-constexpr int WA_MAX_SEGMENTS = 16;
+constexpr int WA_MAX_SEGMENTS = 32;
 __host__ __device__ inline int wa_lanes(uint32_t pins) {
   if (pins > (uint32_t)WA_LOOP_PINS) return 0;
   int g = 1;
@@ -223,6 +223,12 @@ struct Task {
   double2 *dpin;               // [movable pins] weighted WA derivative, grouped by unit
   double *part_wa, *part_energy, *part_gf, *part_ug;
   int nb_wa, nb_energy, nb_gf, nb_ug, nb_conv, nb_tile;
+  // WA blocks [0, nb_wa_live) cover the nets with a movable pin; the blocks
+  // after them the nets of fixed pins.  fixed_wa_mode: 0 no such nets, 1 this
+  // task evaluates them and stores their value in *fixed_wa, 2 it adds the
+  // stored value (same gamma)
+  int nb_wa_live, fixed_wa_mode;
+  double *fixed_wa;
   // request
   const double *coords;
   double *grad;

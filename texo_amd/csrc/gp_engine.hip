@@ -982,9 +982,19 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
         p_cand.none();
       }
     }
+    // the nets of fixed pins run when their value at this round's gamma is
+    // not held yet (every evaluation of a job in a round has one gamma)
+    bool one_gamma = true;
+    for (const Job::EvalReq &r : j->evals) one_gamma = one_gamma && r.gamma == j->evals.front().gamma;
+    const bool fixed_nets = !j->evals.empty() && (!one_gamma || !j->fixed_wa_valid ||
+                                                  j->evals.front().gamma != j->fixed_wa_gamma);
+    if (fixed_nets) {
+      j->fixed_wa_valid = one_gamma;
+      j->fixed_wa_gamma = j->evals.front().gamma;
+    }
     for (const Job::EvalReq &r : j->evals) {
       Task t = j->task(r.slot, r.coords, r.grad, r.metric ? j->d_metric : nullptr, r.gamma, r.skip,
-                       &j->d_scalars->eval[r.scalar_slot]);
+                       &j->d_scalars->eval[r.scalar_slot], fixed_nets);
       if (r.parity) {
         t.raw_units = j->d_raw_units;
         t.raw_fillers = j->d_raw_fillers;
@@ -1003,7 +1013,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
         p_dep.none();
         p_conv.none();
       }
-      p_wa.add((size_t)j->nb_wa, 1);
+      p_wa.add((size_t)t.nb_wa, 1);
       p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
     }
   }
@@ -1134,7 +1144,10 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   // run exactly the planned request: mark the phase so plan() leaves it alone
   j->phase = Phase::Paused;
   std::vector<Task> tasks;
-  Task t = j->task(0, j->d_eval_coords, j->d_grad[0], nullptr, g, false, &j->d_scalars->eval[2]);
+  const bool fixed_nets = !j->fixed_wa_valid || g != j->fixed_wa_gamma;
+  j->fixed_wa_valid = true;
+  j->fixed_wa_gamma = g;
+  Task t = j->task(0, j->d_eval_coords, j->d_grad[0], nullptr, g, false, &j->d_scalars->eval[2], fixed_nets);
   t.raw_units = j->d_raw_units;
   t.raw_fillers = j->d_raw_fillers;
   t.wa_grad_out = j->d_wa_grad;
@@ -1150,7 +1163,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
     p_dep.none();
     p_conv.none();
   }
-  p_wa.add((size_t)j->nb_wa, 1);
+  p_wa.add((size_t)t.nb_wa, 1);
   p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
   std::vector<char> buf;
   const size_t o_tasks = append(buf, tasks), o_dep = append(buf, p_dep.start), o_conv = append(buf, p_conv.start),

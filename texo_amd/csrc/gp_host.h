@@ -185,6 +185,12 @@ struct Job {
   std::vector<uint32_t> net_start, pin_unit, u_pstart, u_pins, pin_slot, u_nbase;
   std::vector<int4> wa_seg;  // WA segments (JobStatic::wa_seg)
   int wa_waves = 0;          // wavefronts of one WA evaluation
+  int wa_waves_live = 0;     // ... of its nets with a movable pin (a multiple of BLOCK / 64)
+  size_t n_live_nets = 0;
+  // WA value of the nets of fixed pins: held on the device for one gamma
+  double *d_fixed_wa = nullptr;
+  bool fixed_wa_valid = false;
+  double fixed_wa_gamma = 0.0;
   std::vector<double> pin_xy, net_weight;
   std::vector<double> h_du_fcharge;  // charge per field and density unit
   // Upload image: the bytes of the arena's leading block of arrays that
@@ -243,7 +249,7 @@ struct Job {
   double *d_s_q = nullptr;
   uint8_t *d_s_layer = nullptr;
   int tiles_x = 0, n_keys = 0;
-  int nb_wa = 0, nb_energy = 0, nb_gf = 0, nb_ug = 0, nb_conv = 0, nb_tile = 0;
+  int nb_wa = 0, nb_wa_live = 0, nb_energy = 0, nb_gf = 0, nb_ug = 0, nb_conv = 0, nb_tile = 0;
 
   // Allocates the arena (from `pool` when given) and enqueues the image copy
   // and the accumulator clear on `s`; the image stays alive until
@@ -253,8 +259,10 @@ struct Job {
   // (waited for before returning).
   void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas, hipStream_t s);
   bool sorted = false;  // charges ordered by tile at least once
+  // fixed_nets: the task evaluates the nets of fixed pins too (their value
+  // at `gamma` is then held for later tasks at the same gamma)
   Task task(int slot, const double *coords, double *grad, const double *metric, double gamma, bool skip,
-            EvalScalars *scalars) const;
+            EvalScalars *scalars, bool fixed_nets) const;
   // next sort, at `coords` (null: the current iterate); the new order
   // becomes the job's order
   SortTask sort_task(const double *coords = nullptr);

@@ -409,20 +409,31 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
       wl_diag[su] += net_weight[n_nets + a];
     }
   }
-  // Device layout of the nets: small nets (at most LARGE_NET_PINS pins)
-  // grouped by degree, then large nets longest first, then the timed arcs.
-  // Lanes of a wavefront take consecutive small nets, so a wave's pin loops
-  // have one length and its pins are contiguous.  Pins are renumbered with
-  // their nets; each unit still lists its pins in the input net order, which
-  // fixes the summation order of its gradient.
+  // Device layout of the nets: the nets with a movable pin ("live"), then
+  // those whose pins are all fixed; in each class small nets (at most
+  // LARGE_NET_PINS pins) grouped by degree, then large nets longest first;
+  // then the timed arcs, live ones first.  Lanes of a wavefront take
+  // consecutive small nets, so a wave's pin loops have one length and its
+  // pins are contiguous.  Pins are renumbered with their nets; each unit
+  // still lists its pins in the input net order, which fixes the summation
+  // order of its gradient.  A net of fixed pins has no gradient and a WA
+  // value that depends on gamma only: its wavefronts form a separate range
+  // that runs once per gamma (an incremental request fixes most units).
   {
     auto degree = [&](uint32_t n) { return net_start[n + 1] - net_start[n]; };
-    // stable counting sort of the small nets by degree, then the large nets
-    // stably by descending degree
-    std::vector<uint32_t> order(n_nets), large;
-    {
-      std::vector<uint32_t> at(LARGE_NET_PINS + 2, 0);
+    std::vector<uint8_t> live(n_net_total, 0);
+    for (size_t n = 0; n < n_net_total; ++n)
+      for (uint32_t i = net_start[n]; i < net_start[n + 1] && !live[n]; ++i) live[n] = pin_unit[i] != NO_UNIT;
+    // per class: stable counting sort of the small nets by degree, then the
+    // large nets stably by descending degree
+    std::vector<uint32_t> order;
+    order.reserve(n_net_total);
+    n_live_nets = 0;
+    for (int cls = 1; cls >= 0; --cls) {
+      std::vector<uint32_t> at(LARGE_NET_PINS + 2, 0), large;
+      const size_t first = order.size();
       for (size_t n = 0; n < n_nets; ++n) {
+        if (live[n] != cls) continue;
         const uint32_t d = degree((uint32_t)n);
         if (d <= LARGE_NET_PINS)
           at[d + 1] += 1;
@@ -430,19 +441,26 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
           large.push_back((uint32_t)n);
       }
       for (size_t d = 1; d < at.size(); ++d) at[d] += at[d - 1];
+      order.resize(first + at.back() + large.size());
       for (size_t n = 0; n < n_nets; ++n) {
+        if (live[n] != cls) continue;
         const uint32_t d = degree((uint32_t)n);
-        if (d <= LARGE_NET_PINS) order[at[d]++] = (uint32_t)n;
+        if (d <= LARGE_NET_PINS) order[first + at[d]++] = (uint32_t)n;
       }
       std::stable_sort(large.begin(), large.end(), [&](uint32_t a, uint32_t b) { return degree(a) > degree(b); });
-      std::copy(large.begin(), large.end(), order.begin() + (n_nets - large.size()));
+      std::copy(large.begin(), large.end(), order.end() - large.size());
     }
+    for (int cls = 1; cls >= 0; --cls)
+      for (size_t n = n_nets; n < n_net_total; ++n)
+        if (live[n] == cls) order.push_back((uint32_t)n);
     std::vector<uint32_t> start2(n_net_total + 1, 0), unit2(n_pins_total), new_pin(n_pins_total);
     std::vector<double> xy2(2 * n_pins_total), w2(n_net_total);
+    std::vector<uint8_t> live2(n_net_total);
     uint32_t q = 0;
     for (size_t k = 0; k < n_net_total; ++k) {
-      const uint32_t n = k < n_nets ? order[k] : (uint32_t)k;
+      const uint32_t n = order[k];
       w2[k] = net_weight[n];
+      live2[k] = live[n];
       for (uint32_t i = net_start[n]; i < net_start[n + 1]; ++i, ++q) {
         new_pin[i] = q;
         unit2[q] = pin_unit[i];
@@ -451,24 +469,33 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
       }
       start2[k + 1] = q;
     }
-    // WA segments: runs of consecutive nets with one lane-group size; the
-    // looping wavefronts (longest nets) are scheduled first
-    std::vector<int4> runs;
+    // WA segments: runs of consecutive nets of one class with one lane-group
+    // size; per class the looping wavefronts (longest nets) come first, and
+    // the fixed class starts on a block boundary
+    std::vector<int4> runs[2];
     for (size_t k = 0; k < n_net_total; ++k) {
       const int g = wa_lanes(start2[k + 1] - start2[k]);
-      if (!runs.empty() && runs.back().z == g && (size_t)(runs.back().x + runs.back().y) == k)
-        runs.back().y += 1;
+      std::vector<int4> &rc = runs[live2[k]];
+      if (!rc.empty() && rc.back().z == g && (size_t)(rc.back().x + rc.back().y) == k)
+        rc.back().y += 1;
       else
-        runs.push_back(make_int4((int)k, 1, g, 0));
+        rc.push_back(make_int4((int)k, 1, g, 0));
     }
-    std::stable_partition(runs.begin(), runs.end(), [](const int4 &r) { return r.z == 0; });
-    if (runs.size() > (size_t)WA_MAX_SEGMENTS) fail(TEXO_GP_ERR_UNSUPPORTED);
+    wa_seg.clear();
     int waves = 0;
-    for (int4 &r : runs) {
-      r.w = waves;
-      waves += r.z == 0 ? r.y : (int)(((int64_t)r.y * r.z + 63) / 64);
+    for (int cls = 1; cls >= 0; --cls) {
+      std::stable_partition(runs[cls].begin(), runs[cls].end(), [](const int4 &r) { return r.z == 0; });
+      for (int4 &r : runs[cls]) {
+        r.w = waves;
+        waves += r.z == 0 ? r.y : (int)(((int64_t)r.y * r.z + 63) / 64);
+        wa_seg.push_back(r);
+      }
+      if (cls == 1) {
+        waves = (waves + BLOCK / 64 - 1) / (BLOCK / 64) * (BLOCK / 64);
+        wa_waves_live = waves;
+      }
     }
-    wa_seg = runs;
+    if (wa_seg.size() > (size_t)WA_MAX_SEGMENTS) fail(TEXO_GP_ERR_UNSUPPORTED);
     wa_waves = waves;
     for (auto &i : u_pins) i = new_pin[i];
     // each pin's derivative goes straight to its unit's entry (nets first,
@@ -719,6 +746,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   const size_t FG = (size_t)std::max(F, 1) * G, n = M + Fl;
   auto blocks = [](size_t count, size_t per) { return std::max<size_t>(1, (count + per - 1) / per); };
   nb_wa = wa_waves == 0 ? 0 : (int)blocks((size_t)wa_waves, BLOCK / 64);
+  nb_wa_live = wa_waves_live / (BLOCK / 64);
   nb_energy = F > 0 ? poisson_energy_blocks((int)W, (int)H, F) : 0;
   nb_gf = (int)blocks(Fl, BLOCK);
   nb_ug = (int)blocks(U, BLOCK);
@@ -832,6 +860,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   arena.reserve(d_eval_coords, C);
   arena.reserve(d_perm_next, n);
   arena.reserve(d_sort_count, (size_t)n_keys + 1);
+  arena.reserve(d_fixed_wa, 1);
 
   if (pool) {
     image = static_cast<char *>(pool->acquire(image_bytes, &image_cap));
@@ -894,7 +923,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
 }
 
 Task Job::task(int sl, const double *coords, double *grad, const double *metric, double g, bool skip,
-               EvalScalars *scalars) const {
+               EvalScalars *scalars, bool fixed_nets) const {
   Task t{};
   t.j = js;
   const Slot &w = slot[sl];
@@ -916,7 +945,11 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.part_energy = w.part_energy;
   t.part_gf = w.part_gf;
   t.part_ug = w.part_ug;
-  t.nb_wa = nb_wa;
+  // the nets of fixed pins run when their value at this gamma is not held
+  t.nb_wa = fixed_nets ? nb_wa : nb_wa_live;
+  t.nb_wa_live = nb_wa_live;
+  t.fixed_wa = d_fixed_wa;
+  t.fixed_wa_mode = nb_wa == nb_wa_live ? 0 : fixed_nets ? 1 : 2;
   t.nb_energy = F > 0 ? nb_energy : 0;
   t.nb_gf = F > 0 && Fl > 0 ? nb_gf : 0;
   t.nb_ug = nb_ug;

@@ -894,6 +894,8 @@ __device__ void finalize_task(const Task &t) {
   for (int k = 0; k < 2 * MAXF; ++k) cv[k] = 0;
   // one pass over the four partial arrays, so that their loads are in
   // flight together
+  // WA: the nets with a movable pin, then (when evaluated) those of fixed pins
+  double fixed_wa = 0.0;
   const int n = max(max(t.nb_wa, t.nb_conv), max(t.nb_ug, t.nb_gf));
   for (int b = threadIdx.x; b < n; b += BLOCK) {
     if (b < t.nb_conv) {
@@ -901,7 +903,10 @@ __device__ void finalize_task(const Task &t) {
 #pragma unroll
       for (int k = 0; k < 2 * MAXF; ++k) cv[k] += pc[k];
     }
-    if (b < t.nb_wa) acc[0] += t.part_wa[b];
+    if (b < t.nb_wa_live)
+      acc[0] += t.part_wa[b];
+    else if (b < t.nb_wa)
+      fixed_wa += t.part_wa[b];
     if (b < t.nb_ug) {
       const double *p = t.part_ug + (size_t)b * UG_STRIDE;
 #pragma unroll
@@ -923,6 +928,9 @@ __device__ void finalize_task(const Task &t) {
     acc[k] = k < NS ? wave_sum(acc[k]) : wave_max(acc[k]);
     if (lane == 0) sh[k][wave] = acc[k];
   }
+  __shared__ double sfw[BLOCK / 64];
+  fixed_wa = wave_sum(fixed_wa);
+  if (lane == 0) sfw[wave] = fixed_wa;
   __syncthreads();
   double r[NS + NM];
 #pragma unroll
@@ -940,7 +948,14 @@ __device__ void finalize_task(const Task &t) {
   // sums (force l1 x MAXF, raw l1 x MAXF), then unit / filler kkt and flag
   if (threadIdx.x == 0) {
     EvalScalars *sc = t.scalars;
-    sc->wa_value = r[0];
+    double fw = 0.0;
+    if (t.fixed_wa_mode == 1) {
+      fw = ((sfw[0] + sfw[1]) + sfw[2]) + sfw[3];
+      *t.fixed_wa = fw;
+    } else if (t.fixed_wa_mode == 2) {
+      fw = *t.fixed_wa;
+    }
+    sc->wa_value = r[0] + fw;
     sc->wa_l1 = r[1];
 #pragma unroll
     for (int f = 0; f < MAXF; ++f) {
