@@ -23,6 +23,78 @@ constexpr double FX_INV = 1.0 / 1099511627776.0;
 constexpr uint32_t NO_UNIT = 0xffffffffu;          // fixed pin marker
 constexpr double AUGMENTED_DENSITY_BETA = 2000.0;  // eplace.rs:125
 constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+// Lane-group reductions on the DPP path (no LDS traffic, unlike ds_bpermute
+// shuffles).  Inside a row of 16 lanes four data-parallel moves pair lanes
+// symmetrically: quad_perm [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1]
+// (lane ^ 2), row_half_mirror (lane i <-> 7 - i within 8) and row_mirror
+// (lane i <-> 15 - i within 16); after log2(G) of them every lane of an
+// aligned group of G <= 16 lanes holds the same bits (each step combines the
+// same two values, and the operations are commutative).  Groups of 32 and 64
+// combine the rows' lane-0 values, read with v_readlane, in row order.  Every
+// lane of the wavefront must be active.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ unsigned int dpp_u32(unsigned int v) {
+  return (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL, class T>
+__device__ __forceinline__ T dpp_move(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lanes");
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp_u32<CTRL>(__builtin_bit_cast(unsigned int, v)));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long lo = dpp_u32<CTRL>((unsigned int)b), hi = dpp_u32<CTRL>((unsigned int)(b >> 32));
+    return __builtin_bit_cast(T, (hi << 32) | lo);
+  }
+}
+template <class T>
+__device__ __forceinline__ T read_lane(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, (unsigned int)__builtin_amdgcn_readlane((int)__builtin_bit_cast(unsigned int, v), lane));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)b, lane);
+    const unsigned long long hi = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(b >> 32), lane);
+    return __builtin_bit_cast(T, (hi << 32) | lo);
+  }
+}
+template <int G, class T, class Op>
+__device__ __forceinline__ T lane_reduce(T v, Op op) {
+  static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "group of 2^k lanes");
+  if constexpr (G >= 2) v = op(v, dpp_move<0xB1>(v));
+  if constexpr (G >= 4) v = op(v, dpp_move<0x4E>(v));
+  if constexpr (G >= 8) v = op(v, dpp_move<0x141>(v));
+  if constexpr (G >= 16) v = op(v, dpp_move<0x140>(v));
+  if constexpr (G == 32) {
+    const T r01 = op(read_lane(v, 0), read_lane(v, 16)), r23 = op(read_lane(v, 32), read_lane(v, 48));
+    v = (threadIdx.x & 32) ? r23 : r01;
+  } else if constexpr (G == 64) {
+    v = op(op(op(read_lane(v, 0), read_lane(v, 16)), read_lane(v, 32)), read_lane(v, 48));
+  }
+  return v;
+}
+struct OpAdd {
+  template <class T>
+  __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }
+};
+struct OpMaxF {
+  __device__ __forceinline__ double operator()(double a, double b) const { return fmax(a, b); }
+  __device__ __forceinline__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+struct OpMinF {
+  __device__ __forceinline__ double operator()(double a, double b) const { return fmin(a, b); }
+};
+struct OpMaxI {
+  template <class T>
+  __device__ __forceinline__ T operator()(T a, T b) const { return a > b ? a : b; }
+};
+struct OpMinI {
+  template <class T>
+  __device__ __forceinline__ T operator()(T a, T b) const { return a < b ? a : b; }
+};
 constexpr int GF_STRIDE = 2 * MAXF + 2;  // filler partials: force l1, raw l1 per field, kkt max, flag
 constexpr int UG_STRIDE = 2 * MAXF + 3;  // unit partials: wa l1, force l1, raw l1 per field, kkt max, flag
 

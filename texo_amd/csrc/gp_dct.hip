@@ -25,10 +25,7 @@ typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return lane_reduce<64>(v, OpAdd{}); }
 
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
   unsigned int u = __float_as_uint(f);
@@ -60,10 +57,7 @@ __device__ __forceinline__ half4_t half4_of(uint2 v) {
   return h;
 }
 
-__device__ __forceinline__ float wave_max_f(float v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
-}
+__device__ __forceinline__ float wave_max_f(float v) { return lane_reduce<64>(v, OpMaxF{}); }
 
 // Stage geometry resolved per block from the task.
 struct Stage {
@@ -250,7 +244,7 @@ __device__ unsigned int input_amax(const Task &t, int f) {
   }
   unsigned int m = 0u;
   for (int i = threadIdx.x; i < n; i += FAST_THREADS) m = max(m, p[(size_t)i * stride]);
-  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, off, 64));
+  m = lane_reduce<64>(m, OpMaxI{});
   __shared__ unsigned int sm[FAST_THREADS / 64];
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
   __syncthreads();

@@ -16,18 +16,9 @@ namespace texo {
 
 namespace {
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return lane_reduce<64>(v, OpAdd{}); }
+__device__ __forceinline__ double wave_max(double v) { return lane_reduce<64>(v, OpMaxF{}); }
+__device__ __forceinline__ double wave_min(double v) { return lane_reduce<64>(v, OpMinF{}); }
 
 // Block-wide sum of K values per thread; thread 0 writes out[0..K).
 template <int K>
@@ -191,7 +182,7 @@ __global__ void __launch_bounds__(BLOCK) k_deposit(const Task *__restrict__ task
   }
   // charges outside the window (moved since the sort): one count per block
   __shared__ unsigned int s_moved[BLOCK / 64];
-  for (int off = 32; off > 0; off >>= 1) moved += __shfl_xor(moved, off, 64);
+  moved = lane_reduce<64>(moved, OpAdd{});
   if ((threadIdx.x & 63) == 0) s_moved[threadIdx.x >> 6] = moved;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -240,10 +231,8 @@ constexpr int SORT_LDS_KEYS = 1024;
 __device__ __forceinline__ void block_key_range(int key, bool valid, int &kmin, int &kmax) {
   __shared__ int s_lo[BLOCK / 64], s_hi[BLOCK / 64];
   int lo = valid ? key : 0x7fffffff, hi = valid ? key : -1;
-  for (int off = 32; off > 0; off >>= 1) {
-    lo = min(lo, __shfl_xor(lo, off, 64));
-    hi = max(hi, __shfl_xor(hi, off, 64));
-  }
+  lo = lane_reduce<64>(lo, OpMinI{});
+  hi = lane_reduce<64>(hi, OpMaxI{});
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
     s_lo[wave] = lo;
@@ -370,10 +359,7 @@ __device__ __forceinline__ void tile_range(int v, int tiles, int &lo, int &hi) {
   hi = min((v + DEP_HALO) / DEP_TILE, tiles - 1);
 }
 
-__device__ __forceinline__ long long wave_sum_ll(long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+__device__ __forceinline__ long long wave_sum_ll(long long v) { return lane_reduce<64>(v, OpAdd{}); }
 
 __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ tasks, BlockPlan plan) {
   __shared__ long long s_over[BLOCK / 64][MAXF], s_net[BLOCK / 64][MAXF];
@@ -422,7 +408,7 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
     const long long o = wave_sum_ll(f == k ? over : 0);
     const long long n = wave_sum_ll(f == k ? net : 0);
     unsigned int mx = f == k ? amax : 0u;
-    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (unsigned int)__shfl_xor((int)mx, off, 64));
+    mx = lane_reduce<64>(mx, OpMaxI{});
     if (lane == 0) {
       s_over[wave][k] = o;
       s_net[wave][k] = n;
@@ -583,30 +569,25 @@ __device__ __forceinline__ double2 wa_derivative(double2 c, double xmin, double 
   return make_double2(ddx, ddy);
 }
 
-// Butterfly reductions inside aligned groups of G lanes.
+// Reductions inside aligned groups of G lanes (every lane of the group gets
+// the same bits).
 template <int G>
 __device__ __forceinline__ double group_min(double v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
+  return lane_reduce<G>(v, OpMinF{});
 }
 template <int G>
 __device__ __forceinline__ double group_max(double v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  return lane_reduce<G>(v, OpMaxF{});
 }
 template <int G>
 __device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  return lane_reduce<G>(v, OpAdd{});
 }
 
 // One net per aligned group of G lanes (a WA segment of G-lane nets).  Lane
 // r of the group holds pins b + r + G k (k < WA_PINS_PER_LANE) in
 // registers: every pin coordinate is loaded once and its four exponentials
-// are computed once; the group combines extremes and sums by butterflies.
+// are computed once; the group combines extremes and sums by lane reductions.
 template <int G>
 __device__ double wa_group_net(const JobStatic &a, const Task &t, const int4 &seg, int wave) {
   constexpr int K = WA_PINS_PER_LANE;
@@ -940,7 +921,7 @@ __device__ void finalize_task(const Task &t) {
   __shared__ long long scv[2 * MAXF][BLOCK / 64];
 #pragma unroll
   for (int k = 0; k < 2 * MAXF; ++k) {
-    for (int off = 32; off > 0; off >>= 1) cv[k] += __shfl_xor(cv[k], off, 64);
+    cv[k] = lane_reduce<64>(cv[k], OpAdd{});
     if (lane == 0) scv[k][wave] = cv[k];
   }
   __syncthreads();
