@@ -6,6 +6,8 @@ sequential sum by at most 9 * charges * 2^-41 per bin; WA runs in FP64
 (relative 1e-12); the Poisson transform carries FP32 (relative field error
 <= 1e-5) or BF16x3 (<= 1e-3) error.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -30,7 +32,7 @@ def pair(problem, precision=abi.DCT_FP32):
     return g, o
 
 
-@pytest.mark.parametrize("name", ["plain_1k", "ecp5_25k"])
+@pytest.mark.parametrize("name", ["plain_1k", "ecp5_25k", "ecp5_85k"])
 @pytest.mark.parametrize("precision,field_tol", [(abi.DCT_FP32, 1e-5), (abi.DCT_BF16X3, 1e-3),
                                                  (abi.DCT_FP16_SCALED, 1e-5)])
 def test_evaluate_matches_oracle(name, precision, field_tol):
@@ -165,6 +167,34 @@ def test_fixed_iterations_track_the_oracle(precision):
     assert abs(rg.field_overflow[0] - ro.field_overflow[0]) <= 0.02
     g.close()
     o.close()
+
+
+@functools.lru_cache(maxsize=None)
+def oracle_trajectory(name, iterations):
+    lib = oracle.load()
+    lib.oracle_set_threads(8)
+    o = abi.Session(lib, "oracle", synth.CONFIGS[name]())
+    rep = o.iterate(iterations).as_dict()
+    o.close()
+    lib.oracle_set_threads(1)
+    return rep
+
+
+@pytest.mark.parametrize("name,iterations", [("ecp5_25k", 150), ("ecp5_85k", 60)])
+@pytest.mark.parametrize("precision", [abi.DCT_FP32, abi.DCT_BF16X3, abi.DCT_FP16_SCALED])
+def test_config_trajectories_track_the_oracle(name, iterations, precision):
+    # the 24k-cell / 128^2 and 84k-cell / 256^2 configurations: final HPWL
+    # within 0.5% of the FP64 oracle after the same fixed iterations, per
+    # field overflow within 0.02, for every transform precision
+    print(f"trajectory {name} x{iterations} precision={precision}", flush=True)
+    g = texo_amd.Session(synth.CONFIGS[name](), abi.options(dct_precision=precision))
+    rg = g.iterate(iterations).as_dict()
+    g.close()
+    ro = oracle_trajectory(name, iterations)
+    print("gpu", rg, "oracle", ro, flush=True)
+    assert rg["iterations"] == ro["iterations"] == iterations
+    assert abs(rg["final_hpwl"] - ro["final_hpwl"]) <= 0.005 * ro["final_hpwl"]
+    assert max(abs(a - b) for a, b in zip(rg["field_overflow"], ro["field_overflow"])) <= 0.02
 
 
 def test_place_with_reference_stop_rules_matches_oracle():
