@@ -288,6 +288,10 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="time the batched form only (for profiling)")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="run the saturated batched serving stream only and print its result (for profiling)")
+    ap.add_argument("--only-eco", action="store_true",
+                    help="run the incremental stream sample only (--eco-requests, --eco-full) and print its result")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference reading for the HBM counters)")
     args = ap.parse_args()
@@ -311,6 +315,14 @@ def main():
 
     prec = {"fp32": abi.DCT_FP32, "bf16x3": abi.DCT_BF16X3, "fp16_scaled": abi.DCT_FP16_SCALED}
     mfma = prec[args.precision]
+    if args.only_serving:
+        stream = synth.serving_stream(args.serving_jobs, rate=args.serving_rate)
+        r = run_serving(stream, mfma, 256, False, args.serving_clients)
+        print(json.dumps({"serving_batched_saturated": r, "precision": args.precision}), flush=True)
+        return
+    if args.only_eco:
+        print(json.dumps({"eco": run_eco(max(1, args.eco_requests), args.eco_full, args.serving_clients)}), flush=True)
+        return
     base = synth.ecp5_85k()
 
     if args.only_batched:

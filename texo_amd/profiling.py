@@ -7,7 +7,12 @@ on bench.py (listed in each summary); this module only reads their CSV files:
         CAL_FETCH_DIR CAL_WRITE_DIR COPY_BYTES BATCH OUT_DIR
 
 writes OUT_DIR/<name>.md (kernel statistics with the bench line's roofline
-for comparison) and OUT_DIR/traffic.json (HBM bytes per launch per kernel).
+for comparison) and OUT_DIR/traffic.json (HBM bytes per launch per kernel);
+
+    python -m texo_amd.profiling --stats STATS_DIR RESULT_FILE OUT_MD TITLE COMMAND
+
+writes the kernel statistics of any other profiled run (a serving stream,
+the incremental stream) with the JSON result line the run printed.
 """
 from __future__ import annotations
 
@@ -105,7 +110,28 @@ def traffic_markdown(t: dict, fetch_cmd: str, write_cmd: str, cal_cmds: list[str
     return "\n".join(out) + "\n"
 
 
+def plain_stats(stats_dir: str, result_file: str, out_md: str, title: str, command: str):
+    rows = _csv(stats_dir, "kernel_stats.csv")
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = [f"# {title}", "", "Command (repository root, on the MI355X):", "", "```", command, "```", "",
+           "| kernel | calls | total ms | average us | % of kernel time |", "|---|---:|---:|---:|---:|"]
+    for r in rows[:24]:
+        t = float(r["TotalDurationNs"])
+        out.append(f"| `{kernel_name(r['Name'])}` | {r['Calls']} | {t / 1e6:.2f} | "
+                   f"{float(r['AverageNs']) / 1e3:.1f} | {100 * t / total:.1f} |")
+    with open(result_file) as f:
+        lines = [ln for ln in f.read().splitlines() if ln.startswith("{")]
+    if lines:
+        out += ["", "Result line of the same run:", "", "```json", lines[-1], "```"]
+    with open(out_md, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
 def main(argv):
+    if argv and argv[0] == "--stats":
+        plain_stats(*argv[1:6])
+        return
     (stats_dir, bench_json, fetch_dir, write_dir, cal_f, cal_w, copy_bytes, batch, out_dir) = argv[:9]
     copy_bytes, batch = int(copy_bytes), int(batch)
     with open(bench_json) as f:
