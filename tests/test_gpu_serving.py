@@ -121,3 +121,47 @@ def test_lockstep_batch_run_matches_place():
         got = [batch.result(i) for i in range(len(jobs))]
     for a, b in zip(ref, got):
         same(a, b)
+
+
+def test_multistart_jobs_share_their_design_and_match_alone():
+    # starts of one design read one shared copy of its design block (from
+    # the second start on); each start, and a different design beside them,
+    # iterates exactly as it does alone
+    base = synth.serving_job(31, 5000, 4)
+    jobs = [(synth.multistart(base, s), abi.DCT_FP16_SCALED) for s in range(3)]
+    jobs.insert(1, (synth.serving_job(32, 3000, 2), abi.DCT_BF16X3))
+    print("multi-start batch", [p.meta.get("start") for p, _ in jobs], flush=True)
+    with texo_amd.Batch() as batch:
+        for p, prec in jobs:
+            batch.add(p, abi.options(dct_precision=prec))
+        batch.iterate(30)
+        got = [batch.coordinates(i) for i in range(len(jobs))]
+        reps = [batch.result(i, targets=False)[1] for i in range(len(jobs))]
+    for (p, prec), x, rep in zip(jobs, got, reps):
+        s = texo_amd.Session(p, abi.options(dct_precision=prec))
+        r = s.iterate(30)
+        assert np.array_equal(s.coordinates(), x)
+        assert r.evaluations == rep["evaluations"] and r.final_hpwl == rep["final_hpwl"]
+        s.close()
+
+
+def test_second_start_joining_a_running_design_matches_alone():
+    # the second start arrives while the first runs and promotes the first
+    # one's design block to a shared copy mid-run; the third arrives after
+    # the first two finished (a new owner)
+    base = synth.serving_job(33, 4000, 4)
+    a, b, c = (synth.multistart(base, s) for s in range(3))
+    ref = [alone(p, abi.DCT_FP16_SCALED, 50) for p in (a, b, c)]
+    opts = abi.options(dct_precision=abi.DCT_FP16_SCALED, fixed_iterations=50)
+    with texo_amd.Server() as srv:
+        ia = srv.submit(a, opts)
+        deadline = time.time() + 60
+        while srv.stats()["rounds"] < 15 and time.time() < deadline:
+            time.sleep(0.001)
+        ib = srv.submit(b, opts)
+        same(ref[0], srv.wait(ia))
+        same(ref[1], srv.wait(ib))
+        ic = srv.submit(c, opts)
+        same(ref[2], srv.wait(ic))
+        st = srv.stats()
+    assert st["completed"] == 3 and st["failed"] == 0

@@ -828,6 +828,7 @@ void texo_gp_server::collect(bool block) {
       const size_t charges = j.M + j.Fl;
       engine->detach(&j);
       j.arena.release();
+      j.design.reset();
       j.release_image();
       std::lock_guard<std::mutex> lk(mu);
       hbm_in_use -= std::min(hbm_in_use, held);

@@ -321,9 +321,42 @@ void Engine::attach(Job *job) {
   if (!job->empty) {
     // the first round sorts the charges by tile (the probe evaluation)
     job->upload(basis((int)job->W, (int)job->H), blocks_, io_stream_);
+    share_design(job);
     HIP(hipEventRecord(ev_upload_, io_stream_));
     for (RoundCtx &c : ctx_) c.upload_wait = true;
   }
+}
+
+// Jobs of one design (same design-block bytes, matched by their 128-bit
+// fingerprint and size) read one device copy of the design block: the first
+// job's arena holds it until a second job arrives, which copies it once into
+// a block of its own that both then use.  The copy runs on the upload stream
+// after the first job's upload; every context waits for the upload stream
+// before its next round, and the block is never written after its upload.
+void Engine::share_design(Job *job) {
+  if (job->design_bytes == 0) return;
+  designs_.erase(std::remove_if(designs_.begin(), designs_.end(),
+                                [](const DesignEntry &e) { return !e.owner && e.shared.expired(); }),
+                 designs_.end());
+  DesignEntry *e = nullptr;
+  for (DesignEntry &d : designs_)
+    if (d.bytes == job->design_bytes && d.hash[0] == job->design_hash[0] && d.hash[1] == job->design_hash[1])
+      e = &d;
+  if (!e) {
+    designs_.push_back({{job->design_hash[0], job->design_hash[1]}, job->design_bytes, job, {}});
+    return;
+  }
+  std::shared_ptr<SharedDesign> sd = e->shared.lock();
+  if (!sd) {
+    sd = std::make_shared<SharedDesign>();
+    sd->pool = blocks_;
+    sd->base = blocks_->acquire(job->design_bytes, &sd->capacity);
+    HIP(hipMemcpyAsync(sd->base, e->owner->arena.base(), job->design_bytes, hipMemcpyDeviceToDevice, io_stream_));
+    e->owner->use_design(sd);
+    e->owner = nullptr;
+    e->shared = sd;
+  }
+  job->use_design(sd);
 }
 
 void Engine::await_uploads(int c) {
@@ -335,6 +368,8 @@ void Engine::await_uploads(int c) {
 void Engine::detach(Job *job) {
   if (job->pool_index >= 0) pool_used_[job->pool_index] = false;
   job->pool_index = -1;
+  for (DesignEntry &d : designs_)
+    if (d.owner == job) d.owner = nullptr;
 }
 
 void Engine::ensure_stage(RoundCtx &c, size_t bytes) {

@@ -63,6 +63,18 @@ class BlockPool {
   bool host_;
 };
 
+// Design-static device arrays shared by the jobs of one design (multi-start
+// placements, repeated submissions): one device copy of the design block that
+// all their kernels read, returned to its pool with the last job using it.
+struct SharedDesign {
+  std::shared_ptr<BlockPool> pool;
+  void *base = nullptr;
+  size_t capacity = 0;
+  ~SharedDesign() {
+    if (base && pool) pool->release(base, capacity);
+  }
+};
+
 // One allocation carved into 256-byte aligned buffers.
 class Arena {
  public:
@@ -198,6 +210,11 @@ struct Job {
   // block of `image_pool` when given) and copied to the device in one piece.
   char *image = nullptr;
   size_t image_bytes = 0, image_cap = 0, zero_begin = 0, zero_end = 0;
+  // the design block: the leading design_bytes of the arena (arrays fixed by
+  // the problem alone) and its 128-bit fingerprint
+  size_t design_bytes = 0;
+  uint64_t design_hash[2] = {0, 0};
+  std::shared_ptr<SharedDesign> design;  // set when the job reads a shared copy
   std::shared_ptr<BlockPool> image_pool;
   std::unique_ptr<char[]> image_heap;
 
@@ -255,6 +272,8 @@ struct Job {
   // and the accumulator clear on `s`; the image stays alive until
   // release_image(), so nothing waits here.
   void upload(const Basis *b, const std::shared_ptr<BlockPool> &pool, hipStream_t s);
+  void use_design(const std::shared_ptr<SharedDesign> &sd);  // read the design block from sd
+  void build_js();
   // New member / filler charges: host state, then the device copies on `s`
   // (waited for before returning).
   void apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas, hipStream_t s);
@@ -383,6 +402,16 @@ class Engine {
   void advance(Job *job);
   void area_round(Job *job);
   void await_uploads(int c);  // orders context c's compute stream after the enqueued uploads
+  void share_design(Job *job);  // at attach: read the design block from a copy shared with its design's jobs
+  // Designs seen by attached jobs: the first job of a design owns the
+  // design block in its arena; the second promotes it to a shared copy.
+  struct DesignEntry {
+    uint64_t hash[2];
+    size_t bytes;
+    Job *owner;  // first job, until promoted or detached
+    std::weak_ptr<SharedDesign> shared;
+  };
+  std::vector<DesignEntry> designs_;
   hipStream_t cur() const { return ctx_[cur_].stream; }  // stream of the context being planned or advanced
   int device_;
   hipStream_t io_stream_ = nullptr;

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <numeric>
 #include <set>
+#include <type_traits>
 
 namespace texo {
 
@@ -649,7 +650,45 @@ void Job::upload(const Basis *b, const std::shared_ptr<BlockPool> &pool, hipStre
   arena.commit(pool);
   HIP(hipMemcpyAsync(arena.base(), image, image_bytes, hipMemcpyHostToDevice, s));
   HIP(hipMemsetAsync(arena.base() + zero_begin, 0, zero_end - zero_begin, s));
+  build_js();
+}
 
+// Points the design block's arrays at a shared copy (same offsets).
+void Job::use_design(const std::shared_ptr<SharedDesign> &sd) {
+  char *const from = arena.base(), *const to = static_cast<char *>(sd->base);
+  auto move = [&](auto *&p) {
+    if (!p) return;
+    const char *c = reinterpret_cast<const char *>(p);
+    if (c < from || c >= from + design_bytes) fail(TEXO_GP_ERR_DEVICE);
+    p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(to + (c - from));
+  };
+  move(d_base);
+  move(d_m_unit);
+  move(d_m_off);
+  move(d_m_off32);
+  move(d_m_field);
+  move(d_du_unit);
+  move(d_du_mstart);
+  move(d_fl_field);
+  move(d_net_start);
+  move(d_pin_unit);
+  move(d_pin_xy);
+  move(d_pin_xy32);
+  move(d_net_weight);
+  move(d_wa_seg);
+  move(d_u_pstart);
+  move(d_pin_slot);
+  move(d_u_nbase);
+  move(d_wl_diag);
+  move(d_unit_du);
+  move(d_lower);
+  move(d_upper);
+  design = sd;
+  build_js();
+}
+
+void Job::build_js() {
+  const Basis *b = basis;
   js = JobStatic{};
   js.W = (int)W;
   js.H = (int)H;
@@ -766,6 +805,8 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
     if (src) copies.push_back({at, src, count * sizeof(*ptr)});
     return at;
   };
+  // the design block: arrays fixed by the problem alone (not by the start
+  // coordinates or the charges), shareable between jobs of one design
   const size_t o_base = data(d_base, FG, nullptr);
   data(d_m_unit, M, m_unit.data());
   // offsets as FP32 pairs when that is exact for all of them
@@ -785,13 +826,10 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
     data(d_m_off, M, m_off.data());
     data(d_m_off32, 0, nullptr);
   }
-  data(d_m_charge, M, m_charge.data());
   data(d_m_field, M, m_field.data());
   data(d_du_unit, DU, du_unit.data());
   data(d_du_mstart, DU + 1, du_mstart.data());
-  data(d_du_fcharge, (size_t)std::max(F, 1) * DU, h_du_fcharge.data());
   data(d_fl_field, Fl, fl_field.data());
-  data(d_fl_charge, Fl, fl_charge.data());
   data(d_net_start, n_net_total + 1, net_start.data());
   data(d_pin_unit, n_pins_total, pin_unit.data());
   if (off32) {
@@ -810,6 +848,11 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   data(d_unit_du, U, unit_du.data());
   data(d_lower, C, lower.data());
   data(d_upper, C, upper.data());
+  design_bytes = arena.planned();
+  // per job: charges (area rounds change them) and the start
+  data(d_m_charge, M, m_charge.data());
+  data(d_du_fcharge, (size_t)std::max(F, 1) * DU, h_du_fcharge.data());
+  data(d_fl_charge, Fl, fl_charge.data());
   // the previous iterate starts as a copy of x (DynamicNesterovState::new,
   // on the device); the sorted charge copies are first written by the sort
   // that precedes any deposit, which walks the input order `perm`
@@ -920,6 +963,38 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   // input order until the engine sorts the charges by tile
   uint32_t *perm = reinterpret_cast<uint32_t *>(image + o_perm);
   for (size_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+  // 128-bit fingerprint of the design block (array contents at their offsets),
+  // matched when jobs of one design share its device copy
+  uint64_t h1 = 0x243F6A8885A308D3ull ^ design_bytes, h2 = 0x13198A2E03707344ull ^ ((uint64_t)W << 32 | H);
+  auto mix = [&](const void *p, size_t bytes, size_t at) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    h1 = (h1 ^ at) * 0x9E3779B97F4A7C15ull;
+    h2 = (h2 + at) * 0xC2B2AE3D27D4EB4Full;
+    size_t k = 0;
+    for (; k + 8 <= bytes; k += 8) {
+      uint64_t w;
+      std::memcpy(&w, c + k, 8);
+      h1 = ((h1 ^ w) * 0x9E3779B97F4A7C15ull);
+      h1 ^= h1 >> 31;
+      h2 = ((h2 + w) * 0xC2B2AE3D27D4EB4Full);
+      h2 ^= h2 >> 29;
+    }
+    uint64_t w = 0;
+    std::memcpy(&w, c + k, bytes - k);
+    h1 = ((h1 ^ w ^ bytes) * 0x9E3779B97F4A7C15ull);
+    h2 = ((h2 + w + bytes) * 0xC2B2AE3D27D4EB4Full);
+  };
+  mix(image + o_base, FG * sizeof(long long), o_base);
+  for (const Copy &c : copies)
+    if (c.at < design_bytes) mix(c.src, c.bytes, c.at);
+  h1 ^= h1 >> 33;
+  h1 *= 0xFF51AFD7ED558CCDull;
+  h1 ^= h1 >> 33;
+  h2 ^= h2 >> 31;
+  h2 *= 0xC4CEB9FE1A85EC53ull;
+  h2 ^= h2 >> 29;
+  design_hash[0] = h1;
+  design_hash[1] = h2;
 }
 
 Task Job::task(int sl, const double *coords, double *grad, const double *metric, double g, bool skip,
