@@ -61,10 +61,14 @@ BF16_PEAK_TFLOPS = 2500.0
 FP16_PEAK_TFLOPS = 2500.0
 
 
-def algorithmic_bytes(kernel: str, sizes: dict, problem) -> float:
-    """Compulsory HBM bytes of one evaluation of one job in a launch of `kernel`: each
-    input read once and each output written once, at storage precision.
-    Gathered unit coordinates count once per coordinate, not once per use."""
+def algorithmic_bytes(kernel: str, sizes: dict, problem):
+    """Compulsory HBM bytes of a launch of `kernel`, as (per design, per
+    evaluation): each input read once and each output written once, at
+    storage precision.  The design-static arrays (netlist, pin and member
+    offsets, unit tables, bounds) are read once per design in a launch, since
+    the jobs of one design share one device copy of them; coordinates,
+    charges, fields and results once per evaluation.  Gathered unit
+    coordinates count once per coordinate, not once per use."""
     M = problem.n_members
     Fl = sizes["fillers"]
     F = sizes["fields"]
@@ -74,22 +78,30 @@ def algorithmic_bytes(kernel: str, sizes: dict, problem) -> float:
     DU = sizes["density_units"]
     pins = problem.pin_cell.size + 2 * problem.arc_extra_weight.size
     nets = problem.n_nets + problem.arc_extra_weight.size
+    pin_cells = np.concatenate([problem.pin_cell, problem.arc_cells]).astype(np.int64)
+    movable_pins = int(np.count_nonzero(problem.unit_fixed[problem.cell_unit[pin_cells]] == 0))
+    # offsets stored as FP32 pairs when every one is exact in FP32
+    offs = np.concatenate([problem.member_offset_xy, problem.cell_offset_xy, problem.cell_fixed_xy])
+    off = 8 if np.array_equal(offs.astype(np.float32).astype(np.float64), offs) else 16
     if kernel == "deposit":
-        # coordinates (8 B each); per charge: coordinate index 4, offset 16,
+        # coordinates 8 B each; per sorted charge: coordinate index 4, offset,
         # charge 8, layer 1; int64 accumulators of members and fillers per
         # field and bin: initial value read 8 + sum written 8
-        return C * 8 + (M + Fl) * 29 + F * G * 2 * 16
+        return 0.0, C * 8 + (M + Fl) * (13 + off) + F * G * 2 * 16
     if kernel == "wa":
-        # per pin: unit 4 + offset/fixed xy 16 + derivative written 16; unit
-        # coordinates 16; per net: start 4 + weight 8 + id 4
-        return pins * 36 + U * 16 + nets * 16
+        # design: per pin unit 4 + offset / fixed coordinate + derivative slot
+        # 4; per net start 4 + weight 8.  Evaluation: unit coordinates 16;
+        # per movable pin its derivative written 16
+        return pins * (8 + off) + nets * 12, U * 16 + movable_pins * 16
     if kernel == "gather":
-        # k_gradient: per unit pin range 4 + arc split 4 + density unit 4 +
-        # coordinates 16 + gradient written 16; per movable pin its WA
-        # derivative 16; per density unit its member range 4; per member field
-        # 1, charge 8, offset 16; the potential F*G FP32 once; per filler field
-        # 1, charge 8, coordinates 16, gradient written 16
-        return U * 44 + pins * 16 + DU * 4 + M * 25 + F * G * 4 + Fl * 41
+        # k_gradient.  Design: per unit pin range 4 + arc split 4 + density
+        # unit 4; bounds 16 per coordinate; per density unit its member range
+        # 4; per member field 1 + offset; per filler field 1.  Evaluation: per
+        # coordinate position 8, metric 8, gradient written 8; per movable pin
+        # its WA derivative 16; per member charge 8; the potential F*G FP32
+        # once; per filler charge 8
+        return (U * 12 + C * 16 + DU * 4 + M * (1 + off) + Fl,
+                C * 24 + movable_pins * 16 + M * 8 + F * G * 4 + Fl * 8)
     raise ValueError(kernel)
 
 
@@ -349,12 +361,15 @@ def main():
                     "traffic": None, "kernel": "dct (4 stages)", "algorithmic_per_launch": flops,
                     "avg_launch_ms": batched["kernel_avg_ms"], "evaluations_per_launch": per_launch}
     else:
-        nbytes = per_launch * algorithmic_bytes(k, sizes, base)
+        # the batch's jobs are starts of one design: its design-static arrays
+        # are read once per launch
+        per_design, per_eval = algorithmic_bytes(k, sizes, base)
+        nbytes = per_design + per_launch * per_eval
         achieved = nbytes / avg_s / 1e9 if avg_s > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(k, args.batch), "kernel": k,
-                    "algorithmic_per_launch": nbytes, "avg_launch_ms": batched["kernel_avg_ms"],
-                    "evaluations_per_launch": per_launch}
+                    "algorithmic_per_launch": nbytes, "design_bytes": per_design, "evaluation_bytes": per_eval,
+                    "avg_launch_ms": batched["kernel_avg_ms"], "evaluations_per_launch": per_launch}
 
     serving = None
     if not args.no_serving and not args.only_batched:
