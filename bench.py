@@ -115,18 +115,45 @@ def progress(msg: str):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(problem, iterations: int):
+def cpu_baseline(problem, iterations: int, precision: int):
+    """The FP64 oracle on one host core over `iterations` GP iterations of
+    the workload (after one untimed iteration), and, as the checker, the
+    error of the HIP path at `precision` against it: one evaluation at the
+    start coordinates (potential, field at the charges, WA gradient, energy,
+    overflow) and the final HPWL and overflow after the same iterations."""
     import oracle
+    import texo_amd
     from texo_amd import abi
     lib = oracle.load()
     lib.oracle_set_threads(1)
     o = abi.Session(lib, "oracle", problem)
+    g = texo_amd.Session(problem, abi.options(dct_precision=precision))
+
+    def rel(a, b):
+        a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    x = g.initial
+    eg, eo = g.evaluate(x, 5.0, want_grids=True), o.evaluate(x, 5.0, want_grids=True)
+    parity = {"precision": precision, "evaluation_at": "start coordinates, gamma 5",
+              "potential_rel_l2": rel(eg["potential"], eo["potential"]),
+              "field_rel_l2": rel(np.concatenate([eg["unit_gradient"].ravel(), eg["filler_gradient"].ravel()]),
+                                  np.concatenate([eo["unit_gradient"].ravel(), eo["filler_gradient"].ravel()])),
+              "wa_gradient_rel_l2": rel(eg["wa_gradient"], eo["wa_gradient"]),
+              "energy_rel_max": float(np.max(np.abs(eg["energy"] - eo["energy"]) / np.abs(eo["energy"]))),
+              "overflow_abs_max": float(np.max(np.abs(eg["overflow"] - eo["overflow"])))}
     o.iterate(1)  # set-up evaluations and the first iteration, untimed
     t0 = time.perf_counter()
-    o.iterate(iterations)
+    ro = o.iterate(iterations)
     dt = time.perf_counter() - t0
+    rg = g.iterate(iterations + 1)
+    parity["iterations"] = iterations + 1
+    parity["hpwl_rel"] = abs(rg.final_hpwl - ro.final_hpwl) / ro.final_hpwl
+    parity["overflow_abs_max_final"] = float(max(abs(a - b) for a, b in zip(rg.as_dict()["field_overflow"],
+                                                                          ro.as_dict()["field_overflow"])))
     o.close()
-    return iterations / dt, dt
+    g.close()
+    return iterations / dt, dt, parity
 
 
 def run_single(problem, precision, steps, warmup):
@@ -393,10 +420,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         progress("cpu baseline")
-        v, dt = cpu_baseline(base, args.cpu_iterations)
+        v, dt, parity = cpu_baseline(base, args.cpu_iterations, mfma)
         cpu = {"value": v, "unit": "GP iterations/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_iterations} GP iterations of the 84k-cell / 256x256 workload, one job "
-                         f"(FP64 C++ restatement of the reference loop, 1 thread, {dt:.1f} s)"}
+                         f"(FP64 C++ restatement of the reference loop, 1 thread, {dt:.1f} s)",
+               "error_vs_oracle": parity}
 
     line = {
         "metric": METRIC,
