@@ -398,6 +398,24 @@ def main():
                     "algorithmic_per_launch": nbytes, "design_bytes": per_design, "evaluation_bytes": per_eval,
                     "avg_launch_ms": batched["kernel_avg_ms"], "evaluations_per_launch": per_launch}
 
+    # the Poisson transform's MFMA use, from a second, shorter timed run of
+    # the same batch with the transform launches timed (algorithmic flops:
+    # F*4*G*(W+H) per evaluation; the split modes issue three products each)
+    transform = None
+    if k != "dct" and not args.only_batched:
+        tb, _ = run_batched(starts, mfma, min(args.steps, 200), args.warmup, "dct")
+        t_s = tb["kernel_avg_ms"] * 1e-3
+        flops = tb["kernel_tasks_per_launch"] * dct_flops(sizes, base)
+        peak = {"fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS, "fp16_scaled": FP16_PEAK_TFLOPS}[args.precision]
+        issued = 1 if args.precision == "fp32" else 3
+        transform = {"bound": "mfma" if issued == 3 else "valu", "kernel": "dct (4 stages)",
+                     "algorithmic_per_launch": flops, "avg_launch_ms": tb["kernel_avg_ms"],
+                     "evaluations_per_launch": tb["kernel_tasks_per_launch"],
+                     "achieved": flops / t_s / 1e12 if t_s > 0 else 0.0, "unit": "TFLOP/s",
+                     "issued": issued * flops / t_s / 1e12 if t_s > 0 else 0.0, "peak": peak,
+                     "issued_frac": issued * flops / t_s / 1e12 / peak if t_s > 0 else 0.0}
+        progress(f"transform: {transform}")
+
     serving = None
     if not args.no_serving and not args.only_batched:
         stream = synth.serving_stream(args.serving_jobs, rate=args.serving_rate)
@@ -453,6 +471,7 @@ def main():
                   "batched_per_job_it_per_s": batched["per_job_it_per_s"],
                   "batched_vs_fp32_plain": batched["it_per_s"] / plain["it_per_s"]},
         "roofline": roofline,
+        "roofline_transform": transform,
         "cpu_baseline": cpu,
         "serving": serving,
         "eco": eco,
