@@ -165,3 +165,28 @@ def test_second_start_joining_a_running_design_matches_alone():
         same(ref[2], srv.wait(ic))
         st = srv.stats()
     assert st["completed"] == 3 and st["failed"] == 0
+
+
+def test_wide_transform_batch_tracks_single_jobs():
+    # 16 starts of the 84k-cell / 256^2 design: each launch of a job group
+    # (8 starts) has 256 wide transform tiles, so the batch runs the wide
+    # FP16-scaled kernel where a single job runs the 32x32 one; both carry
+    # the FP16-scaled transform's error, so they agree to its tolerance
+    base = synth.ecp5_85k()
+    starts = [synth.multistart(base, s) for s in range(16)]
+    opts = abi.options(dct_precision=abi.DCT_FP16_SCALED)
+    print("wide transform batch of 16 starts", flush=True)
+    with texo_amd.Batch() as batch:
+        for p in starts:
+            batch.add(p, opts)
+        batch.iterate(4)
+        reps = [batch.result(i, targets=False)[1] for i in range(len(starts))]
+    for k in (0, 7, 15):
+        s = texo_amd.Session(starts[k], opts)
+        r = s.iterate(4).as_dict()
+        s.close()
+        print(k, "batch", reps[k]["final_hpwl"], reps[k]["field_energy"][:2], "alone", r["final_hpwl"],
+              r["field_energy"][:2], flush=True)
+        assert abs(reps[k]["final_hpwl"] - r["final_hpwl"]) <= 1e-5 * r["final_hpwl"]
+        for a, b in zip(reps[k]["field_energy"][:2], r["field_energy"][:2]):
+            assert abs(a - b) <= 1e-4 * abs(b)
