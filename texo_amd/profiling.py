@@ -137,7 +137,7 @@ def main(argv):
     with open(bench_json) as f:
         line = json.loads([ln for ln in f.read().splitlines() if ln.startswith("{")][-1])
     bench_args = f"--only-batched --no-serving --no-cpu --batch {batch}"
-    stats_cmd = (f"rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -o run -- "
+    stats_cmd = (f"rocprofv3 --kernel-trace --stats --output-format csv -d {stats_dir} -o run -- "
                  f"python bench.py {bench_args} --steps 300 --warmup 20")
     md = stats_markdown(stats_dir, line, f"Kernel statistics: ecp5_85k, {batch} multi-start jobs in lockstep, "
                         f"{line['config'].get('precision')} transform", stats_cmd)
@@ -147,10 +147,10 @@ def main(argv):
     regex = "k_wa|k_deposit|k_sample|k_gradient|k_convert|k_optimizer|k_gemm"
     pmc = lambda c, d: (f"rocprofv3 --pmc {c} --kernel-include-regex \"{regex}\" --output-format csv -d {d} -o run -- "
                         f"python bench.py {bench_args} --steps 20 --warmup 5")
-    cal = [f"rocprofv3 --pmc {c} --output-format csv -d gpurun_out/cal_{c.lower()} -o run -- "
-           f"python bench.py --copy-calibration {copy_bytes >> 20}" for c in ("FETCH_SIZE", "WRITE_SIZE")]
+    cal = [f"rocprofv3 --pmc {c} --output-format csv -d {d} -o run -- "
+           f"python bench.py --copy-calibration {copy_bytes >> 20}" for c, d in (("FETCH_SIZE", cal_f), ("WRITE_SIZE", cal_w))]
     with open(os.path.join(out_dir, f"traffic_ecp5_85k_batch{batch}.md"), "w") as f:
-        f.write(traffic_markdown(t, pmc("FETCH_SIZE", "gpurun_out/pmc_fetch"), pmc("WRITE_SIZE", "gpurun_out/pmc_write"),
+        f.write(traffic_markdown(t, pmc("FETCH_SIZE", fetch_dir), pmc("WRITE_SIZE", write_dir),
                                  cal))
     with open(os.path.join(out_dir, "traffic.json"), "w") as f:
         json.dump(t, f, indent=1)
