@@ -75,11 +75,16 @@ def test_requests_in_a_mix_are_bit_identical_to_alone():
             assert fixed_on_sites(p, s.design, t)
 
 
+@functools.lru_cache(maxsize=None)
+def resident():
+    # the incremental stream's design: 250k cells on 512x512 bins, 300k nets
+    return synth.EcoStream(synth.eco_design())
+
+
 def test_request_on_the_resident_250k_design():
-    # the incremental stream's design: 250k cells on 512x512 bins, 300k nets;
     # one request of 1-5% of the units for 100 iterations, BF16x3 transform
-    d = synth.eco_design()
-    r = synth.EcoStream(d).request(11)
+    d = resident().design
+    r = resident().request(11)
     print(f"eco 250k request: {r.meta['movable_units']} movable units, {r.fixed_occupancy_kind.size} fixed sites",
           flush=True)
     t0 = time.perf_counter()
@@ -92,3 +97,25 @@ def test_request_on_the_resident_250k_design():
     assert np.all(t[keep] >= np.floor(b[:, :2])) and np.all(t[keep] <= np.ceil(b[:, 2:]))
     start = r.unit_initial_xy.reshape(-1, 2)[keep]
     assert t[keep].astype(float).std(axis=0).min() > start.std(axis=0).max()
+
+
+def test_requests_on_512_bins_are_bit_identical_batched_and_alone():
+    # on 512x512 bins a request's transform has 64 wide tiles per field, so
+    # a server round over several requests runs the wide split-MFMA kernels
+    # (K = 512) where a request alone runs the 32x32 ones in most rounds; both
+    # keep one summation order, so the results agree bit for bit
+    s = resident()
+    jobs = [(s.request(31), abi.DCT_BF16X3), (s.request(32), abi.DCT_BF16X3), (s.request(33), abi.DCT_FP16_SCALED),
+            (s.request(34), abi.DCT_BF16X3)]
+    iters = 30
+    print("eco 512 mix alone", [(p.meta["movable_units"], prec) for p, prec in jobs], flush=True)
+    ref = [texo_amd.place(p, dct_precision=prec, fixed_iterations=iters) for p, prec in jobs]
+    with texo_amd.Server() as srv:
+        ids = [srv.submit(p, abi.options(dct_precision=prec, fixed_iterations=iters)) for p, prec in jobs]
+        got = [srv.wait(i) for i in ids]
+    for (ta, ra), (tb, rb) in zip(ref, got):
+        print(ra["final_hpwl"], rb["final_hpwl"], ra["field_energy"][:2], rb["field_energy"][:2], flush=True)
+        assert np.array_equal(ta, tb)
+        for k in ("iterations", "evaluations", "line_search_trials", "final_hpwl", "final_wirelength",
+                  "field_overflow", "field_energy"):
+            assert ra[k] == rb[k], k

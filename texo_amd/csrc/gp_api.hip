@@ -273,7 +273,8 @@ int texo_gp_session_poisson(texo_gp_session *s, const double *density, double *p
     texo::DctLaunch d{};
     d.precision = (int)J.opt.precision;
     d.fast = texo::poisson_fast_shape(W, H);
-    d.wide_ok = d.precision == TEXO_GP_DCT_FP16_SCALED && d.fast && texo::poisson_wide_shape(W, H);
+    d.wide_ok = d.fast && texo::poisson_wide_ok(d.precision, W, H);
+    d.kmax = std::max(W, H);
     d.n_tasks = 1;
     const int tiles[4] = {((F * H + 31) / 32) * ((W + 31) / 32), ((F * W + 31) / 32) * ((H + 31) / 32),
                           d.wide_ok ? (F * H / 32) * (W / 128) : 0, d.wide_ok ? (F * W / 32) * (H / 128) : 0};

@@ -396,13 +396,14 @@ struct DctLaunch {
   BlockPlan h, w, h_wide, w_wide;
   int precision;
   bool fast;     // every task's W and H are multiples of 32
-  bool wide_ok;  // ... and of 128, at most 256
+  bool wide_ok;  // split-MFMA precision, and W and H multiples of 128, at most 512
+  int kmax;      // largest W or H of the launch's tasks
   int n_tasks;   // tasks with tiles in this launch
 };
 void launch_poisson(const Task *tasks, const DctLaunch &d, hipStream_t s);
 int poisson_energy_blocks(int W, int H, int F);  // per task
 bool poisson_fast_shape(int W, int H);
-bool poisson_wide_shape(int W, int H);
+bool poisson_wide_ok(int precision, int W, int H);
 
 // Kernel timing hook: the engine brackets launches of one named kernel.
 struct KernelTimer;

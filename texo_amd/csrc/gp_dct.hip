@@ -182,17 +182,14 @@ __device__ __forceinline__ void write_energy(const Stage &e, int local_rb, int b
 constexpr int SPLIT = 4;
 constexpr int FAST_THREADS = 64 * SPLIT;
 
-// Sums the per-wave partial tiles red[w][32][33] and runs the epilogue:
-// thread t owns column t % 32 and rows 4 (t / 32) .. 4 (t / 32) + 3.
-__device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][33], int m0, int n0, int local_rb,
-                                              int bx, int gx) {
+// Epilogue of one 32x32 output tile over the whole block: thread t owns
+// column t % 32 and rows 4 (t / 32) .. 4 (t / 32) + 3, whose summed values
+// it holds in `v`.  Every summation order (energy, range) is fixed by the
+// tile alone, so any kernel that hands it the same `v` writes the same bits.
+// Block-wide, with a barrier before its partials are read.
+__device__ __forceinline__ void tile_epilogue(const Stage &e, float4_t v, int m0, int n0, int local_rb, int bx,
+                                              int gx) {
   const int col = threadIdx.x & 31, rg = threadIdx.x >> 5;
-  float4_t v;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int row = 4 * rg + q;
-    v[q] = ((red[0][row][col] + red[1][row][col]) + red[2][row][col]) + red[3][row][col];
-  }
   double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
   store4(e, m0 + 4 * rg, n0 + col, v, energy);
   if (e.amax_out) {
@@ -219,6 +216,20 @@ __device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][3
     const int k = threadIdx.x;
     e.energy[((size_t)local_rb * gx + bx) * MAXF + k] = ((se[0][k] + se[1][k]) + se[2][k]) + se[3][k];
   }
+}
+
+// Sums the per-wave partial tiles red[w][32][33] in wave order and runs the
+// tile epilogue.
+__device__ __forceinline__ void fast_epilogue(const Stage &e, float (*red)[32][33], int m0, int n0, int local_rb,
+                                              int bx, int gx) {
+  const int col = threadIdx.x & 31, rg = threadIdx.x >> 5;
+  float4_t v;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = 4 * rg + q;
+    v[q] = ((red[0][row][col] + red[1][row][col]) + red[2][row][col]) + red[3][row][col];
+  }
+  tile_epilogue(e, v, m0, n0, local_rb, bx, gx);
 }
 
 // max |input| (float bits) of field f of stage S's input, from the partials
@@ -396,23 +407,57 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
   fast_epilogue(e, red, m0, n0, rb, bx, gx);
 }
 
-// FP16-scaled split MFMA over 32 rows x 128 columns per block, for K <=
-// WIDE_KMAX: the block splits its 32 rows of A (scaled by 2^s) into FP16
+// Split MFMA (BF16x3, P = 1, or FP16-scaled, P = 2) over 32 rows x 128
+// columns per block, for K <= KMAX: the block splits its 32 rows of A into
 // hi + lo once, into LDS, and each wave owns one 32x32 column tile over the
-// whole K, its B fragments coming from the pre-split basis in L2.  The
-// epilogue stores each wave's tile and writes the per-32x32-tile energy and
-// range partials of the 32-wide kernels.
-constexpr int WIDE_KMAX = 256;
+// whole K, its B fragments coming from the pre-split basis in L2.  The wave
+// accumulates each class of 16-wide k-steps modulo 4 (the k-steps that wave
+// c of the 32x32 kernel takes) on its own, in the same order, and sums the
+// scaled class results in class order; its epilogue then replays the 32x32
+// kernel's, the four 64-lane quarters of that kernel's block one after the
+// other: the stored values, energy and range partials are bit for bit those
+// of the 32x32 kernel, so a task's results do not depend on which of the two
+// kernels its launch picked.
 // Below this many wide tiles (one per CU) a launch leaves CUs idle and the
 // 32x32 split-K kernel is faster; from one per CU the wide kernel wins, also
 // beside the other job group's concurrent launches.
 constexpr int WIDE_MIN_TILES = 256;
-constexpr int WIDE_KP = WIDE_KMAX + 8;  // LDS row stride (halves)
+constexpr int WIDE_KMAX = 512;
 
-template <int S>
-__global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_wide(const Task *__restrict__ tasks,
-                                                                 BlockPlan p) {
-  __shared__ _Float16 Ahi[32][WIDE_KP], Alo[32][WIDE_KP];
+template <int KMAX>
+union WideLds {
+  struct {
+    unsigned short hi[32][KMAX + 8], lo[32][KMAX + 8];  // row stride KMAX + 8 halves
+  } a;
+  float red[SPLIT][32][33];
+};
+
+template <int P>
+__device__ __forceinline__ void split_pair(float x, unsigned short &hi, unsigned short &lo) {
+  if constexpr (P == 1) {
+    hi = bf16_rne(x);
+    lo = bf16_rne(x - bf16_to_float(hi));
+  } else {
+    _Float16 h, l;
+    split_f16(x, h, l);
+    hi = __builtin_bit_cast(unsigned short, h);
+    lo = __builtin_bit_cast(unsigned short, l);
+  }
+}
+
+template <int P>
+__device__ __forceinline__ float4_t mfma16(uint2 a, uint2 b, float4_t c) {
+  if constexpr (P == 1)
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a),
+                                                     __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4_t, a), __builtin_bit_cast(half4_t, b),
+                                                 c, 0, 0, 0);
+}
+
+template <int S, int P, int KMAX>
+__global__ void __launch_bounds__(FAST_THREADS) k_gemm_wide(const Task *__restrict__ tasks, BlockPlan p) {
+  __shared__ WideLds<KMAX> lds;
   int2 bm;
   if (!plan_block(p, bm)) return;
   const Task &t = tasks[bm.x];
@@ -423,77 +468,100 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_wide(const Task *__r
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = rb * 32, n0 = (bw * SPLIT + wave) * 32;
   const int q = lane >> 4, r = lane & 15;
-  const float sa = pow2_scale(__uint_as_float(input_amax<S>(t, m0 / e.R)));
-  const float unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
+  float sa = 1.f, unscale = 1.f;
+  if constexpr (P == 2) {
+    // max |A| over the 32 rows' field, from the partials of the producer of A
+    sa = pow2_scale(__uint_as_float(input_amax<S>(t, m0 / e.R)));
+    unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
+  }
   // stage A: 32 rows x K, coalesced float4 loads
   const int k4 = K / 4;
   for (int idx = threadIdx.x; idx < 32 * k4; idx += FAST_THREADS) {
     const int row = idx / k4, c = idx - row * k4;
-    const float4 v = *reinterpret_cast<const float4 *>(e.A + (size_t)(m0 + row) * K + 4 * c);
-    const float f4[4] = {v.x * sa, v.y * sa, v.z * sa, v.w * sa};
-    half4_t hi, lo;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      _Float16 h, l;
-      split_f16(f4[u], h, l);
-      hi[u] = h;
-      lo[u] = l;
-    }
-    *reinterpret_cast<half4_t *>(&Ahi[row][4 * c]) = hi;
-    *reinterpret_cast<half4_t *>(&Alo[row][4 * c]) = lo;
+    float4 v = *reinterpret_cast<const float4 *>(e.A + (size_t)(m0 + row) * K + 4 * c);
+    if constexpr (P == 2) v = make_float4(v.x * sa, v.y * sa, v.z * sa, v.w * sa);
+    ushort4 hi, lo;
+    split_pair<P>(v.x, hi.x, lo.x);
+    split_pair<P>(v.y, hi.y, lo.y);
+    split_pair<P>(v.z, hi.z, lo.z);
+    split_pair<P>(v.w, hi.w, lo.w);
+    *reinterpret_cast<ushort4 *>(&lds.a.hi[row][4 * c]) = hi;
+    *reinterpret_cast<ushort4 *>(&lds.a.lo[row][4 * c]) = lo;
   }
   __syncthreads();
-  const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.fthi[S]);
-  const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.ftlo[S]);
+  const uint2 *Bth = reinterpret_cast<const uint2 *>(P == 1 ? t.j.bthi[S] : t.j.fthi[S]);
+  const uint2 *Btl = reinterpret_cast<const uint2 *>(P == 1 ? t.j.btlo[S] : t.j.ftlo[S]);
   const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;
   const uint2 *bl0 = Btl + (size_t)(n0 + r) * k4 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k4 + q;
-  float4_t acc[2][2];
+  float4_t v[2][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    const half4_t bhi[2] = {half4_of(bh0[k0 / 4]), half4_of(bh1[k0 / 4])};
-    const half4_t blo[2] = {half4_of(bl0[k0 / 4]), half4_of(bl1[k0 / 4])};
-    half4_t ahi[2], alo[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      ahi[i] = *reinterpret_cast<const half4_t *>(&Ahi[16 * i + r][k0 + 4 * q]);
-      alo[i] = *reinterpret_cast<const half4_t *>(&Alo[16 * i + r][k0 + 4 * q]);
-    }
+  for (int c = 0; c < SPLIT; ++c) {
+    float4_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], blo[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo[i], bhi[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int k = 16 * c; k < K; k += 16 * SPLIT) {
+      const uint2 bhi[2] = {bh0[k / 4], bh1[k / 4]};
+      const uint2 blo[2] = {bl0[k / 4], bl1[k / 4]};
+      uint2 ahi[2], alo[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ahi[i] = *reinterpret_cast<const uint2 *>(&lds.a.hi[16 * i + r][k + 4 * q]);
+        alo[i] = *reinterpret_cast<const uint2 *>(&lds.a.lo[16 * i + r][k + 4 * q]);
       }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = mfma16<P>(ahi[i], bhi[j], acc[i][j]);
+          acc[i][j] = mfma16<P>(ahi[i], blo[j], acc[i][j]);
+          acc[i][j] = mfma16<P>(alo[i], bhi[j], acc[i][j]);
+        }
+    }
+    // the 32x32 kernel sums its waves' scaled partials in wave order
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) v[i][j] = c == 0 ? acc[i][j] * unscale : v[i][j] + acc[i][j] * unscale;
   }
+  __syncthreads();  // every wave is done with A in LDS
   // accumulator lane l holds D[4 (l / 16) + u][l % 16] of each 16x16 tile
-  double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
-  float mx = 0.f;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      float4_t v = acc[i][j] * unscale;
-      store4(e, m0 + 16 * i + 4 * q, n0 + 16 * j + r, v, energy);
-      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
-    }
-  const int gx32 = e.N / 32, tile = rb * gx32 + n0 / 32;
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) lds.red[wave][16 * i + 4 * q + u][16 * j + r] = v[i][j][u];
+  __syncthreads();
+  // quarter h of the 32x32 kernel's block: its thread 64 h + lane owns
+  // column lane % 32 and rows 4 (2 h + lane / 32) .. + 3
+  const int gx = e.N / 32, bx = n0 / 32, col = lane & 31;
+  double se[SPLIT][MAXF];
+  float mx = 0.f;
+#pragma unroll
+  for (int h = 0; h < SPLIT; ++h) {
+    const int row0 = 4 * (2 * h + (lane >> 5));
+    float4_t w;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = lds.red[wave][row0 + u][col];
+    double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
+    store4(e, m0 + row0, n0 + col, w, energy);
+    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fmaxf(fabsf(w[2]), fabsf(w[3]))));
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) se[h][k] = e.energy ? wave_sum(energy[k]) : 0.0;
+  }
   if (e.amax_out) {
     mx = wave_max_f(mx);
-    if (lane == 0) e.amax_out[tile] = __float_as_uint(mx);
+    if (lane == 0) e.amax_out[(size_t)rb * gx + bx] = __float_as_uint(mx);
   }
-  if (e.energy) {
+  if (e.energy && lane < MAXF) {
+    double tot = 0.0;
 #pragma unroll
-    for (int k = 0; k < MAXF; ++k) energy[k] = wave_sum(energy[k]);
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < MAXF; ++k) e.energy[(size_t)tile * MAXF + k] = energy[k];
-    }
+    for (int k = 0; k < MAXF; ++k)
+      if (k == lane) tot = ((se[0][k] + se[1][k]) + se[2][k]) + se[3][k];
+    e.energy[((size_t)rb * gx + bx) * MAXF + lane] = tot;
   }
 }
 
@@ -739,9 +807,11 @@ __global__ void __launch_bounds__(64) k_gemm_f32_any(const Task *__restrict__ ta
 }  // namespace
 
 bool poisson_fast_shape(int W, int H) { return W % 32 == 0 && H % 32 == 0; }
-// both extents multiples of the wide kernel's 128 columns and within its LDS rows
-bool poisson_wide_shape(int W, int H) {
-  return W % (32 * SPLIT) == 0 && H % (32 * SPLIT) == 0 && W <= WIDE_KMAX && H <= WIDE_KMAX;
+// split-MFMA precision with both extents multiples of the wide kernel's 128
+// columns and within its LDS rows
+bool poisson_wide_ok(int precision, int W, int H) {
+  return (precision == 1 || precision == 2) && W % (32 * SPLIT) == 0 && H % (32 * SPLIT) == 0 &&
+         W <= WIDE_KMAX && H <= WIDE_KMAX;
 }
 
 int poisson_energy_blocks(int W, int H, int F) {
@@ -749,20 +819,32 @@ int poisson_energy_blocks(int W, int H, int F) {
   return ((F * W + 31) / 32) * ((H + 31) / 32);
 }
 
+template <int S, int P>
+void launch_wide(const Task *tasks, const BlockPlan &p, int kmax, hipStream_t s) {
+  const dim3 grid(plan_grid(p));
+  if (kmax <= 256)
+    hipLaunchKernelGGL((k_gemm_wide<S, P, 256>), grid, dim3(FAST_THREADS), 0, s, tasks, p);
+  else
+    hipLaunchKernelGGL((k_gemm_wide<S, P, WIDE_KMAX>), grid, dim3(FAST_THREADS), 0, s, tasks, p);
+}
+
 template <int S>
 void launch_stage(const Task *tasks, const DctLaunch &d, bool wide, hipStream_t s) {
   constexpr bool h_rows = S == 0 || S == 3;  // stages with F*H rows
   const BlockPlan &p = wide ? (h_rows ? d.h_wide : d.w_wide) : (h_rows ? d.h : d.w);
   const dim3 grid(plan_grid(p));
-  if (d.precision == 1) {
+  if (wide) {
+    if (d.precision == 1)
+      launch_wide<S, 1>(tasks, p, d.kmax, s);
+    else
+      launch_wide<S, 2>(tasks, p, d.kmax, s);
+  } else if (d.precision == 1) {
     if (d.fast)
       hipLaunchKernelGGL(k_gemm_bf16_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, p);
     else
       hipLaunchKernelGGL(k_gemm_bf16_any<S>, grid, dim3(64), 0, s, tasks, p);
   } else if (d.precision == 2) {
-    if (wide)
-      hipLaunchKernelGGL(k_gemm_f16s_wide<S>, grid, dim3(FAST_THREADS), 0, s, tasks, p);
-    else if (d.fast)
+    if (d.fast)
       hipLaunchKernelGGL(k_gemm_f16s_fast<S>, grid, dim3(FAST_THREADS), 0, s, tasks, p);
     else
       hipLaunchKernelGGL(k_gemm_f16s_any<S>, grid, dim3(64), 0, s, tasks, p);
@@ -777,7 +859,7 @@ void launch_stage(const Task *tasks, const DctLaunch &d, bool wide, hipStream_t 
 void launch_poisson(const Task *tasks, const DctLaunch &d, hipStream_t s) {
   if (d.h.total <= 0) return;
   timer_begin(g_timer, "dct", s);
-  const bool wide = d.precision == 2 && d.wide_ok && d.h_wide.total >= WIDE_MIN_TILES;
+  const bool wide = d.wide_ok && d.h_wide.total >= WIDE_MIN_TILES;
   launch_stage<0>(tasks, d, wide, s);
   launch_stage<1>(tasks, d, wide, s);
   launch_stage<2>(tasks, d, wide, s);

@@ -111,7 +111,7 @@ struct Plan {
 // shape class), one plan entry per task of the round.
 struct DctGroup {
   Plan h, w, h_wide, w_wide;
-  int precision = 0, n_tasks = 0;
+  int precision = 0, n_tasks = 0, kmax = 0;
   bool fast = false, wide_ok = false;
   void pad(size_t n_tasks) {
     for (Plan *p : {&h, &w, &h_wide, &w_wide})
@@ -121,6 +121,7 @@ struct DctGroup {
     pad(task);
     n_tasks += 1;
     const size_t F = (size_t)j.F, W = j.W, H = j.H;
+    kmax = std::max(kmax, (int)std::max(W, H));
     h.add(((F * H + 31) / 32) * ((W + 31) / 32), 1);
     w.add(((F * W + 31) / 32) * ((H + 31) / 32), 1);
     if (wide_ok) {
@@ -140,6 +141,7 @@ struct DctGroup {
     d.precision = precision;
     d.fast = fast;
     d.wide_ok = wide_ok;
+    d.kmax = kmax;
     d.n_tasks = n_tasks;
     return d;
   }
@@ -147,7 +149,7 @@ struct DctGroup {
 
 DctGroup &dct_group(std::map<std::tuple<int, bool, bool>, DctGroup> &groups, const Job &j) {
   const bool fast = poisson_fast_shape((int)j.W, (int)j.H);
-  const bool wide = j.opt.precision == TEXO_GP_DCT_FP16_SCALED && fast && poisson_wide_shape((int)j.W, (int)j.H);
+  const bool wide = fast && poisson_wide_ok((int)j.opt.precision, (int)j.W, (int)j.H);
   DctGroup &g = groups[std::make_tuple((int)j.opt.precision, fast, wide)];
   g.precision = (int)j.opt.precision;
   g.fast = fast;
