@@ -138,7 +138,7 @@ def main(argv):
         line = json.loads([ln for ln in f.read().splitlines() if ln.startswith("{")][-1])
     bench_args = f"--only-batched --no-serving --no-cpu --batch {batch}"
     stats_cmd = (f"rocprofv3 --kernel-trace --stats --output-format csv -d {stats_dir} -o run -- "
-                 f"python bench.py {bench_args} --steps 300 --warmup 20")
+                 f"python bench.py {bench_args} --steps 1000 --warmup 20")
     md = stats_markdown(stats_dir, line, f"Kernel statistics: ecp5_85k, {batch} multi-start jobs in lockstep, "
                         f"{line['config'].get('precision')} transform", stats_cmd)
     with open(os.path.join(out_dir, f"kernels_ecp5_85k_batch{batch}.md"), "w") as f:
