@@ -66,6 +66,33 @@ def load():
     lib.oracle_clustering_areas.argtypes = [u64, P(u32), P(d), u64, P(u32), P(u64), P(u64), u64, P(d)]
     lib.oracle_adjust_instance_areas.argtypes = [u64, C.c_void_p, u64, P(u32), P(d), P(d), P(d),
                                                  P(C.c_int)]
+    i64, u8, i = C.c_int64, C.c_uint8, C.c_int
+    lib.oracle_density_fillers.argtypes = [u32, u32, u32, P(u32), P(u32), u32, P(u32), C.c_void_p, P(d), u64, P(u32),
+                                           P(d), P(d)]
+    lib.oracle_density_fillers.restype = i64
+    lib.oracle_density_evaluate.argtypes = [u32, u32, u32, P(u32), P(u32), u32, P(u32), C.c_void_p, P(d), u64, P(u32),
+                                            P(d), P(d), u32, C.c_void_p, P(d), P(d)]
+    lib.oracle_density_evaluate.restype = i64
+    for name in ("density_is_converged", "area_adjustment_is_ready"):
+        getattr(lib, "oracle_" + name).argtypes = [u32, P(u32), P(d)]
+    lib.oracle_density_kind_is_exchangeable.argtypes = [u32]
+    lib.oracle_routability_adjusted_member_area.argtypes = [u32, d, d, d]
+    lib.oracle_routability_adjusted_member_area.restype = d
+    lib.oracle_area_adjusted_multiplier_step.argtypes = [u32, P(d)]
+    lib.oracle_area_adjusted_multiplier_step.restype = d
+    lib.oracle_area_adjusted_multipliers.argtypes = [u32, P(d), P(d), P(d), u64, P(d), P(d), P(d)]
+    lib.oracle_stationary_target_is_fixed.argtypes = [P(i), P(u32), i, P(u32), u64]
+    lib.oracle_stop_without_convergence.argtypes = [i, u64, u32, P(u32), P(d), C.c_void_p]
+    lib.oracle_iteration_limit_reached.argtypes = [u64]
+    lib.oracle_checkpoint_choice.argtypes = [u32, u32, P(u32), P(d)]
+    lib.oracle_checkpoint_choice.restype = u32
+    lib.oracle_session_gamma_weights.argtypes = [C.c_void_p, P(d)]
+    lib.oracle_net_bounding_boxes.argtypes = [u32, u32, P(d), P(u8), P(d), P(d), u32, P(u32), P(u32), P(u32), P(d),
+                                              P(u64)]
+    lib.oracle_nesterov_step_lengths.argtypes = [C.POINTER(NesterovState), u64, u64, P(d), u64, P(d), u64, P(d),
+                                                 EVALUATE_FN, C.c_void_p, C.POINTER(NesterovResult)]
+    lib.oracle_nesterov_step.argtypes = [C.POINTER(NesterovState), u64, P(d), P(d), P(d), EVALUATE_FN, C.c_void_p,
+                                         C.POINTER(NesterovResult)]
     _lib = lib
     return lib
 
@@ -90,3 +117,13 @@ class NesterovResult(C.Structure):
 
 EVALUATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_double),
                           C.POINTER(C.c_double))
+
+
+class DensityMember(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("offset_x", C.c_double), ("offset_y", C.c_double), ("charge", C.c_double)]
+
+
+class FieldResult(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("available", C.c_uint64), ("real_charge", C.c_uint64),
+                ("real_area", C.c_double), ("filler_charge", C.c_double), ("energy", C.c_double),
+                ("overflow", C.c_double), ("net_charge", C.c_double), ("force_l1", C.c_double)]

@@ -1764,9 +1764,17 @@ std::vector<uint32_t> rounded_targets(const std::vector<double> &coords, size_t 
   return t;
 }
 
+// PlacementCheckpoint (eplace.rs:89-120): the lowest (maximum excess, total
+// excess) pair in total_cmp order; ties keep the earlier checkpoint.
 struct Checkpoint {
   std::vector<uint32_t> targets;
-  double max_excess, total_excess;
+  double max_excess = 0.0, total_excess = 0.0;
+  Checkpoint() = default;
+  Checkpoint(std::vector<uint32_t> t, const std::vector<FieldResult> &fields) : targets(std::move(t)) {
+    auto s = overflow_excess_score(fields);
+    max_excess = s.first;
+    total_excess = s.second;
+  }
   void consider(std::vector<uint32_t> t, const std::vector<FieldResult> &fields) {
     auto s = overflow_excess_score(fields);
     bool better = s.first < max_excess || (s.first == max_excess && s.second < total_excess);
@@ -1777,6 +1785,27 @@ struct Checkpoint {
     }
   }
 };
+
+// stationary_target_is_fixed (eplace.rs:973-985): a NumericallyStationary
+// step whose rounded targets equal the previous stationary step's.
+struct StationaryTrack {
+  bool has = false;
+  std::vector<uint32_t> previous;
+};
+bool stationary_target_is_fixed(StationaryTrack &t, bool accepted, const std::vector<uint32_t> &targets) {
+  if (accepted) {
+    t.has = false;
+    t.previous.clear();
+    return false;
+  }
+  const bool fixed = t.has && t.previous == targets;
+  t.previous = targets;
+  t.has = true;
+  return fixed;
+}
+
+// eplace_iteration_limit_reached (eplace.rs:1011-1013)
+bool iteration_limit_reached(uint64_t completed) { return completed >= MAX_EPLACE_ITERATIONS; }
 
 struct RunStats {
   uint64_t iterations = 0, evaluations = 0, trials = 0, area_rounds = 0;
@@ -1798,6 +1827,12 @@ struct OptResult {
   if (!fields.empty()) f.v0 = fields[0].overflow;
   if (fields.size() > 1) f.v1 = fields[1].overflow;
   throw f;
+}
+
+// stop_without_convergence (eplace.rs:989-999): best effort keeps the
+// checkpoint, a strict solve reports DidNotConverge with the overflows.
+void stop_without_convergence(bool best_effort, uint64_t iterations, const std::vector<FieldResult> &fields) {
+  if (!best_effort) did_not_converge(iterations, fields);
 }
 
 struct Options {
@@ -1834,11 +1869,7 @@ OptResult optimize_density_once(const Problem &P, const std::vector<double> &coo
   NState st;
   st.x = project(coords, bounds);
   st.xp = st.x;
-  Checkpoint best;
-  best.targets = rounded_targets(st.x, P.n_units, P.W, P.H);
-  auto s0 = overflow_excess_score(initial.fields);
-  best.max_excess = s0.first;
-  best.total_excess = s0.second;
+  Checkpoint best(rounded_targets(st.x, P.n_units, P.W, P.H), initial.fields);
   OptResult out;
   bool fixed_mode = opt.fixed_iterations > 0;
   if (!fixed_mode && density_stop_reached(initial.fields, stop)) {
@@ -1860,8 +1891,7 @@ OptResult optimize_density_once(const Problem &P, const std::vector<double> &coo
     mstep = MULTIPLIER_ALPHA_HIGH - 1.0;
   }
   Evaluation current = std::move(initial);
-  std::vector<uint32_t> prev_stationary;
-  bool has_prev_stationary = false;
+  StationaryTrack stationary;
   uint64_t local_iterations = 0;
   for (;;) {
     if (fixed_mode) {
@@ -1869,7 +1899,7 @@ OptResult optimize_density_once(const Problem &P, const std::vector<double> &coo
     } else {
       if (density_stop_reached(current.fields, stop)) break;
       if (stats.iterations >= opt.max_iterations) {
-        if (!P.best_effort) did_not_converge(stats.iterations, current.fields);
+        stop_without_convergence(P.best_effort, stats.iterations, current.fields);
         out.exhausted = true;
         break;
       }
@@ -1900,17 +1930,8 @@ OptResult optimize_density_once(const Problem &P, const std::vector<double> &coo
     best.consider(rt, current.fields);
     if (!fixed_mode) {
       if (density_stop_reached(current.fields, stop)) break;
-      // stationary_target_is_fixed (eplace.rs:973-985)
-      bool fixed_target = false;
-      if (step.accepted) {
-        has_prev_stationary = false;
-      } else {
-        fixed_target = has_prev_stationary && prev_stationary == rt;
-        prev_stationary = rt;
-        has_prev_stationary = true;
-      }
-      if (fixed_target) {
-        if (!P.best_effort) did_not_converge(stats.iterations, current.fields);
+      if (stationary_target_is_fixed(stationary, step.accepted, rt)) {
+        stop_without_convergence(P.best_effort, stats.iterations, current.fields);
         out.exhausted = true;
         break;
       }
@@ -2515,6 +2536,12 @@ int oracle_wirelength_preconditioner(uint32_t n_units, uint32_t n_nets, const ui
 int oracle_nesterov_step(oracle_nesterov_state *state, uint64_t n, const double *lower, const double *upper,
                          const double *metric, oracle_evaluate_fn evaluate_fn, void *context,
                          oracle_nesterov_result *result) {
+  return oracle_nesterov_step_lengths(state, n, n, lower, n, upper, n, metric, evaluate_fn, context, result);
+}
+
+int oracle_nesterov_step_lengths(oracle_nesterov_state *state, uint64_t n, uint64_t n_lower, const double *lower,
+                                 uint64_t n_upper, const double *upper, uint64_t n_metric, const double *metric,
+                                 oracle_evaluate_fn evaluate_fn, void *context, oracle_nesterov_result *result) {
   std::memset(result, 0, sizeof(*result));
   NState st;
   st.x.assign(state->coordinates, state->coordinates + n);
@@ -2523,9 +2550,9 @@ int oracle_nesterov_step(oracle_nesterov_state *state, uint64_t n, const double 
   st.has_l = state->has_lipschitz != 0;
   st.l = state->lipschitz;
   Bounds b;
-  b.lower.assign(lower, lower + n);
-  b.upper.assign(upper, upper + n);
-  std::vector<double> m(metric, metric + n);
+  b.lower.assign(lower, lower + n_lower);
+  b.upper.assign(upper, upper + n_upper);
+  std::vector<double> m(metric, metric + n_metric);
   uint64_t evaluations = 0;
   try {
     NStep s = dynamic_nesterov_step(st, b, m, [&](const std::vector<double> &x, size_t &slot) {
@@ -2657,5 +2684,199 @@ int oracle_clustering_areas(uint64_t n_registers, const uint32_t *register_cell,
 }
 
 double oracle_soft_division_ceiling(double value, double divisor) { return soft_division_ceiling(value, divisor); }
+
+
+// ---- density model with explicit charges --------------------------------
+namespace {
+DensityModel fixture_model(uint32_t width, uint32_t height, uint32_t n_kinds, const uint32_t *kinds,
+                           const uint32_t *capacity) {
+  std::map<uint32_t, std::vector<uint32_t>> raw;
+  const size_t G = (size_t)width * height;
+  for (uint32_t k = 0; k < n_kinds; ++k) raw[kinds[k]] = std::vector<uint32_t>(capacity + k * G, capacity + (k + 1) * G);
+  return make_density_model(width, height, raw, {});
+}
+std::vector<DensityUnit> fixture_units(uint32_t n_units, const uint32_t *start, const oracle_density_member *members) {
+  std::vector<DensityUnit> units(n_units);
+  for (uint32_t u = 0; u < n_units; ++u)
+    for (uint32_t m = start[u]; m < start[u + 1]; ++m)
+      units[u].members.push_back({members[m].kind, members[m].offset_x, members[m].offset_y, members[m].charge, m});
+  return units;
+}
+}  // namespace
+
+int64_t oracle_density_fillers(uint32_t width, uint32_t height, uint32_t n_kinds, const uint32_t *kinds,
+                               const uint32_t *capacity, uint32_t n_units, const uint32_t *unit_member_start,
+                               const oracle_density_member *members, const double *unit_xy, uint64_t max_fillers,
+                               uint32_t *filler_kind, double *filler_xy, double *filler_charge) {
+  try {
+    DensityModel model = fixture_model(width, height, n_kinds, kinds, capacity);
+    std::vector<DensityUnit> units = fixture_units(n_units, unit_member_start, members);
+    std::vector<Filler> f =
+        initial_fillers(model, units, [&](size_t i) { return std::make_pair(unit_xy[2 * i], unit_xy[2 * i + 1]); });
+    for (size_t i = 0; i < f.size() && i < max_fillers; ++i) {
+      filler_kind[i] = f[i].kind;
+      filler_xy[2 * i] = f[i].x;
+      filler_xy[2 * i + 1] = f[i].y;
+      filler_charge[i] = f[i].charge;
+    }
+    return (int64_t)f.size();
+  } catch (const Failure &e) {
+    return -(int64_t)e.status;
+  }
+}
+
+int64_t oracle_density_evaluate(uint32_t width, uint32_t height, uint32_t n_kinds, const uint32_t *kinds,
+                                const uint32_t *capacity, uint32_t n_units, const uint32_t *unit_member_start,
+                                const oracle_density_member *members, const double *unit_xy, uint64_t n_fillers,
+                                const uint32_t *filler_kind, const double *filler_xy, const double *filler_charge,
+                                uint32_t max_fields, oracle_field_result *fields, double *unit_gradients,
+                                double *filler_gradients) {
+  try {
+    DensityModel model = fixture_model(width, height, n_kinds, kinds, capacity);
+    std::vector<DensityUnit> units = fixture_units(n_units, unit_member_start, members);
+    std::vector<Filler> fl(n_fillers);
+    for (uint64_t i = 0; i < n_fillers; ++i) fl[i] = {filler_kind[i], filler_xy[2 * i], filler_xy[2 * i + 1], filler_charge[i]};
+    std::vector<FieldResult> r = evaluate_density(
+        model, units, fl, [&](size_t i) { return std::make_pair(unit_xy[2 * i], unit_xy[2 * i + 1]); },
+        [&](size_t i) { return std::make_pair(fl[i].x, fl[i].y); });
+    for (size_t f = 0; f < r.size() && f < max_fields; ++f) {
+      fields[f] = {r[f].kind,        (uint64_t)r[f].available, (uint64_t)r[f].real_charge, r[f].real_area,
+                   r[f].filler_charge, r[f].energy,             r[f].overflow,              r[f].net_charge,
+                   r[f].force_l1};
+      if (unit_gradients)
+        for (size_t u = 0; u < n_units; ++u) {
+          unit_gradients[2 * (f * n_units + u)] = r[f].unit_gradients[u].first;
+          unit_gradients[2 * (f * n_units + u) + 1] = r[f].unit_gradients[u].second;
+        }
+      if (filler_gradients)
+        for (auto &g : r[f].filler_gradients) {
+          filler_gradients[2 * g.index] = g.gx;
+          filler_gradients[2 * g.index + 1] = g.gy;
+        }
+    }
+    return (int64_t)r.size();
+  } catch (const Failure &e) {
+    return -(int64_t)e.status;
+  }
+}
+
+// ---- schedule and stop rules ---------------------------------------------
+namespace {
+std::vector<FieldResult> fixture_fields(uint32_t n, const uint32_t *kinds, const double *overflows) {
+  std::vector<FieldResult> f(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    f[i].kind = kinds[i];
+    f[i].overflow = overflows[i];
+  }
+  return f;
+}
+}  // namespace
+
+int oracle_density_kind_is_exchangeable(uint32_t kind) { return exchangeable(kind) ? 1 : 0; }
+
+int oracle_density_is_converged(uint32_t n, const uint32_t *kinds, const double *overflows) {
+  return density_is_converged(fixture_fields(n, kinds, overflows)) ? 1 : 0;
+}
+
+int oracle_area_adjustment_is_ready(uint32_t n, const uint32_t *kinds, const double *overflows) {
+  return area_adjustment_is_ready(fixture_fields(n, kinds, overflows)) ? 1 : 0;
+}
+
+double oracle_routability_adjusted_member_area(uint32_t kind, double area, double horizontal, double vertical) {
+  try {
+    return routability_adjusted_member_area(kind, area, horizontal, vertical);
+  } catch (const Failure &) {
+    return NAN;
+  }
+}
+
+double oracle_area_adjusted_multiplier_step(uint32_t n, const double *multipliers) {
+  try {
+    return area_adjusted_multiplier_step(std::vector<double>(multipliers, multipliers + n));
+  } catch (const Failure &) {
+    return NAN;
+  }
+}
+
+int oracle_area_adjusted_multipliers(uint32_t n_fields, const double *energy, const double *force_l1,
+                                     const double *initial_energies, uint64_t n_units, const double *wa_gradient_x,
+                                     const double *wa_gradient_y, double *multipliers) {
+  try {
+    Evaluation ev;
+    ev.wa.gx.assign(wa_gradient_x, wa_gradient_x + n_units);
+    ev.wa.gy.assign(wa_gradient_y, wa_gradient_y + n_units);
+    ev.fields.resize(n_fields);
+    for (uint32_t f = 0; f < n_fields; ++f) {
+      ev.fields[f].energy = energy[f];
+      ev.fields[f].force_l1 = force_l1[f];
+    }
+    std::vector<double> m =
+        area_adjusted_density_multipliers(ev, std::vector<double>(initial_energies, initial_energies + n_fields));
+    std::copy(m.begin(), m.end(), multipliers);
+    return 0;
+  } catch (const Failure &e) {
+    return e.status;
+  }
+}
+
+int oracle_stationary_target_is_fixed(int *has_previous, uint32_t *previous_xy, int accepted,
+                                      const uint32_t *targets_xy, uint64_t n_units) {
+  StationaryTrack t;
+  t.has = *has_previous != 0;
+  if (t.has) t.previous.assign(previous_xy, previous_xy + 2 * n_units);
+  const bool fixed = stationary_target_is_fixed(t, accepted != 0, std::vector<uint32_t>(targets_xy, targets_xy + 2 * n_units));
+  *has_previous = t.has ? 1 : 0;
+  if (t.has) std::copy(t.previous.begin(), t.previous.end(), previous_xy);
+  return fixed ? 1 : 0;
+}
+
+int oracle_stop_without_convergence(int best_effort, uint64_t iterations, uint32_t n, const uint32_t *kinds,
+                                    const double *overflows, texo_gp_report *report) {
+  try {
+    stop_without_convergence(best_effort != 0, iterations, fixture_fields(n, kinds, overflows));
+    return 0;
+  } catch (const Failure &f) {
+    fill_failure(report, f);
+    return f.status;
+  }
+}
+
+int oracle_iteration_limit_reached(uint64_t completed_iterations) {
+  return iteration_limit_reached(completed_iterations) ? 1 : 0;
+}
+
+uint32_t oracle_checkpoint_choice(uint32_t n_candidates, uint32_t n_fields, const uint32_t *kinds,
+                                  const double *overflows) {
+  if (n_candidates == 0) return 0;
+  Checkpoint c({0u}, fixture_fields(n_fields, kinds, overflows));
+  for (uint32_t i = 1; i < n_candidates; ++i) c.consider({i}, fixture_fields(n_fields, kinds, overflows + (size_t)i * n_fields));
+  return c.targets[0];
+}
+
+int oracle_session_gamma_weights(oracle_session *s, double *weights) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::set<uint32_t> kinds;
+  for (auto &du : s->P.density_units)
+    for (auto &m : du.members) kinds.insert(m.kind);
+  size_t f = 0;
+  for (uint32_t k : kinds) weights[f++] = s->P.gamma_weights.count(k) ? s->P.gamma_weights.at(k) : 0.0;
+  return 0;
+}
+
+int oracle_net_bounding_boxes(uint32_t n_units, uint32_t n_cells, const double *positions, const uint8_t *fixed_mask,
+                              const double *fixed_xy, const double *offsets, uint32_t n_nets,
+                              const uint32_t *net_pin_start, const uint32_t *pin_unit, const uint32_t *pin_cell,
+                              double *boxes, uint64_t *n_boxes) {
+  Hypergraph hg;
+  hg.net_start.assign(net_pin_start, net_pin_start + n_nets + 1);
+  for (uint32_t i = 0; i < (n_nets ? net_pin_start[n_nets] : 0); ++i) hg.pins.push_back({pin_unit[i], pin_cell[i]});
+  std::vector<uint8_t> fm(fixed_mask, fixed_mask + n_cells);
+  std::vector<double> fx(fixed_xy, fixed_xy + 2 * n_cells), of(offsets, offsets + 2 * n_cells);
+  auto b = net_bounding_boxes(hg, std::vector<double>(positions, positions + 2 * n_units), CellGeometry{&fm, &fx, &of});
+  for (size_t i = 0; i < b.size(); ++i)
+    for (int k = 0; k < 4; ++k) boxes[4 * i + k] = b[i][k];
+  *n_boxes = b.size();
+  return 0;
+}
 
 }  // extern "C"

@@ -165,6 +165,82 @@ int oracle_clustering_areas(uint64_t n_registers, const uint32_t *register_cell,
                             uint64_t movable_instance_count, double *areas);
 double oracle_soft_division_ceiling(double value, double divisor);
 
+/* ---- density model with explicit charges (test fixtures) --------------- */
+typedef struct oracle_density_member {
+  uint32_t kind;
+  double offset_x, offset_y, charge;
+} oracle_density_member;
+typedef struct oracle_field_result {
+  uint32_t kind;
+  uint64_t available, real_charge;
+  double real_area, filler_charge, energy, overflow, net_charge, force_l1;
+} oracle_field_result;
+/* DensityModel::new + initial_fillers (electrostatic_placement.rs:139-240)
+ * for units whose members carry their own charges; capacity is
+ * [n_kinds][W*H].  Returns the filler count (at most max_fillers written) or
+ * -(status). */
+int64_t oracle_density_fillers(uint32_t width, uint32_t height, uint32_t n_kinds, const uint32_t *kinds,
+                               const uint32_t *capacity, uint32_t n_units, const uint32_t *unit_member_start,
+                               const oracle_density_member *members, const double *unit_xy, uint64_t max_fillers,
+                               uint32_t *filler_kind, double *filler_xy, double *filler_charge);
+/* DensityModel::evaluate_with_positions (electrostatic_placement.rs:265-421)
+ * at unit origins unit_xy[2U] and filler positions filler_xy[2n].  Writes up
+ * to max_fields records, unit_gradients[F][U][2] and filler_gradients[n][2]
+ * (each filler in its own field; either may be NULL).  Returns the field
+ * count or -(status). */
+int64_t oracle_density_evaluate(uint32_t width, uint32_t height, uint32_t n_kinds, const uint32_t *kinds,
+                                const uint32_t *capacity, uint32_t n_units, const uint32_t *unit_member_start,
+                                const oracle_density_member *members, const double *unit_xy, uint64_t n_fillers,
+                                const uint32_t *filler_kind, const double *filler_xy, const double *filler_charge,
+                                uint32_t max_fields, oracle_field_result *fields, double *unit_gradients,
+                                double *filler_gradients);
+
+/* ---- schedule and stop rules (fields given as kind + overflow) ----------- */
+int oracle_density_kind_is_exchangeable(uint32_t kind);
+/* density_is_converged (eplace.rs:1397-1401) */
+int oracle_density_is_converged(uint32_t n, const uint32_t *kinds, const double *overflows);
+/* area_adjustment_is_ready (eplace.rs:1403-1408) */
+int oracle_area_adjustment_is_ready(uint32_t n, const uint32_t *kinds, const double *overflows);
+/* routability_adjusted_member_area (eplace.rs:741-758); NaN on error */
+double oracle_routability_adjusted_member_area(uint32_t kind, double area, double horizontal, double vertical);
+/* area_adjusted_multiplier_step (eplace.rs:1220-1232); NaN on error */
+double oracle_area_adjusted_multiplier_step(uint32_t n, const double *multipliers);
+/* area_adjusted_density_multipliers (eplace.rs:1183-1217) for fields with
+ * (energy, force_l1) against initial energies and a WA gradient of n_units
+ * entries per axis.  Returns 0 or a status. */
+int oracle_area_adjusted_multipliers(uint32_t n_fields, const double *energy, const double *force_l1,
+                                     const double *initial_energies, uint64_t n_units, const double *wa_gradient_x,
+                                     const double *wa_gradient_y, double *multipliers);
+/* stationary_target_is_fixed (eplace.rs:973-985) on a caller-held tracker
+ * (*has_previous, previous_xy[2n]). */
+int oracle_stationary_target_is_fixed(int *has_previous, uint32_t *previous_xy, int accepted,
+                                      const uint32_t *targets_xy, uint64_t n_units);
+/* stop_without_convergence (eplace.rs:989-999): 0, or DID_NOT_CONVERGE with
+ * iterations and the first two overflows in `report` (may be NULL). */
+int oracle_stop_without_convergence(int best_effort, uint64_t iterations, uint32_t n, const uint32_t *kinds,
+                                    const double *overflows, texo_gp_report *report);
+/* eplace_iteration_limit_reached (eplace.rs:1011-1013) */
+int oracle_iteration_limit_reached(uint64_t completed_iterations);
+/* PlacementCheckpoint (eplace.rs:89-120): candidate 0 makes the checkpoint,
+ * candidates 1.. are considered in order; returns the retained index. */
+uint32_t oracle_checkpoint_choice(uint32_t n_candidates, uint32_t n_fields, const uint32_t *kinds,
+                                  const double *overflows);
+/* Session-level schedule inputs: density_gamma_weights (eplace.rs:1424-1439)
+ * per field, in field order. */
+int oracle_session_gamma_weights(oracle_session *session, double *weights);
+/* external_baseline_net_bounding_boxes (analytical_placement.rs:238-276):
+ * boxes [4*k] (min x, max x, min y, max y) of the nets spanning two or more
+ * units, k returned in *n_boxes (at most n_nets written). */
+int oracle_net_bounding_boxes(uint32_t n_units, uint32_t n_cells, const double *positions, const uint8_t *fixed_mask,
+                              const double *fixed_xy, const double *offsets, uint32_t n_nets,
+                              const uint32_t *net_pin_start, const uint32_t *pin_unit, const uint32_t *pin_cell,
+                              double *boxes, uint64_t *n_boxes);
+/* dynamic_nesterov_step with bounds and metric of their own lengths (the
+ * reference validates them before any evaluation). */
+int oracle_nesterov_step_lengths(oracle_nesterov_state *state, uint64_t n, uint64_t n_lower, const double *lower,
+                                 uint64_t n_upper, const double *upper, uint64_t n_metric, const double *metric,
+                                 oracle_evaluate_fn evaluate, void *context, oracle_nesterov_result *result);
+
 #ifdef __cplusplus
 }
 #endif
