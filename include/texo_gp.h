@@ -325,6 +325,43 @@ int texo_gp_session_kernel_timing(texo_gp_session *session, double *avg_ms,
 int texo_gp_session_iterate(texo_gp_session *session, uint32_t iterations,
                             texo_gp_report *report);
 
+/* Combined objective (combined_objective, eplace.rs:1022-1044, with
+ * augmented_density_value_and_coefficient, eplace.rs:1111-1124) at
+ * `coordinates[C]` with explicit multipliers[F] and initial-energy scales[F]:
+ * the value and the interleaved gradient[C] the optimizer steps on. */
+int texo_gp_session_combined(texo_gp_session *session, const double *coordinates, double gamma,
+                             const double *multipliers, const double *scales, double *value,
+                             double *gradient);
+
+/* Diagonal preconditioner (diagonal_preconditioner, eplace.rs:1046-1100):
+ * wirelength incidence plus multiplier-weighted charges, floored at 1.0, for
+ * `multipliers[F]`; diagonal[C]. */
+int texo_gp_session_preconditioner(texo_gp_session *session, const double *multipliers,
+                                   double *diagonal);
+
+/* Rounded unit targets of interleaved `coordinates` (the first 2*n_units
+ * entries) as the device checkpoint rounds them (rounded_targets_from_coordinates,
+ * eplace.rs:954-971, rounded_coordinate, lib.rs:4566-4568); out[2*n_units]. */
+int texo_gp_session_round_targets(texo_gp_session *session, const double *coordinates,
+                                  uint32_t *out_targets_xy);
+
+/* Optimizer and schedule state after the last iterate/run call: density
+ * multipliers and their step (eplace.rs:1126-1298), initial energies E0, the
+ * wirelength gamma of the last iteration and the per-field gamma weights
+ * (eplace.rs:1325-1439), Nesterov momentum and Lipschitz estimate
+ * (global_placement.rs:30-58). */
+typedef struct texo_gp_schedule_state {
+  uint32_t n_fields;
+  double multiplier[TEXO_GP_MAX_FIELDS];
+  double multiplier_step;
+  double initial_energy[TEXO_GP_MAX_FIELDS];
+  double gamma;
+  double gamma_weight[TEXO_GP_MAX_FIELDS];
+  double momentum;
+  double lipschitz; /* 0 when not yet estimated */
+} texo_gp_schedule_state;
+int texo_gp_session_schedule(const texo_gp_session *session, texo_gp_schedule_state *state);
+
 /* ---- lockstep batches: multi-start placement ----------------------------- */
 /* No counterpart in the reference (it places one design per call).  A batch
  * owns one GPU and runs every job it holds in lockstep rounds, one launch per
@@ -381,7 +418,9 @@ typedef struct texo_gp_server_stats {
 } texo_gp_server_stats;
 
 texo_gp_server *texo_gp_server_create(int32_t device, uint64_t hbm_budget_bytes, uint32_t max_batch);
-/* Returns a job id > 0, or -(status) if the problem is rejected. */
+/* Returns a job id > 0, or -(status) if the problem is rejected: invalid
+ * input (the EplaceError variants of set-up), or TEXO_GP_ERR_OUT_OF_MEMORY
+ * when the job's device footprint exceeds the server's HBM budget. */
 int64_t texo_gp_server_submit(texo_gp_server *server, const texo_gp_problem *problem,
                               const texo_gp_options *options);
 /* TEXO_GP_ERR_NOT_READY while the job runs; otherwise the job's status, with
@@ -392,6 +431,13 @@ int texo_gp_server_poll(texo_gp_server *server, int64_t job, uint32_t *out_targe
                         texo_gp_report *report);
 int texo_gp_server_wait(texo_gp_server *server, int64_t job, uint32_t *out_targets_xy,
                         texo_gp_report *report);
+/* Cancels a job: a pending job is never admitted, a running one stops at
+ * the next round boundary and releases its device memory; either then
+ * completes with TEXO_GP_ERR_CANCELLED (its result is taken by poll/wait as
+ * usual).  Returns 0 when the cancellation was registered, the job's status
+ * when it had already finished, TEXO_GP_ERR_INVALID_ARGUMENT for an unknown
+ * or already-taken id.  No counterpart in the reference. */
+int texo_gp_server_cancel(texo_gp_server *server, int64_t job);
 int texo_gp_server_stats_get(texo_gp_server *server, texo_gp_server_stats *stats);
 void texo_gp_server_destroy(texo_gp_server *server);
 

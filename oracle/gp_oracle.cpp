@@ -2125,6 +2125,7 @@ struct oracle_session {
   bool started = false;
   std::vector<double> scales, mult;
   double mstep = 0.0;
+  double gamma = 0.0;  // wirelength gamma of the last iteration
   NState st;
   Evaluation current;
 };
@@ -2390,6 +2391,7 @@ int oracle_session_iterate(oracle_session *s, uint32_t iterations, texo_gp_repor
     }
     for (uint32_t it = 0; it < iterations; ++it) {
       double gamma = adaptive_wirelength_gamma(P, s->current.fields, s->opt.gamma_base);
+      s->gamma = gamma;
       s->stats.iterations += 1;
       std::vector<double> precond = diagonal_preconditioner(P, s->current, s->mult, C);
       std::vector<Evaluation> payloads;
@@ -2436,6 +2438,28 @@ int oracle_session_rudy(oracle_session *s, const double *coordinates, double *ho
   } catch (const Failure &f) {
     return f.status;
   }
+}
+
+int oracle_session_schedule(const oracle_session *s, texo_gp_schedule_state *out) {
+  if (!s || !out) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  std::set<uint32_t> kinds;
+  for (auto &du : s->P.density_units)
+    for (auto &m : du.members) kinds.insert(m.kind);
+  out->n_fields = (uint32_t)std::min<size_t>(kinds.size(), TEXO_GP_MAX_FIELDS);
+  size_t f = 0;
+  for (uint32_t k : kinds) {
+    if (f >= TEXO_GP_MAX_FIELDS) break;
+    out->gamma_weight[f] = s->P.gamma_weights.count(k) ? s->P.gamma_weights.at(k) : 0.0;
+    out->multiplier[f] = f < s->mult.size() ? s->mult[f] : 0.0;
+    out->initial_energy[f] = f < s->scales.size() ? s->scales[f] : 0.0;
+    ++f;
+  }
+  out->multiplier_step = s->mstep;
+  out->gamma = s->gamma;
+  out->momentum = s->st.momentum;
+  out->lipschitz = s->st.has_l ? s->st.l : 0.0;
+  return 0;
 }
 
 double oracle_session_hpwl(oracle_session *s, const double *coordinates) {

@@ -51,6 +51,9 @@ int oracle_session_rudy(oracle_session *session, const double *coordinates,
                         double *horizontal, double *vertical);
 /* Continuous bbox HPWL of the retained nets at `coordinates` (FP64). */
 double oracle_session_hpwl(oracle_session *session, const double *coordinates);
+/* Schedule and optimizer state after oracle_session_iterate (multipliers,
+ * their step, E0 scales, last gamma, gamma weights, momentum, Lipschitz). */
+int oracle_session_schedule(const oracle_session *session, texo_gp_schedule_state *state);
 /* Combined objective (combined_objective, eplace.rs:1022-1044) at
  * `coordinates` with explicit multipliers and scales; gradient[C]. */
 int oracle_session_combined(oracle_session *session, const double *coordinates,

@@ -38,7 +38,9 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_the_header(tmp_path):
     # the ctypes mirror must agree with what a C compiler makes of the header
     structs = {"texo_gp_problem": abi.texo_gp_problem, "texo_gp_options": abi.texo_gp_options,
-               "texo_gp_report": abi.texo_gp_report, "texo_gp_field_eval": abi.texo_gp_field_eval}
+               "texo_gp_report": abi.texo_gp_report, "texo_gp_field_eval": abi.texo_gp_field_eval,
+               "texo_gp_schedule_state": abi.texo_gp_schedule_state,
+               "texo_gp_server_stats": abi.texo_gp_server_stats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "texo_gp.h"', "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'printf("{name} %zu\\n", sizeof({name}));')

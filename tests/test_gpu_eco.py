@@ -119,3 +119,25 @@ def test_requests_on_512_bins_are_bit_identical_batched_and_alone():
         for k in ("iterations", "evaluations", "line_search_trials", "final_hpwl", "final_wirelength",
                   "field_overflow", "field_energy"):
             assert ra[k] == rb[k], k
+
+
+@pytest.mark.parametrize("precision", [abi.DCT_FP32, abi.DCT_BF16X3, abi.DCT_FP16_SCALED])
+def test_request_field_error_on_512_bins(precision):
+    # the incremental stream is specified at <= 1e-3 relative field error:
+    # the field at every charge (units and fillers) of a 250k-cell / 512x512
+    # request, against the FP64 oracle, at its start and at spread coordinates
+    r = resident().request(12)
+    print(f"eco 512 field error precision={precision}, {r.meta['movable_units']} movable units", flush=True)
+    g = texo_amd.Session(r, abi.options(dct_precision=precision))
+    o = abi.Session(oracle.load(), "oracle", r)
+    rng = np.random.default_rng(4)
+    for x in (g.initial, np.clip(g.initial + rng.normal(0, 3.0, g.initial.size), g.lower, g.upper)):
+        eg, eo = g.evaluate(x, 4.0), o.evaluate(x, 4.0)
+        a = np.concatenate([eg["unit_gradient"].ravel(), eg["filler_gradient"].ravel()])
+        b = np.concatenate([eo["unit_gradient"].ravel(), eo["filler_gradient"].ravel()])
+        field = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        energy = float(np.max(np.abs(eg["energy"] - eo["energy"]) / eo["energy"]))
+        print(f"  field rel {field:.3e} energy rel {energy:.3e}", flush=True)
+        assert field <= 1e-3 and energy <= 1e-3
+    g.close()
+    o.close()

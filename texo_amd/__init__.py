@@ -45,6 +45,8 @@ def load():
     lib.texo_gp_session_set_kernel_timing.restype = C.c_int
     lib.texo_gp_session_kernel_timing.argtypes = [C.c_void_p, P(d), P(C.c_uint64), P(d)]
     lib.texo_gp_session_kernel_timing.restype = C.c_int
+    lib.texo_gp_session_round_targets.argtypes = [C.c_void_p, P(d), P(C.c_uint32)]
+    lib.texo_gp_session_round_targets.restype = C.c_int
     _lib = lib
     return lib
 
@@ -70,6 +72,16 @@ class Session(abi.Session):
         if rc != 0:
             raise PlacementError.from_status(rc)
         return phi.reshape(F, self.problem.height, self.problem.width), energy
+
+    def round_targets(self, coordinates: np.ndarray) -> np.ndarray:
+        """Device-rounded unit targets [n_units, 2] of interleaved coordinates."""
+        x = np.ascontiguousarray(coordinates, dtype=np.float64)
+        assert x.size >= 2 * self.problem.n_units
+        t = np.zeros(2 * self.problem.n_units, np.uint32)
+        rc = self.lib.texo_gp_session_round_targets(self.handle, abi.ptr(x, C.c_double), abi.ptr(t, C.c_uint32))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return t.reshape(-1, 2)
 
     def set_kernel_timing(self, kernel: Optional[str]):
         self.lib.texo_gp_session_set_kernel_timing(self.handle, (kernel or "").encode(), 1 if kernel else 0)
@@ -142,6 +154,11 @@ class Server:
 
     def wait(self, job: int, raise_on_error: bool = True):
         return self._result(self.lib.texo_gp_server_wait, job, raise_on_error)
+
+    def cancel(self, job: int) -> int:
+        """Cancels a pending or running job (it then completes with status
+        Cancelled); returns the library's status code."""
+        return self.lib.texo_gp_server_cancel(self.handle, job)
 
     def stats(self) -> dict:
         st = abi.texo_gp_server_stats()

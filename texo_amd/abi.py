@@ -121,6 +121,21 @@ class texo_gp_field_eval(C.Structure):
     ]
 
 
+class texo_gp_schedule_state(C.Structure):
+    _fields_ = [
+        ("n_fields", C.c_uint32), ("multiplier", C.c_double * MAX_FIELDS), ("multiplier_step", C.c_double),
+        ("initial_energy", C.c_double * MAX_FIELDS), ("gamma", C.c_double),
+        ("gamma_weight", C.c_double * MAX_FIELDS), ("momentum", C.c_double), ("lipschitz", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        n = self.n_fields
+        return {"multiplier": list(self.multiplier[:n]), "multiplier_step": self.multiplier_step,
+                "initial_energy": list(self.initial_energy[:n]), "gamma": self.gamma,
+                "gamma_weight": list(self.gamma_weight[:n]), "momentum": self.momentum,
+                "lipschitz": self.lipschitz}
+
+
 def ptr(a: Optional[np.ndarray], ctype):
     if a is None:
         return C.POINTER(ctype)()
@@ -273,6 +288,9 @@ def declare_session_api(lib, prefix: str):
     f("session_iterate", C.c_int, sess, C.c_uint32, P(texo_gp_report))
     f("session_coordinates", C.c_int, sess, P(d))
     f("session_rudy", C.c_int, sess, P(d), P(d), P(d))
+    f("session_combined", C.c_int, sess, P(d), d, P(d), P(d), P(d), P(d))
+    f("session_preconditioner", C.c_int, sess, P(d), P(d))
+    f("session_schedule", C.c_int, sess, P(texo_gp_schedule_state))
 
 
 class Session:
@@ -379,6 +397,38 @@ class Session:
         return h.reshape(shape), v.reshape(shape)
 
 
+    def combined(self, coordinates: np.ndarray, gamma: float, multipliers, scales):
+        """(value, gradient[C]) of the combined objective with explicit
+        multipliers and initial-energy scales."""
+        x = np.ascontiguousarray(coordinates, dtype=np.float64)
+        m = np.ascontiguousarray(multipliers, dtype=np.float64)
+        sc = np.ascontiguousarray(scales, dtype=np.float64)
+        g = np.zeros(self.n_coordinates)
+        v = C.c_double()
+        rc = getattr(self.lib, f"{self.prefix}_session_combined")(
+            self.handle, ptr(x, C.c_double), gamma, ptr(m, C.c_double), ptr(sc, C.c_double), C.byref(v),
+            ptr(g, C.c_double))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return v.value, g
+
+    def preconditioner(self, multipliers) -> np.ndarray:
+        m = np.ascontiguousarray(multipliers, dtype=np.float64)
+        d = np.zeros(self.n_coordinates)
+        rc = getattr(self.lib, f"{self.prefix}_session_preconditioner")(self.handle, ptr(m, C.c_double),
+                                                                       ptr(d, C.c_double))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return d
+
+    def schedule(self) -> dict:
+        st = texo_gp_schedule_state()
+        rc = getattr(self.lib, f"{self.prefix}_session_schedule")(self.handle, C.byref(st))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return st.as_dict()
+
+
 class PlacementError(RuntimeError):
     """A typed failure of the placement call (mirrors EplaceError)."""
 
@@ -426,6 +476,7 @@ def declare_server_api(lib, prefix: str):
     f("server_submit", C.c_int64, C.c_void_p, P(texo_gp_problem), P(texo_gp_options))
     f("server_poll", C.c_int, C.c_void_p, C.c_int64, P(C.c_uint32), P(texo_gp_report))
     f("server_wait", C.c_int, C.c_void_p, C.c_int64, P(C.c_uint32), P(texo_gp_report))
+    f("server_cancel", C.c_int, C.c_void_p, C.c_int64)
     f("server_stats_get", C.c_int, C.c_void_p, P(texo_gp_server_stats))
     f("server_destroy", None, C.c_void_p)
 

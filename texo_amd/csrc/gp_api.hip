@@ -142,11 +142,8 @@ texo_gp_session *texo_gp_session_create(const texo_gp_problem *problem, const te
     s->engine.reset(new Engine(s->job.opt.device, 4));
     s->engine->attach(&s->job);
     return s.release();
-  } catch (const Failure &f) {
-    put_failure(report, f);
-    return nullptr;
-  } catch (const std::bad_alloc &) {
-    if (report) report->status = TEXO_GP_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    put_failure(report, texo::current_failure());
     return nullptr;
   }
 }
@@ -234,7 +231,8 @@ int texo_gp_session_evaluate(texo_gp_session *s, const double *coordinates, doub
     }
     for (size_t i = 0; i < FG && potential; ++i) potential[i] = static_cast<double>(phi[i]);
     return 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     return f.status;
   }
 }
@@ -306,7 +304,8 @@ int texo_gp_session_poisson(texo_gp_session *s, const double *density, double *p
       energy[f] = e;
     }
     return 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     return f.status;
   }
 }
@@ -324,7 +323,8 @@ int texo_gp_session_run(texo_gp_session *s, uint32_t *out_targets_xy, texo_gp_re
     copy_targets(*s->engine, J, out_targets_xy);
     if (report) report->elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     put_failure(report, f);
     return f.status;
   }
@@ -342,7 +342,8 @@ int texo_gp_session_iterate(texo_gp_session *s, uint32_t iterations, texo_gp_rep
     drive(*s->engine, J);
     fill_report(*s->engine, J, report);
     return J.phase == Phase::Failed ? J.status : 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     put_failure(report, f);
     return f.status;
   }
@@ -360,7 +361,7 @@ int texo_gp_session_coordinates(const texo_gp_session *s, double *coordinates) {
     const hipStream_t st = s->engine->stream();
     HIP(hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP(hipStreamSynchronize(st));
-  } catch (const Failure &) {
+  } catch (...) {
     return TEXO_GP_ERR_DEVICE;
   }
   return 0;
@@ -383,7 +384,8 @@ int texo_gp_session_rudy(texo_gp_session *s, const double *coordinates, double *
     HIP(hipMemcpyAsync(&err, J.d_rudy_err, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
     HIP(hipStreamSynchronize(st));
     return err ? TEXO_GP_ERR_INVALID_NORMALIZATION : 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     return f.status;
   }
 }
@@ -400,11 +402,107 @@ int texo_gp_session_kernel_timing(texo_gp_session *s, double *avg_ms, uint64_t *
 
 int texo_gp_place(const texo_gp_problem *problem, const texo_gp_options *options, uint32_t *out_targets_xy,
                   texo_gp_report *report) {
-  texo_gp_session *s = texo_gp_session_create(problem, options, report);
-  if (!s) return report ? report->status : TEXO_GP_ERR_INVALID_ARGUMENT;
-  const int rc = texo_gp_session_run(s, out_targets_xy, report);
+  // a caller without a report still gets the typed status
+  texo_gp_report local;
+  texo_gp_report *r = report ? report : &local;
+  texo_gp_session *s = texo_gp_session_create(problem, options, r);
+  if (!s) return r->status ? r->status : TEXO_GP_ERR_INVALID_ARGUMENT;
+  const int rc = texo_gp_session_run(s, out_targets_xy, r);
   texo_gp_session_destroy(s);
   return rc;
+}
+
+int texo_gp_session_combined(texo_gp_session *s, const double *coordinates, double gamma, const double *multipliers,
+                             const double *scales, double *value, double *gradient) {
+  if (!s || !coordinates || !multipliers || !scales || !value || !gradient) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &J = s->job;
+  if (J.empty || !(std::isfinite(gamma) && gamma > 0.0)) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  for (int f = 0; f < J.F; ++f)
+    if (!(std::isfinite(multipliers[f]) && multipliers[f] >= 0.0) || !(std::isfinite(scales[f]) && scales[f] > 0.0))
+      return TEXO_GP_ERR_INVALID_ARGUMENT;
+  for (size_t i = 0; i < J.C; ++i)
+    if (!std::isfinite(coordinates[i])) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const std::vector<double> mult = J.mult, scl = J.scales;
+  int rc = 0;
+  try {
+    J.mult.assign(multipliers, multipliers + J.F);
+    J.scales.assign(scales, scales + J.F);
+    Engine &E = *s->engine;
+    E.evaluate_once(&J, coordinates, gamma, false);
+    *value = J.combined_value(J.h_scalars->eval[2]);
+    const hipStream_t st = E.stream();
+    HIP(hipMemcpyAsync(gradient, J.d_grad[0], J.C * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+  } catch (...) {
+    rc = texo::current_failure().status;
+  }
+  J.mult = mult;
+  J.scales = scl;
+  return rc;
+}
+
+int texo_gp_session_preconditioner(texo_gp_session *s, const double *multipliers, double *diagonal) {
+  if (!s || !multipliers || !diagonal) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &J = s->job;
+  if (J.empty) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    // the coefficient check of diagonal_preconditioner (eplace.rs:1060-1063)
+    std::vector<double> lambda(multipliers, multipliers + J.F);
+    for (double l : lambda)
+      if (!std::isfinite(l) || l < 0.0) return TEXO_GP_ERR_INVALID_NORMALIZATION;
+    Engine &E = *s->engine;
+    if (!E.precondition_once(&J, lambda)) return TEXO_GP_ERR_INVALID_NORMALIZATION;
+    const hipStream_t st = E.stream();
+    HIP(hipMemcpyAsync(diagonal, J.d_metric, J.C * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+    return 0;
+  } catch (...) {
+    return texo::current_failure().status;
+  }
+}
+
+int texo_gp_session_round_targets(texo_gp_session *s, const double *coordinates, uint32_t *out_targets_xy) {
+  if (!s || !coordinates || !out_targets_xy) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &J = s->job;
+  if (J.U == 0) return 0;
+  if (J.empty) {
+    for (size_t i = 0; i < 2 * J.U; ++i)
+      out_targets_xy[i] = texo::rounded_coordinate(coordinates[i], (i & 1) ? J.H : J.W);
+    return 0;
+  }
+  uint32_t *d_out = nullptr;
+  int rc = 0;
+  try {
+    Engine &E = *s->engine;
+    const hipStream_t st = E.stream();
+    HIP(hipMalloc(&d_out, 2 * J.U * sizeof(uint32_t)));
+    HIP(hipMemcpyAsync(J.d_eval_coords, coordinates, 2 * J.U * sizeof(double), hipMemcpyHostToDevice, st));
+    texo::launch_round_targets_to(J.d_eval_coords, d_out, J.U, J.W, J.H, st);
+    HIP(hipGetLastError());
+    HIP(hipMemcpyAsync(out_targets_xy, d_out, 2 * J.U * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP(hipStreamSynchronize(st));
+  } catch (...) {
+    rc = texo::current_failure().status;
+  }
+  if (d_out) (void)hipFree(d_out);
+  return rc;
+}
+
+int texo_gp_session_schedule(const texo_gp_session *s, texo_gp_schedule_state *out) {
+  if (!s || !out) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const Job &J = s->job;
+  std::memset(out, 0, sizeof(*out));
+  out->n_fields = (uint32_t)std::min(J.F, TEXO_GP_MAX_FIELDS);
+  for (uint32_t f = 0; f < out->n_fields; ++f) {
+    out->multiplier[f] = f < J.mult.size() ? J.mult[f] : 0.0;
+    out->initial_energy[f] = f < J.scales.size() ? J.scales[f] : 0.0;
+    out->gamma_weight[f] = f < J.gamma_weight.size() ? J.gamma_weight[f] : 0.0;
+  }
+  out->multiplier_step = J.mstep;
+  out->gamma = J.gamma;
+  out->momentum = J.momentum;
+  out->lipschitz = J.has_lipschitz ? J.lipschitz : 0.0;
+  return 0;
 }
 
 }  // extern "C"
@@ -468,7 +566,8 @@ int64_t texo_gp_batch_add(texo_gp_batch *b, const texo_gp_problem *problem, cons
     j->load(problem, options, b->engine->image_pool());
     if (j->opt.device != b->engine->device()) texo::fail(TEXO_GP_ERR_INVALID_ARGUMENT);
     b->engine->attach(j.get());
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     put_failure(report, f);
     if (j->pool_index >= 0) b->engine->detach(j.get());
     return -(int64_t)f.status;
@@ -485,7 +584,8 @@ int texo_gp_batch_run(texo_gp_batch *b) {
       if (j->phase != Phase::Failed) j->begin_run(false);
     b->drive_all();
     return 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     return f.status;
   }
 }
@@ -501,7 +601,8 @@ int texo_gp_batch_iterate(texo_gp_batch *b, uint32_t iterations) {
     }
     b->drive_all();
     return 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     return f.status;
   }
 }
@@ -518,7 +619,8 @@ int texo_gp_batch_result(texo_gp_batch *b, uint64_t job, uint32_t *out_targets_x
       copy_targets(*b->engine, j, out_targets_xy);
     }
     return 0;
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     put_failure(report, f);
     return f.status;
   }
@@ -547,7 +649,7 @@ int texo_gp_batch_coordinates(texo_gp_batch *b, uint64_t job, double *coordinate
     const hipStream_t st = b->engine->stream();
     HIP(hipMemcpyAsync(coordinates, J.d_vec[J.ix], J.C * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP(hipStreamSynchronize(st));
-  } catch (const Failure &) {
+  } catch (...) {
     return TEXO_GP_ERR_DEVICE;
   }
   return 0;
@@ -575,6 +677,7 @@ struct texo_gp_server {
     std::unique_ptr<Job> job;  // host part kept until the result is taken
     std::chrono::steady_clock::time_point submitted;
     bool done = false;
+    bool cancel = false;  // texo_gp_server_cancel on a running job
     int status = 0;
     std::vector<uint32_t> targets;
     std::vector<double> x;  // final coordinates, for the report's HPWL
@@ -633,7 +736,7 @@ void texo_gp_server::loop() {
           if (live[c]) {
             try {
               engine->finish(c);
-            } catch (const Failure &) {
+            } catch (...) {
             }
           }
         collect(true);
@@ -651,7 +754,8 @@ void texo_gp_server::loop() {
         try {
           engine->attach(e.job.get());
           e.job->begin_run(false);
-        } catch (const Failure &f) {
+        } catch (...) {
+          const Failure f = texo::current_failure();
           e.job->failure = f;
           e.job->status = f.status;
           e.job->phase = Phase::Failed;
@@ -679,7 +783,8 @@ void texo_gp_server::loop() {
         const auto t0 = std::chrono::steady_clock::now();
         try {
           engine->finish(c);
-        } catch (const Failure &f) {
+        } catch (...) {
+          const Failure f = texo::current_failure();
           for (int64_t id : group[c]) {
             Job &j = *entries[id]->job;
             if (j.phase != Phase::Done && j.phase != Phase::Failed) {
@@ -697,14 +802,30 @@ void texo_gp_server::loop() {
       }
       retire(c, group[c]);
       std::vector<Job *> jobs;
+      bool cancelled = false;
       {
         std::lock_guard<std::mutex> lk(mu);
-        for (int64_t id : group[c]) jobs.push_back(entries[id]->job.get());
+        for (int64_t id : group[c]) {
+          Entry &e = *entries[id];
+          Job *j = e.job.get();
+          // a cancelled job stops at this round boundary: its previous
+          // round has finished, so its memory can go back to the pools
+          if (e.cancel && j->phase != Phase::Done && j->phase != Phase::Failed) {
+            j->failure = Failure{TEXO_GP_ERR_CANCELLED};
+            j->status = TEXO_GP_ERR_CANCELLED;
+            j->phase = Phase::Failed;
+            cancelled = true;
+            continue;
+          }
+          jobs.push_back(j);
+        }
       }
+      if (cancelled) retire(c, group[c]);
       const auto t1 = std::chrono::steady_clock::now();
       try {
         live[c] = engine->launch(c, jobs);
-      } catch (const Failure &f) {
+      } catch (...) {
+        const Failure f = texo::current_failure();
         for (Job *j : jobs) {
           if (j->phase != Phase::Done && j->phase != Phase::Failed) {
             j->failure = f;
@@ -755,7 +876,8 @@ void texo_gp_server::retire(int c, std::vector<int64_t> &group) {
         char *dst = static_cast<char *>(e.readback);
         HIP(hipMemcpyAsync(dst, j.d_vec[j.ix], j.U * 16, hipMemcpyDeviceToHost, st));
         if (j.phase == Phase::Done) HIP(hipMemcpyAsync(dst + j.U * 16, j.d_best, j.U * 8, hipMemcpyDeviceToHost, st));
-      } catch (const Failure &f) {
+      } catch (...) {
+        const Failure f = texo::current_failure();
         j.failure = f;
         j.status = f.status;
         j.phase = Phase::Failed;
@@ -894,8 +1016,11 @@ int64_t texo_gp_server_submit(texo_gp_server *s, const texo_gp_problem *problem,
     // deep copy, validation, upload image and (for a new grid size) the
     // transform basis on the caller's thread
     e->job->load(problem, options, s->engine->image_pool());
+    // a job that can never fit the budget is rejected, not admitted alone
+    if (e->job->device_bytes() > s->budget) return -(int64_t)TEXO_GP_ERR_OUT_OF_MEMORY;
     if (!e->job->empty) s->engine->basis((int)e->job->W, (int)e->job->H);
-  } catch (const Failure &f) {
+  } catch (...) {
+    const Failure f = texo::current_failure();
     return -(int64_t)f.status;
   }
   std::lock_guard<std::mutex> lk(s->mu);
@@ -929,6 +1054,36 @@ int texo_gp_server_wait(texo_gp_server *s, int64_t id, uint32_t *out, texo_gp_re
   // another waiter may have taken the result first
   if (s->entries.find(id) == s->entries.end()) return TEXO_GP_ERR_INVALID_ARGUMENT;
   return take_result(s, lk, id, out, report);
+}
+
+int texo_gp_server_cancel(texo_gp_server *s, int64_t id) {
+  if (!s) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->entries.find(id);
+  if (it == s->entries.end()) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  texo_gp_server::Entry &e = *it->second;
+  if (e.done) return e.status;
+  auto p = std::find(s->pending.begin(), s->pending.end(), id);
+  if (p != s->pending.end()) {
+    // never admitted: complete it here
+    s->pending.erase(p);
+    Job &j = *e.job;
+    j.release_image();
+    e.targets.assign(2 * j.U, 0);
+    e.x = j.initial_xy;
+    std::memset(&e.report, 0, sizeof(e.report));
+    e.status = TEXO_GP_ERR_CANCELLED;
+    e.report.status = e.status;
+    e.report.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
+    e.done = true;
+    s->stats.failed += 1;
+    lk.unlock();
+    s->cv_done.notify_all();
+    return 0;
+  }
+  e.cancel = true;
+  s->cv_work.notify_all();
+  return 0;
 }
 
 int texo_gp_server_stats_get(texo_gp_server *s, texo_gp_server_stats *stats) {

@@ -174,6 +174,39 @@ void launch_collect_scalars(const int *index, int n, const JobScalars *pool, Job
   if (n > 0) hipLaunchKernelGGL(k_collect_scalars, dim3(n), dim3(64), 0, s, index, pool, out);
 }
 
+void launch_round_targets_to(const double *coords, uint32_t *out, size_t units, uint32_t W, uint32_t H,
+                             hipStream_t s) {
+  if (units == 0) return;
+  TargetBatch b{};
+  b.coords[0] = coords;
+  b.out[0] = out;
+  b.n[0] = (int)(2 * units);
+  b.W[0] = (int)W;
+  b.H[0] = (int)H;
+  b.block_end[0] = (int)((2 * units + BLOCK - 1) / BLOCK);
+  b.count = 1;
+  hipLaunchKernelGGL(k_round_targets, dim3(b.block_end[0]), dim3(BLOCK), 0, s, b);
+}
+
+Failure current_failure() {
+  Failure f{};
+  try {
+    throw;
+  } catch (const Failure &e) {
+    f = e;
+  } catch (const std::bad_alloc &) {
+    f.status = TEXO_GP_ERR_OUT_OF_MEMORY;
+    g_last_error = "host memory exhausted";
+  } catch (const std::exception &e) {
+    f.status = TEXO_GP_ERR_DEVICE;
+    g_last_error = e.what();
+  } catch (...) {
+    f.status = TEXO_GP_ERR_DEVICE;
+    g_last_error = "unknown exception";
+  }
+  return f;
+}
+
 // Rounds the pending best checkpoints of `jobs` (PlacementCheckpoint::consider
 // stores the rounded targets, eplace.rs:107-120) in launches of TARGET_BATCH.
 void launch_round_targets(const std::vector<Job *> &jobs, hipStream_t s) {
@@ -932,9 +965,9 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
     try {
       plan(j);
       if (!j->evals.empty()) active.push_back(j);
-    } catch (const Failure &f) {
-      j->failure = f;
-      j->status = f.status;
+    } catch (...) {
+      j->failure = current_failure();
+      j->status = j->failure.status;
       j->phase = Phase::Failed;
     }
   }
@@ -1156,9 +1189,9 @@ void Engine::finish(int c) {
       if (!r.skip) j->moved = std::max(j->moved, j->h_scalars->eval[r.scalar_slot].moved);
     try {
       advance(j);
-    } catch (const Failure &f) {
-      j->failure = f;
-      j->status = f.status;
+    } catch (...) {
+      j->failure = current_failure();
+      j->status = j->failure.status;
       j->phase = Phase::Failed;
     }
   }
@@ -1244,6 +1277,52 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   j->h_scalars = h_pool_ + j->pool_index;
   j->phase = saved;
   j->evals.clear();
+}
+
+bool Engine::precondition_once(Job *j, const std::vector<double> &lambda) {
+  HIP(hipSetDevice(device_));
+  const hipStream_t st = stream();
+  OptTask o{};
+  o.n = j->C;
+  o.n_units = (int)j->U;
+  o.n_fillers = (int)j->Fl;
+  o.F = j->F;
+  o.DU = (int)j->DU;
+  o.lower = j->d_lower;
+  o.upper = j->d_upper;
+  o.wl_diag = j->d_wl_diag;
+  o.du_fcharge = j->d_du_fcharge;
+  o.fl_charge = j->d_fl_charge;
+  o.unit_du = j->d_unit_du;
+  o.fl_field = j->d_fl_field;
+  for (int f = 0; f < MAXF; ++f) o.lambda[f] = f < j->F ? lambda[f] : 0.0;
+  o.metric = j->d_metric;
+  o.part_metric = j->d_part_metric;
+  o.metric_scalars = &j->d_scalars->metric;
+  o.do_precond = 1;
+  Plan pre, none;
+  pre.add(j->U + j->Fl, BLOCK);
+  none.none();
+  std::vector<OptTask> tasks{o};
+  std::vector<char> buf;
+  const size_t o_t = append(buf, tasks), o_pre = append(buf, pre.start), o_none = append(buf, none.start);
+  HIP(hipStreamSynchronize(st));
+  ensure_stage(aux_, buf.size());
+  std::memcpy(aux_.h_stage, buf.data(), buf.size());
+  HIP(hipMemcpyAsync(aux_.d_stage, aux_.h_stage, buf.size(), hipMemcpyHostToDevice, st));
+  OptMaps om{};
+  om.precond = pre.device(aux_.d_stage, o_pre);
+  om.extrapolate = none.device(aux_.d_stage, o_none);
+  om.candidate = none.device(aux_.d_stage, o_none);
+  om.n_tasks = 1;
+  const OptTask *d_opt = reinterpret_cast<const OptTask *>(aux_.d_stage + o_t);
+  launch_optimizer(d_opt, om, st);
+  launch_optimizer_finalize(d_opt, om, st);
+  HIP(hipGetLastError());
+  MetricScalars ms{};
+  HIP(hipMemcpyAsync(&ms, &j->d_scalars->metric, sizeof(ms), hipMemcpyDeviceToHost, st));
+  HIP(hipStreamSynchronize(st));
+  return ms.nonfinite == 0;
 }
 
 }  // namespace texo

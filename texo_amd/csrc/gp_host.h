@@ -152,6 +152,13 @@ struct Job;
 // Jobs per launch of the batched best-checkpoint rounding.
 constexpr int TARGET_BATCH = 64;
 void launch_round_targets(const std::vector<Job *> &jobs, hipStream_t s);
+// Rounds 2*units interleaved coordinates into `out` (one job, one launch).
+void launch_round_targets_to(const double *coords, uint32_t *out, size_t units, uint32_t W, uint32_t H,
+                             hipStream_t s);
+// The typed failure of the exception being handled: a Failure as thrown,
+// std::bad_alloc as OUT_OF_MEMORY, anything else as DEVICE (with its message
+// in the thread's last error).  Call only inside a catch block.
+Failure current_failure();
 void launch_collect_scalars(const int *index, int n, const JobScalars *pool, JobScalars *out, hipStream_t s);
 
 enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed };
@@ -395,6 +402,9 @@ class Engine {
   double last_wait_seconds() const { return last_wait_; }
   // Parity evaluation of one job at host coordinates.
   void evaluate_once(Job *job, const double *coords, double gamma, bool density_out);
+  // Preconditioner of one job for `lambda` into job->d_metric (parity seam);
+  // returns false when an entry is not finite or negative before the floor.
+  bool precondition_once(Job *job, const std::vector<double> &lambda);
   KernelTimer timer;
 
  private:
