@@ -441,6 +441,64 @@ int texo_gp_server_cancel(texo_gp_server *server, int64_t job);
 int texo_gp_server_stats_get(texo_gp_server *server, texo_gp_server_stats *stats);
 void texo_gp_server_destroy(texo_gp_server *server);
 
+/* ---- GP input builders ------------------------------------------------- */
+
+/* channel_orientation (crates/texo-pnr/src/routing_demand.rs:185-198): 1 for
+ * a general horizontal wire name ("H##..." after the last '/'), 2 for a
+ * vertical one ("V##..."), 0 otherwise.  Host only. */
+int texo_gp_channel_orientation(const char *name);
+
+/* routing_capacity_map (crates/texo-pnr/src/routing_demand.rs:201-239):
+ * usable general-channel capacity per tile.  A wire (orientation code from
+ * texo_gp_channel_orientation, tile point wire_xy[2w], capacity) counts when
+ * at least one pip incident to it is not blocked (pip_blocked may be NULL);
+ * per-tile sums saturate at UINT32_MAX.  Outputs horizontal/vertical
+ * [width*height] row-major.  Runs on `device`; bit-exact. */
+int texo_gp_routing_capacity_map(int32_t device, uint32_t width, uint32_t height, uint32_t n_wires,
+                                 const uint8_t *wire_orientation, const uint32_t *wire_xy,
+                                 const uint16_t *wire_capacity, uint32_t n_pips, const uint32_t *pip_from,
+                                 const uint32_t *pip_to, const uint8_t *pip_blocked, uint32_t *horizontal,
+                                 uint32_t *vertical);
+
+/* routing_demand_map_with_capacity (crates/texo-pnr/src/routing_demand.rs:262-313):
+ * bbox-uniform directional demand of a legal placement.  Nets are CSR over
+ * pins at tile points pin_xy[2p] (driver first; 0xffffffff marks a cell with
+ * no placement, skipped; a net whose driver is unplaced is skipped);
+ * net_clock[n] = 1 excludes a clock-driven net.  Every tile of a net's
+ * inclusive bbox receives dx/((dx+1)(dy+1)) horizontal and dy/(...) vertical
+ * demand.  Sums are 2^-40 fixed point (within nets * 2^-41 of the
+ * reference's FP64 sums in net order; run-to-run identical).  A placed pin
+ * outside the grid is TEXO_GP_ERR_POINT_OUTSIDE_GRID. */
+int texo_gp_routing_demand_map(int32_t device, uint32_t width, uint32_t height, uint32_t n_nets,
+                               const uint32_t *net_pin_start, const uint32_t *pin_xy, const uint8_t *net_clock,
+                               double *horizontal, double *vertical, uint64_t *included_nets,
+                               uint64_t *excluded_clock_nets);
+
+/* ecp5_timing_placement_weights (crates/texo-flow/src/lib.rs:3185-3201) with
+ * ecp5_placement_criticality_weight (:3355-3390): per timing edge (net,
+ * sink pin, path delay, domain worst path delay; delays are 128-bit values
+ * given as (low, high) 64-bit words) the integer weight 1 + round(10 c^k) on
+ * a 1/1024 grid; the maximum per (net, sink) is written in (net, sink) order
+ * to out_*[<= n_edges].  Returns the number of entries.  Host only. */
+uint64_t texo_gp_timing_placement_weights(uint64_t n_edges, const uint32_t *net, const uint32_t *sink,
+                                          const uint64_t *path_delay_ps, const uint64_t *worst_path_delay_ps,
+                                          uint32_t exponent, uint32_t *out_net, uint32_t *out_sink,
+                                          uint64_t *out_weight);
+
+/* Ecp5InitialPlacementAlgorithm (crates/texo-flow/src/lib.rs:300-324). */
+#define TEXO_GP_ALGORITHM_IMPORTED 0u
+#define TEXO_GP_ALGORITHM_ROUTABILITY_ELECTROSTATIC 1u
+#define TEXO_GP_ALGORITHM_TIMING_ROUTABILITY_ELECTROSTATIC 2u
+
+/* Provenance: the checkpoint's placement-model record
+ * (checkpoint_placement_model, crates/texo-cli/src/checkpoint.rs:87-100) as
+ * JSON, extended for the electrostatic algorithms with a "global_placement"
+ * object naming this engine, its transform precision, density accumulation
+ * and options, so that a checkpoint records how its placement was computed.
+ * Returns 0, or -(bytes needed) when `capacity` is too small. */
+int texo_gp_placement_model(uint32_t algorithm, uint32_t weight_exponent, const texo_gp_options *options,
+                            char *out, size_t capacity);
+
 /* Last error message of the calling thread (static storage). */
 const char *texo_gp_last_error(void);
 

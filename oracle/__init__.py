@@ -93,8 +93,22 @@ def load():
                                                  EVALUATE_FN, C.c_void_p, C.POINTER(NesterovResult)]
     lib.oracle_nesterov_step.argtypes = [C.POINTER(NesterovState), u64, P(d), P(d), P(d), EVALUATE_FN, C.c_void_p,
                                          C.POINTER(NesterovResult)]
+    from texo_amd import builders
+    builders.declare(_Named(lib), "oracle")
+    lib.oracle_criticality_weight.argtypes = [u64, u64, u64, u64, u32]
+    lib.oracle_criticality_weight.restype = u64
     _lib = lib
     return lib
+
+
+class _Named:
+    """Item access to a CDLL's functions by name (for shared declarations)."""
+
+    def __init__(self, lib):
+        self.lib = lib
+
+    def __getitem__(self, name):
+        return getattr(self.lib, name)
 
 
 class OracleAreaMember(C.Structure):

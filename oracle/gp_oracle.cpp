@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <cfloat>
 #include <cmath>
 #include <complex>
@@ -2901,6 +2902,174 @@ int oracle_net_bounding_boxes(uint32_t n_units, uint32_t n_cells, const double *
     for (int k = 0; k < 4; ++k) boxes[4 * i + k] = b[i][k];
   *n_boxes = b.size();
   return 0;
+}
+
+
+// ---- GP input builders ----------------------------------------------------
+int oracle_channel_orientation(const char *name) {
+  // channel_orientation (routing_demand.rs:185-198): the last '/' component
+  const char *local = std::strrchr(name, '/');
+  local = local ? local + 1 : name;
+  if (std::strlen(local) < 3 || !std::isdigit((unsigned char)local[1]) || !std::isdigit((unsigned char)local[2]))
+    return 0;
+  return local[0] == 'H' ? 1 : local[0] == 'V' ? 2 : 0;
+}
+
+int oracle_routing_capacity_map(uint32_t width, uint32_t height, uint32_t n_wires, const uint8_t *wire_orientation,
+                                const uint32_t *wire_xy, const uint16_t *wire_capacity, uint32_t n_pips,
+                                const uint32_t *pip_from, const uint32_t *pip_to, const uint8_t *pip_blocked,
+                                uint32_t *horizontal, uint32_t *vertical) {
+  const size_t tiles = (size_t)width * height;
+  std::fill(horizontal, horizontal + tiles, 0u);
+  std::fill(vertical, vertical + tiles, 0u);
+  std::vector<bool> usable(n_wires, false);
+  for (uint32_t i = 0; i < n_pips; ++i) {
+    if (pip_blocked && pip_blocked[i]) continue;
+    if (pip_from[i] >= n_wires || pip_to[i] >= n_wires) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    usable[pip_from[i]] = true;
+    usable[pip_to[i]] = true;
+  }
+  for (uint32_t w = 0; w < n_wires; ++w) {
+    if (!usable[w] || wire_orientation[w] == 0) continue;
+    const uint32_t x = wire_xy[2 * w], y = wire_xy[2 * w + 1];
+    if (x >= width || y >= height) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    uint32_t &slot = (wire_orientation[w] == 1 ? horizontal : vertical)[(size_t)y * width + x];
+    const uint64_t sum = (uint64_t)slot + wire_capacity[w];
+    slot = sum > 0xffffffffull ? 0xffffffffu : (uint32_t)sum;  // saturating_add
+  }
+  return 0;
+}
+
+int oracle_routing_demand_map(uint32_t width, uint32_t height, uint32_t n_nets, const uint32_t *net_pin_start,
+                              const uint32_t *pin_xy, const uint8_t *net_clock, double *horizontal, double *vertical,
+                              uint64_t *included_nets, uint64_t *excluded_clock_nets) {
+  const size_t tiles = (size_t)width * height;
+  std::fill(horizontal, horizontal + tiles, 0.0);
+  std::fill(vertical, vertical + tiles, 0.0);
+  uint64_t included = 0, excluded = 0;
+  for (uint32_t n = 0; n < n_nets; ++n) {
+    if (net_clock && net_clock[n]) {
+      excluded += 1;
+      continue;
+    }
+    const uint32_t b = net_pin_start[n], e = net_pin_start[n + 1];
+    if (b == e || pin_xy[2 * b] == 0xffffffffu) continue;  // unplaced driver
+    uint32_t x0 = pin_xy[2 * b], x1 = x0, y0 = pin_xy[2 * b + 1], y1 = y0;
+    for (uint32_t p = b + 1; p < e; ++p) {
+      if (pin_xy[2 * p] == 0xffffffffu) continue;
+      x0 = std::min(x0, pin_xy[2 * p]);
+      x1 = std::max(x1, pin_xy[2 * p]);
+      y0 = std::min(y0, pin_xy[2 * p + 1]);
+      y1 = std::max(y1, pin_xy[2 * p + 1]);
+    }
+    if (x1 >= width || y1 >= height) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    included += 1;
+    const uint32_t dx = x1 - x0, dy = y1 - y0;
+    const double area = (double)(dx + 1) * (double)(dy + 1);
+    const double h = (double)dx / area, v = (double)dy / area;
+    for (uint32_t y = y0; y <= y1; ++y)
+      for (uint32_t x = x0; x <= x1; ++x) {
+        horizontal[(size_t)y * width + x] += h;
+        vertical[(size_t)y * width + x] += v;
+      }
+  }
+  if (included_nets) *included_nets = included;
+  if (excluded_clock_nets) *excluded_clock_nets = excluded;
+  return 0;
+}
+
+namespace {
+using u128 = unsigned __int128;
+// scaled_binary_fraction (texo-flow lib.rs:3413-3433)
+uint64_t scaled_binary_fraction(u128 numerator, u128 denominator, uint32_t bits) {
+  if (numerator >= denominator) return 1ull << bits;
+  u128 remainder = numerator;
+  uint64_t scaled = 0;
+  for (uint32_t i = 0; i < bits; ++i) {
+    scaled <<= 1;
+    const u128 complement = denominator - remainder;
+    if (remainder >= complement) {
+      remainder -= complement;
+      scaled |= 1;
+    } else {
+      remainder *= 2;
+    }
+  }
+  return scaled;
+}
+bool checked_pow(u128 base, uint32_t exponent, u128 &out) {
+  u128 r = 1;
+  for (uint32_t i = 0; i < exponent; ++i) {
+    if (base != 0 && r > ~(u128)0 / base) return false;
+    r *= base;
+  }
+  out = r;
+  return true;
+}
+// scaled_fraction_power (texo-flow lib.rs:3395-3411)
+uint64_t scaled_fraction_power(uint64_t scaled, uint32_t exponent, uint64_t scale) {
+  const u128 s = scale;
+  u128 result = s, factor = scaled;
+  while (exponent != 0) {
+    if (exponent & 1) result = (result * factor + s / 2) / s;
+    exponent >>= 1;
+    if (exponent != 0) factor = (factor * factor + s / 2) / s;
+  }
+  return (uint64_t)result;
+}
+}  // namespace
+
+uint64_t oracle_criticality_weight(uint64_t path_lo, uint64_t path_hi, uint64_t worst_lo, uint64_t worst_hi,
+                                   uint32_t exponent) {
+  // ecp5_placement_criticality_weight (texo-flow lib.rs:3355-3390)
+  const uint64_t SCALE = 1ull << 10, TIMING_WEIGHT = 10;
+  exponent = std::max(exponent, 1u);
+  const u128 path = ((u128)path_hi << 64) | path_lo, worst = ((u128)worst_hi << 64) | worst_lo;
+  const uint64_t scaled = worst == 0 ? 0 : scaled_binary_fraction(std::min(path, worst), worst, 10);
+  u128 num, den;
+  bool exact = checked_pow(scaled, exponent, num) && checked_pow(SCALE, exponent, den);
+  u128 extra = 0;
+  if (exact) {
+    // checked_mul of TIMING_WEIGHT * numerator
+    if (num > ~(u128)0 / TIMING_WEIGHT) {
+      exact = false;
+    } else {
+      extra = (TIMING_WEIGHT * num + den / 2) / den;
+    }
+  }
+  if (!exact) {
+    const uint64_t powered = scaled_fraction_power(scaled, exponent, SCALE);
+    // TIMING_WEIGHT.saturating_mul(powered).saturating_add(SCALE / 2) / SCALE in u64
+    uint64_t m = powered > UINT64_MAX / TIMING_WEIGHT ? UINT64_MAX : TIMING_WEIGHT * powered;
+    m = m > UINT64_MAX - SCALE / 2 ? UINT64_MAX : m + SCALE / 2;
+    extra = m / SCALE;
+  }
+  const uint64_t e = extra > (u128)UINT64_MAX ? TIMING_WEIGHT : (uint64_t)extra;
+  return 1 + std::min(e, TIMING_WEIGHT);
+}
+
+uint64_t oracle_timing_placement_weights(uint64_t n_edges, const uint32_t *net, const uint32_t *sink,
+                                         const uint64_t *path_delay_ps, const uint64_t *worst_path_delay_ps,
+                                         uint32_t exponent, uint32_t *out_net, uint32_t *out_sink,
+                                         uint64_t *out_weight) {
+  std::map<std::pair<uint32_t, uint32_t>, uint64_t> w;
+  for (uint64_t i = 0; i < n_edges; ++i) {
+    const uint64_t v = oracle_criticality_weight(path_delay_ps[2 * i], path_delay_ps[2 * i + 1],
+                                                 worst_path_delay_ps[2 * i], worst_path_delay_ps[2 * i + 1], exponent);
+    auto it = w.find({net[i], sink[i]});
+    if (it == w.end())
+      w[{net[i], sink[i]}] = v;
+    else
+      it->second = std::max(it->second, v);
+  }
+  uint64_t k = 0;
+  for (auto &kv : w) {
+    out_net[k] = kv.first.first;
+    out_sink[k] = kv.first.second;
+    out_weight[k] = kv.second;
+    ++k;
+  }
+  return k;
 }
 
 }  // extern "C"

@@ -244,6 +244,32 @@ int oracle_nesterov_step_lengths(oracle_nesterov_state *state, uint64_t n, uint6
                                  uint64_t n_upper, const double *upper, uint64_t n_metric, const double *metric,
                                  oracle_evaluate_fn evaluate, void *context, oracle_nesterov_result *result);
 
+/* ---- GP input builders (routing_demand.rs, texo-flow lib.rs) ------------ */
+/* channel_orientation (routing_demand.rs:185-198): 1 H, 2 V, 0 neither. */
+int oracle_channel_orientation(const char *name);
+/* routing_capacity_map (routing_demand.rs:201-239) over wires (orientation
+ * code, tile point, capacity) and pips (from, to, blocked). */
+int oracle_routing_capacity_map(uint32_t width, uint32_t height, uint32_t n_wires, const uint8_t *wire_orientation,
+                                const uint32_t *wire_xy, const uint16_t *wire_capacity, uint32_t n_pips,
+                                const uint32_t *pip_from, const uint32_t *pip_to, const uint8_t *pip_blocked,
+                                uint32_t *horizontal, uint32_t *vertical);
+/* routing_demand_map_with_capacity (routing_demand.rs:262-313): nets of
+ * pins at tile points (0xffffffff: unplaced), clock-driven flags; FP64
+ * demand summed in net order. */
+int oracle_routing_demand_map(uint32_t width, uint32_t height, uint32_t n_nets, const uint32_t *net_pin_start,
+                              const uint32_t *pin_xy, const uint8_t *net_clock, double *horizontal, double *vertical,
+                              uint64_t *included_nets, uint64_t *excluded_clock_nets);
+/* ecp5_placement_criticality_weight (texo-flow lib.rs:3355-3390) with
+ * 128-bit delays given as (lo, hi) words. */
+uint64_t oracle_criticality_weight(uint64_t path_lo, uint64_t path_hi, uint64_t worst_lo, uint64_t worst_hi,
+                                   uint32_t exponent);
+/* ecp5_timing_placement_weights (texo-flow lib.rs:3185-3201): the maximum
+ * weight per (net, sink) in (net, sink) order; returns the entry count. */
+uint64_t oracle_timing_placement_weights(uint64_t n_edges, const uint32_t *net, const uint32_t *sink,
+                                         const uint64_t *path_delay_ps, const uint64_t *worst_path_delay_ps,
+                                         uint32_t exponent, uint32_t *out_net, uint32_t *out_sink,
+                                         uint64_t *out_weight);
+
 #ifdef __cplusplus
 }
 #endif
