@@ -474,6 +474,21 @@ int texo_gp_routing_demand_map(int32_t device, uint32_t width, uint32_t height, 
                                double *horizontal, double *vertical, uint64_t *included_nets,
                                uint64_t *excluded_clock_nets);
 
+/* texo_gp_routing_demand_map without any device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code around HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results, bit for bit; the call's
+ * staged inputs, sums and result live in a 64 MiB device array of the library
+ * that each GPU holds for the life of the process, so a call whose layout
+ * exceeds it (about 4 bytes per net, 8 per pin and 32 per tile) returns
+ * TEXO_GP_ERR_OUT_OF_MEMORY, with the sizes in texo_gp_last_error().  Calls
+ * are serialised per device inside one process; calls from several processes
+ * on one GPU each use their own copy.  Synchronous, on the null stream. */
+int texo_gp_routing_demand_map_noalloc(int32_t device, uint32_t width, uint32_t height, uint32_t n_nets,
+                                       const uint32_t *net_pin_start, const uint32_t *pin_xy,
+                                       const uint8_t *net_clock, double *horizontal, double *vertical,
+                                       uint64_t *included_nets, uint64_t *excluded_clock_nets);
+
 /* ecp5_timing_placement_weights (crates/texo-flow/src/lib.rs:3185-3201) with
  * ecp5_placement_criticality_weight (:3355-3390): per timing edge (net,
  * sink pin, path delay, domain worst path delay; delays are 128-bit values

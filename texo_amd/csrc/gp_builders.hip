@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 
 namespace texo {
@@ -190,6 +191,31 @@ void enqueue_demand(char *ws, const DemandLayout &L, uint32_t width, uint32_t he
   HIP(hipMemcpyAsync(vertical, d_out + tiles, 8 * tiles, hipMemcpyDeviceToHost, s));
   HIP(hipMemcpyAsync(counts_host, d_counts, 16, hipMemcpyDeviceToHost, s));
 }
+
+// Scratch of the allocation-free demand map (texo_gp_routing_demand_map_noalloc):
+// staged inputs, fixed-point sums, the FP64 result and the net counters of
+// one call, laid out by DemandLayout.  Every GPU the library runs on holds
+// its own copy for the life of the process.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+// The device address of `scratch` on each device, looked up once per device
+// with that device current (the host symbol is only a stand-in); one lock
+// per device serialises this process's calls on it.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu[MAX_DEVICES];
+char *g_scratch_ptr[MAX_DEVICES] = {};
+
+char *scratch_on(int32_t device) {
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<char *>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 }  // namespace texo
 
@@ -393,6 +419,37 @@ int texo_gp_routing_demand_map(int32_t device, uint32_t width, uint32_t height, 
   }
   if (ws) (void)hipFree(ws);
   return rc;
+}
+
+int texo_gp_routing_demand_map_noalloc(int32_t device, uint32_t width, uint32_t height, uint32_t n_nets,
+                                       const uint32_t *net_pin_start, const uint32_t *pin_xy,
+                                       const uint8_t *net_clock, double *horizontal, double *vertical,
+                                       uint64_t *included_nets, uint64_t *excluded_clock_nets) {
+  if (!horizontal || !vertical || device < 0 || device >= texo::MAX_DEVICES) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  const int bad = texo::validate_demand(width, height, n_nets, net_pin_start, pin_xy, net_clock);
+  if (bad) return bad;
+  const uint64_t tiles = (uint64_t)width * height;
+  const texo::DemandLayout L(n_nets, net_pin_start[n_nets], tiles);
+  if (L.total > texo::NOALLOC_SCRATCH_BYTES) {
+    texo::g_last_error = "routing demand map needs " + std::to_string(L.total) + " bytes of device scratch; the " +
+                         "allocation-free entry point holds " + std::to_string(texo::NOALLOC_SCRATCH_BYTES);
+    return TEXO_GP_ERR_OUT_OF_MEMORY;
+  }
+  std::lock_guard<std::mutex> lk(texo::g_scratch_mu[device]);
+  try {
+    HIP(hipSetDevice(device));
+    char *ws = texo::scratch_on(device);
+    // the null stream: no stream object is created
+    unsigned long long counts[2] = {0, 0};
+    texo::enqueue_demand(ws, L, width, height, n_nets, net_pin_start, pin_xy, net_clock, horizontal, vertical, counts,
+                         nullptr);
+    HIP(hipStreamSynchronize(nullptr));
+    if (included_nets) *included_nets = counts[0];
+    if (excluded_clock_nets) *excluded_clock_nets = counts[1];
+    return 0;
+  } catch (...) {
+    return texo::current_failure().status;
+  }
 }
 
 uint64_t texo_gp_timing_placement_weights(uint64_t n_edges, const uint32_t *net, const uint32_t *sink,
