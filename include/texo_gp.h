@@ -345,6 +345,28 @@ int texo_gp_session_preconditioner(texo_gp_session *session, const double *multi
 int texo_gp_session_round_targets(texo_gp_session *session, const double *coordinates,
                                   uint32_t *out_targets_xy);
 
+/* The area proposal of an area round at `coordinates[C]`
+ * (propose_area_adjustment, eplace.rs:627-729): RUDY utilisation
+ * (ContinuousRoutingDemand::utilization, eplace.rs:822-854), routability
+ * areas (eplace.rs:741-758), register-clustering areas
+ * (register_clustering.rs:17-81) and adjust_instance_areas
+ * (instance_area.rs:157-372).  Writes the proposed member areas[n_members],
+ * filler areas[n_fillers], whether the optimizer would reset and the total
+ * relative area increase; applies nothing.  The problem needs a routing
+ * capacity map. */
+int texo_gp_session_area_proposal(texo_gp_session *session, const double *coordinates, double *member_areas,
+                                  double *filler_areas, int32_t *requires_reset,
+                                  double *total_relative_increase);
+
+/* clustering_areas (crates/texo-pnr/src/register_clustering.rs:17-81) on
+ * `device`: registers (cell, x, y), control sets (cell, clock, lsr, ce) and
+ * the movable instance count; areas[n_registers] aligned with the registers.
+ * TEXO_GP_ERR_INVALID_NORMALIZATION where the reference returns None. */
+int texo_gp_clustering_areas(int32_t device, uint64_t n_registers, const uint32_t *register_cell,
+                             const double *register_xy, uint64_t n_controls, const uint32_t *control_cell,
+                             const uint64_t *control_clock_lsr, const uint64_t *control_ce,
+                             uint64_t movable_instance_count, double *areas);
+
 /* Optimizer and schedule state after the last iterate/run call: density
  * multipliers and their step (eplace.rs:1126-1298), initial energies E0, the
  * wirelength gamma of the last iteration and the per-field gamma weights

@@ -2441,6 +2441,22 @@ int oracle_session_rudy(oracle_session *s, const double *coordinates, double *ho
   }
 }
 
+int oracle_session_area_proposal(oracle_session *s, const double *coordinates, double *member_areas,
+                                 double *filler_areas, int32_t *requires_reset, double *total_relative_increase) {
+  if (!s || s->empty || !s->P.has_routing) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    std::vector<double> x(coordinates, coordinates + s->coords.size());
+    AreaMetrics m = propose_area_adjustment(s->P, x);
+    std::copy(m.adjustment.member_areas.begin(), m.adjustment.member_areas.end(), member_areas);
+    std::copy(m.adjustment.filler_areas.begin(), m.adjustment.filler_areas.end(), filler_areas);
+    if (requires_reset) *requires_reset = m.adjustment.requires_reset ? 1 : 0;
+    if (total_relative_increase) *total_relative_increase = m.total_relative_increase;
+    return 0;
+  } catch (const Failure &f) {
+    return f.status;
+  }
+}
+
 int oracle_session_schedule(const oracle_session *s, texo_gp_schedule_state *out) {
   if (!s || !out) return TEXO_GP_ERR_INVALID_ARGUMENT;
   std::memset(out, 0, sizeof(*out));

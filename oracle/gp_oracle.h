@@ -51,6 +51,10 @@ int oracle_session_rudy(oracle_session *session, const double *coordinates,
                         double *horizontal, double *vertical);
 /* Continuous bbox HPWL of the retained nets at `coordinates` (FP64). */
 double oracle_session_hpwl(oracle_session *session, const double *coordinates);
+/* propose_area_adjustment (eplace.rs:627-729) at `coordinates`, as
+ * texo_gp_session_area_proposal. */
+int oracle_session_area_proposal(oracle_session *session, const double *coordinates, double *member_areas,
+                                 double *filler_areas, int32_t *requires_reset, double *total_relative_increase);
 /* Schedule and optimizer state after oracle_session_iterate (multipliers,
  * their step, E0 scales, last gamma, gamma weights, momentum, Lipschitz). */
 int oracle_session_schedule(const oracle_session *session, texo_gp_schedule_state *state);

@@ -291,6 +291,7 @@ def declare_session_api(lib, prefix: str):
     f("session_combined", C.c_int, sess, P(d), d, P(d), P(d), P(d), P(d))
     f("session_preconditioner", C.c_int, sess, P(d), P(d))
     f("session_schedule", C.c_int, sess, P(texo_gp_schedule_state))
+    f("session_area_proposal", C.c_int, sess, P(d), P(d), P(d), P(C.c_int32), P(d))
 
 
 class Session:
@@ -420,6 +421,20 @@ class Session:
         if rc != 0:
             raise PlacementError.from_status(rc)
         return d
+
+    def area_proposal(self, coordinates: np.ndarray) -> dict:
+        """propose_area_adjustment at `coordinates` (nothing is applied)."""
+        x = np.ascontiguousarray(coordinates, dtype=np.float64)
+        ma = np.zeros(max(self.problem.n_members, 1))
+        fa = np.zeros(max(self.n_fillers, 1))
+        reset, inc = C.c_int32(), C.c_double()
+        rc = getattr(self.lib, f"{self.prefix}_session_area_proposal")(
+            self.handle, ptr(x, C.c_double), ptr(ma, C.c_double), ptr(fa, C.c_double), C.byref(reset),
+            C.byref(inc))
+        if rc != 0:
+            raise PlacementError.from_status(rc)
+        return {"member_areas": ma[:self.problem.n_members], "filler_areas": fa[:self.n_fillers],
+                "requires_reset": bool(reset.value), "total_relative_increase": inc.value}
 
     def schedule(self) -> dict:
         st = texo_gp_schedule_state()

@@ -488,6 +488,26 @@ int texo_gp_session_round_targets(texo_gp_session *s, const double *coordinates,
   return rc;
 }
 
+int texo_gp_session_area_proposal(texo_gp_session *s, const double *coordinates, double *member_areas,
+                                  double *filler_areas, int32_t *requires_reset, double *total_relative_increase) {
+  if (!s || !coordinates || !member_areas || !filler_areas) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  Job &J = s->job;
+  if (J.empty || !J.has_routing) return TEXO_GP_ERR_INVALID_ARGUMENT;
+  try {
+    Engine &E = *s->engine;
+    const hipStream_t st = E.stream();
+    HIP(hipMemcpyAsync(J.d_eval_coords, coordinates, J.C * sizeof(double), hipMemcpyHostToDevice, st));
+    const Engine::AreaProposal p = E.propose_areas(&J, J.d_eval_coords, st);
+    std::copy(p.member_areas.begin(), p.member_areas.end(), member_areas);
+    std::copy(p.filler_areas.begin(), p.filler_areas.end(), filler_areas);
+    if (requires_reset) *requires_reset = p.requires_reset ? 1 : 0;
+    if (total_relative_increase) *total_relative_increase = p.total_relative_increase;
+    return 0;
+  } catch (...) {
+    return texo::current_failure().status;
+  }
+}
+
 int texo_gp_session_schedule(const texo_gp_session *s, texo_gp_schedule_state *out) {
   if (!s || !out) return TEXO_GP_ERR_INVALID_ARGUMENT;
   const Job &J = s->job;

@@ -585,23 +585,23 @@ void Engine::plan(Job *j) {
   }
 }
 
-// propose_area_adjustment / apply_area_adjustment (eplace.rs:627-819): RUDY
+// propose_area_adjustment (eplace.rs:627-729): RUDY and register clustering
 // on the device, the order-dependent charge bookkeeping on the host.
-void Engine::area_round(Job *j) {
-  await_uploads(cur_);
-  HIP(hipMemsetAsync(j->d_rudy_err, 0, sizeof(unsigned int), cur()));
-  launch_rudy((int)j->n_nets, j->d_net_start, j->d_pin_unit, j->js.pin_xy, j->d_vec[j->ix], (int)j->W, (int)j->H,
-              j->d_rudy_fx, j->d_rudy, j->d_rudy_err, cur());
+Engine::AreaProposal Engine::propose_areas(Job *j, const double *coords, hipStream_t s) {
+  HIP(hipMemsetAsync(j->d_rudy_err, 0, sizeof(unsigned int), s));
+  launch_rudy((int)j->n_nets, j->d_net_start, j->d_pin_unit, j->js.pin_xy, coords, (int)j->W, (int)j->H, j->d_rudy_fx,
+              j->d_rudy, j->d_rudy_err, s);
   const size_t G = j->G, U = j->U;
   std::vector<double> demand(2 * G), x(2 * U);
   unsigned int err = 0;
-  HIP(hipMemcpyAsync(demand.data(), j->d_rudy, 2 * G * sizeof(double), hipMemcpyDeviceToHost, cur()));
-  HIP(hipMemcpyAsync(x.data(), j->d_vec[j->ix], 2 * U * sizeof(double), hipMemcpyDeviceToHost, cur()));
-  HIP(hipMemcpyAsync(&err, j->d_rudy_err, sizeof(unsigned int), hipMemcpyDeviceToHost, cur()));
-  HIP(hipStreamSynchronize(cur()));
+  HIP(hipMemcpyAsync(demand.data(), j->d_rudy, 2 * G * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP(hipMemcpyAsync(x.data(), coords, 2 * U * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP(hipMemcpyAsync(&err, j->d_rudy_err, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  HIP(hipStreamSynchronize(s));
   if (err) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
   std::map<uint32_t, double> clustering;
   if (!j->controls.empty()) {
+    // propose_register_clustering_areas (eplace.rs:760-791)
     std::vector<std::pair<uint32_t, std::pair<double, double>>> regs;
     for (size_t i = 0; i < j->DU; ++i) {
       const uint32_t u = j->du_unit[i];
@@ -610,8 +610,9 @@ void Engine::area_round(Job *j) {
           regs.push_back({j->m_cell[m], {x[2 * u] + j->m_off[2 * m], x[2 * u + 1] + j->m_off[2 * m + 1]}});
     }
     std::vector<double> areas;
-    if (!register_clustering(regs, j->controls, j->movable_instance_count, areas))
-      fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+    int ok = register_clustering_device(regs, j->controls, j->movable_instance_count, areas, blocks_.get(), s);
+    if (ok < 0) ok = register_clustering(regs, j->controls, j->movable_instance_count, areas) ? 1 : 0;
+    if (!ok) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
     for (size_t i = 0; i < regs.size(); ++i) clustering[regs[i].first] = areas[i];
   }
   // ContinuousRoutingDemand::utilization (eplace.rs:822-854)
@@ -643,19 +644,30 @@ void Engine::area_round(Job *j) {
   }
   std::vector<std::pair<uint32_t, double>> fillers(j->Fl);
   for (size_t k = 0; k < j->Fl; ++k) fillers[k] = {j->field_kind[j->fl_field[k]], j->fl_charge[k]};
-  std::vector<double> ma, fa;
-  const bool requires_reset = adjust_areas(members, fillers, ma, fa);
+  AreaProposal p;
+  p.requires_reset = adjust_areas(members, fillers, p.member_areas, p.filler_areas);
   std::map<uint32_t, double> before, inc;
   for (size_t m = 0; m < j->M; ++m) {
     before[members[m].kind] += members[m].current;
-    inc[members[m].kind] += ma[m] - members[m].current;
+    inc[members[m].kind] += p.member_areas[m] - members[m].current;
   }
   double bt = 0.0, it = 0.0;
   for (auto &kv : before) bt += kv.second;
   for (auto &kv : inc) it += kv.second;
-  const double total_inc = it / bt;
-  if (!std::isfinite(total_inc)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
-  if (!requires_reset) {
+  p.total_relative_increase = it / bt;
+  if (!std::isfinite(p.total_relative_increase)) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
+  return p;
+}
+
+// The area round of place() (eplace.rs:384-419): propose, then either
+// restart with the same areas (no growth) or install them
+// (apply_area_adjustment, eplace.rs:793-819).
+void Engine::area_round(Job *j) {
+  await_uploads(cur_);
+  const AreaProposal p = propose_areas(j, j->d_vec[j->ix], cur());
+  const std::vector<double> &ma = p.member_areas, &fa = p.filler_areas;
+  const double total_inc = p.total_relative_increase;
+  if (!p.requires_reset) {
     j->has_prev_change = true;
     j->prev_change = 0.0;
     j->reset_after_area = true;

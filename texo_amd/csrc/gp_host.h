@@ -145,6 +145,12 @@ struct AreaMember {
 // clustering_areas (register_clustering.rs:17-81); false is the reference's None.
 bool register_clustering(const std::vector<std::pair<uint32_t, std::pair<double, double>>> &regs,
                          const std::map<uint32_t, Control> &controls, uint64_t movable, std::vector<double> &areas);
+// The same on the GPU (gp_cluster.hip): 1 done, 0 the reference's None, -1
+// a group with more clock-enable classes than the kernel holds (use the host
+// version).  Device memory comes from `pool` when given.
+int register_clustering_device(const std::vector<std::pair<uint32_t, std::pair<double, double>>> &regs,
+                               const std::map<uint32_t, Control> &controls, uint64_t movable,
+                               std::vector<double> &areas, BlockPool *pool, hipStream_t s);
 // adjust_instance_areas (instance_area.rs:157-372); returns requires_optimizer_reset.
 bool adjust_areas(const std::vector<AreaMember> &members, const std::vector<std::pair<uint32_t, double>> &fillers,
                   std::vector<double> &member_out, std::vector<double> &filler_out);
@@ -402,6 +408,15 @@ class Engine {
   double last_wait_seconds() const { return last_wait_; }
   // Parity evaluation of one job at host coordinates.
   void evaluate_once(Job *job, const double *coords, double gamma, bool density_out);
+  // propose_area_adjustment (eplace.rs:627-729) at device coordinates
+  // `coords`: RUDY and register clustering on the GPU, utilisation,
+  // routability areas and adjust_instance_areas on the host.
+  struct AreaProposal {
+    std::vector<double> member_areas, filler_areas;
+    bool requires_reset = false;
+    double total_relative_increase = 0.0;
+  };
+  AreaProposal propose_areas(Job *job, const double *coords, hipStream_t s);
   // Preconditioner of one job for `lambda` into job->d_metric (parity seam);
   // returns false when an entry is not finite or negative before the floor.
   bool precondition_once(Job *job, const std::vector<double> &lambda);
