@@ -310,10 +310,12 @@ int texo_gp_session_rudy(texo_gp_session *session, const double *coordinates,
                          double *horizontal, double *vertical);
 
 /* Kernel timing: when enabled, each launch of the named kernel ("deposit",
- * "wa", "gather", "dct", "rudy") is bracketed by HIP events on the stream it
- * runs on; `avg_ms` is the mean duration, `count` the number of launches
- * since the last reset and `avg_tasks` (may be null) the mean number of
- * evaluations one launch covered. */
+ * "wa", "gather", "dct", "rudy", "optimizer") is bracketed by HIP events on
+ * the stream it runs on; `avg_ms` is the mean duration, `count` the number
+ * of launches since the last reset and `avg_tasks` (may be null) the mean
+ * number of evaluations one launch covered (for "optimizer": the mean number
+ * of coordinate-long vectors it read or wrote).  Timed rounds are issued
+ * launch by launch, never replayed from a captured graph. */
 int texo_gp_session_set_kernel_timing(texo_gp_session *session, const char *kernel,
                                       int enable);
 int texo_gp_session_kernel_timing(texo_gp_session *session, double *avg_ms,
@@ -437,6 +439,7 @@ typedef struct texo_gp_server_stats {
   double retire_seconds;    /* server-thread time reading back and releasing finished jobs */
   double wait_seconds;      /* part of busy_seconds spent waiting for the GPU to end a round */
   double launch_seconds;    /* server-thread time planning rounds and enqueueing their work */
+  uint64_t graph_rounds;    /* rounds replayed from a captured HIP graph */
 } texo_gp_server_stats;
 
 texo_gp_server *texo_gp_server_create(int32_t device, uint64_t hbm_budget_bytes, uint32_t max_batch);

@@ -471,7 +471,7 @@ class texo_gp_server_stats(C.Structure):
         ("failed", C.c_uint64), ("rounds", C.c_uint64), ("active", C.c_uint64),
         ("pending", C.c_uint64), ("hbm_in_use", C.c_uint64), ("busy_seconds", C.c_double),
         ("admit_seconds", C.c_double), ("retire_seconds", C.c_double),
-        ("wait_seconds", C.c_double), ("launch_seconds", C.c_double),
+        ("wait_seconds", C.c_double), ("launch_seconds", C.c_double), ("graph_rounds", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

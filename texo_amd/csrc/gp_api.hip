@@ -1113,6 +1113,7 @@ int texo_gp_server_stats_get(texo_gp_server *s, texo_gp_server_stats *stats) {
   stats->active = s->active.size();
   stats->pending = s->pending.size();
   stats->hbm_in_use = s->hbm_in_use;
+  stats->graph_rounds = s->engine->graph_rounds();
   return 0;
 }
 

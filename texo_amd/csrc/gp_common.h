@@ -376,7 +376,9 @@ struct OptMaps {
   BlockPlan precond, extrapolate, candidate;
   int n_tasks;  // one finalize block per task
 };
-void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s);
+// `passes`: the C-long vectors the launch reads or writes, summed over its
+// tasks (preconditioner 2, extrapolation 5, candidate 7), for kernel timing.
+void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s, int passes = 0);
 // The optimizer's scalars (read by the host only): may run beside the density chain.
 void launch_optimizer_finalize(const OptTask *tasks, const OptMaps &m, hipStream_t s);
 

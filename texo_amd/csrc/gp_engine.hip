@@ -20,6 +20,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <unordered_map>
 #include <array>
 #include <tuple>
 
@@ -271,6 +274,12 @@ void Job::begin_round() {
 // ---------------------------------------------------------------------------
 Engine::Engine(int device, size_t pool_capacity) : device_(device) {
   HIP(hipSetDevice(device));
+  // Replaying recurring rounds from captured graphs is opt-in: measured on
+  // MI355X with ROCm 7.2 it gave single jobs no gain and slowed the
+  // two-context lockstep batch by 8% (hipGraphLaunch holds the host about as
+  // long as the launches it replaces).
+  const char *graphs = std::getenv("TEXO_GP_GRAPHS");
+  use_graphs_ = graphs && graphs[0] == '1';
   // every stream keeps the default priority
   for (RoundCtx &c : ctx_) {
     HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -411,6 +420,7 @@ void Engine::ensure_stage(RoundCtx &c, size_t bytes) {
   if (bytes <= c.stage_bytes) return;
   size_t cap = std::max<size_t>(bytes, 2 * c.stage_bytes);
   cap = (cap + 4095) & ~size_t(4095);
+  drop_graphs(c);  // their kernel arguments point into the old buffers
   if (c.d_stage) HIP(hipFree(c.d_stage));
   if (c.h_stage) HIP(hipHostFree(c.h_stage));
   c.d_stage = nullptr;
@@ -423,6 +433,7 @@ void Engine::ensure_stage(RoundCtx &c, size_t bytes) {
 void Engine::ensure_compact(RoundCtx &c, size_t n) {
   if (n <= c.compact_cap) return;
   const size_t cap = std::max<size_t>(n, 2 * c.compact_cap);
+  drop_graphs(c);
   if (c.h_compact) HIP(hipHostFree(c.h_compact));
   c.d_compact = nullptr;
   c.h_compact = nullptr;
@@ -433,7 +444,14 @@ void Engine::ensure_compact(RoundCtx &c, size_t n) {
   c.compact_cap = cap;
 }
 
+void Engine::drop_graphs(RoundCtx &c) {
+  for (auto &kv : c.graphs) (void)hipGraphExecDestroy(kv.second);
+  c.graphs.clear();
+  c.seen.clear();
+}
+
 void Engine::free_ctx(RoundCtx &c) {
+  drop_graphs(c);
   if (c.d_stage) (void)hipFree(c.d_stage);
   if (c.h_stage) (void)hipHostFree(c.h_stage);
   if (c.h_compact) (void)hipHostFree(c.h_compact);
@@ -1133,11 +1151,6 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   om.candidate = p_cand.device(d_stage_, o_pcand);
   om.n_tasks = (int)opt.size();
   const OptTask *d_opt = reinterpret_cast<const OptTask *>(dptr(o_opt));
-  if (!opt.empty()) launch_optimizer(d_opt, om, cx.stream);
-  // The second stream forks after the optimizer's vectors: it reduces the
-  // optimizer's scalars (only the host reads them) and runs WA (it depends
-  // only on coordinates) beside the sort and the density chain, and joins
-  // before the combination.
   Maps mp{};
   mp.deposit = p_dep.device(d_stage_, o_pdep);
   mp.convert = p_conv.device(d_stage_, o_pconv);
@@ -1145,31 +1158,114 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   mp.gradient = p_gr.device(d_stage_, o_pgr);
   mp.n_tasks = (int)tasks.size();
   mp.n_density = n_density;
-  HIP(hipEventRecord(cx.fork, cx.stream));
-  HIP(hipStreamWaitEvent(cx.wa_stream, cx.fork, 0));
-  if (!opt.empty()) launch_optimizer_finalize(d_opt, om, cx.wa_stream);
-  launch_wa(d_tasks, mp, cx.wa_stream);
-  HIP(hipEventRecord(cx.join, cx.wa_stream));
+  SortMaps sm{};
   if (!sorts.empty()) {
-    SortMaps sm{};
     sm.charges = p_sort_c.device(d_stage_, o_psc);
     sm.keys = p_sort_k.device(d_stage_, o_psk);
     sm.list = reinterpret_cast<const int *>(dptr(o_lsort));
     sm.n_list = (int)l_sort.size();
-    launch_sort(reinterpret_cast<const SortTask *>(dptr(o_sorts)), sm, cx.stream);
   }
-  launch_density(d_tasks, mp, cx.stream);
-  size_t gi = 0;
-  for (auto &kv : groups) {
-    const size_t off[4] = {o_groups[gi][0], o_groups[gi][1], o_groups[gi][2], o_groups[gi][3]};
-    launch_poisson(d_tasks, kv.second.device(d_stage_, off), cx.stream);
-    ++gi;
+  std::vector<DctLaunch> dct;
+  {
+    size_t gi = 0;
+    for (auto &kv : groups) {
+      const size_t off[4] = {o_groups[gi][0], o_groups[gi][1], o_groups[gi][2], o_groups[gi][3]};
+      dct.push_back(kv.second.device(d_stage_, off));
+      ++gi;
+    }
   }
-  launch_gather(d_tasks, mp, cx.stream);
-  HIP(hipStreamWaitEvent(cx.stream, cx.join, 0));
-  launch_combine(d_tasks, mp, cx.stream);
-  launch_collect_scalars(reinterpret_cast<const int *>(dptr(o_pool)), (int)active.size(), d_pool_, cx.d_compact,
-                         cx.stream);
+  int opt_passes = 0;
+  for (const OptTask &o : opt) opt_passes += 2 * o.do_precond + 5 * o.do_extrapolate + 7 * o.do_candidate;
+  const int *d_pool_index = reinterpret_cast<const int *>(dptr(o_pool));
+  const SortTask *d_sorts = reinterpret_cast<const SortTask *>(dptr(o_sorts));
+  auto issue = [&] {
+    if (!opt.empty()) launch_optimizer(d_opt, om, cx.stream, opt_passes);
+    // The second stream forks after the optimizer's vectors: it reduces the
+    // optimizer's scalars (only the host reads them) and runs WA (it depends
+    // only on coordinates) beside the sort and the density chain, and joins
+    // before the combination.
+    HIP(hipEventRecord(cx.fork, cx.stream));
+    HIP(hipStreamWaitEvent(cx.wa_stream, cx.fork, 0));
+    if (!opt.empty()) launch_optimizer_finalize(d_opt, om, cx.wa_stream);
+    launch_wa(d_tasks, mp, cx.wa_stream);
+    HIP(hipEventRecord(cx.join, cx.wa_stream));
+    if (!sorts.empty()) launch_sort(d_sorts, sm, cx.stream);
+    launch_density(d_tasks, mp, cx.stream);
+    for (const DctLaunch &d : dct) launch_poisson(d_tasks, d, cx.stream);
+    launch_gather(d_tasks, mp, cx.stream);
+    HIP(hipStreamWaitEvent(cx.stream, cx.join, 0));
+    launch_combine(d_tasks, mp, cx.stream);
+    launch_collect_scalars(d_pool_index, (int)active.size(), d_pool_, cx.d_compact, cx.stream);
+  };
+  // A round's launch sequence is fully described by the by-value kernel
+  // arguments below (every table it reads sits in the staging buffer, copied
+  // above).  Sequences that recur (a single job's iteration rounds, a stable
+  // batch) replay as one captured graph: the same kernels with the same
+  // arguments, so results do not change, with one launch call instead of a
+  // dozen.  Kernel timing needs its events between launches: no graphs then.
+  if (use_graphs_ && !timer.enabled) {
+    std::string key;
+    key.reserve(512);
+    auto add = [&](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
+    auto add_plan = [&](const BlockPlan &b) {
+      add(&b.start, sizeof b.start);
+      add(&b.n_tasks, sizeof b.n_tasks);
+      add(&b.total, sizeof b.total);
+      add(&b.per, sizeof b.per);
+    };
+    const size_t n_opt = opt.size(), n_sorts = sorts.size(), n_active = active.size();
+    add(&n_opt, sizeof n_opt);
+    add(&n_sorts, sizeof n_sorts);
+    add(&n_active, sizeof n_active);
+    add(&d_stage_, sizeof d_stage_);
+    add(&cx.d_compact, sizeof cx.d_compact);
+    for (const BlockPlan *b : {&om.precond, &om.extrapolate, &om.candidate, &mp.deposit, &mp.convert, &mp.wa,
+                               &mp.gradient, &sm.charges, &sm.keys})
+      add_plan(*b);
+    add(&mp.n_tasks, sizeof mp.n_tasks);
+    add(&mp.n_density, sizeof mp.n_density);
+    add(&sm.list, sizeof sm.list);
+    add(&sm.n_list, sizeof sm.n_list);
+    for (const DctLaunch &d : dct) {
+      for (const BlockPlan *b : {&d.h, &d.w, &d.h_wide, &d.w_wide}) add_plan(*b);
+      add(&d.precision, sizeof d.precision);
+      add(&d.kmax, sizeof d.kmax);
+      add(&d.n_tasks, sizeof d.n_tasks);
+      const char flags[2] = {d.fast ? '1' : '0', d.wide_ok ? '1' : '0'};
+      add(flags, 2);
+    }
+    auto hit = cx.graphs.find(key);
+    if (hit == cx.graphs.end() && ++cx.seen[key] >= GRAPH_AFTER_ROUNDS) {
+      if (cx.graphs.size() >= GRAPH_CACHE) {
+        for (auto &kv : cx.graphs) (void)hipGraphExecDestroy(kv.second);
+        cx.graphs.clear();
+      }
+      if (cx.seen.size() > 64 * GRAPH_CACHE) cx.seen.clear();
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIP(hipStreamBeginCapture(cx.stream, hipStreamCaptureModeThreadLocal));
+      try {
+        issue();
+      } catch (...) {
+        (void)hipStreamEndCapture(cx.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      HIP(hipStreamEndCapture(cx.stream, &graph));
+      const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP(ie);
+      hit = cx.graphs.emplace(std::move(key), exec).first;
+    }
+    if (hit != cx.graphs.end()) {
+      HIP(hipGraphLaunch(hit->second, cx.stream));
+      graph_rounds_ += 1;
+    } else {
+      issue();
+    }
+  } else {
+    issue();
+  }
   HIP(hipGetLastError());
   HIP(hipEventRecord(cx.done, cx.stream));
   cx.pending = true;

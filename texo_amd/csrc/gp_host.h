@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -406,6 +407,8 @@ class Engine {
   bool pending(int c) const { return ctx_[c].pending; }
   // Seconds the last finish() spent waiting for the GPU to end the round.
   double last_wait_seconds() const { return last_wait_; }
+  // Rounds replayed from a captured graph since the engine started.
+  uint64_t graph_rounds() const { return graph_rounds_; }
   // Parity evaluation of one job at host coordinates.
   void evaluate_once(Job *job, const double *coords, double gamma, bool density_out);
   // propose_area_adjustment (eplace.rs:627-729) at device coordinates
@@ -461,7 +464,18 @@ class Engine {
     hipEvent_t done = nullptr;
     std::vector<Job *> active;
     bool pending = false;
+    // captured launch sequences by their argument bytes, and how often each
+    // sequence was issued before capture
+    std::unordered_map<std::string, hipGraphExec_t> graphs;
+    std::unordered_map<std::string, int> seen;
   };
+  // A sequence is captured on its GRAPH_AFTER_ROUNDS-th occurrence; at most
+  // GRAPH_CACHE graphs are kept per context.
+  static constexpr int GRAPH_AFTER_ROUNDS = 3;
+  static constexpr size_t GRAPH_CACHE = 64;
+  bool use_graphs_ = false;  // TEXO_GP_GRAPHS=1 replays recurring rounds from captured graphs
+  uint64_t graph_rounds_ = 0;
+  void drop_graphs(RoundCtx &c);
   RoundCtx ctx_[ROUND_CONTEXTS], aux_;  // aux_: set-up sorts and parity evaluations (on context 0's stream)
   void ensure_stage(RoundCtx &c, size_t bytes);
   void ensure_compact(RoundCtx &c, size_t n);

@@ -1275,9 +1275,12 @@ void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
 
-void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
+void launch_optimizer(const OptTask *tasks, const OptMaps &m, hipStream_t s, int passes) {
   const int grid = plan_grid(m.precond) + plan_grid(m.extrapolate) + plan_grid(m.candidate);
-  if (grid > 0) hipLaunchKernelGGL(k_optimizer, dim3(grid), dim3(BLOCK), 0, s, tasks, m);
+  if (grid <= 0) return;
+  timer_begin(g_timer, "optimizer", s);
+  hipLaunchKernelGGL(k_optimizer, dim3(grid), dim3(BLOCK), 0, s, tasks, m);
+  timer_end(g_timer, "optimizer", s, passes);
 }
 
 void launch_optimizer_finalize(const OptTask *tasks, const OptMaps &m, hipStream_t s) {
