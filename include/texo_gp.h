@@ -539,6 +539,32 @@ uint64_t texo_gp_timing_placement_weights(uint64_t n_edges, const uint32_t *net,
 int texo_gp_placement_model(uint32_t algorithm, uint32_t weight_exponent, const texo_gp_options *options,
                             char *out, size_t capacity);
 
+/* ---- coarse quadratic seed ----------------------------------------------- */
+/* The initial coordinates of analytical placement (analytical_place,
+ * crates/texo-pnr/src/lib.rs:4042-4066): per axis, the bound-to-bound edges
+ * at the device centre (AnalyticalHypergraph::linearize_axis,
+ * analytical_placement.rs:283-356), the axis equations
+ * (analytical_axis_equations, lib.rs:4464-4527) and their Jacobi-PCG solve
+ * (solve_analytical_axis, lib.rs:4529-4556), both axes on `device`.
+ * out_xy[2*n_units] interleaved x, y.  report->iterations = PCG iterations
+ * of x plus y.  TEXO_GP_ERR_DID_NOT_CONVERGE where the reference's
+ * solve_quadratic fails (err_index = iterations, err_index2 = axis,
+ * err_kind = 1 for a breakdown, 0 for the dimension bound, err_value[0] =
+ * relative residual). */
+int texo_gp_quadratic_seed(const texo_gp_problem *problem, int32_t device, double *out_xy,
+                           texo_gp_report *report);
+
+/* solve_analytical_axis (lib.rs:4529-4556) of explicit axis equations on
+ * `device`: diagonal[n], rhs[n], anchored[n], the off-diagonal terms as a
+ * CSR in insertion order (adjacency_start[n+1], adjacency_other,
+ * adjacency_weight), the starting solution initial[n] and the device centre
+ * that floating components are re-centred on.  `iterations` receives the PCG
+ * iterations, also on DID_NOT_CONVERGE. */
+int texo_gp_solve_axis(int32_t device, uint32_t n, const double *diagonal, const uint32_t *adjacency_start,
+                       const uint32_t *adjacency_other, const double *adjacency_weight, const double *rhs,
+                       const uint8_t *anchored, const double *initial, uint32_t center, double *solution,
+                       uint64_t *iterations);
+
 /* Last error message of the calling thread (static storage). */
 const char *texo_gp_last_error(void);
 

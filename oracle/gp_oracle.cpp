@@ -936,6 +936,279 @@ void continuous_routing_demand(uint32_t width, uint32_t height, const std::vecto
 }
 
 // ---------------------------------------------------------------------------
+// Coarse quadratic seed (analytical_place, crates/texo-pnr/src/lib.rs:4042-4066).
+// ---------------------------------------------------------------------------
+constexpr double PLACEMENT_WEIGHT_SCALE = 64.0;  // analytical_placement.rs:13
+
+struct AxisEdge {
+  uint32_t left, right, left_cell, right_cell;
+  double weight;
+  bool timing;
+};
+
+// axis_bounds (analytical_placement.rs:793-815): stable lowest / highest pin
+// (total_cmp, then index); a collapsed pair on one unit takes the last pin of
+// another unit as the upper bound.
+bool axis_bounds(const Pin *pins, size_t n, const double *pos, size_t &lower, size_t &upper) {
+  if (n == 0) return false;
+  bool same = true;
+  for (size_t i = 1; i < n; ++i) same &= pins[i].unit == pins[0].unit;
+  if (same) return false;
+  auto cmp = [&](size_t a, size_t b) {  // total_cmp then index
+    if (pos[a] < pos[b]) return -1;
+    if (pos[a] > pos[b]) return 1;
+    return a < b ? -1 : (a > b ? 1 : 0);
+  };
+  lower = 0;
+  upper = 0;
+  for (size_t i = 1; i < n; ++i) {
+    if (cmp(i, lower) < 0) lower = i;
+    if (cmp(i, upper) > 0) upper = i;  // max_by keeps the last maximum
+  }
+  if (pos[lower] == pos[upper] && pins[lower].unit == pins[upper].unit) {
+    bool found = false;
+    for (size_t i = n; i-- > 0;)
+      if (pins[i].unit != pins[lower].unit) {
+        upper = i;
+        found = true;
+        break;
+      }
+    if (!found) return false;
+  }
+  return true;
+}
+
+// linearize_axis (analytical_placement.rs:283-356): bound-to-bound edges of
+// every net at `positions` (one axis), then the timing overlays.
+std::vector<AxisEdge> linearize_axis(const Hypergraph &hg, const std::vector<double> &positions,
+                                     const std::vector<uint8_t> &fixed, const std::vector<double> &fixed_coord,
+                                     const std::vector<double> &offsets) {
+  auto coord = [&](Pin p) { return fixed[p.cell] ? fixed_coord[p.cell] : positions[p.unit] + offsets[p.cell]; };
+  std::vector<AxisEdge> edges;
+  std::vector<double> pp;
+  for (size_t n = 0; n < hg.n_nets(); ++n) {
+    const uint32_t b = hg.net_start[n], e = hg.net_start[n + 1];
+    const Pin *pins = hg.pins.data() + b;
+    const size_t m = e - b;
+    pp.resize(m);
+    for (size_t i = 0; i < m; ++i) pp[i] = coord(pins[i]);
+    size_t lower, upper;
+    if (!axis_bounds(pins, m, pp.data(), lower, upper)) continue;
+    const uint32_t fanout = (uint32_t)std::max<size_t>(m - 1, 1);
+    for (size_t i = 0; i < m; ++i)
+      for (size_t bound : {lower, upper}) {
+        if (i == bound) continue;
+        if (i == upper && bound == lower) continue;
+        const Pin l = pins[i], r = pins[bound];
+        if (l.unit == r.unit) continue;
+        const double separation = std::max(std::fabs(pp[i] - pp[bound]), 1.0);
+        edges.push_back({l.unit, r.unit, l.cell, r.cell, PLACEMENT_WEIGHT_SCALE / ((double)fanout * separation), false});
+      }
+  }
+  for (auto &a : hg.arcs) {
+    if (a.driver.unit == a.sink.unit) continue;
+    const double separation = std::max(std::fabs(coord(a.driver) - coord(a.sink)), 1.0);
+    edges.push_back({a.driver.unit, a.sink.unit, a.driver.cell, a.sink.cell,
+                     PLACEMENT_WEIGHT_SCALE * u64_to_f64(a.extra) / separation, true});
+  }
+  return edges;
+}
+
+struct AxisEquations {
+  std::vector<double> diagonal, rhs;
+  std::vector<std::vector<std::pair<uint32_t, double>>> adjacency;
+  std::vector<uint8_t> anchored;
+};
+
+// analytical_axis_equations (lib.rs:4464-4527)
+AxisEquations axis_equations(size_t n_units, const std::vector<uint8_t> &unit_fixed,
+                             const std::vector<double> &unit_origin, const std::vector<double> &fixed_coord,
+                             const std::vector<double> &offsets, const std::vector<AxisEdge> &edges) {
+  AxisEquations q;
+  q.diagonal.assign(n_units, 0.0);
+  q.rhs.assign(n_units, 0.0);
+  q.adjacency.assign(n_units, {});
+  q.anchored.assign(n_units, 0);
+  for (size_t u = 0; u < n_units; ++u)
+    if (unit_fixed[u]) {
+      q.diagonal[u] = 1.0;
+      q.rhs[u] = unit_origin[u];
+      q.anchored[u] = 1;
+    }
+  for (auto &e : edges) {
+    const double lo = offsets[e.left_cell], ro = offsets[e.right_cell], w = e.weight;
+    const bool lf = unit_fixed[e.left], rf = unit_fixed[e.right];
+    if (lf && rf) continue;
+    if (lf) {
+      q.diagonal[e.right] += w;
+      q.rhs[e.right] += w * (fixed_coord[e.left_cell] - ro);
+      q.anchored[e.right] = 1;
+    } else if (rf) {
+      q.diagonal[e.left] += w;
+      q.rhs[e.left] += w * (fixed_coord[e.right_cell] - lo);
+      q.anchored[e.left] = 1;
+    } else {
+      q.diagonal[e.left] += w;
+      q.diagonal[e.right] += w;
+      q.adjacency[e.left].push_back({e.right, w});
+      q.adjacency[e.right].push_back({e.left, w});
+      q.rhs[e.left] += w * (ro - lo);
+      q.rhs[e.right] += w * (lo - ro);
+    }
+  }
+  return q;
+}
+
+// AxisEquations::finalize_component_gauges (lib.rs:4419-4461)
+std::vector<std::vector<uint32_t>> finalize_component_gauges(AxisEquations &q) {
+  const size_t n = q.diagonal.size();
+  std::vector<uint8_t> visited(n, 0), gauge(n, 0);
+  std::vector<std::vector<uint32_t>> floating;
+  for (size_t start = 0; start < n; ++start) {
+    if (visited[start]) continue;
+    visited[start] = 1;
+    std::vector<uint32_t> pending{(uint32_t)start}, component;
+    while (!pending.empty()) {
+      const uint32_t u = pending.back();
+      pending.pop_back();
+      component.push_back(u);
+      for (auto &ow : q.adjacency[u])
+        if (!visited[ow.first]) {
+          visited[ow.first] = 1;
+          pending.push_back(ow.first);
+        }
+    }
+    std::sort(component.begin(), component.end());
+    bool anchored = false;
+    for (uint32_t u : component) anchored |= q.anchored[u] != 0;
+    if (anchored) continue;
+    gauge[component[0]] = 1;
+    floating.push_back(std::move(component));
+  }
+  for (size_t u = 0; u < n; ++u) {
+    if (gauge[u]) {
+      q.diagonal[u] = 1.0;
+      q.rhs[u] = 0.0;
+      q.adjacency[u].clear();
+    } else {
+      auto &a = q.adjacency[u];
+      a.erase(std::remove_if(a.begin(), a.end(), [&](const std::pair<uint32_t, double> &ow) { return gauge[ow.first] != 0; }),
+              a.end());
+    }
+  }
+  return floating;
+}
+
+void multiply_quadratic(const AxisEquations &q, const std::vector<double> &v, std::vector<double> &out) {
+  // multiply_quadratic (lib.rs:4767-4782): fold from the diagonal term
+  out.resize(v.size());
+  for (size_t i = 0; i < v.size(); ++i) {
+    double sum = q.diagonal[i] * v[i];
+    for (auto &ow : q.adjacency[i]) sum = sum - ow.second * v[ow.first];
+    out[i] = sum;
+  }
+}
+
+double jacobi(const std::vector<double> &r, const std::vector<double> &d, std::vector<double> &z) {
+  // apply_jacobi_preconditioner (lib.rs:4754-4765)
+  z.resize(r.size());
+  for (size_t i = 0; i < r.size(); ++i) z[i] = r[i] / d[i];
+  return dot(r.data(), z.data(), r.size());
+}
+
+struct QuadraticResult {
+  std::vector<double> solution;
+  uint64_t iterations = 0;
+  bool converged = false, breakdown = false;
+  double residual_ratio = 0.0;
+};
+
+// solve_quadratic (lib.rs:4586-4692) with finish_quadratic_solve (:4702-4752)
+QuadraticResult solve_quadratic(const AxisEquations &q, std::vector<double> x) {
+  const size_t n = q.diagonal.size();
+  const double TOL = 1e-8;
+  std::vector<double> product, r(n), z, p;
+  multiply_quadratic(q, x, product);
+  for (size_t i = 0; i < n; ++i) r[i] = q.rhs[i] - product[i];
+  double rr = dot(r.data(), r.data(), n);
+  const double r0 = rr;
+  const double target = std::max(r0, DBL_EPSILON) * TOL;
+  double rho = jacobi(r, q.diagonal, z);
+  p = z;
+  QuadraticResult out;
+  for (size_t k = 0; k < n; ++k) {
+    if (rr <= target) break;
+    multiply_quadratic(q, p, product);
+    const double den = dot(p.data(), product.data(), n);
+    if (!std::isfinite(den) || den <= 0.0 || !std::isfinite(rho) || rho <= 0.0) {
+      out.breakdown = true;
+      break;
+    }
+    const double alpha = rho / den;
+    if (!std::isfinite(alpha)) {
+      out.breakdown = true;
+      break;
+    }
+    for (size_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * product[i];
+    }
+    out.iterations += 1;
+    rr = dot(r.data(), r.data(), n);
+    if (rr <= target) {
+      multiply_quadratic(q, x, product);
+      for (size_t i = 0; i < n; ++i) r[i] = q.rhs[i] - product[i];
+      rr = dot(r.data(), r.data(), n);
+      if (rr <= target) break;
+      rho = jacobi(r, q.diagonal, z);
+      if (!std::isfinite(rho) || rho <= 0.0) {
+        out.breakdown = true;
+        break;
+      }
+      p = z;
+      continue;
+    }
+    const double next = jacobi(r, q.diagonal, z);
+    if (!std::isfinite(next) || next <= 0.0) {
+      out.breakdown = true;
+      break;
+    }
+    const double beta = next / rho;
+    for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    rho = next;
+  }
+  multiply_quadratic(q, x, product);
+  double fr = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double res = q.rhs[i] - product[i];
+    fr += res * res;
+  }
+  out.residual_ratio = r0 > 0.0 ? std::sqrt(fr / r0) : 0.0;
+  out.converged = fr <= target;
+  out.solution = std::move(x);
+  return out;
+}
+
+// solve_analytical_axis (lib.rs:4529-4556)
+QuadraticResult solve_analytical_axis(AxisEquations q, std::vector<double> solution, uint32_t center) {
+  auto floating = finalize_component_gauges(q);
+  for (auto &c : floating) {
+    const double g = solution[c[0]];
+    for (uint32_t u : c) solution[u] -= g;
+  }
+  QuadraticResult r = solve_quadratic(q, std::move(solution));
+  if (!r.converged) return r;
+  for (auto &c : floating) {
+    double mean = 0.0;
+    for (uint32_t u : c) mean += r.solution[u];
+    mean /= (double)c.size();
+    const double t = (double)center - mean;
+    for (uint32_t u : c) r.solution[u] += t;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
 // Dynamic Nesterov step (global_placement.rs:121-335, helpers :518-712).
 // ---------------------------------------------------------------------------
 struct Bounds {
@@ -3086,6 +3359,93 @@ uint64_t oracle_timing_placement_weights(uint64_t n_edges, const uint32_t *net, 
     ++k;
   }
   return k;
+}
+
+
+// ---- coarse quadratic seed -------------------------------------------------
+uint64_t oracle_linearize_axis(uint32_t n_units, uint32_t n_cells, const double *positions,
+                               const uint8_t *fixed_mask, const double *fixed_coordinate, const double *offsets,
+                               uint32_t n_nets, const uint32_t *net_pin_start, const uint32_t *pin_unit,
+                               const uint32_t *pin_cell, uint32_t n_arcs, const uint32_t *arc_units,
+                               const uint32_t *arc_cells, const uint64_t *arc_extra, uint64_t max_edges,
+                               uint32_t *edge_left, uint32_t *edge_right, double *edge_weight, uint8_t *edge_timing) {
+  Hypergraph hg;
+  hg.net_start.assign(net_pin_start, net_pin_start + n_nets + 1);
+  for (uint32_t i = 0; i < (n_nets ? net_pin_start[n_nets] : 0); ++i) hg.pins.push_back({pin_unit[i], pin_cell[i]});
+  for (uint32_t a = 0; a < n_arcs; ++a)
+    hg.arcs.push_back({{arc_units[2 * a], arc_cells[2 * a]}, {arc_units[2 * a + 1], arc_cells[2 * a + 1]}, arc_extra[a]});
+  std::vector<uint8_t> fm(fixed_mask, fixed_mask + n_cells);
+  std::vector<double> fc(fixed_coordinate, fixed_coordinate + n_cells), of(offsets, offsets + n_cells);
+  auto edges = linearize_axis(hg, std::vector<double>(positions, positions + n_units), fm, fc, of);
+  for (size_t i = 0; i < edges.size() && i < max_edges; ++i) {
+    edge_left[i] = edges[i].left;
+    edge_right[i] = edges[i].right;
+    edge_weight[i] = edges[i].weight;
+    edge_timing[i] = edges[i].timing ? 1 : 0;
+  }
+  return edges.size();
+}
+
+int oracle_solve_axis(uint32_t n, const double *diagonal, const uint32_t *adjacency_start,
+                      const uint32_t *adjacency_other, const double *adjacency_weight, const double *rhs,
+                      const uint8_t *anchored, const double *initial, uint32_t center, double *solution,
+                      uint64_t *iterations) {
+  AxisEquations q;
+  q.diagonal.assign(diagonal, diagonal + n);
+  q.rhs.assign(rhs, rhs + n);
+  q.anchored.assign(anchored, anchored + n);
+  q.adjacency.resize(n);
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint32_t k = adjacency_start[u]; k < adjacency_start[u + 1]; ++k)
+      q.adjacency[u].push_back({adjacency_other[k], adjacency_weight[k]});
+  QuadraticResult r = solve_analytical_axis(std::move(q), std::vector<double>(initial, initial + n), center);
+  if (iterations) *iterations = r.iterations;
+  if (!r.converged) return TEXO_GP_ERR_DID_NOT_CONVERGE;
+  std::copy(r.solution.begin(), r.solution.end(), solution);
+  return 0;
+}
+
+int oracle_quadratic_seed(const texo_gp_problem *problem, double *out_xy, texo_gp_report *report) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  try {
+    Problem P = load_problem(problem);
+    const uint32_t cx = P.W / 2, cy = P.H / 2;
+    uint64_t total_iterations = 0;
+    std::vector<std::vector<double>> solved(2);
+    for (int axis = 0; axis < 2; ++axis) {
+      std::vector<double> fc(P.n_cells), of(P.n_cells), origin(P.n_units);
+      for (size_t c = 0; c < P.n_cells; ++c) {
+        fc[c] = P.cell_fixed_xy[2 * c + axis];
+        of[c] = P.cell_offsets[2 * c + axis];
+      }
+      for (size_t u = 0; u < P.n_units; ++u) origin[u] = P.bounds4[4 * u + axis];
+      const double center = axis ? (double)cy : (double)cx;
+      std::vector<double> initial(P.n_units, center);
+      auto edges = linearize_axis(P.hg, initial, P.cell_fixed, fc, of);
+      AxisEquations q = axis_equations(P.n_units, P.unit_fixed, origin, fc, of, edges);
+      QuadraticResult r = solve_analytical_axis(std::move(q), initial, axis ? cy : cx);
+      total_iterations += r.iterations;
+      if (!r.converged) {
+        Failure f;
+        f.status = TEXO_GP_ERR_DID_NOT_CONVERGE;
+        f.index = (int64_t)r.iterations;
+        f.index2 = axis;
+        f.kind = r.breakdown ? 1u : 0u;
+        f.v0 = r.residual_ratio;
+        throw f;
+      }
+      solved[axis] = std::move(r.solution);
+    }
+    for (size_t u = 0; u < P.n_units; ++u) {
+      out_xy[2 * u] = solved[0][u];
+      out_xy[2 * u + 1] = solved[1][u];
+    }
+    if (report) report->iterations = total_iterations;
+    return 0;
+  } catch (const Failure &f) {
+    fill_failure(report, f);
+    return f.status;
+  }
 }
 
 }  // extern "C"

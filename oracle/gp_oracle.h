@@ -274,6 +274,31 @@ uint64_t oracle_timing_placement_weights(uint64_t n_edges, const uint32_t *net, 
                                          uint32_t exponent, uint32_t *out_net, uint32_t *out_sink,
                                          uint64_t *out_weight);
 
+/* ---- coarse quadratic seed (lib.rs:4042-4066, 4404-4692) ----------------- */
+/* linearize_axis (analytical_placement.rs:283-356) on one axis of an
+ * explicit hypergraph (pins as (unit, cell), driver first; arcs as
+ * (driver unit, driver cell, sink unit, sink cell, extra)).  Writes up to
+ * max_edges edges (left, right unit; weight; timing flag) and returns the
+ * edge count. */
+uint64_t oracle_linearize_axis(uint32_t n_units, uint32_t n_cells, const double *positions,
+                               const uint8_t *fixed_mask, const double *fixed_coordinate, const double *offsets,
+                               uint32_t n_nets, const uint32_t *net_pin_start, const uint32_t *pin_unit,
+                               const uint32_t *pin_cell, uint32_t n_arcs, const uint32_t *arc_units,
+                               const uint32_t *arc_cells, const uint64_t *arc_extra, uint64_t max_edges,
+                               uint32_t *edge_left, uint32_t *edge_right, double *edge_weight, uint8_t *edge_timing);
+/* solve_analytical_axis (lib.rs:4529-4556) of explicit AxisEquations
+ * (adjacency CSR in insertion order); `iterations` receives the PCG
+ * iterations.  DID_NOT_CONVERGE where the reference reports a PCG
+ * breakdown or non-convergence. */
+int oracle_solve_axis(uint32_t n, const double *diagonal, const uint32_t *adjacency_start,
+                      const uint32_t *adjacency_other, const double *adjacency_weight, const double *rhs,
+                      const uint8_t *anchored, const double *initial, uint32_t center, double *solution,
+                      uint64_t *iterations);
+/* The coarse seed of analytical_place (lib.rs:4042-4066): both axes
+ * linearized at the device centre and solved; out_xy[2*n_units]
+ * interleaved.  report->iterations = PCG iterations of x plus y. */
+int oracle_quadratic_seed(const texo_gp_problem *problem, double *out_xy, texo_gp_report *report);
+
 #ifdef __cplusplus
 }
 #endif
