@@ -102,6 +102,9 @@ def algorithmic_bytes(kernel: str, sizes: dict, problem):
         # once; per filler charge 8
         return (U * 12 + C * 16 + DU * 4 + M * (1 + off) + Fl,
                 C * 24 + movable_pins * 16 + M * 8 + F * G * 4 + Fl * 8)
+    if kernel == "optimizer":
+        # per coordinate-long vector read or written (the timer counts them)
+        return 0.0, C * 8
     raise ValueError(kernel)
 
 
@@ -113,6 +116,36 @@ def dct_flops(sizes: dict, problem) -> float:
 def progress(msg: str):
     """Progress on stderr; stdout carries only the result line."""
     print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_all_cores(problem, iterations: int, threads: int):
+    """The oracle with its parallel sections threaded as the reference's
+    rayon sections are (density fields in parallel, the four WA sums
+    {baseline, timing} x {x, y} in parallel): GP iterations/s over
+    `iterations` after one untimed iteration."""
+    import oracle
+    from texo_amd import abi
+    lib = oracle.load()
+    lib.oracle_set_threads(threads)
+    o = abi.Session(lib, "oracle", problem)
+    o.iterate(1)
+    t0 = time.perf_counter()
+    o.iterate(iterations)
+    dt = time.perf_counter() - t0
+    o.close()
+    lib.oracle_set_threads(1)
+    return iterations / dt, dt
 
 
 def cpu_baseline(problem, iterations: int, precision: int):
@@ -173,7 +206,10 @@ def run_single(problem, precision, steps, warmup):
     return out
 
 
-def run_batched(problems, precision, steps, warmup, timing_kernel):
+def run_batched(problems, precision, steps, warmup, timing_kernel, extra_kernels=(), extra_steps=100):
+    """The lockstep batch: `steps` timed iterations with `timing_kernel`'s
+    launches timed, then `extra_steps` more iterations for each kernel of
+    `extra_kernels` with that kernel timed (its own window)."""
     import texo_amd
     from texo_amd import abi
     b = texo_amd.Batch()
@@ -188,13 +224,19 @@ def run_batched(problems, precision, steps, warmup, timing_kernel):
     avg_ms, launches, tasks_per_launch = b.kernel_timing()
     r1 = [b.result(i, targets=False)[1] for i in range(len(problems))]
     sizes = b.sizes(0)
+    extra = {}
+    for k in extra_kernels:
+        b.set_kernel_timing(k)
+        b.iterate(extra_steps)
+        extra[k] = b.kernel_timing()
+    b.set_kernel_timing(None)
     n = len(problems)
     out = {"it_per_s": n * steps / dt, "per_job_it_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
            "kernel_avg_ms": avg_ms, "kernel_launches": launches, "kernel_tasks_per_launch": tasks_per_launch,
            "evaluations_per_iteration": sum(b1["evaluations"] - a["evaluations"] for a, b1 in zip(r0, r1)) / (n * steps),
            "trials_per_iteration": sum(b1["line_search_trials"] - a["line_search_trials"]
                                        for a, b1 in zip(r0, r1)) / (n * steps),
-           "hpwl": [r["final_hpwl"] for r in r1]}
+           "hpwl": [r["final_hpwl"] for r in r1], "extra_timing": extra}
     b.close()
     return out, sizes
 
@@ -321,12 +363,14 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--serving-clients", type=int, default=8, help="threads submitting the serving stream")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--eco-requests", type=int, default=0,
-                    help="also run the incremental stream with this many requests (0: skip)")
-    ap.add_argument("--eco-full", type=int, default=2, help="full multi-start jobs interleaved with the requests")
+    ap.add_argument("--eco-requests", type=int, default=64,
+                    help="incremental-stream requests (a bounded sample of the 1,024-request stream; 0: skip)")
+    ap.add_argument("--eco-full", type=int, default=1, help="full multi-start jobs interleaved with the requests")
+    ap.add_argument("--config2-steps", type=int, default=200,
+                    help="GP iterations of the 24k-cell / 128^2 single-job forms (FP32 vs BF16x3; 0: skip)")
     ap.add_argument("--cpu-iterations", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only-batched", action="store_true", help="time the batched form only (for profiling)")
+    ap.add_argument("--only-batched", action="store_true", help="time the batched form only, with no single-job, config-2, transform, serving, ECO or CPU legs (for profiling)")
     ap.add_argument("--only-serving", action="store_true",
                     help="run the saturated batched serving stream only and print its result (for profiling)")
     ap.add_argument("--only-eco", action="store_true",
@@ -372,7 +416,9 @@ def main():
         single = run_single(base, mfma, args.steps, args.warmup)
         progress(f"{args.precision} single job: {single['it_per_s']:.1f} it/s")
     starts = [synth.multistart(base, s) for s in range(args.batch)]
-    batched, sizes = run_batched(starts, mfma, args.steps, args.warmup, args.roofline_kernel)
+    extra_kernels = () if args.only_batched else tuple(
+        k for k in ("gather", "deposit", "optimizer") if k != args.roofline_kernel)
+    batched, sizes = run_batched(starts, mfma, args.steps, args.warmup, args.roofline_kernel, extra_kernels)
     progress(f"{args.precision} batch of {args.batch}: {batched['it_per_s']:.1f} it/s")
 
     k = args.roofline_kernel
@@ -397,6 +443,32 @@ def main():
                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(k, args.batch), "kernel": k,
                     "algorithmic_per_launch": nbytes, "design_bytes": per_design, "evaluation_bytes": per_eval,
                     "avg_launch_ms": batched["kernel_avg_ms"], "evaluations_per_launch": per_launch}
+
+    roofline["launches"] = batched["kernel_launches"]
+    # the other HBM-bound kernels, each timed over its own 100 iterations of
+    # the same batch
+    kernels = {}
+    for kname, (k_ms, k_count, k_tasks) in batched["extra_timing"].items():
+        per_design, per_unit = algorithmic_bytes(kname, sizes, base)
+        nbytes = per_design + k_tasks * per_unit
+        k_s = k_ms * 1e-3
+        gbs = nbytes / k_s / 1e9 if k_s > 0 else 0.0
+        kernels[kname] = {"achieved": gbs, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                          "traffic": measured_traffic(kname, args.batch),
+                          "algorithmic_per_launch": nbytes, "avg_launch_ms": k_ms, "launches": k_count,
+                          ("vectors_per_launch" if kname == "optimizer" else "evaluations_per_launch"): k_tasks}
+        progress(f"kernel {kname}: {kernels[kname]}")
+
+    # the 24k-cell / 128^2 configuration: FP32 plain vs BF16x3 single jobs
+    config2 = None
+    if args.config2_steps > 0 and not args.only_batched:
+        p2 = synth.ecp5_25k()
+        config2 = {"workload": f"ecp5_25k: {p2.meta['cells']} cells, {p2.width}x{p2.height} bins, "
+                               f"{p2.meta['nets']} nets, one job", "steps": args.config2_steps}
+        for name, prec in (("fp32", abi.DCT_FP32), ("bf16x3", abi.DCT_BF16X3)):
+            config2[f"{name}_it_per_s"] = run_single(p2, prec, args.config2_steps, args.warmup)["it_per_s"]
+        config2["bf16x3_vs_fp32"] = config2["bf16x3_it_per_s"] / config2["fp32_it_per_s"]
+        progress(f"config 2: {config2}")
 
     # the Poisson transform's MFMA use, from a second, shorter timed run of
     # the same batch with the transform launches timed (algorithmic flops:
@@ -432,16 +504,24 @@ def main():
                                                  / serving["one_at_a_time_fp32_saturated"]["jobs_per_s"])
 
     eco = None
-    if args.eco_requests > 0:
+    if args.eco_requests > 0 and not args.only_batched:
         eco = run_eco(args.eco_requests, args.eco_full, args.serving_clients)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.only_batched:
         progress("cpu baseline")
-        v, dt, parity = cpu_baseline(base, args.cpu_iterations, mfma)
-        cpu = {"value": v, "unit": "GP iterations/s", "cores": 1, "kind": "port",
+        v1, dt1, parity = cpu_baseline(base, args.cpu_iterations, mfma)
+        nproc = os.cpu_count() or 1
+        # the reference's parallel sections on this workload: 2 density fields
+        # and 4 WA sums, so at most 4 threads work at once
+        threads = min(4, nproc)
+        va, dta = cpu_all_cores(base, args.cpu_iterations, max(nproc, 2))
+        cpu = {"value": va, "unit": "GP iterations/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_iterations} GP iterations of the 84k-cell / 256x256 workload, one job "
-                         f"(FP64 C++ restatement of the reference loop, 1 thread, {dt:.1f} s)",
+                         f"(FP64 C++ restatement of the reference loop, its density fields and four WA sums "
+                         f"in parallel as the reference's rayon sections are, {dta:.1f} s)",
+               "single_core": {"value": v1, "cores": 1, "seconds": dt1},
+               "nproc": nproc, "cpu_model": cpu_model(),
                "error_vs_oracle": parity}
 
     line = {
@@ -471,7 +551,9 @@ def main():
                   "batched_per_job_it_per_s": batched["per_job_it_per_s"],
                   "batched_vs_fp32_plain": batched["it_per_s"] / plain["it_per_s"]},
         "roofline": roofline,
+        "roofline_kernels": kernels,
         "roofline_transform": transform,
+        "config2": config2,
         "cpu_baseline": cpu,
         "serving": serving,
         "eco": eco,

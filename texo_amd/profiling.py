@@ -69,11 +69,23 @@ def traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, copy_bytes: int,
     write = counter_per_dispatch(write_dir, "WRITE_SIZE")
     kernels = {}
     for k in sorted(set(fetch) & set(write)):
-        name = {"k_wa": "wa", "k_deposit": "deposit", "k_sample": "gather"}.get(k, k)
+        name = {"k_wa": "wa", "k_deposit": "deposit", "k_gradient": "gather", "k_optimizer": "optimizer"}.get(k, k)
         kernels[name] = {"kernel": k, "fetch_bytes": fetch[k][0] * ff, "write_bytes": write[k][0] * wf,
                          "fetch_kib": fetch[k][0], "write_kib": write[k][0], "dispatches": fetch[k][1]}
     return {"batch": batch, "copy_bytes": copy_bytes, "fetch_bytes_per_kib": ff, "write_bytes_per_kib": wf,
             "kernels": kernels}
+
+
+def window_average_us(stats_dir: str, kernel: str, last: int):
+    """Mean duration (us) of the last `last` dispatches of `kernel` in the
+    kernel trace: the timed window of a run that ends with it."""
+    rows = [r for r in _csv(stats_dir, "kernel_trace.csv") if kernel_name(r["Kernel_Name"]) == kernel]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    rows = rows[-last:] if last > 0 else rows
+    if not rows:
+        return None, 0
+    d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows]
+    return sum(d) / len(d) / 1e3, len(d)
 
 
 def stats_markdown(stats_dir: str, bench_line: dict, title: str, command: str, top: int = 24) -> str:
@@ -91,8 +103,16 @@ def stats_markdown(stats_dir: str, bench_line: dict, title: str, command: str, t
             json.dumps({k: bench_line.get(k) for k in ("metric", "value", "unit", "ms_per_step", "config")}),
             json.dumps({"roofline": rl}), "```", ""]
     if rl.get("kernel"):
-        out.append(f"HIP-event average of `{rl['kernel']}` in the timed region: "
-                   f"{rl.get('avg_launch_ms', 0) * 1e3:.1f} us per launch.")
+        kname = {"wa": "k_wa", "deposit": "k_deposit", "gather": "k_gradient"}.get(rl["kernel"], rl["kernel"])
+        avg, n = window_average_us(stats_dir, kname, int(rl.get("launches", 0)))
+        event_us = rl.get("avg_launch_ms", 0) * 1e3
+        out.append(f"HIP-event average of `{kname}` in the timed region: {event_us:.1f} us per launch over "
+                   f"{rl.get('launches')} launches.")
+        if avg:
+            frac = rl["algorithmic_per_launch"] / (avg * 1e-6) / 1e9 / rl["peak"]
+            out.append(f"rocprofv3 average of the same last {n} `{kname}` dispatches (the timed window): "
+                       f"{avg:.1f} us, {100 * (avg / event_us - 1):+.1f}% against the event timer; "
+                       f"roofline fraction from it: {frac:.4f} (the line reports {rl['frac']:.4f}).")
     return "\n".join(out) + "\n"
 
 
@@ -136,9 +156,11 @@ def main(argv):
     copy_bytes, batch = int(copy_bytes), int(batch)
     with open(bench_json) as f:
         line = json.loads([ln for ln in f.read().splitlines() if ln.startswith("{")][-1])
-    bench_args = f"--only-batched --no-serving --no-cpu --batch {batch}"
+    # the driver's benchmark window (20 timed iterations after 5 warm-up
+    # iterations), its batched form only so that the run ends with the window
+    bench_args = f"--gpus 1 --steps 20 --warmup 5 --only-batched --batch {batch}"
     stats_cmd = (f"rocprofv3 --kernel-trace --stats --output-format csv -d {stats_dir} -o run -- "
-                 f"python bench.py {bench_args} --steps 1000 --warmup 20")
+                 f"python bench.py {bench_args}")
     md = stats_markdown(stats_dir, line, f"Kernel statistics: ecp5_85k, {batch} multi-start jobs in lockstep, "
                         f"{line['config'].get('precision')} transform", stats_cmd)
     with open(os.path.join(out_dir, f"kernels_ecp5_85k_batch{batch}.md"), "w") as f:
@@ -146,7 +168,7 @@ def main(argv):
     t = traffic(fetch_dir, write_dir, cal_f, cal_w, copy_bytes, batch)
     regex = "k_wa|k_deposit|k_sample|k_gradient|k_convert|k_optimizer|k_gemm"
     pmc = lambda c, d: (f"rocprofv3 --pmc {c} --kernel-include-regex \"{regex}\" --output-format csv -d {d} -o run -- "
-                        f"python bench.py {bench_args} --steps 20 --warmup 5")
+                        f"python bench.py {bench_args}")
     cal = [f"rocprofv3 --pmc {c} --output-format csv -d {d} -o run -- "
            f"python bench.py --copy-calibration {copy_bytes >> 20}" for c, d in (("FETCH_SIZE", cal_f), ("WRITE_SIZE", cal_w))]
     with open(os.path.join(out_dir, f"traffic_ecp5_85k_batch{batch}.md"), "w") as f:
