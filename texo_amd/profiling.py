@@ -88,6 +88,31 @@ def window_average_us(stats_dir: str, kernel: str, last: int):
     return sum(d) / len(d) / 1e3, len(d)
 
 
+def window_occupancy(stats_dir: str, window_ms: float) -> dict:
+    """Over the last `window_ms` of the kernel trace (the timed window of a
+    run that ends with it): the time at least one kernel ran, the time with
+    k kernels in flight, and each kernel's summed duration there."""
+    ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), kernel_name(r["Kernel_Name"]))
+                for r in _csv(stats_dir, "kernel_trace.csv"))
+    if not ev:
+        return {}
+    end = max(e[1] for e in ev)
+    ev = [e for e in ev if e[0] >= end - window_ms * 1e6]
+    points = sorted([(s, 1) for s, _, _ in ev] + [(e, -1) for _, e, _ in ev])
+    inflight, last, by_k = 0, None, defaultdict(float)
+    for t, d in points:
+        if last is not None and inflight > 0:
+            by_k[inflight] += t - last
+        inflight += d
+        last = t
+    per_kernel = defaultdict(float)
+    for s, e, k in ev:
+        per_kernel[k] += e - s
+    return {"window_ms": window_ms, "busy_ms": sum(by_k.values()) / 1e6,
+            "in_flight_ms": {k: v / 1e6 for k, v in sorted(by_k.items())},
+            "kernel_ms": {k: v / 1e6 for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1])}}
+
+
 def stats_markdown(stats_dir: str, bench_line: dict, title: str, command: str, top: int = 24) -> str:
     rows = _csv(stats_dir, "kernel_stats.csv")
     total = sum(float(r["TotalDurationNs"]) for r in rows)
@@ -113,6 +138,18 @@ def stats_markdown(stats_dir: str, bench_line: dict, title: str, command: str, t
             out.append(f"rocprofv3 average of the same last {n} `{kname}` dispatches (the timed window): "
                        f"{avg:.1f} us, {100 * (avg / event_us - 1):+.1f}% against the event timer; "
                        f"roofline fraction from it: {frac:.4f} (the line reports {rl['frac']:.4f}).")
+    window = bench_line.get("steps", 0) * bench_line.get("ms_per_step", 0.0)
+    occ = window_occupancy(stats_dir, window) if window > 0 else {}
+    if occ:
+        total = sum(occ["kernel_ms"].values())
+        out += ["", f"Timed window (the last {window:.1f} ms of the trace): a kernel runs {occ['busy_ms']:.1f} ms "
+                f"({100 * occ['busy_ms'] / window:.1f}%); time with k kernels in flight: "
+                + ", ".join(f"{k}: {v:.1f} ms" for k, v in occ["in_flight_ms"].items())
+                + f".  Kernel durations there sum to {total:.1f} ms "
+                f"({total / max(occ['busy_ms'], 1e-9):.2f} kernels in flight on average); by kernel:", "",
+                "| kernel | ms in the window | share |", "|---|---:|---:|"]
+        for k, v in list(occ["kernel_ms"].items())[:12]:
+            out.append(f"| `{k}` | {v:.2f} | {100 * v / total:.1f}% |")
     return "\n".join(out) + "\n"
 
 
