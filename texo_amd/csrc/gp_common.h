@@ -126,7 +126,15 @@ struct BlockPlan {
 // nets of more than WA_LOOP_PINS pins get a wavefront that loops.
 constexpr int WA_PINS_PER_LANE = 4;
 constexpr int WA_LOOP_PINS = 64 * WA_PINS_PER_LANE;
-constexpr int WA_MAX_SEGMENTS = 32;
+constexpr int WA_MAX_SEGMENTS = 96;
+// A WA segment: consecutive nets of one class with one lane-group size
+// (`lanes`, 0: one looping wavefront per net), starting at wavefront
+// `first_wave`.  `degree` > 0: every net of the segment has that many pins,
+// the first at `first_pin`, so a lane finds its pins without loading the net
+// starts; 0: the degrees differ.
+struct WaSeg {
+  int first_net, nets, lanes, first_wave, first_pin, degree, pad0, pad1;
+};
 __host__ __device__ inline int wa_lanes(uint32_t pins) {
   if (pins > (uint32_t)WA_LOOP_PINS) return 0;
   int g = 1;
@@ -260,7 +268,7 @@ struct JobStatic {
   // WA segments: {first net, nets, lanes per net (0: one wavefront looping
   // over a long net), first wavefront}
   int n_wa_seg;
-  const int4 *wa_seg;
+  const WaSeg *wa_seg;
   const uint32_t *net_start;
   const double *net_weight;
   const uint32_t *pin_unit;   // NO_UNIT for fixed pins

@@ -209,7 +209,7 @@ struct Job {
   std::map<uint32_t, Control> controls;
   size_t n_nets = 0, n_arcs = 0, n_net_total = 0, n_pins_total = 0, first_arc_pin = 0;
   std::vector<uint32_t> net_start, pin_unit, u_pstart, u_pins, pin_slot, u_nbase;
-  std::vector<int4> wa_seg;  // WA segments (JobStatic::wa_seg)
+  std::vector<WaSeg> wa_seg;  // WA segments (JobStatic::wa_seg)
   int wa_waves = 0;          // wavefronts of one WA evaluation
   int wa_waves_live = 0;     // ... of its nets with a movable pin (a multiple of BLOCK / 64)
   size_t n_live_nets = 0;
@@ -250,7 +250,7 @@ struct Job {
   double *d_m_charge = nullptr, *d_fl_charge = nullptr, *d_du_fcharge = nullptr;
   uint8_t *d_m_field = nullptr, *d_fl_field = nullptr;
   uint32_t *d_net_start = nullptr, *d_pin_unit = nullptr;
-  int4 *d_wa_seg = nullptr;
+  WaSeg *d_wa_seg = nullptr;
   uint32_t *d_u_pstart = nullptr, *d_pin_slot = nullptr, *d_u_nbase = nullptr;
   double *d_net_weight = nullptr, *d_wl_diag = nullptr;
   int32_t *d_unit_du = nullptr;

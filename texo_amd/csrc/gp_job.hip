@@ -471,24 +471,29 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
       start2[k + 1] = q;
     }
     // WA segments: runs of consecutive nets of one class with one lane-group
-    // size; per class the looping wavefronts (longest nets) come first, and
-    // the fixed class starts on a block boundary
-    std::vector<int4> runs[2];
+    // size and, for small nets and timed arcs, one degree (a lane then
+    // computes its pins from the segment's first pin); per class the
+    // looping wavefronts (longest nets) come first, and the fixed class
+    // starts on a block boundary
+    std::vector<WaSeg> runs[2];
     for (size_t k = 0; k < n_net_total; ++k) {
-      const int g = wa_lanes(start2[k + 1] - start2[k]);
-      std::vector<int4> &rc = runs[live2[k]];
-      if (!rc.empty() && rc.back().z == g && (size_t)(rc.back().x + rc.back().y) == k)
-        rc.back().y += 1;
+      const uint32_t d = start2[k + 1] - start2[k];
+      const int g = wa_lanes(d);
+      const int deg = g > 0 && (d <= LARGE_NET_PINS || k >= n_nets) ? (int)d : 0;
+      std::vector<WaSeg> &rc = runs[live2[k]];
+      if (!rc.empty() && rc.back().lanes == g && rc.back().degree == deg &&
+          (size_t)(rc.back().first_net + rc.back().nets) == k)
+        rc.back().nets += 1;
       else
-        rc.push_back(make_int4((int)k, 1, g, 0));
+        rc.push_back(WaSeg{(int)k, 1, g, 0, (int)start2[k], deg, 0, 0});
     }
     wa_seg.clear();
     int waves = 0;
     for (int cls = 1; cls >= 0; --cls) {
-      std::stable_partition(runs[cls].begin(), runs[cls].end(), [](const int4 &r) { return r.z == 0; });
-      for (int4 &r : runs[cls]) {
-        r.w = waves;
-        waves += r.z == 0 ? r.y : (int)(((int64_t)r.y * r.z + 63) / 64);
+      std::stable_partition(runs[cls].begin(), runs[cls].end(), [](const WaSeg &r) { return r.lanes == 0; });
+      for (WaSeg &r : runs[cls]) {
+        r.first_wave = waves;
+        waves += r.lanes == 0 ? r.nets : (int)(((int64_t)r.nets * r.lanes + 63) / 64);
         wa_seg.push_back(r);
       }
       if (cls == 1) {

@@ -589,29 +589,38 @@ __device__ __forceinline__ double group_sum(double v) {
 // registers: every pin coordinate is loaded once and its four exponentials
 // are computed once; the group combines extremes and sums by lane reductions.
 template <int G>
-__device__ double wa_group_net(const JobStatic &a, const Task &t, const int4 &seg, int wave) {
+__device__ double wa_group_net(const JobStatic &a, const Task &t, const WaSeg &seg, int wave) {
   constexpr int K = WA_PINS_PER_LANE;
   const int lane = threadIdx.x & 63, r = lane & (G - 1);
-  const int local = (wave - seg.w) * (64 / G) + lane / G;
-  const bool valid = local < seg.y;
+  const int local = (wave - seg.first_wave) * (64 / G) + lane / G;
+  const bool valid = local < seg.nets;
   uint32_t b = 0, e = 0;
   double w = 0.0;
   if (valid) {
-    const uint32_t net = (uint32_t)(seg.x + local);
-    b = a.net_start[net];
-    e = a.net_start[net + 1];
+    const uint32_t net = (uint32_t)(seg.first_net + local);
+    if (seg.degree > 0) {  // one degree: the pins follow from the segment
+      b = (uint32_t)seg.first_pin + (uint32_t)local * (uint32_t)seg.degree;
+      e = b + (uint32_t)seg.degree;
+    } else {
+      b = a.net_start[net];
+      e = a.net_start[net + 1];
+    }
     w = a.net_weight[net];
   }
   const double ig = 1.0 / t.gamma;
   double2 c[K];
   bool in[K];
+  uint32_t slot[K];
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint32_t p = b + (uint32_t)(r + G * k);
     in[k] = p < e;
-    c[k] = in[k] ? pin_coordinate(a, t.coords, p) : make_double2(0.0, 0.0);
+    c[k] = make_double2(0.0, 0.0);
+    slot[k] = NO_UNIT;
     if (in[k]) {
+      c[k] = pin_coordinate(a, t.coords, p);
+      slot[k] = a.pin_slot[p];
       xmin = fmin(xmin, c[k].x);
       xmax = fmax(xmax, c[k].x);
       ymin = fmin(ymin, c[k].y);
@@ -626,13 +635,14 @@ __device__ double wa_group_net(const JobStatic &a, const Task &t, const int4 &se
   WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
-    // exactly 1.0 where the delta is zero (analytical_placement.rs:666-678)
-    ex[k] = !in[k] ? 0.0 : dxm == 0.0 ? 1.0 : exp(dxm * ig);
-    enx[k] = !in[k] ? 0.0 : dxn == 0.0 ? 1.0 : exp(-dxn * ig);
-    ey[k] = !in[k] ? 0.0 : dym == 0.0 ? 1.0 : exp(dym * ig);
-    eny[k] = !in[k] ? 0.0 : dyn == 0.0 ? 1.0 : exp(-dyn * ig);
+    ex[k] = enx[k] = ey[k] = eny[k] = 0.0;
     if (in[k]) {
+      const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
+      // exactly 1.0 where the delta is zero (analytical_placement.rs:666-678)
+      ex[k] = dxm == 0.0 ? 1.0 : exp(dxm * ig);
+      enx[k] = dxn == 0.0 ? 1.0 : exp(-dxn * ig);
+      ey[k] = dym == 0.0 ? 1.0 : exp(dym * ig);
+      eny[k] = dyn == 0.0 ? 1.0 : exp(-dyn * ig);
       s.psx += ex[k];
       s.pdx += ex[k] * dxm;
       s.nsx += enx[k];
@@ -656,12 +666,10 @@ __device__ double wa_group_net(const JobStatic &a, const Task &t, const int4 &se
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (!in[k]) continue;
-    const uint32_t p = b + (uint32_t)(r + G * k);
     const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
     const double ddx = (ex[k] * rs.x) * (1.0 + (dxm - pmx) * ig) - (enx[k] * rs.y) * (1.0 - (dxn - nmx) * ig);
     const double ddy = (ey[k] * rs.z) * (1.0 + (dym - pmy) * ig) - (eny[k] * rs.w) * (1.0 - (dyn - nmy) * ig);
-    const uint32_t slot = a.pin_slot[p];
-    if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * ddx, w * ddy);
+    if (slot[k] != NO_UNIT) t.dpin[slot[k]] = make_double2(w * ddx, w * ddy);
   }
   return valid && r == 0 ? w * ((xmax - xmin) + pmx - nmx) + w * ((ymax - ymin) + pmy - nmy) : 0.0;
 }
@@ -713,16 +721,26 @@ __global__ void __launch_bounds__(BLOCK) k_wa(const Task *__restrict__ tasks, Bl
   const Task &t = tasks[bm.x];
   const JobStatic &a = t.j;
   const int wave = bm.y * (BLOCK / 64) + (threadIdx.x >> 6);
-  int4 seg = make_int4(0, 0, -1, 0);  // past the last segment: no net
-  for (int k = 0; k < a.n_wa_seg; ++k) {
-    const int4 sk = a.wa_seg[k];
-    const int n = sk.z == 0 ? sk.y : (sk.y * sk.z + 63) / 64;
-    if (wave >= sk.w && wave < sk.w + n) seg = sk;
+  // the last segment starting at or before this wavefront (segments are in
+  // wavefront order); past its end the wavefront has no net
+  int lo = 0, hi = a.n_wa_seg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.wa_seg[mid].first_wave <= wave)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  WaSeg seg{0, 0, -1, 0, 0, 0, 0, 0};
+  if (a.n_wa_seg > 0) {
+    seg = a.wa_seg[lo];
+    const int n = seg.lanes == 0 ? seg.nets : (seg.nets * seg.lanes + 63) / 64;
+    if (wave < seg.first_wave || wave >= seg.first_wave + n) seg.lanes = -1;
   }
   double value[1] = {0.0};
-  switch (seg.z) {
+  switch (seg.lanes) {
     case 0:
-      value[0] = wa_loop_net(a, t, (uint32_t)(seg.x + (wave - seg.w)));
+      value[0] = wa_loop_net(a, t, (uint32_t)(seg.first_net + (wave - seg.first_wave)));
       break;
     case 1:
       value[0] = wa_group_net<1>(a, t, seg, wave);
