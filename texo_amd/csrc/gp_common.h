@@ -254,6 +254,7 @@ struct JobStatic {
   const double *fl_charge;    // filler charge
   const long long *base_fx;   // [F*G] -smoothed capacity, fixed point
   const int32_t *unit_du;     // unit -> density unit or -1
+  const uint32_t *u_mstart;   // unit -> first member ([n_units + 1]; empty for a unit without members)
   const uint32_t *perm;       // deposit order of the charges (members, then fillers)
   int tiles_x, n_keys;        // DEP_TILE x DEP_TILE-bin tiles of the charge sort
   const uint32_t *key_end;    // [n_keys] end of each tile's charges in deposit order

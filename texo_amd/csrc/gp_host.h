@@ -200,6 +200,7 @@ struct Job {
   std::vector<uint8_t> fl_field;
   std::vector<double> fl_charge, fl_xy;
   std::vector<int32_t> unit_du;
+  std::vector<uint32_t> u_mstart;  // unit -> first member (JobStatic::u_mstart)
   std::map<uint32_t, std::vector<uint32_t>> capacity;  // after fixed occupancy
   std::vector<size_t> available;                       // per field
   std::vector<double> real_area, gamma_weight, wl_diag;
@@ -254,6 +255,7 @@ struct Job {
   uint32_t *d_u_pstart = nullptr, *d_pin_slot = nullptr, *d_u_nbase = nullptr;
   double *d_net_weight = nullptr, *d_wl_diag = nullptr;
   int32_t *d_unit_du = nullptr;
+  uint32_t *d_u_mstart = nullptr;
   double *d_lower = nullptr, *d_upper = nullptr, *d_metric = nullptr, *d_part_metric = nullptr, *d_part_step = nullptr;
   double *d_vec[4] = {};   // coordinate buffers (rotated by index)
   double *d_grad[3] = {};  // origin, y, candidate gradients

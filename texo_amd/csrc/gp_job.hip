@@ -300,6 +300,14 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
   }
   du_mstart.push_back(static_cast<uint32_t>(m_unit.size()));
   DU = du_unit.size();
+  // members by unit (an empty range for a unit that is no density unit), so
+  // the gradient finds a unit's members with one indirection
+  u_mstart.assign(U + 1, 0);
+  for (size_t u = 0, m = 0; u < U; ++u) {
+    u_mstart[u] = static_cast<uint32_t>(m);
+    if (unit_du[u] >= 0) m = du_mstart[unit_du[u] + 1];
+  }
+  u_mstart[U] = static_cast<uint32_t>(m_unit.size());
   M = m_unit.size();
   m_charge.assign(M, 1.0);
   // fields in ascending kind order (the reference's BTreeMap over ResourceKind)
@@ -686,6 +694,7 @@ void Job::use_design(const std::shared_ptr<SharedDesign> &sd) {
   move(d_u_nbase);
   move(d_wl_diag);
   move(d_unit_du);
+  move(d_u_mstart);
   move(d_lower);
   move(d_upper);
   design = sd;
@@ -712,6 +721,7 @@ void Job::build_js() {
   js.fl_charge = d_fl_charge;
   js.base_fx = d_base;
   js.unit_du = d_unit_du;
+  js.u_mstart = d_u_mstart;
   js.perm = d_perm;
   js.s_ci = d_s_ci;
   js.s_off = OffArr{off32 ? nullptr : d_s_off, off32 ? d_s_off32 : nullptr};
@@ -851,6 +861,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   data(d_u_nbase, U, u_nbase.data());
   data(d_wl_diag, U, wl_diag.data());
   data(d_unit_du, U, unit_du.data());
+  data(d_u_mstart, U + 1, u_mstart.data());
   data(d_lower, C, lower.data());
   data(d_upper, C, upper.data());
   design_bytes = arena.planned();

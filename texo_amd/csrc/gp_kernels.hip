@@ -22,11 +22,14 @@ __device__ __forceinline__ double wave_min(double v) { return lane_reduce<64>(v,
 
 // Block-wide sum of K values per thread; thread 0 writes out[0..K).
 template <int K>
-__device__ void block_sum(double (&v)[K], double *out) {
+__device__ void block_sum(double (&v)[K], double *out, uint32_t active = ~0u) {
+  // entries outside `active` (a block-uniform mask) are zero in every lane:
+  // their sums are written as 0.0 without reducing
   __shared__ double s[K][BLOCK / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  for (int k = 0; k < K; ++k)
+    if (active >> k & 1u) v[k] = wave_sum(v[k]);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) s[k][wave] = v[k];
@@ -34,9 +37,12 @@ __device__ void block_sum(double (&v)[K], double *out) {
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = ((s[k][0] + s[k][1]) + s[k][2]) + s[k][3];
+    for (int k = 0; k < K; ++k) out[k] = (active >> k & 1u) ? ((s[k][0] + s[k][1]) + s[k][2]) + s[k][3] : 0.0;
   }
 }
+
+// Mask of the per-field entries [first, first + F) of a partial record.
+__device__ __forceinline__ uint32_t field_mask(int F, int first) { return ((1u << F) - 1u) << first; }
 
 // Block-wide maximum of a non-negative value and OR of a flag; thread 0
 // writes out[0] = max and out[1] = flag as 0.0 / 1.0.
@@ -517,7 +523,7 @@ __device__ void filler_gradient_block(const Task &t, int lb) {
                  kkt_scaled(vy, t.coords[c + 1], a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
     }
   }
-  block_sum<2 * MAXF>(red, t.part_gf + (size_t)lb * GF_STRIDE);
+  block_sum<2 * MAXF>(red, t.part_gf + (size_t)lb * GF_STRIDE, field_mask(a.F, 0) | field_mask(a.F, MAXF));
   block_max_flag(kkt, bad, t.part_gf + (size_t)lb * GF_STRIDE + 2 * MAXF);
 }
 
@@ -823,14 +829,14 @@ __device__ void unit_gradient_block(const Task &t, int lb) {
       t.wa_grad_out[2 * (size_t)u] = gx;
       t.wa_grad_out[2 * (size_t)u + 1] = gy;
     }
-    const int du = a.unit_du[u];
-    if (du >= 0) {
+    const uint32_t mb = a.u_mstart[u], me = a.u_mstart[u + 1];
+    if (mb < me) {
       double gxf[MAXF], gyf[MAXF];
 #pragma unroll
       for (int f = 0; f < MAXF; ++f) gxf[f] = gyf[f] = 0.0;
       const double ux = t.coords[2 * (size_t)u], uy = t.coords[2 * (size_t)u + 1];
       const size_t G = (size_t)a.W * a.H;
-      for (uint32_t m = a.du_mstart[du]; m < a.du_mstart[du + 1]; ++m) {
+      for (uint32_t m = mb; m < me; ++m) {
         const int f = a.m_field[m];
         const double q = a.m_charge[m];
         // the field at the member (sample_quadratic_bspline_with_gradient,
@@ -859,6 +865,7 @@ __device__ void unit_gradient_block(const Task &t, int lb) {
           gx += coef * gxf[f];
           gy += coef * gyf[f];
           if (t.raw_units) {
+            const int du = a.unit_du[u];
             t.raw_units[2 * ((size_t)f * a.n_density_units + du)] = gxf[f];
             t.raw_units[2 * ((size_t)f * a.n_density_units + du) + 1] = gyf[f];
           }
@@ -874,7 +881,7 @@ __device__ void unit_gradient_block(const Task &t, int lb) {
                  kkt_scaled(gy, t.coords[c + 1], a.lower[c + 1], a.upper[c + 1], t.metric[c + 1]));
     }
   }
-  block_sum<1 + 2 * MAXF>(red, t.part_ug + (size_t)lb * UG_STRIDE);
+  block_sum<1 + 2 * MAXF>(red, t.part_ug + (size_t)lb * UG_STRIDE, 1u | field_mask(a.F, 1) | field_mask(a.F, 1 + MAXF));
   block_max_flag(kkt, bad, t.part_ug + (size_t)lb * UG_STRIDE + 1 + 2 * MAXF);
 }
 
@@ -922,9 +929,13 @@ __device__ void finalize_task(const Task &t) {
     }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // sums of fields past F are zero in every lane: not reduced
+  const int F = t.j.F;
+  const uint32_t live = 3u | field_mask(F, 2) | field_mask(F, 2 + MAXF) | field_mask(F, 2 + 2 * MAXF) |
+                        field_mask(F, 2 + 3 * MAXF) | (((1u << NM) - 1u) << NS);
 #pragma unroll
   for (int k = 0; k < NS + NM; ++k) {
-    acc[k] = k < NS ? wave_sum(acc[k]) : wave_max(acc[k]);
+    if (live >> k & 1u) acc[k] = k < NS ? wave_sum(acc[k]) : wave_max(acc[k]);
     if (lane == 0) sh[k][wave] = acc[k];
   }
   __shared__ double sfw[BLOCK / 64];
@@ -939,7 +950,7 @@ __device__ void finalize_task(const Task &t) {
   __shared__ long long scv[2 * MAXF][BLOCK / 64];
 #pragma unroll
   for (int k = 0; k < 2 * MAXF; ++k) {
-    cv[k] = lane_reduce<64>(cv[k], OpAdd{});
+    if ((k & (MAXF - 1)) < F) cv[k] = lane_reduce<64>(cv[k], OpAdd{});
     if (lane == 0) scv[k][wave] = cv[k];
   }
   __syncthreads();
