@@ -206,6 +206,39 @@ def run_single(problem, precision, steps, warmup):
     return out
 
 
+def baseline_table(batch: int) -> dict:
+    """GP iterations/s of each form on the 1k / 64^2, 24k / 128^2 and 84k /
+    256^2 configurations: the FP64 oracle on one core and on all cores
+    (bounded samples of about 2-15 s each), the GPU with the FP32 transform
+    (one job), with its configuration's MFMA transform (one job) and a
+    lockstep batch of `batch` multi-start jobs with it."""
+    from texo_amd import abi, synth
+    nproc = os.cpu_count() or 1
+    rows = {}
+    for name, make, cpu_its, mfma in (("1: plain_1k", synth.plain_1k, 200, abi.DCT_BF16X3),
+                                      ("2: ecp5_25k", synth.ecp5_25k, 40, abi.DCT_BF16X3),
+                                      ("3: ecp5_85k", synth.ecp5_85k, 12, abi.DCT_FP16_SCALED)):
+        p = make()
+        one, t1, _ = cpu_baseline(p, cpu_its, abi.DCT_FP32)
+        allc, ta = cpu_all_cores(p, cpu_its, max(nproc, 2))
+        fp32 = run_single(p, abi.DCT_FP32, 200, 5)
+        mf = run_single(p, mfma, 200, 5)
+        starts = [synth.multistart(p, s) for s in range(batch)]
+        bt, _ = run_batched(starts, mfma, 100, 5, "wa")
+        rows[name] = {"cells": p.meta["cells"], "bins": f"{p.width}x{p.height}", "cpu_iterations": cpu_its,
+                      "cpu_1_core_it_per_s": one, "cpu_all_cores_it_per_s": allc,
+                      "cpu_seconds": {"1_core": t1, "all_cores": ta},
+                      "gpu_fp32_single_it_per_s": fp32["it_per_s"],
+                      "gpu_mfma_precision": {abi.DCT_BF16X3: "bf16x3", abi.DCT_FP16_SCALED: "fp16_scaled"}[mfma],
+                      "gpu_mfma_single_it_per_s": mf["it_per_s"],
+                      f"gpu_mfma_batch{batch}_it_per_s": bt["it_per_s"],
+                      "batched_vs_fp32_single": bt["it_per_s"] / fp32["it_per_s"]}
+        progress(f"table {name}: {rows[name]}")
+    return {"baseline_table": rows, "nproc": nproc, "cpu_model": cpu_model(),
+            "cpu_threads_note": "the oracle's parallel sections are the reference's rayon sections "
+                                "(density fields, four WA sums): at most 4 threads work at once"}
+
+
 def run_batched(problems, precision, steps, warmup, timing_kernel, extra_kernels=(), extra_steps=100):
     """The lockstep batch: `steps` timed iterations with `timing_kernel`'s
     launches timed, then `extra_steps` more iterations for each kernel of
@@ -375,6 +408,9 @@ def main():
                     help="run the saturated batched serving stream only and print its result (for profiling)")
     ap.add_argument("--only-eco", action="store_true",
                     help="run the incremental stream sample only (--eco-requests, --eco-full) and print its result")
+    ap.add_argument("--baseline-table", action="store_true",
+                    help="only the per-configuration table: the oracle on 1 core and on all cores, and the GPU "
+                         "forms, on the 1k, 24k and 84k configurations (bounded samples)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference reading for the HBM counters)")
     args = ap.parse_args()
@@ -387,6 +423,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
+        return
+    if args.baseline_table:
+        print(json.dumps(baseline_table(args.batch)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the placement loop of one job runs on one "
