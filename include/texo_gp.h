@@ -94,6 +94,12 @@ enum texo_gp_status {
   TEXO_GP_ERR_INVALID_FILLER_AREA = 301,
   TEXO_GP_ERR_POSTCONDITION_VIOLATION = 302,
   TEXO_GP_ERR_ARITHMETIC_OVERFLOW = 303,
+  /* AuctionError (crates/texo-pnr/src/legalization/auction.rs:44-53) of the
+   * legalization auction */
+  TEXO_GP_ERR_AUCTION_OVERFLOW = 400,
+  TEXO_GP_ERR_AUCTION_OBJECT_OUT_OF_RANGE = 401,
+  TEXO_GP_ERR_AUCTION_BASE_COST_OUT_OF_RANGE = 402,
+  TEXO_GP_ERR_AUCTION_DUPLICATE_BEST_OBJECT = 403,
   /* Boundary-level failures (no reference counterpart). */
   TEXO_GP_ERR_INVALID_ARGUMENT = 900,
   TEXO_GP_ERR_DEVICE = 901,
@@ -564,6 +570,27 @@ int texo_gp_solve_axis(int32_t device, uint32_t n, const double *diagonal, const
                        const uint32_t *adjacency_other, const double *adjacency_weight, const double *rhs,
                        const uint8_t *anchored, const double *initial, uint32_t center, double *solution,
                        uint64_t *iterations);
+
+/* ---- legalization auction ------------------------------------------------ */
+/* The relaxed unit-to-object assignment of legalization::project
+ * (assign_implicitly, crates/texo-pnr/src/legalization/auction.rs:123-253,
+ * called from auction_units, legalization/mod.rs:781-833) over explicit
+ * candidate domains, on `device`: bidder b's domain is
+ * domain_object/domain_cost[domain_start[b] .. domain_start[b+1]) (objects
+ * < n_objects, base costs <= max_base_cost, no object twice).  Every
+ * unassigned bidder bids in the same round at the same prices (the
+ * reference's bid: the best object's price + second - best + 1; each bidder's
+ * private dummy at n * max + n + 1); an object takes its highest bid, ties to
+ * the lowest bidder.  The result is deterministic and carries the reference's
+ * guarantee (exact maximum cardinality, base cost within n_bidders of the
+ * minimum of that cardinality); where several assignments meet it, it need
+ * not be the reference's sequential queue's.  assignment[n_bidders]: object
+ * or -1 (unmatched); stats[4] = bids that took an object, candidates examined,
+ * unmatched bidders, rounds.  Errors: TEXO_GP_ERR_AUCTION_* as the
+ * reference's AuctionError. */
+int texo_gp_auction(int32_t device, uint64_t n_bidders, uint64_t n_objects, uint64_t max_base_cost,
+                    const uint32_t *domain_start, const uint32_t *domain_object, const uint64_t *domain_cost,
+                    int64_t *assignment, uint64_t *stats);
 
 /* Last error message of the calling thread (static storage). */
 const char *texo_gp_last_error(void);

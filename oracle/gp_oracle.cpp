@@ -18,6 +18,7 @@
 #include "gp_oracle.h"
 
 #include <algorithm>
+#include <deque>
 #include <array>
 #include <cctype>
 #include <cfloat>
@@ -2429,6 +2430,97 @@ void fill_report_tail(oracle_session *s, texo_gp_report *r, const std::vector<Fi
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Legalization auction (crates/texo-pnr/src/legalization/auction.rs).
+// ---------------------------------------------------------------------------
+struct AuctionOutcome {
+  std::vector<int64_t> assignment;  // object, or -1 for the bidder's private dummy
+  uint64_t bids = 0, examined = 0, unmatched = 0;
+};
+
+// assign_implicitly (auction.rs:123-253) over explicit domains; the two-best
+// search is the reference tests' explicit_best_two (auction.rs:258-276): the
+// two smallest (reduced cost, object) of the bidder's domain at the current
+// prices, reduced cost = base cost + price (cost scale 1).  Domains are
+// validated up front: objects in range, base costs <= max_base_cost, no
+// object twice in one domain.
+AuctionOutcome assign_explicitly(size_t bidders, size_t objects, uint64_t max_base, const uint32_t *dstart,
+                                 const uint32_t *dobj, const uint64_t *dcost) {
+  AuctionOutcome out;
+  if (bidders == 0) return out;
+  for (size_t b = 0; b < bidders; ++b) {
+    for (uint32_t k = dstart[b]; k < dstart[b + 1]; ++k) {
+      if (dobj[k] >= objects) fail(TEXO_GP_ERR_AUCTION_OBJECT_OUT_OF_RANGE, (int64_t)b, (int64_t)k);
+      if (dcost[k] > max_base) fail(TEXO_GP_ERR_AUCTION_BASE_COST_OUT_OF_RANGE, (int64_t)b, (int64_t)k);
+      for (uint32_t j = dstart[b]; j < k; ++j)
+        if (dobj[j] == dobj[k]) fail(TEXO_GP_ERR_AUCTION_DUPLICATE_BEST_OBJECT, (int64_t)b, (int64_t)k);
+    }
+  }
+  // the private dummy's cost n * max + n + 1, and every price + max base cost
+  // representable (auction.rs:145-167)
+  uint64_t dummy = 0, bound = 0;
+  if (__builtin_mul_overflow((uint64_t)bidders, max_base, &dummy) ||
+      __builtin_add_overflow(dummy, (uint64_t)bidders, &dummy) || __builtin_add_overflow(dummy, (uint64_t)1, &dummy) ||
+      __builtin_add_overflow(dummy, (uint64_t)1, &bound) || __builtin_add_overflow(bound, max_base, &bound))
+    fail(TEXO_GP_ERR_AUCTION_OVERFLOW);
+  std::vector<uint64_t> prices(objects, 0);
+  std::vector<int64_t> owners(objects, -1);
+  out.assignment.assign(bidders, -1);
+  std::deque<size_t> queue;
+  for (size_t b = 0; b < bidders; ++b) queue.push_back(b);
+  constexpr uint64_t DUMMY = ~0ull;
+  struct Choice {
+    uint64_t object, base, reduced;
+  };
+  auto before = [](const Choice &a, const Choice &b) {
+    return a.reduced < b.reduced || (a.reduced == b.reduced && a.object < b.object);
+  };
+  while (!queue.empty()) {
+    const size_t b = queue.front();
+    queue.pop_front();
+    // the two best real objects at the current prices
+    Choice c[3];
+    int count = 0;
+    for (uint32_t k = dstart[b]; k < dstart[b + 1]; ++k) {
+      const Choice x{dobj[k], dcost[k], dcost[k] + prices[dobj[k]]};
+      if (count < 2) {
+        c[count++] = x;
+        if (count == 2 && before(c[1], c[0])) std::swap(c[0], c[1]);
+      } else if (before(x, c[0])) {
+        c[1] = c[0];
+        c[0] = x;
+      } else if (before(x, c[1])) {
+        c[1] = x;
+      }
+    }
+    out.examined += dstart[b + 1] - dstart[b];
+    // the private dummy; a real object wins an equal-cost tie (auction.rs:205-217)
+    c[count++] = Choice{DUMMY, max_base, dummy};
+    std::sort(c, c + count, before);
+    const Choice best = c[0];
+    if (best.object == DUMMY) {
+      out.assignment[b] = -1;
+      continue;
+    }
+    const Choice second = c[1];
+    uint64_t increment = 0;
+    if (__builtin_sub_overflow(second.reduced, best.reduced, &increment) ||
+        __builtin_add_overflow(increment, (uint64_t)1, &increment) ||
+        __builtin_add_overflow(prices[best.object], increment, &prices[best.object]))
+      fail(TEXO_GP_ERR_AUCTION_OVERFLOW);
+    const int64_t displaced = owners[best.object];
+    owners[best.object] = (int64_t)b;
+    if (displaced >= 0) {
+      out.assignment[(size_t)displaced] = -1;
+      queue.push_back((size_t)displaced);
+    }
+    out.assignment[b] = (int64_t)best.object;
+    out.bids += 1;
+  }
+  for (int64_t a : out.assignment) out.unmatched += a < 0 ? 1 : 0;
+  return out;
+}
+
 extern "C" {
 
 void oracle_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
@@ -3444,6 +3536,25 @@ int oracle_quadratic_seed(const texo_gp_problem *problem, double *out_xy, texo_g
     return 0;
   } catch (const Failure &f) {
     fill_failure(report, f);
+    return f.status;
+  }
+}
+
+
+// ---- legalization auction ---------------------------------------------------
+int oracle_auction(uint64_t n_bidders, uint64_t n_objects, uint64_t max_base_cost, const uint32_t *domain_start,
+                   const uint32_t *domain_object, const uint64_t *domain_cost, int64_t *assignment,
+                   uint64_t *stats) {
+  try {
+    AuctionOutcome r = assign_explicitly(n_bidders, n_objects, max_base_cost, domain_start, domain_object, domain_cost);
+    std::copy(r.assignment.begin(), r.assignment.end(), assignment);
+    if (stats) {
+      stats[0] = r.bids;
+      stats[1] = r.examined;
+      stats[2] = r.unmatched;
+    }
+    return 0;
+  } catch (const Failure &f) {
     return f.status;
   }
 }

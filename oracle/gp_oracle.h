@@ -299,6 +299,16 @@ int oracle_solve_axis(uint32_t n, const double *diagonal, const uint32_t *adjace
  * interleaved.  report->iterations = PCG iterations of x plus y. */
 int oracle_quadratic_seed(const texo_gp_problem *problem, double *out_xy, texo_gp_report *report);
 
+/* ---- legalization auction ----------------------------------------------- */
+/* assign_implicitly (legalization/auction.rs:123-253) over explicit domains
+ * (CSR by bidder: object, base cost), the two-best search of the reference's
+ * tests (auction.rs:258-276).  assignment[n_bidders]: object or -1 (the
+ * bidder's private dummy); stats[3] = bids, candidates examined, unmatched.
+ * Returns 0 or an AuctionError status (TEXO_GP_ERR_AUCTION_*). */
+int oracle_auction(uint64_t n_bidders, uint64_t n_objects, uint64_t max_base_cost, const uint32_t *domain_start,
+                   const uint32_t *domain_object, const uint64_t *domain_cost, int64_t *assignment,
+                   uint64_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

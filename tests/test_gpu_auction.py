@@ -1,0 +1,138 @@
+"""The legalization auction on the GPU (texo_gp_auction) against the
+reference's auction tests (legalization/auction.rs:354-506) and the oracle's
+sequential restatement.
+
+The GPU auction bids all unassigned bidders in parallel rounds, so where
+several assignments meet the reference's guarantee its assignment may differ
+from the sequential queue's.  What the reference's tests assert is asserted
+here: the known assignments of its small cases, exact maximum cardinality
+(the oracle's unmatched count, the brute-force optimum's on small graphs) and
+base cost within n of the minimum of that cardinality; in addition the
+result is the same on every run."""
+import time
+
+import numpy as np
+import pytest
+
+from texo_amd import abi
+from texo_amd import auction as A
+from tests.auction_cases import grid_instance
+from tests.test_auction_oracle import brute_force, declare, quality
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def olib(oracle_lib):
+    declare(oracle_lib)
+    return oracle_lib
+
+
+def gpu(domains, objects, max_base):
+    start, obj, cost = A.domains_csr(domains)
+    a, st = A.auction(start, obj, cost, objects, max_base)
+    return [None if x < 0 else int(x) for x in a], st
+
+
+def oracle_run(olib, start, obj, cost, objects, max_base):
+    n = start.size - 1
+    out = np.zeros(max(n, 1), np.int64)
+    st = np.zeros(3, np.uint64)
+    o = np.ascontiguousarray(obj if len(obj) else np.zeros(1), np.uint32)
+    c = np.ascontiguousarray(cost if len(cost) else np.zeros(1), np.uint64)
+    import ctypes as C
+    rc = olib.oracle_auction(n, objects, max_base, abi.ptr(start, C.c_uint32), abi.ptr(o, C.c_uint32),
+                             abi.ptr(c, C.c_uint64), abi.ptr(out, C.c_int64), abi.ptr(st, C.c_uint64))
+    assert rc == 0
+    return out[:n], st
+
+
+def base_cost(start, obj, cost, assignment):
+    total = 0
+    for b, a in enumerate(assignment):
+        if a >= 0:
+            row = np.flatnonzero(obj[start[b]:start[b + 1]] == a)
+            assert row.size == 1
+            total += int(cost[start[b] + row[0]])
+    return total
+
+
+def test_reference_known_assignments():
+    # auction.rs:354-418
+    assert gpu([[(0, 0), (1, 1), (2, 100)], [(0, 0), (2, 100)]], 3, 100)[0] == [1, 0]
+    assert gpu([[(0, 0), (1, 1)], [(0, 0)]], 2, 1)[0] == [1, 0]
+    assert gpu([[(1, 0), (2, 0)], [(0, 0), (1, 0)], [(0, 0)]], 3, 0)[0] == [2, 1, 0]
+    domains = [[(o, 0)] for o in range(65)] + [[(o, 0) for o in range(66)]]
+    a1, s1 = gpu(domains, 66, 0)
+    a2, s2 = gpu(domains, 66, 0)
+    assert a1 == a2 and s1[2] == 0 and a1[65] == 65
+    a, st = gpu([[(0, 0)], [(0, 0)]], 1, 0)
+    assert sum(x is not None for x in a) == 1 and st[2] == 1
+
+
+def test_malformed_domains_and_overflow_match_the_reference_errors():
+    for domains, objects, mx, name in (([[]], 0, 2 ** 64 - 1, "AuctionArithmeticOverflow"),
+                                       ([[(3, 0)]], 3, 0, "ObjectOutOfRange"),
+                                       ([[(0, 2)]], 1, 1, "BaseCostOutOfRange"),
+                                       ([[(0, 0), (0, 1)]], 1, 1, "DuplicateBestObject")):
+        with pytest.raises(abi.PlacementError) as err:
+            gpu(domains, objects, mx)
+        assert err.value.name == name
+
+
+def test_small_graphs_against_brute_force():
+    # a sample of the reference's exhaustive small-graph cases
+    # (auction.rs:447-506): every 2 x 0..3 pattern and 600 random 3 x 3 and
+    # 4 x 4 ones; cardinality equals the optimum's and cost is within n of it
+    rng = np.random.default_rng(7)
+    cases = []
+    for objects in range(0, 4):
+        for code in range(3 ** (2 * objects)):
+            d, c = [], code
+            for _ in range(2):
+                row = []
+                for o in range(objects):
+                    e = c % 3
+                    c //= 3
+                    if e:
+                        row.append((o, e - 1))
+                d.append(row)
+            cases.append((d, objects))
+    for _ in range(600):
+        bidders, objects = int(rng.integers(3, 5)), int(rng.integers(1, 5))
+        d = [[(o, int(rng.integers(0, 2))) for o in range(objects) if rng.random() < 0.6] for _ in range(bidders)]
+        cases.append((d, objects))
+    t0 = time.perf_counter()
+    for d, objects in cases:
+        a, _ = gpu(d, objects, 1)
+        got = [o for o in a if o is not None]
+        assert len(got) == len(set(got))
+        actual, optimum = quality(d, a), brute_force(d, objects)
+        assert actual[0] == optimum[0], (d, a)
+        assert actual[1] <= optimum[1] + len(d), (d, a)
+    print(f"{len(cases)} small graphs in {time.perf_counter() - t0:.1f} s", flush=True)
+
+
+@pytest.mark.parametrize("seed,side,per,bidders,radius,sigma",
+                         [(1, 64, 2, 4000, 3, 20.0), (2, 128, 2, 16000, 4, 0.0), (3, 256, 2, 84000, 5, 0.0)])
+def test_grid_instances_match_the_oracle(olib, seed, side, per, bidders, radius, sigma):
+    # legalization-shaped: ring-cost domains around clustered or uniform
+    # targets, no deficiency; the same unmatched count as the sequential queue and base
+    # cost within n of its (both lie within n of the minimum)
+    start, obj, cost, objects, mx = grid_instance(seed, side, per, bidders, radius, sigma)
+    t0 = time.perf_counter()
+    a, st = A.auction(start, obj, cost, objects, mx)
+    t_gpu = time.perf_counter() - t0
+    a2, _ = A.auction(start, obj, cost, objects, mx)
+    t0 = time.perf_counter()
+    ao, so = oracle_run(olib, start, obj, cost, objects, mx)
+    t_cpu = time.perf_counter() - t0
+    cg, co = base_cost(start, obj, cost, a), base_cost(start, obj, cost, ao)
+    print(f"{bidders} bidders on {objects} objects: gpu {t_gpu * 1e3:.1f} ms rounds {st[3]} bids {st[0]} "
+          f"cost {cg} unmatched {st[2]}; oracle {t_cpu * 1e3:.1f} ms bids {so[0]} cost {co} unmatched {so[2]}",
+          flush=True)
+    assert np.array_equal(a, a2)
+    used = a[a >= 0]
+    assert np.unique(used).size == used.size
+    assert st[2] == so[2]
+    assert abs(cg - co) <= bidders

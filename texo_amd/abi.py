@@ -48,6 +48,8 @@ STATUS = {
     211: "Evaluation",
     300: "InvalidMemberArea", 301: "InvalidFillerArea", 302: "PostconditionViolation",
     303: "ArithmeticOverflow",
+    400: "AuctionArithmeticOverflow", 401: "ObjectOutOfRange", 402: "BaseCostOutOfRange",
+    403: "DuplicateBestObject",
     900: "InvalidArgument", 901: "Device", 902: "OutOfMemory", 903: "Unsupported",
     904: "Cancelled", 905: "NotReady", 906: "Injected",
 }
