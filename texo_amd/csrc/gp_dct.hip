@@ -24,6 +24,9 @@ namespace {
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned short ushort8_t __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ double wave_sum(double v) { return lane_reduce<64>(v, OpAdd{}); }
 
@@ -175,7 +178,7 @@ __device__ __forceinline__ void write_energy(const Stage &e, int local_rb, int b
 
 // ---------------------------------------------------------------------------
 // Fast shape (W, H multiples of 32): one 256-thread block per 32x32 output
-// tile, K split over its four wavefronts (wave w takes the 16-wide k-steps
+// tile, K split over its four wavefronts (wave w takes the 32-wide k-steps
 // w, w+4, ...).  The four partial tiles meet in LDS and are summed in wave
 // order, so results do not depend on scheduling.
 // ---------------------------------------------------------------------------
@@ -262,13 +265,74 @@ __device__ unsigned int input_amax(const Task &t, int f) {
   return max(max(sm[0], sm[1]), max(sm[2], sm[3]));
 }
 
+// gfx950's double-rate v_mfma_f32_16x16x32_{bf16,f16}: lane l holds
+// A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16], i = 0..7, eight
+// halves (16 bytes) each; the accumulator layout is that of 16x16x16.
+template <int P>
+__device__ __forceinline__ float4_t mfma32(uint4 a, uint4 b, float4_t c) {
+  if constexpr (P == 1)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, a), __builtin_bit_cast(half8_t, b), c,
+                                                  0, 0, 0);
+}
+
+template <int P>
+__device__ __forceinline__ void split_pair(float x, unsigned short &hi, unsigned short &lo) {
+  if constexpr (P == 1) {
+    hi = bf16_rne(x);
+    lo = bf16_rne(x - bf16_to_float(hi));
+  } else {
+    _Float16 h, l;
+    split_f16(x, h, l);
+    hi = __builtin_bit_cast(unsigned short, h);
+    lo = __builtin_bit_cast(unsigned short, l);
+  }
+}
+
+// Split MFMA of one 32x32 output tile's k-steps w, w + 4, ... (32 wide),
+// P = 1: BF16x3, P = 2: FP16-scaled (A multiplied by `sa` before the split).
+// Each step issues hi*hi + hi*lo + lo*hi with FP32 accumulation.
+template <int P>
+__device__ __forceinline__ void split_steps(const float *a0, const float *a1, const uint4 *bh0, const uint4 *bh1,
+                                            const uint4 *bl0, const uint4 *bl1, int K, int wave, float sa,
+                                            float4_t (&acc)[2][2]) {
+#pragma unroll 2
+  for (int k0 = 32 * wave; k0 < K; k0 += 32 * SPLIT) {
+    const float *ap[2] = {a0 + k0, a1 + k0};
+    uint4 ahi[2], alo[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float4 v0 = *reinterpret_cast<const float4 *>(ap[i]), v1 = *reinterpret_cast<const float4 *>(ap[i] + 4);
+      const float f8[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      ushort8_t h, l;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        unsigned short hs, ls;
+        split_pair<P>(P == 2 ? f8[s] * sa : f8[s], hs, ls);
+        h[s] = hs;
+        l[s] = ls;
+      }
+      ahi[i] = __builtin_bit_cast(uint4, h);
+      alo[i] = __builtin_bit_cast(uint4, l);
+    }
+    const uint4 bhi[2] = {bh0[k0 / 8], bh1[k0 / 8]};
+    const uint4 blo[2] = {bl0[k0 / 8], bl1[k0 / 8]};
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = mfma32<P>(ahi[i], bhi[j], acc[i][j]);
+        acc[i][j] = mfma32<P>(ahi[i], blo[j], acc[i][j]);
+        acc[i][j] = mfma32<P>(alo[i], bhi[j], acc[i][j]);
+      }
+  }
+}
+
 // BF16x3 split-precision MFMA.  A (FP32 activations) is split on the fly
 // into hi + lo BF16, B (static basis) is pre-split and pre-transposed so each
-// fragment is one 8-byte load.  Each 16x16x16 step issues hi*hi + hi*lo +
-// lo*hi with FP32 accumulation.
-// v_mfma_f32_16x16x16_bf16 operand layout (64 lanes): A lane l holds
-// A[l%16][4*(l/16) + i], B lane l holds B[4*(l/16) + i][l%16], and the
-// accumulator lane l holds D[4*(l/16) + i][l%16], i = 0..3.
+// fragment is one 16-byte load.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__restrict__ tasks,
                                                                  BlockPlan p) {
@@ -279,51 +343,23 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__r
   const Stage e = stage_of<S>(t);
   // the task's tiles: 32-row blocks x 32-column tiles, row-major
   const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
-  const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.bthi[S]);
-  const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.btlo[S]);
+  const uint4 *Bth = reinterpret_cast<const uint4 *>(t.j.bthi[S]);
+  const uint4 *Btl = reinterpret_cast<const uint4 *>(t.j.btlo[S]);
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = rb * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
-  const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
-  const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
-  const size_t k4 = (size_t)K / 4;
-  const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;
-  const uint2 *bl0 = Btl + (size_t)(n0 + r) * k4 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k4 + q;
+  const float *a0 = e.A + (size_t)(m0 + r) * K + 8 * q;
+  const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 8 * q;
+  const size_t k8 = (size_t)K / 8;
+  const uint4 *bh0 = Bth + (size_t)(n0 + r) * k8 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k8 + q;
+  const uint4 *bl0 = Btl + (size_t)(n0 + r) * k8 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k8 + q;
   float4_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 16 * wave; k0 < K; k0 += 16 * SPLIT) {
-    const float4 av[2] = {*reinterpret_cast<const float4 *>(a0 + k0), *reinterpret_cast<const float4 *>(a1 + k0)};
-    const uint2 bhv[2] = {bh0[k0 / 4], bh1[k0 / 4]};
-    const uint2 blv[2] = {bl0[k0 / 4], bl1[k0 / 4]};
-    short4_t ahi[2], alo[2], bhi[2], blo[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const float f4[4] = {av[i].x, av[i].y, av[i].z, av[i].w};
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const unsigned short h = bf16_rne(f4[s]);
-        ahi[i][s] = static_cast<short>(h);
-        alo[i][s] = static_cast<short>(bf16_rne(f4[s] - bf16_to_float(h)));
-      }
-      bhi[i] = short4_t{static_cast<short>(bhv[i].x & 0xffff), static_cast<short>(bhv[i].x >> 16),
-                        static_cast<short>(bhv[i].y & 0xffff), static_cast<short>(bhv[i].y >> 16)};
-      blo[i] = short4_t{static_cast<short>(blv[i].x & 0xffff), static_cast<short>(blv[i].x >> 16),
-                        static_cast<short>(blv[i].y & 0xffff), static_cast<short>(blv[i].y >> 16)};
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[i], blo[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo[i], bhi[j], acc[i][j], 0, 0, 0);
-      }
-  }
+  split_steps<1>(a0, a1, bh0, bh1, bl0, bl1, K, wave, 1.f, acc);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -338,7 +374,7 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_bf16_fast(const Task *__r
 // previous stage's epilogue) recorded max |A| per field; A is scaled by a
 // power of two into the top of FP16's range and split into hi + lo FP16 (22
 // significant bits); the basis is pre-scaled, pre-split and pre-transposed.  hi*hi + hi*lo + lo*hi run on
-// v_mfma_f32_16x16x16_f16 with FP32 accumulation; the product of the two
+// v_mfma_f32_16x16x32_f16 with FP32 accumulation; the product of the two
 // power-of-two scales is divided out exactly before the epilogue.
 template <int S>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__restrict__ tasks,
@@ -350,53 +386,26 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
   const Stage e = stage_of<S>(t);
   // the task's tiles: 32-row blocks x 32-column tiles, row-major
   const int gx = (e.N + 31) / 32, rb = bm.y / gx, bx = bm.y - rb * gx;
-  const uint2 *Bth = reinterpret_cast<const uint2 *>(t.j.fthi[S]);
-  const uint2 *Btl = reinterpret_cast<const uint2 *>(t.j.ftlo[S]);
+  const uint4 *Bth = reinterpret_cast<const uint4 *>(t.j.fthi[S]);
+  const uint4 *Btl = reinterpret_cast<const uint4 *>(t.j.ftlo[S]);
   const int K = e.K;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = rb * 32, n0 = bx * 32;
   const int q = lane >> 4, r = lane & 15;
-  const float *a0 = e.A + (size_t)(m0 + r) * K + 4 * q;
-  const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 4 * q;
+  const float *a0 = e.A + (size_t)(m0 + r) * K + 8 * q;
+  const float *a1 = e.A + (size_t)(m0 + 16 + r) * K + 8 * q;
   // max |A| over the 32 rows' field, from the partials of the producer of A
   const float sa = pow2_scale(__uint_as_float(input_amax<S>(t, m0 / e.R)));
   const float unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
-  const size_t k4 = (size_t)K / 4;
-  const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;
-  const uint2 *bl0 = Btl + (size_t)(n0 + r) * k4 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k4 + q;
+  const size_t k8 = (size_t)K / 8;
+  const uint4 *bh0 = Bth + (size_t)(n0 + r) * k8 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k8 + q;
+  const uint4 *bl0 = Btl + (size_t)(n0 + r) * k8 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k8 + q;
   float4_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 16 * wave; k0 < K; k0 += 16 * SPLIT) {
-    const float4 av[2] = {*reinterpret_cast<const float4 *>(a0 + k0), *reinterpret_cast<const float4 *>(a1 + k0)};
-    half4_t ahi[2], alo[2], bhi[2], blo[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const float f4[4] = {av[i].x * sa, av[i].y * sa, av[i].z * sa, av[i].w * sa};
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        _Float16 h, l;
-        split_f16(f4[s], h, l);
-        ahi[i][s] = h;
-        alo[i][s] = l;
-      }
-    }
-    bhi[0] = half4_of(bh0[k0 / 4]);
-    bhi[1] = half4_of(bh1[k0 / 4]);
-    blo[0] = half4_of(bl0[k0 / 4]);
-    blo[1] = half4_of(bl1[k0 / 4]);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], bhi[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi[i], blo[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo[i], bhi[j], acc[i][j], 0, 0, 0);
-      }
-  }
+  split_steps<2>(a0, a1, bh0, bh1, bl0, bl1, K, wave, sa, acc);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -411,7 +420,7 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_f16s_fast(const Task *__r
 // columns per block, for K <= KMAX: the block splits its 32 rows of A into
 // hi + lo once, into LDS, and each wave owns one 32x32 column tile over the
 // whole K, its B fragments coming from the pre-split basis in L2.  The wave
-// accumulates each class of 16-wide k-steps modulo 4 (the k-steps that wave
+// accumulates each class of 32-wide k-steps modulo 4 (the k-steps that wave
 // c of the 32x32 kernel takes) on its own, in the same order, and sums the
 // scaled class results in class order; its epilogue then replays the 32x32
 // kernel's, the four 64-lane quarters of that kernel's block one after the
@@ -432,28 +441,6 @@ union WideLds {
   float red[SPLIT][32][33];
 };
 
-template <int P>
-__device__ __forceinline__ void split_pair(float x, unsigned short &hi, unsigned short &lo) {
-  if constexpr (P == 1) {
-    hi = bf16_rne(x);
-    lo = bf16_rne(x - bf16_to_float(hi));
-  } else {
-    _Float16 h, l;
-    split_f16(x, h, l);
-    hi = __builtin_bit_cast(unsigned short, h);
-    lo = __builtin_bit_cast(unsigned short, l);
-  }
-}
-
-template <int P>
-__device__ __forceinline__ float4_t mfma16(uint2 a, uint2 b, float4_t c) {
-  if constexpr (P == 1)
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a),
-                                                     __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
-  else
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4_t, a), __builtin_bit_cast(half4_t, b),
-                                                 c, 0, 0, 0);
-}
 
 template <int S, int P, int KMAX>
 __global__ void __launch_bounds__(FAST_THREADS) k_gemm_wide(const Task *__restrict__ tasks, BlockPlan p) {
@@ -489,10 +476,11 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_wide(const Task *__restri
     *reinterpret_cast<ushort4 *>(&lds.a.lo[row][4 * c]) = lo;
   }
   __syncthreads();
-  const uint2 *Bth = reinterpret_cast<const uint2 *>(P == 1 ? t.j.bthi[S] : t.j.fthi[S]);
-  const uint2 *Btl = reinterpret_cast<const uint2 *>(P == 1 ? t.j.btlo[S] : t.j.ftlo[S]);
-  const uint2 *bh0 = Bth + (size_t)(n0 + r) * k4 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k4 + q;
-  const uint2 *bl0 = Btl + (size_t)(n0 + r) * k4 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k4 + q;
+  const uint4 *Bth = reinterpret_cast<const uint4 *>(P == 1 ? t.j.bthi[S] : t.j.fthi[S]);
+  const uint4 *Btl = reinterpret_cast<const uint4 *>(P == 1 ? t.j.btlo[S] : t.j.ftlo[S]);
+  const size_t k8 = (size_t)K / 8;
+  const uint4 *bh0 = Bth + (size_t)(n0 + r) * k8 + q, *bh1 = Bth + (size_t)(n0 + 16 + r) * k8 + q;
+  const uint4 *bl0 = Btl + (size_t)(n0 + r) * k8 + q, *bl1 = Btl + (size_t)(n0 + 16 + r) * k8 + q;
   float4_t v[2][2];
 #pragma unroll
   for (int c = 0; c < SPLIT; ++c) {
@@ -502,22 +490,22 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_wide(const Task *__restri
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
-    for (int k = 16 * c; k < K; k += 16 * SPLIT) {
-      const uint2 bhi[2] = {bh0[k / 4], bh1[k / 4]};
-      const uint2 blo[2] = {bl0[k / 4], bl1[k / 4]};
-      uint2 ahi[2], alo[2];
+    for (int k = 32 * c; k < K; k += 32 * SPLIT) {
+      const uint4 bhi[2] = {bh0[k / 8], bh1[k / 8]};
+      const uint4 blo[2] = {bl0[k / 8], bl1[k / 8]};
+      uint4 ahi[2], alo[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        ahi[i] = *reinterpret_cast<const uint2 *>(&lds.a.hi[16 * i + r][k + 4 * q]);
-        alo[i] = *reinterpret_cast<const uint2 *>(&lds.a.lo[16 * i + r][k + 4 * q]);
+        ahi[i] = *reinterpret_cast<const uint4 *>(&lds.a.hi[16 * i + r][k + 8 * q]);
+        alo[i] = *reinterpret_cast<const uint4 *>(&lds.a.lo[16 * i + r][k + 8 * q]);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          acc[i][j] = mfma16<P>(ahi[i], bhi[j], acc[i][j]);
-          acc[i][j] = mfma16<P>(ahi[i], blo[j], acc[i][j]);
-          acc[i][j] = mfma16<P>(alo[i], bhi[j], acc[i][j]);
+          acc[i][j] = mfma32<P>(ahi[i], bhi[j], acc[i][j]);
+          acc[i][j] = mfma32<P>(ahi[i], blo[j], acc[i][j]);
+          acc[i][j] = mfma32<P>(alo[i], bhi[j], acc[i][j]);
         }
     }
     // the 32x32 kernel sums its waves' scaled partials in wave order
