@@ -114,9 +114,10 @@ def test_solve_axis_matches_the_oracle(olib, n, degree, anchored):
     assert rc == 0
     # one PCG step more or less at the stopping threshold at most
     assert abs(it - ito) <= 1
-    # both stop at the same relative residual; their solutions differ by
-    # less than that residual's effect on a coordinate range of ~100
-    assert err <= 1e-3
+    # on the same Krylov path the dot products' summation order is the only
+    # difference (measured <= 3e-11); a step more or less at the stopping
+    # threshold leaves both within the solver's residual of ~100 coordinates
+    assert err <= (1e-8 if it == ito else 1e-3)
 
 
 @pytest.mark.parametrize("name", ["plain_1k", "ecp5_25k", "ecp5_85k"])
@@ -136,5 +137,6 @@ def test_quadratic_seed_matches_the_oracle(olib, name):
           f"{t_cpu * 1e3:.1f} ms, max |dxy| {err:.3e}", flush=True)
     assert rc == 0
     assert abs(it - rep.iterations) <= 2
-    # coordinates on a grid of at most ~200 sites a side: 1e-3 of a site
-    assert err <= 1e-3
+    # the same iterations: the dot products' order is the only difference
+    # (measured <= 1e-11); otherwise 1e-3 of a site on grids of ~200 a side
+    assert err <= (1e-8 if it == rep.iterations else 1e-3)
