@@ -638,6 +638,16 @@ __device__ double wa_group_net(const JobStatic &a, const Task &t, const WaSeg &s
   ymin = group_min<G>(ymin);
   ymax = group_max<G>(ymax);
   double ex[K], enx[K], ey[K], eny[K];
+  // A two-pin net has one exponential per axis: its lower pin's exp(dxm/g)
+  // and its upper pin's exp(-dxn/g) take the same argument, bit for bit
+  // ((xmin - xmax) * ig, the subtraction and the negation being exact), and
+  // the other two are 1.0.
+  const bool pair = G == 1 && seg.degree == 2;
+  double pair_x = 1.0, pair_y = 1.0;
+  if (pair) {
+    if (xmax != xmin) pair_x = exp((xmin - xmax) * ig);
+    if (ymax != ymin) pair_y = exp((ymin - ymax) * ig);
+  }
   WaSums s{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -645,10 +655,17 @@ __device__ double wa_group_net(const JobStatic &a, const Task &t, const WaSeg &s
     if (in[k]) {
       const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
       // exactly 1.0 where the delta is zero (analytical_placement.rs:666-678)
-      ex[k] = dxm == 0.0 ? 1.0 : exp(dxm * ig);
-      enx[k] = dxn == 0.0 ? 1.0 : exp(-dxn * ig);
-      ey[k] = dym == 0.0 ? 1.0 : exp(dym * ig);
-      eny[k] = dyn == 0.0 ? 1.0 : exp(-dyn * ig);
+      if (pair) {
+        ex[k] = dxm == 0.0 ? 1.0 : pair_x;
+        enx[k] = dxn == 0.0 ? 1.0 : pair_x;
+        ey[k] = dym == 0.0 ? 1.0 : pair_y;
+        eny[k] = dyn == 0.0 ? 1.0 : pair_y;
+      } else {
+        ex[k] = dxm == 0.0 ? 1.0 : exp(dxm * ig);
+        enx[k] = dxn == 0.0 ? 1.0 : exp(-dxn * ig);
+        ey[k] = dym == 0.0 ? 1.0 : exp(dym * ig);
+        eny[k] = dyn == 0.0 ? 1.0 : exp(-dyn * ig);
+      }
       s.psx += ex[k];
       s.pdx += ex[k] * dxm;
       s.nsx += enx[k];

@@ -22,6 +22,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <thread>
 #include <vector>
 
 namespace texo {
@@ -142,9 +144,11 @@ __global__ void __launch_bounds__(BLOCK) k_seed_direction(SeedSystems ss, double
 }
 
 // Host side of one axis: the equations, then the gauge.
+// The off-diagonal terms are a CSR whose rows keep insertion order.
 struct HostAxis {
   std::vector<double> diagonal, rhs, initial;
-  std::vector<std::vector<std::pair<uint32_t, double>>> adjacency;
+  std::vector<uint32_t> start, col;
+  std::vector<double> weight;
   std::vector<uint8_t> anchored;
   std::vector<std::vector<uint32_t>> floating;
   uint32_t center = 0;
@@ -200,30 +204,40 @@ void gauge(HostAxis &q) {
       const uint32_t u = pending.back();
       pending.pop_back();
       component.push_back(u);
-      for (const auto &ow : q.adjacency[u])
-        if (!visited[ow.first]) {
-          visited[ow.first] = 1;
-          pending.push_back(ow.first);
+      for (uint32_t k = q.start[u]; k < q.start[u + 1]; ++k)
+        if (!visited[q.col[k]]) {
+          visited[q.col[k]] = 1;
+          pending.push_back(q.col[k]);
         }
     }
-    std::sort(component.begin(), component.end());
     bool anchored = false;
     for (uint32_t u : component) anchored = anchored || q.anchored[u];
     if (anchored) continue;
+    std::sort(component.begin(), component.end());
     g[component[0]] = 1;
     q.floating.push_back(component);
   }
+  // the gauge rows become x = 0; every other row drops its terms on them
+  size_t out = 0;
+  uint32_t row_begin = 0;
   for (size_t u = 0; u < n; ++u) {
+    const uint32_t b = row_begin, e = q.start[u + 1];
+    row_begin = e;
     if (g[u]) {
       q.diagonal[u] = 1.0;
       q.rhs[u] = 0.0;
-      q.adjacency[u].clear();
     } else {
-      auto &a = q.adjacency[u];
-      a.erase(std::remove_if(a.begin(), a.end(), [&](const std::pair<uint32_t, double> &ow) { return g[ow.first] != 0; }),
-              a.end());
+      for (uint32_t k = b; k < e; ++k)
+        if (!g[q.col[k]]) {
+          q.col[out] = q.col[k];
+          q.weight[out] = q.weight[k];
+          ++out;
+        }
     }
+    q.start[u + 1] = (uint32_t)out;
   }
+  q.col.resize(out);
+  q.weight.resize(out);
   for (auto &c : q.floating) {
     const double shift = q.initial[c[0]];
     for (uint32_t u : c) q.initial[u] -= shift;
@@ -267,15 +281,8 @@ uint64_t solve_axes(std::vector<HostAxis *> axes, int device) {
       }
       HostAxis &q = *axes[a];
       const size_t n = q.diagonal.size();
-      std::vector<uint32_t> start(n + 1, 0), col;
-      std::vector<double> w;
-      for (size_t u = 0; u < n; ++u) {
-        for (const auto &ow : q.adjacency[u]) {
-          col.push_back(ow.first);
-          w.push_back(ow.second);
-        }
-        start[u + 1] = (uint32_t)col.size();
-      }
+      const std::vector<uint32_t> &start = q.start, &col = q.col;
+      const std::vector<double> &w = q.weight;
       const size_t nnz = std::max<size_t>(col.size(), 1);
       char *blk = nullptr;
       const size_t o_start = 0, o_col = (4 * (n + 1) + 255) & ~size_t(255);
@@ -458,12 +465,16 @@ int texo_gp_solve_axis(int32_t device, uint32_t n, const double *diagonal, const
     q.anchored.assign(anchored, anchored + n);
     q.initial.assign(initial, initial + n);
     q.center = center;
-    q.adjacency.resize(n);
+    if (adjacency_start[0] != 0) return TEXO_GP_ERR_INVALID_ARGUMENT;
     for (uint32_t u = 0; u < n; ++u)
-      for (uint32_t k = adjacency_start[u]; k < adjacency_start[u + 1]; ++k) {
-        if (adjacency_other[k] >= n) return TEXO_GP_ERR_INVALID_ARGUMENT;
-        q.adjacency[u].push_back({adjacency_other[k], adjacency_weight[k]});
-      }
+      if (adjacency_start[u + 1] < adjacency_start[u]) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    const uint32_t nnz = adjacency_start[n];
+    if (nnz && (!adjacency_other || !adjacency_weight)) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    for (uint32_t k = 0; k < nnz; ++k)
+      if (adjacency_other[k] >= n) return TEXO_GP_ERR_INVALID_ARGUMENT;
+    q.start.assign(adjacency_start, adjacency_start + n + 1);
+    q.col.assign(adjacency_other, adjacency_other + nnz);
+    q.weight.assign(adjacency_weight, adjacency_weight + nnz);
     texo::gauge(q);
     const uint64_t it = texo::solve_axes({&q}, device);
     if (iterations) *iterations = it;
@@ -492,8 +503,7 @@ int texo_gp_quadratic_seed(const texo_gp_problem *p, int32_t device, double *out
     texo::HostAxis axes[2];
     std::vector<texo::SeedPin> pins(n_pins);
     for (uint32_t k = 0; k < n_pins; ++k) pins[k] = {p->cell_unit[p->pin_cell[k]], p->pin_cell[k]};
-    std::vector<double> pp;
-    for (int axis = 0; axis < 2; ++axis) {
+    auto assemble = [&](int axis) {
       texo::HostAxis &q = axes[axis];
       q.center = axis ? p->height / 2 : p->width / 2;
       const double center = (double)q.center;
@@ -504,6 +514,8 @@ int texo_gp_quadratic_seed(const texo_gp_problem *p, int32_t device, double *out
       auto offset = [&](uint32_t cell) { return p->cell_offset_xy[2 * cell + axis]; };
       // linearize_axis at the centre (analytical_placement.rs:283-356)
       std::vector<texo::SeedEdge> edges;
+      edges.reserve(2 * (size_t)n_pins + p->n_arcs);
+      std::vector<double> pp;
       for (uint32_t n = 0; n < p->n_nets; ++n) {
         const uint32_t b = p->net_pin_start[n], e = p->net_pin_start[n + 1];
         const size_t m = e - b;
@@ -529,12 +541,13 @@ int texo_gp_quadratic_seed(const texo_gp_problem *p, int32_t device, double *out
         edges.push_back({d.unit, t.unit, d.cell, t.cell,
                          texo::WEIGHT_SCALE * texo::u64_to_f64(p->arc_extra_weight[a]) / separation});
       }
-      // analytical_axis_equations (lib.rs:4464-4527)
+      // analytical_axis_equations (lib.rs:4464-4527); each row's terms in
+      // edge order
       q.diagonal.assign(U, 0.0);
       q.rhs.assign(U, 0.0);
-      q.adjacency.assign(U, {});
       q.anchored.assign(U, 0);
       q.initial.assign(U, center);
+      q.start.assign(U + 1, 0);
       for (size_t u = 0; u < U; ++u)
         if (p->unit_fixed[u]) {
           q.diagonal[u] = 1.0;
@@ -556,14 +569,42 @@ int texo_gp_quadratic_seed(const texo_gp_problem *p, int32_t device, double *out
         } else {
           q.diagonal[e.left] += w;
           q.diagonal[e.right] += w;
-          q.adjacency[e.left].push_back({e.right, w});
-          q.adjacency[e.right].push_back({e.left, w});
+          q.start[e.left + 1] += 1;
+          q.start[e.right + 1] += 1;
           q.rhs[e.left] += w * (ro - lo);
           q.rhs[e.right] += w * (lo - ro);
         }
       }
+      for (size_t u = 0; u < U; ++u) q.start[u + 1] += q.start[u];
+      q.col.resize(q.start[U]);
+      q.weight.resize(q.start[U]);
+      std::vector<uint32_t> at(q.start.begin(), q.start.end() - 1);
+      for (const texo::SeedEdge &e : edges) {
+        if (p->unit_fixed[e.left] || p->unit_fixed[e.right]) continue;
+        q.col[at[e.left]] = e.right;
+        q.weight[at[e.left]++] = e.weight;
+        q.col[at[e.right]] = e.left;
+        q.weight[at[e.right]++] = e.weight;
+      }
       texo::gauge(q);
+    };
+    // the two axes are independent: y on a second host thread
+    std::exception_ptr y_error;
+    std::thread y_thread([&] {
+      try {
+        assemble(1);
+      } catch (...) {
+        y_error = std::current_exception();
+      }
+    });
+    try {
+      assemble(0);
+    } catch (...) {
+      y_thread.join();
+      throw;
     }
+    y_thread.join();
+    if (y_error) std::rethrow_exception(y_error);
     const uint64_t it = texo::solve_axes({&axes[0], &axes[1]}, device);
     for (size_t u = 0; u < U; ++u) {
       out_xy[2 * u] = axes[0].initial[u];
