@@ -581,10 +581,13 @@ int texo_gp_solve_axis(int32_t device, uint32_t n, const double *diagonal, const
  * unassigned bidder bids in the same round at the same prices (the
  * reference's bid: the best object's price + second - best + 1; each bidder's
  * private dummy at n * max + n + 1); an object takes its highest bid, ties to
- * the lowest bidder.  The result is deterministic and carries the reference's
- * guarantee (exact maximum cardinality, base cost within n_bidders of the
- * minimum of that cardinality); where several assignments meet it, it need
- * not be the reference's sequential queue's.  assignment[n_bidders]: object
+ * the lowest bidder.  Once at most 512 bidders are active (a price war among
+ * the last bidders of a congested region) they bid one at a time on the host
+ * by the reference's rule, from the device's prices.  The result is
+ * deterministic and carries the reference's guarantee (exact maximum
+ * cardinality, base cost within n_bidders of the minimum of that
+ * cardinality); where several assignments meet it, it need not be the
+ * reference's sequential queue's.  assignment[n_bidders]: object
  * or -1 (unmatched); stats[4] = bids that took an object, candidates examined,
  * unmatched bidders, rounds.  Errors: TEXO_GP_ERR_AUCTION_* as the
  * reference's AuctionError. */

@@ -19,12 +19,18 @@
 //
 // A round is four grid launches over the active list (bid, tie-break,
 // resolve, reset) and one swap of the list; the host enqueues rounds in
-// batches and reads the active count back between batches.
+// batches and reads the active count back between batches.  Once at most
+// AUCTION_TAIL_BIDDERS bidders are active the remaining ones bid one at a
+// time on the host, by the reference's rule, from the device's prices and
+// owners: the invariants the guarantee rests on (every assigned object owned,
+// unowned objects at price zero, each assignment within 1 of its bidder's
+// best) hold at every round boundary, so the guarantee carries over.
 
 #include "gp_host.h"
 
 #include <algorithm>
 #include <cstring>
+#include <deque>
 #include <vector>
 
 namespace texo {
@@ -32,6 +38,11 @@ namespace {
 
 constexpr uint32_t NO_WINNER = 0xffffffffu;
 constexpr int AUCTION_ROUNDS_PER_SYNC = 8;
+// From this few active bidders on, a round's launches cost more than its
+// bids: the rest of the auction (typically a price war among the last
+// bidders of a congested region, which needs many small price steps) bids
+// one bidder at a time on the host.
+constexpr uint32_t AUCTION_TAIL_BIDDERS = 512;
 
 struct AuctionState {
   const uint32_t *start, *object;
@@ -162,6 +173,51 @@ __global__ void __launch_bounds__(BLOCK) k_auction_init(AuctionState s, uint64_t
   }
 }
 
+// The reference's bidding loop (auction.rs:169-248) from a state the
+// parallel rounds reached: every assigned bidder owns its object, every
+// object with a price has an owner, the queue holds the unassigned bidders
+// that have not chosen their dummy.  The same bid, tie and dummy rules.
+void sequential_tail(const std::vector<uint32_t> &queue_init, const uint32_t *start, const uint32_t *object,
+                     const uint64_t *cost, uint64_t dummy, std::vector<uint64_t> &price, std::vector<int32_t> &owner,
+                     int64_t *assignment, uint64_t &bids, uint64_t &examined) {
+  std::deque<uint32_t> queue(queue_init.begin(), queue_init.end());
+  const uint64_t od = ~0ull - 1;  // the dummy's object index: above every real one
+  auto before = [](uint64_t ra, uint64_t oa, uint64_t rb, uint64_t ob) { return ra < rb || (ra == rb && oa < ob); };
+  while (!queue.empty()) {
+    const uint32_t b = queue.front();
+    queue.pop_front();
+    uint64_t r0 = ~0ull, o0 = ~0ull, c0 = 0, r1 = ~0ull, o1 = ~0ull;
+    for (uint32_t k = start[b]; k < start[b + 1]; ++k) {
+      const uint64_t o = object[k], c = cost[k], r = c + price[o];
+      if (before(r, o, r0, o0)) {
+        r1 = r0;
+        o1 = o0;
+        r0 = r;
+        o0 = o;
+        c0 = c;
+      } else if (before(r, o, r1, o1)) {
+        r1 = r;
+        o1 = o;
+      }
+    }
+    examined += start[b + 1] - start[b];
+    if (before(dummy, od, r0, o0)) {
+      assignment[b] = -1;
+      continue;
+    }
+    if (before(dummy, od, r1, o1)) r1 = dummy;
+    price[o0] = r1 - c0 + 1;
+    const int32_t prev = owner[o0];
+    owner[o0] = (int32_t)b;
+    if (prev >= 0) {
+      assignment[prev] = -1;
+      queue.push_back((uint32_t)prev);
+    }
+    assignment[b] = (int64_t)o0;
+    bids += 1;
+  }
+}
+
 }  // namespace
 }  // namespace texo
 
@@ -251,6 +307,7 @@ extern "C" int texo_gp_auction(int32_t device, uint64_t n_bidders, uint64_t n_ob
     // every round raises a price by at least 1 or retires a bidder, and no
     // price exceeds the bound: the loop ends; the cap only guards the host
     const uint64_t cap = 1ull << 40;
+    uint32_t active = 0;
     for (;;) {
       for (int r = 0; r < AUCTION_ROUNDS_PER_SYNC; ++r, parity ^= 1) {
         hipLaunchKernelGGL(k_auction_bid, dim3(grid), dim3(BLOCK), 0, s, st, parity);
@@ -260,20 +317,34 @@ extern "C" int texo_gp_auction(int32_t device, uint64_t n_bidders, uint64_t n_ob
         hipLaunchKernelGGL(k_auction_swap, dim3(1), dim3(64), 0, s, st, parity);
       }
       HIP(hipGetLastError());
-      uint32_t active = 0;
       HIP(hipMemcpyAsync(&active, st.count + parity, sizeof active, hipMemcpyDeviceToHost, s));
       HIP(hipStreamSynchronize(s));
       rounds += AUCTION_ROUNDS_PER_SYNC;
-      if (active == 0) break;
+      if (active <= AUCTION_TAIL_BIDDERS) break;
       if (rounds >= cap) fail(TEXO_GP_ERR_DEVICE);
     }
     unsigned long long dev_stats[2] = {0, 0};
     HIP(hipMemcpyAsync(assignment, st.assignment, 8 * n, hipMemcpyDeviceToHost, s));
     HIP(hipMemcpyAsync(dev_stats, st.stats, 16, hipMemcpyDeviceToHost, s));
+    uint64_t tail_bids = 0, tail_examined = 0;
+    if (active > 0) {
+      // the few bidders still active bid one at a time on the host, from the
+      // device's prices and owners, in bidder order
+      std::vector<uint64_t> price(m);
+      std::vector<int32_t> owner(m);
+      std::vector<uint32_t> queue_init(active);
+      HIP(hipMemcpyAsync(price.data(), st.price, 8 * m, hipMemcpyDeviceToHost, s));
+      HIP(hipMemcpyAsync(owner.data(), st.owner, 4 * m, hipMemcpyDeviceToHost, s));
+      HIP(hipMemcpyAsync(queue_init.data(), st.list[parity], 4 * (size_t)active, hipMemcpyDeviceToHost, s));
+      HIP(hipStreamSynchronize(s));
+      std::sort(queue_init.begin(), queue_init.end());
+      sequential_tail(queue_init, domain_start, domain_object, domain_cost, dummy, price, owner, assignment,
+                      tail_bids, tail_examined);
+    }
     HIP(hipStreamSynchronize(s));
     if (stats) {
-      stats[0] = dev_stats[0];
-      stats[1] = dev_stats[1];
+      stats[0] = dev_stats[0] + tail_bids;
+      stats[1] = dev_stats[1] + tail_examined;
       uint64_t unmatched = 0;
       for (size_t b = 0; b < n; ++b) unmatched += assignment[b] < 0 ? 1 : 0;
       stats[2] = unmatched;
