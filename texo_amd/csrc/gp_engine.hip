@@ -492,7 +492,7 @@ void Engine::sort_now(Job *j) {
 // requests for the next round are recorded.
 void Engine::plan(Job *j) {
   j->evals.clear();
-  j->want_precond = j->want_extrapolate = j->want_candidate = j->want_sort = false;
+  j->want_precond = j->want_extrapolate = j->want_candidate = j->want_sort = j->want_spec = false;
   for (;;) {
     switch (j->phase) {
       case Phase::Probe: {
@@ -556,12 +556,12 @@ void Engine::plan(Job *j) {
         if (!std::isfinite(j->next_momentum)) fail(TEXO_GP_ERR_ITERATION_OVERFLOW);
         j->beta = (m - 1.0) / j->next_momentum;
         j->want_precond = true;
-        const int p = j->p_slot, q = 1 - p;
-        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, j->p_valid, false, p});
+        const int p = j->p_slot, q = (p + 1) % WORKING_SLOTS;
+        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, j->p_valid, false, scalar_of_slot(p)});
         j->y_evaluated = j->beta != 0.0;
         if (j->y_evaluated) {
           j->want_extrapolate = true;
-          j->evals.push_back({q, j->d_vec[j->iy], j->d_grad[1], true, j->gamma, false, false, q});
+          j->evals.push_back({q, j->d_vec[j->iy], j->d_grad[1], true, j->gamma, false, false, scalar_of_slot(q)});
         }
         return;
       }
@@ -584,8 +584,17 @@ void Engine::plan(Job *j) {
         // many charges outside their windows at the last evaluation: sort
         // again, at this candidate
         j->want_sort = (size_t)j->moved * MOVED_RESORT > j->M + j->Fl;
-        const int q = 1 - j->p_slot;
-        j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, q});
+        const int q = (j->p_slot + 1) % WORKING_SLOTS, r = (j->p_slot + 2) % WORKING_SLOTS;
+        j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, scalar_of_slot(q)});
+        // the candidate of 2L beside it: what the next trial evaluates when
+        // this one only grows L (global_placement.rs:302-309), the common
+        // outcome of an iteration's first trial; same coordinates, same
+        // results, one round fewer
+        j->want_spec = speculate_ && std::isfinite(2.0 * j->L) && std::isfinite(2.0 * j->L * mmax);
+        if (j->want_spec) {
+          j->spec_L = 2.0 * j->L;
+          j->evals.push_back({r, j->d_vec[j->ic2], j->d_grad[3], true, j->gamma, false, false, scalar_of_slot(r)});
+        }
         return;
       }
       case Phase::RoundEnd: {
@@ -806,6 +815,75 @@ void decide_y(Job *j, hipStream_t s) {
   }
 }
 
+// One trial's outcome (global_placement.rs:241-311) from its step scalars
+// and its candidate's evaluation in working slot `slot` (coordinates
+// d_vec[ic], or d_vec[ic2] for the speculative candidate).  Returns true
+// when the trial only doubled L, the next trial being due.
+bool trial_outcome(Job *j, const StepScalars &st, const EvalScalars &ce, int slot, bool spec, hipStream_t s) {
+  if (!st.changed) {
+    // candidate == y (global_placement.rs:241-272)
+    if (j->may_reduce) {
+      const double smaller = half_nonzero(j->L);
+      if (smaller < j->L) {
+        j->L = smaller;
+        return false;
+      }
+    }
+    if (!j->y_is_origin) {
+      j->has_wl = true;
+      j->wl = j->L;
+      restart_from_origin(j);
+      decide_y(j, s);
+      return false;
+    }
+    stationary(j, true, j->L, s);
+    return false;
+  }
+  j->count_evaluation();
+  const double value = j->combined_value(ce);
+  check_evaluation(ce, value);
+  // majorizer and monotonicity (global_placement.rs:274-290)
+  const double upper = j->y_value + st.gdelta + 0.5 * j->L * st.msq;
+  if (!std::isfinite(upper)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+  const bool smooth = value <= upper + rounding_margin(value, upper);
+  const bool monotone = value <= j->origin_value + rounding_margin(value, j->origin_value);
+  if (smooth && monotone) {
+    const double stat = kkt_of(ce);
+    if (!std::isfinite(stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+    // previous <- origin, coordinates <- candidate (global_placement.rs:322-325)
+    const int old_prev = j->ixp;
+    j->ixp = j->ix;
+    int &cand = spec ? j->ic2 : j->ic;
+    j->ix = cand;
+    cand = old_prev;
+    j->momentum = j->next_momentum;
+    j->has_lipschitz = true;
+    j->lipschitz = j->L;
+    j->current = ce;
+    j->p_slot = slot;
+    j->p_valid = true;
+    if (j->opt.trace)
+      std::fprintf(stderr,
+                   "TEXO_PNR_METRICS eplace iteration=%llu objective=%.9e stationarity=%.9e "
+                   "coordinate_change=%.9e line_search_trials=%llu gamma=%.9e wa=%.9e\n",
+                   (unsigned long long)j->iterations, value, stat, bits_to_double(st.change_bits),
+                   (unsigned long long)j->trials, j->gamma, ce.wa_value);
+    iteration_end(j, true, s);
+    return false;
+  }
+  if (smooth && !j->restart) {
+    j->has_wl = true;
+    j->wl = j->L;
+    restart_from_origin(j);
+    decide_y(j, s);
+    return false;
+  }
+  j->L *= 2.0;
+  j->may_reduce = false;
+  if (!std::isfinite(j->L)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+  return true;
+}
+
 }  // namespace
 
 void Engine::advance(Job *j) {
@@ -887,8 +965,8 @@ void Engine::advance(Job *j) {
       // origin (and y) evaluated (global_placement.rs:143-182)
       if (hs.metric.nonfinite) fail(TEXO_GP_ERR_INVALID_NORMALIZATION);
       j->count_evaluation();
-      const int p = j->p_slot, q = 1 - p;
-      j->origin_s = hs.eval[p];
+      const int p = j->p_slot, q = (p + 1) % WORKING_SLOTS;
+      j->origin_s = hs.eval[scalar_of_slot(p)];
       j->origin_value = j->combined_value(j->origin_s);
       check_evaluation(j->origin_s, j->origin_value);
       j->p_valid = true;
@@ -896,7 +974,7 @@ void Engine::advance(Job *j) {
       if (j->y_evaluated && hs.step.differs) {
         j->y_is_origin = false;
         j->count_evaluation();
-        j->y_s = hs.eval[q];
+        j->y_s = hs.eval[scalar_of_slot(q)];
         j->y_value = j->combined_value(j->y_s);
         check_evaluation(j->y_s, j->y_value);
       }
@@ -908,69 +986,21 @@ void Engine::advance(Job *j) {
       return;
     }
     case Phase::Trial: {
-      const StepScalars st = hs.step;
-      if (!st.changed) {
-        // candidate == y (global_placement.rs:241-272)
-        if (j->may_reduce) {
-          const double smaller = half_nonzero(j->L);
-          if (smaller < j->L) {
-            j->L = smaller;
-            return;
-          }
-        }
-        if (!j->y_is_origin) {
-          j->has_wl = true;
-          j->wl = j->L;
-          restart_from_origin(j);
-          decide_y(j, cur());
-          return;
-        }
-        stationary(j, true, j->L, cur());
-        return;
+      const int q = (j->p_slot + 1) % WORKING_SLOTS, r = (j->p_slot + 2) % WORKING_SLOTS;
+      const bool spec = j->want_spec;  // planned and evaluated in the round just finished
+      if (!trial_outcome(j, hs.step, hs.eval[scalar_of_slot(q)], q, false, cur())) return;
+      // the trial only grew L: the speculative candidate is the next trial's
+      if (!spec || j->L != j->spec_L) return;
+      const double mmax = bits_to_double(hs.metric.max_bits), mmin = bits_to_double(hs.metric.min_bits);
+      // the per-trial steps of plan (global_placement.rs:213-240); L * mmin
+      // cannot vanish here (the trial's L / 2 did not)
+      if (!std::isfinite(j->L * mmax) || !std::isfinite(j->L * mmin)) {
+        j->trials += 1;
+        fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
       }
-      j->count_evaluation();
-      const int q = 1 - j->p_slot;
-      const EvalScalars ce = hs.eval[q];
-      const double value = j->combined_value(ce);
-      check_evaluation(ce, value);
-      // majorizer and monotonicity (global_placement.rs:274-290)
-      const double upper = j->y_value + st.gdelta + 0.5 * j->L * st.msq;
-      if (!std::isfinite(upper)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
-      const bool smooth = value <= upper + rounding_margin(value, upper);
-      const bool monotone = value <= j->origin_value + rounding_margin(value, j->origin_value);
-      if (smooth && monotone) {
-        const double stat = kkt_of(ce);
-        if (!std::isfinite(stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
-        // previous <- origin, coordinates <- candidate (global_placement.rs:322-325)
-        const int old_prev = j->ixp;
-        j->ixp = j->ix;
-        j->ix = j->ic;
-        j->ic = old_prev;
-        j->momentum = j->next_momentum;
-        j->has_lipschitz = true;
-        j->lipschitz = j->L;
-        j->current = ce;
-        j->p_slot = q;
-        j->p_valid = true;
-        if (j->opt.trace)
-          std::fprintf(stderr,
-                       "TEXO_PNR_METRICS eplace iteration=%llu objective=%.9e stationarity=%.9e "
-                       "coordinate_change=%.9e line_search_trials=%llu gamma=%.9e wa=%.9e\n",
-                       (unsigned long long)j->iterations, value, stat, bits_to_double(st.change_bits),
-                       (unsigned long long)j->trials, j->gamma, ce.wa_value);
-        iteration_end(j, true, cur());
-        return;
-      }
-      if (smooth && !j->restart) {
-        j->has_wl = true;
-        j->wl = j->L;
-        restart_from_origin(j);
-        decide_y(j, cur());
-        return;
-      }
-      j->L *= 2.0;
-      j->may_reduce = false;
-      if (!std::isfinite(j->L)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
+      if (j->L * mmin == 0.0) return;
+      j->trials += 1;
+      (void)trial_outcome(j, hs.step_spec, hs.eval[scalar_of_slot(r)], r, true, cur());
       return;
     }
     default:
@@ -990,6 +1020,9 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   g_timer = &timer;
   std::vector<Job *> &active = cx.active;
   active.clear();
+  size_t live = 0;
+  for (Job *j : jobs) live += j->phase != Phase::Done && j->phase != Phase::Failed && j->phase != Phase::Paused;
+  speculate_ = live <= SPECULATE_MAX_JOBS;
   for (Job *j : jobs) {
     if (j->phase == Phase::Done || j->phase == Phase::Failed || j->phase == Phase::Paused) continue;
     try {
@@ -1080,6 +1113,22 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
         l_cand.push_back(oi);
       } else {
         p_cand.none();
+      }
+      if (j->want_spec) {
+        // the speculative candidate at 2L, its own step partials and scalars
+        OptTask sp = o;
+        sp.do_precond = sp.do_extrapolate = 0;
+        sp.do_candidate = 1;
+        sp.cand = j->d_vec[j->ic2];
+        sp.lipschitz = j->spec_L;
+        sp.part_step = j->d_part_step_spec;
+        sp.step = &j->d_scalars->step_spec;
+        const int si = (int)opt.size();
+        opt.push_back(sp);
+        p_pre.none();
+        p_ext.none();
+        p_cand.add(j->C, BLOCK);
+        l_cand.push_back(si);
       }
     }
     // the nets of fixed pins run when their value at this round's gamma is

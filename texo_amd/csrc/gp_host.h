@@ -134,10 +134,19 @@ struct Control {
 
 // Scalars of one job read back by the host after every round.
 struct JobScalars {
-  EvalScalars eval[3];  // working slot 0, working slot 1, parity evaluation
-  StepScalars step;
+  EvalScalars eval[4];  // working slots 0 and 1, the parity evaluation, working slot 2
+  StepScalars step, step_spec;  // the trial candidate, the speculative next one
   MetricScalars metric;
 };
+// Working slots of a job: its current point's field, a trial's, and the
+// speculative next trial's.
+constexpr int WORKING_SLOTS = 3;
+// The scalar record of working slot s (record 2 is the parity evaluation's).
+inline int scalar_of_slot(int s) { return s == 2 ? 3 : s; }
+// A round of at most this many jobs evaluates, beside each trial candidate,
+// the candidate of the step length the trial doubles to when it is rejected
+// (a single job leaves most of the GPU idle; a batch fills it).
+constexpr size_t SPECULATE_MAX_JOBS = 2;
 
 struct AreaMember {
   uint32_t kind;
@@ -256,9 +265,10 @@ struct Job {
   double *d_net_weight = nullptr, *d_wl_diag = nullptr;
   int32_t *d_unit_du = nullptr;
   uint32_t *d_u_mstart = nullptr;
-  double *d_lower = nullptr, *d_upper = nullptr, *d_metric = nullptr, *d_part_metric = nullptr, *d_part_step = nullptr;
-  double *d_vec[4] = {};   // coordinate buffers (rotated by index)
-  double *d_grad[3] = {};  // origin, y, candidate gradients
+  double *d_lower = nullptr, *d_upper = nullptr, *d_metric = nullptr, *d_part_metric = nullptr, *d_part_step = nullptr,
+         *d_part_step_spec = nullptr;
+  double *d_vec[5] = {};   // coordinate buffers (rotated by index)
+  double *d_grad[4] = {};  // origin, y, candidate, speculative candidate gradients
   uint32_t *d_best = nullptr;
   struct Slot {
     long long *acc_m = nullptr, *acc_f = nullptr;
@@ -270,7 +280,7 @@ struct Job {
     long long *part_conv = nullptr;
     uint32_t *tile_flag = nullptr;
     double *part_wa = nullptr, *part_energy = nullptr, *part_gf = nullptr, *part_ug = nullptr;
-  } slot[2];
+  } slot[WORKING_SLOTS];
   double *d_rudy = nullptr;
   long long *d_rudy_fx = nullptr;
   unsigned int *d_rudy_err = nullptr;
@@ -331,8 +341,8 @@ struct Job {
   bool want_targets = false;
   std::vector<uint32_t> prev_stationary;
   bool has_prev_stationary = false;
-  int ix = 0, ixp = 1, iy = 2, ic = 3;
-  int p_slot = 0;
+  int ix = 0, ixp = 1, iy = 2, ic = 3, ic2 = 4;
+  int p_slot = 0;  // working slot of the current point's field; trials use the next two
   bool p_valid = false;
   double momentum = 1.0, lipschitz = 0.0;
   bool has_lipschitz = false;
@@ -354,6 +364,9 @@ struct Job {
   };
   std::vector<EvalReq> evals;
   bool want_precond = false, want_extrapolate = false, want_candidate = false, want_sort = false;
+  // the speculative trial of this round: its step length (twice the trial's)
+  bool want_spec = false;
+  double spec_L = 0.0;
   uint32_t moved = 0;  // most charges any density evaluation of the last round deposited outside their window
 
   // schedule helpers
@@ -476,6 +489,7 @@ class Engine {
   static constexpr int GRAPH_AFTER_ROUNDS = 3;
   static constexpr size_t GRAPH_CACHE = 64;
   bool use_graphs_ = false;  // TEXO_GP_GRAPHS=1 replays recurring rounds from captured graphs
+  bool speculate_ = false;   // this round evaluates speculative trials (few jobs)
   uint64_t graph_rounds_ = 0;
   void drop_graphs(RoundCtx &c);
   RoundCtx ctx_[ROUND_CONTEXTS], aux_;  // aux_: set-up sorts and parity evaluations (on context 0's stream)

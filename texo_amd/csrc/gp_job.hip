@@ -884,7 +884,8 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   arena.reserve(d_metric, C);
   arena.reserve(d_part_metric, blocks(U + Fl, BLOCK) * 3);
   arena.reserve(d_part_step, blocks(C, BLOCK) * 4);
-  for (int k = 0; k < 4; ++k)
+  arena.reserve(d_part_step_spec, blocks(C, BLOCK) * 4);
+  for (int k = 0; k < 5; ++k)
     if (k != ix) arena.reserve(d_vec[k], C);
   arena.reserve(d_s_ci, n);
   arena.reserve(d_s_off, off32 ? 0 : n);
