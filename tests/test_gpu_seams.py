@@ -122,3 +122,25 @@ def test_nesterov_counts_and_schedule_match_oracle(name, iterations):
     assert abs(rg.final_hpwl - ro.final_hpwl) <= 1e-5 * ro.final_hpwl
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("name", ["plain_1k", "ecp5_25k", "ecp5_85k"])
+def test_reported_hpwl_matches_oracle_at_the_same_coordinates(name):
+    # The report's final HPWL is an FP64 reduction on the device over the
+    # job's nets at its current coordinates; the oracle's continuous_hpwl
+    # (pin coordinate map of analytical_placement.rs:589-597) sums the same
+    # nets sequentially: equal within the summation-order rounding.
+    print(f"reported hpwl {name}", flush=True)
+    problem = synth.CONFIGS[name]()
+    with texo_amd.Batch() as b:
+        b.add(problem, abi.options(dct_precision=abi.DCT_FP32))
+        b.iterate(3)
+        _, rep = b.result(0, targets=False)
+        x = b.coordinates(0)
+    olib = oracle.load()
+    o = abi.Session(olib, "oracle", problem)
+    ho = olib.oracle_session_hpwl(o.handle, x.ctypes.data_as(C.POINTER(C.c_double)))
+    o.close()
+    print(f"  device {rep['final_hpwl']!r} oracle {ho!r}", flush=True)
+    assert ho > 0.0
+    assert abs(rep["final_hpwl"] - ho) <= 1e-12 * ho

@@ -57,17 +57,17 @@ void fill_report_at(Job &j, const std::vector<double> *x, texo_gp_report *r) {
 
 void fill_report(Engine &e, Job &j, texo_gp_report *r) {
   if (!r) return;
-  std::vector<double> x(2 * j.U);
-  if (j.U) {
-    if (j.empty) {
-      x = j.initial_xy;
-    } else {
-      const hipStream_t st = e.stream();
-      HIP(hipMemcpyAsync(x.data(), j.d_vec[j.ix], 2 * j.U * sizeof(double), hipMemcpyDeviceToHost, st));
-      HIP(hipStreamSynchronize(st));
-    }
+  if (j.U && !j.empty && j.arena.base()) {
+    // the final HPWL reduced on the device (FP64) from the current coordinates
+    const hipStream_t st = e.stream();
+    double h = 0.0;
+    j.enqueue_hpwl(j.d_vec[j.ix], &h, st);
+    HIP(hipStreamSynchronize(st));
+    fill_report_at(j, nullptr, r);
+    r->final_hpwl = h;
+    return;
   }
-  fill_report_at(j, &x, r);
+  fill_report_at(j, &j.initial_xy, r);
 }
 
 // Targets of a finished job: the best checkpoint (eplace.rs:553), or the
@@ -704,6 +704,8 @@ struct texo_gp_server {
     texo_gp_report report{};
     void *readback = nullptr;  // pinned landing block of the result copies
     size_t readback_cap = 0;
+    double hpwl = 0.0;  // final HPWL reduced on the device, when has_hpwl
+    bool has_hpwl = false;
   };
   // Result copies of the jobs one retire() found finished, enqueued on the
   // compute stream after their last work; `ev` marks their landing.
@@ -892,10 +894,12 @@ void texo_gp_server::retire(int c, std::vector<int64_t> &group) {
     if (!j.empty && j.arena.base() && j.U) {
       try {
         if (!st) st = engine->stream(c);
-        e.readback = engine->image_pool()->acquire(j.U * 24, &e.readback_cap);
+        e.readback = engine->image_pool()->acquire(j.U * 24 + sizeof(double), &e.readback_cap);
         char *dst = static_cast<char *>(e.readback);
         HIP(hipMemcpyAsync(dst, j.d_vec[j.ix], j.U * 16, hipMemcpyDeviceToHost, st));
         if (j.phase == Phase::Done) HIP(hipMemcpyAsync(dst + j.U * 16, j.d_best, j.U * 8, hipMemcpyDeviceToHost, st));
+        // the report's HPWL, reduced on the device from the same coordinates
+        j.enqueue_hpwl(j.d_vec[j.ix], reinterpret_cast<double *>(dst + j.U * 24), st);
       } catch (...) {
         const Failure f = texo::current_failure();
         j.failure = f;
@@ -960,10 +964,13 @@ void texo_gp_server::collect(bool block) {
         e.x.resize(2 * j.U);
         std::memcpy(e.x.data(), src, j.U * 16);
         if (j.phase == Phase::Done) std::memcpy(e.targets.data(), src + j.U * 16, j.U * 8);
+        std::memcpy(&e.hpwl, src + j.U * 24, sizeof(double));
+        e.has_hpwl = true;
         engine->image_pool()->release(e.readback, e.readback_cap);
         e.readback = nullptr;
       }
       fill_report_at(j, nullptr, &e.report);
+      if (e.has_hpwl) e.report.final_hpwl = e.hpwl;
       e.status = j.phase == Phase::Failed ? j.status : 0;
       e.report.status = e.status;
       e.report.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - e.submitted).count();
@@ -1004,7 +1011,7 @@ static int take_result(texo_gp_server *s, std::unique_lock<std::mutex> &lk, int6
   s->entries.erase(it);
   lk.unlock();
   if (e->job) {
-    e->report.final_hpwl = e->job->hpwl(e->x);
+    if (!e->has_hpwl) e->report.final_hpwl = e->job->hpwl(e->x);
     e->job.reset();
   }
   if (out) std::copy(e->targets.begin(), e->targets.end(), out);

@@ -396,6 +396,12 @@ void launch_optimizer_finalize(const OptTask *tasks, const OptMaps &m, hipStream
 void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, OffArr pin_xy,
                  const double *coords, int W, int H, long long *fx, double *out, unsigned int *error, hipStream_t s);
 
+// Continuous bounding-box HPWL of nets [0, n_nets) at `coords`, FP64 on the
+// device: `part` is max(1, ceil(n_nets / BLOCK)) doubles of scratch, `out`
+// one double.
+void launch_hpwl(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, OffArr pin_xy,
+                 const double *coords, double *part, double *out, hipStream_t s);
+
 // Poisson transform (gp_dct.hip) of every task of one grid size.  map_h has
 // one entry per 32-row block of the F*H-row stages, map_w of the F*W-row
 // stages: (task, local row block).

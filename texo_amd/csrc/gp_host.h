@@ -225,6 +225,8 @@ struct Job {
   size_t n_live_nets = 0;
   // WA value of the nets of fixed pins: held on the device for one gamma
   double *d_fixed_wa = nullptr;
+  // final HPWL on the device: block sums of the nets, then the total
+  double *d_hpwl = nullptr;
   bool fixed_wa_valid = false;
   double fixed_wa_gamma = 0.0;
   std::vector<double> pin_xy, net_weight;
@@ -381,6 +383,9 @@ struct Job {
   size_t area_round_limit() const;
   void update_multipliers();
   double hpwl(const std::vector<double> &x) const;
+  // HPWL of the device coordinates `coords` into `out` (device or pinned
+  // host memory), enqueued on `s` (a job with an arena)
+  void enqueue_hpwl(const double *coords, double *out, hipStream_t s) const;
   void stage(const std::shared_ptr<BlockPool> &pool);
   void set_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas);
   void begin_run(bool iterate_mode);

@@ -921,6 +921,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   arena.reserve(d_perm_next, n);
   arena.reserve(d_sort_count, (size_t)n_keys + 1);
   arena.reserve(d_fixed_wa, 1);
+  arena.reserve(d_hpwl, blocks(n_nets, BLOCK) + 1);
 
   if (pool) {
     image = static_cast<char *>(pool->acquire(image_bytes, &image_cap));
@@ -1183,6 +1184,12 @@ double Job::hpwl(const std::vector<double> &x) const {
     total += (xmax - xmin) + (ymax - ymin);
   }
   return total;
+}
+
+void Job::enqueue_hpwl(const double *coords, double *out, hipStream_t s) const {
+  const size_t nb = std::max<size_t>(1, (n_nets + BLOCK - 1) / BLOCK);
+  launch_hpwl((int)n_nets, js.net_start, js.pin_unit, js.pin_xy, coords, d_hpwl, d_hpwl + nb, s);
+  HIP(hipMemcpyAsync(out, d_hpwl + nb, sizeof(double), hipMemcpyDeviceToHost, s));
 }
 
 size_t Job::area_round_limit() const {

@@ -1265,6 +1265,42 @@ __global__ void __launch_bounds__(BLOCK) k_rudy(int n_nets, const uint32_t *__re
   }
 }
 
+// Continuous bounding-box HPWL of a job's nets at `coords` (the reported
+// final HPWL, over the pin coordinate map of analytical_placement.rs:589-597),
+// FP64: one thread per net, a block's sum of its nets, then one block adds
+// the block sums; the same bits for a job alone or among others.
+__global__ void __launch_bounds__(BLOCK) k_hpwl_nets(int n_nets, const uint32_t *__restrict__ net_start,
+                                                     const uint32_t *__restrict__ pin_unit, OffArr pin_xy,
+                                                     const double *__restrict__ coords, double *__restrict__ part) {
+  const int n = blockIdx.x * BLOCK + threadIdx.x;
+  double v[1] = {0.0};
+  if (n < n_nets) {
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (uint32_t p = net_start[n]; p < net_start[n + 1]; ++p) {
+      const uint32_t u = pin_unit[p];
+      const double2 o = pin_xy[p];
+      double px = o.x, py = o.y;
+      if (u != NO_UNIT) {
+        px += coords[2 * (size_t)u];
+        py += coords[2 * (size_t)u + 1];
+      }
+      xmin = fmin(xmin, px);
+      xmax = fmax(xmax, px);
+      ymin = fmin(ymin, py);
+      ymax = fmax(ymax, py);
+    }
+    v[0] = (xmax - xmin) + (ymax - ymin);
+  }
+  block_sum<1>(v, part + blockIdx.x);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_hpwl_total(int n_blocks, const double *__restrict__ part,
+                                                      double *__restrict__ out) {
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < n_blocks; b += BLOCK) v[0] += part[b];
+  block_sum<1>(v, out);
+}
+
 __global__ void __launch_bounds__(BLOCK) k_fill_ll(size_t n, long long *p) {
   const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -1343,6 +1379,13 @@ void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit
                      fx + G, error);
   timer_end(g_timer, "rudy", s, 1);
   hipLaunchKernelGGL(k_fx_to_double, dim3(std::max(1, blocks_for(2 * G))), dim3(BLOCK), 0, s, 2 * G, fx, out);
+}
+
+void launch_hpwl(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, OffArr pin_xy,
+                 const double *coords, double *part, double *out, hipStream_t s) {
+  const int nb = std::max(1, blocks_for((size_t)n_nets));
+  hipLaunchKernelGGL(k_hpwl_nets, dim3(nb), dim3(BLOCK), 0, s, n_nets, net_start, pin_unit, pin_xy, coords, part);
+  hipLaunchKernelGGL(k_hpwl_total, dim3(1), dim3(BLOCK), 0, s, nb, part, out);
 }
 
 }  // namespace texo
