@@ -1020,9 +1020,13 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   g_timer = &timer;
   std::vector<Job *> &active = cx.active;
   active.clear();
-  size_t live = 0;
-  for (Job *j : jobs) live += j->phase != Phase::Done && j->phase != Phase::Failed && j->phase != Phase::Paused;
-  speculate_ = live <= SPECULATE_MAX_JOBS;
+  size_t live = 0, live_charges = 0;
+  for (Job *j : jobs)
+    if (j->phase != Phase::Done && j->phase != Phase::Failed && j->phase != Phase::Paused) {
+      live += 1;
+      live_charges += j->M + j->Fl;
+    }
+  speculate_ = live <= SPECULATE_MAX_JOBS || live_charges <= SPECULATE_MAX_CHARGES;
   for (Job *j : jobs) {
     if (j->phase == Phase::Done || j->phase == Phase::Failed || j->phase == Phase::Paused) continue;
     try {

@@ -143,10 +143,13 @@ struct JobScalars {
 constexpr int WORKING_SLOTS = 3;
 // The scalar record of working slot s (record 2 is the parity evaluation's).
 inline int scalar_of_slot(int s) { return s == 2 ? 3 : s; }
-// A round of at most this many jobs evaluates, beside each trial candidate,
+// A round of at most this many jobs, or of at most this many charges
+// (members + fillers) over its jobs, evaluates beside each trial candidate
 // the candidate of the step length the trial doubles to when it is rejected
-// (a single job leaves most of the GPU idle; a batch fills it).
+// (a single job or a few small ones leave most of the GPU idle; a batch
+// fills it).
 constexpr size_t SPECULATE_MAX_JOBS = 2;
+constexpr size_t SPECULATE_MAX_CHARGES = 1u << 19;
 
 struct AreaMember {
   uint32_t kind;
