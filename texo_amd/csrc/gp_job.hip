@@ -353,32 +353,8 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
     const uint64_t w = p->arc_extra_weight[a];
     net_weight[n_nets + a] = (double)(uint32_t)(w >> 32) * 4294967296.0 + (double)(uint32_t)(w & 0xffffffffu);
   }
-  pin_unit.resize(n_pins_total);
-  pin_xy.resize(2 * n_pins_total);
-  for (size_t i = 0; i < n_pins_total; ++i) {
-    const uint32_t c = pin_cell[i];
-    if (c >= n_cells) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)i);
-    if (p->cell_fixed[c]) {
-      pin_unit[i] = NO_UNIT;
-      pin_xy[2 * i] = p->cell_fixed_xy[2 * c];
-      pin_xy[2 * i + 1] = p->cell_fixed_xy[2 * c + 1];
-    } else {
-      pin_unit[i] = p->cell_unit[c];
-      pin_xy[2 * i] = p->cell_offset_xy[2 * c];
-      pin_xy[2 * i + 1] = p->cell_offset_xy[2 * c + 1];
-    }
-  }
-  // unit -> incident movable pins (pin order; nets before arcs)
-  u_pstart.assign(U + 1, 0);
   for (size_t i = 0; i < n_pins_total; ++i)
-    if (pin_unit[i] != NO_UNIT) u_pstart[pin_unit[i] + 1] += 1;
-  for (size_t u = 0; u < U; ++u) u_pstart[u + 1] += u_pstart[u];
-  u_pins.assign(u_pstart[U], 0);
-  {
-    std::vector<uint32_t> fill(u_pstart.begin(), u_pstart.end() - 1);
-    for (size_t i = 0; i < n_pins_total; ++i)
-      if (pin_unit[i] != NO_UNIT) u_pins[fill[pin_unit[i]]++] = static_cast<uint32_t>(i);
-  }
+    if (pin_cell[i] >= n_cells) fail(TEXO_GP_ERR_INVALID_ARGUMENT, (int64_t)i);
   // wirelength_preconditioner (analytical_placement.rs:177-211) and
   // baseline_cell_incidence_weights (:219-231)
   wl_diag.assign(U, 0.0);
@@ -430,9 +406,19 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
   // that runs once per gamma (an incremental request fixes most units).
   {
     auto degree = [&](uint32_t n) { return net_start[n + 1] - net_start[n]; };
+    // a pin of a fixed cell is a fixed point (its cell's coordinate); a net
+    // with a pin of a movable cell is live.  unit -> number of incident
+    // movable pins (nets before arcs, input order)
     std::vector<uint8_t> live(n_net_total, 0);
+    u_pstart.assign(U + 1, 0);
     for (size_t n = 0; n < n_net_total; ++n)
-      for (uint32_t i = net_start[n]; i < net_start[n + 1] && !live[n]; ++i) live[n] = pin_unit[i] != NO_UNIT;
+      for (uint32_t i = net_start[n]; i < net_start[n + 1]; ++i) {
+        const uint32_t c = pin_cell[i];
+        if (p->cell_fixed[c]) continue;
+        live[n] = 1;
+        u_pstart[p->cell_unit[c] + 1] += 1;
+      }
+    for (size_t u = 0; u < U; ++u) u_pstart[u + 1] += u_pstart[u];
     // per class: stable counting sort of the small nets by degree, then the
     // large nets stably by descending degree
     std::vector<uint32_t> order;
@@ -462,21 +448,49 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
     for (int cls = 1; cls >= 0; --cls)
       for (size_t n = n_nets; n < n_net_total; ++n)
         if (live[n] == cls) order.push_back((uint32_t)n);
-    std::vector<uint32_t> start2(n_net_total + 1, 0), unit2(n_pins_total), new_pin(n_pins_total);
+    // nets and pins go straight to their device positions in one pass over
+    // the input order: each pin's unit (NO_UNIT for a fixed pin) and offset
+    // (or fixed coordinate), and each unit's list of its movable pins in
+    // input order (nets first, then timed arcs), which fixes the summation
+    // order of its gradient; each movable pin's derivative goes to its entry
+    // of that list (pin_slot), so the per-unit sums read contiguously
+    std::vector<uint32_t> start2(n_net_total + 1, 0), unit2(n_pins_total);
     std::vector<double> xy2(2 * n_pins_total), w2(n_net_total);
     std::vector<uint8_t> live2(n_net_total);
-    uint32_t q = 0;
-    for (size_t k = 0; k < n_net_total; ++k) {
-      const uint32_t n = order[k];
-      w2[k] = net_weight[n];
-      live2[k] = live[n];
-      for (uint32_t i = net_start[n]; i < net_start[n + 1]; ++i, ++q) {
-        new_pin[i] = q;
-        unit2[q] = pin_unit[i];
-        xy2[2 * q] = pin_xy[2 * i];
-        xy2[2 * q + 1] = pin_xy[2 * i + 1];
+    u_pins.assign(u_pstart[U], 0);
+    pin_slot.assign(n_pins_total, NO_UNIT);
+    u_nbase.assign(U, 0);
+    {
+      std::vector<uint32_t> rank(n_net_total);
+      for (size_t k = 0; k < n_net_total; ++k) rank[order[k]] = (uint32_t)k;
+      for (size_t n = 0; n < n_net_total; ++n) {
+        const uint32_t k = rank[n];
+        start2[k + 1] = degree((uint32_t)n);
+        w2[k] = net_weight[n];
+        live2[k] = live[n];
       }
-      start2[k + 1] = q;
+      for (size_t k = 0; k < n_net_total; ++k) start2[k + 1] += start2[k];
+      std::vector<uint32_t> fill(u_pstart.begin(), u_pstart.end() - 1);
+      for (size_t n = 0; n < n_net_total; ++n) {
+        uint32_t q = start2[rank[n]];
+        for (uint32_t i = net_start[n]; i < net_start[n + 1]; ++i, ++q) {
+          const uint32_t c = pin_cell[i];
+          if (p->cell_fixed[c]) {
+            unit2[q] = NO_UNIT;
+            xy2[2 * q] = p->cell_fixed_xy[2 * c];
+            xy2[2 * q + 1] = p->cell_fixed_xy[2 * c + 1];
+          } else {
+            const uint32_t u = p->cell_unit[c];
+            unit2[q] = u;
+            xy2[2 * q] = p->cell_offset_xy[2 * c];
+            xy2[2 * q + 1] = p->cell_offset_xy[2 * c + 1];
+            const uint32_t k = fill[u]++;
+            u_pins[k] = q;
+            pin_slot[q] = k;
+            if (i < first_arc_pin) u_nbase[u] += 1;
+          }
+        }
+      }
     }
     // WA segments: runs of consecutive nets of one class with one lane-group
     // size and, for small nets and timed arcs, one degree (a lane then
@@ -511,17 +525,6 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
     }
     if (wa_seg.size() > (size_t)WA_MAX_SEGMENTS) fail(TEXO_GP_ERR_UNSUPPORTED);
     wa_waves = waves;
-    for (auto &i : u_pins) i = new_pin[i];
-    // each pin's derivative goes straight to its unit's entry (nets first,
-    // then timed arcs, each in input order), so the per-unit sums read
-    // contiguously in the reference's order
-    pin_slot.assign(n_pins_total, NO_UNIT);
-    u_nbase.assign(U, 0);
-    for (size_t u = 0; u < U; ++u)
-      for (uint32_t k = u_pstart[u]; k < u_pstart[u + 1]; ++k) {
-        pin_slot[u_pins[k]] = k;
-        if (u_pins[k] < first_arc_pin) u_nbase[u] += 1;
-      }
     net_start.swap(start2);
     pin_unit.swap(unit2);
     pin_xy.swap(xy2);
