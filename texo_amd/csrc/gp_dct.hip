@@ -461,19 +461,35 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_wide(const Task *__restri
     sa = pow2_scale(__uint_as_float(input_amax<S>(t, m0 / e.R)));
     unscale = 1.f / (sa * t.j.f16_scale[S]);  // exact: a power of two
   }
-  // stage A: 32 rows x K, coalesced float4 loads
+  // stage A: 32 rows x K, coalesced float4 loads, every load of the thread
+  // in flight before the first split (a loop of load, split, store waits
+  // out one memory latency per iteration)
   const int k4 = K / 4;
-  for (int idx = threadIdx.x; idx < 32 * k4; idx += FAST_THREADS) {
-    const int row = idx / k4, c = idx - row * k4;
-    float4 v = *reinterpret_cast<const float4 *>(e.A + (size_t)(m0 + row) * K + 4 * c);
-    if constexpr (P == 2) v = make_float4(v.x * sa, v.y * sa, v.z * sa, v.w * sa);
-    ushort4 hi, lo;
-    split_pair<P>(v.x, hi.x, lo.x);
-    split_pair<P>(v.y, hi.y, lo.y);
-    split_pair<P>(v.z, hi.z, lo.z);
-    split_pair<P>(v.w, hi.w, lo.w);
-    *reinterpret_cast<ushort4 *>(&lds.a.hi[row][4 * c]) = hi;
-    *reinterpret_cast<ushort4 *>(&lds.a.lo[row][4 * c]) = lo;
+  constexpr int AL = 32 * (KMAX / 4) / FAST_THREADS;  // loads per thread at K = KMAX
+  float4 va[AL];
+#pragma unroll
+  for (int u = 0; u < AL; ++u) {
+    const int idx = threadIdx.x + u * FAST_THREADS;
+    if (idx < 32 * k4) {
+      const int row = idx / k4, c = idx - row * k4;
+      va[u] = *reinterpret_cast<const float4 *>(e.A + (size_t)(m0 + row) * K + 4 * c);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < AL; ++u) {
+    const int idx = threadIdx.x + u * FAST_THREADS;
+    if (idx < 32 * k4) {
+      const int row = idx / k4, c = idx - row * k4;
+      float4 v = va[u];
+      if constexpr (P == 2) v = make_float4(v.x * sa, v.y * sa, v.z * sa, v.w * sa);
+      ushort4 hi, lo;
+      split_pair<P>(v.x, hi.x, lo.x);
+      split_pair<P>(v.y, hi.y, lo.y);
+      split_pair<P>(v.z, hi.z, lo.z);
+      split_pair<P>(v.w, hi.w, lo.w);
+      *reinterpret_cast<ushort4 *>(&lds.a.hi[row][4 * c]) = hi;
+      *reinterpret_cast<ushort4 *>(&lds.a.lo[row][4 * c]) = lo;
+    }
   }
   __syncthreads();
   const uint4 *Bth = reinterpret_cast<const uint4 *>(P == 1 ? t.j.bthi[S] : t.j.fthi[S]);
