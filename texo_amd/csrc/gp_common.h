@@ -397,10 +397,12 @@ void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit
                  const double *coords, int W, int H, long long *fx, double *out, unsigned int *error, hipStream_t s);
 
 // Continuous bounding-box HPWL of nets [0, n_nets) at `coords`, FP64 on the
-// device: `part` is max(1, ceil(n_nets / BLOCK)) doubles of scratch, `out`
-// one double.
+// device; `large` lists (ascending) the nets of more than 32 pins, which get
+// a wavefront each.  `part` is hpwl_blocks(n_nets, n_large) doubles of
+// scratch, `out` one double.
+int hpwl_blocks(int n_nets, int n_large);
 void launch_hpwl(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, OffArr pin_xy,
-                 const double *coords, double *part, double *out, hipStream_t s);
+                 const double *coords, const uint32_t *large, int n_large, double *part, double *out, hipStream_t s);
 
 // Poisson transform (gp_dct.hip) of every task of one grid size.  map_h has
 // one entry per 32-row block of the F*H-row stages, map_w of the F*W-row

@@ -228,8 +228,11 @@ struct Job {
   size_t n_live_nets = 0;
   // WA value of the nets of fixed pins: held on the device for one gamma
   double *d_fixed_wa = nullptr;
-  // final HPWL on the device: block sums of the nets, then the total
+  // final HPWL on the device: block sums of the nets, then the total; the
+  // nets of more than 32 pins (ascending) take a wavefront each
   double *d_hpwl = nullptr;
+  std::vector<uint32_t> hpwl_large;
+  uint32_t *d_hpwl_large = nullptr;
   bool fixed_wa_valid = false;
   double fixed_wa_gamma = 0.0;
   std::vector<double> pin_xy, net_weight;

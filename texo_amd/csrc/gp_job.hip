@@ -692,6 +692,7 @@ void Job::use_design(const std::shared_ptr<SharedDesign> &sd) {
   move(d_pin_xy32);
   move(d_net_weight);
   move(d_wa_seg);
+  move(d_hpwl_large);
   move(d_u_pstart);
   move(d_pin_slot);
   move(d_u_nbase);
@@ -859,6 +860,10 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   }
   data(d_net_weight, n_net_total, net_weight.data());
   data(d_wa_seg, wa_seg.size(), wa_seg.data());
+  hpwl_large.clear();
+  for (size_t k = 0; k < n_nets; ++k)
+    if (net_start[k + 1] - net_start[k] > 32) hpwl_large.push_back((uint32_t)k);
+  data(d_hpwl_large, hpwl_large.size(), hpwl_large.data());
   data(d_u_pstart, U + 1, u_pstart.data());
   data(d_pin_slot, n_pins_total, pin_slot.data());
   data(d_u_nbase, U, u_nbase.data());
@@ -924,7 +929,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
   arena.reserve(d_perm_next, n);
   arena.reserve(d_sort_count, (size_t)n_keys + 1);
   arena.reserve(d_fixed_wa, 1);
-  arena.reserve(d_hpwl, blocks(n_nets, BLOCK) + 1);
+  arena.reserve(d_hpwl, (size_t)hpwl_blocks((int)n_nets, (int)hpwl_large.size()) + 1);
 
   if (pool) {
     image = static_cast<char *>(pool->acquire(image_bytes, &image_cap));
@@ -1190,8 +1195,9 @@ double Job::hpwl(const std::vector<double> &x) const {
 }
 
 void Job::enqueue_hpwl(const double *coords, double *out, hipStream_t s) const {
-  const size_t nb = std::max<size_t>(1, (n_nets + BLOCK - 1) / BLOCK);
-  launch_hpwl((int)n_nets, js.net_start, js.pin_unit, js.pin_xy, coords, d_hpwl, d_hpwl + nb, s);
+  const size_t nb = (size_t)hpwl_blocks((int)n_nets, (int)hpwl_large.size());
+  launch_hpwl((int)n_nets, js.net_start, js.pin_unit, js.pin_xy, coords, d_hpwl_large, (int)hpwl_large.size(), d_hpwl,
+              d_hpwl + nb, s);
   HIP(hipMemcpyAsync(out, d_hpwl + nb, sizeof(double), hipMemcpyDeviceToHost, s));
 }
 

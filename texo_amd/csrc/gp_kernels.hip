@@ -1267,29 +1267,61 @@ __global__ void __launch_bounds__(BLOCK) k_rudy(int n_nets, const uint32_t *__re
 
 // Continuous bounding-box HPWL of a job's nets at `coords` (the reported
 // final HPWL, over the pin coordinate map of analytical_placement.rs:589-597),
-// FP64: one thread per net, a block's sum of its nets, then one block adds
-// the block sums; the same bits for a job alone or among others.
-__global__ void __launch_bounds__(BLOCK) k_hpwl_nets(int n_nets, const uint32_t *__restrict__ net_start,
+// FP64: the first nb_small blocks take one net per thread (nets of more than
+// HPWL_THREAD_PINS pins contribute nothing there), the blocks after them one
+// listed large net per wavefront; each block writes the sum of its nets, and
+// one block adds the block sums in block order.  The same bits for a job
+// alone or among others.
+constexpr uint32_t HPWL_THREAD_PINS = 32;
+
+__device__ __forceinline__ double2 hpwl_pin(const uint32_t *__restrict__ pin_unit, const OffArr &pin_xy,
+                                            const double *__restrict__ coords, uint32_t p) {
+  const uint32_t u = pin_unit[p];
+  const double2 o = pin_xy[p];
+  if (u == NO_UNIT) return o;
+  return make_double2(o.x + coords[2 * (size_t)u], o.y + coords[2 * (size_t)u + 1]);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_hpwl_nets(int n_nets, int nb_small, const uint32_t *__restrict__ net_start,
                                                      const uint32_t *__restrict__ pin_unit, OffArr pin_xy,
-                                                     const double *__restrict__ coords, double *__restrict__ part) {
-  const int n = blockIdx.x * BLOCK + threadIdx.x;
+                                                     const double *__restrict__ coords,
+                                                     const uint32_t *__restrict__ large, int n_large,
+                                                     double *__restrict__ part) {
   double v[1] = {0.0};
-  if (n < n_nets) {
-    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-    for (uint32_t p = net_start[n]; p < net_start[n + 1]; ++p) {
-      const uint32_t u = pin_unit[p];
-      const double2 o = pin_xy[p];
-      double px = o.x, py = o.y;
-      if (u != NO_UNIT) {
-        px += coords[2 * (size_t)u];
-        py += coords[2 * (size_t)u + 1];
+  if ((int)blockIdx.x < nb_small) {
+    const int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n < n_nets) {
+      const uint32_t b = net_start[n], e = net_start[n + 1];
+      if (e - b <= HPWL_THREAD_PINS) {
+        double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+        for (uint32_t p = b; p < e; ++p) {
+          const double2 c = hpwl_pin(pin_unit, pin_xy, coords, p);
+          xmin = fmin(xmin, c.x);
+          xmax = fmax(xmax, c.x);
+          ymin = fmin(ymin, c.y);
+          ymax = fmax(ymax, c.y);
+        }
+        v[0] = (xmax - xmin) + (ymax - ymin);
       }
-      xmin = fmin(xmin, px);
-      xmax = fmax(xmax, px);
-      ymin = fmin(ymin, py);
-      ymax = fmax(ymax, py);
     }
-    v[0] = (xmax - xmin) + (ymax - ymin);
+  } else {
+    const int w = ((int)blockIdx.x - nb_small) * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (w < n_large) {  // wavefront-uniform
+      const uint32_t n = large[w], b = net_start[n], e = net_start[n + 1];
+      double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+      for (uint32_t p = b + (threadIdx.x & 63); p < e; p += 64) {
+        const double2 c = hpwl_pin(pin_unit, pin_xy, coords, p);
+        xmin = fmin(xmin, c.x);
+        xmax = fmax(xmax, c.x);
+        ymin = fmin(ymin, c.y);
+        ymax = fmax(ymax, c.y);
+      }
+      xmin = wave_min(xmin);
+      xmax = wave_max(xmax);
+      ymin = wave_min(ymin);
+      ymax = wave_max(ymax);
+      if ((threadIdx.x & 63) == 0) v[0] = (xmax - xmin) + (ymax - ymin);
+    }
   }
   block_sum<1>(v, part + blockIdx.x);
 }
@@ -1381,10 +1413,15 @@ void launch_rudy(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit
   hipLaunchKernelGGL(k_fx_to_double, dim3(std::max(1, blocks_for(2 * G))), dim3(BLOCK), 0, s, 2 * G, fx, out);
 }
 
+int hpwl_blocks(int n_nets, int n_large) {
+  return std::max(1, blocks_for((size_t)n_nets)) + (n_large + BLOCK / 64 - 1) / (BLOCK / 64);
+}
+
 void launch_hpwl(int n_nets, const uint32_t *net_start, const uint32_t *pin_unit, OffArr pin_xy,
-                 const double *coords, double *part, double *out, hipStream_t s) {
-  const int nb = std::max(1, blocks_for((size_t)n_nets));
-  hipLaunchKernelGGL(k_hpwl_nets, dim3(nb), dim3(BLOCK), 0, s, n_nets, net_start, pin_unit, pin_xy, coords, part);
+                 const double *coords, const uint32_t *large, int n_large, double *part, double *out, hipStream_t s) {
+  const int nb_small = std::max(1, blocks_for((size_t)n_nets)), nb = hpwl_blocks(n_nets, n_large);
+  hipLaunchKernelGGL(k_hpwl_nets, dim3(nb), dim3(BLOCK), 0, s, n_nets, nb_small, net_start, pin_unit, pin_xy, coords,
+                     large, n_large, part);
   hipLaunchKernelGGL(k_hpwl_total, dim3(1), dim3(BLOCK), 0, s, nb, part, out);
 }
 
