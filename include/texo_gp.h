@@ -100,6 +100,12 @@ enum texo_gp_status {
   TEXO_GP_ERR_AUCTION_OBJECT_OUT_OF_RANGE = 401,
   TEXO_GP_ERR_AUCTION_BASE_COST_OUT_OF_RANGE = 402,
   TEXO_GP_ERR_AUCTION_DUPLICATE_BEST_OBJECT = 403,
+  /* the transactional repair (crates/texo-pnr/src/legalization/repair.rs):
+   * PnrError::InvalidPlacement of its projection_error (mod.rs:1184-1188), a
+   * relaxed projection it cannot start from, and PnrError::NoBel when no
+   * complete legal assignment exists */
+  TEXO_GP_ERR_REPAIR_PROJECTION = 410,
+  TEXO_GP_ERR_REPAIR_NO_BEL = 411,
   /* Boundary-level failures (no reference counterpart). */
   TEXO_GP_ERR_INVALID_ARGUMENT = 900,
   TEXO_GP_ERR_DEVICE = 901,
@@ -594,6 +600,58 @@ int texo_gp_solve_axis(int32_t device, uint32_t n, const double *diagonal, const
 int texo_gp_auction(int32_t device, uint64_t n_bidders, uint64_t n_objects, uint64_t max_base_cost,
                     const uint32_t *domain_start, const uint32_t *domain_object, const uint64_t *domain_cost,
                     int64_t *assignment, uint64_t *stats);
+
+/* ---- legalization repair ------------------------------------------------- */
+/* A relaxed unit-to-object projection and the legality of every candidate,
+ * as explicit footprints.  Candidate j of unit u is number
+ * c = cand_start[u] + j; it occupies the objects
+ * cand_object[cand_object_start[c] .. cand_object_start[c + 1]) (one per
+ * cell of the unit; the first one's point orders the unit's candidates), and
+ * it requires the shared-resource values (resource, value) =
+ * (cand_shared[2k], cand_shared[2k + 1]) and puts the nets (wire, net) =
+ * (cand_wire[2k], cand_wire[2k + 1]) on pin wires, k over
+ * [cand_shared_start[c], cand_shared_start[c + 1]) and
+ * [cand_wire_start[c], cand_wire_start[c + 1]).  A legal placement has every
+ * object at most once, at most one value per shared resource and at most
+ * wire_capacity[w] distinct nets per wire (the reference's
+ * PlacementResourceUsage, legalization/mod.rs:1080-1128). */
+typedef struct texo_gp_repair_problem {
+  uint32_t width, height;          /* device grid; object points lie in it */
+  uint64_t n_objects;
+  const uint32_t *object_xy;       /* [2 * n_objects] */
+  uint64_t n_wires;
+  const uint32_t *wire_capacity;   /* [n_wires] */
+  uint64_t n_units;
+  const uint32_t *unit_cells;      /* [n_units] cells of the unit, >= 1 */
+  const uint64_t *unit_first_cell; /* [n_units] id of the unit's first cell */
+  const uint32_t *unit_target_xy;  /* [2 * n_units] the GP target */
+  const uint8_t *unit_fixed;       /* [n_units] */
+  const int64_t *preferred;        /* [n_units] candidate j of the unit, or -1 */
+  const uint64_t *cand_start;      /* [n_units + 1] */
+  const uint64_t *cand_object_start; /* [n_candidates + 1] */
+  const uint32_t *cand_object;
+  const uint64_t *cand_shared_start; /* [n_candidates + 1] */
+  const uint64_t *cand_shared;
+  const uint64_t *cand_wire_start;   /* [n_candidates + 1] */
+  const uint64_t *cand_wire;
+} texo_gp_repair_problem;
+
+/* repair_relaxed (crates/texo-pnr/src/legalization/repair.rs:60-155), the
+ * step of legalization::project after the auctions (mod.rs:377-386): the
+ * conflicting part of the relaxed projection is set aside
+ * (initial_pending_units, :322-425), the rest installed, and the set-aside
+ * units placed by the reference's searches in its order: greedily on free
+ * candidates, by single-blocker alternating paths, then by the complete
+ * transactional search; each unit's candidates are tried preferred first,
+ * then by Manhattan rings around its target (ChoiceCursor, :649-793).  On the
+ * host (the search is serial).  placed[n_units] receives the candidate j of
+ * every unit; stats[12] = initial pending, direct choices, direct moves,
+ * direct fallbacks, augmenting roots, augmenting fallbacks, states visited,
+ * failed states, max unplaced, max blocker branches, units evicted, choices
+ * examined.  Returns 0, TEXO_GP_ERR_REPAIR_PROJECTION, or
+ * TEXO_GP_ERR_REPAIR_NO_BEL with *error_unit = the first unit that was
+ * pending; TEXO_GP_ERR_INVALID_ARGUMENT for malformed arrays. */
+int texo_gp_repair(const texo_gp_repair_problem *problem, int64_t *placed, uint64_t *stats, int64_t *error_unit);
 
 /* Last error message of the calling thread (static storage). */
 const char *texo_gp_last_error(void);

@@ -309,6 +309,23 @@ int oracle_auction(uint64_t n_bidders, uint64_t n_objects, uint64_t max_base_cos
                    const uint32_t *domain_object, const uint64_t *domain_cost, int64_t *assignment,
                    uint64_t *stats);
 
+/* ---- legalization repair (gp_repair.cpp) -------------------------------- */
+/* repair_relaxed (legalization/repair.rs:60-155) over explicit candidate
+ * footprints (texo_gp_repair_problem); placed[n_units] = candidate j per
+ * unit; stats[12] as texo_gp_repair.  Returns 0, TEXO_GP_ERR_REPAIR_PROJECTION
+ * or TEXO_GP_ERR_REPAIR_NO_BEL (*error_unit = first pending unit). */
+int oracle_repair(const texo_gp_repair_problem *problem, int64_t *placed, uint64_t *stats, int64_t *error_unit);
+/* pin_blocker_branches (repair.rs:1169-1241) of (unit, choice) on a state
+ * holding the listed installed units: branches as unit bitmasks (units < 64)
+ * in out[max_out]; returns the number of branches (-1: malformed input). */
+int oracle_repair_pin_branches(const texo_gp_repair_problem *problem, uint64_t n_installed,
+                               const uint64_t *installed_unit, const uint64_t *installed_choice, uint64_t unit,
+                               uint64_t choice, uint64_t mandatory_mask, uint64_t *out, int max_out);
+
+/* ChoiceCursor order (repair.rs:649-793) of `unit`'s candidates: up to
+ * max_out choices into out; returns their number (-1: malformed input). */
+int oracle_repair_cursor(const texo_gp_repair_problem *problem, uint64_t unit, uint64_t *out, int max_out);
+
 #ifdef __cplusplus
 }
 #endif

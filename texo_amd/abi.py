@@ -50,6 +50,7 @@ STATUS = {
     303: "ArithmeticOverflow",
     400: "AuctionArithmeticOverflow", 401: "ObjectOutOfRange", 402: "BaseCostOutOfRange",
     403: "DuplicateBestObject",
+    410: "InvalidPlacement", 411: "NoBel",
     900: "InvalidArgument", 901: "Device", 902: "OutOfMemory", 903: "Unsupported",
     904: "Cancelled", 905: "NotReady", 906: "Injected",
 }

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtexo_gp.so")
-SOURCES = ["gp_kernels.hip", "gp_dct.hip", "gp_job.hip", "gp_engine.hip", "gp_api.hip", "gp_builders.hip", "gp_cluster.hip", "gp_seed.hip", "gp_auction.hip"]
+SOURCES = ["gp_kernels.hip", "gp_dct.hip", "gp_job.hip", "gp_engine.hip", "gp_api.hip", "gp_builders.hip", "gp_cluster.hip", "gp_seed.hip", "gp_auction.hip", "gp_repair.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
