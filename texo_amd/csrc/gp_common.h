@@ -316,6 +316,9 @@ struct Task {
   const double *metric;  // nullptr: no KKT residual
   double gamma;
   int skip_density;      // the slot already holds this point's density field
+  // value only: WA value and field energies, no gradient (no derivative
+  // stores, no gradient blocks; the gradient and KKT scalars stay zero)
+  int value_only;
   double lambda[MAXF], scale[MAXF];
   EvalScalars *scalars;
   // parity outputs (nullptr in the hot loop)
@@ -372,7 +375,11 @@ struct OptTask {
 struct Maps {
   BlockPlan deposit, convert, wa, gradient;
   int n_tasks;
-  int n_density;  // tasks whose density is computed (not skipped)
+  int n_density;   // tasks whose density is computed (not skipped)
+  int n_gradient;  // tasks with a gradient (not value only)
+  // WA work in full evaluations: a value-only task counts its share of a
+  // full evaluation's algorithmic bytes (no derivative stores)
+  double wa_evaluations;
 };
 
 void launch_density(const Task *tasks, const Maps &m, hipStream_t s);
@@ -427,7 +434,7 @@ bool poisson_wide_ok(int precision, int W, int H);
 // Kernel timing hook: the engine brackets launches of one named kernel.
 struct KernelTimer;
 void timer_begin(KernelTimer *t, const char *name, hipStream_t s);
-void timer_end(KernelTimer *t, const char *name, hipStream_t s, int tasks);  // tasks: evaluations the launch covers
+void timer_end(KernelTimer *t, const char *name, hipStream_t s, double tasks);  // tasks: evaluations the launch covers
 extern thread_local KernelTimer *g_timer;
 
 }  // namespace texo

@@ -282,6 +282,8 @@ Engine::Engine(int device, size_t pool_capacity) : device_(device) {
   // long as the launches it replaces).
   const char *graphs = std::getenv("TEXO_GP_GRAPHS");
   use_graphs_ = graphs && graphs[0] == '1';
+  const char *origin = std::getenv("TEXO_GP_ORIGIN_VALUE_ONLY");
+  if (origin && (origin[0] == '0' || origin[0] == '2')) origin_mode_ = origin[0] - '0';
   // every stream keeps the default priority
   for (RoundCtx &c : ctx_) {
     HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -500,13 +502,13 @@ void Engine::plan(Job *j) {
       case Phase::Probe: {
         // density probe at gamma = base (eplace.rs:455)
         j->want_sort = true;
-        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->opt.gamma_base, false, false, 0});
+        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->opt.gamma_base, false, false, 0, false});
         return;
       }
       case Phase::Init: {
         // initial evaluation at the adaptive gamma (eplace.rs:457-458): same
         // coordinates as the probe, whose density field slot 0 still holds
-        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->g0, true, false, 0});
+        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->g0, true, false, 0, false});
         return;
       }
       case Phase::IterStart: {
@@ -559,12 +561,26 @@ void Engine::plan(Job *j) {
         j->beta = (m - 1.0) / j->next_momentum;
         j->want_precond = true;
         const int p = j->p_slot, q = (p + 1) % WORKING_SLOTS;
-        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, j->p_valid, false, scalar_of_slot(p)});
+        // with momentum, y is evaluated beside the origin, whose gradient is
+        // then read only when y turns out to be the origin or on a restart
+        // (global_placement.rs:176-181, 291-301): the origin's value alone,
+        // its field being the slot's
+        j->origin_value_only = origin_mode_ != 0 && j->p_valid && j->beta != 0.0 && j->wa_bound_ok;
+        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, j->p_valid, false, scalar_of_slot(p),
+                            j->origin_value_only});
         j->y_evaluated = j->beta != 0.0;
         if (j->y_evaluated) {
           j->want_extrapolate = true;
-          j->evals.push_back({q, j->d_vec[j->iy], j->d_grad[1], true, j->gamma, false, false, scalar_of_slot(q)});
+          j->evals.push_back({q, j->d_vec[j->iy], j->d_grad[1], true, j->gamma, false, false, scalar_of_slot(q), false});
         }
+        return;
+      }
+      case Phase::OriginFull: {
+        // the origin again, with its gradient: same point, field, gamma and
+        // multipliers, so the same value bit for bit
+        if (!j->p_valid) fail(TEXO_GP_ERR_DEVICE);
+        const int p = j->p_slot;
+        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, true, false, scalar_of_slot(p), false});
         return;
       }
       case Phase::Trial: {
@@ -587,7 +603,7 @@ void Engine::plan(Job *j) {
         // again, at this candidate
         j->want_sort = (size_t)j->moved * MOVED_RESORT > j->M + j->Fl;
         const int q = (j->p_slot + 1) % WORKING_SLOTS, r = (j->p_slot + 2) % WORKING_SLOTS;
-        j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, scalar_of_slot(q)});
+        j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, scalar_of_slot(q), false});
         // the candidate of 2L beside it: what the next trial evaluates when
         // this one only grows L (global_placement.rs:302-309), the common
         // outcome of an iteration's first trial; same coordinates, same
@@ -595,7 +611,7 @@ void Engine::plan(Job *j) {
         j->want_spec = speculate_ && std::isfinite(2.0 * j->L) && std::isfinite(2.0 * j->L * mmax);
         if (j->want_spec) {
           j->spec_L = 2.0 * j->L;
-          j->evals.push_back({r, j->d_vec[j->ic2], j->d_grad[3], true, j->gamma, false, false, scalar_of_slot(r)});
+          j->evals.push_back({r, j->d_vec[j->ic2], j->d_grad[3], true, j->gamma, false, false, scalar_of_slot(r), false});
         }
         return;
       }
@@ -800,6 +816,12 @@ void stationary(Job *j, bool set_l, double l, hipStream_t s) {
 // y-stationarity decision after y is known (global_placement.rs:182-211).
 void decide_y(Job *j, hipStream_t s) {
   for (;;) {
+    if (j->y_is_origin && !j->origin_full) {
+      // y is the origin, whose gradient is not evaluated yet
+      j->held_y = false;
+      j->phase = Phase::OriginFull;
+      return;
+    }
     const double y_stat = kkt_of(j->y_s);
     if (!std::isfinite(y_stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
     if (y_stat == 0.0) {
@@ -886,6 +908,22 @@ bool trial_outcome(Job *j, const StepScalars &st, const EvalScalars &ce, int slo
   return true;
 }
 
+// y's evaluation of an iteration (global_placement.rs:162-182), after the
+// origin's: `has_y` when y was evaluated and differs from the origin.
+void take_y(Job *j, bool has_y, const EvalScalars &ys, hipStream_t s) {
+  j->y_is_origin = !has_y;
+  if (has_y) {
+    j->count_evaluation();
+    j->y_s = ys;
+    j->y_value = j->combined_value(j->y_s);
+    check_evaluation(j->y_s, j->y_value);
+  } else {
+    j->y_s = j->origin_s;
+    j->y_value = j->origin_value;
+  }
+  decide_y(j, s);
+}
+
 }  // namespace
 
 void Engine::advance(Job *j) {
@@ -970,20 +1008,44 @@ void Engine::advance(Job *j) {
       const int p = j->p_slot, q = (p + 1) % WORKING_SLOTS;
       j->origin_s = hs.eval[scalar_of_slot(p)];
       j->origin_value = j->combined_value(j->origin_s);
-      check_evaluation(j->origin_s, j->origin_value);
       j->p_valid = true;
-      j->y_is_origin = true;
-      if (j->y_evaluated && hs.step.differs) {
-        j->y_is_origin = false;
-        j->count_evaluation();
-        j->y_s = hs.eval[scalar_of_slot(q)];
-        j->y_value = j->combined_value(j->y_s);
-        check_evaluation(j->y_s, j->y_value);
+      const bool has_y = j->y_evaluated && hs.step.differs;
+      if (j->origin_value_only) {
+        // checked_evaluation of the origin (global_placement.rs:143-149):
+        // the value here; the gradient is finite when the bound says so,
+        // else an OriginFull round checks it before y is taken
+        if (!std::isfinite(j->origin_value)) fail(TEXO_GP_ERR_INVALID_OBJECTIVE);
+        j->origin_full = false;
+        if (origin_mode_ == 2 || !j->gradient_certainly_finite(j->origin_s, j->gamma)) {
+          j->held_y = true;
+          j->held_y_has = has_y;
+          if (has_y) j->held_y_s = hs.eval[scalar_of_slot(q)];
+          j->phase = Phase::OriginFull;
+          return;
+        }
+      } else {
+        check_evaluation(j->origin_s, j->origin_value);
+        j->origin_full = true;
       }
-      if (j->y_is_origin) {
-        j->y_s = j->origin_s;
-        j->y_value = j->origin_value;
+      take_y(j, has_y, hs.eval[scalar_of_slot(q)], cur());
+      return;
+    }
+    case Phase::OriginFull: {
+      const EvalScalars full = hs.eval[scalar_of_slot(j->p_slot)];
+      const double value = j->combined_value(full);
+      // the value-only evaluation's value, bit for bit
+      if (std::memcmp(&value, &j->origin_value, sizeof value) != 0) fail(TEXO_GP_ERR_DEVICE);
+      check_evaluation(full, value);
+      j->origin_s = full;
+      j->origin_full = true;
+      if (j->held_y) {
+        j->held_y = false;
+        take_y(j, j->held_y_has, j->held_y_s, cur());
+        return;
       }
+      // a decision on y at the origin (restart or y == origin)
+      j->y_s = j->origin_s;
+      j->y_value = j->origin_value;
       decide_y(j, cur());
       return;
     }
@@ -1051,7 +1113,8 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   std::vector<int> l_sort, l_pre, l_cand;
   Plan p_sort_c, p_sort_k, p_pre, p_ext, p_cand;
   std::vector<Task> tasks;
-  int n_density = 0;
+  int n_density = 0, n_gradient = 0;
+  double wa_evaluations = 0.0;
   Plan p_dep, p_conv, p_wa, p_gr;
   std::map<std::tuple<int, bool, bool>, DctGroup> groups;
   for (Job *j : active) {
@@ -1149,7 +1212,7 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
     }
     for (const Job::EvalReq &r : j->evals) {
       Task t = j->task(r.slot, r.coords, r.grad, r.metric ? j->d_metric : nullptr, r.gamma, r.skip,
-                       &j->d_scalars->eval[r.scalar_slot], fixed_nets);
+                       &j->d_scalars->eval[r.scalar_slot], fixed_nets, r.value_only);
       if (r.parity) {
         t.raw_units = j->d_raw_units;
         t.raw_fillers = j->d_raw_fillers;
@@ -1169,7 +1232,13 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
         p_conv.none();
       }
       p_wa.add((size_t)t.nb_wa, 1);
-      p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
+      wa_evaluations += r.value_only ? j->wa_value_only_share : 1.0;
+      if (r.value_only) {
+        p_gr.none();
+      } else {
+        n_gradient += 1;
+        p_gr.add((size_t)(t.nb_ug + t.nb_gf), 1);
+      }
     }
   }
   std::vector<char> buf;
@@ -1213,6 +1282,8 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   mp.gradient = p_gr.device(d_stage_, o_pgr);
   mp.n_tasks = (int)tasks.size();
   mp.n_density = n_density;
+  mp.n_gradient = n_gradient;
+  mp.wa_evaluations = wa_evaluations;
   SortMaps sm{};
   if (!sorts.empty()) {
     sm.charges = p_sort_c.device(d_stage_, o_psc);
@@ -1371,7 +1442,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   if (j->p_slot == 0) j->p_valid = false;
   const Phase saved = j->phase;
   j->evals.clear();
-  j->evals.push_back({0, j->d_eval_coords, j->d_grad[0], false, g, false, true, 2});
+  j->evals.push_back({0, j->d_eval_coords, j->d_grad[0], false, g, false, true, 2, false});
   j->want_precond = j->want_extrapolate = j->want_candidate = false;
   std::vector<Job *> one{j};
   // run exactly the planned request: mark the phase so plan() leaves it alone
@@ -1421,6 +1492,8 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   mp.gradient = p_gr.device(d_stage_, o_gr);
   mp.n_tasks = 1;
   mp.n_density = j->F > 0 ? 1 : 0;
+  mp.n_gradient = 1;
+  mp.wa_evaluations = 1.0;
   g_timer = &timer;
   launch_wa(d_tasks, mp, st);
   launch_density(d_tasks, mp, st);

@@ -180,7 +180,9 @@ void launch_round_targets_to(const double *coords, uint32_t *out, size_t units, 
 Failure current_failure();
 void launch_collect_scalars(const int *index, int n, const JobScalars *pool, JobScalars *out, hipStream_t s);
 
-enum class Phase { Probe, Init, IterStart, Trial, RoundEnd, Paused, Done, Failed };
+// OriginFull: the gradient of an origin whose value was evaluated alone,
+// before a decision that reads it (y at the origin, a restart).
+enum class Phase { Probe, Init, IterStart, OriginFull, Trial, RoundEnd, Paused, Done, Failed };
 
 // From this many charges (members + fillers) in flight the jobs are split
 // into two groups whose rounds overlap on the GPU (each context has its own
@@ -314,8 +316,9 @@ struct Job {
   bool sorted = false;  // charges ordered by tile at least once
   // fixed_nets: the task evaluates the nets of fixed pins too (their value
   // at `gamma` is then held for later tasks at the same gamma)
+  // value_only: the WA value and field energies only (skip must be set)
   Task task(int slot, const double *coords, double *grad, const double *metric, double gamma, bool skip,
-            EvalScalars *scalars, bool fixed_nets) const;
+            EvalScalars *scalars, bool fixed_nets, bool value_only = false) const;
   // next sort, at `coords` (null: the current iterate); the new order
   // becomes the job's order
   SortTask sort_task(const double *coords = nullptr);
@@ -359,6 +362,22 @@ struct Job {
   double wl = 0.0, L = 0.0, next_momentum = 1.0, beta = 0.0;
   EvalScalars origin_s{}, y_s{};
   double origin_value = 0.0, y_value = 0.0;
+  // The origin evaluation of an iteration with momentum evaluates only the
+  // value (the line search reads the origin's gradient only when y is the
+  // origin or on a restart); origin_full says whether origin_s holds the
+  // gradient scalars too.  held_y: y's evaluation, taken after an
+  // OriginFull round that checks the origin's gradient first.
+  bool origin_value_only = false, origin_full = true, held_y = false, held_y_has = false;
+  EvalScalars held_y_s{};
+  // A bound on every WA gradient entry: each pin's derivative is at most
+  // 3 (1 + 3 extent / gamma) times its net weight in magnitude, so a unit's
+  // entry is at most 3 wa_bound_w (1 + 3 wa_bound_ext / gamma), with
+  // wa_bound_w the sum over nets of weight x pins and wa_bound_ext the x
+  // plus y extent of every pin position the bounds allow.
+  double wa_bound_w = 0.0, wa_bound_ext = 0.0;
+  bool wa_bound_ok = false;
+  // WA bytes of a value-only evaluation per full evaluation's
+  double wa_value_only_share = 1.0;
   // requests planned for the next round
   struct EvalReq {
     int slot;
@@ -369,6 +388,7 @@ struct Job {
     bool skip;
     bool parity;
     int scalar_slot;
+    bool value_only;
   };
   std::vector<EvalReq> evals;
   bool want_precond = false, want_extrapolate = false, want_candidate = false, want_sort = false;
@@ -388,6 +408,9 @@ struct Job {
   std::pair<double, double> excess(const EvalScalars &s) const;
   size_t area_round_limit() const;
   void update_multipliers();
+  // true when the gradient of an evaluation at the current point with the
+  // value-only scalars `s` (energies at gamma) has only finite entries
+  bool gradient_certainly_finite(const EvalScalars &s, double gamma) const;
   double hpwl(const std::vector<double> &x) const;
   // HPWL of the device coordinates `coords` into `out` (device or pinned
   // host memory), enqueued on `s` (a job with an arena)
@@ -501,6 +524,10 @@ class Engine {
   static constexpr size_t GRAPH_CACHE = 64;
   bool use_graphs_ = false;  // TEXO_GP_GRAPHS=1 replays recurring rounds from captured graphs
   bool speculate_ = false;   // this round evaluates speculative trials (few jobs)
+  // origin evaluations of iterations with momentum: 1 value only (default),
+  // 0 full, 2 value only with every gradient check taken by an OriginFull
+  // round (TEXO_GP_ORIGIN_VALUE_ONLY; results are the same in all three)
+  int origin_mode_ = 1;
   uint64_t graph_rounds_ = 0;
   void drop_graphs(RoundCtx &c);
   RoundCtx ctx_[ROUND_CONTEXTS], aux_;  // aux_: set-up sorts and parity evaluations (on context 0's stream)

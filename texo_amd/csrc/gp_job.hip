@@ -81,7 +81,7 @@ void timer_begin(KernelTimer *t, const char *name, hipStream_t s) {
   (void)hipEventRecord(t->events[t->used].first, s);
 }
 
-void timer_end(KernelTimer *t, const char *name, hipStream_t s, int tasks) {
+void timer_end(KernelTimer *t, const char *name, hipStream_t s, double tasks) {
   if (!t || !t->enabled || t->name != name || t->used >= t->events.size()) return;
   (void)hipEventRecord(t->events[t->used].second, s);
   t->used += 1;
@@ -656,6 +656,35 @@ void Job::load(const texo_gp_problem *p, const texo_gp_options *o, const std::sh
     upper[2 * (U + j)] = (double)(W - 1);
     upper[2 * (U + j) + 1] = (double)(H - 1);
   }
+  // the WA gradient bound of value-only origin evaluations: every pin
+  // position the bounds allow (a unit's coordinates stay inside them)
+  {
+    double w = 0.0, lx = INFINITY, hx = -INFINITY, ly = INFINITY, hy = -INFINITY;
+    size_t movable = 0;
+    for (size_t n = 0; n < n_net_total; ++n) w += std::fabs(net_weight[n]) * (double)(net_start[n + 1] - net_start[n]);
+    for (size_t q = 0; q < n_pins_total; ++q) {
+      const uint32_t u = pin_unit[q];
+      const double ox = pin_xy[2 * q], oy = pin_xy[2 * q + 1];
+      double x0 = ox, x1 = ox, y0 = oy, y1 = oy;
+      if (u != NO_UNIT) {
+        movable += 1;
+        x0 = lower[2 * (size_t)u] + ox;
+        x1 = upper[2 * (size_t)u] + ox;
+        y0 = lower[2 * (size_t)u + 1] + oy;
+        y1 = upper[2 * (size_t)u + 1] + oy;
+      }
+      lx = std::min(lx, x0);
+      hx = std::max(hx, x1);
+      ly = std::min(ly, y0);
+      hy = std::max(hy, y1);
+    }
+    wa_bound_w = w;
+    wa_bound_ext = n_pins_total ? (hx - lx) + (hy - ly) : 0.0;
+    wa_bound_ok = std::isfinite(wa_bound_w) && std::isfinite(wa_bound_ext);
+    // WA bytes per evaluation: 16 per unit coordinate pair gathered and 16
+    // per movable pin derivative stored; a value-only evaluation stores none
+    wa_value_only_share = U + movable > 0 ? (double)U / (double)(U + movable) : 1.0;
+  }
   set_charges(std::vector<double>(m_charge), std::vector<double>(fl_charge));
   stage(pool);
 }
@@ -1031,7 +1060,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
 }
 
 Task Job::task(int sl, const double *coords, double *grad, const double *metric, double g, bool skip,
-               EvalScalars *scalars, bool fixed_nets) const {
+               EvalScalars *scalars, bool fixed_nets, bool value_only) const {
   Task t{};
   t.j = js;
   const Slot &w = slot[sl];
@@ -1059,8 +1088,9 @@ Task Job::task(int sl, const double *coords, double *grad, const double *metric,
   t.fixed_wa = d_fixed_wa;
   t.fixed_wa_mode = nb_wa == nb_wa_live ? 0 : fixed_nets ? 1 : 2;
   t.nb_energy = F > 0 ? nb_energy : 0;
-  t.nb_gf = F > 0 && Fl > 0 ? nb_gf : 0;
-  t.nb_ug = nb_ug;
+  t.nb_gf = F > 0 && Fl > 0 && !value_only ? nb_gf : 0;
+  t.nb_ug = value_only ? 0 : nb_ug;
+  t.value_only = value_only ? 1 : 0;
   t.coords = coords;
   t.grad = grad;
   t.metric = metric;
@@ -1094,6 +1124,18 @@ SortTask Job::sort_task(const double *coords) {
 
 double Job::overflow(const EvalScalars &s, int f) const {
   return static_cast<double>(s.over_fx[f]) * FX_INV / real_area[f];
+}
+
+bool Job::gradient_certainly_finite(const EvalScalars &s, double g) const {
+  if (!wa_bound_ok || !(g > 0.0)) return false;
+  double b = 3.0 * wa_bound_w * (1.0 + 3.0 * wa_bound_ext / g);
+  // a field term of an entry is its coefficient times a raw entry, at most
+  // the raw l1 norm of the current point (the same point, field and charges)
+  for (int f = 0; f < F; ++f) {
+    const double coef = mult[f] * (1.0 + AUGMENTED_DENSITY_BETA * (s.energy[f] / scales[f]));
+    b += 2.0 * std::fabs(coef) * current.raw_l1[f];
+  }
+  return std::isfinite(b) && b <= 1e300;
 }
 
 double Job::combined_value(const EvalScalars &s) const {

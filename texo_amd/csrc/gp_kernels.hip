@@ -626,7 +626,7 @@ __device__ double wa_group_net(const JobStatic &a, const Task &t, const WaSeg &s
     slot[k] = NO_UNIT;
     if (in[k]) {
       c[k] = pin_coordinate(a, t.coords, p);
-      slot[k] = a.pin_slot[p];
+      if (!t.value_only) slot[k] = a.pin_slot[p];
       xmin = fmin(xmin, c[k].x);
       xmax = fmax(xmax, c[k].x);
       ymin = fmin(ymin, c[k].y);
@@ -688,7 +688,7 @@ __device__ double wa_group_net(const JobStatic &a, const Task &t, const WaSeg &s
   const double pmx = s.pdx * rs.x, nmx = s.ndx * rs.y, pmy = s.pdy * rs.z, nmy = s.ndy * rs.w;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (!in[k]) continue;
+    if (!in[k] || t.value_only) continue;
     const double dxm = c[k].x - xmax, dxn = c[k].x - xmin, dym = c[k].y - ymax, dyn = c[k].y - ymin;
     const double ddx = (ex[k] * rs.x) * (1.0 + (dxm - pmx) * ig) - (enx[k] * rs.y) * (1.0 - (dxn - nmx) * ig);
     const double ddy = (ey[k] * rs.z) * (1.0 + (dym - pmy) * ig) - (eny[k] * rs.w) * (1.0 - (dyn - nmy) * ig);
@@ -729,7 +729,7 @@ __device__ double wa_loop_net(const JobStatic &a, const Task &t, uint32_t net) {
   s.ndy = wave_sum(s.ndy);
   const double4 r = make_double4(1.0 / s.psx, 1.0 / s.nsx, 1.0 / s.psy, 1.0 / s.nsy);
   const double pmx = s.pdx * r.x, nmx = s.ndx * r.y, pmy = s.pdy * r.z, nmy = s.ndy * r.w;
-  for (uint32_t p = b + lane; p < e; p += 64) {
+  for (uint32_t p = b + lane; p < e && !t.value_only; p += 64) {
     const double2 d = wa_derivative(pin_coordinate(a, t.coords, p), xmin, xmax, ymin, ymax, ig, r, pmx, nmx, pmy, nmy);
     const uint32_t slot = a.pin_slot[p];
     if (slot != NO_UNIT) t.dpin[slot] = make_double2(w * d.x, w * d.y);
@@ -1377,14 +1377,14 @@ void launch_wa(const Task *tasks, const Maps &m, hipStream_t s) {
   if (m.wa.total <= 0) return;
   timer_begin(g_timer, "wa", s);
   hipLaunchKernelGGL(k_wa, dim3(plan_grid(m.wa)), dim3(BLOCK), 0, s, tasks, m.wa);
-  timer_end(g_timer, "wa", s, m.n_tasks);
+  timer_end(g_timer, "wa", s, m.wa_evaluations);
 }
 
 void launch_combine(const Task *tasks, const Maps &m, hipStream_t s) {
   if (m.gradient.total > 0) {
     timer_begin(g_timer, "gather", s);
     hipLaunchKernelGGL(k_gradient, dim3(plan_grid(m.gradient)), dim3(BLOCK), 0, s, tasks, m.gradient);
-    timer_end(g_timer, "gather", s, m.n_tasks);
+    timer_end(g_timer, "gather", s, m.n_gradient);
   }
   if (m.n_tasks > 0) hipLaunchKernelGGL(k_finalize, dim3(m.n_tasks), dim3(BLOCK), 0, s, tasks);
 }
