@@ -173,7 +173,9 @@ struct EvalScalars {
   unsigned int nonfinite;       // any non-finite gradient entry
   unsigned int moved;           // charges deposited outside their tile's window
   unsigned int moved_acc;       // (counted by k_deposit, moved over by k_finalize)
-  unsigned int pad;
+  // a line-search trial whose gradient blocks did not run: rejected by the
+  // acceptance test with its gradient bounded (the gradient sums are zero)
+  unsigned int gradient_skipped;
 };
 
 // Reductions of one line-search candidate and one extrapolation.
@@ -319,6 +321,21 @@ struct Task {
   // value only: WA value and field energies, no gradient (no derivative
   // stores, no gradient blocks; the gradient and KKT scalars stay zero)
   int value_only;
+  // A line-search trial (global_placement.rs:241-311): k_coefficients runs
+  // its acceptance test (value against the majorizer and the origin) and
+  // skips the gradient blocks of a candidate that fails it, when every
+  // gradient entry is bounded by trial_wa_bound + sum over fields of
+  // |coefficient| x 4 trial_charge[f] x max |phi_f| (finite, so the
+  // reference's gradient check passes); the host takes the same decision
+  // from the same scalars.
+  int trial;
+  double trial_y_value, trial_origin_value, trial_L, trial_wa_bound;
+  double trial_charge[MAXF];  // sum of |charge| per field
+  const StepScalars *trial_step;
+  // max |phi| bits (NaN above infinity) per 32x32 tile of the last
+  // transform stage, [F][phi_tiles]; null when the transform does not write them
+  const unsigned int *phi_amax;
+  int phi_tiles;
   double lambda[MAXF], scale[MAXF];
   EvalScalars *scalars;
   // parity outputs (nullptr in the hot loop)

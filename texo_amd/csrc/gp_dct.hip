@@ -61,6 +61,8 @@ __device__ __forceinline__ half4_t half4_of(uint2 v) {
 }
 
 __device__ __forceinline__ float wave_max_f(float v) { return lane_reduce<64>(v, OpMaxF{}); }
+// |x| as bits: ordered like the magnitudes, a NaN above infinity
+__device__ __forceinline__ unsigned int abs_bits(float x) { return __float_as_uint(x) & 0x7fffffffu; }
 
 // Stage geometry resolved per block from the task.
 struct Stage {
@@ -81,7 +83,9 @@ __device__ __forceinline__ Stage stage_of(const Task &t) {
   s.W = W;
   s.inv_lambda = nullptr;
   s.energy = nullptr;
-  s.amax_out = S < 3 ? t.amax_part + (size_t)t.nb_conv * MAXF + (size_t)S * t.nb_tile : nullptr;
+  // stages 0-2: the next stage's input range; stage 3: |phi| for the bound
+  // on a trial's gradient
+  s.amax_out = t.amax_part + (size_t)t.nb_conv * MAXF + (size_t)S * t.nb_tile;
   if constexpr (S == 0) {
     s.A = t.rho;
     s.out = t.t0;
@@ -199,9 +203,9 @@ __device__ __forceinline__ void tile_epilogue(const Stage &e, float4_t v, int m0
     // the 32 rows of a fast-shape tile lie in one field; v now holds the
     // stored values (after the spectral scaling in stage 1)
     __shared__ unsigned int sm[SPLIT];
-    float mx = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
-    mx = wave_max_f(mx);
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = __float_as_uint(mx);
+    unsigned int mx = max(max(abs_bits(v[0]), abs_bits(v[1])), max(abs_bits(v[2]), abs_bits(v[3])));
+    mx = lane_reduce<64>(mx, OpMaxI{});
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) e.amax_out[(size_t)local_rb * gx + bx] = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
   }
@@ -543,7 +547,7 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_wide(const Task *__restri
   // column lane % 32 and rows 4 (2 h + lane / 32) .. + 3
   const int gx = e.N / 32, bx = n0 / 32, col = lane & 31;
   double se[SPLIT][MAXF];
-  float mx = 0.f;
+  unsigned int mx = 0u;
 #pragma unroll
   for (int h = 0; h < SPLIT; ++h) {
     const int row0 = 4 * (2 * h + (lane >> 5));
@@ -552,13 +556,13 @@ __global__ void __launch_bounds__(FAST_THREADS) k_gemm_wide(const Task *__restri
     for (int u = 0; u < 4; ++u) w[u] = lds.red[wave][row0 + u][col];
     double energy[MAXF] = {0.0, 0.0, 0.0, 0.0};
     store4(e, m0 + row0, n0 + col, w, energy);
-    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(w[0]), fabsf(w[1])), fmaxf(fabsf(w[2]), fabsf(w[3]))));
+    mx = max(mx, max(max(abs_bits(w[0]), abs_bits(w[1])), max(abs_bits(w[2]), abs_bits(w[3]))));
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) se[h][k] = e.energy ? wave_sum(energy[k]) : 0.0;
   }
   if (e.amax_out) {
-    mx = wave_max_f(mx);
-    if (lane == 0) e.amax_out[(size_t)rb * gx + bx] = __float_as_uint(mx);
+    mx = lane_reduce<64>(mx, OpMaxI{});
+    if (lane == 0) e.amax_out[(size_t)rb * gx + bx] = mx;
   }
   if (e.energy && lane < MAXF) {
     double tot = 0.0;

@@ -284,6 +284,8 @@ Engine::Engine(int device, size_t pool_capacity) : device_(device) {
   use_graphs_ = graphs && graphs[0] == '1';
   const char *origin = std::getenv("TEXO_GP_ORIGIN_VALUE_ONLY");
   if (origin && (origin[0] == '0' || origin[0] == '2')) origin_mode_ = origin[0] - '0';
+  const char *skip = std::getenv("TEXO_GP_SKIP_REJECTED");
+  skip_rejected_ = !(skip && skip[0] == '0');
   // every stream keeps the default priority
   for (RoundCtx &c : ctx_) {
     HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -502,13 +504,13 @@ void Engine::plan(Job *j) {
       case Phase::Probe: {
         // density probe at gamma = base (eplace.rs:455)
         j->want_sort = true;
-        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->opt.gamma_base, false, false, 0, false});
+        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->opt.gamma_base, false, false, 0, false, 0});
         return;
       }
       case Phase::Init: {
         // initial evaluation at the adaptive gamma (eplace.rs:457-458): same
         // coordinates as the probe, whose density field slot 0 still holds
-        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->g0, true, false, 0, false});
+        j->evals.push_back({0, j->d_vec[j->ix], j->d_grad[0], false, j->g0, true, false, 0, false, 0});
         return;
       }
       case Phase::IterStart: {
@@ -567,11 +569,11 @@ void Engine::plan(Job *j) {
         // its field being the slot's
         j->origin_value_only = origin_mode_ != 0 && j->p_valid && j->beta != 0.0 && j->wa_bound_ok;
         j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, j->p_valid, false, scalar_of_slot(p),
-                            j->origin_value_only});
+                            j->origin_value_only, 0});
         j->y_evaluated = j->beta != 0.0;
         if (j->y_evaluated) {
           j->want_extrapolate = true;
-          j->evals.push_back({q, j->d_vec[j->iy], j->d_grad[1], true, j->gamma, false, false, scalar_of_slot(q), false});
+          j->evals.push_back({q, j->d_vec[j->iy], j->d_grad[1], true, j->gamma, false, false, scalar_of_slot(q), false, 0});
         }
         return;
       }
@@ -580,7 +582,7 @@ void Engine::plan(Job *j) {
         // multipliers, so the same value bit for bit
         if (!j->p_valid) fail(TEXO_GP_ERR_DEVICE);
         const int p = j->p_slot;
-        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, true, false, scalar_of_slot(p), false});
+        j->evals.push_back({p, j->d_vec[j->ix], j->d_grad[0], true, j->gamma, true, false, scalar_of_slot(p), false, 0});
         return;
       }
       case Phase::Trial: {
@@ -603,7 +605,7 @@ void Engine::plan(Job *j) {
         // again, at this candidate
         j->want_sort = (size_t)j->moved * MOVED_RESORT > j->M + j->Fl;
         const int q = (j->p_slot + 1) % WORKING_SLOTS, r = (j->p_slot + 2) % WORKING_SLOTS;
-        j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, scalar_of_slot(q), false});
+        j->evals.push_back({q, j->d_vec[j->ic], j->d_grad[2], true, j->gamma, false, false, scalar_of_slot(q), false, 1});
         // the candidate of 2L beside it: what the next trial evaluates when
         // this one only grows L (global_placement.rs:302-309), the common
         // outcome of an iteration's first trial; same coordinates, same
@@ -611,7 +613,7 @@ void Engine::plan(Job *j) {
         j->want_spec = speculate_ && std::isfinite(2.0 * j->L) && std::isfinite(2.0 * j->L * mmax);
         if (j->want_spec) {
           j->spec_L = 2.0 * j->L;
-          j->evals.push_back({r, j->d_vec[j->ic2], j->d_grad[3], true, j->gamma, false, false, scalar_of_slot(r), false});
+          j->evals.push_back({r, j->d_vec[j->ic2], j->d_grad[3], true, j->gamma, false, false, scalar_of_slot(r), false, 2});
         }
         return;
       }
@@ -866,12 +868,15 @@ bool trial_outcome(Job *j, const StepScalars &st, const EvalScalars &ce, int slo
   j->count_evaluation();
   const double value = j->combined_value(ce);
   check_evaluation(ce, value);
+  const bool skipped = ce.gradient_skipped != 0u;
   // majorizer and monotonicity (global_placement.rs:274-290)
   const double upper = j->y_value + st.gdelta + 0.5 * j->L * st.msq;
   if (!std::isfinite(upper)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
   const bool smooth = value <= upper + rounding_margin(value, upper);
   const bool monotone = value <= j->origin_value + rounding_margin(value, j->origin_value);
   if (smooth && monotone) {
+    // the device's test accepts whatever the host accepts: its gradient ran
+    if (skipped) fail(TEXO_GP_ERR_DEVICE);
     const double stat = kkt_of(ce);
     if (!std::isfinite(stat)) fail(TEXO_GP_ERR_LINE_SEARCH_OVERFLOW);
     // previous <- origin, coordinates <- candidate (global_placement.rs:322-325)
@@ -1213,6 +1218,21 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
     for (const Job::EvalReq &r : j->evals) {
       Task t = j->task(r.slot, r.coords, r.grad, r.metric ? j->d_metric : nullptr, r.gamma, r.skip,
                        &j->d_scalars->eval[r.scalar_slot], fixed_nets, r.value_only);
+      if (r.trial && skip_rejected_) {
+        // the acceptance test of the trial planned at L (or the speculative
+        // one at 2L), against this iteration's y and origin values
+        t.trial = 1;
+        t.trial_y_value = j->y_value;
+        t.trial_origin_value = j->origin_value;
+        t.trial_L = r.trial == 2 ? j->spec_L : j->L;
+        t.trial_step = r.trial == 2 ? &j->d_scalars->step_spec : &j->d_scalars->step;
+        t.trial_wa_bound = j->wa_bound_ok ? 3.0 * j->wa_bound_w * (1.0 + 3.0 * j->wa_bound_ext / r.gamma) : INFINITY;
+        for (int f = 0; f < MAXF; ++f) t.trial_charge[f] = f < j->F ? j->field_charge[f] : 0.0;
+        if (poisson_fast_shape((int)j->W, (int)j->H) && j->F > 0) {
+          t.phi_amax = j->slot[r.slot].amax_part + (size_t)j->nb_conv * MAXF + 3 * (size_t)j->nb_tile;
+          t.phi_tiles = (int)(j->G / 1024);
+        }
+      }
       if (r.parity) {
         t.raw_units = j->d_raw_units;
         t.raw_fillers = j->d_raw_fillers;
@@ -1306,20 +1326,21 @@ bool Engine::launch(int c, const std::vector<Job *> &jobs) {
   const SortTask *d_sorts = reinterpret_cast<const SortTask *>(dptr(o_sorts));
   auto issue = [&] {
     if (!opt.empty()) launch_optimizer(d_opt, om, cx.stream, opt_passes);
-    // The second stream forks after the optimizer's vectors: it reduces the
-    // optimizer's scalars (only the host reads them) and runs WA (it depends
-    // only on coordinates) beside the sort and the density chain, and joins
-    // before the combination.
+    // The second stream forks after the optimizer's vectors: it runs WA (it
+    // depends only on coordinates) and then reduces the optimizer's scalars
+    // (read by the trial tests and the host) beside the sort and the density
+    // chain, and joins before the coefficients.
     HIP(hipEventRecord(cx.fork, cx.stream));
     HIP(hipStreamWaitEvent(cx.wa_stream, cx.fork, 0));
-    if (!opt.empty()) launch_optimizer_finalize(d_opt, om, cx.wa_stream);
     launch_wa(d_tasks, mp, cx.wa_stream);
+    if (!opt.empty()) launch_optimizer_finalize(d_opt, om, cx.wa_stream);
     HIP(hipEventRecord(cx.join, cx.wa_stream));
     if (!sorts.empty()) launch_sort(d_sorts, sm, cx.stream);
     launch_density(d_tasks, mp, cx.stream);
     for (const DctLaunch &d : dct) launch_poisson(d_tasks, d, cx.stream);
-    launch_gather(d_tasks, mp, cx.stream);
+    // the coefficients' trial tests read the WA value
     HIP(hipStreamWaitEvent(cx.stream, cx.join, 0));
+    launch_gather(d_tasks, mp, cx.stream);
     launch_combine(d_tasks, mp, cx.stream);
     launch_collect_scalars(d_pool_index, (int)active.size(), d_pool_, cx.d_compact, cx.stream);
   };
@@ -1419,8 +1440,12 @@ void Engine::finish(int c) {
     j->release_image();  // the round ran after the job's upload
     j->h_scalars = cx.h_compact + k;
     j->moved = 0;
-    for (const Job::EvalReq &r : j->evals)
+    for (const Job::EvalReq &r : j->evals) {
       if (!r.skip) j->moved = std::max(j->moved, j->h_scalars->eval[r.scalar_slot].moved);
+      // the timed gradient launch did not cover a skipped trial
+      if (r.trial && j->h_scalars->eval[r.scalar_slot].gradient_skipped && timer.enabled && timer.name == "gather")
+        timer.tasks -= 1.0;
+    }
     try {
       advance(j);
     } catch (...) {
@@ -1442,7 +1467,7 @@ void Engine::evaluate_once(Job *j, const double *coords, double g, bool density_
   if (j->p_slot == 0) j->p_valid = false;
   const Phase saved = j->phase;
   j->evals.clear();
-  j->evals.push_back({0, j->d_eval_coords, j->d_grad[0], false, g, false, true, 2, false});
+  j->evals.push_back({0, j->d_eval_coords, j->d_grad[0], false, g, false, true, 2, false, 0});
   j->want_precond = j->want_extrapolate = j->want_candidate = false;
   std::vector<Job *> one{j};
   // run exactly the planned request: mark the phase so plan() leaves it alone

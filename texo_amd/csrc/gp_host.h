@@ -218,6 +218,7 @@ struct Job {
   std::map<uint32_t, std::vector<uint32_t>> capacity;  // after fixed occupancy
   std::vector<size_t> available;                       // per field
   std::vector<double> real_area, gamma_weight, wl_diag;
+  std::vector<double> field_charge;  // sum of |charge| of the members and fillers per field
   uint64_t movable_instance_count = 0;
   bool empty = false, best_effort = false, has_routing = false;
   std::vector<uint32_t> routing_h, routing_v;
@@ -389,6 +390,7 @@ struct Job {
     bool parity;
     int scalar_slot;
     bool value_only;
+    int trial;  // 1 the line-search trial, 2 the speculative one (device acceptance test)
   };
   std::vector<EvalReq> evals;
   bool want_precond = false, want_extrapolate = false, want_candidate = false, want_sort = false;
@@ -528,6 +530,9 @@ class Engine {
   // 0 full, 2 value only with every gradient check taken by an OriginFull
   // round (TEXO_GP_ORIGIN_VALUE_ONLY; results are the same in all three)
   int origin_mode_ = 1;
+  // the gradient blocks of a rejected line-search trial with a bounded
+  // gradient are skipped (TEXO_GP_SKIP_REJECTED=0 runs them; same results)
+  bool skip_rejected_ = true;
   uint64_t graph_rounds_ = 0;
   void drop_graphs(RoundCtx &c);
   RoundCtx ctx_[ROUND_CONTEXTS], aux_;  // aux_: set-up sorts and parity evaluations (on context 0's stream)

@@ -794,6 +794,9 @@ void Job::set_charges(const std::vector<double> &member_areas, const std::vector
     for (uint32_t m = du_mstart[i]; m < du_mstart[i + 1]; ++m) h_du_fcharge[(size_t)m_field[m] * DU + i] += m_charge[m];
   real_area.assign(F, 0.0);
   for (size_t m = 0; m < M; ++m) real_area[m_field[m]] += m_charge[m];
+  field_charge.assign(F, 0.0);
+  for (size_t m = 0; m < M; ++m) field_charge[m_field[m]] += std::fabs(m_charge[m]);
+  for (size_t k = 0; k < fl_charge.size(); ++k) field_charge[fl_field[k]] += std::fabs(fl_charge[k]);
 }
 
 void Job::apply_charges(const std::vector<double> &member_areas, const std::vector<double> &filler_areas,
@@ -939,7 +942,7 @@ void Job::stage(const std::shared_ptr<BlockPool> &pool) {
     arena.reserve(sl.field_coef, 2 * MAXF);
     arena.reserve(sl.partial, (size_t)n_keys * 2 * std::max(F, 1) * DEP_WIN2);
     arena.reserve(sl.tile_flag, (size_t)n_keys);
-    arena.reserve(sl.amax_part, (size_t)nb_conv * MAXF + 3 * (size_t)nb_tile);
+    arena.reserve(sl.amax_part, (size_t)nb_conv * MAXF + 4 * (size_t)nb_tile);
     arena.reserve(sl.part_conv, (size_t)nb_conv * 2 * MAXF);
     arena.reserve(sl.dpin, u_pins.size());
     arena.reserve(sl.part_wa, (size_t)nb_wa);

@@ -10,6 +10,7 @@
 #include "gp_common.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 
 namespace texo {
@@ -439,6 +440,82 @@ __global__ void __launch_bounds__(BLOCK) k_convert(const Task *__restrict__ task
 // One block per task: E_f = 1/2 <rho_bar, phi> from the transform's
 // partials (fixed order) and the coefficient lambda (1 + beta E/E0)
 // (eplace.rs:1111-1124).
+// The WA value of an evaluation from its block partials, summed as
+// finalize_task sums them (each thread its partials in order, then the
+// wavefronts, then the four waves), so both give the same bits.  Block-wide.
+__device__ double wa_value_of(const Task &t) {
+  __shared__ double s[2][BLOCK / 64];
+  double live = 0.0, fixed = 0.0;
+  for (int b = threadIdx.x; b < t.nb_wa; b += BLOCK) {
+    if (b < t.nb_wa_live)
+      live += t.part_wa[b];
+    else
+      fixed += t.part_wa[b];
+  }
+  live = wave_sum(live);
+  fixed = wave_sum(fixed);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s[0][wave] = live;
+    s[1][wave] = fixed;
+  }
+  __syncthreads();
+  const double v = ((s[0][0] + s[0][1]) + s[0][2]) + s[0][3];
+  double fw = 0.0;
+  if (t.fixed_wa_mode == 1)
+    fw = ((s[1][0] + s[1][1]) + s[1][2]) + s[1][3];
+  else if (t.fixed_wa_mode == 2)
+    fw = *t.fixed_wa;
+  __syncthreads();
+  return v + fw;
+}
+
+// The acceptance test of a line-search trial (global_placement.rs:274-290)
+// with a margin four times the reference's, so that a candidate the host
+// accepts always passes it; and whether every gradient entry is bounded
+// (then the reference's gradient check passes).  Sets gradient_skipped.
+__device__ void trial_test(const Task &t, const double (&e)[MAXF]) {
+  const double wa = wa_value_of(t);
+  // max |phi| per field from the last transform stage's tile partials
+  __shared__ unsigned int sm[MAXF][BLOCK / 64];
+  unsigned int m[MAXF];
+#pragma unroll
+  for (int f = 0; f < MAXF; ++f) {
+    m[f] = 0u;
+    if (t.phi_amax && f < t.j.F)
+      for (int i = threadIdx.x; i < t.phi_tiles; i += BLOCK) m[f] = max(m[f], t.phi_amax[(size_t)f * t.phi_tiles + i]);
+    m[f] = lane_reduce<64>(m[f], OpMaxI{});
+    if ((threadIdx.x & 63) == 0) sm[f][threadIdx.x >> 6] = m[f];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  // combined_objective (eplace.rs:1022-1044), as the host computes it
+  double value = wa;
+  double bound = t.trial_wa_bound;
+  bool bounded = t.phi_amax != nullptr;
+  for (int f = 0; f < t.j.F; ++f) {
+    const double ef = e[f] / t.scale[f];
+    value += t.lambda[f] * t.scale[f] * (ef + 0.5 * AUGMENTED_DENSITY_BETA * ef * ef);
+    const unsigned int mf = max(max(sm[f][0], sm[f][1]), max(sm[f][2], sm[f][3]));
+    bounded = bounded && mf < 0x7f800000u;
+    const double coef = t.lambda[f] * (1.0 + AUGMENTED_DENSITY_BETA * (e[f] / t.scale[f]));
+    // a raw entry is the sum of its charges times a field sample, and a
+    // sample is at most 2 max |phi| (the stencil's derivative weights sum to
+    // at most 2 in magnitude, its value weights to 1)
+    bound += fabs(coef) * 4.0 * t.trial_charge[f] * static_cast<double>(__uint_as_float(mf));
+  }
+  bounded = bounded && isfinite(bound) && bound <= 1e300;
+  const StepScalars &st = *t.trial_step;
+  const double upper = t.trial_y_value + st.gdelta + 0.5 * t.trial_L * st.msq;
+  const double sm_ = 4.0 * DBL_EPSILON * (1.0 + fabs(value) + fabs(upper));
+  const double mo_ = 4.0 * DBL_EPSILON * (1.0 + fabs(value) + fabs(t.trial_origin_value));
+  // NaN compares false: a non-finite value or bound is not "maybe accepted"
+  const bool maybe = !(value > upper + sm_) && !(value > t.trial_origin_value + mo_);
+  // candidate == y: the reference does not evaluate it at all
+  const bool skip = st.changed == 0u || (!maybe && bounded && isfinite(value) && isfinite(upper));
+  t.scalars->gradient_skipped = skip ? 1u : 0u;
+}
+
 __device__ void task_coefficients(const Task &t) {
   double e[MAXF];
   sum_partials<MAXF>(t.part_energy, t.nb_energy, e, MAXF, 0);
@@ -448,6 +525,10 @@ __device__ void task_coefficients(const Task &t) {
     t.field_coef[f] = t.lambda[f] * (1.0 + AUGMENTED_DENSITY_BETA * (ef / t.scale[f]));
     t.field_coef[MAXF + f] = ef;
   }
+  if (t.trial)
+    trial_test(t, e);
+  else if (threadIdx.x == 0)
+    t.scalars->gradient_skipped = 0u;
 }
 
 // d/dx, d/dy of the quadratic B-spline interpolant of phi at (x, y)
@@ -919,7 +1000,10 @@ __device__ void finalize_task(const Task &t) {
   // flight together
   // WA: the nets with a movable pin, then (when evaluated) those of fixed pins
   double fixed_wa = 0.0;
-  const int n = max(max(t.nb_wa, t.nb_conv), max(t.nb_ug, t.nb_gf));
+  // the gradient blocks of a skipped trial wrote no partials
+  const bool skipped = t.trial && t.scalars->gradient_skipped;
+  const int nb_ug = skipped ? 0 : t.nb_ug, nb_gf = skipped ? 0 : t.nb_gf;
+  const int n = max(max(t.nb_wa, t.nb_conv), max(nb_ug, nb_gf));
   for (int b = threadIdx.x; b < n; b += BLOCK) {
     if (b < t.nb_conv) {
       const long long *pc = t.part_conv + (size_t)b * 2 * MAXF;
@@ -930,14 +1014,14 @@ __device__ void finalize_task(const Task &t) {
       acc[0] += t.part_wa[b];
     else if (b < t.nb_wa)
       fixed_wa += t.part_wa[b];
-    if (b < t.nb_ug) {
+    if (b < nb_ug) {
       const double *p = t.part_ug + (size_t)b * UG_STRIDE;
 #pragma unroll
       for (int k = 0; k < 1 + 2 * MAXF; ++k) acc[1 + k] += p[k];
       acc[NS] = fmax(acc[NS], p[1 + 2 * MAXF]);
       acc[NS + 1] = fmax(acc[NS + 1], p[2 + 2 * MAXF]);
     }
-    if (b < t.nb_gf) {
+    if (b < nb_gf) {
       const double *p = t.part_gf + (size_t)b * GF_STRIDE;
 #pragma unroll
       for (int k = 0; k < 2 * MAXF; ++k) acc[2 + 2 * MAXF + k] += p[k];
@@ -1012,6 +1096,8 @@ __global__ void __launch_bounds__(BLOCK) k_gradient(const Task *__restrict__ tas
   int2 bm;
   if (!plan_block(plan, bm)) return;
   const Task &t = tasks[bm.x];
+  // a rejected trial with a bounded gradient (the same flag for the block)
+  if (t.trial && t.scalars->gradient_skipped) return;
   if (bm.y < t.nb_ug)
     unit_gradient_block(t, bm.y);
   else
