@@ -91,7 +91,10 @@ def algorithmic_bytes(kernel: str, sizes: dict, problem):
     if kernel == "wa":
         # design: per pin unit 4 + offset / fixed coordinate + derivative slot
         # 4; per net start 4 + weight 8.  Evaluation: unit coordinates 16;
-        # per movable pin its derivative written 16
+        # per movable pin its derivative written 16.  A value-only origin
+        # evaluation writes no derivatives: the library's timer counts it as
+        # U / (U + movable pins) of a full one, so the evaluations per launch
+        # are full-evaluation equivalents
         return pins * (8 + off) + nets * 12, U * 16 + movable_pins * 16
     if kernel == "gather":
         # k_gradient.  Design: per unit pin range 4 + arc split 4 + density
