@@ -55,11 +55,12 @@ constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
 // Iterations between re-sorts of a job's charges for the deposit; the order
 // only affects speed, never the density sums.  (Every 32 iterations or at
 // 1/32 drift measured 0.7-1% faster than every 8 or at 1/64 on the 84k-cell
-// batch and the serving stream: fewer sorts outweigh the added atomics.)
+// batch and the serving stream: fewer sorts outweigh the added atomics; 1/8
+// drift then measured another 0.4-0.8% on the batch, 1/16 and 1/64 less.)
 constexpr uint64_t SORT_PERIOD = 32;
 // ... or at the next iteration once more than 1/MOVED_RESORT of the charges
 // fell outside their tile's window (those take global atomics)
-constexpr size_t MOVED_RESORT = 32;
+constexpr size_t MOVED_RESORT = 8;
 
 // Rounded targets (rounded_coordinate, lib.rs:4566-4568) of up to
 // TARGET_BATCH jobs in one launch; the table travels as the kernel argument.
