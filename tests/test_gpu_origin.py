@@ -115,3 +115,47 @@ def test_skipping_rejected_trial_gradients_batch_is_bit_identical():
         assert np.array_equal(t, t0)
         for k in KEYS:
             assert r[k] == r0[k], k
+
+
+def _timed_tasks(kernel, env):
+    """Evaluations per timed launch of `kernel` over 30 iterations of a
+    3-start batch of the 24k-cell design, with the engine switch `env`."""
+    base = synth.ecp5_25k()
+    starts = [synth.multistart(base, s) for s in range(3)]
+    key, value = env
+    old = os.environ.get(key)
+    os.environ[key] = value
+    try:
+        with texo_amd.Batch() as b:
+            for q in starts:
+                b.add(q, abi.options(dct_precision=abi.DCT_FP16_SCALED))
+            b.iterate(5)
+            b.set_kernel_timing(kernel)
+            b.iterate(30)
+            _, launches, tasks = b.kernel_timing()
+            b.set_kernel_timing(None)
+    finally:
+        if old is None:
+            os.environ.pop(key, None)
+        else:
+            os.environ[key] = old
+    assert launches > 0
+    return tasks
+
+
+def test_rejected_trials_skip_their_gradient_blocks():
+    """The gradient launches cover fewer evaluations with the skip on: the
+    first trial of most iterations is rejected, and its blocks do not run."""
+    on = _timed_tasks("gather", ("TEXO_GP_SKIP_REJECTED", "1"))
+    off = _timed_tasks("gather", ("TEXO_GP_SKIP_REJECTED", "0"))
+    print("gradient evaluations per launch: skip", on, "no skip", off, flush=True)
+    assert on < 0.9 * off
+
+
+def test_value_only_origins_count_as_partial_wa_evaluations():
+    """A value-only origin evaluation stores no derivatives: the WA timer
+    counts it as U / (U + movable pins) of a full evaluation."""
+    vo = _timed_tasks("wa", ("TEXO_GP_ORIGIN_VALUE_ONLY", "1"))
+    full = _timed_tasks("wa", ("TEXO_GP_ORIGIN_VALUE_ONLY", "0"))
+    print("WA evaluations per launch: value-only", vo, "full", full, flush=True)
+    assert vo < 0.95 * full
